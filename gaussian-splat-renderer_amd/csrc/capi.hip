@@ -1,0 +1,653 @@
+// Host side of the splat-tiles path: the C ABI of include/gsr.h. Replaces the Vulkan
+// SplatTileRenderer host code (reference render/splat_tile_renderer.cpp:126-954): it owns
+// the per-gaussian and list buffers, records the stages of a frame on one HIP stream and
+// provides the blocking test readbacks. No count is read back during a frame: V, M and
+// entries stay in the device control block and every kernel reads them there.
+#include "gsr_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace gsr;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+gsr_status fail(gsr_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_lastError = buf;
+    return s;
+}
+
+gsr_status hipFail(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? GSR_ERROR_OUT_OF_MEMORY : GSR_ERROR_HIP, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+#define GSR_HIP(call)                                          \
+    do {                                                       \
+        const hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return hipFail(e_, #call);       \
+    } while (0)
+
+// A device allocation that frees itself.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    hipError_t allocate(size_t n) {
+        release();
+        const size_t b = std::max<size_t>(n, 256);
+        hipError_t e = hipMalloc(&ptr, b);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        bytes = b;
+        return hipSuccess;
+    }
+    // Grows (contents not kept).
+    hipError_t ensure(size_t n) { return bytes >= n && ptr ? hipSuccess : allocate(n); }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+uint32_t bitsFor(uint32_t v) {  // bits needed to hold values 0..v
+    uint32_t b = 0;
+    while (b < 32 && (uint64_t(1) << b) <= v) ++b;
+    return b;
+}
+
+}  // namespace
+
+struct gsr_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceBuffer scratch;  // stand-alone sort/scan scratch
+};
+
+struct gsr_buffer {
+    gsr_context* ctx = nullptr;
+    DeviceBuffer mem;
+};
+
+struct gsr_scene {
+    gsr_context* ctx = nullptr;
+    uint32_t count = 0;
+    int32_t shDegree = 0;
+    uint32_t K = 1;
+    gsr_scene_format format = GSR_SCENE_F32_SOA;
+    DeviceBuffer posAlpha, scales, rotations, sh, geometry, color;  // owned copies
+    SceneRef ref{};
+};
+
+struct gsr_renderer {
+    gsr_context* ctx = nullptr;
+    const gsr_scene* scene = nullptr;
+    gsr_settings settings{};
+    uint32_t capacity = 0;
+    uint32_t sortBlocks = 0;
+    int32_t degree = 0;
+    // Per gaussian.
+    DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control;
+    // The list (ping-pong) and the sort's histograms.
+    DeviceBuffer keysA, keysB, valuesA, valuesB, hist, digitTotals;
+    // Per tile (sized for the largest camera seen).
+    DeviceBuffer ranges;
+    uint32_t rangesTiles = 0;
+    // Last frame.
+    gsr_camera lastCam{};
+    bool haveFrame = false;
+    int sortedIn = 0;  // 0 = A, 1 = B
+    hipStream_t lastStream = nullptr;
+    hipEvent_t events[GSR_STAGE_COUNT] = {};
+    bool eventsRecorded = false;
+};
+
+extern "C" {
+
+const char* gsr_last_error(void) { return g_lastError.c_str(); }
+const char* gsr_version(void) { return "gsr-mi355x 0.1 (gfx950)"; }
+
+gsr_status gsr_context_create(int32_t device, gsr_context** out) {
+    if (!out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_context_create: out is null");
+    *out = nullptr;
+    int n = 0;
+    GSR_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_context_create: device %d of %d", device, n);
+    GSR_HIP(hipSetDevice(device));
+    gsr_context* c = new (std::nothrow) gsr_context();
+    if (!c) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_context_create: host allocation");
+    c->device = device;
+    const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hipFail(e, "hipStreamCreateWithFlags");
+    }
+    *out = c;
+    return GSR_OK;
+}
+
+gsr_status gsr_context_synchronize(gsr_context* ctx) {
+    if (!ctx) return fail(GSR_ERROR_INVALID_ARGUMENT, "null context");
+    GSR_HIP(hipSetDevice(ctx->device));
+    GSR_HIP(hipDeviceSynchronize());
+    return GSR_OK;
+}
+
+void gsr_context_destroy(gsr_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->scratch.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+gsr_status gsr_buffer_create(gsr_context* ctx, size_t bytes, gsr_buffer** out) {
+    if (!ctx || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_buffer_create: null argument");
+    *out = nullptr;
+    GSR_HIP(hipSetDevice(ctx->device));
+    gsr_buffer* b = new (std::nothrow) gsr_buffer();
+    if (!b) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_buffer_create: host allocation");
+    b->ctx = ctx;
+    const hipError_t e = b->mem.allocate(bytes);
+    if (e != hipSuccess) {
+        delete b;
+        return hipFail(e, "gsr_buffer_create: hipMalloc");
+    }
+    *out = b;
+    return GSR_OK;
+}
+
+void* gsr_buffer_device_ptr(gsr_buffer* buf) { return buf ? buf->mem.ptr : nullptr; }
+
+gsr_status gsr_buffer_read(gsr_buffer* buf, void* dst, size_t bytes) {
+    if (!buf || !dst || bytes > buf->mem.bytes) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_buffer_read: bad arguments");
+    GSR_HIP(hipSetDevice(buf->ctx->device));
+    GSR_HIP(hipDeviceSynchronize());
+    GSR_HIP(hipMemcpy(dst, buf->mem.ptr, bytes, hipMemcpyDeviceToHost));
+    return GSR_OK;
+}
+
+gsr_status gsr_buffer_write(gsr_buffer* buf, const void* src, size_t bytes) {
+    if (!buf || !src || bytes > buf->mem.bytes) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_buffer_write: bad arguments");
+    GSR_HIP(hipSetDevice(buf->ctx->device));
+    GSR_HIP(hipDeviceSynchronize());
+    GSR_HIP(hipMemcpy(buf->mem.ptr, src, bytes, hipMemcpyHostToDevice));
+    return GSR_OK;
+}
+
+void gsr_buffer_destroy(gsr_buffer* buf) {
+    if (!buf) return;
+    (void)hipSetDevice(buf->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete buf;
+}
+
+// --- Scene ---------------------------------------------------------------------------------
+
+gsr_status gsr_scene_create(gsr_context* ctx, const gsr_scene_desc* d, gsr_scene** out) {
+    if (!ctx || !d || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_scene_create: null argument");
+    *out = nullptr;
+    if (d->sh_degree < 0 || d->sh_degree > 3)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_scene_create: sh_degree %d outside 0..3", d->sh_degree);
+    if (d->format != GSR_SCENE_F32_SOA && d->format != GSR_SCENE_F16)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_scene_create: unknown format %d", int(d->format));
+    const uint32_t K = uint32_t((d->sh_degree + 1) * (d->sh_degree + 1));
+    if (d->count > 0) {
+        if (d->format == GSR_SCENE_F32_SOA && (!d->pos_alpha || !d->scales || !d->rotations || !d->sh))
+            return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_scene_create: f32 SoA needs pos_alpha, scales, rotations, sh");
+        if (d->format == GSR_SCENE_F16 && (!d->geometry || !d->color))
+            return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_scene_create: f16 needs geometry and color records");
+    }
+    GSR_HIP(hipSetDevice(ctx->device));
+    gsr_scene* s = new (std::nothrow) gsr_scene();
+    if (!s) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_scene_create: host allocation");
+    s->ctx = ctx;
+    s->count = d->count;
+    s->shDegree = d->sh_degree;
+    s->K = K;
+    s->format = d->format;
+    const size_t n = d->count;
+    const uint32_t colorHalves = (K * 3 + 7) / 8 * 8;
+    auto bind = [&](DeviceBuffer& buf, const void* src, size_t bytes, const void** dst) -> hipError_t {
+        if (d->device_pointers || n == 0) {
+            *dst = src;
+            return hipSuccess;
+        }
+        hipError_t e = buf.allocate(bytes);
+        if (e != hipSuccess) return e;
+        e = hipMemcpy(buf.ptr, src, bytes, hipMemcpyHostToDevice);
+        *dst = buf.ptr;
+        return e;
+    };
+    SceneRef& r = s->ref;
+    r.count = d->count;
+    r.K = K;
+    r.format = d->format;
+    r.degree = d->sh_degree;
+    r.colorQuads = colorHalves / 8;
+    hipError_t e = hipSuccess;
+    const void* p = nullptr;
+    if (d->format == GSR_SCENE_F32_SOA) {
+        if ((e = bind(s->posAlpha, d->pos_alpha, n * 16, &p)) == hipSuccess) r.posAlpha = static_cast<const float4*>(p);
+        if (e == hipSuccess && (e = bind(s->scales, d->scales, n * 16, &p)) == hipSuccess)
+            r.scales = static_cast<const float4*>(p);
+        if (e == hipSuccess && (e = bind(s->rotations, d->rotations, n * 16, &p)) == hipSuccess)
+            r.rotations = static_cast<const float4*>(p);
+        if (e == hipSuccess && (e = bind(s->sh, d->sh, n * K * 3 * 4, &p)) == hipSuccess)
+            r.sh = static_cast<const float*>(p);
+    } else {
+        if ((e = bind(s->geometry, d->geometry, n * 32, &p)) == hipSuccess) r.geometry = static_cast<const uint4*>(p);
+        if (e == hipSuccess && (e = bind(s->color, d->color, n * colorHalves * 2, &p)) == hipSuccess)
+            r.color = static_cast<const uint4*>(p);
+    }
+    if (e != hipSuccess) {
+        delete s;  // frees the partial upload (gpu_scene.cpp:108-114)
+        return hipFail(e, "gsr_scene_create: upload");
+    }
+    *out = s;
+    return GSR_OK;
+}
+
+void gsr_scene_destroy(gsr_scene* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete s;
+}
+
+// --- fp16 encoder (host) ---------------------------------------------------------------------
+
+namespace {
+uint16_t floatToHalfRne(float value) {
+    uint32_t f;
+    std::memcpy(&f, &value, 4);
+    const uint32_t sign = (f >> 16) & 0x8000u;
+    const uint32_t a = f & 0x7FFFFFFFu;
+    if (a >= 0x7F800000u) return uint16_t(sign | 0x7C00u | (a > 0x7F800000u ? 0x200u : 0u));  // Inf / NaN
+    if (a >= 0x477FF000u) return uint16_t(sign | 0x7C00u);  // rounds past 65504
+    if (a < 0x38800000u) {                                  // half subnormal or zero
+        if (a < 0x33000000u) return uint16_t(sign);
+        const uint32_t e = a >> 23;
+        const uint32_t m = (a & 0x7FFFFFu) | 0x800000u;
+        const uint32_t shift = 126u - e;
+        uint32_t h = m >> shift;
+        const uint32_t rest = m & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1u);
+        if (rest > half || (rest == half && (h & 1u))) ++h;
+        return uint16_t(sign | h);
+    }
+    uint32_t h = (a - 0x38000000u) >> 13;  // rebias 127 -> 15
+    const uint32_t rest = a & 0x1FFFu;
+    if (rest > 0x1000u || (rest == 0x1000u && (h & 1u))) ++h;
+    return uint16_t(sign | h);
+}
+uint32_t asBits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+}  // namespace
+
+gsr_status gsr_encode_f16(uint32_t count, int32_t shDegree, const float* posAlpha, const float* scales,
+                          const float* rotations, const float* sh, uint32_t* geometry, uint16_t* color) {
+    if (shDegree < 0 || shDegree > 3) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_encode_f16: sh_degree %d", shDegree);
+    if (count && (!posAlpha || !scales || !rotations || !sh || !geometry || !color))
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_encode_f16: null array");
+    const uint32_t K = uint32_t((shDegree + 1) * (shDegree + 1));
+    const uint32_t colorHalves = (K * 3 + 7) / 8 * 8;
+    for (uint32_t i = 0; i < count; ++i) {
+        const float* pa = posAlpha + size_t(i) * 4;
+        const float* sc = scales + size_t(i) * 4;
+        const float* q = rotations + size_t(i) * 4;
+        uint32_t* w = geometry + size_t(i) * 8;
+        w[0] = asBits(pa[0]);
+        w[1] = asBits(pa[1]);
+        w[2] = asBits(pa[2]);
+        w[3] = uint32_t(floatToHalfRne(pa[3])) | (uint32_t(floatToHalfRne(sc[0])) << 16);
+        w[4] = uint32_t(floatToHalfRne(sc[1])) | (uint32_t(floatToHalfRne(sc[2])) << 16);
+        w[5] = uint32_t(floatToHalfRne(q[0])) | (uint32_t(floatToHalfRne(q[1])) << 16);
+        w[6] = uint32_t(floatToHalfRne(q[2])) | (uint32_t(floatToHalfRne(q[3])) << 16);
+        w[7] = 0u;
+        uint16_t* c = color + size_t(i) * colorHalves;
+        const float* src = sh + size_t(i) * K * 3;
+        for (uint32_t j = 0; j < colorHalves; ++j) c[j] = j < K * 3 ? floatToHalfRne(src[j]) : uint16_t(0);
+    }
+    return GSR_OK;
+}
+
+// --- Camera ------------------------------------------------------------------------------------
+
+gsr_status gsr_camera_make(const float viewInria[16], uint32_t width, uint32_t height, float tanX, float tanY,
+                           gsr_camera* out) {
+    if (!viewInria || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_camera_make: null argument");
+    if (!(tanX > 0.0f) || !(tanY > 0.0f)) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_camera_make: tan(fov/2) <= 0");
+    gsr_camera c{};
+    std::memcpy(c.view_inria, viewInria, sizeof(c.view_inria));
+    // Camera centre c = -R^T t (viewInria * c = 0 for a rigid transform).
+    const float* v = viewInria;
+    for (int i = 0; i < 3; ++i) {
+        const float s = v[i * 4 + 0] * v[12] + v[i * 4 + 1] * v[13] + v[i * 4 + 2] * v[14];
+        c.position[i] = -s;
+    }
+    c.width = width;
+    c.height = height;
+    c.tan_half_fov_x = tanX;
+    c.tan_half_fov_y = tanY;
+    c.focal_x = float(width) / (2.0f * tanX);
+    c.focal_y = float(height) / (2.0f * tanY);
+    c.tiles_x = (width + kTileSize - 1) / kTileSize;
+    c.tiles_y = (height + kTileSize - 1) / kTileSize;
+    *out = c;
+    return GSR_OK;
+}
+
+gsr_status gsr_camera_from_viewer(const float view[16], const float proj[16], const float model[16], uint32_t width,
+                                  uint32_t height, gsr_camera* out) {
+    if (!view || !proj || !model || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_camera_from_viewer: null argument");
+    // viewInria = diag(1, -1, -1, 1) * view * model, column-major products summed in order.
+    auto mul = [](const float* A, const float* B, float* R) {
+        for (int j = 0; j < 4; ++j)
+            for (int i = 0; i < 4; ++i)
+                R[j * 4 + i] =
+                    A[i] * B[j * 4 + 0] + A[4 + i] * B[j * 4 + 1] + A[8 + i] * B[j * 4 + 2] + A[12 + i] * B[j * 4 + 3];
+    };
+    const float flip[16] = {1, 0, 0, 0, 0, -1, 0, 0, 0, 0, -1, 0, 0, 0, 0, 1};
+    float fv[16], vi[16];
+    mul(flip, view, fv);
+    mul(fv, model, vi);
+    const float tanX = 1.0f / proj[0];
+    const float tanY = 1.0f / std::fabs(proj[5]);
+    return gsr_camera_make(vi, width, height, tanX, tanY, out);
+}
+
+// --- Renderer ----------------------------------------------------------------------------------
+
+static uint32_t capacityForBytes(double bytes) {
+    // Per entry: 64-bit key + value, both ping-ponged, + the sort's histograms: the
+    // reference's global-mode 24.5 B/entry (splat_tile_renderer.cpp:117-124), blocks of 2048.
+    const double maxEntries = double(1u << 30);
+    double c = bytes / 24.5;
+    c = std::min(std::max(c, 2048.0), maxEntries);
+    const uint32_t e = uint32_t(c);
+    return e - e % 2048u;
+}
+
+gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const gsr_settings* settings,
+                               gsr_renderer** out) {
+    if (!ctx || !scene || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: null argument");
+    *out = nullptr;
+    gsr_settings s{};
+    if (settings) s = *settings;
+    else {
+        s.bounds = GSR_BOUNDS_TIGHT;
+        s.max_sh_degree = 3;
+    }
+    if (s.bounds != GSR_BOUNDS_RECT && s.bounds != GSR_BOUNDS_TIGHT && s.bounds != GSR_BOUNDS_CUTOFF_ONLY)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown bounds %d", int(s.bounds));
+    if (s.format != GSR_TARGET_RGBA8 && s.format != GSR_TARGET_RGBA32F)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown target format %d", int(s.format));
+    if (s.budget_mb == 0) s.budget_mb = 256;
+    GSR_HIP(hipSetDevice(ctx->device));
+    gsr_renderer* r = new (std::nothrow) gsr_renderer();
+    if (!r) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_renderer_create: host allocation");
+    r->ctx = ctx;
+    r->scene = scene;
+    r->settings = s;
+    r->degree = std::min(scene->shDegree, std::max(0, std::min(3, s.max_sh_degree)));
+    r->capacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0);
+    r->sortBlocks = s.sort_blocks ? std::min<uint32_t>(s.sort_blocks, kMaxSortBlocks)
+                                  : std::min<uint32_t>(kMaxSortBlocks, std::max<uint32_t>(1, (r->capacity + 2047) / 2048));
+    const size_t n = std::max<size_t>(scene->count, 1);
+    const size_t gBlocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
+    hipError_t e = hipSuccess;
+    auto alloc = [&](DeviceBuffer& b, size_t bytes) {
+        if (e == hipSuccess) e = b.allocate(bytes);
+    };
+    alloc(r->splats, n * 48);
+    alloc(r->tileCounts, n * 4);
+    alloc(r->blockOffsets, gBlocks * 4);
+    alloc(r->control, sizeof(gsr_control));
+    alloc(r->keysA, size_t(r->capacity) * 8);
+    alloc(r->keysB, size_t(r->capacity) * 8);
+    alloc(r->valuesA, size_t(r->capacity) * 4);
+    alloc(r->valuesB, size_t(r->capacity) * 4);
+    alloc(r->hist, size_t(kRadix) * kMaxSortBlocks * 4);
+    alloc(r->digitTotals, kRadix * 4);
+    if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
+    for (int i = 0; i < GSR_STAGE_COUNT && e == hipSuccess; ++i) e = hipEventCreate(&r->events[i]);
+    if (e != hipSuccess) {
+        gsr_renderer_destroy(r);
+        return hipFail(e, "gsr_renderer_create: allocation");
+    }
+    *out = r;
+    return GSR_OK;
+}
+
+void gsr_renderer_destroy(gsr_renderer* r) {
+    if (!r) return;
+    (void)hipSetDevice(r->ctx->device);
+    (void)hipDeviceSynchronize();
+    for (hipEvent_t& ev : r->events)
+        if (ev) (void)hipEventDestroy(ev);
+    delete r;
+}
+
+uint32_t gsr_renderer_capacity(const gsr_renderer* r) { return r ? r->capacity : 0; }
+
+uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
+    if (!r) return 0;
+    const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
+                                  &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->hist,
+                                  &r->digitTotals, &r->ranges};
+    uint64_t s = 0;
+    for (const DeviceBuffer* b : bufs) s += b->bytes;
+    return s;
+}
+
+gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, void* streamArg) {
+    if (!r || !cam || !outImage) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_render: null argument");
+    if (cam->width == 0 || cam->height == 0 || cam->width > 65536 || cam->height > 65536)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_render: image size %ux%u", cam->width, cam->height);
+    if (cam->tiles_x != (cam->width + 15) / 16 || cam->tiles_y != (cam->height + 15) / 16)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_render: camera tiles do not match its size (use gsr_camera_make)");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : r->ctx->stream;
+    const gsr_scene* sc = r->scene;
+    const uint32_t tiles = cam->tiles_x * cam->tiles_y;
+    // Per-size buffers (onRenderTargetsChanged, splat_tile_renderer.cpp:537-570).
+    if (tiles > r->rangesTiles) {
+        GSR_HIP(hipStreamSynchronize(stream));
+        GSR_HIP(r->ranges.allocate(size_t(tiles) * 8));
+        r->rangesTiles = tiles;
+    }
+    const uint32_t rangeWords = 2 * tiles;
+    const uint32_t preBlocks = (std::max(sc->count, rangeWords) + kPreprocessThreads - 1) / kPreprocessThreads;
+    if (r->blockStats.bytes < size_t(preBlocks) * sizeof(BlockStats)) {
+        GSR_HIP(hipStreamSynchronize(stream));
+        GSR_HIP(r->blockStats.allocate(size_t(preBlocks) * sizeof(BlockStats)));
+    }
+    DevCamera dc{};
+    std::memcpy(dc.view, cam->view_inria, sizeof(dc.view));
+    std::memcpy(dc.pos, cam->position, sizeof(dc.pos));
+    dc.tanX = cam->tan_half_fov_x;
+    dc.tanY = cam->tan_half_fov_y;
+    dc.focalX = cam->focal_x;
+    dc.focalY = cam->focal_y;
+    dc.width = cam->width;
+    dc.height = cam->height;
+    dc.tilesX = cam->tiles_x;
+    dc.tilesY = cam->tiles_y;
+    SceneRef ref = sc->ref;
+    ref.degree = r->degree;
+
+    const bool prof = r->settings.profile_stages != 0;
+    auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(r->events[i], stream) : hipSuccess; };
+    gsr_control* control = r->control.as<gsr_control>();
+    const uint32_t gBlocks = (sc->count + kPreprocessThreads - 1) / kPreprocessThreads;
+    const uint32_t keyBits = 32 + bitsFor(tiles);  // tile ids 0..tiles (tiles = the sentinel)
+
+    GSR_HIP(mark(0));
+    launchPreprocess(ref, dc, uint32_t(r->settings.bounds), r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
+                     r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords, stream);
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(mark(1));
+    launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, r->blockOffsets.as<uint32_t>(),
+                     control, stream);
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(mark(2));
+    launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(), control,
+                    sc->count, uint32_t(r->settings.bounds), cam->tiles_x, cam->tiles_y, r->keysA.as<uint64_t>(),
+                    r->valuesA.as<uint32_t>(), control, stream);
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(mark(3));
+    const int which = launchRadixSort(r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),
+                                      r->valuesB.as<uint32_t>(), r->capacity, &control->entries, keyBits,
+                                      r->sortBlocks, r->hist.as<uint32_t>(), r->digitTotals.as<uint32_t>(), stream);
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(mark(4));
+    const uint64_t* sortedKeys = which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>();
+    const uint32_t* sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
+    launchRanges(sortedKeys, control, r->capacity, tiles, r->ranges.as<uint2>(), stream);
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(mark(5));
+    launchRender(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc, r->settings.format, outImage,
+                 stream);
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(mark(6));
+    r->lastCam = *cam;
+    r->haveFrame = true;
+    r->sortedIn = which;
+    r->lastStream = stream;
+    r->eventsRecorded = prof;
+    return GSR_OK;
+}
+
+gsr_status gsr_read_control(gsr_renderer* r, gsr_control* out) {
+    if (!r || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_control: null argument");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    GSR_HIP(hipDeviceSynchronize());
+    GSR_HIP(hipMemcpy(out, r->control.ptr, sizeof(gsr_control), hipMemcpyDeviceToHost));
+    return GSR_OK;
+}
+
+gsr_status gsr_read_lists(gsr_renderer* r, uint32_t* tileKeys, uint32_t* depthKeys, uint32_t* values,
+                          uint32_t* ranges, uint32_t maxEntries) {
+    if (!r) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_lists: null renderer");
+    if (!r->haveFrame) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_lists: no frame rendered yet");
+    gsr_control c;
+    gsr_status st = gsr_read_control(r, &c);
+    if (st != GSR_OK) return st;
+    const uint32_t n = std::min(c.entries, maxEntries);
+    if (n > 0 && (tileKeys || depthKeys)) {
+        std::vector<uint64_t> k(n);
+        const void* src = r->sortedIn ? r->keysB.ptr : r->keysA.ptr;
+        GSR_HIP(hipMemcpy(k.data(), src, size_t(n) * 8, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (tileKeys) tileKeys[i] = uint32_t(k[i] >> 32);
+            if (depthKeys) depthKeys[i] = uint32_t(k[i]);
+        }
+    }
+    if (n > 0 && values)
+        GSR_HIP(hipMemcpy(values, r->sortedIn ? r->valuesB.ptr : r->valuesA.ptr, size_t(n) * 4, hipMemcpyDeviceToHost));
+    if (ranges)
+        GSR_HIP(hipMemcpy(ranges, r->ranges.ptr, size_t(r->lastCam.tiles_x) * r->lastCam.tiles_y * 8,
+                          hipMemcpyDeviceToHost));
+    return GSR_OK;
+}
+
+gsr_status gsr_read_splats(gsr_renderer* r, gsr_projected_splat* splats, uint32_t* tileCounts) {
+    if (!r) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_splats: null renderer");
+    if (!r->haveFrame) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_splats: no frame rendered yet");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    GSR_HIP(hipDeviceSynchronize());
+    const size_t n = r->scene->count;
+    if (n && splats) GSR_HIP(hipMemcpy(splats, r->splats.ptr, n * 48, hipMemcpyDeviceToHost));
+    if (n && tileCounts) GSR_HIP(hipMemcpy(tileCounts, r->tileCounts.ptr, n * 4, hipMemcpyDeviceToHost));
+    return GSR_OK;
+}
+
+gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n) {
+    if (!r || !ms) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_stage_times: null argument");
+    if (!r->eventsRecorded) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_stage_times: no profiled frame (profile_stages = 0?)");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    GSR_HIP(hipEventSynchronize(r->events[GSR_STAGE_COUNT - 1]));
+    float t[GSR_STAGE_COUNT];
+    for (int i = 0; i < GSR_STAGE_COUNT - 1; ++i) GSR_HIP(hipEventElapsedTime(&t[i], r->events[i], r->events[i + 1]));
+    GSR_HIP(hipEventElapsedTime(&t[GSR_STAGE_TOTAL], r->events[0], r->events[GSR_STAGE_COUNT - 1]));
+    for (uint32_t i = 0; i < n && i < uint32_t(GSR_STAGE_COUNT); ++i) ms[i] = t[i];
+    return GSR_OK;
+}
+
+// --- Stand-alone sort and scan -------------------------------------------------------------------
+
+gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, uint32_t maxCount,
+                          const uint32_t* countDevice, uint32_t keyBits, void* streamArg) {
+    if (!ctx || (maxCount && (!keys || !values))) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_sort_pairs: null argument");
+    if (keyBits == 0 || keyBits > 64) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_sort_pairs: key_bits %u", keyBits);
+    if (maxCount == 0) return GSR_OK;
+    GSR_HIP(hipSetDevice(ctx->device));
+    hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : ctx->stream;
+    const size_t n = maxCount;
+    const size_t histBytes = size_t(kRadix) * kMaxSortBlocks * 4;
+    const size_t need = n * 8 + n * 4 + histBytes + kRadix * 4 + 256;
+    if (ctx->scratch.bytes < need) {
+        GSR_HIP(hipDeviceSynchronize());
+        GSR_HIP(ctx->scratch.allocate(need));
+    }
+    char* base = ctx->scratch.as<char>();
+    uint64_t* keysB = reinterpret_cast<uint64_t*>(base);
+    uint32_t* valuesB = reinterpret_cast<uint32_t*>(base + n * 8);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(base + n * 12);
+    uint32_t* totals = reinterpret_cast<uint32_t*>(base + n * 12 + histBytes);
+    const uint32_t blocks = std::min<uint32_t>(kMaxSortBlocks, std::max<uint32_t>(1, (maxCount + 2047) / 2048));
+    const int which = launchRadixSort(keys, values, keysB, valuesB, maxCount, countDevice, keyBits, blocks, hist,
+                                      totals, stream);
+    GSR_HIP(hipGetLastError());
+    if (which) {  // odd pass count: the result is in the scratch copy
+        GSR_HIP(hipMemcpyAsync(keys, keysB, n * 8, hipMemcpyDeviceToDevice, stream));
+        GSR_HIP(hipMemcpyAsync(values, valuesB, n * 4, hipMemcpyDeviceToDevice, stream));
+    }
+    return GSR_OK;
+}
+
+gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* out, uint32_t count,
+                              uint32_t* totalDevice, void* streamArg) {
+    if (!ctx || (count && (!in || !out))) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_exclusive_scan: null argument");
+    GSR_HIP(hipSetDevice(ctx->device));
+    hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : ctx->stream;
+    const size_t blocks = (size_t(count) + kScanBlock - 1) / kScanBlock;
+    const size_t need = (blocks + 1) * 4;
+    if (ctx->scratch.bytes < need) {
+        GSR_HIP(hipDeviceSynchronize());
+        GSR_HIP(ctx->scratch.allocate(need));
+    }
+    launchExclusiveScan(in, out, count, ctx->scratch.as<uint32_t>(), totalDevice, stream);
+    GSR_HIP(hipGetLastError());
+    return GSR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,213 @@
+// Device-side per-gaussian math for gfx950: the INRIA preprocess and the tight tile
+// coverage. Every expression follows the reference's written FP32 order
+// (splat/splat_math.cpp:44-193, splat/tile_coverage.cpp:89-209) and is compiled with
+// -ffp-contract=off and IEEE division/sqrt, so the count pass (preprocess) and the write
+// pass (duplicate) compute the same bits, and both equal the CPU restatement.
+#pragma once
+
+#include "gsr_internal.hpp"
+
+namespace gsr {
+
+__device__ __forceinline__ float selMin(float a, float b) { return b < a ? b : a; }   // std::min(a, b)
+__device__ __forceinline__ float selMax(float a, float b) { return a < b ? b : a; }   // std::max(a, b)
+__device__ __forceinline__ float clampSel(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+// Natural log as the CPU oracle computes it: x = 2^e m, m in [sqrt(1/2), sqrt(2)),
+// log = e ln2 + 2 atanh((m-1)/(m+1)), atanh by its series to s^21, in double with + - * /
+// only (IEEE on both sides), rounded once to float.
+__device__ __forceinline__ float detLogf(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (__builtin_isinf(x)) return x;
+    uint32_t b = __float_as_uint(x);
+    int e = 0;
+    if ((b >> 23) == 0) {
+        b = __float_as_uint(x * 16777216.0f);
+        e = -24;
+    }
+    e += int(b >> 23) - 127;
+    double m = double(__uint_as_float((b & 0x7FFFFFu) | 0x3F800000u));
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 21.0;
+    p = p * s2 + 1.0 / 19.0;
+    p = p * s2 + 1.0 / 17.0;
+    p = p * s2 + 1.0 / 15.0;
+    p = p * s2 + 1.0 / 13.0;
+    p = p * s2 + 1.0 / 11.0;
+    p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0;
+    p = p * s2 + 1.0 / 5.0;
+    p = p * s2 + 1.0 / 3.0;
+    p = p * s2 + 1.0;
+    const double r = 2.0 * s * p;
+    return float(double(e) * 0.6931471805599453 + r);
+}
+
+// --- Tile rectangle (getRect, splat_math.cpp:116-127) -----------------------------------------
+struct Rect {
+    int x0, y0, x1, y1;
+};
+__device__ __forceinline__ int clampTile(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+__device__ __forceinline__ Rect tileRect(float px, float py, float radius, uint32_t tilesX, uint32_t tilesY) {
+    const int gx = int(tilesX), gy = int(tilesY);
+    const int r = int(radius);
+    const float S = float(kTileSize);
+    Rect t;
+    t.x0 = clampTile(int((px - float(r)) / S), gx);
+    t.y0 = clampTile(int((py - float(r)) / S), gy);
+    t.x1 = clampTile(int((px + float(r) + float(kTileSize - 1)) / S), gx);
+    t.y1 = clampTile(int((py + float(r) + float(kTileSize - 1)) / S), gy);
+    return t;
+}
+
+// --- Cutoff ellipse (makeCutoffEllipse, tile_coverage.cpp:89-115) ------------------------------
+struct Ellipse {
+    float cx, cy, A, B, C, D, t, exPad, eyPad;
+    bool valid;
+};
+
+__device__ __forceinline__ Ellipse makeEllipse(float cx, float cy, float A, float B, float C, float opacity) {
+    Ellipse e;
+    e.cx = cx;
+    e.cy = cy;
+    e.A = A;
+    e.B = B;
+    e.C = C;
+    e.D = A * C - B * B;
+    e.t = -1.0f;
+    e.exPad = 0.0f;
+    e.eyPad = 0.0f;
+    e.valid = A > 0.0f && C > 0.0f && e.D > 0.0f;
+    if (!e.valid) return e;
+    if (!(opacity >= kMinAlpha)) return e;
+    const float qmax = 2.0f * detLogf(255.0f * opacity);
+    const float rho = fabsf(B) / sqrtf(A * C);
+    const float omr = 1.0f - rho;
+    const float K = 2.0f / (omr < 1e-30f ? 1e-30f : omr);
+    const float qm1 = qmax < 1.0f ? 1.0f : qmax;
+    const float t = qmax + (1e-3f + 2e-5f * K) * qm1 + 2e-3f;
+    if (!(t < 1e30f)) {
+        e.valid = false;
+        return e;
+    }
+    e.t = t;
+    const float ex = sqrtf(t * C / e.D);
+    const float ey = sqrtf(t * A / e.D);
+    e.exPad = ex + (0.02f + 2e-3f * ex);
+    e.eyPad = ey + (0.02f + 2e-3f * ey);
+    return e;
+}
+
+__device__ __forceinline__ int floorDivClamped(float v, int S) {
+    return int(floorf(clampSel(v, -65536.0f, 1048576.0f) / float(S)));
+}
+
+// Coverage of one splat in 16-px tiles: either the rectangle (useEllipse = false) or the
+// rectangle intersected with the cutoff ellipse, one contiguous run per tile row.
+struct Coverage {
+    Rect bins;
+    Ellipse e;
+    bool useEllipse;
+};
+
+__device__ __forceinline__ Coverage makeCoverage(uint32_t mode, const Rect& rect, int gridX, int gridY, float cx,
+                                                 float cy, float A, float B, float C, float opacity) {
+    Coverage cv;
+    if (mode == GSR_BOUNDS_CUTOFF_ONLY) {
+        cv.bins = Rect{0, 0, gridX, gridY};
+    } else {
+        cv.bins = Rect{rect.x0, rect.y0, rect.x1 < gridX ? rect.x1 : gridX, rect.y1 < gridY ? rect.y1 : gridY};
+    }
+    cv.useEllipse = false;
+    cv.e.valid = false;
+    cv.e.t = -1.0f;
+    if (mode != GSR_BOUNDS_RECT) {
+        cv.e = makeEllipse(cx, cy, A, B, C, opacity);
+        cv.useEllipse = cv.e.valid;
+        if (!cv.e.valid && mode == GSR_BOUNDS_CUTOFF_ONLY)
+            cv.bins = Rect{rect.x0, rect.y0, rect.x1 < gridX ? rect.x1 : gridX, rect.y1 < gridY ? rect.y1 : gridY};
+    }
+    return cv;
+}
+
+__device__ __forceinline__ void coverageRows(const Coverage& cv, int* r0, int* r1) {
+    if (!cv.useEllipse) {
+        *r0 = cv.bins.y0;
+        *r1 = cv.bins.y1 > cv.bins.y0 ? cv.bins.y1 : cv.bins.y0;
+        return;
+    }
+    if (cv.e.t < 0.0f) {
+        *r0 = 0;
+        *r1 = 0;
+        return;
+    }
+    const float y0 = ceilf(cv.e.cy - cv.e.eyPad);
+    const float y1 = floorf(cv.e.cy + cv.e.eyPad);
+    const int a = floorDivClamped(y0, kTileSize);
+    const int b = floorDivClamped(y1, kTileSize) + 1;
+    const int lo = cv.bins.y0 > a ? cv.bins.y0 : a;
+    const int hi = cv.bins.y1 < b ? cv.bins.y1 : b;
+    *r0 = lo;
+    *r1 = hi > lo ? hi : lo;
+}
+
+// ellipseRowSpan (tile_coverage.cpp:135-161): the run [x0, x1) of tile row `row`.
+__device__ __forceinline__ void coverageRowSpan(const Coverage& cv, int row, int* x0, int* x1) {
+    if (!cv.useEllipse) {
+        *x0 = cv.bins.x0;
+        *x1 = cv.bins.x1 > cv.bins.x0 ? cv.bins.x1 : cv.bins.x0;
+        return;
+    }
+    const Ellipse& e = cv.e;
+    const int S = kTileSize;
+    const float ey0 = float(row * S) - e.cy;
+    const float ey1 = float(row * S + S - 1) - e.cy;
+    const float nep = -e.eyPad;
+    const float lo = ey0 < nep ? nep : ey0;
+    const float hi = e.eyPad < ey1 ? e.eyPad : ey1;
+    if (!(lo <= hi)) {
+        *x0 = cv.bins.x0;
+        *x1 = cv.bins.x0;
+        return;
+    }
+    const float exTrue = sqrtf(e.t * e.C / e.D);
+    const float eyRight = clampSel(-e.B * exTrue / e.C, lo, hi);
+    const float eyLeft = clampSel(e.B * exTrue / e.C, lo, hi);
+    const float At = e.A * e.t;
+    const float dr = At - e.D * eyRight * eyRight;
+    const float dl = At - e.D * eyLeft * eyLeft;
+    const float xRight = (-e.B * eyRight + sqrtf(0.0f < dr ? dr : 0.0f)) / e.A;
+    const float xLeft = (-e.B * eyLeft - sqrtf(0.0f < dl ? dl : 0.0f)) / e.A;
+    const float pad = 0.02f + 2e-3f * exTrue;
+    const float bx0 = e.cx + xLeft - pad;
+    const float bx1 = e.cx + xRight + pad;
+    const int a = floorDivClamped(ceilf(bx0), S);
+    const int b = floorDivClamped(floorf(bx1), S) + 1;
+    const int lo2 = cv.bins.x0 > a ? cv.bins.x0 : a;
+    const int hi2 = cv.bins.x1 < b ? cv.bins.x1 : b;
+    *x0 = lo2;
+    *x1 = hi2 > lo2 ? hi2 : lo2;
+}
+
+__device__ __forceinline__ uint32_t coverageCount(const Coverage& cv) {
+    int r0, r1;
+    coverageRows(cv, &r0, &r1);
+    if (!cv.useEllipse) {
+        const int w = cv.bins.x1 - cv.bins.x0;
+        return uint32_t((w > 0 ? w : 0) * (r1 - r0));
+    }
+    uint32_t n = 0;
+    for (int r = r0; r < r1; ++r) {
+        int x0, x1;
+        coverageRowSpan(cv, r, &x0, &x1);
+        n += uint32_t(x1 - x0);
+    }
+    return n;
+}
+
+}  // namespace gsr

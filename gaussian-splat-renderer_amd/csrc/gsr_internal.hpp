@@ -1,0 +1,89 @@
+// Internal types shared by the host orchestration (capi.cpp) and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/gsr.h"
+
+namespace gsr {
+
+// INRIA constants (reference splat/splat_math.h:25-40).
+constexpr float kNearCullDepth = 0.2f;
+constexpr float kLowPassFilter = 0.3f;
+constexpr float kFrustumClampFactor = 1.3f;
+constexpr float kMaxAlpha = 0.99f;
+constexpr float kMinAlpha = 1.0f / 255.0f;
+constexpr float kMinTransmittance = 1e-4f;
+constexpr int kTileSize = 16;
+constexpr uint32_t kNoGaussian = 0xFFFFFFFFu;
+
+// Preprocess: one thread per gaussian, 256 per block; the block also reduces its tile
+// counts and cull counts into one BlockStats record (so no global atomics are needed).
+constexpr int kPreprocessThreads = 256;
+struct BlockStats {
+    uint32_t tileSum;       // sum of the block's tile counts
+    uint32_t nonZero;       // gaussians with a non-zero tile count
+    uint32_t cull[5];       // per cull reason
+    uint32_t pad;
+};
+static_assert(sizeof(BlockStats) == 32, "BlockStats is 32 bytes");
+
+// Kernel-side camera (SplatCamera, splat_math.h:43-50). view[c*4+r], column-major.
+struct DevCamera {
+    float view[16];
+    float pos[3];
+    float tanX, tanY, focalX, focalY;
+    uint32_t width, height, tilesX, tilesY;
+};
+
+struct SceneRef {
+    uint32_t count;
+    uint32_t K;              // coefficients per channel stored
+    int32_t format;          // gsr_scene_format
+    int32_t degree;          // bands evaluated
+    const float4* posAlpha;  // F32
+    const float4* scales;
+    const float4* rotations;
+    const float* sh;
+    const uint4* geometry;   // F16: 2 x uint4 per gaussian
+    const uint4* color;      // F16: colorQuads x uint4 per gaussian
+    uint32_t colorQuads;     // uint4s per colour record
+};
+
+// Sorting: stable LSD radix sort, 8-bit digits, 64-bit keys (tile << 32 | depth bits).
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortKeysPerLane = 8;                               // per wave round-robin
+constexpr int kSortTile = kSortThreads * kSortKeysPerLane;        // 2048 keys per block tile
+constexpr uint32_t kMaxSortBlocks = 1024;
+
+// Launch wrappers (each in its .hip file).
+void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
+                      uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
+                      hipStream_t stream);
+void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
+                      uint32_t* blockOffsets, gsr_control* control, hipStream_t stream);
+void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
+                     const gsr_control* control, uint32_t count, uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
+                     uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream);
+// Returns which buffer (0 = keysA/valuesA, 1 = B) holds the sorted result.
+int launchRadixSort(uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB, uint32_t maxCount,
+                    const uint32_t* countDevice, uint32_t keyBits, uint32_t blocks, uint32_t* hist,
+                    uint32_t* digitTotals, hipStream_t stream);
+void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
+                  uint2* ranges, hipStream_t stream);
+void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+                  gsr_target_format format, void* out, hipStream_t stream);
+
+// Stand-alone scan (GpuScan): reduce, block-sum scan, downsweep.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanBlock = kScanThreads * kScanItems;  // 2048
+void launchExclusiveScan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* blockSums, uint32_t* total,
+                         hipStream_t stream);
+
+}  // namespace gsr

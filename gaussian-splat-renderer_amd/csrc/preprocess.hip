@@ -1,0 +1,320 @@
+// K1 splat_preprocess (reference shaders/splat_preprocess.comp:273-355 + 411-450, the GSR_TILES
+// build; CPU twin splat/splat_math.cpp:153-193): one thread per gaussian.
+//
+//   - near cull !(z > 0.2) (NaN-safe), opacity cull o < 1/255, EWA covariance, conic, radius,
+//     centre, tile rectangle (off-screen cull), SH colour;
+//   - the 48-byte Splat of every gaussian (culled ones carry depth + reason) and its tile
+//     count: the rectangle's area (rect) or the tiles of the alpha >= 1/255 ellipse (tight);
+//   - fused reduce of the reduce-then-scan: each 256-thread block writes one BlockStats
+//     (sum of its tile counts, non-zero counts, per-reason cull counts), so there are no
+//     global atomics and the scan reads 32 B per block instead of 4 B per gaussian;
+//   - zeroes the per-tile ranges for the ranges pass later in the frame.
+#include "gsr_device.hpp"
+
+namespace gsr {
+
+namespace {
+
+__device__ __forceinline__ float halfBitsToFloat(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16;
+    const uint32_t exponent = (h >> 10) & 0x1Fu;
+    const uint32_t mant = h & 0x3FFu;
+    if (exponent == 0) {
+        const float v = float(mant) * (1.0f / 16777216.0f);
+        return sign ? -v : v;
+    }
+    if (exponent == 31) return __uint_as_float(sign | 0x7F800000u | (mant << 13));
+    return __uint_as_float(sign | ((exponent + 112u) << 23) | (mant << 13));
+}
+
+constexpr float kC0 = 0.28209479177387814f;
+constexpr float kC1 = 0.4886025119029199f;
+constexpr float kC2_0 = 1.0925484305920792f, kC2_1 = -1.0925484305920792f, kC2_2 = 0.31539156525252005f,
+                kC2_3 = -1.0925484305920792f, kC2_4 = 0.5462742152960396f;
+constexpr float kC3_0 = -0.5900435899266435f, kC3_1 = 2.890611442640554f, kC3_2 = -0.4570457994644658f,
+                kC3_3 = 0.3731763325901154f, kC3_4 = -0.4570457994644658f, kC3_5 = 1.445305721320277f,
+                kC3_6 = -0.5900435899266435f;
+
+// evalSH (splat_math.cpp:129-151), one channel; c = the 16 coefficients of that channel.
+__device__ __forceinline__ float evalShChannel(int degree, const float* c, float x, float y, float z) {
+    float result = kC0 * c[0];
+    if (degree > 0) {
+        result = result - kC1 * y * c[1] + kC1 * z * c[2] - kC1 * x * c[3];
+        if (degree > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z;
+            const float xy = x * y, yz = y * z, xz = x * z;
+            result = result + kC2_0 * xy * c[4] + kC2_1 * yz * c[5] + kC2_2 * (2.0f * zz - xx - yy) * c[6] +
+                     kC2_3 * xz * c[7] + kC2_4 * (xx - yy) * c[8];
+            if (degree > 2) {
+                result = result + kC3_0 * y * (3.0f * xx - yy) * c[9] + kC3_1 * xy * z * c[10] +
+                         kC3_2 * y * (4.0f * zz - xx - yy) * c[11] +
+                         kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * c[12] +
+                         kC3_4 * x * (4.0f * zz - xx - yy) * c[13] + kC3_5 * z * (xx - yy) * c[14] +
+                         kC3_6 * x * (xx - 3.0f * yy) * c[15];
+            }
+        }
+    }
+    result = result + 0.5f;
+    return result < 0.0f ? 0.0f : result;
+}
+
+// The coefficients of the evaluated bands, channel-major: co[ch][k].
+template <int FMT>
+__device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, float co[3][16]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) co[0][k] = co[1][k] = co[2][k] = 0.0f;
+    if (FMT == GSR_SCENE_F32_SOA) {
+        const float* p = s.sh + size_t(i) * s.K * 3;
+        if ((s.K * 3) % 4 == 0) {
+            const float4* p4 = reinterpret_cast<const float4*>(p);
+            const int quads = (used * 3 + 3) / 4;
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                if (q < quads) {
+                    const float4 v = p4[q];
+                    const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int j = q * 4 + e;
+                        if (j < used * 3) co[j % 3][j / 3] = vv[e];
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 48; ++j)
+                if (j < used * 3 && uint32_t(j) < s.K * 3) co[j % 3][j / 3] = p[j];
+        }
+    } else {
+        const uint4* p = s.color + size_t(i) * s.colorQuads;
+        const int quads = (used * 3 + 7) / 8;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            if (q < quads) {
+                const uint4 v = p[q];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int j = q * 8 + e;
+                    const uint32_t h = (e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xFFFFu);
+                    if (j < used * 3) co[j % 3][j / 3] = halfBitsToFloat(h);
+                }
+            }
+        }
+    }
+}
+
+template <int FMT>
+__global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene, DevCamera cam, uint32_t bounds,
+                                                                  float4* __restrict__ splats,
+                                                                  uint32_t* __restrict__ tileCounts,
+                                                                  BlockStats* __restrict__ blockStats,
+                                                                  uint32_t* __restrict__ rangesWords,
+                                                                  uint32_t rangeWordCount) {
+    const uint32_t i = blockIdx.x * kPreprocessThreads + threadIdx.x;
+    if (i < rangeWordCount) rangesWords[i] = 0u;
+
+    uint32_t reason = kNoGaussian;
+    uint32_t touched = 0;
+    if (i < scene.count) {
+        float px, py, pz, alpha;
+        uint4 g0, g1;
+        if (FMT == GSR_SCENE_F32_SOA) {
+            const float4 pa = scene.posAlpha[i];
+            px = pa.x;
+            py = pa.y;
+            pz = pa.z;
+            alpha = pa.w;
+        } else {
+            g0 = scene.geometry[2 * size_t(i)];
+            g1 = scene.geometry[2 * size_t(i) + 1];
+            px = __uint_as_float(g0.x);
+            py = __uint_as_float(g0.y);
+            pz = __uint_as_float(g0.z);
+            alpha = halfBitsToFloat(g0.w & 0xFFFFu);
+        }
+        const float* v = cam.view;
+        // transformPoint (splat_math.h:65-69), per component, no contraction.
+        const float t0 = v[0] * px + v[4] * py + v[8] * pz + v[12];
+        const float t1 = v[1] * px + v[5] * py + v[9] * pz + v[13];
+        const float t2 = v[2] * px + v[6] * py + v[10] * pz + v[14];
+
+        float4 o0 = make_float4(0.0f, 0.0f, t2, 0.0f);
+        float4 o1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 o2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (!(t2 > kNearCullDepth)) {
+            reason = GSR_CULL_NEAR_PLANE;
+        } else if (alpha < kMinAlpha) {
+            reason = GSR_CULL_LOW_OPACITY;
+        } else {
+            float sx, sy, sz, qw, qx, qy, qz;
+            if (FMT == GSR_SCENE_F32_SOA) {
+                const float4 sc = scene.scales[i];
+                const float4 q = scene.rotations[i];
+                sx = sc.x;
+                sy = sc.y;
+                sz = sc.z;
+                qw = q.x;
+                qx = q.y;
+                qy = q.z;
+                qz = q.w;
+            } else {
+                sx = halfBitsToFloat(g0.w >> 16);
+                sy = halfBitsToFloat(g1.x & 0xFFFFu);
+                sz = halfBitsToFloat(g1.x >> 16);
+                const float hw = halfBitsToFloat(g1.y & 0xFFFFu), hx = halfBitsToFloat(g1.y >> 16);
+                const float hy = halfBitsToFloat(g1.z & 0xFFFFu), hz = halfBitsToFloat(g1.z >> 16);
+                // decodeGaussian (scene_compression.cpp:408-415): q * (1 / sqrt(dot(q, q))).
+                const float len2 = hw * hw + hx * hx + hy * hy + hz * hz;
+                if (len2 > 0.0f) {
+                    const float inv = 1.0f / sqrtf(len2);
+                    qw = hw * inv;
+                    qx = hx * inv;
+                    qy = hy * inv;
+                    qz = hz * inv;
+                } else {
+                    qw = 1.0f;
+                    qx = qy = qz = 0.0f;
+                }
+            }
+            // quaternionToMatrix + computeCov3D (splat_math.cpp:44-63), R[col][row].
+            const float R00 = 1.0f - 2.0f * (qy * qy + qz * qz), R01 = 2.0f * (qx * qy + qw * qz),
+                        R02 = 2.0f * (qx * qz - qw * qy);
+            const float R10 = 2.0f * (qx * qy - qw * qz), R11 = 1.0f - 2.0f * (qx * qx + qz * qz),
+                        R12 = 2.0f * (qy * qz + qw * qx);
+            const float R20 = 2.0f * (qx * qz + qw * qy), R21 = 2.0f * (qy * qz - qw * qx),
+                        R22 = 1.0f - 2.0f * (qx * qx + qy * qy);
+            const float M[3][3] = {{R00 * sx, R01 * sx, R02 * sx}, {R10 * sy, R11 * sy, R12 * sy},
+                                   {R20 * sz, R21 * sz, R22 * sz}};
+            float S[3][3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) S[j][k] = M[0][k] * M[0][j] + M[1][k] * M[1][j] + M[2][k] * M[2][j];
+            // computeCov2D (splat_math.cpp:65-91).
+            const float limX = kFrustumClampFactor * cam.tanX;
+            const float limY = kFrustumClampFactor * cam.tanY;
+            const float txtz = t0 / t2;
+            const float tytz = t1 / t2;
+            const float cx = txtz < -limX ? -limX : txtz;
+            const float cy = tytz < -limY ? -limY : tytz;
+            const float tx = (limX < cx ? limX : cx) * t2;
+            const float ty = (limY < cy ? limY : cy) * t2;
+            const float tz = t2;
+            const float j00 = cam.focalX / tz;
+            const float j20 = -(cam.focalX * tx) / (tz * tz);
+            const float j11 = cam.focalY / tz;
+            const float j21 = -(cam.focalY * ty) / (tz * tz);
+            float T[3][2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                T[j][0] = j00 * v[j * 4 + 0] + j20 * v[j * 4 + 2];
+                T[j][1] = j11 * v[j * 4 + 1] + j21 * v[j * 4 + 2];
+            }
+            float U[3][2];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 2; ++k) U[j][k] = T[0][k] * S[j][0] + T[1][k] * S[j][1] + T[2][k] * S[j][2];
+            const float c00 = U[0][0] * T[0][0] + U[1][0] * T[1][0] + U[2][0] * T[2][0];
+            const float c01 = U[0][1] * T[0][0] + U[1][1] * T[1][0] + U[2][1] * T[2][0];
+            const float c11 = U[0][1] * T[0][1] + U[1][1] * T[1][1] + U[2][1] * T[2][1];
+            const float ca = c00 + kLowPassFilter, cb = c01, cc = c11 + kLowPassFilter;
+            // conicAndRadius (splat_math.cpp:102-114).
+            const float det = ca * cc - cb * cb;
+            if (det == 0.0f) {
+                reason = GSR_CULL_SINGULAR_COV;
+            } else {
+                const float detInv = 1.0f / det;
+                const float ka = cc * detInv, kb = -cb * detInv, kc = ca * detInv;
+                const float mid = 0.5f * (ca + cc);
+                const float disc = mid * mid - det;
+                const float root = sqrtf(disc > 0.1f ? disc : 0.1f);
+                const float l1 = mid + root, l2 = mid - root;
+                const float radius = ceilf(3.0f * sqrtf(l1 > l2 ? l1 : l2));
+                // projectToPixel (splat_math.cpp:93-100).
+                const float ndcX = t0 / (t2 * cam.tanX);
+                const float ndcY = t1 / (t2 * cam.tanY);
+                const float ctrX = ((ndcX + 1.0f) * float(cam.width) - 1.0f) * 0.5f;
+                const float ctrY = ((ndcY + 1.0f) * float(cam.height) - 1.0f) * 0.5f;
+                const Rect rect = tileRect(ctrX, ctrY, radius, cam.tilesX, cam.tilesY);
+                o0.x = ctrX;
+                o0.y = ctrY;
+                if (rect.x1 <= rect.x0 || rect.y1 <= rect.y0) {
+                    reason = GSR_CULL_OFF_SCREEN;
+                } else {
+                    // SH colour, dir = normalize(mean - camPos) in data space.
+                    const int degree = scene.degree;
+                    const int used = (degree + 1) * (degree + 1);
+                    float co[3][16];
+                    loadSh<FMT>(scene, i, used, co);
+                    const float dx = px - cam.pos[0], dy = py - cam.pos[1], dz = pz - cam.pos[2];
+                    const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+                    const float x = dx * inv, y = dy * inv, z = dz * inv;
+                    o2.x = evalShChannel(degree, co[0], x, y, z);
+                    o2.y = evalShChannel(degree, co[1], x, y, z);
+                    o2.z = evalShChannel(degree, co[2], x, y, z);
+                    o2.w = radius;
+                    o1 = make_float4(ka, kb, kc, alpha);
+                    reason = GSR_CULL_VISIBLE;
+                    if (bounds == GSR_BOUNDS_RECT) {
+                        touched = uint32_t((rect.x1 - rect.x0) * (rect.y1 - rect.y0));
+                    } else {
+                        const Coverage cv = makeCoverage(bounds, rect, int(cam.tilesX), int(cam.tilesY), ctrX, ctrY,
+                                                         ka, kb, kc, alpha);
+                        touched = coverageCount(cv);
+                    }
+                }
+            }
+        }
+        o0.w = __uint_as_float(reason);
+        float4* dst = splats + 3 * size_t(i);
+        dst[0] = o0;
+        dst[1] = o1;
+        dst[2] = o2;
+        tileCounts[i] = touched;
+    }
+
+    // Block reduction: wave ballots for the cull reasons, shuffles for the tile sum.
+    __shared__ uint32_t sStats[kPreprocessThreads / 64][8];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t sum = touched;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    const uint32_t nz = uint32_t(__popcll(__ballot(touched != 0u)));
+    uint32_t cull[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) cull[r] = uint32_t(__popcll(__ballot(reason == uint32_t(r))));
+    if (lane == 0) {
+        sStats[wave][0] = sum;
+        sStats[wave][1] = nz;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) sStats[wave][2 + r] = cull[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < kPreprocessThreads / 64; ++w) acc += sStats[w][threadIdx.x];
+        reinterpret_cast<uint32_t*>(&blockStats[blockIdx.x])[threadIdx.x] = acc;
+    }
+    if (threadIdx.x == 7) blockStats[blockIdx.x].pad = 0u;
+}
+
+}  // namespace
+
+void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
+                      uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
+                      hipStream_t stream) {
+    uint32_t n = scene.count > rangeWordCount ? scene.count : rangeWordCount;
+    const uint32_t blocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
+    if (blocks == 0) return;
+    if (scene.format == GSR_SCENE_F32_SOA)
+        hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene,
+                           cam, bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount);
+    else
+        hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
+                           bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount);
+}
+
+}  // namespace gsr

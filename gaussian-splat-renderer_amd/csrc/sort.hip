@@ -1,0 +1,229 @@
+// K6 radix sort (reference GpuRadixSort, render/gpu_radix_sort.{h,cpp}; shaders
+// radix_histogram.comp, radix_scan.comp, radix_scatter_reorder.comp): stable LSD sort of
+// (64-bit key, u32 value) pairs, 8-bit digits, one histogram / scan / scatter per digit.
+//
+// gfx950 design:
+//   - the key count lives on the device (the frame's entries = min(M, capacity)); the grid
+//     is fixed (G blocks) and block b sorts the contiguous slice [b*chunk, (b+1)*chunk),
+//     chunk = ceil(count / G), in tiles of 2048 keys: no host read-back, no indirect launch;
+//   - histogram: per-wave LDS histograms, written digit-major hist[d * G + b];
+//   - scan: one workgroup per digit scans its G block counts (exclusive) and the digit total;
+//   - scatter: each wave64 owns 512 consecutive keys of the tile and ranks them with 8
+//     64-lane ballots per key (the lanes holding the same digit) plus a running per-wave
+//     digit counter in LDS: ranks follow the input order (stable) without any workgroup
+//     barrier inside the wave's rounds. The tile is then reordered by digit in LDS and
+//     written out as contiguous per-digit runs (coalesced stores).
+#include "gsr_internal.hpp"
+
+namespace gsr {
+
+namespace {
+
+__device__ __forceinline__ uint32_t sortCount(const uint32_t* countDevice, uint32_t maxCount) {
+    if (!countDevice) return maxCount;
+    const uint32_t c = *countDevice;
+    return c < maxCount ? c : maxCount;
+}
+
+__device__ __forceinline__ uint32_t digitOf(uint64_t key, uint32_t shift) { return uint32_t(key >> shift) & 0xFFu; }
+
+__global__ __launch_bounds__(kSortThreads) void kSortHistogram(const uint64_t* __restrict__ keys,
+                                                               const uint32_t* __restrict__ countDevice,
+                                                               uint32_t maxCount, uint32_t shift,
+                                                               uint32_t* __restrict__ hist) {
+    __shared__ uint32_t sHist[kSortWaves][kRadix];
+    const uint32_t t = threadIdx.x, wave = t >> 6;
+    for (uint32_t d = t; d < kSortWaves * kRadix; d += kSortThreads) (&sHist[0][0])[d] = 0u;
+    __syncthreads();
+    const uint32_t count = sortCount(countDevice, maxCount);
+    const uint32_t G = gridDim.x;
+    const uint32_t chunk = (count + G - 1) / G;
+    const uint32_t begin = blockIdx.x * chunk;
+    const uint32_t end = begin + chunk < count ? begin + chunk : count;
+    for (uint32_t idx = begin + t; idx < end; idx += kSortThreads) atomicAdd(&sHist[wave][digitOf(keys[idx], shift)], 1u);
+    __syncthreads();
+    for (uint32_t d = t; d < kRadix; d += kSortThreads) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) s += sHist[w][d];
+        hist[d * G + blockIdx.x] = s;
+    }
+}
+
+// Exclusive scan of v over the 256 threads of a workgroup.
+__device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* sWave, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t n = __shfl_up(inc, off, 64);
+        if (lane >= uint32_t(off)) inc += n;
+    }
+    if (lane == 63) sWave[wave] = inc;
+    __syncthreads();
+    uint32_t prefix = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w)
+        if (uint32_t(w) < wave) prefix += sWave[w];
+    if (total) *total = sWave[0] + sWave[1] + sWave[2] + sWave[3];
+    __syncthreads();
+    return prefix + inc - v;
+}
+
+// One workgroup per digit: exclusive scan of hist[d * G .. d * G + G), digit total.
+__global__ __launch_bounds__(kSortThreads) void kSortScan(uint32_t* __restrict__ hist, uint32_t G,
+                                                          uint32_t* __restrict__ digitTotals) {
+    __shared__ uint32_t sWave[kSortWaves];
+    uint32_t* row = hist + size_t(blockIdx.x) * G;
+    constexpr int kPer = int(kMaxSortBlocks) / kSortThreads;  // 4
+    const uint32_t first = threadIdx.x * kPer;
+    uint32_t v[kPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        v[j] = first + j < G ? row[first + j] : 0u;
+        s += v[j];
+    }
+    uint32_t total;
+    uint32_t run = blockExclusiveScan256(s, sWave, &total);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        if (first + j < G) row[first + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0) digitTotals[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kSortThreads) void kSortScatter(const uint64_t* __restrict__ keysIn,
+                                                             const uint32_t* __restrict__ valsIn,
+                                                             uint64_t* __restrict__ keysOut,
+                                                             uint32_t* __restrict__ valsOut,
+                                                             const uint32_t* __restrict__ countDevice,
+                                                             uint32_t maxCount, uint32_t shift,
+                                                             const uint32_t* __restrict__ hist,
+                                                             const uint32_t* __restrict__ digitTotals) {
+    __shared__ uint64_t sKeys[kSortTile];
+    __shared__ uint32_t sVals[kSortTile];
+    __shared__ uint32_t sWaveHist[kSortWaves][kRadix];
+    __shared__ uint32_t sGlobalBase[kRadix];
+    __shared__ uint32_t sLocalStart[kRadix];
+    __shared__ uint32_t sTileCount[kRadix];
+    __shared__ uint32_t sWave[kSortWaves];
+
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t count = sortCount(countDevice, maxCount);
+    const uint32_t G = gridDim.x;
+    const uint32_t chunk = (count + G - 1) / G;
+    const uint32_t begin = blockIdx.x * chunk;
+    const uint32_t end = begin + chunk < count ? begin + chunk : count;
+    if (begin >= end) return;  // uniform
+
+    {  // global base of (digit, block) = exclusive prefix of the digit totals + the in-digit scan
+        const uint32_t d = t;  // kSortThreads == kRadix
+        const uint32_t start = blockExclusiveScan256(digitTotals[d], sWave, nullptr);
+        sGlobalBase[d] = start + hist[d * G + blockIdx.x];
+    }
+    const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
+
+    for (uint32_t tileBegin = begin; tileBegin < end; tileBegin += kSortTile) {
+        const uint32_t tileN = end - tileBegin < uint32_t(kSortTile) ? end - tileBegin : uint32_t(kSortTile);
+        for (uint32_t d = lane; d < kRadix; d += 64) sWaveHist[wave][d] = 0u;
+
+        uint64_t key[kSortKeysPerLane];
+        uint32_t val[kSortKeysPerLane], dig[kSortKeysPerLane], rank[kSortKeysPerLane];
+#pragma unroll
+        for (int r = 0; r < kSortKeysPerLane; ++r) {
+            const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
+            const bool valid = pos < tileN;
+            key[r] = valid ? keysIn[tileBegin + pos] : 0ull;
+            val[r] = valid ? valsIn[tileBegin + pos] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < kSortKeysPerLane; ++r) {
+            const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
+            const bool valid = pos < tileN;
+            const uint32_t d = digitOf(key[r], shift);
+            dig[r] = d;
+            // Lanes holding the same digit: 8 ballots, one per digit bit.
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < kRadixBits; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t bal = __ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            if (valid) {
+                const uint64_t lower = peers & laneMaskLt;
+                const uint32_t old = sWaveHist[wave][d];
+                rank[r] = old + uint32_t(__popcll(lower));
+                if (lower == 0u) sWaveHist[wave][d] = old + uint32_t(__popcll(peers));
+            }
+        }
+        __syncthreads();
+        {
+            const uint32_t d = t;
+            uint32_t run = 0;
+#pragma unroll
+            for (int w = 0; w < kSortWaves; ++w) {
+                const uint32_t c = sWaveHist[w][d];
+                sWaveHist[w][d] = run;
+                run += c;
+            }
+            sTileCount[d] = run;
+            sLocalStart[d] = blockExclusiveScan256(run, sWave, nullptr);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortKeysPerLane; ++r) {
+            const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
+            if (pos < tileN) {
+                const uint32_t lr = sLocalStart[dig[r]] + sWaveHist[wave][dig[r]] + rank[r];
+                sKeys[lr] = key[r];
+                sVals[lr] = val[r];
+            }
+        }
+        __syncthreads();
+        for (uint32_t i = t; i < tileN; i += kSortThreads) {
+            const uint64_t k = sKeys[i];
+            const uint32_t d = digitOf(k, shift);
+            const uint32_t dst = sGlobalBase[d] + (i - sLocalStart[d]);
+            keysOut[dst] = k;
+            valsOut[dst] = sVals[i];
+        }
+        __syncthreads();
+        sGlobalBase[t] += sTileCount[t];
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int launchRadixSort(uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB, uint32_t maxCount,
+                    const uint32_t* countDevice, uint32_t keyBits, uint32_t blocks, uint32_t* hist,
+                    uint32_t* digitTotals, hipStream_t stream) {
+    if (blocks == 0) blocks = 1;
+    if (blocks > kMaxSortBlocks) blocks = kMaxSortBlocks;
+    if (maxCount == 0) return 0;
+    const uint32_t passes = (keyBits + kRadixBits - 1) / kRadixBits;
+    uint64_t* kin = keysA;
+    uint32_t* vin = valuesA;
+    uint64_t* kout = keysB;
+    uint32_t* vout = valuesB;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t shift = p * kRadixBits;
+        hipLaunchKernelGGL(kSortHistogram, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice, maxCount,
+                           shift, hist);
+        hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, hist, blocks, digitTotals);
+        hipLaunchKernelGGL(kSortScatter, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout, vout,
+                           countDevice, maxCount, shift, hist, digitTotals);
+        uint64_t* tk = kin;
+        kin = kout;
+        kout = tk;
+        uint32_t* tv = vin;
+        vin = vout;
+        vout = tv;
+    }
+    return int(passes & 1u);
+}
+
+}  // namespace gsr

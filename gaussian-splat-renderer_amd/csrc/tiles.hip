@@ -1,0 +1,187 @@
+// K5 tile_duplicate, global 64-bit keys, per entry (reference shaders/tile_duplicate.comp:119-173,
+// GSR_TILE_GLOBAL64 + GSR_DUPLICATE_PER_ENTRY) and K7 tile_ranges (shaders/tile_ranges.comp:22-43).
+//
+// Duplicate: one workgroup per 256 gaussians (the preprocess's blocks). It finishes the
+// reduce-then-scan itself: the in-block exclusive scan of the tile counts plus the block's
+// scanned offset gives every gaussian its first entry, so no per-gaussian offsets array is
+// written or binary-searched in HBM. Each gaussian's coverage (rectangle, or the cutoff
+// ellipse and its rows) is set up once into LDS; then the block's entries are handed out
+// to all 256 threads (load-balanced over big splats) and each finds its gaussian by an
+// 8-step binary search in LDS and its tile by walking that coverage's rows. Keys are
+// (tile << 32 | depth bits), values the gaussian index; entries >= capacity are dropped
+// (index order), the tile id #tiles marks an entry without a tile (never produced).
+#include "gsr_device.hpp"
+
+namespace gsr {
+
+namespace {
+
+constexpr int kDupThreads = kPreprocessThreads;
+
+__global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restrict__ splats,
+                                                          const uint32_t* __restrict__ tileCounts,
+                                                          const uint32_t* __restrict__ blockOffsets,
+                                                          const gsr_control* __restrict__ control, uint32_t count,
+                                                          uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
+                                                          gsr_control* controlRW) {
+    __shared__ uint32_t sScan[kDupThreads];
+    __shared__ uint32_t sWave[kDupThreads / 64];
+    // Per-gaussian coverage, SoA in LDS.
+    __shared__ int sBins[4][kDupThreads];
+    __shared__ float sEl[9][kDupThreads];  // cx cy A B C D t exPad eyPad
+    __shared__ int sRows[2][kDupThreads];
+    __shared__ uint32_t sDepth[kDupThreads];
+    __shared__ uint8_t sUseEllipse[kDupThreads];
+
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = blockIdx.x * kDupThreads + t;
+    const uint32_t c = i < count ? tileCounts[i] : 0u;
+
+    // Block exclusive scan of the counts.
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    uint32_t v = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t n = __shfl_up(v, off, 64);
+        if (lane >= uint32_t(off)) v += n;
+    }
+    if (lane == 63) sWave[wave] = v;
+    __syncthreads();
+    uint32_t prefix = 0;
+    for (uint32_t w = 0; w < wave; ++w) prefix += sWave[w];
+    const uint32_t total = sWave[0] + sWave[1] + sWave[2] + sWave[3];
+    const uint32_t excl = prefix + v - c;
+    sScan[t] = excl;
+
+    const uint32_t base = blockOffsets[blockIdx.x];
+    const uint32_t entries = control->entries;
+    if (base >= entries || total == 0) return;  // uniform: the whole block is past the capacity or empty
+
+    if (c != 0u) {
+        const float4* s = splats + 3 * size_t(i);
+        const float4 o0 = s[0], o1 = s[1], o2 = s[2];
+        const Rect rect = tileRect(o0.x, o0.y, o2.w, tilesX, tilesY);
+        const Coverage cv = makeCoverage(bounds, rect, int(tilesX), int(tilesY), o0.x, o0.y, o1.x, o1.y, o1.z, o1.w);
+        int r0, r1;
+        coverageRows(cv, &r0, &r1);
+        sBins[0][t] = cv.bins.x0;
+        sBins[1][t] = cv.bins.y0;
+        sBins[2][t] = cv.bins.x1;
+        sBins[3][t] = cv.bins.y1;
+        sEl[0][t] = cv.e.cx;
+        sEl[1][t] = cv.e.cy;
+        sEl[2][t] = cv.e.A;
+        sEl[3][t] = cv.e.B;
+        sEl[4][t] = cv.e.C;
+        sEl[5][t] = cv.e.D;
+        sEl[6][t] = cv.e.t;
+        sEl[7][t] = cv.e.exPad;
+        sEl[8][t] = cv.e.eyPad;
+        sRows[0][t] = r0;
+        sRows[1][t] = r1;
+        sDepth[t] = __float_as_uint(o0.z);
+        sUseEllipse[t] = cv.useEllipse ? 1 : 0;
+        // Count/write agreement (tile_duplicate.comp:375-376), once per gaussian whose
+        // first entry is written.
+        if (bounds != GSR_BOUNDS_RECT && base + excl < entries && coverageCount(cv) != c)
+            atomicAdd(&controlRW->coverage_mismatch, 1u);
+    }
+    __syncthreads();
+
+    const uint32_t end = (base + total < entries) ? base + total : entries;
+    const uint32_t gridTiles = tilesX * tilesY;
+    for (uint32_t k = base + t; k < end; k += kDupThreads) {
+        const uint32_t rl = k - base;
+        uint32_t lo = 0, hi = kDupThreads;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sScan[mid] <= rl) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t r = rl - sScan[lo];
+        uint32_t tile = gridTiles;
+        Coverage cv;
+        cv.bins = Rect{sBins[0][lo], sBins[1][lo], sBins[2][lo], sBins[3][lo]};
+        cv.useEllipse = sUseEllipse[lo] != 0;
+        const int r0 = sRows[0][lo], r1 = sRows[1][lo];
+        if (!cv.useEllipse) {
+            const int w = cv.bins.x1 - cv.bins.x0;
+            if (w > 0 && r < uint32_t(w) * uint32_t(r1 - r0))
+                tile = (uint32_t(r0) + r / uint32_t(w)) * tilesX + uint32_t(cv.bins.x0) + r % uint32_t(w);
+        } else {
+            cv.e.cx = sEl[0][lo];
+            cv.e.cy = sEl[1][lo];
+            cv.e.A = sEl[2][lo];
+            cv.e.B = sEl[3][lo];
+            cv.e.C = sEl[4][lo];
+            cv.e.D = sEl[5][lo];
+            cv.e.t = sEl[6][lo];
+            cv.e.exPad = sEl[7][lo];
+            cv.e.eyPad = sEl[8][lo];
+            cv.e.valid = true;
+            uint32_t before = 0;
+            for (int row = r0; row < r1; ++row) {
+                int x0, x1;
+                coverageRowSpan(cv, row, &x0, &x1);
+                const uint32_t w = uint32_t(x1 - x0);
+                if (r < before + w) {
+                    tile = uint32_t(row) * tilesX + uint32_t(x0) + (r - before);
+                    break;
+                }
+                before += w;
+            }
+        }
+        keys[k] = (uint64_t(tile) << 32) | uint64_t(sDepth[lo]);
+        values[k] = blockIdx.x * kDupThreads + lo;
+    }
+}
+
+constexpr int kRangesThreads = 256;
+
+__global__ __launch_bounds__(kRangesThreads) void kRanges(const uint64_t* __restrict__ keys,
+                                                          const gsr_control* __restrict__ control, uint32_t tileCount,
+                                                          uint2* __restrict__ ranges) {
+    const uint32_t n = control->entries;
+    for (uint32_t k = blockIdx.x * kRangesThreads + threadIdx.x; k < n; k += gridDim.x * kRangesThreads) {
+        const uint32_t tile = uint32_t(keys[k] >> 32);
+        if (tile >= tileCount) {
+            if (k > 0) {
+                const uint32_t prev = uint32_t(keys[k - 1] >> 32);
+                if (prev < tileCount) ranges[prev].y = k;
+            }
+            continue;
+        }
+        if (k == 0) {
+            ranges[tile].x = 0u;
+        } else {
+            const uint32_t prev = uint32_t(keys[k - 1] >> 32);
+            if (prev != tile) {
+                if (prev < tileCount) ranges[prev].y = k;
+                ranges[tile].x = k;
+            }
+        }
+        if (k == n - 1) ranges[tile].y = n;
+    }
+}
+
+}  // namespace
+
+void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
+                     const gsr_control* control, uint32_t count, uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
+                     uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream) {
+    const uint32_t blocks = (count + kDupThreads - 1) / kDupThreads;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kDuplicate, dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts, blockOffsets,
+                       control, count, bounds, tilesX, tilesY, keys, values, controlRW);
+}
+
+void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
+                  uint2* ranges, hipStream_t stream) {
+    uint32_t blocks = (capacity + kRangesThreads - 1) / kRangesThreads;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kRanges, dim3(blocks), dim3(kRangesThreads), 0, stream, keys, control, tileCount, ranges);
+}
+
+}  // namespace gsr

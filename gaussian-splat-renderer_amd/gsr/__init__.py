@@ -1,0 +1,473 @@
+"""Python binding of libgsr.so, the MI355X (gfx950) splat-tiles forward renderer.
+
+The C ABI (include/gsr.h) is the product; this module is a thin ctypes layer over it that
+mirrors the reference's SplatTileRenderer host interface (render/splat_tile_renderer.h:143-196):
+
+    ctx = gsr.Context(0)
+    scene = gsr.Scene(ctx, pos_alpha, scales, rotations, sh, sh_degree=3)        # GpuScene::upload
+    r = gsr.Renderer(ctx, scene, bounds="tight", target="rgba8")                  # init
+    img = r.render_to_host(cam)                                                   # render + readBack
+    keys, depth, values, ranges, control = r.read_back()                          # readBack
+    splats, counts = r.read_back_splats()                                         # readBackSplats
+
+Device memory is owned by the library (no torch on the path). The library must be built
+(``__graft_entry__.build()``); importing this module without it raises, there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import camera, scenes  # noqa: F401  (re-exported helpers)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgsr.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libgsr.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+_lib = ctypes.CDLL(LIB_PATH)
+
+# --- enums (gsr.h) ----------------------------------------------------------------------------
+SCENE_F32_SOA, SCENE_F16 = 0, 1
+BOUNDS = {"rect": 0, "tight": 1, "cutoff-only": 2, "cutoff": 2}
+TARGETS = {"rgba8": 0, "rgba32f": 1}
+STAGES = ("preprocess", "scan", "duplicate", "sort", "ranges", "render", "total")
+CULL_VISIBLE, CULL_NEAR, CULL_SINGULAR, CULL_OFFSCREEN, CULL_LOW_OPACITY = 0, 1, 2, 3, 4
+
+
+class GsrError(RuntimeError):
+    pass
+
+
+class Camera(ctypes.Structure):
+    """gsr_camera = SplatCamera (splat_math.h:43-50)."""
+
+    _fields_ = [
+        ("view_inria", ctypes.c_float * 16),
+        ("position", ctypes.c_float * 3),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("tan_half_fov_x", ctypes.c_float),
+        ("tan_half_fov_y", ctypes.c_float),
+        ("focal_x", ctypes.c_float),
+        ("focal_y", ctypes.c_float),
+        ("tiles_x", ctypes.c_uint32),
+        ("tiles_y", ctypes.c_uint32),
+    ]
+
+    @property
+    def view(self) -> np.ndarray:
+        return np.array(self.view_inria[:], dtype=np.float32)
+
+    @property
+    def tiles(self) -> int:
+        return int(self.tiles_x) * int(self.tiles_y)
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("count", ctypes.c_uint32),
+        ("sh_degree", ctypes.c_int32),
+        ("format", ctypes.c_int32),
+        ("device_pointers", ctypes.c_int32),
+        ("pos_alpha", ctypes.c_void_p),
+        ("scales", ctypes.c_void_p),
+        ("rotations", ctypes.c_void_p),
+        ("sh", ctypes.c_void_p),
+        ("geometry", ctypes.c_void_p),
+        ("color", ctypes.c_void_p),
+    ]
+
+
+class Settings(ctypes.Structure):
+    _fields_ = [
+        ("bounds", ctypes.c_int32),
+        ("budget_mb", ctypes.c_uint32),
+        ("max_sh_degree", ctypes.c_int32),
+        ("format", ctypes.c_int32),
+        ("profile_stages", ctypes.c_int32),
+        ("sort_blocks", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 6),
+    ]
+
+
+class Control(ctypes.Structure):
+    """gsr_control = TileControl (splat_tile_renderer.h:116-128), 80 bytes."""
+
+    _fields_ = [
+        ("gaussian_count", ctypes.c_uint32),
+        ("visible", ctypes.c_uint32),
+        ("duplicates", ctypes.c_uint32),
+        ("entries", ctypes.c_uint32),
+        ("cull_counts", ctypes.c_uint32 * 8),
+        ("overflow", ctypes.c_uint32),
+        ("ranges_groups", ctypes.c_uint32 * 3),
+        ("coverage_mismatch", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32 * 3),
+    ]
+
+    def as_dict(self) -> dict:
+        return {
+            "gaussian_count": self.gaussian_count,
+            "visible": self.visible,
+            "duplicates": self.duplicates,
+            "entries": self.entries,
+            "cull_counts": list(self.cull_counts),
+            "overflow": self.overflow,
+            "coverage_mismatch": self.coverage_mismatch,
+        }
+
+
+assert ctypes.sizeof(Control) == 80
+assert ctypes.sizeof(Camera) == 108
+
+# 48-byte ProjectedSplat as a numpy record.
+SPLAT_DTYPE = np.dtype(
+    [
+        ("center", np.float32, 2),
+        ("depth", np.float32),
+        ("cull_reason", np.uint32),
+        ("conic_opacity", np.float32, 4),
+        ("color", np.float32, 3),
+        ("radius", np.float32),
+    ]
+)
+assert SPLAT_DTYPE.itemsize == 48
+
+_vp = ctypes.c_void_p
+_u32, _i32, _u64, _f32, _sz = ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_float, ctypes.c_size_t
+
+
+def _sig(name, restype, argtypes):
+    fn = getattr(_lib, name)
+    fn.restype = restype
+    fn.argtypes = argtypes
+    return fn
+
+
+_lib_last_error = _sig("gsr_last_error", ctypes.c_char_p, [])
+gsr_version = _sig("gsr_version", ctypes.c_char_p, [])
+_context_create = _sig("gsr_context_create", _i32, [_i32, ctypes.POINTER(_vp)])
+_context_sync = _sig("gsr_context_synchronize", _i32, [_vp])
+_context_destroy = _sig("gsr_context_destroy", None, [_vp])
+_buffer_create = _sig("gsr_buffer_create", _i32, [_vp, _sz, ctypes.POINTER(_vp)])
+_buffer_ptr = _sig("gsr_buffer_device_ptr", _vp, [_vp])
+_buffer_read = _sig("gsr_buffer_read", _i32, [_vp, _vp, _sz])
+_buffer_write = _sig("gsr_buffer_write", _i32, [_vp, _vp, _sz])
+_buffer_destroy = _sig("gsr_buffer_destroy", None, [_vp])
+_scene_create = _sig("gsr_scene_create", _i32, [_vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(_vp)])
+_scene_destroy = _sig("gsr_scene_destroy", None, [_vp])
+_encode_f16 = _sig("gsr_encode_f16", _i32, [_u32, _i32, _vp, _vp, _vp, _vp, _vp, _vp])
+_camera_make = _sig("gsr_camera_make", _i32, [_vp, _u32, _u32, _f32, _f32, ctypes.POINTER(Camera)])
+_camera_from_viewer = _sig("gsr_camera_from_viewer", _i32, [_vp, _vp, _vp, _u32, _u32, ctypes.POINTER(Camera)])
+_renderer_create = _sig("gsr_renderer_create", _i32, [_vp, _vp, ctypes.POINTER(Settings), ctypes.POINTER(_vp)])
+_renderer_destroy = _sig("gsr_renderer_destroy", None, [_vp])
+_renderer_capacity = _sig("gsr_renderer_capacity", _u32, [_vp])
+_renderer_total_bytes = _sig("gsr_renderer_total_bytes", _u64, [_vp])
+_render = _sig("gsr_render", _i32, [_vp, ctypes.POINTER(Camera), _vp, _vp])
+_read_control = _sig("gsr_read_control", _i32, [_vp, ctypes.POINTER(Control)])
+_read_lists = _sig("gsr_read_lists", _i32, [_vp, _vp, _vp, _vp, _vp, _u32])
+_read_splats = _sig("gsr_read_splats", _i32, [_vp, _vp, _vp])
+_read_stage_times = _sig("gsr_read_stage_times", _i32, [_vp, _vp, _u32])
+_sort_pairs = _sig("gsr_sort_pairs", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _vp])
+_exclusive_scan = _sig("gsr_exclusive_scan", _i32, [_vp, _vp, _vp, _u32, _vp, _vp])
+
+# Every symbol include/gsr.h declares (checked by the CPU tests).
+EXPORTED = (
+    "gsr_last_error gsr_version gsr_context_create gsr_context_synchronize gsr_context_destroy "
+    "gsr_buffer_create gsr_buffer_device_ptr gsr_buffer_read gsr_buffer_write gsr_buffer_destroy "
+    "gsr_scene_create gsr_scene_destroy gsr_encode_f16 gsr_camera_make gsr_camera_from_viewer "
+    "gsr_renderer_create gsr_renderer_destroy gsr_renderer_capacity gsr_renderer_total_bytes gsr_render "
+    "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_stage_times gsr_sort_pairs gsr_exclusive_scan"
+).split()
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise GsrError(f"{what} failed (status {status}): {_lib_last_error().decode()}")
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    return None if a is None else a.ctypes.data
+
+
+def _f32c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# --- camera (host) ------------------------------------------------------------------------------
+
+
+def camera_make(view_inria, width: int, height: int, tan_half_fov_x: float, tan_half_fov_y: float) -> Camera:
+    """makeSplatCamera (splat_math.cpp:9-27)."""
+    v = _f32c(view_inria).reshape(16)
+    cam = Camera()
+    _check(_camera_make(_ptr(v), width, height, float(tan_half_fov_x), float(tan_half_fov_y), ctypes.byref(cam)),
+           "gsr_camera_make")
+    return cam
+
+
+def camera_from_viewer(view, proj, model, width: int, height: int) -> Camera:
+    """splatCameraFromViewer (splat_math.cpp:29-42); matrices column-major, flattened m[col*4+row]."""
+    v, p, m = (_f32c(x).reshape(16) for x in (view, proj, model))
+    cam = Camera()
+    _check(_camera_from_viewer(_ptr(v), _ptr(p), _ptr(m), width, height, ctypes.byref(cam)), "gsr_camera_from_viewer")
+    return cam
+
+
+def encode_f16(pos_alpha, scales, rotations, sh, sh_degree: int):
+    """The fp16 scene records (scene_compression.cpp:34-49): geometry (N, 8) u32, colour (N, H) u16."""
+    pa, sc, ro, s = (_f32c(x) for x in (pos_alpha, scales, rotations, sh))
+    n = pa.shape[0]
+    K = (sh_degree + 1) ** 2
+    halves = (K * 3 + 7) // 8 * 8
+    geom = np.zeros((n, 8), np.uint32)
+    color = np.zeros((n, halves), np.uint16)
+    _check(_encode_f16(n, sh_degree, _ptr(pa), _ptr(sc), _ptr(ro), _ptr(s), _ptr(geom), _ptr(color)), "gsr_encode_f16")
+    return geom, color
+
+
+# --- objects ------------------------------------------------------------------------------------
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(_context_create(device, ctypes.byref(h)), "gsr_context_create")
+        self.handle = h
+        self.device = device
+
+    def synchronize(self) -> None:
+        _check(_context_sync(self.handle), "gsr_context_synchronize")
+
+    def close(self) -> None:
+        if self.handle:
+            _context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A device allocation owned by the library (the caller-owned colour target)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        h = _vp()
+        _check(_buffer_create(ctx.handle, nbytes, ctypes.byref(h)), "gsr_buffer_create")
+        self.handle, self.nbytes, self.ctx = h, nbytes, ctx
+
+    @property
+    def ptr(self) -> int:
+        return _buffer_ptr(self.handle)
+
+    def read(self, out: np.ndarray) -> np.ndarray:
+        _check(_buffer_read(self.handle, _ptr(out), out.nbytes), "gsr_buffer_read")
+        return out
+
+    def write(self, src: np.ndarray) -> None:
+        src = np.ascontiguousarray(src)
+        _check(_buffer_write(self.handle, _ptr(src), src.nbytes), "gsr_buffer_write")
+
+    def close(self) -> None:
+        if self.handle:
+            _buffer_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """GpuScene: the scene buffers on the device, f32 SoA or fp16 records."""
+
+    def __init__(self, ctx: Context, pos_alpha=None, scales=None, rotations=None, sh=None, sh_degree: int = 3,
+                 fmt: str = "f32", geometry=None, color=None):
+        desc = SceneDesc()
+        desc.sh_degree = sh_degree
+        desc.device_pointers = 0
+        self._keep = []
+        if fmt == "f32":
+            arrays = [_f32c(x) for x in (pos_alpha, scales, rotations, sh)]
+            n = arrays[0].shape[0]
+            K = (sh_degree + 1) ** 2
+            if arrays[0].shape != (n, 4) or arrays[1].shape != (n, 4) or arrays[2].shape != (n, 4):
+                raise ValueError("pos_alpha, scales, rotations must be (N, 4) float32")
+            if arrays[3].size != n * K * 3:
+                raise ValueError(f"sh must hold N*K*3 = {n * K * 3} floats")
+            desc.format = SCENE_F32_SOA
+            desc.count = n
+            desc.pos_alpha, desc.scales, desc.rotations, desc.sh = (_ptr(a) for a in arrays)
+            self._keep = arrays
+        elif fmt == "f16":
+            if geometry is None:
+                geometry, color = encode_f16(pos_alpha, scales, rotations, sh, sh_degree)
+            g = np.ascontiguousarray(geometry, np.uint32)
+            c = np.ascontiguousarray(color, np.uint16)
+            desc.format = SCENE_F16
+            desc.count = g.shape[0]
+            desc.geometry, desc.color = _ptr(g), _ptr(c)
+            self._keep = [g, c]
+        else:
+            raise ValueError(f"unknown scene format {fmt!r}")
+        h = _vp()
+        _check(_scene_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)), "gsr_scene_create")
+        self._keep = []  # copied to the device
+        self.handle, self.count, self.sh_degree, self.format, self.ctx = h, int(desc.count), sh_degree, fmt, ctx
+
+    def close(self) -> None:
+        if self.handle:
+            _scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Renderer:
+    """SplatTileRenderer for one scene (init + render + the test readbacks)."""
+
+    def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
+                 max_sh_degree: int = 3, profile_stages: bool = False, sort_blocks: int = 0):
+        s = Settings()
+        s.bounds = BOUNDS[bounds]
+        s.budget_mb = budget_mb
+        s.max_sh_degree = max_sh_degree
+        s.format = TARGETS[target]
+        s.profile_stages = 1 if profile_stages else 0
+        s.sort_blocks = sort_blocks
+        h = _vp()
+        _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
+        self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
+        self._image: Optional[DeviceBuffer] = None
+        self._cam: Optional[Camera] = None
+
+    @property
+    def capacity(self) -> int:
+        return int(_renderer_capacity(self.handle))
+
+    @property
+    def total_bytes(self) -> int:
+        return int(_renderer_total_bytes(self.handle))
+
+    def image_bytes(self, cam: Camera) -> int:
+        return int(cam.width) * int(cam.height) * (4 if self.target == "rgba8" else 16)
+
+    def render(self, cam: Camera, out_ptr: Optional[int] = None, stream: Optional[int] = None) -> int:
+        """Record one frame (asynchronous). Returns the device pointer written."""
+        if out_ptr is None:
+            need = self.image_bytes(cam)
+            if self._image is None or self._image.nbytes < need:
+                self._image = DeviceBuffer(self.ctx, need)
+            out_ptr = self._image.ptr
+        _check(_render(self.handle, ctypes.byref(cam), out_ptr, stream), "gsr_render")
+        self._cam = cam
+        return out_ptr
+
+    def read_image(self, cam: Optional[Camera] = None) -> np.ndarray:
+        cam = cam or self._cam
+        h, w = int(cam.height), int(cam.width)
+        if self.target == "rgba8":
+            out = np.empty((h, w, 4), np.uint8)
+        else:
+            out = np.empty((h, w, 4), np.float32)
+        return self._image.read(out)
+
+    def render_to_host(self, cam: Camera) -> np.ndarray:
+        self.render(cam)
+        return self.read_image(cam)
+
+    def read_control(self) -> Control:
+        c = Control()
+        _check(_read_control(self.handle, ctypes.byref(c)), "gsr_read_control")
+        return c
+
+    def read_back(self):
+        """readBack: sorted tile keys, depth keys, values (entries each), ranges (tiles, 2), control."""
+        c = self.read_control()
+        n = int(c.entries)
+        tiles = self._cam.tiles
+        keys = np.empty(n, np.uint32)
+        depth = np.empty(n, np.uint32)
+        vals = np.empty(n, np.uint32)
+        ranges = np.empty((tiles, 2), np.uint32)
+        _check(_read_lists(self.handle, _ptr(keys), _ptr(depth), _ptr(vals), _ptr(ranges), n), "gsr_read_lists")
+        return keys, depth, vals, ranges, c
+
+    def read_back_splats(self):
+        """readBackSplats: (N,) SPLAT_DTYPE records and tile counts, index order."""
+        n = self.scene.count
+        splats = np.empty(n, SPLAT_DTYPE)
+        counts = np.empty(n, np.uint32)
+        _check(_read_splats(self.handle, _ptr(splats), _ptr(counts)), "gsr_read_splats")
+        return splats, counts
+
+    def stage_times(self) -> dict:
+        ms = (ctypes.c_float * len(STAGES))()
+        _check(_read_stage_times(self.handle, ctypes.cast(ms, _vp), len(STAGES)), "gsr_read_stage_times")
+        return {k: float(v) for k, v in zip(STAGES, ms)}
+
+    def close(self) -> None:
+        if self.handle:
+            _renderer_destroy(self.handle)
+            self.handle = None
+        if self._image is not None:
+            self._image.close()
+            self._image = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sort_pairs(ctx: Context, keys: np.ndarray, values: np.ndarray, key_bits: int = 64, count: Optional[int] = None):
+    """GpuRadixSort on host arrays: returns (sorted keys, values) by the low key_bits bits, stable."""
+    keys = np.ascontiguousarray(keys, np.uint64)
+    values = np.ascontiguousarray(values, np.uint32)
+    n = keys.shape[0]
+    kb = DeviceBuffer(ctx, max(8 * n, 8))
+    vb = DeviceBuffer(ctx, max(4 * n, 4))
+    cb = DeviceBuffer(ctx, 4)
+    if n:
+        kb.write(keys)
+        vb.write(values)
+    cnt_ptr = None
+    if count is not None:
+        cb.write(np.array([count], np.uint32))
+        cnt_ptr = cb.ptr
+    _check(_sort_pairs(ctx.handle, kb.ptr, vb.ptr, n, cnt_ptr, key_bits, None), "gsr_sort_pairs")
+    ok, ov = np.empty_like(keys), np.empty_like(values)
+    if n:
+        kb.read(ok)
+        vb.read(ov)
+    return ok, ov
+
+
+def exclusive_scan(ctx: Context, values: np.ndarray):
+    """GpuScan: exclusive prefix sum of u32 values and the total."""
+    v = np.ascontiguousarray(values, np.uint32)
+    n = v.shape[0]
+    ib, ob, tb = DeviceBuffer(ctx, max(4 * n, 4)), DeviceBuffer(ctx, max(4 * n, 4)), DeviceBuffer(ctx, 4)
+    if n:
+        ib.write(v)
+    _check(_exclusive_scan(ctx.handle, ib.ptr, ob.ptr, n, tb.ptr, None), "gsr_exclusive_scan")
+    out = np.empty_like(v)
+    if n:
+        ob.read(out)
+    total = tb.read(np.empty(1, np.uint32))[0]
+    return out, int(total)

@@ -1,0 +1,211 @@
+/* gsr.h: C ABI of the MI355X (gfx950) splat-tiles forward renderer, libgsr.so.
+ *
+ * Drop-in boundary for the reference's `splat-tiles` path. Each entry point names the
+ * reference interface it replaces (paths are in netolcc06/gaussian-splat-renderer):
+ *
+ *   gsr_context_*          VulkanContext for this path: one device, one stream (core/context.h:9-10)
+ *   gsr_scene_*            GpuScene::upload / destroy (render/gpu_scene.h:42-78) for the f32 SoA
+ *                          layout (scene/gaussian_scene.h:1-36) and the fp16 records
+ *                          (scene/scene_compression.h:28-35: G16 geometry + S16 colour)
+ *   gsr_camera_*           makeSplatCamera / splatCameraFromViewer (splat/splat_math.cpp:9-42)
+ *   gsr_renderer_create    SplatTileRenderer::init + onRenderTargetsChanged
+ *                          (render/splat_tile_renderer.h:145-147, splat_tile_renderer.cpp:126-570)
+ *   gsr_render             SplatTileRenderer::render (splat_tile_renderer.cpp:624-833): the whole
+ *                          frame, asynchronous on the given stream, no host synchronisation
+ *   gsr_read_*             readBack / readBackSplats / readBackImage / lastStats
+ *                          (splat_tile_renderer.h:156-196); blocking, for tests and tools
+ *
+ * Conventions: plain pointers and sizes, no C++ or torch types. Every call returns a
+ * gsr_status; on failure gsr_last_error() (per thread) says why. Handles are not thread
+ * safe; one thread records a renderer's frames, as in the reference (application.h:55).
+ * Device pointers are HIP device addresses of the context's device.
+ */
+#ifndef GSR_H
+#define GSR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gsr_status {
+    GSR_OK = 0,
+    GSR_ERROR_INVALID_ARGUMENT = 1, /* std::runtime_error on bad settings/formats (splat_tile_renderer.cpp:139-163) */
+    GSR_ERROR_HIP = 2,              /* a HIP runtime call failed (VK_CHECK, vk_common.h:28-35) */
+    GSR_ERROR_OUT_OF_MEMORY = 3,    /* device allocation failed; nothing is left allocated (gpu_scene.cpp:108-114) */
+    GSR_ERROR_UNSUPPORTED = 4       /* a mode this build does not provide */
+} gsr_status;
+
+typedef struct gsr_context gsr_context;
+typedef struct gsr_scene gsr_scene;
+typedef struct gsr_renderer gsr_renderer;
+typedef struct gsr_buffer gsr_buffer;
+
+/* Scene formats (scene_compression.h:6-35). */
+typedef enum gsr_scene_format {
+    GSR_SCENE_F32_SOA = 0, /* posAlpha f32x4, scales f32x4 (sigma, 0), rotations f32x4 (w,x,y,z), sh f32[N*K*3] */
+    GSR_SCENE_F16 = 1      /* geometry 8 words (pos f32 x3, h(alpha,sx), h(sy,sz), h(qw,qx), h(qy,qz), 0) +
+                              colour K*3 fp16 padded to a multiple of 8 halves; q renormalised on decode */
+} gsr_scene_format;
+
+/* TileBounds (splat/tile_coverage.h:21). */
+typedef enum gsr_bounds { GSR_BOUNDS_RECT = 0, GSR_BOUNDS_TIGHT = 1, GSR_BOUNDS_CUTOFF_ONLY = 2 } gsr_bounds;
+
+/* SplatTargetFormat for this path: RGBA8 (the product) or RGBA32F (float image, tests). */
+typedef enum gsr_target_format { GSR_TARGET_RGBA8 = 0, GSR_TARGET_RGBA32F = 1 } gsr_target_format;
+
+/* CullReason (splat/splat_math.h:110). */
+typedef enum gsr_cull_reason {
+    GSR_CULL_VISIBLE = 0,
+    GSR_CULL_NEAR_PLANE = 1,
+    GSR_CULL_SINGULAR_COV = 2,
+    GSR_CULL_OFF_SCREEN = 3,
+    GSR_CULL_LOW_OPACITY = 4
+} gsr_cull_reason;
+
+/* SplatCamera (splat/splat_math.h:43-50). view_inria is column-major (m[col*4+row]). */
+typedef struct gsr_camera {
+    float view_inria[16];
+    float position[3];
+    uint32_t width, height;
+    float tan_half_fov_x, tan_half_fov_y;
+    float focal_x, focal_y;
+    uint32_t tiles_x, tiles_y;
+} gsr_camera;
+
+/* ProjectedSplat / GLSL Splat, 48 bytes (splat/splat_math.h:115-123). */
+typedef struct gsr_projected_splat {
+    float center[2];
+    float depth;
+    uint32_t cull_reason;
+    float conic_opacity[4];
+    float color[3];
+    float radius;
+} gsr_projected_splat;
+
+/* TileControl, 80 bytes (render/splat_tile_renderer.h:116-128). */
+typedef struct gsr_control {
+    uint32_t gaussian_count;   /* N */
+    uint32_t visible;          /* splats with a non-zero tile count */
+    uint32_t duplicates;       /* M, unclamped */
+    uint32_t entries;          /* min(M, capacity) */
+    uint32_t cull_counts[8];   /* per gsr_cull_reason, [0] = visible */
+    uint32_t overflow;         /* M - entries */
+    uint32_t ranges_groups[3]; /* grid of the tile-ranges pass (informational) */
+    uint32_t coverage_mismatch;/* splats whose written tiles != their count (always 0) */
+    uint32_t pad[3];
+} gsr_control;
+
+/* Scene description. Host pointers are copied at creation; with device_pointers != 0 the
+ * arrays are bound in place (zero copy) and must outlive the scene (gpu_scene.h:4-6). */
+typedef struct gsr_scene_desc {
+    uint32_t count;
+    int32_t sh_degree; /* 0..3; K = (sh_degree + 1)^2 coefficients per channel */
+    gsr_scene_format format;
+    int32_t device_pointers;
+    const float* pos_alpha;   /* F32_SOA */
+    const float* scales;      /* F32_SOA */
+    const float* rotations;   /* F32_SOA */
+    const float* sh;          /* F32_SOA */
+    const uint32_t* geometry; /* F16: 8 words per gaussian */
+    const uint16_t* color;    /* F16: round_up(K*3, 8) halves per gaussian */
+} gsr_scene_desc;
+
+/* SplatSettings + TileSettings for this path (splat_tile_renderer.h:73-114). */
+typedef struct gsr_settings {
+    gsr_bounds bounds;           /* default TIGHT (cli.h:44) */
+    uint32_t budget_mb;          /* list memory, 24.5 B/entry (splat_tile_renderer.cpp:117-124); 0 = 256 */
+    int32_t max_sh_degree;       /* evaluated bands = min(scene degree, this) */
+    gsr_target_format format;    /* output image format */
+    int32_t profile_stages;      /* 1: hipEvent pairs around every stage (gsr_read_stage_times) */
+    uint32_t sort_blocks;        /* 0 = default; radix-sort grid (blocks), tuning knob */
+    uint32_t reserved[6];
+} gsr_settings;
+
+/* Stage timings of the last profiled frame, ms (the GpuProfiler scopes, profiler.h:1-87). */
+typedef enum gsr_stage {
+    GSR_STAGE_PREPROCESS = 0, /* preprocess + cull + tile counts + block sums */
+    GSR_STAGE_SCAN = 1,       /* block-sum scan + control (tile_setup) */
+    GSR_STAGE_DUPLICATE = 2,  /* in-block scan + duplication */
+    GSR_STAGE_SORT = 3,       /* all radix passes */
+    GSR_STAGE_RANGES = 4,     /* tile ranges */
+    GSR_STAGE_RENDER = 5,     /* tile render incl. RGBA8 resolve */
+    GSR_STAGE_TOTAL = 6,
+    GSR_STAGE_COUNT = 7
+} gsr_stage;
+
+const char* gsr_last_error(void);
+const char* gsr_version(void);
+
+/* Context: device ordinal + the stream used when gsr_render gets a null stream. */
+gsr_status gsr_context_create(int32_t device, gsr_context** out);
+gsr_status gsr_context_synchronize(gsr_context* ctx);
+void gsr_context_destroy(gsr_context* ctx);
+
+/* Plain device buffers for callers without their own allocator (the caller-owned colour
+ * target of FrameInput, renderer.h:32-35). */
+gsr_status gsr_buffer_create(gsr_context* ctx, size_t bytes, gsr_buffer** out);
+void* gsr_buffer_device_ptr(gsr_buffer* buf);
+gsr_status gsr_buffer_read(gsr_buffer* buf, void* host_dst, size_t bytes);
+gsr_status gsr_buffer_write(gsr_buffer* buf, const void* host_src, size_t bytes);
+void gsr_buffer_destroy(gsr_buffer* buf);
+
+/* GpuScene::upload (gpu_scene.h:42-78) / destroy. */
+gsr_status gsr_scene_create(gsr_context* ctx, const gsr_scene_desc* desc, gsr_scene** out);
+void gsr_scene_destroy(gsr_scene* scene);
+
+/* The fp16 encoder of scene_compression.cpp:34-49 (geometry) and the S16 colour record:
+ * fp32 -> fp16 round to nearest even (half.h:32-64). Host only. */
+gsr_status gsr_encode_f16(uint32_t count, int32_t sh_degree, const float* pos_alpha, const float* scales,
+                          const float* rotations, const float* sh, uint32_t* geometry_out, uint16_t* color_out);
+
+/* makeSplatCamera (splat_math.cpp:9-27) and splatCameraFromViewer (splat_math.cpp:29-42).
+ * Host only; matrices column-major. */
+gsr_status gsr_camera_make(const float view_inria[16], uint32_t width, uint32_t height, float tan_half_fov_x,
+                           float tan_half_fov_y, gsr_camera* out);
+gsr_status gsr_camera_from_viewer(const float view[16], const float proj[16], const float model[16], uint32_t width,
+                                  uint32_t height, gsr_camera* out);
+
+/* SplatTileRenderer::init (+ the per-size buffers of onRenderTargetsChanged, created on
+ * first use of a size). The renderer binds the scene; the scene must outlive it. */
+gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const gsr_settings* settings,
+                               gsr_renderer** out);
+void gsr_renderer_destroy(gsr_renderer* r);
+uint32_t gsr_renderer_capacity(const gsr_renderer* r);     /* list capacity, entries */
+uint64_t gsr_renderer_total_bytes(const gsr_renderer* r);  /* device memory owned by the renderer */
+
+/* SplatTileRenderer::render: the whole frame on `stream` (null = the context's stream),
+ * no host synchronisation. out_image: device pointer, width*height pixels of
+ * settings.format (RGBA8: 4 bytes; RGBA32F: 16 bytes), row 0 = top, tightly packed. Every
+ * pixel is written: C + T * background (black), alpha = 1 - T. */
+gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* out_image, void* stream);
+
+/* Test/debug readbacks of the last rendered frame (blocking; the GPU is synchronised).
+ * read_control: the 80-byte control block. read_lists: the sorted list (control.entries
+ * entries: tile id, depth bits, gaussian index) and [start, end) per tile (2 per tile);
+ * any pointer may be null. read_splats: N projected splats + tile counts (index order). */
+gsr_status gsr_read_control(gsr_renderer* r, gsr_control* out);
+gsr_status gsr_read_lists(gsr_renderer* r, uint32_t* tile_keys, uint32_t* depth_keys, uint32_t* values,
+                          uint32_t* ranges, uint32_t max_entries);
+gsr_status gsr_read_splats(gsr_renderer* r, gsr_projected_splat* splats, uint32_t* tile_counts);
+gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n);
+
+/* GpuRadixSort on its own (gpu_radix_sort.h:38-122, test_gpu_sort.cpp): stable LSD sort of
+ * (key, value) pairs by the low `key_bits` bits of 64-bit keys, in place, on device
+ * pointers. `count_device` (may be null) holds the key count on the device, clamped to
+ * max_count; null = max_count keys. Asynchronous on `stream`. */
+gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, uint32_t max_count,
+                          const uint32_t* count_device, uint32_t key_bits, void* stream);
+
+/* GpuScan on its own (gpu_scan.h:25-76): exclusive prefix sum of u32 values, the total in
+ * *total_device. Asynchronous on `stream`. */
+gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* out, uint32_t count,
+                              uint32_t* total_device, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSR_H */

@@ -1,0 +1,280 @@
+"""ctypes binding of the CPU oracle (liboracle.so). TEST INFRASTRUCTURE ONLY: imported by
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker or as the
+timed CPU baseline; never by the product.
+
+The oracle is a C++ restatement of the reference's CPU forward pass (src/splat/*); see
+gsr_oracle.hpp for the file-by-file mapping.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+if not os.path.exists(LIB_PATH):
+    build()
+_lib = ctypes.CDLL(LIB_PATH)
+
+_vp, _u32, _i32, _f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_float
+
+SPLAT_DTYPE = np.dtype([("center", np.float32, 2), ("depth", np.float32), ("cull_reason", np.uint32),
+                        ("conic_opacity", np.float32, 4), ("color", np.float32, 3), ("radius", np.float32)])
+RECT, TIGHT, CUTOFF = 0, 1, 2
+
+
+class RenderOptions(ctypes.Structure):
+    _fields_ = [("maxShDegree", _i32), ("earlyTermination", _i32), ("bound", _u32), ("bounds", _u32),
+                ("threads", _i32), ("background", _f32 * 3)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("gaussians", _u32), ("visible", _u32), ("culledNear", _u32), ("culledSingular", _u32),
+                ("culledOffScreen", _u32), ("threads", _u32), ("tileInstances", ctypes.c_uint64),
+                ("evaluatedFragments", ctypes.c_uint64), ("blendedFragments", ctypes.c_uint64),
+                ("terminatedPixels", ctypes.c_uint64), ("preprocessMs", ctypes.c_double),
+                ("sortMs", ctypes.c_double), ("binMs", ctypes.c_double), ("rasterMs", ctypes.c_double),
+                ("totalMs", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _sig(name, restype, argtypes):
+    f = getattr(_lib, name)
+    f.restype, f.argtypes = restype, argtypes
+    return f
+
+
+_camera_from_viewer = _sig("oracle_camera_from_viewer", None, [_vp] * 3 + [_u32, _u32, _vp, _vp, _vp])
+_make_camera = _sig("oracle_make_camera", None, [_vp, _u32, _u32, _f32, _f32, _vp, _vp, _vp])
+det_logf = _sig("oracle_det_logf", _f32, [_f32])
+float_to_half = _sig("oracle_float_to_half", ctypes.c_uint16, [_f32])
+half_to_float = _sig("oracle_half_to_float", _f32, [ctypes.c_uint16])
+_splat_math = _sig("oracle_splat_math", None, [_vp, _vp, _vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp, _vp, _vp, _vp,
+                                                _vp])
+_splat_math_f64 = _sig("oracle_splat_math_f64", None, [_vp, _vp, _vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp, _vp,
+                                                        _vp, _vp, _vp])
+_tile_rect = _sig("oracle_tile_rect", None, [_f32, _f32, _f32, _u32, _u32, _vp])
+_eval_sh = _sig("oracle_eval_sh", None, [_i32, _vp, _vp, _vp])
+_alpha_at = _sig("oracle_splat_alpha_at", _f32, [_vp, _f32, _f32])
+_preprocess = _sig("oracle_preprocess", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32, _f32, _i32,
+                                               _u32, _vp, _vp, _i32])
+_preprocess_f64 = _sig("oracle_preprocess_f64", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32,
+                                                       _f32, _i32, _vp, _i32])
+_coverage_tiles = _sig("oracle_coverage_tiles", _u32, [_vp, _u32, _i32, _u32, _u32, _i32, _i32, _vp, _u32, _vp])
+_tile_lists = _sig("oracle_tile_lists", _u32, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp,
+                                               _vp])
+_radix_sort_pairs = _sig("oracle_radix_sort_pairs", _u32, [_vp, _vp, _u32])
+_encode_f16 = _sig("oracle_encode_f16", None, [_u32, _vp, _vp, _vp, _vp, _u32, _vp, _vp])
+_decode_f16 = _sig("oracle_decode_f16", None, [_u32, _vp, _vp, _u32, _vp, _vp, _vp, _vp])
+_render = _sig("oracle_render", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32, _f32, _vp, _vp,
+                                       _vp, _vp, _vp, ctypes.c_uint64])
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32c(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+class OracleCamera:
+    """The oracle's SplatCamera: viewInria (column-major) + size + tan(fov/2)."""
+
+    def __init__(self, view_inria, width, height, tan_x, tan_y):
+        self.view = _f32c(view_inria).reshape(16).copy()
+        self.width, self.height = int(width), int(height)
+        self.tan_x, self.tan_y = float(np.float32(tan_x)), float(np.float32(tan_y))
+        pos = np.zeros(3, np.float32)
+        focal = np.zeros(2, np.float32)
+        tiles = np.zeros(2, np.uint32)
+        _make_camera(_p(self.view), self.width, self.height, self.tan_x, self.tan_y, _p(pos), _p(focal), _p(tiles))
+        self.position, self.focal, self.tiles_x, self.tiles_y = pos, focal, int(tiles[0]), int(tiles[1])
+
+    @classmethod
+    def from_viewer(cls, view, proj, model, width, height):
+        vi = np.zeros(16, np.float32)
+        tx, ty = ctypes.c_float(), ctypes.c_float()
+        v, p, m = (_f32c(x).reshape(16) for x in (view, proj, model))
+        _camera_from_viewer(_p(v), _p(p), _p(m), width, height, _p(vi), ctypes.byref(tx), ctypes.byref(ty))
+        return cls(vi, width, height, tx.value, ty.value)
+
+    @classmethod
+    def from_gsr(cls, cam):
+        return cls(np.array(cam.view_inria[:], np.float32), cam.width, cam.height, cam.tan_half_fov_x,
+                   cam.tan_half_fov_y)
+
+    def args(self):
+        return (_p(self.view), self.width, self.height, self.tan_x, self.tan_y)
+
+
+def axis_camera(w, h, f):
+    """A camera at the origin looking down +z (INRIA convention), focal f (test_splat_math.cpp:52-55)."""
+    return OracleCamera(np.eye(4, dtype=np.float32), w, h, np.float32(w) / (np.float32(2.0) * np.float32(f)),
+                        np.float32(h) / (np.float32(2.0) * np.float32(f)))
+
+
+def splat_math(t, sigma, q, cam: OracleCamera, f64: bool = False):
+    """cov3D (3x3 [col][row]), cov2D (a, b, c), conic, radius, invertible, centre."""
+    dt = np.float64 if f64 else np.float32
+    t, sigma, q = (np.ascontiguousarray(x, dt) for x in (t, sigma, q))
+    cov3 = np.zeros(9, dt)
+    cov2 = np.zeros(3, dt)
+    conic = np.zeros(3, dt)
+    radius = (ctypes.c_double if f64 else ctypes.c_float)()
+    inv = ctypes.c_int32()
+    center = np.zeros(2, dt)
+    fn = _splat_math_f64 if f64 else _splat_math
+    fn(_p(t), _p(sigma), _p(q), *cam.args(), _p(cov3), _p(cov2), _p(conic), ctypes.byref(radius), ctypes.byref(inv),
+       _p(center))
+    return dict(cov3d=cov3.reshape(3, 3), cov2d=cov2, conic=conic, radius=radius.value, invertible=bool(inv.value),
+                center=center)
+
+
+def tile_rect(cx, cy, radius, tiles_x, tiles_y):
+    out = np.zeros(4, np.int32)
+    _tile_rect(cx, cy, radius, tiles_x, tiles_y, _p(out))
+    return tuple(int(v) for v in out)  # minX, minY, maxX, maxY
+
+
+def eval_sh(degree, sh, direction):
+    sh = _f32c(sh).reshape(-1)
+    if sh.size < 48:
+        sh = np.concatenate([sh, np.zeros(48 - sh.size, np.float32)])
+    d = _f32c(direction)
+    out = np.zeros(3, np.float32)
+    _eval_sh(degree, _p(sh), _p(d), _p(out))
+    return out
+
+
+def splat_alpha_at(splat, px, py):
+    s = np.ascontiguousarray(np.asarray(splat, SPLAT_DTYPE).reshape(1))
+    return float(_alpha_at(_p(s), px, py))
+
+
+def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGHT, threads=0):
+    """preprocessGaussian over the scene: (N,) splats and the per-gaussian tile counts."""
+    n = scene.count
+    out = np.zeros(n, SPLAT_DTYPE)
+    counts = np.zeros(n, np.uint32)
+    deg = scene.sh_degree if degree is None else min(degree, scene.sh_degree)
+    _preprocess(n, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), scene.K, deg, *cam.args(),
+                1 if gpu_cull else 0, bounds, _p(out), _p(counts), threads)
+    return out, counts
+
+
+def preprocess_f64(scene, cam: OracleCamera, degree=None, gpu_cull=True, threads=0):
+    """FP64 twin: (N, 12) doubles: cx, cy, depth, cull, conic a b c, opacity, r g b, radius."""
+    n = scene.count
+    out = np.zeros((n, 12), np.float64)
+    deg = scene.sh_degree if degree is None else min(degree, scene.sh_degree)
+    _preprocess_f64(n, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), scene.K, deg,
+                    *cam.args(), 1 if gpu_cull else 0, _p(out), threads)
+    return out
+
+
+def coverage_tiles(splat, bounds, tiles_x, tiles_y, S=16, grid=None):
+    """The bin tiles a splat is binned into, its count and whether it is borderline."""
+    s = np.ascontiguousarray(np.asarray(splat, SPLAT_DTYPE).reshape(1))
+    gx, gy = grid if grid is not None else (tiles_x, tiles_y)
+    cap = 1 << 16
+    buf = np.zeros(cap, np.uint32)
+    border = ctypes.c_int32()
+    cnt = _coverage_tiles(_p(s), bounds, S, tiles_x, tiles_y, gx, gy, _p(buf), cap, ctypes.byref(border))
+    return buf[: min(cnt, cap)].copy(), int(cnt), bool(border.value)
+
+
+def tile_lists(splats, counts, tiles_x, tiles_y, bounds=TIGHT, capacity=None, mode=0):
+    """The GPU path's sorted list: tile keys, depth keys, values, ranges (tiles, 2), duplicates, mismatches."""
+    splats = np.ascontiguousarray(splats, SPLAT_DTYPE)
+    counts = np.ascontiguousarray(counts, np.uint32)
+    n = splats.shape[0]
+    dup = ctypes.c_uint64()
+    mism = ctypes.c_uint32()
+    total = int(counts.astype(np.uint64).sum())
+    cap = total if capacity is None else min(int(capacity), total)
+    hi = np.zeros(max(cap, 1), np.uint32)
+    lo = np.zeros(max(cap, 1), np.uint32)
+    vals = np.zeros(max(cap, 1), np.uint32)
+    ranges = np.zeros((tiles_x * tiles_y, 2), np.uint32)
+    cap_arg = cap if capacity is not None else 0xFFFFFFFF
+    e = _tile_lists(_p(splats), _p(counts), n, tiles_x, tiles_y, bounds, cap_arg, mode, _p(hi), _p(lo), _p(vals),
+                    _p(ranges), ctypes.byref(dup), ctypes.byref(mism))
+    return hi[:e], lo[:e], vals[:e], ranges, int(dup.value), int(mism.value)
+
+
+def radix_sort_pairs(keys, values):
+    k = np.ascontiguousarray(keys, np.uint32).copy()
+    v = np.ascontiguousarray(values, np.uint32).copy()
+    passes = _radix_sort_pairs(_p(k), _p(v), k.shape[0])
+    return k, v, int(passes)
+
+
+def encode_f16(scene):
+    n, K = scene.count, scene.K
+    halves = (K * 3 + 7) // 8 * 8
+    geom = np.zeros((n, 8), np.uint32)
+    color = np.zeros((n, halves), np.uint16)
+    _encode_f16(n, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), K, _p(geom), _p(color))
+    return geom, color
+
+
+def decode_f16(geom, color, sh_degree):
+    """decodeGaussian, f16 records: pos_alpha, scales, rotations, sh as float32 arrays."""
+    geom = np.ascontiguousarray(geom, np.uint32)
+    color = np.ascontiguousarray(color, np.uint16)
+    n = geom.shape[0]
+    K = (sh_degree + 1) ** 2
+    pa, sc, ro = (np.zeros((n, 4), np.float32) for _ in range(3))
+    sh = np.zeros((n, K * 3), np.float32)
+    _decode_f16(n, _p(geom), _p(color), K, _p(pa), _p(sc), _p(ro), _p(sh))
+    return pa, sc, ro, sh
+
+
+def render(scene, cam: OracleCamera, max_sh_degree=3, early_termination=True, bound=0, bounds=RECT, threads=0,
+           background=(0.0, 0.0, 0.0), want_lists=False):
+    """renderReference: (H, W, 3) float32 image (unclamped) and ReferenceStats (+ per-tile lists)."""
+    opt = RenderOptions(max_sh_degree, 1 if early_termination else 0, bound, bounds, threads,
+                        (ctypes.c_float * 3)(*background))
+    rgb = np.zeros((cam.height, cam.width, 3), np.float32)
+    st = Stats()
+    tiles = cam.tiles_x * cam.tiles_y
+    start = np.zeros(tiles + 1, np.uint32) if want_lists else None
+    lists = None
+    if want_lists:
+        # tile instances are not known in advance: one pass to get them.
+        _render(scene.count, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), scene.K,
+                scene.sh_degree, *cam.args(), ctypes.byref(opt), _p(rgb), ctypes.byref(st), _p(start), None, 0)
+        lists = np.zeros(max(int(st.tileInstances), 1), np.uint32)
+    _render(scene.count, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), scene.K,
+            scene.sh_degree, *cam.args(), ctypes.byref(opt), _p(rgb), ctypes.byref(st), _p(start), _p(lists),
+            0 if lists is None else lists.size)
+    if want_lists:
+        return rgb, st.as_dict(), start, lists[: int(st.tileInstances)]
+    return rgb, st.as_dict()
+
+
+def quantize8(rgb):
+    """writePngRgb (image_io.cpp:51-57): clamp to [0, 1], NaN -> 0, lround(v * 255)."""
+    v = np.nan_to_num(np.clip(np.asarray(rgb, np.float32), 0.0, 1.0), nan=0.0).astype(np.float32)
+    x = (v * np.float32(255.0)).astype(np.float32)  # the float product, then round half away from zero
+    return np.floor(x.astype(np.float64) + 0.5).astype(np.uint8)
+
+
+def psnr(a, b):
+    """imgdiff (tools/imgdiff.cpp:108-127): clamp both to [0, 1], PSNR with peak 1."""
+    a = np.clip(np.asarray(a, np.float64), 0.0, 1.0)
+    b = np.clip(np.asarray(b, np.float64), 0.0, 1.0)
+    mse = float(np.mean((a - b) ** 2))
+    return float("inf") if mse == 0 else 10.0 * np.log10(1.0 / mse)

@@ -1,0 +1,279 @@
+"""GPU parity of the splat-tiles path (through the C ABI) against the CPU oracle.
+
+Mirrors the reference's GPU suites (tests/test_tile_raster.cpp, test_tight_bounds.cpp,
+test_gpu_sort.cpp):
+  - per-gaussian splats and tile counts: bit-exact (FP32, same operation order);
+  - the sorted (tile, depth, gaussian) list and per-tile ranges: bit-exact, incl. overflow;
+  - control block counts (N, V, M, entries, overflow, cull counts, coverage mismatch = 0);
+  - image: float PSNR > 60 dB (test_tile_raster.cpp:498) and RGBA8 within 1 level per
+    channel of the quantised oracle image (>= 58.7 dB);
+  - the stand-alone radix sort (stable, 64-bit keys) and scan on adversarial sizes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _camera(gsr, view, W, H, fov=60.0):
+    from gsr import camera
+
+    return gsr.camera_from_viewer(view, camera.perspective(fov, W / H), camera.IDENTITY, W, H)
+
+
+def _check_splats(gsr_splats, gsr_counts, o_splats, o_counts):
+    """Bit-exact comparison of the 48-byte splats (culled ones: reason + depth only)."""
+    assert np.array_equal(gsr_splats["cull_reason"], o_splats["cull_reason"]), "cull reasons differ"
+    assert np.array_equal(gsr_splats["depth"].view(np.uint32), o_splats["depth"].view(np.uint32)), "depths differ"
+    vis = o_splats["cull_reason"] == 0
+    for f in ("center", "conic_opacity", "color", "radius"):
+        a = gsr_splats[f][vis].view(np.uint32)
+        b = o_splats[f][vis].view(np.uint32)
+        bad = np.nonzero((a != b).reshape(a.shape[0], -1).any(axis=1))[0]
+        assert bad.size == 0, f"{f}: {bad.size} visible splats differ, first {np.nonzero(vis)[0][bad[:5]]}"
+    assert np.array_equal(gsr_counts, o_counts), f"tile counts differ at {np.nonzero(gsr_counts != o_counts)[0][:10]}"
+
+
+def _check_lists(r, o_splats, o_counts, tiles_x, tiles_y, bounds, capacity=None):
+    import oracle
+
+    keys, depth, vals, ranges, ctl = r.read_back()
+    hi, lo, ov, orng, dup, mism = oracle.tile_lists(o_splats, o_counts, tiles_x, tiles_y, bounds=bounds,
+                                                    capacity=capacity)
+    assert ctl.duplicates == dup, (ctl.duplicates, dup)
+    assert ctl.entries == len(hi), (ctl.entries, len(hi))
+    assert ctl.overflow == dup - len(hi)
+    assert ctl.coverage_mismatch == 0 and mism == 0
+    assert np.array_equal(keys, hi), f"tile keys differ at {np.nonzero(keys != hi)[0][:10]}"
+    assert np.array_equal(depth, lo), "depth keys differ"
+    assert np.array_equal(vals, ov), f"values differ at {np.nonzero(vals != ov)[0][:10]}"
+    assert np.array_equal(ranges, orng), "ranges differ"
+    return ctl
+
+
+def _render_case(gsr_ctx, sc, view, W, H, bounds, fmt="f32", target="rgba32f", budget_mb=256, fov=60.0):
+    import gsr
+    import oracle
+
+    cam = _camera(gsr, view, W, H, fov)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
+    r = gsr.Renderer(gsr_ctx, scene, bounds=bounds, target=target, budget_mb=budget_mb)
+    img = r.render_to_host(cam)
+    if fmt == "f16":  # the oracle renders exactly what the GPU decodes (gsplat_ref.cpp:182-190)
+        from gsr.scenes import SyntheticScene
+
+        g, c = oracle.encode_f16(sc)
+        sc = SyntheticScene(*oracle.decode_f16(g, c, sc.sh_degree), sc.sh_degree, sc.seed)
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    ob = {"rect": oracle.RECT, "tight": oracle.TIGHT, "cutoff-only": oracle.CUTOFF}[bounds]
+    o_splats, o_counts = oracle.preprocess(sc, ocam, bounds=ob)
+    return r, scene, cam, img, sc, ocam, ob, o_splats, o_counts
+
+
+@pytest.mark.parametrize("bounds", ["rect", "tight"])
+def test_100k_720p_splats_lists_image(gsr_ctx, bounds):
+    """The small workload: 100k gaussians, N(0, 2^2), camera at radius 8, 1280x720."""
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case 100k-720p bounds={bounds}")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    r, scene, cam, img, sc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, camera.orbit_view(8.0, 0.0, 0.0), 1280,
+                                                              720, bounds)
+    gs, gc = r.read_back_splats()
+    _check_splats(gs, gc, os_, oc)
+    ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+    assert ctl.gaussian_count == 100_000
+    cull = np.bincount(os_["cull_reason"], minlength=5)
+    assert list(ctl.cull_counts[:5]) == list(cull[:5])
+    assert ctl.visible == int((oc != 0).sum())
+    ref, st = oracle.render(sc, ocam, bounds=ob)
+    p = oracle.psnr(img[..., :3], ref)
+    log(f"  entries {ctl.entries} (oracle tile instances {st['tileInstances']}), PSNR {p:.2f} dB")
+    assert p > 60.0
+    r.close()
+    scene.close()
+
+
+def test_rgba8_within_one_level(gsr_ctx):
+    import oracle
+    from gsr import camera, scenes
+
+    log("case rgba8 100k-720p tight")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    r, scene, cam, img, sc, ocam, ob, *_ = _render_case(gsr_ctx, sc, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720,
+                                                        "tight", target="rgba8")
+    ref, _ = oracle.render(sc, ocam, bounds=ob)
+    q = oracle.quantize8(ref)
+    diff = np.abs(img[..., :3].astype(np.int32) - q.astype(np.int32))
+    p = oracle.psnr(img[..., :3].astype(np.float64) / 255.0, ref)
+    log(f"  max diff {diff.max()} levels, {int((diff > 0).sum())} channel values off by one, PSNR {p:.2f} dB")
+    assert diff.max() <= 1
+    assert p >= 58.7
+    r.close()
+    scene.close()
+
+
+def test_f16_scene_bit_exact(gsr_ctx):
+    """The fp16 records: the GPU decode equals decodeGaussian, so splats and lists stay bit-exact."""
+    import oracle
+    from gsr import camera, scenes
+
+    log("case f16 100k-720p tight")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    r, scene, cam, img, sc16, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, camera.orbit_view(8.0, 0.0, 30.0),
+                                                                1280, 720, "tight", fmt="f16")
+    gs, gc = r.read_back_splats()
+    _check_splats(gs, gc, os_, oc)
+    _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+    ref, _ = oracle.render(sc16, ocam, bounds=ob)
+    assert oracle.psnr(img[..., :3], ref) > 60.0
+    r.close()
+    scene.close()
+
+
+def test_reference_test_scene_ties_and_overflow(gsr_ctx):
+    """test_tile_raster.cpp's renderer scene (250k, 640x360, eye (0.3, 0.5, 6)): exact depth
+    ties inside tiles, near/behind/off-screen/transparent gaussians, and a 1 MB budget."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log("case test-scene 250k 640x360")
+    sc = scenes.make_test_scene(250_000, seed=99)
+    view = camera.look_at([0.3, 0.5, 6.0], [0.0, 0.0, 0.0])
+    for bounds in ("rect", "tight"):
+        r, scene, cam, img, sc2, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 640, 360, bounds)
+        gs, gc = r.read_back_splats()
+        _check_splats(gs, gc, os_, oc)
+        keys, depth, vals, ranges, ctl = r.read_back()
+        ties = int(((keys[1:] == keys[:-1]) & (depth[1:] == depth[:-1])).sum())
+        log(f"  {bounds}: entries {ctl.entries}, adjacent equal (tile, depth): {ties}")
+        assert ties > 0
+        _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+        ref, st = oracle.render(sc2, ocam, bounds=ob)
+        assert st["terminatedPixels"] > 0
+        assert oracle.psnr(img[..., :3], ref) > 60.0
+        r.close()
+        scene.close()
+    # Overflow: the entries past the capacity are dropped in gaussian-index order.
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", budget_mb=1)
+    cam = _camera(gsr, view, 640, 360)
+    r.render(cam)
+    assert r.capacity == 40960
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    os_, oc = oracle.preprocess(sc, ocam, bounds=oracle.TIGHT)
+    ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, oracle.TIGHT, capacity=r.capacity)
+    log(f"  1 MB budget: duplicates {ctl.duplicates}, entries {ctl.entries}, overflow {ctl.overflow}")
+    assert ctl.overflow > 0 and ctl.entries == r.capacity
+    r.close()
+    scene.close()
+
+
+def test_1_2m_1080p_orbit_frame(gsr_ctx):
+    """The headline workload at full size: 1.2M gaussians, 1920x1080, orbit frame 40 (48 deg)."""
+    import oracle
+    from gsr import camera, scenes
+
+    log("case 1.2M-1080p orbit frame 40, tight")
+    sc = scenes.make_scene(1_200_000, seed=0, mean_sigma=3.0)
+    view = camera.orbit_view(10.0, 15.0, 360.0 * 40 / 300)
+    r, scene, cam, img, sc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 1920, 1080, "tight")
+    gs, gc = r.read_back_splats()
+    _check_splats(gs, gc, os_, oc)
+    ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+    ref, st = oracle.render(sc, ocam, bounds=ob)
+    p = oracle.psnr(img[..., :3], ref)
+    log(f"  V {ctl.visible}, M {ctl.duplicates}, PSNR {p:.2f} dB")
+    assert p > 60.0
+    r.close()
+    scene.close()
+
+
+def test_deterministic_and_empty(gsr_ctx):
+    import gsr
+    from gsr import camera, scenes
+
+    log("case determinism + empty scene")
+    sc = scenes.make_scene(50_000, seed=5, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8")
+    cam = _camera(gsr, camera.orbit_view(8.0, 10.0, 20.0), 640, 360)
+    a = r.render_to_host(cam).copy()
+    la = r.read_back()
+    b = r.render_to_host(cam).copy()
+    lb = r.read_back()
+    assert np.array_equal(a, b)
+    for x, y in zip(la[:4], lb[:4]):
+        assert np.array_equal(x, y)
+    r.close()
+    scene.close()
+    empty = gsr.Scene(gsr_ctx, np.zeros((0, 4)), np.zeros((0, 4)), np.zeros((0, 4)), np.zeros((0, 48)), 3)
+    r = gsr.Renderer(gsr_ctx, empty, bounds="tight", target="rgba8")
+    img = r.render_to_host(cam)
+    assert np.all(img == 0)
+    ctl = r.read_control()
+    assert ctl.duplicates == 0 and ctl.entries == 0 and ctl.gaussian_count == 0
+    r.close()
+    empty.close()
+
+
+SORT_SIZES = [0, 1, 31, 32, 33, 255, 256, 257, 2047, 2048, 2049, 4095, 4097, 65537, 1_000_003]
+
+
+@pytest.mark.parametrize("n", SORT_SIZES)
+def test_sort_pairs_stable(gsr_ctx, n):
+    """test_gpu_sort.cpp:198-258: sizes around block boundaries, adversarial key patterns."""
+    import gsr
+
+    rng = np.random.default_rng(1234 + n)
+    patterns = {
+        "random44": rng.integers(0, 1 << 44, n, dtype=np.uint64),
+        "all-equal": np.full(n, 0xDEADBEEF, np.uint64),
+        "sorted": np.arange(n, dtype=np.uint64),
+        "reversed": np.arange(n, dtype=np.uint64)[::-1].copy(),
+        "4-values": rng.integers(0, 4, n, dtype=np.uint64) << np.uint64(20),
+        "top-byte": rng.integers(0, 256, n, dtype=np.uint64) << np.uint64(36),
+    }
+    for name, keys in patterns.items():
+        if n > 100_000 and name not in ("random44", "4-values"):
+            continue
+        vals = np.arange(n, dtype=np.uint32)
+        ok, ov = gsr.sort_pairs(gsr_ctx, keys, vals, key_bits=44)
+        order = np.argsort(keys, kind="stable")
+        assert np.array_equal(ok, keys[order]), f"{name}: keys"
+        assert np.array_equal(ov, vals[order]), f"{name}: values (stability)"
+
+
+def test_sort_pairs_count_below_capacity(gsr_ctx):
+    """A device-side count < the uploaded keys: only the first `count` are sorted, the tail is untouched."""
+    import gsr
+
+    n, count = 10_000, 6_001
+    keys = np.random.default_rng(7).integers(0, 1 << 40, n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    ok, ov = gsr.sort_pairs(gsr_ctx, keys, vals, key_bits=40, count=count)
+    order = np.argsort(keys[:count], kind="stable")
+    assert np.array_equal(ok[:count], keys[:count][order])
+    assert np.array_equal(ov[:count], vals[:count][order])
+    assert np.array_equal(ok[count:], keys[count:]) and np.array_equal(ov[count:], vals[count:])
+
+
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 100_000, 1_000_003])
+def test_exclusive_scan(gsr_ctx, n):
+    """test_tile_raster.cpp:114-170: the scan vs a CPU prefix sum."""
+    import gsr
+
+    v = np.random.default_rng(n).integers(0, 50, n, dtype=np.uint32)
+    out, total = gsr.exclusive_scan(gsr_ctx, v)
+    ref = np.concatenate([[0], np.cumsum(v, dtype=np.uint64)[:-1]]).astype(np.uint32) if n else v
+    assert np.array_equal(out, ref)
+    assert total == int(v.astype(np.uint64).sum())
