@@ -438,6 +438,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     alloc(r->hist, size_t(kRadix) * kMaxSortBlocks * 4);
     alloc(r->digitTotals, kRadix * 4);
     if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
+    // Lists start as valid (zero) keys and gaussian indices.
+    for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB})
+        if (e == hipSuccess) e = hipMemset(b->ptr, 0, b->bytes);
     for (int i = 0; i < GSR_STAGE_COUNT && e == hipSuccess; ++i) e = hipEventCreate(&r->events[i]);
     if (e != hipSuccess) {
         gsr_renderer_destroy(r);
@@ -534,8 +537,8 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     launchRanges(sortedKeys, control, r->capacity, tiles, r->ranges.as<uint2>(), stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(5));
-    launchRender(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc, r->settings.format, outImage,
-                 stream);
+    launchRender(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
+                 outImage, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
     r->lastCam = *cam;
@@ -627,9 +630,9 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
     const int which = launchRadixSort(keys, values, keysB, valuesB, maxCount, countDevice, keyBits, blocks, hist,
                                       totals, stream);
     GSR_HIP(hipGetLastError());
-    if (which) {  // odd pass count: the result is in the scratch copy
-        GSR_HIP(hipMemcpyAsync(keys, keysB, n * 8, hipMemcpyDeviceToDevice, stream));
-        GSR_HIP(hipMemcpyAsync(values, valuesB, n * 4, hipMemcpyDeviceToDevice, stream));
+    if (which) {  // odd pass count: the first `count` pairs are in the scratch copy
+        launchCopyPairs(keysB, valuesB, keys, values, countDevice, maxCount, stream);
+        GSR_HIP(hipGetLastError());
     }
     return GSR_OK;
 }

@@ -74,10 +74,12 @@ void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uin
 int launchRadixSort(uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB, uint32_t maxCount,
                     const uint32_t* countDevice, uint32_t keyBits, uint32_t blocks, uint32_t* hist,
                     uint32_t* digitTotals, hipStream_t stream);
+void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,
+                     uint32_t maxCount, hipStream_t stream);
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
                   uint2* ranges, hipStream_t stream);
 void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-                  gsr_target_format format, void* out, hipStream_t stream);
+                  uint32_t gaussianCount, gsr_target_format format, void* out, hipStream_t stream);
 
 // Stand-alone scan (GpuScan): reduce, block-sum scan, downsweep.
 constexpr int kScanThreads = 256;

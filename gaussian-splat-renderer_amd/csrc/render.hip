@@ -32,7 +32,8 @@ __global__ __launch_bounds__(kRenderThreads) void kRender(const float4* __restri
                                                           const uint32_t* __restrict__ values,
                                                           const uint2* __restrict__ ranges, uint32_t width,
                                                           uint32_t height, uint32_t tilesX, uint32_t tileCount,
-                                                          uint32_t tilesPerXcd, void* __restrict__ out) {
+                                                          uint32_t tilesPerXcd, uint32_t gaussianCount,
+                                                          void* __restrict__ out) {
     __shared__ float2 sCenter[kRenderThreads];
     __shared__ float4 sConic[kRenderThreads];
     __shared__ float4 sColor[kRenderThreads];
@@ -55,7 +56,8 @@ __global__ __launch_bounds__(kRenderThreads) void kRender(const float4* __restri
         if (__syncthreads_count(done) == kRenderThreads) break;
         const uint32_t e = batch + t;
         if (e < range.y) {
-            const uint32_t g = values[e];
+            uint32_t g = values[e];
+            g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
             const float4* s = splats + 3 * size_t(g);
             const float4 o0 = s[0];
             sCenter[t] = make_float2(o0.x, o0.y);
@@ -99,17 +101,17 @@ __global__ __launch_bounds__(kRenderThreads) void kRender(const float4* __restri
 }  // namespace
 
 void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-                  gsr_target_format format, void* out, hipStream_t stream) {
+                  uint32_t gaussianCount, gsr_target_format format, void* out, hipStream_t stream) {
     const uint32_t tileCount = cam.tilesX * cam.tilesY;
     if (tileCount == 0) return;
     const uint32_t perXcd = (tileCount + kXcds - 1) / kXcds;
     const uint32_t blocks = perXcd * kXcds;
     if (format == GSR_TARGET_RGBA8)
         hipLaunchKernelGGL(kRender<GSR_TARGET_RGBA8>, dim3(blocks), dim3(kRenderThreads), 0, stream, splats, values,
-                           ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, out);
+                           ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, gaussianCount, out);
     else
         hipLaunchKernelGGL(kRender<GSR_TARGET_RGBA32F>, dim3(blocks), dim3(kRenderThreads), 0, stream, splats,
-                           values, ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, out);
+                           values, ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, gaussianCount, out);
 }
 
 }  // namespace gsr

@@ -196,7 +196,29 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const uint64_t* __r
     }
 }
 
+// Copies the first `count` pairs (count on the device): moves an odd-pass result back into
+// the caller's buffers without touching the keys past the count.
+__global__ __launch_bounds__(kSortThreads) void kCopyPairs(const uint64_t* __restrict__ ks,
+                                                           const uint32_t* __restrict__ vs, uint64_t* __restrict__ kd,
+                                                           uint32_t* __restrict__ vd,
+                                                           const uint32_t* __restrict__ countDevice,
+                                                           uint32_t maxCount) {
+    const uint32_t count = sortCount(countDevice, maxCount);
+    for (uint32_t i = blockIdx.x * kSortThreads + threadIdx.x; i < count; i += gridDim.x * kSortThreads) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
+    }
+}
+
 }  // namespace
+
+void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,
+                     uint32_t maxCount, hipStream_t stream) {
+    uint32_t blocks = (maxCount + kSortThreads - 1) / kSortThreads;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kCopyPairs, dim3(blocks), dim3(kSortThreads), 0, stream, ks, vs, kd, vd, countDevice, maxCount);
+}
 
 int launchRadixSort(uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB, uint32_t maxCount,
                     const uint32_t* countDevice, uint32_t keyBits, uint32_t blocks, uint32_t* hist,

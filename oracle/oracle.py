@@ -81,6 +81,13 @@ _render = _sig("oracle_render", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp
                                        _vp, _vp, _vp, ctypes.c_uint64])
 
 
+def default_threads() -> int:
+    """Host threads for the oracle: OMP_NUM_THREADS when set (the CPU share of a GPU box), else
+    every core up to 16."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return env if env > 0 else min(os.cpu_count() or 1, 16)
+
+
 def _p(a):
     return None if a is None else a.ctypes.data
 
@@ -163,8 +170,9 @@ def splat_alpha_at(splat, px, py):
     return float(_alpha_at(_p(s), px, py))
 
 
-def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGHT, threads=0):
+def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGHT, threads=None):
     """preprocessGaussian over the scene: (N,) splats and the per-gaussian tile counts."""
+    threads = default_threads() if threads is None else threads
     n = scene.count
     out = np.zeros(n, SPLAT_DTYPE)
     counts = np.zeros(n, np.uint32)
@@ -174,8 +182,9 @@ def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGH
     return out, counts
 
 
-def preprocess_f64(scene, cam: OracleCamera, degree=None, gpu_cull=True, threads=0):
+def preprocess_f64(scene, cam: OracleCamera, degree=None, gpu_cull=True, threads=None):
     """FP64 twin: (N, 12) doubles: cx, cy, depth, cull, conic a b c, opacity, r g b, radius."""
+    threads = default_threads() if threads is None else threads
     n = scene.count
     out = np.zeros((n, 12), np.float64)
     deg = scene.sh_degree if degree is None else min(degree, scene.sh_degree)
@@ -242,9 +251,10 @@ def decode_f16(geom, color, sh_degree):
     return pa, sc, ro, sh
 
 
-def render(scene, cam: OracleCamera, max_sh_degree=3, early_termination=True, bound=0, bounds=RECT, threads=0,
+def render(scene, cam: OracleCamera, max_sh_degree=3, early_termination=True, bound=0, bounds=RECT, threads=None,
            background=(0.0, 0.0, 0.0), want_lists=False):
     """renderReference: (H, W, 3) float32 image (unclamped) and ReferenceStats (+ per-tile lists)."""
+    threads = default_threads() if threads is None else threads
     opt = RenderOptions(max_sh_degree, 1 if early_termination else 0, bound, bounds, threads,
                         (ctypes.c_float * 3)(*background))
     rgb = np.zeros((cam.height, cam.width, 3), np.float32)
