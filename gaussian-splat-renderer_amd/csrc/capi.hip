@@ -411,6 +411,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (s.format != GSR_TARGET_RGBA8 && s.format != GSR_TARGET_RGBA32F)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown target format %d", int(s.format));
     if (s.budget_mb == 0) s.budget_mb = 256;
+    if (s.render_pixels_per_thread == 0) s.render_pixels_per_thread = 1;
+    if (s.render_pixels_per_thread != 1 && s.render_pixels_per_thread != 2 && s.render_pixels_per_thread != 4)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: render_pixels_per_thread must be 1, 2 or 4");
     GSR_HIP(hipSetDevice(ctx->device));
     gsr_renderer* r = new (std::nothrow) gsr_renderer();
     if (!r) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_renderer_create: host allocation");
@@ -538,7 +541,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(5));
     launchRender(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
-                 outImage, stream);
+                 r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0, outImage, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
     r->lastCam = *cam;

@@ -1,15 +1,22 @@
 // K8 tile_render + K9 resolve (reference shaders/tile_render.comp:147-199, 281-378; CPU twin
-// splat/reference_renderer.cpp:190-222): one 256-thread workgroup per 16x16 tile, one
-// thread per pixel, front to back over the tile's range of the sorted list.
+// splat/reference_renderer.cpp:190-222): one workgroup per 16x16 tile, front to back over
+// the tile's range of the sorted list, INRIA's blend:
+//   alpha = min(0.99, o exp(power)), skip power > 0 and alpha < 1/255,
+//   test-before-accept T (1 - alpha) < 1e-4, C += colour alpha T, output C + T * black and
+//   alpha 1 - T, as RGBA8 (clamp, round(v * 255)) or RGBA32F.
 //
-//   - batches of 256 splats are staged in LDS (centre, conic + opacity, colour), every lane
-//     then reads the same LDS word (broadcast, no bank conflicts);
-//   - whole-tile early exit: the workgroup stops once all 256 pixels are done
-//     (__syncthreads_count, INRIA's __syncthreads_count(done));
-//   - per pixel: alpha = min(0.99, o exp(power)), skip power > 0 and alpha < 1/255,
-//     test-before-accept T (1 - alpha) < 1e-4;
-//   - the result C + T * black, alpha 1 - T, is written once: RGBA8 (clamp, round(v * 255))
-//     or RGBA32F;
+// gfx950 design (VALU-bound: every (pixel, splat) pair costs a few dozen lane ops):
+//   - batches of 256 splats are staged in LDS; all lanes read the same LDS words
+//     (broadcast, no bank conflicts);
+//   - PPT pixels per thread (1, 2 or 4 rows of the same column): the LDS reads, the x
+//     offset and the loop overhead are shared by the thread's pixels; 256 / PPT threads per
+//     tile;
+//   - alpha in the log2 domain (FAST): at fetch time each splat's conic is scaled by
+//     -log2(e)/2 and log2(o) is stored, so a pair costs 5 FMA-class ops to get
+//     q = log2(o exp(power)); pairs with q < log2(1/255) (alpha < 1/255) are skipped before
+//     any exponential, the rest take one v_exp_f32. EXACT keeps the reference's operation
+//     order and expf for every pair (bitwise the CPU expression, up to expf's last ulp);
+//   - whole-tile early exit: the workgroup stops once all pixels are done (__syncthreads_count);
 //   - XCD-aware tile order: workgroup b runs on XCD b % 8, so b -> tile maps each XCD to a
 //     contiguous band of tiles and neighbouring tiles (which share splats) share an L2.
 #include "gsr_internal.hpp"
@@ -18,8 +25,10 @@ namespace gsr {
 
 namespace {
 
-constexpr int kRenderThreads = kTileSize * kTileSize;
 constexpr uint32_t kXcds = 8;
+constexpr int kBatch = 256;
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLog2MinAlpha = -7.994353436858858f;  // log2(1/255)
 
 __device__ __forceinline__ uint32_t quantize8(float v) {
     v = v > 0.0f ? v : 0.0f;  // also NaN -> 0
@@ -27,91 +36,160 @@ __device__ __forceinline__ uint32_t quantize8(float v) {
     return uint32_t(roundf(v * 255.0f));
 }
 
-template <int FORMAT>
-__global__ __launch_bounds__(kRenderThreads) void kRender(const float4* __restrict__ splats,
-                                                          const uint32_t* __restrict__ values,
-                                                          const uint2* __restrict__ ranges, uint32_t width,
-                                                          uint32_t height, uint32_t tilesX, uint32_t tileCount,
-                                                          uint32_t tilesPerXcd, uint32_t gaussianCount,
-                                                          void* __restrict__ out) {
-    __shared__ float2 sCenter[kRenderThreads];
-    __shared__ float4 sConic[kRenderThreads];
-    __shared__ float4 sColor[kRenderThreads];
+template <int FORMAT, bool FAST, int PPT>
+__global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ splats,
+                                                     const uint32_t* __restrict__ values,
+                                                     const uint2* __restrict__ ranges, uint32_t width,
+                                                     uint32_t height, uint32_t tilesX, uint32_t tileCount,
+                                                     uint32_t tilesPerXcd, uint32_t gaussianCount,
+                                                     void* __restrict__ out) {
+    constexpr int kThreads = 256 / PPT;
+    constexpr int kRowStep = 16 / PPT;
+    // FAST: s0 = (cx, cy, A, B), s1 = (C, log2 o), A/B/C the conic scaled by -log2(e)/2 (B: -log2 e).
+    // EXACT: s0 = (cx, cy, a, b), s1 = (c, o).
+    __shared__ float4 sS0[kBatch];
+    __shared__ float2 sS1[kBatch];
+    __shared__ float4 sColor[kBatch];
 
     const uint32_t tile = (blockIdx.x % kXcds) * tilesPerXcd + blockIdx.x / kXcds;
     if (tile >= tileCount) return;  // uniform
     const uint32_t t = threadIdx.x;
-    const uint32_t px = (tile % tilesX) * kTileSize + (t % kTileSize);
-    const uint32_t py = (tile / tilesX) * kTileSize + (t / kTileSize);
-    const bool inside = px < width && py < height;
-    const float pxf = float(px), pyf = float(py);
+    const uint32_t px = (tile % tilesX) * 16 + (t % 16);
+    const uint32_t py0 = (tile / tilesX) * 16 + (t / 16);
+    const float pxf = float(px);
+
+    float T[PPT], C0[PPT], C1[PPT], C2[PPT], pyf[PPT];
+    bool done[PPT];
+    bool alive = false;
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        const uint32_t py = py0 + i * kRowStep;
+        pyf[i] = float(py);
+        T[i] = 1.0f;
+        C0[i] = C1[i] = C2[i] = 0.0f;
+        done[i] = !(px < width && py < height);
+        alive = alive || !done[i];
+    }
 
     const uint2 range = ranges[tile];
-    float T = 1.0f;
-    float C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
-    bool done = !inside;
-
-    for (uint32_t batch = range.x; batch < range.y; batch += kRenderThreads) {
+    for (uint32_t batch = range.x; batch < range.y; batch += kBatch) {
         // Whole-tile vote; also the barrier that protects the previous batch's LDS.
-        if (__syncthreads_count(done) == kRenderThreads) break;
-        const uint32_t e = batch + t;
-        if (e < range.y) {
-            uint32_t g = values[e];
-            g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
-            const float4* s = splats + 3 * size_t(g);
-            const float4 o0 = s[0];
-            sCenter[t] = make_float2(o0.x, o0.y);
-            sConic[t] = s[1];
-            sColor[t] = s[2];
+        if (__syncthreads_count(alive ? 0 : 1) == kThreads) break;
+#pragma unroll
+        for (int f = 0; f < kBatch / kThreads; ++f) {
+            const uint32_t slot = t + f * kThreads;
+            const uint32_t e = batch + slot;
+            if (e < range.y) {
+                uint32_t g = values[e];
+                g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
+                const float4* s = splats + 3 * size_t(g);
+                const float4 o0 = s[0];
+                const float4 co = s[1];
+                sColor[slot] = s[2];
+                if (FAST) {
+                    sS0[slot] = make_float4(o0.x, o0.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+                    sS1[slot] = make_float2(-0.5f * kLog2e * co.z, __log2f(co.w));
+                } else {
+                    sS0[slot] = make_float4(o0.x, o0.y, co.x, co.y);
+                    sS1[slot] = make_float2(co.z, co.w);
+                }
+            }
         }
         __syncthreads();
-        const uint32_t n = range.y - batch < uint32_t(kRenderThreads) ? range.y - batch : uint32_t(kRenderThreads);
-        for (uint32_t j = 0; !done && j < n; ++j) {
-            const float2 c = sCenter[j];
-            const float4 co = sConic[j];
-            const float dx = c.x - pxf;
-            const float dy = c.y - pyf;
-            const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-            if (power > 0.0f) continue;
-            const float ex = co.w * expf(power);
-            const float alpha = kMaxAlpha < ex ? kMaxAlpha : ex;
-            if (alpha < kMinAlpha) continue;
-            const float testT = T * (1.0f - alpha);
-            if (testT < kMinTransmittance) {
-                done = true;
-                break;
+        const uint32_t n = range.y - batch < uint32_t(kBatch) ? range.y - batch : uint32_t(kBatch);
+        for (uint32_t j = 0; alive && j < n; ++j) {
+            const float4 s0 = sS0[j];
+            const float2 s1 = sS1[j];
+            const float dx = s0.x - pxf;
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) {
+                if (done[i]) continue;
+                const float dy = s0.y - pyf[i];
+                float alpha;
+                if (FAST) {
+                    // q = log2(o) + log2(e) * power, power = -0.5 (a dx^2 + c dy^2) - b dx dy.
+                    const float p = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, s1.x * dy * dy);
+                    const float q = p + s1.y;
+                    if (q < kLog2MinAlpha || p > 0.0f) continue;
+                    const float ex = __builtin_amdgcn_exp2f(q);
+                    alpha = kMaxAlpha < ex ? kMaxAlpha : ex;
+                    if (alpha < kMinAlpha) continue;
+                } else {
+                    const float power = -0.5f * (s0.z * dx * dx + s1.x * dy * dy) - s0.w * dx * dy;
+                    if (power > 0.0f) continue;
+                    const float ex = s1.y * expf(power);
+                    alpha = kMaxAlpha < ex ? kMaxAlpha : ex;
+                    if (alpha < kMinAlpha) continue;
+                }
+                const float testT = T[i] * (1.0f - alpha);
+                if (testT < kMinTransmittance) {
+                    done[i] = true;
+                    continue;
+                }
+                const float4 col = sColor[j];
+                if (FAST) {
+                    const float w = alpha * T[i];
+                    C0[i] = fmaf(col.x, w, C0[i]);
+                    C1[i] = fmaf(col.y, w, C1[i]);
+                    C2[i] = fmaf(col.z, w, C2[i]);
+                } else {
+                    C0[i] += col.x * alpha * T[i];
+                    C1[i] += col.y * alpha * T[i];
+                    C2[i] += col.z * alpha * T[i];
+                }
+                T[i] = testT;
             }
-            const float4 col = sColor[j];
-            C0 += col.x * alpha * T;
-            C1 += col.y * alpha * T;
-            C2 += col.z * alpha * T;
-            T = testT;
+            bool any = false;
+#pragma unroll
+            for (int i = 0; i < PPT; ++i) any = any || !done[i];
+            alive = any;
         }
     }
-    if (!inside) return;
-    const size_t p = size_t(py) * width + px;
-    if (FORMAT == GSR_TARGET_RGBA8) {
-        const uint32_t rgba = quantize8(C0) | (quantize8(C1) << 8) | (quantize8(C2) << 16) | (quantize8(1.0f - T) << 24);
-        static_cast<uint32_t*>(out)[p] = rgba;
-    } else {
-        static_cast<float4*>(out)[p] = make_float4(C0, C1, C2, 1.0f - T);
+#pragma unroll
+    for (int i = 0; i < PPT; ++i) {
+        const uint32_t py = py0 + i * kRowStep;
+        if (!(px < width && py < height)) continue;
+        const size_t p = size_t(py) * width + px;
+        if (FORMAT == GSR_TARGET_RGBA8) {
+            const uint32_t rgba = quantize8(C0[i]) | (quantize8(C1[i]) << 8) | (quantize8(C2[i]) << 16) |
+                                  (quantize8(1.0f - T[i]) << 24);
+            static_cast<uint32_t*>(out)[p] = rgba;
+        } else {
+            static_cast<float4*>(out)[p] = make_float4(C0[i], C1[i], C2[i], 1.0f - T[i]);
+        }
     }
+}
+
+template <int FORMAT, bool FAST, int PPT>
+void launchOne(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+               uint32_t gaussianCount, void* out, hipStream_t stream) {
+    const uint32_t tileCount = cam.tilesX * cam.tilesY;
+    const uint32_t perXcd = (tileCount + kXcds - 1) / kXcds;
+    hipLaunchKernelGGL((kRender<FORMAT, FAST, PPT>), dim3(perXcd * kXcds), dim3(256 / PPT), 0, stream, splats, values,
+                       ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, gaussianCount, out);
+}
+
+template <int FORMAT, bool FAST>
+void launchPpt(int ppt, const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+               uint32_t gaussianCount, void* out, hipStream_t stream) {
+    if (ppt == 4) launchOne<FORMAT, FAST, 4>(splats, values, ranges, cam, gaussianCount, out, stream);
+    else if (ppt == 2) launchOne<FORMAT, FAST, 2>(splats, values, ranges, cam, gaussianCount, out, stream);
+    else launchOne<FORMAT, FAST, 1>(splats, values, ranges, cam, gaussianCount, out, stream);
 }
 
 }  // namespace
 
 void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-                  uint32_t gaussianCount, gsr_target_format format, void* out, hipStream_t stream) {
-    const uint32_t tileCount = cam.tilesX * cam.tilesY;
-    if (tileCount == 0) return;
-    const uint32_t perXcd = (tileCount + kXcds - 1) / kXcds;
-    const uint32_t blocks = perXcd * kXcds;
-    if (format == GSR_TARGET_RGBA8)
-        hipLaunchKernelGGL(kRender<GSR_TARGET_RGBA8>, dim3(blocks), dim3(kRenderThreads), 0, stream, splats, values,
-                           ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, gaussianCount, out);
-    else
-        hipLaunchKernelGGL(kRender<GSR_TARGET_RGBA32F>, dim3(blocks), dim3(kRenderThreads), 0, stream, splats,
-                           values, ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, gaussianCount, out);
+                  uint32_t gaussianCount, gsr_target_format format, int pixelsPerThread, bool exactAlpha, void* out,
+                  hipStream_t stream) {
+    if (cam.tilesX * cam.tilesY == 0) return;
+    if (format == GSR_TARGET_RGBA8) {
+        if (exactAlpha) launchPpt<GSR_TARGET_RGBA8, false>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
+        else launchPpt<GSR_TARGET_RGBA8, true>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
+    } else {
+        if (exactAlpha) launchPpt<GSR_TARGET_RGBA32F, false>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
+        else launchPpt<GSR_TARGET_RGBA32F, true>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
+    }
 }
 
 }  // namespace gsr

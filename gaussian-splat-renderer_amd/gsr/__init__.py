@@ -91,7 +91,9 @@ class Settings(ctypes.Structure):
         ("format", ctypes.c_int32),
         ("profile_stages", ctypes.c_int32),
         ("sort_blocks", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 6),
+        ("render_pixels_per_thread", ctypes.c_int32),
+        ("render_exact_alpha", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
 
 
@@ -341,7 +343,8 @@ class Renderer:
     """SplatTileRenderer for one scene (init + render + the test readbacks)."""
 
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
-                 max_sh_degree: int = 3, profile_stages: bool = False, sort_blocks: int = 0):
+                 max_sh_degree: int = 3, profile_stages: bool = False, sort_blocks: int = 0,
+                 pixels_per_thread: int = 0, exact_alpha: bool = False):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -349,6 +352,8 @@ class Renderer:
         s.format = TARGETS[target]
         s.profile_stages = 1 if profile_stages else 0
         s.sort_blocks = sort_blocks
+        s.render_pixels_per_thread = pixels_per_thread
+        s.render_exact_alpha = 1 if exact_alpha else 0
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds

@@ -121,7 +121,11 @@ typedef struct gsr_settings {
     gsr_target_format format;    /* output image format */
     int32_t profile_stages;      /* 1: hipEvent pairs around every stage (gsr_read_stage_times) */
     uint32_t sort_blocks;        /* 0 = default; radix-sort grid (blocks), tuning knob */
-    uint32_t reserved[6];
+    int32_t render_pixels_per_thread; /* tile_render: 1, 2 or 4 pixels per thread (0 = default 1) */
+    int32_t render_exact_alpha;  /* 1: alpha = min(0.99, o expf(power)) in the reference's operation order for
+                                    every pair; 0 (default): log2-domain alpha with v_exp_f32 and a cutoff
+                                    test before the exponential (same image within 1/255) */
+    uint32_t reserved[4];
 } gsr_settings;
 
 /* Stage timings of the last profiled frame, ms (the GpuProfiler scopes, profiler.h:1-87). */
