@@ -84,7 +84,9 @@ uint32_t bitsFor(uint32_t v) {  // bits needed to hold values 0..v
 struct gsr_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    DeviceBuffer scratch;  // stand-alone sort/scan scratch
+    DeviceBuffer scratch;     // stand-alone sort/scan scratch
+    DeviceBuffer sortStatus;  // stand-alone sort: look-back status words (epoch-tagged, zeroed once)
+    uint32_t sortEpoch = 0;
 };
 
 struct gsr_buffer {
@@ -107,12 +109,12 @@ struct gsr_renderer {
     const gsr_scene* scene = nullptr;
     gsr_settings settings{};
     uint32_t capacity = 0;
-    uint32_t sortBlocks = 0;
     int32_t degree = 0;
     // Per gaussian.
     DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control;
-    // The list (ping-pong) and the sort's histograms.
-    DeviceBuffer keysA, keysB, valuesA, valuesB, hist, digitTotals;
+    // The list (ping-pong) and the sort's work areas.
+    DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus;
+    uint32_t sortEpoch = 0;
     // Per tile (sized for the largest camera seen).
     DeviceBuffer ranges;
     uint32_t rangesTiles = 0;
@@ -422,8 +424,6 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     r->settings = s;
     r->degree = std::min(scene->shDegree, std::max(0, std::min(3, s.max_sh_degree)));
     r->capacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0);
-    r->sortBlocks = s.sort_blocks ? std::min<uint32_t>(s.sort_blocks, kMaxSortBlocks)
-                                  : std::min<uint32_t>(kMaxSortBlocks, std::max<uint32_t>(1, (r->capacity + 2047) / 2048));
     const size_t n = std::max<size_t>(scene->count, 1);
     const size_t gBlocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     hipError_t e = hipSuccess;
@@ -438,8 +438,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     alloc(r->keysB, size_t(r->capacity) * 8);
     alloc(r->valuesA, size_t(r->capacity) * 4);
     alloc(r->valuesB, size_t(r->capacity) * 4);
-    alloc(r->hist, size_t(kRadix) * kMaxSortBlocks * 4);
-    alloc(r->digitTotals, kRadix * 4);
+    alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses) * 4);
+    alloc(r->sortStatus, sortStatusBytes(r->capacity));
+    if (e == hipSuccess) e = hipMemset(r->sortStatus.ptr, 0, r->sortStatus.bytes);
     if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
     // Lists start as valid (zero) keys and gaussian indices.
     for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB})
@@ -467,8 +468,8 @@ uint32_t gsr_renderer_capacity(const gsr_renderer* r) { return r ? r->capacity :
 uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     if (!r) return 0;
     const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
-                                  &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->hist,
-                                  &r->digitTotals, &r->ranges};
+                                  &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
+                                  &r->sortStatus, &r->ranges};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     return s;
@@ -530,9 +531,11 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
                     r->valuesA.as<uint32_t>(), control, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(3));
-    const int which = launchRadixSort(r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),
-                                      r->valuesB.as<uint32_t>(), r->capacity, &control->entries, keyBits,
-                                      r->sortBlocks, r->hist.as<uint32_t>(), r->digitTotals.as<uint32_t>(), stream);
+    SortWork work{r->sortSmall.as<uint32_t>(), r->sortSmall.as<uint32_t>() + kMaxSortPasses * kRadix,
+                  r->sortStatus.as<uint64_t>(), &control->sort_timeouts};
+    const int which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(),
+                                      r->keysB.as<uint64_t>(), r->valuesB.as<uint32_t>(), r->capacity,
+                                      &control->entries, keyBits, &r->sortEpoch, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(4));
     const uint64_t* sortedKeys = which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>();
@@ -618,20 +621,26 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
     GSR_HIP(hipSetDevice(ctx->device));
     hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : ctx->stream;
     const size_t n = maxCount;
-    const size_t histBytes = size_t(kRadix) * kMaxSortBlocks * 4;
-    const size_t need = n * 8 + n * 4 + histBytes + kRadix * 4 + 256;
+    const size_t smallBytes = (kMaxSortPasses * kRadix + kMaxSortPasses + 1) * 4;
+    const size_t need = n * 8 + n * 4 + smallBytes + 256;
     if (ctx->scratch.bytes < need) {
         GSR_HIP(hipDeviceSynchronize());
         GSR_HIP(ctx->scratch.allocate(need));
     }
+    if (ctx->sortStatus.bytes < sortStatusBytes(maxCount)) {
+        GSR_HIP(hipDeviceSynchronize());
+        GSR_HIP(ctx->sortStatus.allocate(sortStatusBytes(maxCount)));
+        GSR_HIP(hipMemset(ctx->sortStatus.ptr, 0, ctx->sortStatus.bytes));
+        ctx->sortEpoch = 0;
+    }
     char* base = ctx->scratch.as<char>();
     uint64_t* keysB = reinterpret_cast<uint64_t*>(base);
     uint32_t* valuesB = reinterpret_cast<uint32_t*>(base + n * 8);
-    uint32_t* hist = reinterpret_cast<uint32_t*>(base + n * 12);
-    uint32_t* totals = reinterpret_cast<uint32_t*>(base + n * 12 + histBytes);
-    const uint32_t blocks = std::min<uint32_t>(kMaxSortBlocks, std::max<uint32_t>(1, (maxCount + 2047) / 2048));
-    const int which = launchRadixSort(keys, values, keysB, valuesB, maxCount, countDevice, keyBits, blocks, hist,
-                                      totals, stream);
+    uint32_t* small = reinterpret_cast<uint32_t*>(base + n * 12);
+    SortWork work{small, small + kMaxSortPasses * kRadix, ctx->sortStatus.as<uint64_t>(),
+                  small + kMaxSortPasses * kRadix + kMaxSortPasses};
+    const int which = launchRadixSort(work, keys, values, keysB, valuesB, maxCount, countDevice, keyBits,
+                                      &ctx->sortEpoch, stream);
     GSR_HIP(hipGetLastError());
     if (which) {  // odd pass count: the first `count` pairs are in the scratch copy
         launchCopyPairs(keysB, valuesB, keys, values, countDevice, maxCount, stream);

@@ -52,14 +52,26 @@ struct SceneRef {
     uint32_t colorQuads;     // uint4s per colour record
 };
 
-// Sorting: stable LSD radix sort, 8-bit digits, 64-bit keys (tile << 32 | depth bits).
+// Sorting: stable LSD radix sort, 8-bit digits, 64-bit keys (tile << 32 | depth bits),
+// one sweep per digit over tiles of 2048 keys.
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortKeysPerLane = 8;                               // per wave round-robin
-constexpr int kSortTile = kSortThreads * kSortKeysPerLane;        // 2048 keys per block tile
-constexpr uint32_t kMaxSortBlocks = 1024;
+constexpr int kSortKeysPerLane = 8;                               // rounds of 64 keys per wave
+constexpr int kSortTile = kSortThreads * kSortKeysPerLane;        // 2048 keys per tile
+constexpr uint32_t kMaxSortPasses = 8;                            // 64-bit keys
+
+// Device work areas of one sort: global digit histograms (passes x 256), the per-pass
+// tile tickets, the look-back status words (tiles x 256, never cleared: epoch-tagged) and
+// a timeout counter (0 unless a look-back wait gave up).
+struct SortWork {
+    uint32_t* globalHist;
+    uint32_t* tickets;
+    uint64_t* status;
+    uint32_t* timeouts;
+};
+size_t sortStatusBytes(uint32_t maxCount);
 
 // Launch wrappers (each in its .hip file).
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
@@ -70,10 +82,11 @@ void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
                      const gsr_control* control, uint32_t count, uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
                      uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream);
-// Returns which buffer (0 = keysA/valuesA, 1 = B) holds the sorted result.
-int launchRadixSort(uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB, uint32_t maxCount,
-                    const uint32_t* countDevice, uint32_t keyBits, uint32_t blocks, uint32_t* hist,
-                    uint32_t* digitTotals, hipStream_t stream);
+// Returns which buffer (0 = keysA/valuesA, 1 = B) holds the sorted result. *epoch is the
+// caller's status-epoch counter (one per status array), advanced once per pass.
+int launchRadixSort(const SortWork& work, uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB,
+                    uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, uint32_t* epoch,
+                    hipStream_t stream);
 void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,
                      uint32_t maxCount, hipStream_t stream);
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,

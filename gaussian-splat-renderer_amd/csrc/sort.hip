@@ -1,23 +1,31 @@
 // K6 radix sort (reference GpuRadixSort, render/gpu_radix_sort.{h,cpp}; shaders
 // radix_histogram.comp, radix_scan.comp, radix_scatter_reorder.comp): stable LSD sort of
-// (64-bit key, u32 value) pairs, 8-bit digits, one histogram / scan / scatter per digit.
+// (64-bit key, u32 value) pairs by 8-bit digits.
 //
-// gfx950 design:
-//   - the key count lives on the device (the frame's entries = min(M, capacity)); the grid
-//     is fixed (G blocks) and block b sorts the contiguous slice [b*chunk, (b+1)*chunk),
-//     chunk = ceil(count / G), in tiles of 2048 keys: no host read-back, no indirect launch;
-//   - histogram: per-wave LDS histograms, written digit-major hist[d * G + b];
-//   - scan: one workgroup per digit scans its G block counts (exclusive) and the digit total;
-//   - scatter: each wave64 owns 512 consecutive keys of the tile and ranks them with 8
-//     64-lane ballots per key (the lanes holding the same digit) plus a running per-wave
-//     digit counter in LDS: ranks follow the input order (stable) without any workgroup
-//     barrier inside the wave's rounds. The tile is then reordered by digit in LDS and
-//     written out as contiguous per-digit runs (coalesced stores).
+// gfx950 design, one sweep per digit:
+//   - the key count lives on the device (the frame's entries = min(M, capacity)): every
+//     kernel reads it there; grids are sized for the capacity and idle blocks exit;
+//   - one up-front pass reads the keys once and builds the global histograms of every
+//     digit position (LDS, then one atomic per block and bin);
+//   - each digit pass is ONE kernel over tiles of 2048 keys. Each wave64 owns 512
+//     consecutive keys of the tile and ranks them with 8 64-lane ballots per key (the lanes
+//     holding the same digit) and a running per-wave digit counter in LDS, so ranks follow
+//     the input order (stable) with no workgroup barrier inside the wave's rounds. The tile's
+//     per-digit offset among all earlier tiles comes from a decoupled look-back over
+//     per-(tile, digit) status words (aggregate / inclusive prefix, tagged with the pass's
+//     epoch, so the status array never needs clearing). Tiles take their index from an
+//     atomic ticket, so a tile only waits on tiles that are already running; the wait is
+//     bounded and a timeout is counted, never hung on. The tile is then reordered by digit
+//     in LDS and written out as contiguous per-digit runs (coalesced stores).
 #include "gsr_internal.hpp"
 
 namespace gsr {
 
 namespace {
+
+constexpr uint64_t kFlagAggregate = 1;
+constexpr uint64_t kFlagPrefix = 2;
+constexpr uint32_t kSpinLimit = 1u << 20;
 
 __device__ __forceinline__ uint32_t sortCount(const uint32_t* countDevice, uint32_t maxCount) {
     if (!countDevice) return maxCount;
@@ -27,30 +35,11 @@ __device__ __forceinline__ uint32_t sortCount(const uint32_t* countDevice, uint3
 
 __device__ __forceinline__ uint32_t digitOf(uint64_t key, uint32_t shift) { return uint32_t(key >> shift) & 0xFFu; }
 
-__global__ __launch_bounds__(kSortThreads) void kSortHistogram(const uint64_t* __restrict__ keys,
-                                                               const uint32_t* __restrict__ countDevice,
-                                                               uint32_t maxCount, uint32_t shift,
-                                                               uint32_t* __restrict__ hist) {
-    __shared__ uint32_t sHist[kSortWaves][kRadix];
-    const uint32_t t = threadIdx.x, wave = t >> 6;
-    for (uint32_t d = t; d < kSortWaves * kRadix; d += kSortThreads) (&sHist[0][0])[d] = 0u;
-    __syncthreads();
-    const uint32_t count = sortCount(countDevice, maxCount);
-    const uint32_t G = gridDim.x;
-    const uint32_t chunk = (count + G - 1) / G;
-    const uint32_t begin = blockIdx.x * chunk;
-    const uint32_t end = begin + chunk < count ? begin + chunk : count;
-    for (uint32_t idx = begin + t; idx < end; idx += kSortThreads) atomicAdd(&sHist[wave][digitOf(keys[idx], shift)], 1u);
-    __syncthreads();
-    for (uint32_t d = t; d < kRadix; d += kSortThreads) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) s += sHist[w][d];
-        hist[d * G + blockIdx.x] = s;
-    }
+__device__ __forceinline__ uint64_t packStatus(uint32_t epoch, uint64_t flag, uint32_t value) {
+    return (uint64_t(epoch) << 34) | (flag << 32) | uint64_t(value);
 }
 
-// Exclusive scan of v over the 256 threads of a workgroup.
+// Exclusive scan of v over the 256 threads of a workgroup (total optional).
 __device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* sWave, uint32_t* total) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint32_t inc = v;
@@ -70,129 +59,155 @@ __device__ __forceinline__ uint32_t blockExclusiveScan256(uint32_t v, uint32_t* 
     return prefix + inc - v;
 }
 
-// One workgroup per digit: exclusive scan of hist[d * G .. d * G + G), digit total.
-__global__ __launch_bounds__(kSortThreads) void kSortScan(uint32_t* __restrict__ hist, uint32_t G,
-                                                          uint32_t* __restrict__ digitTotals) {
-    __shared__ uint32_t sWave[kSortWaves];
-    uint32_t* row = hist + size_t(blockIdx.x) * G;
-    constexpr int kPer = int(kMaxSortBlocks) / kSortThreads;  // 4
-    const uint32_t first = threadIdx.x * kPer;
-    uint32_t v[kPer];
-    uint32_t s = 0;
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        v[j] = first + j < G ? row[first + j] : 0u;
-        s += v[j];
-    }
-    uint32_t total;
-    uint32_t run = blockExclusiveScan256(s, sWave, &total);
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        if (first + j < G) row[first + j] = run;
-        run += v[j];
-    }
-    if (threadIdx.x == 0) digitTotals[blockIdx.x] = total;
+// Zeroes the global histograms and the tickets of this sort.
+__global__ __launch_bounds__(kSortThreads) void kSortInit(uint32_t* __restrict__ globalHist,
+                                                          uint32_t* __restrict__ tickets, uint32_t passes) {
+    for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kSortThreads) globalHist[i] = 0u;
+    if (threadIdx.x < kMaxSortPasses) tickets[threadIdx.x] = 0u;
 }
 
-__global__ __launch_bounds__(kSortThreads) void kSortScatter(const uint64_t* __restrict__ keysIn,
-                                                             const uint32_t* __restrict__ valsIn,
-                                                             uint64_t* __restrict__ keysOut,
-                                                             uint32_t* __restrict__ valsOut,
+// Global digit histograms of every pass, in one read of the keys.
+__global__ __launch_bounds__(kSortThreads) void kSortUpfront(const uint64_t* __restrict__ keys,
                                                              const uint32_t* __restrict__ countDevice,
-                                                             uint32_t maxCount, uint32_t shift,
-                                                             const uint32_t* __restrict__ hist,
-                                                             const uint32_t* __restrict__ digitTotals) {
+                                                             uint32_t maxCount, uint32_t passes,
+                                                             uint32_t* __restrict__ globalHist) {
+    __shared__ uint32_t sHist[kMaxSortPasses][kRadix];
+    for (uint32_t i = threadIdx.x; i < kMaxSortPasses * kRadix; i += kSortThreads) (&sHist[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t count = sortCount(countDevice, maxCount);
+    for (uint32_t idx = blockIdx.x * kSortThreads + threadIdx.x; idx < count; idx += gridDim.x * kSortThreads) {
+        const uint64_t k = keys[idx];
+        for (uint32_t p = 0; p < passes; ++p) atomicAdd(&sHist[p][digitOf(k, p * kRadixBits)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kSortThreads) {
+        const uint32_t c = (&sHist[0][0])[i];
+        if (c) atomicAdd(&globalHist[i], c);
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void kSortOnesweep(const uint64_t* __restrict__ keysIn,
+                                                              const uint32_t* __restrict__ valsIn,
+                                                              uint64_t* __restrict__ keysOut,
+                                                              uint32_t* __restrict__ valsOut,
+                                                              const uint32_t* __restrict__ countDevice,
+                                                              uint32_t maxCount, uint32_t shift,
+                                                              const uint32_t* __restrict__ passHist,
+                                                              uint64_t* __restrict__ status, uint32_t* ticket,
+                                                              uint32_t epoch, uint32_t* timeouts) {
     __shared__ uint64_t sKeys[kSortTile];
     __shared__ uint32_t sVals[kSortTile];
     __shared__ uint32_t sWaveHist[kSortWaves][kRadix];
     __shared__ uint32_t sGlobalBase[kRadix];
     __shared__ uint32_t sLocalStart[kRadix];
-    __shared__ uint32_t sTileCount[kRadix];
     __shared__ uint32_t sWave[kSortWaves];
+    __shared__ uint32_t sTile;
 
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    if (t == 0) sTile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tileIdx = sTile;
     const uint32_t count = sortCount(countDevice, maxCount);
-    const uint32_t G = gridDim.x;
-    const uint32_t chunk = (count + G - 1) / G;
-    const uint32_t begin = blockIdx.x * chunk;
-    const uint32_t end = begin + chunk < count ? begin + chunk : count;
-    if (begin >= end) return;  // uniform
+    const uint32_t tileBegin = tileIdx * uint32_t(kSortTile);
+    if (tileBegin >= count) return;  // uniform: past the keys
+    const uint32_t tileN = count - tileBegin < uint32_t(kSortTile) ? count - tileBegin : uint32_t(kSortTile);
 
-    {  // global base of (digit, block) = exclusive prefix of the digit totals + the in-digit scan
-        const uint32_t d = t;  // kSortThreads == kRadix
-        const uint32_t start = blockExclusiveScan256(digitTotals[d], sWave, nullptr);
-        sGlobalBase[d] = start + hist[d * G + blockIdx.x];
-    }
+    for (uint32_t d = lane; d < kRadix; d += 64) sWaveHist[wave][d] = 0u;
     const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
+    uint64_t key[kSortKeysPerLane];
+    uint32_t val[kSortKeysPerLane], dig[kSortKeysPerLane], rank[kSortKeysPerLane];
+#pragma unroll
+    for (int r = 0; r < kSortKeysPerLane; ++r) {
+        const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
+        const bool valid = pos < tileN;
+        key[r] = valid ? keysIn[tileBegin + pos] : 0ull;
+        val[r] = valid ? valsIn[tileBegin + pos] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kSortKeysPerLane; ++r) {
+        const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
+        const bool valid = pos < tileN;
+        const uint32_t d = digitOf(key[r], shift);
+        dig[r] = d;
+        // Lanes holding the same digit: 8 ballots, one per digit bit.
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        if (valid) {
+            const uint64_t lower = peers & laneMaskLt;
+            const uint32_t old = sWaveHist[wave][d];
+            rank[r] = old + uint32_t(__popcll(lower));
+            if (lower == 0u) sWaveHist[wave][d] = old + uint32_t(__popcll(peers));
+        }
+    }
+    __syncthreads();
 
-    for (uint32_t tileBegin = begin; tileBegin < end; tileBegin += kSortTile) {
-        const uint32_t tileN = end - tileBegin < uint32_t(kSortTile) ? end - tileBegin : uint32_t(kSortTile);
-        for (uint32_t d = lane; d < kRadix; d += 64) sWaveHist[wave][d] = 0u;
+    // Thread d: the tile's count of digit d, its offset per wave, and the decoupled look-back.
+    const uint32_t d = t;  // kSortThreads == kRadix
+    uint32_t tileCount = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) {
+        const uint32_t c = sWaveHist[w][d];
+        sWaveHist[w][d] = tileCount;
+        tileCount += c;
+    }
+    uint64_t* myStatus = status + size_t(tileIdx) * kRadix + d;
+    if (tileIdx == 0) {
+        __hip_atomic_store(myStatus, packStatus(epoch, kFlagPrefix, tileCount), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(myStatus, packStatus(epoch, kFlagAggregate, tileCount), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    sLocalStart[d] = blockExclusiveScan256(tileCount, sWave, nullptr);
+    const uint32_t digitStart = blockExclusiveScan256(passHist[d], sWave, nullptr);
+    uint32_t before = 0;
+    if (tileIdx > 0) {
+        int32_t j = int32_t(tileIdx) - 1;
+        uint32_t spins = 0;
+        while (j >= 0) {
+            const uint64_t s = __hip_atomic_load(status + size_t(j) * kRadix + d, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t se = uint32_t(s >> 34);
+            const uint64_t flag = (s >> 32) & 3u;
+            if (se != epoch || flag == 0) {
+                if (++spins > kSpinLimit) {  // never expected: count it and give up rather than hang
+                    atomicAdd(timeouts, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            before += uint32_t(s);
+            if (flag == kFlagPrefix) break;
+            --j;
+        }
+        __hip_atomic_store(myStatus, packStatus(epoch, kFlagPrefix, before + tileCount), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    sGlobalBase[d] = digitStart + before;
+    __syncthreads();
 
-        uint64_t key[kSortKeysPerLane];
-        uint32_t val[kSortKeysPerLane], dig[kSortKeysPerLane], rank[kSortKeysPerLane];
 #pragma unroll
-        for (int r = 0; r < kSortKeysPerLane; ++r) {
-            const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
-            const bool valid = pos < tileN;
-            key[r] = valid ? keysIn[tileBegin + pos] : 0ull;
-            val[r] = valid ? valsIn[tileBegin + pos] : 0u;
+    for (int r = 0; r < kSortKeysPerLane; ++r) {
+        const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
+        if (pos < tileN) {
+            const uint32_t lr = sLocalStart[dig[r]] + sWaveHist[wave][dig[r]] + rank[r];
+            sKeys[lr] = key[r];
+            sVals[lr] = val[r];
         }
-#pragma unroll
-        for (int r = 0; r < kSortKeysPerLane; ++r) {
-            const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
-            const bool valid = pos < tileN;
-            const uint32_t d = digitOf(key[r], shift);
-            dig[r] = d;
-            // Lanes holding the same digit: 8 ballots, one per digit bit.
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < kRadixBits; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t bal = __ballot(bit);
-                peers &= bit ? bal : ~bal;
-            }
-            if (valid) {
-                const uint64_t lower = peers & laneMaskLt;
-                const uint32_t old = sWaveHist[wave][d];
-                rank[r] = old + uint32_t(__popcll(lower));
-                if (lower == 0u) sWaveHist[wave][d] = old + uint32_t(__popcll(peers));
-            }
-        }
-        __syncthreads();
-        {
-            const uint32_t d = t;
-            uint32_t run = 0;
-#pragma unroll
-            for (int w = 0; w < kSortWaves; ++w) {
-                const uint32_t c = sWaveHist[w][d];
-                sWaveHist[w][d] = run;
-                run += c;
-            }
-            sTileCount[d] = run;
-            sLocalStart[d] = blockExclusiveScan256(run, sWave, nullptr);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kSortKeysPerLane; ++r) {
-            const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
-            if (pos < tileN) {
-                const uint32_t lr = sLocalStart[dig[r]] + sWaveHist[wave][dig[r]] + rank[r];
-                sKeys[lr] = key[r];
-                sVals[lr] = val[r];
-            }
-        }
-        __syncthreads();
-        for (uint32_t i = t; i < tileN; i += kSortThreads) {
-            const uint64_t k = sKeys[i];
-            const uint32_t d = digitOf(k, shift);
-            const uint32_t dst = sGlobalBase[d] + (i - sLocalStart[d]);
-            keysOut[dst] = k;
-            valsOut[dst] = sVals[i];
-        }
-        __syncthreads();
-        sGlobalBase[t] += sTileCount[t];
-        __syncthreads();
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < tileN; i += kSortThreads) {
+        const uint64_t k = sKeys[i];
+        const uint32_t dd = digitOf(k, shift);
+        const uint32_t dst = sGlobalBase[dd] + (i - sLocalStart[dd]);
+        keysOut[dst] = k;
+        valsOut[dst] = sVals[i];
     }
 }
 
@@ -212,6 +227,11 @@ __global__ __launch_bounds__(kSortThreads) void kCopyPairs(const uint64_t* __res
 
 }  // namespace
 
+size_t sortStatusBytes(uint32_t maxCount) {
+    const size_t tiles = (size_t(maxCount) + kSortTile - 1) / kSortTile;
+    return (tiles > 0 ? tiles : 1) * kRadix * sizeof(uint64_t);
+}
+
 void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,
                      uint32_t maxCount, hipStream_t stream) {
     uint32_t blocks = (maxCount + kSortThreads - 1) / kSortThreads;
@@ -220,24 +240,28 @@ void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint3
     hipLaunchKernelGGL(kCopyPairs, dim3(blocks), dim3(kSortThreads), 0, stream, ks, vs, kd, vd, countDevice, maxCount);
 }
 
-int launchRadixSort(uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB, uint32_t maxCount,
-                    const uint32_t* countDevice, uint32_t keyBits, uint32_t blocks, uint32_t* hist,
-                    uint32_t* digitTotals, hipStream_t stream) {
-    if (blocks == 0) blocks = 1;
-    if (blocks > kMaxSortBlocks) blocks = kMaxSortBlocks;
+int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB,
+                    uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, uint32_t* epoch,
+                    hipStream_t stream) {
     if (maxCount == 0) return 0;
-    const uint32_t passes = (keyBits + kRadixBits - 1) / kRadixBits;
+    uint32_t passes = (keyBits + kRadixBits - 1) / kRadixBits;
+    if (passes > kMaxSortPasses) passes = kMaxSortPasses;
+    const uint32_t tiles = (maxCount + kSortTile - 1) / kSortTile;
+    hipLaunchKernelGGL(kSortInit, dim3(1), dim3(kSortThreads), 0, stream, w.globalHist, w.tickets, passes);
+    uint32_t upBlocks = (maxCount + kSortThreads * 16 - 1) / (kSortThreads * 16);
+    if (upBlocks > 512) upBlocks = 512;
+    hipLaunchKernelGGL(kSortUpfront, dim3(upBlocks), dim3(kSortThreads), 0, stream, keysA, countDevice, maxCount, passes,
+                       w.globalHist);
     uint64_t* kin = keysA;
     uint32_t* vin = valuesA;
     uint64_t* kout = keysB;
     uint32_t* vout = valuesB;
     for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t shift = p * kRadixBits;
-        hipLaunchKernelGGL(kSortHistogram, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice, maxCount,
-                           shift, hist);
-        hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, hist, blocks, digitTotals);
-        hipLaunchKernelGGL(kSortScatter, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout, vout,
-                           countDevice, maxCount, shift, hist, digitTotals);
+        *epoch = (*epoch + 1u) & 0x3FFFFFFFu;
+        if (*epoch == 0) *epoch = 1;  // 0 is the value of never-written status words
+        hipLaunchKernelGGL(kSortOnesweep, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, kout, vout, countDevice,
+                           maxCount, p * kRadixBits, w.globalHist + p * kRadix, w.status, w.tickets + p, *epoch,
+                           w.timeouts);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;

@@ -18,6 +18,8 @@ namespace {
 
 constexpr int kDupThreads = kPreprocessThreads;
 
+constexpr int kCachedRows = 8;  // row spans kept in LDS per gaussian; longer splats walk the rest
+
 __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restrict__ splats,
                                                           const uint32_t* __restrict__ tileCounts,
                                                           const uint32_t* __restrict__ blockOffsets,
@@ -27,18 +29,21 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                                                           gsr_control* controlRW) {
     __shared__ uint32_t sScan[kDupThreads];
     __shared__ uint32_t sWave[kDupThreads / 64];
-    // Per-gaussian coverage, SoA in LDS.
-    __shared__ int sBins[4][kDupThreads];
-    __shared__ float sEl[9][kDupThreads];  // cx cy A B C D t exPad eyPad
-    __shared__ int sRows[2][kDupThreads];
+    // Per gaussian: first row, row count, depth bits, and either the rectangle's x0/width
+    // (one span for every row) or the first kCachedRows row spans (x0 | width << 16) plus
+    // the ellipse for the rows past them.
+    __shared__ int sRow0[kDupThreads];
+    __shared__ int sRows[kDupThreads];
     __shared__ uint32_t sDepth[kDupThreads];
-    __shared__ uint8_t sUseEllipse[kDupThreads];
+    __shared__ uint32_t sSpan[kCachedRows][kDupThreads];
+    __shared__ float sEl[9][kDupThreads];  // cx cy A B C D t exPad eyPad
+    __shared__ int sBinX[2][kDupThreads];
 
     const uint32_t t = threadIdx.x;
     const uint32_t i = blockIdx.x * kDupThreads + t;
     const uint32_t c = i < count ? tileCounts[i] : 0u;
 
-    // Block exclusive scan of the counts.
+    // Block exclusive scan of the counts (the downsweep of the reduce-then-scan).
     const uint32_t lane = t & 63u, wave = t >> 6;
     uint32_t v = c;
 #pragma unroll
@@ -65,27 +70,36 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
         const Coverage cv = makeCoverage(bounds, rect, int(tilesX), int(tilesY), o0.x, o0.y, o1.x, o1.y, o1.z, o1.w);
         int r0, r1;
         coverageRows(cv, &r0, &r1);
-        sBins[0][t] = cv.bins.x0;
-        sBins[1][t] = cv.bins.y0;
-        sBins[2][t] = cv.bins.x1;
-        sBins[3][t] = cv.bins.y1;
-        sEl[0][t] = cv.e.cx;
-        sEl[1][t] = cv.e.cy;
-        sEl[2][t] = cv.e.A;
-        sEl[3][t] = cv.e.B;
-        sEl[4][t] = cv.e.C;
-        sEl[5][t] = cv.e.D;
-        sEl[6][t] = cv.e.t;
-        sEl[7][t] = cv.e.exPad;
-        sEl[8][t] = cv.e.eyPad;
-        sRows[0][t] = r0;
-        sRows[1][t] = r1;
+        sRow0[t] = r0;
+        sRows[t] = r1 - r0;
         sDepth[t] = __float_as_uint(o0.z);
-        sUseEllipse[t] = cv.useEllipse ? 1 : 0;
-        // Count/write agreement (tile_duplicate.comp:375-376), once per gaussian whose
-        // first entry is written.
-        if (bounds != GSR_BOUNDS_RECT && base + excl < entries && coverageCount(cv) != c)
-            atomicAdd(&controlRW->coverage_mismatch, 1u);
+        if (!cv.useEllipse) {
+            sBinX[0][t] = cv.bins.x0;
+            sBinX[1][t] = cv.bins.x1 - cv.bins.x0;
+            sRows[t] = -(r1 - r0) - 1;  // negative: a rectangle
+        } else {
+            sEl[0][t] = cv.e.cx;
+            sEl[1][t] = cv.e.cy;
+            sEl[2][t] = cv.e.A;
+            sEl[3][t] = cv.e.B;
+            sEl[4][t] = cv.e.C;
+            sEl[5][t] = cv.e.D;
+            sEl[6][t] = cv.e.t;
+            sEl[7][t] = cv.e.exPad;
+            sEl[8][t] = cv.e.eyPad;
+            sBinX[0][t] = cv.bins.x0;
+            sBinX[1][t] = cv.bins.x1;
+            // Walk the rows once: cache the first spans, and check the count/write
+            // agreement (tile_duplicate.comp:375-376) from the same walk.
+            uint32_t sum = 0;
+            for (int row = r0; row < r1; ++row) {
+                int x0, x1;
+                coverageRowSpan(cv, row, &x0, &x1);
+                if (row - r0 < kCachedRows) sSpan[row - r0][t] = uint32_t(x0) | (uint32_t(x1 - x0) << 16);
+                sum += uint32_t(x1 - x0);
+            }
+            if (sum != c && base + excl < entries) atomicAdd(&controlRW->coverage_mismatch, 1u);
+        }
     }
     __syncthreads();
 
@@ -101,35 +115,49 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
         }
         const uint32_t r = rl - sScan[lo];
         uint32_t tile = gridTiles;
-        Coverage cv;
-        cv.bins = Rect{sBins[0][lo], sBins[1][lo], sBins[2][lo], sBins[3][lo]};
-        cv.useEllipse = sUseEllipse[lo] != 0;
-        const int r0 = sRows[0][lo], r1 = sRows[1][lo];
-        if (!cv.useEllipse) {
-            const int w = cv.bins.x1 - cv.bins.x0;
-            if (w > 0 && r < uint32_t(w) * uint32_t(r1 - r0))
-                tile = (uint32_t(r0) + r / uint32_t(w)) * tilesX + uint32_t(cv.bins.x0) + r % uint32_t(w);
+        const int rows = sRows[lo];
+        const int r0 = sRow0[lo];
+        if (rows < 0) {  // rectangle
+            const uint32_t nr = uint32_t(-rows - 1);
+            const uint32_t w = uint32_t(sBinX[1][lo]);
+            if (w > 0 && r < w * nr) tile = (uint32_t(r0) + r / w) * tilesX + uint32_t(sBinX[0][lo]) + r % w;
         } else {
-            cv.e.cx = sEl[0][lo];
-            cv.e.cy = sEl[1][lo];
-            cv.e.A = sEl[2][lo];
-            cv.e.B = sEl[3][lo];
-            cv.e.C = sEl[4][lo];
-            cv.e.D = sEl[5][lo];
-            cv.e.t = sEl[6][lo];
-            cv.e.exPad = sEl[7][lo];
-            cv.e.eyPad = sEl[8][lo];
-            cv.e.valid = true;
             uint32_t before = 0;
-            for (int row = r0; row < r1; ++row) {
-                int x0, x1;
-                coverageRowSpan(cv, row, &x0, &x1);
-                const uint32_t w = uint32_t(x1 - x0);
+            const int cached = rows < kCachedRows ? rows : kCachedRows;
+            int row = 0;
+            for (; row < cached; ++row) {
+                const uint32_t sp = sSpan[row][lo];
+                const uint32_t w = sp >> 16;
                 if (r < before + w) {
-                    tile = uint32_t(row) * tilesX + uint32_t(x0) + (r - before);
+                    tile = uint32_t(r0 + row) * tilesX + (sp & 0xFFFFu) + (r - before);
                     break;
                 }
                 before += w;
+            }
+            if (tile == gridTiles && row == kCachedRows) {  // a long splat: walk the remaining rows
+                Coverage cv;
+                cv.bins = Rect{sBinX[0][lo], 0, sBinX[1][lo], 0};
+                cv.useEllipse = true;
+                cv.e.cx = sEl[0][lo];
+                cv.e.cy = sEl[1][lo];
+                cv.e.A = sEl[2][lo];
+                cv.e.B = sEl[3][lo];
+                cv.e.C = sEl[4][lo];
+                cv.e.D = sEl[5][lo];
+                cv.e.t = sEl[6][lo];
+                cv.e.exPad = sEl[7][lo];
+                cv.e.eyPad = sEl[8][lo];
+                cv.e.valid = true;
+                for (int rr = r0 + kCachedRows; rr < r0 + rows; ++rr) {
+                    int x0, x1;
+                    coverageRowSpan(cv, rr, &x0, &x1);
+                    const uint32_t w = uint32_t(x1 - x0);
+                    if (r < before + w) {
+                        tile = uint32_t(rr) * tilesX + uint32_t(x0) + (r - before);
+                        break;
+                    }
+                    before += w;
+                }
             }
         }
         keys[k] = (uint64_t(tile) << 32) | uint64_t(sDepth[lo]);

@@ -90,10 +90,9 @@ class Settings(ctypes.Structure):
         ("max_sh_degree", ctypes.c_int32),
         ("format", ctypes.c_int32),
         ("profile_stages", ctypes.c_int32),
-        ("sort_blocks", ctypes.c_uint32),
         ("render_pixels_per_thread", ctypes.c_int32),
         ("render_exact_alpha", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 4),
+        ("reserved", ctypes.c_uint32 * 5),
     ]
 
 
@@ -109,7 +108,8 @@ class Control(ctypes.Structure):
         ("overflow", ctypes.c_uint32),
         ("ranges_groups", ctypes.c_uint32 * 3),
         ("coverage_mismatch", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32 * 3),
+        ("sort_timeouts", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32 * 2),
     ]
 
     def as_dict(self) -> dict:
@@ -121,6 +121,7 @@ class Control(ctypes.Structure):
             "cull_counts": list(self.cull_counts),
             "overflow": self.overflow,
             "coverage_mismatch": self.coverage_mismatch,
+            "sort_timeouts": self.sort_timeouts,
         }
 
 
@@ -343,15 +344,14 @@ class Renderer:
     """SplatTileRenderer for one scene (init + render + the test readbacks)."""
 
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
-                 max_sh_degree: int = 3, profile_stages: bool = False, sort_blocks: int = 0,
-                 pixels_per_thread: int = 0, exact_alpha: bool = False):
+                 max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
+                 exact_alpha: bool = False):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
         s.max_sh_degree = max_sh_degree
         s.format = TARGETS[target]
         s.profile_stages = 1 if profile_stages else 0
-        s.sort_blocks = sort_blocks
         s.render_pixels_per_thread = pixels_per_thread
         s.render_exact_alpha = 1 if exact_alpha else 0
         h = _vp()

@@ -95,7 +95,8 @@ typedef struct gsr_control {
     uint32_t overflow;         /* M - entries */
     uint32_t ranges_groups[3]; /* grid of the tile-ranges pass (informational) */
     uint32_t coverage_mismatch;/* splats whose written tiles != their count (always 0) */
-    uint32_t pad[3];
+    uint32_t sort_timeouts;    /* radix-sort look-back waits that gave up (always 0) */
+    uint32_t pad[2];
 } gsr_control;
 
 /* Scene description. Host pointers are copied at creation; with device_pointers != 0 the
@@ -120,12 +121,11 @@ typedef struct gsr_settings {
     int32_t max_sh_degree;       /* evaluated bands = min(scene degree, this) */
     gsr_target_format format;    /* output image format */
     int32_t profile_stages;      /* 1: hipEvent pairs around every stage (gsr_read_stage_times) */
-    uint32_t sort_blocks;        /* 0 = default; radix-sort grid (blocks), tuning knob */
     int32_t render_pixels_per_thread; /* tile_render: 1, 2 or 4 pixels per thread (0 = default 1) */
     int32_t render_exact_alpha;  /* 1: alpha = min(0.99, o expf(power)) in the reference's operation order for
                                     every pair; 0 (default): log2-domain alpha with v_exp_f32 and a cutoff
                                     test before the exponential (same image within 1/255) */
-    uint32_t reserved[4];
+    uint32_t reserved[5];
 } gsr_settings;
 
 /* Stage timings of the last profiled frame, ms (the GpuProfiler scopes, profiler.h:1-87). */

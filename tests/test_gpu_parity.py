@@ -51,6 +51,7 @@ def _check_lists(r, o_splats, o_counts, tiles_x, tiles_y, bounds, capacity=None)
     assert ctl.entries == len(hi), (ctl.entries, len(hi))
     assert ctl.overflow == dup - len(hi)
     assert ctl.coverage_mismatch == 0 and mism == 0
+    assert ctl.sort_timeouts == 0
     assert np.array_equal(keys, hi), f"tile keys differ at {np.nonzero(keys != hi)[0][:10]}"
     assert np.array_equal(depth, lo), "depth keys differ"
     assert np.array_equal(vals, ov), f"values differ at {np.nonzero(vals != ov)[0][:10]}"
