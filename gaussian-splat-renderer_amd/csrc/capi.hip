@@ -414,6 +414,8 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown target format %d", int(s.format));
     if (s.budget_mb == 0) s.budget_mb = 256;
     if (s.render_pixels_per_thread == 0) s.render_pixels_per_thread = 1;
+    if (s.sort_algorithm != GSR_SORT_REDUCE_THEN_SCAN && s.sort_algorithm != GSR_SORT_ONESWEEP)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown sort_algorithm %d", int(s.sort_algorithm));
     if (s.render_pixels_per_thread != 1 && s.render_pixels_per_thread != 2 && s.render_pixels_per_thread != 4)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: render_pixels_per_thread must be 1, 2 or 4");
     GSR_HIP(hipSetDevice(ctx->device));
@@ -438,7 +440,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     alloc(r->keysB, size_t(r->capacity) * 8);
     alloc(r->valuesA, size_t(r->capacity) * 4);
     alloc(r->valuesB, size_t(r->capacity) * 4);
-    alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses) * 4);
+    alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses + size_t(kRadix) * kMaxSortBlocks) * 4);
     alloc(r->sortStatus, sortStatusBytes(r->capacity));
     if (e == hipSuccess) e = hipMemset(r->sortStatus.ptr, 0, r->sortStatus.bytes);
     if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
@@ -531,8 +533,9 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
                     r->valuesA.as<uint32_t>(), control, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(3));
-    SortWork work{r->sortSmall.as<uint32_t>(), r->sortSmall.as<uint32_t>() + kMaxSortPasses * kRadix,
-                  r->sortStatus.as<uint64_t>(), &control->sort_timeouts};
+    uint32_t* small = r->sortSmall.as<uint32_t>();
+    SortWork work{r->settings.sort_algorithm, small, small + kMaxSortPasses * kRadix, r->sortStatus.as<uint64_t>(),
+                  &control->sort_timeouts, small + kMaxSortPasses * kRadix + kMaxSortPasses};
     const int which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(),
                                       r->keysB.as<uint64_t>(), r->valuesB.as<uint32_t>(), r->capacity,
                                       &control->entries, keyBits, &r->sortEpoch, stream);
@@ -614,14 +617,17 @@ gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n) {
 // --- Stand-alone sort and scan -------------------------------------------------------------------
 
 gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, uint32_t maxCount,
-                          const uint32_t* countDevice, uint32_t keyBits, void* streamArg) {
+                          const uint32_t* countDevice, uint32_t keyBits, gsr_sort_algorithm algorithm,
+                          void* streamArg) {
     if (!ctx || (maxCount && (!keys || !values))) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_sort_pairs: null argument");
     if (keyBits == 0 || keyBits > 64) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_sort_pairs: key_bits %u", keyBits);
+    if (algorithm != GSR_SORT_REDUCE_THEN_SCAN && algorithm != GSR_SORT_ONESWEEP)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_sort_pairs: unknown algorithm %d", int(algorithm));
     if (maxCount == 0) return GSR_OK;
     GSR_HIP(hipSetDevice(ctx->device));
     hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : ctx->stream;
     const size_t n = maxCount;
-    const size_t smallBytes = (kMaxSortPasses * kRadix + kMaxSortPasses + 1) * 4;
+    const size_t smallBytes = (kMaxSortPasses * kRadix + kMaxSortPasses + 1 + size_t(kRadix) * kMaxSortBlocks) * 4;
     const size_t need = n * 8 + n * 4 + smallBytes + 256;
     if (ctx->scratch.bytes < need) {
         GSR_HIP(hipDeviceSynchronize());
@@ -637,8 +643,9 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
     uint64_t* keysB = reinterpret_cast<uint64_t*>(base);
     uint32_t* valuesB = reinterpret_cast<uint32_t*>(base + n * 8);
     uint32_t* small = reinterpret_cast<uint32_t*>(base + n * 12);
-    SortWork work{small, small + kMaxSortPasses * kRadix, ctx->sortStatus.as<uint64_t>(),
-                  small + kMaxSortPasses * kRadix + kMaxSortPasses};
+    SortWork work{int32_t(algorithm), small, small + kMaxSortPasses * kRadix, ctx->sortStatus.as<uint64_t>(),
+                  small + kMaxSortPasses * kRadix + kMaxSortPasses,
+                  small + kMaxSortPasses * kRadix + kMaxSortPasses + 1};
     const int which = launchRadixSort(work, keys, values, keysB, valuesB, maxCount, countDevice, keyBits,
                                       &ctx->sortEpoch, stream);
     GSR_HIP(hipGetLastError());

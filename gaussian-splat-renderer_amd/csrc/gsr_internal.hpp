@@ -62,14 +62,20 @@ constexpr int kSortKeysPerLane = 8;                               // rounds of 6
 constexpr int kSortTile = kSortThreads * kSortKeysPerLane;        // 2048 keys per tile
 constexpr uint32_t kMaxSortPasses = 8;                            // 64-bit keys
 
-// Device work areas of one sort: global digit histograms (passes x 256), the per-pass
-// tile tickets, the look-back status words (tiles x 256, never cleared: epoch-tagged) and
-// a timeout counter (0 unless a look-back wait gave up).
+// Device work areas of one sort. Reduce-then-scan (3 kernels per digit): per-block digit
+// histograms and digit totals. One-sweep (1 kernel per digit after one histogram pass):
+// global digit histograms (passes x 256), per-pass tile tickets, look-back status words
+// (tiles x 256, never cleared: epoch-tagged) and a timeout counter (0 unless a look-back
+// wait gave up).
+constexpr uint32_t kMaxSortBlocks = 1024;                         // reduce-then-scan grid
+enum SortAlgorithm : int32_t { kSortAlgReduceThenScan = 0, kSortAlgOnesweep = 1 };
 struct SortWork {
-    uint32_t* globalHist;
-    uint32_t* tickets;
-    uint64_t* status;
-    uint32_t* timeouts;
+    int32_t algorithm;
+    uint32_t* globalHist;  // one-sweep: passes x 256 digit counts; reduce-then-scan: 256 digit totals
+    uint32_t* tickets;     // one-sweep
+    uint64_t* status;      // one-sweep
+    uint32_t* timeouts;    // one-sweep
+    uint32_t* blockHist;   // reduce-then-scan: 256 x kMaxSortBlocks
 };
 size_t sortStatusBytes(uint32_t maxCount);
 

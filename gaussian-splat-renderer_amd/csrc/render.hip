@@ -17,8 +17,9 @@
 //     any exponential, the rest take one v_exp_f32. EXACT keeps the reference's operation
 //     order and expf for every pair (bitwise the CPU expression, up to expf's last ulp);
 //   - whole-tile early exit: the workgroup stops once all pixels are done (__syncthreads_count);
-//   - XCD-aware tile order: workgroup b runs on XCD b % 8, so b -> tile maps each XCD to a
-//     contiguous band of tiles and neighbouring tiles (which share splats) share an L2.
+//   - XCD-aware tile order: workgroup b runs on XCD b % 8; XCD x gets the tile rows x, x + 8,
+//     ... so horizontally neighbouring tiles (which share splats) share an L2 while every
+//     XCD gets an even share of the dense (image-centre) and sparse rows.
 #include "gsr_internal.hpp"
 
 namespace gsr {
@@ -51,7 +52,11 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     __shared__ float2 sS1[kBatch];
     __shared__ float4 sColor[kBatch];
 
-    const uint32_t tile = (blockIdx.x % kXcds) * tilesPerXcd + blockIdx.x / kXcds;
+    // Row-interleaved XCD mapping: workgroup b runs on XCD b % 8; XCD x renders tile rows
+    // x, x + 8, x + 16, ... in row-major order.
+    const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
+    const uint32_t row = xcd + kXcds * (idx / tilesX);
+    const uint32_t tile = row * tilesX + idx % tilesX;
     if (tile >= tileCount) return;  // uniform
     const uint32_t t = threadIdx.x;
     const uint32_t px = (tile % tilesX) * 16 + (t % 16);
@@ -108,12 +113,28 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
                 float alpha;
                 if (FAST) {
                     // q = log2(o) + log2(e) * power, power = -0.5 (a dx^2 + c dy^2) - b dx dy.
+                    // The exponential, the cutoff and the blend form one guarded block, so the
+                    // (quarter-rate) v_exp_f32 is not speculated for the pairs the cutoff skips.
                     const float p = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, s1.x * dy * dy);
                     const float q = p + s1.y;
-                    if (q < kLog2MinAlpha || p > 0.0f) continue;
-                    const float ex = __builtin_amdgcn_exp2f(q);
-                    alpha = kMaxAlpha < ex ? kMaxAlpha : ex;
-                    if (alpha < kMinAlpha) continue;
+                    if (q >= kLog2MinAlpha && p <= 0.0f) {
+                        const float ex = __builtin_amdgcn_exp2f(q);
+                        const float a = kMaxAlpha < ex ? kMaxAlpha : ex;
+                        if (a >= kMinAlpha) {
+                            const float testT = T[i] * (1.0f - a);
+                            if (testT < kMinTransmittance) {
+                                done[i] = true;
+                            } else {
+                                const float4 col = sColor[j];
+                                const float w = a * T[i];
+                                C0[i] = fmaf(col.x, w, C0[i]);
+                                C1[i] = fmaf(col.y, w, C1[i]);
+                                C2[i] = fmaf(col.z, w, C2[i]);
+                                T[i] = testT;
+                            }
+                        }
+                    }
+                    continue;
                 } else {
                     const float power = -0.5f * (s0.z * dx * dx + s1.x * dy * dy) - s0.w * dx * dy;
                     if (power > 0.0f) continue;
@@ -127,16 +148,9 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
                     continue;
                 }
                 const float4 col = sColor[j];
-                if (FAST) {
-                    const float w = alpha * T[i];
-                    C0[i] = fmaf(col.x, w, C0[i]);
-                    C1[i] = fmaf(col.y, w, C1[i]);
-                    C2[i] = fmaf(col.z, w, C2[i]);
-                } else {
-                    C0[i] += col.x * alpha * T[i];
-                    C1[i] += col.y * alpha * T[i];
-                    C2[i] += col.z * alpha * T[i];
-                }
+                C0[i] += col.x * alpha * T[i];
+                C1[i] += col.y * alpha * T[i];
+                C2[i] += col.z * alpha * T[i];
                 T[i] = testT;
             }
             bool any = false;
@@ -164,7 +178,8 @@ template <int FORMAT, bool FAST, int PPT>
 void launchOne(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
                uint32_t gaussianCount, void* out, hipStream_t stream) {
     const uint32_t tileCount = cam.tilesX * cam.tilesY;
-    const uint32_t perXcd = (tileCount + kXcds - 1) / kXcds;
+    const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
+    const uint32_t perXcd = rowsPerXcd * cam.tilesX;
     hipLaunchKernelGGL((kRender<FORMAT, FAST, PPT>), dim3(perXcd * kXcds), dim3(256 / PPT), 0, stream, splats, values,
                        ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, gaussianCount, out);
 }

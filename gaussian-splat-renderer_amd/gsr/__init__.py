@@ -35,6 +35,7 @@ _lib = ctypes.CDLL(LIB_PATH)
 SCENE_F32_SOA, SCENE_F16 = 0, 1
 BOUNDS = {"rect": 0, "tight": 1, "cutoff-only": 2, "cutoff": 2}
 TARGETS = {"rgba8": 0, "rgba32f": 1}
+SORT_ALGORITHMS = {"reduce-then-scan": 0, "onesweep": 1}
 STAGES = ("preprocess", "scan", "duplicate", "sort", "ranges", "render", "total")
 CULL_VISIBLE, CULL_NEAR, CULL_SINGULAR, CULL_OFFSCREEN, CULL_LOW_OPACITY = 0, 1, 2, 3, 4
 
@@ -92,7 +93,8 @@ class Settings(ctypes.Structure):
         ("profile_stages", ctypes.c_int32),
         ("render_pixels_per_thread", ctypes.c_int32),
         ("render_exact_alpha", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 5),
+        ("sort_algorithm", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 4),
     ]
 
 
@@ -176,7 +178,7 @@ _read_control = _sig("gsr_read_control", _i32, [_vp, ctypes.POINTER(Control)])
 _read_lists = _sig("gsr_read_lists", _i32, [_vp, _vp, _vp, _vp, _vp, _u32])
 _read_splats = _sig("gsr_read_splats", _i32, [_vp, _vp, _vp])
 _read_stage_times = _sig("gsr_read_stage_times", _i32, [_vp, _vp, _u32])
-_sort_pairs = _sig("gsr_sort_pairs", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _vp])
+_sort_pairs = _sig("gsr_sort_pairs", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _i32, _vp])
 _exclusive_scan = _sig("gsr_exclusive_scan", _i32, [_vp, _vp, _vp, _u32, _vp, _vp])
 
 # Every symbol include/gsr.h declares (checked by the CPU tests).
@@ -345,7 +347,7 @@ class Renderer:
 
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
                  max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
-                 exact_alpha: bool = False):
+                 exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan"):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -354,6 +356,7 @@ class Renderer:
         s.profile_stages = 1 if profile_stages else 0
         s.render_pixels_per_thread = pixels_per_thread
         s.render_exact_alpha = 1 if exact_alpha else 0
+        s.sort_algorithm = SORT_ALGORITHMS[sort_algorithm]
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
@@ -440,7 +443,8 @@ class Renderer:
             pass
 
 
-def sort_pairs(ctx: Context, keys: np.ndarray, values: np.ndarray, key_bits: int = 64, count: Optional[int] = None):
+def sort_pairs(ctx: Context, keys: np.ndarray, values: np.ndarray, key_bits: int = 64, count: Optional[int] = None,
+               algorithm: str = "reduce-then-scan"):
     """GpuRadixSort on host arrays: returns (sorted keys, values) by the low key_bits bits, stable."""
     keys = np.ascontiguousarray(keys, np.uint64)
     values = np.ascontiguousarray(values, np.uint32)
@@ -455,7 +459,8 @@ def sort_pairs(ctx: Context, keys: np.ndarray, values: np.ndarray, key_bits: int
     if count is not None:
         cb.write(np.array([count], np.uint32))
         cnt_ptr = cb.ptr
-    _check(_sort_pairs(ctx.handle, kb.ptr, vb.ptr, n, cnt_ptr, key_bits, None), "gsr_sort_pairs")
+    _check(_sort_pairs(ctx.handle, kb.ptr, vb.ptr, n, cnt_ptr, key_bits, SORT_ALGORITHMS[algorithm], None),
+           "gsr_sort_pairs")
     ok, ov = np.empty_like(keys), np.empty_like(values)
     if n:
         kb.read(ok)

@@ -125,8 +125,15 @@ typedef struct gsr_settings {
     int32_t render_exact_alpha;  /* 1: alpha = min(0.99, o expf(power)) in the reference's operation order for
                                     every pair; 0 (default): log2-domain alpha with v_exp_f32 and a cutoff
                                     test before the exponential (same image within 1/255) */
-    uint32_t reserved[5];
+    int32_t sort_algorithm;      /* GSR_SORT_*: radix-sort pass structure (same result, bit for bit) */
+    uint32_t reserved[4];
 } gsr_settings;
+
+/* Radix-sort pass structure. */
+typedef enum gsr_sort_algorithm {
+    GSR_SORT_REDUCE_THEN_SCAN = 0, /* per digit: block histograms, scan, ballot-ranked scatter (default) */
+    GSR_SORT_ONESWEEP = 1          /* one histogram pass, then one scatter per digit with decoupled look-back */
+} gsr_sort_algorithm;
 
 /* Stage timings of the last profiled frame, ms (the GpuProfiler scopes, profiler.h:1-87). */
 typedef enum gsr_stage {
@@ -201,7 +208,7 @@ gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n);
  * pointers. `count_device` (may be null) holds the key count on the device, clamped to
  * max_count; null = max_count keys. Asynchronous on `stream`. */
 gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, uint32_t max_count,
-                          const uint32_t* count_device, uint32_t key_bits, void* stream);
+                          const uint32_t* count_device, uint32_t key_bits, gsr_sort_algorithm algorithm, void* stream);
 
 /* GpuScan on its own (gpu_scan.h:25-76): exclusive prefix sum of u32 values, the total in
  * *total_device. Asynchronous on `stream`. */

@@ -183,6 +183,28 @@ def test_reference_test_scene_ties_and_overflow(gsr_ctx):
     scene.close()
 
 
+def test_onesweep_lists_equal_reduce_then_scan(gsr_ctx):
+    """Both sort pass structures produce the same list, bit for bit, on the 100k workload."""
+    import gsr
+    from gsr import camera, scenes
+
+    log("case onesweep vs reduce-then-scan, 100k-720p")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cam = _camera(gsr, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720)
+    out = []
+    for alg in ("reduce-then-scan", "onesweep"):
+        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", sort_algorithm=alg)
+        img = r.render_to_host(cam).copy()
+        out.append((img, r.read_back()))
+        assert out[-1][1][4].sort_timeouts == 0
+        r.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1][:4], out[1][1][:4]):
+        assert np.array_equal(a, b)
+    scene.close()
+
+
 def test_1_2m_1080p_orbit_frame(gsr_ctx):
     """The headline workload at full size: 1.2M gaussians, 1920x1080, orbit frame 40 (48 deg)."""
     import oracle
@@ -234,8 +256,9 @@ def test_deterministic_and_empty(gsr_ctx):
 SORT_SIZES = [0, 1, 31, 32, 33, 255, 256, 257, 2047, 2048, 2049, 4095, 4097, 65537, 1_000_003]
 
 
+@pytest.mark.parametrize("algorithm", ["reduce-then-scan", "onesweep"])
 @pytest.mark.parametrize("n", SORT_SIZES)
-def test_sort_pairs_stable(gsr_ctx, n):
+def test_sort_pairs_stable(gsr_ctx, n, algorithm):
     """test_gpu_sort.cpp:198-258: sizes around block boundaries, adversarial key patterns."""
     import gsr
 
@@ -252,20 +275,21 @@ def test_sort_pairs_stable(gsr_ctx, n):
         if n > 100_000 and name not in ("random44", "4-values"):
             continue
         vals = np.arange(n, dtype=np.uint32)
-        ok, ov = gsr.sort_pairs(gsr_ctx, keys, vals, key_bits=44)
+        ok, ov = gsr.sort_pairs(gsr_ctx, keys, vals, key_bits=44, algorithm=algorithm)
         order = np.argsort(keys, kind="stable")
         assert np.array_equal(ok, keys[order]), f"{name}: keys"
         assert np.array_equal(ov, vals[order]), f"{name}: values (stability)"
 
 
-def test_sort_pairs_count_below_capacity(gsr_ctx):
+@pytest.mark.parametrize("algorithm", ["reduce-then-scan", "onesweep"])
+def test_sort_pairs_count_below_capacity(gsr_ctx, algorithm):
     """A device-side count < the uploaded keys: only the first `count` are sorted, the tail is untouched."""
     import gsr
 
     n, count = 10_000, 6_001
     keys = np.random.default_rng(7).integers(0, 1 << 40, n, dtype=np.uint64)
     vals = np.arange(n, dtype=np.uint32)
-    ok, ov = gsr.sort_pairs(gsr_ctx, keys, vals, key_bits=40, count=count)
+    ok, ov = gsr.sort_pairs(gsr_ctx, keys, vals, key_bits=40, count=count, algorithm=algorithm)
     order = np.argsort(keys[:count], kind="stable")
     assert np.array_equal(ok[:count], keys[:count][order])
     assert np.array_equal(ov[:count], vals[:count][order])
