@@ -547,7 +547,8 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(5));
     launchRender(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
-                 r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0, outImage, stream);
+                 r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0, r->settings.render_loop,
+                 outImage, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
     r->lastCam = *cam;

@@ -98,8 +98,8 @@ void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint3
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
                   uint2* ranges, hipStream_t stream);
 void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-                  uint32_t gaussianCount, gsr_target_format format, int pixelsPerThread, bool exactAlpha, void* out,
-                  hipStream_t stream);
+                  uint32_t gaussianCount, gsr_target_format format, int pixelsPerThread, bool exactAlpha, int loop,
+                  void* out, hipStream_t stream);
 
 // Stand-alone scan (GpuScan): reduce, block-sum scan, downsweep.
 constexpr int kScanThreads = 256;

@@ -37,12 +37,12 @@ __device__ __forceinline__ uint32_t quantize8(float v) {
     return uint32_t(roundf(v * 255.0f));
 }
 
-template <int FORMAT, bool FAST, int PPT>
+template <int FORMAT, bool FAST, int PPT, bool PRED>
 __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ splats,
                                                      const uint32_t* __restrict__ values,
                                                      const uint2* __restrict__ ranges, uint32_t width,
                                                      uint32_t height, uint32_t tilesX, uint32_t tileCount,
-                                                     uint32_t tilesPerXcd, uint32_t gaussianCount,
+                                                     uint32_t gaussianCount,
                                                      void* __restrict__ out) {
     constexpr int kThreads = 256 / PPT;
     constexpr int kRowStep = 16 / PPT;
@@ -51,6 +51,7 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     __shared__ float4 sS0[kBatch];
     __shared__ float2 sS1[kBatch];
     __shared__ float4 sColor[kBatch];
+    static_assert(!PRED || (FAST && PPT == 1), "the predicated loop is the fast one-pixel form");
 
     // Row-interleaved XCD mapping: workgroup b runs on XCD b % 8; XCD x renders tile rows
     // x, x + 8, x + 16, ... in row-major order.
@@ -84,6 +85,11 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
         for (int f = 0; f < kBatch / kThreads; ++f) {
             const uint32_t slot = t + f * kThreads;
             const uint32_t e = batch + slot;
+            if (PRED && e >= range.y && slot < ((range.y - batch + 3u) & ~3u)) {
+                // Padding up to a multiple of 4 for the unrolled loop: q = -inf never passes.
+                sS0[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                sS1[slot] = make_float2(0.0f, -__builtin_inff());
+            }
             if (e < range.y) {
                 uint32_t g = values[e];
                 g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
@@ -102,6 +108,46 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
         }
         __syncthreads();
         const uint32_t n = range.y - batch < uint32_t(kBatch) ? range.y - batch : uint32_t(kBatch);
+        if (PRED) {
+            // Predicated form (FAST, one pixel per thread): the wave walks the batch in step,
+            // four splats per iteration. A lane's state is folded into its cutoff `thr`
+            // (log2(1/255) while alive, +inf once done or outside the image), so every branch is
+            // one vector compare: the exponential and the blend run only when some lane of the
+            // wave passes its cutoff, and the blend itself is branch-free (selects).
+            float& Tl = T[0];
+            float& c0 = C0[0];
+            float& c1 = C1[0];
+            float& c2 = C2[0];
+            const float py = pyf[0];
+            float thr = done[0] ? __builtin_inff() : kLog2MinAlpha;
+            const uint32_t n4 = (n + 3u) & ~3u;
+            for (uint32_t j = 0; j < n4; j += 4) {
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const float4 s0 = sS0[j + u];
+                    const float2 s1 = sS1[j + u];
+                    const float dx = s0.x - pxf;
+                    const float dy = s0.y - py;
+                    const float q = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, fmaf(s1.x * dy, dy, s1.y));
+                    if (__builtin_amdgcn_ballot_w64(q >= thr) == 0) continue;
+                    const float a = fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q));
+                    const float aEff = q >= thr ? a : 0.0f;
+                    const float testT = Tl * (1.0f - aEff);
+                    const bool term = testT < kMinTransmittance;
+                    const float w = term ? 0.0f : aEff * Tl;
+                    const float4 col = sColor[j + u];
+                    c0 = fmaf(col.x, w, c0);
+                    c1 = fmaf(col.y, w, c1);
+                    c2 = fmaf(col.z, w, c2);
+                    Tl = term ? Tl : testT;
+                    thr = term ? __builtin_inff() : thr;
+                }
+                if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
+            }
+            done[0] = !(thr < __builtin_inff());
+            alive = !done[0];
+            continue;
+        }
         for (uint32_t j = 0; alive && j < n; ++j) {
             const float4 s0 = sS0[j];
             const float2 s1 = sS1[j];
@@ -174,37 +220,46 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     }
 }
 
-template <int FORMAT, bool FAST, int PPT>
+template <int FORMAT, bool FAST, int PPT, bool PRED>
 void launchOne(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
                uint32_t gaussianCount, void* out, hipStream_t stream) {
     const uint32_t tileCount = cam.tilesX * cam.tilesY;
     const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
     const uint32_t perXcd = rowsPerXcd * cam.tilesX;
-    hipLaunchKernelGGL((kRender<FORMAT, FAST, PPT>), dim3(perXcd * kXcds), dim3(256 / PPT), 0, stream, splats, values,
-                       ranges, cam.width, cam.height, cam.tilesX, tileCount, perXcd, gaussianCount, out);
+    hipLaunchKernelGGL((kRender<FORMAT, FAST, PPT, PRED>), dim3(perXcd * kXcds), dim3(256 / PPT), 0, stream, splats,
+                       values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out);
 }
 
-template <int FORMAT, bool FAST>
-void launchPpt(int ppt, const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-               uint32_t gaussianCount, void* out, hipStream_t stream) {
-    if (ppt == 4) launchOne<FORMAT, FAST, 4>(splats, values, ranges, cam, gaussianCount, out, stream);
-    else if (ppt == 2) launchOne<FORMAT, FAST, 2>(splats, values, ranges, cam, gaussianCount, out, stream);
-    else launchOne<FORMAT, FAST, 1>(splats, values, ranges, cam, gaussianCount, out, stream);
+template <int FORMAT>
+void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uint32_t* values, const uint2* ranges,
+                  const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+    if (exact) {
+        if (ppt == 4) launchOne<FORMAT, false, 4, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+        else if (ppt == 2) launchOne<FORMAT, false, 2, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+        else launchOne<FORMAT, false, 1, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+    } else if (ppt == 4) {
+        launchOne<FORMAT, true, 4, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+    } else if (ppt == 2) {
+        launchOne<FORMAT, true, 2, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+    } else if (loop == GSR_RENDER_LOOP_DIVERGENT) {
+        launchOne<FORMAT, true, 1, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+    } else {
+        launchOne<FORMAT, true, 1, true>(splats, values, ranges, cam, gaussianCount, out, stream);
+    }
 }
 
 }  // namespace
 
 void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-                  uint32_t gaussianCount, gsr_target_format format, int pixelsPerThread, bool exactAlpha, void* out,
-                  hipStream_t stream) {
+                  uint32_t gaussianCount, gsr_target_format format, int pixelsPerThread, bool exactAlpha, int loop,
+                  void* out, hipStream_t stream) {
     if (cam.tilesX * cam.tilesY == 0) return;
-    if (format == GSR_TARGET_RGBA8) {
-        if (exactAlpha) launchPpt<GSR_TARGET_RGBA8, false>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
-        else launchPpt<GSR_TARGET_RGBA8, true>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
-    } else {
-        if (exactAlpha) launchPpt<GSR_TARGET_RGBA32F, false>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
-        else launchPpt<GSR_TARGET_RGBA32F, true>(pixelsPerThread, splats, values, ranges, cam, gaussianCount, out, stream);
-    }
+    if (format == GSR_TARGET_RGBA8)
+        launchFormat<GSR_TARGET_RGBA8>(pixelsPerThread, exactAlpha, loop, splats, values, ranges, cam, gaussianCount,
+                                       out, stream);
+    else
+        launchFormat<GSR_TARGET_RGBA32F>(pixelsPerThread, exactAlpha, loop, splats, values, ranges, cam,
+                                         gaussianCount, out, stream);
 }
 
 }  // namespace gsr

@@ -126,8 +126,15 @@ typedef struct gsr_settings {
                                     every pair; 0 (default): log2-domain alpha with v_exp_f32 and a cutoff
                                     test before the exponential (same image within 1/255) */
     int32_t sort_algorithm;      /* GSR_SORT_*: radix-sort pass structure (same result, bit for bit) */
-    uint32_t reserved[4];
+    int32_t render_loop;         /* GSR_RENDER_LOOP_*: loop form of the fast one-pixel render */
+    uint32_t reserved[3];
 } gsr_settings;
+
+/* Loop form of the fast one-pixel tile render (same blend, same image within 1/255). */
+typedef enum gsr_render_loop {
+    GSR_RENDER_LOOP_PREDICATED = 0, /* wave-uniform, 4 splats per iteration, branch-free blend (default) */
+    GSR_RENDER_LOOP_DIVERGENT = 1   /* per-lane exits, one splat per iteration */
+} gsr_render_loop;
 
 /* Radix-sort pass structure. */
 typedef enum gsr_sort_algorithm {

@@ -60,14 +60,14 @@ def _check_lists(r, o_splats, o_counts, tiles_x, tiles_y, bounds, capacity=None)
 
 
 def _render_case(gsr_ctx, sc, view, W, H, bounds, fmt="f32", target="rgba32f", budget_mb=256, fov=60.0,
-                 exact_alpha=False, ppt=0):
+                 exact_alpha=False, ppt=0, loop="predicated"):
     import gsr
     import oracle
 
     cam = _camera(gsr, view, W, H, fov)
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
     r = gsr.Renderer(gsr_ctx, scene, bounds=bounds, target=target, budget_mb=budget_mb, exact_alpha=exact_alpha,
-                     pixels_per_thread=ppt)
+                     pixels_per_thread=ppt, render_loop=loop)
     img = r.render_to_host(cam)
     if fmt == "f16":  # the oracle renders exactly what the GPU decodes (gsplat_ref.cpp:182-190)
         from gsr.scenes import SyntheticScene
@@ -105,8 +105,10 @@ def test_100k_720p_splats_lists_image(gsr_ctx, bounds):
     scene.close()
 
 
-@pytest.mark.parametrize("exact_alpha,ppt", [(False, 1), (True, 1), (False, 2), (True, 4)])
-def test_rgba8_within_one_level(gsr_ctx, exact_alpha, ppt):
+@pytest.mark.parametrize("exact_alpha,ppt,loop", [(False, 1, "predicated"), (False, 1, "divergent"),
+                                                  (True, 1, "predicated"), (False, 2, "predicated"),
+                                                  (True, 4, "predicated")])
+def test_rgba8_within_one_level(gsr_ctx, exact_alpha, ppt, loop):
     """Every render variant: RGBA8 within one level of the quantised oracle image."""
     import oracle
     from gsr import camera, scenes
@@ -114,7 +116,8 @@ def test_rgba8_within_one_level(gsr_ctx, exact_alpha, ppt):
     log(f"case rgba8 100k-720p tight exact_alpha={exact_alpha} ppt={ppt}")
     sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
     r, scene, cam, img, sc, ocam, ob, *_ = _render_case(gsr_ctx, sc, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720,
-                                                        "tight", target="rgba8", exact_alpha=exact_alpha, ppt=ppt)
+                                                        "tight", target="rgba8", exact_alpha=exact_alpha, ppt=ppt,
+                                                        loop=loop)
     ref, _ = oracle.render(sc, ocam, bounds=ob)
     q = oracle.quantize8(ref)
     diff = np.abs(img[..., :3].astype(np.int32) - q.astype(np.int32))
