@@ -2,14 +2,15 @@
 
 Workload (the headline configuration): 1.2M synthetic gaussians (means ~ N(0, 3^2 I), SH
 degree 3, seed 0), 1920x1080, the 300-frame 360-degree orbit at radius 10 and elevation
-15 degrees. A step is one frame: preprocess, scan, duplicate, 64-bit radix sort, ranges and
-render to RGBA8, with the scene already resident in HBM. W warm-up frames are rendered
-first (the reference excludes frames/10 + 5 = 35), then K frames are timed with HIP events
-on the render stream.
+15 degrees. A step is one frame: preprocess, scan, tile-list build and sort (binning mode:
+compaction, 32-bit depth sort, duplicate, 32-bit tile sort; global mode: duplicate, 64-bit
+(tile, depth) sort), ranges and render to RGBA8, with the scene already resident in HBM.
+W warm-up frames are rendered first (the reference excludes frames/10 + 5 = 35), then K
+frames are timed with HIP events on the render stream.
 
 Prints ONE JSON line: ms/frame (value), per-stage ms, sorted keys/s, the roofline of the
-dominant kernel, the fp16-scene frame time, the image quality of one frame against the CPU
-oracle, and the oracle's own CPU time on the same frame (cpu_baseline).
+dominant kernel, the fp16-scene and other-sort-mode frame times, the image quality of one
+frame against the CPU oracle, and the oracle's own CPU time on the same frame (cpu_baseline).
 """
 import argparse
 import ctypes
@@ -33,6 +34,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=35)
     ap.add_argument("--count", type=int, default=1_200_000)
     ap.add_argument("--bounds", default="tight")
+    ap.add_argument("--sort-mode", default="binning", choices=("binning", "global"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     return ap.parse_args()
@@ -100,48 +102,68 @@ def main():
         assert timer.hip.hipStreamDestroy(hip_stream) == 0
         return ms
 
-    result = {}
-    for fmt in ("f32", "f16"):
+    def profile_stages(scene, mode):
+        """Per-stage ms and list counts: a second renderer with events between the stages."""
+        rp = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", profile_stages=True, sort_mode=mode)
+        stage_sum = {k: 0.0 for k in gsr.STAGES}
+        entries = vis = 0
+        n_prof = min(args.steps, 60)
+        for i in range(n_prof):
+            cam = cams[(args.warmup + i * (ORBIT_FRAMES // n_prof)) % ORBIT_FRAMES]
+            rp.render(cam, out.ptr)
+            st = rp.stage_times()
+            ctl = rp.read_control()
+            for k in stage_sum:
+                stage_sum[k] += st[k]
+            entries += ctl.entries
+            vis += ctl.visible
+        rp.close()
+        return {k: v / n_prof for k, v in stage_sum.items()}, entries / n_prof, vis / n_prof
+
+    other = "global" if args.sort_mode == "binning" else "binning"
+    result, modes = {}, {}
+    for fmt, mode in (("f32", args.sort_mode), ("f16", args.sort_mode), ("f32", other)):
         scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
-        r = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8")
+        r = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", sort_mode=mode)
         ms = timed_orbit(r, 0, args.steps)
-        result[fmt] = ms / args.steps
-        print(f"# {fmt}: {ms / args.steps:.4f} ms/frame over {args.steps} frames", file=sys.stderr, flush=True)
+        result[(fmt, mode)] = ms / args.steps
+        print(f"# {fmt} {mode}: {ms / args.steps:.4f} ms/frame over {args.steps} frames", file=sys.stderr, flush=True)
         if fmt == "f32":
-            # Per-stage times, counts and keys/s: a second pass with events between stages.
-            rp = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", profile_stages=True)
-            stage_sum = {k: 0.0 for k in gsr.STAGES}
-            entries = dup = vis = 0
-            n_prof = min(args.steps, 60)
-            for i in range(n_prof):
-                cam = cams[(args.warmup + i * (ORBIT_FRAMES // n_prof)) % ORBIT_FRAMES]
-                rp.render(cam, out.ptr)
-                st = rp.stage_times()
-                ctl = rp.read_control()
-                for k in stage_sum:
-                    stage_sum[k] += st[k]
-                entries += ctl.entries
-                dup += ctl.duplicates
-                vis += ctl.visible
-            stages = {k: v / n_prof for k, v in stage_sum.items()}
-            mean_entries = entries / n_prof
-            tiles = cams[0].tiles
-            key_bits = 32 + int(tiles).bit_length()
-            passes = (key_bits + 7) // 8
-            # Quality: one frame (the first timed one) vs the oracle, RGBA8.
-            qcam = cams[args.warmup % ORBIT_FRAMES]
-            r.render(qcam, out.ptr)
-            img = out.read(np.empty((H, W, 4), np.uint8))
-            rp.close()
+            st, ent, vis = profile_stages(scene, mode)
+            modes[mode] = {"ms_per_frame": round(ms / args.steps, 4), "stages_ms": {k: round(v, 4) for k, v in st.items()},
+                           "mean_entries": int(ent), "mean_visible": int(vis)}
+            if mode == args.sort_mode:
+                stages, mean_entries, mean_visible = st, ent, vis
+                # Quality: one frame (the first timed one) vs the oracle, RGBA8.
+                qcam = cams[args.warmup % ORBIT_FRAMES]
+                r.render(qcam, out.ptr)
+                img = out.read(np.empty((H, W, 4), np.uint8))
         r.close()
         scene.close()
 
-    # Roofline of the dominant stage's kernels.
+    # Roofline of the dominant HBM-bound stage: algorithmic bytes / stage time.
     n = args.count
-    sort_bytes = passes * 32.0 * mean_entries  # per pass: key read (histogram) + 12 B read + 12 B write
+    tiles = cams[0].tiles
+    tile_bits = int(tiles).bit_length()
+    if args.sort_mode == "binning":
+        tile_passes = (tile_bits + 7) // 8
+        # per pass: key read (histogram) + key/value read + key/value write, 4 + 8 + 8 B
+        sort_bytes = tile_passes * 20.0 * mean_entries
+        depth_sort_bytes = 4 * 20.0 * mean_visible
+        dup_bytes = mean_entries * 8.0 + mean_visible * (48 + 4 + 4)  # list written; splat, order, count read
+        stage_bytes = {"depth_sort": depth_sort_bytes}
+        sorted_keys = mean_entries * tile_passes + mean_visible * 4
+        passes = {"depth": 4, "tile": tile_passes}
+    else:
+        key_passes = (32 + tile_bits + 7) // 8
+        sort_bytes = key_passes * 32.0 * mean_entries  # 8 + 12 + 12 B per pass
+        dup_bytes = mean_entries * 12.0 + n * 4.0
+        stage_bytes = {}
+        sorted_keys = mean_entries * key_passes
+        passes = {"key": key_passes}
     pre_bytes = n * (16 + 32 + 192 + 48 + 4)  # upper bound: every gaussian's record read, splat + count written
-    stage_bytes = {"preprocess": pre_bytes, "sort": sort_bytes, "duplicate": mean_entries * 12.0 + n * 4.0,
-                   "ranges": mean_entries * 8.0 + tiles * 8.0}
+    stage_bytes.update({"preprocess": pre_bytes, "sort": sort_bytes, "duplicate": dup_bytes,
+                        "ranges": mean_entries * (4.0 if args.sort_mode == "binning" else 8.0) + tiles * 8.0})
     hbm_stages = {k: stages[k] for k in stage_bytes}
     dom = max(hbm_stages, key=hbm_stages.get)
     achieved = stage_bytes[dom] / (stages[dom] * 1e-3) / 1e9
@@ -177,12 +199,12 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": round(result["f32"], 4),
+        "value": round(result[("f32", args.sort_mode)], 4),
         "unit": "ms/frame",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(result["f32"], 4),
+        "ms_per_step": round(result[("f32", args.sort_mode)], 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -190,12 +212,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": "1.2M-1080p-orbit", "gaussians": n, "width": W, "height": H,
                    "orbit": "300 frames, radius 10, elevation 15 deg", "bounds": args.bounds, "scene_format": "f32",
-                   "target": "rgba8"},
-        "fp16_scene_ms_per_frame": round(result["f16"], 4),
+                   "target": "rgba8", "sort_mode": args.sort_mode},
+        "fp16_scene_ms_per_frame": round(result[("f16", args.sort_mode)], 4),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "mean_entries": int(mean_entries),
+        "mean_visible": int(mean_visible),
         "sort_passes": passes,
-        "sorted_keys_per_s": round(mean_entries * passes / (stages["sort"] * 1e-3), 1),
+        "sorted_keys_per_s": round(sorted_keys / ((stages["sort"] + stages["depth_sort"]) * 1e-3), 1),
+        "sort_modes": modes,
         "psnr_db_vs_oracle": None if psnr is None else round(psnr, 2),
         "max_diff_levels_vs_oracle": maxdiff,
         "roofline": roofline,

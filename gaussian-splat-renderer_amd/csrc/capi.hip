@@ -112,8 +112,13 @@ struct gsr_renderer {
     int32_t degree = 0;
     // Per gaussian.
     DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control;
-    // The list (ping-pong) and the sort's work areas.
+    // The list (ping-pong; keys are 32-bit tile ids in binning mode, 64-bit (tile, depth) in
+    // global mode) and the sort's work areas.
     DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus;
+    // Binning mode, per gaussian: compacted depth keys / indices (ping-pong), depth-ordered
+    // tile counts and their per-256 sums.
+    DeviceBuffer depthKeysA, depthKeysB, depthValuesA, depthValuesB, sortedCounts, visibleSums;
+    bool binning = false;
     uint32_t sortEpoch = 0;
     // Per tile (sized for the largest camera seen).
     DeviceBuffer ranges;
@@ -388,11 +393,12 @@ gsr_status gsr_camera_from_viewer(const float view[16], const float proj[16], co
 
 // --- Renderer ----------------------------------------------------------------------------------
 
-static uint32_t capacityForBytes(double bytes) {
-    // Per entry: 64-bit key + value, both ping-ponged, + the sort's histograms: the
-    // reference's global-mode 24.5 B/entry (splat_tile_renderer.cpp:117-124), blocks of 2048.
+static uint32_t capacityForBytes(double bytes, bool binning) {
+    // Per entry: key + value (64-bit keys in global mode), both ping-ponged, + the sort's
+    // block histograms (0.5 B): 16.5 or 24.5 B (splat_tile_renderer.cpp:117-124), in blocks
+    // of 2048 entries.
     const double maxEntries = double(1u << 30);
-    double c = bytes / 24.5;
+    double c = bytes / (binning ? 16.5 : 24.5);
     c = std::min(std::max(c, 2048.0), maxEntries);
     const uint32_t e = uint32_t(c);
     return e - e % 2048u;
@@ -416,6 +422,11 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (s.render_pixels_per_thread == 0) s.render_pixels_per_thread = 1;
     if (s.sort_algorithm != GSR_SORT_REDUCE_THEN_SCAN && s.sort_algorithm != GSR_SORT_ONESWEEP)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown sort_algorithm %d", int(s.sort_algorithm));
+    if (s.sort_mode != GSR_SORT_MODE_BINNING && s.sort_mode != GSR_SORT_MODE_GLOBAL)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown sort_mode %d", int(s.sort_mode));
+    if (s.sort_mode == GSR_SORT_MODE_BINNING && s.sort_algorithm == GSR_SORT_ONESWEEP)
+        return fail(GSR_ERROR_INVALID_ARGUMENT,
+                    "gsr_renderer_create: the one-sweep sort takes 64-bit keys (sort_mode GLOBAL)");
     if (s.render_pixels_per_thread != 1 && s.render_pixels_per_thread != 2 && s.render_pixels_per_thread != 4)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: render_pixels_per_thread must be 1, 2 or 4");
     GSR_HIP(hipSetDevice(ctx->device));
@@ -425,7 +436,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     r->scene = scene;
     r->settings = s;
     r->degree = std::min(scene->shDegree, std::max(0, std::min(3, s.max_sh_degree)));
-    r->capacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0);
+    r->binning = s.sort_mode == GSR_SORT_MODE_BINNING;
+    r->capacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0, r->binning);
+    const size_t keyBytes = r->binning ? 4 : 8;
     const size_t n = std::max<size_t>(scene->count, 1);
     const size_t gBlocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     hipError_t e = hipSuccess;
@@ -436,8 +449,13 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     alloc(r->tileCounts, n * 4);
     alloc(r->blockOffsets, gBlocks * 4);
     alloc(r->control, sizeof(gsr_control));
-    alloc(r->keysA, size_t(r->capacity) * 8);
-    alloc(r->keysB, size_t(r->capacity) * 8);
+    alloc(r->keysA, size_t(r->capacity) * keyBytes);
+    alloc(r->keysB, size_t(r->capacity) * keyBytes);
+    if (r->binning) {
+        for (DeviceBuffer* b : {&r->depthKeysA, &r->depthKeysB, &r->depthValuesA, &r->depthValuesB, &r->sortedCounts})
+            alloc(*b, n * 4);
+        alloc(r->visibleSums, gBlocks * 4);
+    }
     alloc(r->valuesA, size_t(r->capacity) * 4);
     alloc(r->valuesB, size_t(r->capacity) * 4);
     alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses + size_t(kRadix) * kMaxSortBlocks) * 4);
@@ -471,7 +489,8 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     if (!r) return 0;
     const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
-                                  &r->sortStatus, &r->ranges};
+                                  &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
+                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     return s;
@@ -517,40 +536,76 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(r->events[i], stream) : hipSuccess; };
     gsr_control* control = r->control.as<gsr_control>();
     const uint32_t gBlocks = (sc->count + kPreprocessThreads - 1) / kPreprocessThreads;
-    const uint32_t keyBits = 32 + bitsFor(tiles);  // tile ids 0..tiles (tiles = the sentinel)
-
-    GSR_HIP(mark(0));
-    launchPreprocess(ref, dc, uint32_t(r->settings.bounds), r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
-                     r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords, stream);
-    GSR_HIP(hipGetLastError());
-    GSR_HIP(mark(1));
-    launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, r->blockOffsets.as<uint32_t>(),
-                     control, stream);
-    GSR_HIP(hipGetLastError());
-    GSR_HIP(mark(2));
-    launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(), control,
-                    sc->count, uint32_t(r->settings.bounds), cam->tiles_x, cam->tiles_y, r->keysA.as<uint64_t>(),
-                    r->valuesA.as<uint32_t>(), control, stream);
-    GSR_HIP(hipGetLastError());
-    GSR_HIP(mark(3));
+    const uint32_t tileBits = bitsFor(tiles);  // tile ids 0..tiles (tiles = the sentinel)
     uint32_t* small = r->sortSmall.as<uint32_t>();
     SortWork work{r->settings.sort_algorithm, small, small + kMaxSortPasses * kRadix, r->sortStatus.as<uint64_t>(),
                   &control->sort_timeouts, small + kMaxSortPasses * kRadix + kMaxSortPasses};
-    const int which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(),
-                                      r->keysB.as<uint64_t>(), r->valuesB.as<uint32_t>(), r->capacity,
-                                      &control->entries, keyBits, &r->sortEpoch, stream);
+    const uint32_t bounds = uint32_t(r->settings.bounds);
+
+    GSR_HIP(mark(0));
+    launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
+                     r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords, stream);
     GSR_HIP(hipGetLastError());
-    GSR_HIP(mark(4));
-    const uint64_t* sortedKeys = which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>();
-    const uint32_t* sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
-    launchRanges(sortedKeys, control, r->capacity, tiles, r->ranges.as<uint2>(), stream);
+    GSR_HIP(mark(1));
+    launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, r->binning,
+                     r->blockOffsets.as<uint32_t>(), control, stream);
     GSR_HIP(hipGetLastError());
-    GSR_HIP(mark(5));
+    int which = 0;
+    const uint32_t* sortedValues = nullptr;
+    if (r->binning) {
+        // Binning (splat_tile_renderer.cpp:680-760): compact -> depth sort -> gather + scan
+        // of the depth-ordered counts -> duplicate (tile keys) -> stable tile sort.
+        launchCompactVisible(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(),
+                             sc->count, r->depthKeysA.as<uint32_t>(), r->depthValuesA.as<uint32_t>(), stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(2));
+        const int dw = launchRadixSort32(work, r->depthKeysA.as<uint32_t>(), r->depthValuesA.as<uint32_t>(),
+                                         r->depthKeysB.as<uint32_t>(), r->depthValuesB.as<uint32_t>(), sc->count,
+                                         &control->visible, 32, stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(3));
+        const uint32_t* order = dw ? r->depthValuesB.as<uint32_t>() : r->depthValuesA.as<uint32_t>();
+        launchGatherCounts(r->tileCounts.as<uint32_t>(), order, control, sc->count, r->sortedCounts.as<uint32_t>(),
+                           r->visibleSums.as<uint32_t>(), stream);
+        GSR_HIP(hipGetLastError());
+        launchScanVisibleBlocks(r->visibleSums.as<uint32_t>(), control, stream);
+        GSR_HIP(hipGetLastError());
+        launchDuplicateBinning(r->splats.as<float4>(), r->sortedCounts.as<uint32_t>(), order,
+                               r->visibleSums.as<uint32_t>(), control, sc->count, bounds, cam->tiles_x, cam->tiles_y,
+                               r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), control, stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(4));
+        which = launchRadixSort32(work, r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint32_t>(),
+                                  r->valuesB.as<uint32_t>(), r->capacity, &control->entries, tileBits, stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(5));
+        sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
+        launchRanges32(which ? r->keysB.as<uint32_t>() : r->keysA.as<uint32_t>(), control, r->capacity, tiles,
+                       r->ranges.as<uint2>(), stream);
+    } else {
+        GSR_HIP(mark(2));
+        GSR_HIP(mark(3));
+        launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(), control,
+                        sc->count, bounds, cam->tiles_x, cam->tiles_y, r->keysA.as<uint64_t>(),
+                        r->valuesA.as<uint32_t>(), control, stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(4));
+        which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),
+                                r->valuesB.as<uint32_t>(), r->capacity, &control->entries, 32 + tileBits,
+                                &r->sortEpoch, stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(5));
+        sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
+        launchRanges(which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>(), control, r->capacity, tiles,
+                     r->ranges.as<uint2>(), stream);
+    }
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(mark(6));
     launchRender(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
                  r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0, r->settings.render_loop,
                  outImage, stream);
     GSR_HIP(hipGetLastError());
-    GSR_HIP(mark(6));
+    GSR_HIP(mark(7));
     r->lastCam = *cam;
     r->haveFrame = true;
     r->sortedIn = which;
@@ -575,17 +630,32 @@ gsr_status gsr_read_lists(gsr_renderer* r, uint32_t* tileKeys, uint32_t* depthKe
     gsr_status st = gsr_read_control(r, &c);
     if (st != GSR_OK) return st;
     const uint32_t n = std::min(c.entries, maxEntries);
-    if (n > 0 && (tileKeys || depthKeys)) {
+    const void* keySrc = r->sortedIn ? r->keysB.ptr : r->keysA.ptr;
+    const void* valueSrc = r->sortedIn ? r->valuesB.ptr : r->valuesA.ptr;
+    std::vector<uint32_t> vals;
+    if (n > 0 && (values || (depthKeys && r->binning))) {
+        vals.resize(n);
+        GSR_HIP(hipMemcpy(vals.data(), valueSrc, size_t(n) * 4, hipMemcpyDeviceToHost));
+        if (values) std::memcpy(values, vals.data(), size_t(n) * 4);
+    }
+    if (n > 0 && r->binning) {
+        if (tileKeys) GSR_HIP(hipMemcpy(tileKeys, keySrc, size_t(n) * 4, hipMemcpyDeviceToHost));
+        if (depthKeys) {  // binning keys carry no depth: it is the listed splat's depth bits
+            std::vector<float4> sp(size_t(r->scene->count) * 3);
+            GSR_HIP(hipMemcpy(sp.data(), r->splats.ptr, sp.size() * sizeof(float4), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < n; ++i) {
+                const float d = vals[i] < r->scene->count ? sp[size_t(vals[i]) * 3].z : 0.0f;
+                std::memcpy(&depthKeys[i], &d, 4);
+            }
+        }
+    } else if (n > 0 && (tileKeys || depthKeys)) {
         std::vector<uint64_t> k(n);
-        const void* src = r->sortedIn ? r->keysB.ptr : r->keysA.ptr;
-        GSR_HIP(hipMemcpy(k.data(), src, size_t(n) * 8, hipMemcpyDeviceToHost));
+        GSR_HIP(hipMemcpy(k.data(), keySrc, size_t(n) * 8, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < n; ++i) {
             if (tileKeys) tileKeys[i] = uint32_t(k[i] >> 32);
             if (depthKeys) depthKeys[i] = uint32_t(k[i]);
         }
     }
-    if (n > 0 && values)
-        GSR_HIP(hipMemcpy(values, r->sortedIn ? r->valuesB.ptr : r->valuesA.ptr, size_t(n) * 4, hipMemcpyDeviceToHost));
     if (ranges)
         GSR_HIP(hipMemcpy(ranges, r->ranges.ptr, size_t(r->lastCam.tiles_x) * r->lastCam.tiles_y * 8,
                           hipMemcpyDeviceToHost));

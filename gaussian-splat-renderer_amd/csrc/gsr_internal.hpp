@@ -83,8 +83,10 @@ size_t sortStatusBytes(uint32_t maxCount);
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
                       hipStream_t stream);
+// scanVisible = false: blockOffsets = exclusive scan of the blocks' tile sums (global mode);
+// true: of their visible (non-zero count) gaussians (binning mode's compaction).
 void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
-                      uint32_t* blockOffsets, gsr_control* control, hipStream_t stream);
+                      bool scanVisible, uint32_t* blockOffsets, gsr_control* control, hipStream_t stream);
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
                      const gsr_control* control, uint32_t count, uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
                      uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream);
@@ -93,10 +95,28 @@ void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uin
 int launchRadixSort(const SortWork& work, uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB,
                     uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, uint32_t* epoch,
                     hipStream_t stream);
+int launchRadixSort32(const SortWork& work, uint32_t* keysA, uint32_t* valuesA, uint32_t* keysB, uint32_t* valuesB,
+                      uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, hipStream_t stream);
 void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,
                      uint32_t maxCount, hipStream_t stream);
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
                   uint2* ranges, hipStream_t stream);
+void launchRanges32(const uint32_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
+                    uint2* ranges, hipStream_t stream);
+
+// Binning mode (the reference's default tile sort, splat_tile_renderer.h:8-26): compact the
+// visible splats (depth bits, index) in index order, sort them by depth (32-bit keys), scan
+// their tile counts in that order, duplicate with 32-bit tile keys, stable sort by tile.
+void launchCompactVisible(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
+                          uint32_t count, uint32_t* depthKeys, uint32_t* depthValues, hipStream_t stream);
+void launchGatherCounts(const uint32_t* tileCounts, const uint32_t* order, const gsr_control* control,
+                        uint32_t maxVisible, uint32_t* sortedCounts, uint32_t* blockSums, hipStream_t stream);
+// Exclusive scan of the ceil(visible / 256) block sums in place (one workgroup).
+void launchScanVisibleBlocks(uint32_t* blockSums, const gsr_control* control, hipStream_t stream);
+void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, const uint32_t* order,
+                            const uint32_t* blockOffsets, const gsr_control* control, uint32_t maxVisible,
+                            uint32_t bounds, uint32_t tilesX, uint32_t tilesY, uint32_t* keys, uint32_t* values,
+                            gsr_control* controlRW, hipStream_t stream);
 void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
                   uint32_t gaussianCount, gsr_target_format format, int pixelsPerThread, bool exactAlpha, int loop,
                   void* out, hipStream_t stream);

@@ -43,20 +43,21 @@ __device__ __forceinline__ uint32_t blockInclusiveScan1024(uint32_t v, uint32_t*
 
 __global__ __launch_bounds__(kSetupThreads) void kFrameSetup(const BlockStats* __restrict__ stats, uint32_t numBlocks,
                                                              uint32_t gaussianCount, uint32_t capacity,
-                                                             uint32_t* __restrict__ blockOffsets,
+                                                             uint32_t scanVisible, uint32_t* __restrict__ blockOffsets,
                                                              gsr_control* __restrict__ control) {
     __shared__ uint32_t sWave[kSetupThreads / 64];
-    __shared__ uint32_t sCull[6];
-    if (threadIdx.x < 6) sCull[threadIdx.x] = 0u;
+    __shared__ uint32_t sCull[7];  // 5 cull reasons, non-zero counts, M
+    if (threadIdx.x < 7) sCull[threadIdx.x] = 0u;
     __syncthreads();
-    uint32_t carry = 0;
+    uint32_t carry = 0, mTotal = 0;
     uint32_t cull[6] = {0, 0, 0, 0, 0, 0};  // [0..4] reasons, [5] non-zero counts
     for (uint32_t base = 0; base < numBlocks; base += kSetupThreads) {
         const uint32_t b = base + threadIdx.x;
         uint32_t v = 0;
         if (b < numBlocks) {
             const BlockStats s = stats[b];
-            v = s.tileSum;
+            v = scanVisible ? s.nonZero : s.tileSum;
+            mTotal += s.tileSum;
             cull[5] += s.nonZero;
 #pragma unroll
             for (int r = 0; r < 5; ++r) cull[r] += s.cull[r];
@@ -65,6 +66,12 @@ __global__ __launch_bounds__(kSetupThreads) void kFrameSetup(const BlockStats* _
         const uint32_t inc = blockInclusiveScan1024(v, sWave, &total);
         if (b < numBlocks) blockOffsets[b] = carry + inc - v;
         carry += total;
+    }
+    {  // M, reduced across the workgroup
+        uint32_t m = mTotal;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off, 64);
+        if ((threadIdx.x & 63u) == 0 && m) atomicAdd(&sCull[6], m);
     }
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
@@ -75,7 +82,7 @@ __global__ __launch_bounds__(kSetupThreads) void kFrameSetup(const BlockStats* _
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t m = carry;
+        const uint32_t m = sCull[6];
         const uint32_t entries = m < capacity ? m : capacity;
         gsr_control c{};
         c.gaussian_count = gaussianCount;
@@ -145,6 +152,22 @@ __global__ __launch_bounds__(kSetupThreads) void kScanBlockSums(uint32_t* __rest
     if (threadIdx.x == 0 && total) *total = carry;
 }
 
+// Binning mode: exclusive scan of the per-256 sums of the depth-ordered tile counts.
+__global__ __launch_bounds__(kSetupThreads) void kScanVisibleBlocks(uint32_t* __restrict__ blockSums,
+                                                                    const gsr_control* __restrict__ control) {
+    __shared__ uint32_t sWave[kSetupThreads / 64];
+    const uint32_t numBlocks = (control->visible + 255u) / 256u;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < numBlocks; base += kSetupThreads) {
+        const uint32_t b = base + threadIdx.x;
+        const uint32_t v = b < numBlocks ? blockSums[b] : 0u;
+        uint32_t t;
+        const uint32_t inc = blockInclusiveScan1024(v, sWave, &t);
+        if (b < numBlocks) blockSums[b] = carry + inc - v;
+        carry += t;
+    }
+}
+
 // Each thread scans kScanItems consecutive elements; the block adds its scanned offset.
 __global__ __launch_bounds__(kScanThreads) void kScanDownsweep(const uint32_t* __restrict__ in,
                                                                uint32_t* __restrict__ out, uint32_t count,
@@ -170,9 +193,13 @@ __global__ __launch_bounds__(kScanThreads) void kScanDownsweep(const uint32_t* _
 }  // namespace
 
 void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
-                      uint32_t* blockOffsets, gsr_control* control, hipStream_t stream) {
+                      bool scanVisible, uint32_t* blockOffsets, gsr_control* control, hipStream_t stream) {
     hipLaunchKernelGGL(kFrameSetup, dim3(1), dim3(kSetupThreads), 0, stream, blockStats, numBlocks, gaussianCount,
-                       capacity, blockOffsets, control);
+                       capacity, scanVisible ? 1u : 0u, blockOffsets, control);
+}
+
+void launchScanVisibleBlocks(uint32_t* blockSums, const gsr_control* control, hipStream_t stream) {
+    hipLaunchKernelGGL(kScanVisibleBlocks, dim3(1), dim3(kSetupThreads), 0, stream, blockSums, control);
 }
 
 void launchExclusiveScan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* blockSums, uint32_t* total,

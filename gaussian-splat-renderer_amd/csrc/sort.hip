@@ -33,7 +33,8 @@ __device__ __forceinline__ uint32_t sortCount(const uint32_t* countDevice, uint3
     return c < maxCount ? c : maxCount;
 }
 
-__device__ __forceinline__ uint32_t digitOf(uint64_t key, uint32_t shift) { return uint32_t(key >> shift) & 0xFFu; }
+template <typename Key>
+__device__ __forceinline__ uint32_t digitOf(Key key, uint32_t shift) { return uint32_t(key >> shift) & 0xFFu; }
 
 __device__ __forceinline__ uint64_t packStatus(uint32_t epoch, uint64_t flag, uint32_t value) {
     return (uint64_t(epoch) << 34) | (flag << 32) | uint64_t(value);
@@ -214,7 +215,8 @@ __global__ __launch_bounds__(kSortThreads) void kSortOnesweep(const uint64_t* __
 
 // --- Reduce-then-scan passes: histogram, per-digit scan, scatter (3 kernels per digit) ---
 
-__global__ __launch_bounds__(kSortThreads) void kSortHistogram(const uint64_t* __restrict__ keys,
+template <typename Key>
+__global__ __launch_bounds__(kSortThreads) void kSortHistogram(const Key* __restrict__ keys,
                                                                const uint32_t* __restrict__ countDevice,
                                                                uint32_t maxCount, uint32_t shift,
                                                                uint32_t* __restrict__ hist) {
@@ -261,15 +263,16 @@ __global__ __launch_bounds__(kSortThreads) void kSortScan(uint32_t* __restrict__
     if (threadIdx.x == 0) digitTotals[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kSortThreads) void kSortScatter(const uint64_t* __restrict__ keysIn,
+template <typename Key>
+__global__ __launch_bounds__(kSortThreads) void kSortScatter(const Key* __restrict__ keysIn,
                                                              const uint32_t* __restrict__ valsIn,
-                                                             uint64_t* __restrict__ keysOut,
+                                                             Key* __restrict__ keysOut,
                                                              uint32_t* __restrict__ valsOut,
                                                              const uint32_t* __restrict__ countDevice,
                                                              uint32_t maxCount, uint32_t shift,
                                                              const uint32_t* __restrict__ hist,
                                                              const uint32_t* __restrict__ digitTotals) {
-    __shared__ uint64_t sKeys[kSortTile];
+    __shared__ Key sKeys[kSortTile];
     __shared__ uint32_t sVals[kSortTile];
     __shared__ uint32_t sWaveHist[kSortWaves][kRadix];
     __shared__ uint32_t sGlobalBase[kRadix];
@@ -296,13 +299,13 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const uint64_t* __r
         const uint32_t tileN = end - tileBegin < uint32_t(kSortTile) ? end - tileBegin : uint32_t(kSortTile);
         for (uint32_t d = lane; d < kRadix; d += 64) sWaveHist[wave][d] = 0u;
 
-        uint64_t key[kSortKeysPerLane];
+        Key key[kSortKeysPerLane];
         uint32_t val[kSortKeysPerLane], dig[kSortKeysPerLane], rank[kSortKeysPerLane];
 #pragma unroll
         for (int r = 0; r < kSortKeysPerLane; ++r) {
             const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
             const bool valid = pos < tileN;
-            key[r] = valid ? keysIn[tileBegin + pos] : 0ull;
+            key[r] = valid ? keysIn[tileBegin + pos] : Key(0);
             val[r] = valid ? valsIn[tileBegin + pos] : 0u;
         }
 #pragma unroll
@@ -351,7 +354,7 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const uint64_t* __r
         }
         __syncthreads();
         for (uint32_t i = t; i < tileN; i += kSortThreads) {
-            const uint64_t k = sKeys[i];
+            const Key k = sKeys[i];
             const uint32_t d = digitOf(k, shift);
             const uint32_t dst = sGlobalBase[d] + (i - sLocalStart[d]);
             keysOut[dst] = k;
@@ -365,8 +368,9 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const uint64_t* __r
 
 // Copies the first `count` pairs (count on the device): moves an odd-pass result back into
 // the caller's buffers without touching the keys past the count.
-__global__ __launch_bounds__(kSortThreads) void kCopyPairs(const uint64_t* __restrict__ ks,
-                                                           const uint32_t* __restrict__ vs, uint64_t* __restrict__ kd,
+template <typename Key>
+__global__ __launch_bounds__(kSortThreads) void kCopyPairs(const Key* __restrict__ ks,
+                                                           const uint32_t* __restrict__ vs, Key* __restrict__ kd,
                                                            uint32_t* __restrict__ vd,
                                                            const uint32_t* __restrict__ countDevice,
                                                            uint32_t maxCount) {
@@ -389,7 +393,37 @@ void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint3
     uint32_t blocks = (maxCount + kSortThreads - 1) / kSortThreads;
     if (blocks > 4096) blocks = 4096;
     if (blocks == 0) return;
-    hipLaunchKernelGGL(kCopyPairs, dim3(blocks), dim3(kSortThreads), 0, stream, ks, vs, kd, vd, countDevice, maxCount);
+    hipLaunchKernelGGL(kCopyPairs<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, ks, vs, kd, vd, countDevice,
+                       maxCount);
+}
+
+// Reduce-then-scan LSD sort of 32-bit keys (depth keys, tile ids): 3 kernels per digit.
+int launchRadixSort32(const SortWork& w, uint32_t* keysA, uint32_t* valuesA, uint32_t* keysB, uint32_t* valuesB,
+                      uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, hipStream_t stream) {
+    if (maxCount == 0) return 0;
+    uint32_t passes = (keyBits + kRadixBits - 1) / kRadixBits;
+    if (passes > 4) passes = 4;
+    uint32_t blocks = (maxCount + kSortTile - 1) / kSortTile;
+    if (blocks > kMaxSortBlocks) blocks = kMaxSortBlocks;
+    uint32_t* kin = keysA;
+    uint32_t* vin = valuesA;
+    uint32_t* kout = keysB;
+    uint32_t* vout = valuesB;
+    for (uint32_t p = 0; p < passes; ++p) {
+        const uint32_t shift = p * kRadixBits;
+        hipLaunchKernelGGL(kSortHistogram<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
+                           maxCount, shift, w.blockHist);
+        hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks, w.globalHist);
+        hipLaunchKernelGGL(kSortScatter<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout, vout,
+                           countDevice, maxCount, shift, w.blockHist, w.globalHist);
+        uint32_t* tk = kin;
+        kin = kout;
+        kout = tk;
+        uint32_t* tv = vin;
+        vin = vout;
+        vout = tv;
+    }
+    return int(passes & 1u);
 }
 
 int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB,
@@ -407,12 +441,12 @@ int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint6
         if (blocks > kMaxSortBlocks) blocks = kMaxSortBlocks;
         for (uint32_t p = 0; p < passes; ++p) {
             const uint32_t shift = p * kRadixBits;
-            hipLaunchKernelGGL(kSortHistogram, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice, maxCount,
-                               shift, w.blockHist);
+            hipLaunchKernelGGL(kSortHistogram<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
+                               maxCount, shift, w.blockHist);
             hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks,
                                w.globalHist);
-            hipLaunchKernelGGL(kSortScatter, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout, vout,
-                               countDevice, maxCount, shift, w.blockHist, w.globalHist);
+            hipLaunchKernelGGL(kSortScatter<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout,
+                               vout, countDevice, maxCount, shift, w.blockHist, w.globalHist);
             uint64_t* tk = kin;
             kin = kout;
             kout = tk;

@@ -20,12 +20,18 @@ constexpr int kDupThreads = kPreprocessThreads;
 
 constexpr int kCachedRows = 8;  // row spans kept in LDS per gaussian; longer splats walk the rest
 
+// BINNING = false: block b = gaussians [256 b, 256 b + 256) in index order, 64-bit keys
+// (tile << 32 | depth bits). BINNING = true: block b = the depth-sorted visible splats
+// [256 b, ...) of `order` (count control->visible), tileCounts in that order, 32-bit keys =
+// tile id.
+template <bool BINNING, typename Key>
 __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restrict__ splats,
                                                           const uint32_t* __restrict__ tileCounts,
+                                                          const uint32_t* __restrict__ order,
                                                           const uint32_t* __restrict__ blockOffsets,
                                                           const gsr_control* __restrict__ control, uint32_t count,
                                                           uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
-                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
+                                                          Key* __restrict__ keys, uint32_t* __restrict__ values,
                                                           gsr_control* controlRW) {
     __shared__ uint32_t sScan[kDupThreads];
     __shared__ uint32_t sWave[kDupThreads / 64];
@@ -40,8 +46,13 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     __shared__ int sBinX[2][kDupThreads];
 
     const uint32_t t = threadIdx.x;
-    const uint32_t i = blockIdx.x * kDupThreads + t;
-    const uint32_t c = i < count ? tileCounts[i] : 0u;
+    const uint32_t j = blockIdx.x * kDupThreads + t;
+    const uint32_t live = BINNING ? control->visible : count;
+    if (BINNING && blockIdx.x * kDupThreads >= live) return;  // uniform: past the visible splats
+    const uint32_t i = BINNING ? (j < live ? order[j] : 0u) : j;
+    const uint32_t c = j < live ? tileCounts[BINNING ? j : i] : 0u;  // binning: counts in depth order
+    __shared__ uint32_t sIndex[kDupThreads];
+    sIndex[t] = i;  // read after the barrier below
 
     // Block exclusive scan of the counts (the downsweep of the reduce-then-scan).
     const uint32_t lane = t & 63u, wave = t >> 6;
@@ -160,22 +171,29 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                 }
             }
         }
-        keys[k] = (uint64_t(tile) << 32) | uint64_t(sDepth[lo]);
-        values[k] = blockIdx.x * kDupThreads + lo;
+        if (BINNING) keys[k] = Key(tile);
+        else keys[k] = Key((uint64_t(tile) << 32) | uint64_t(sDepth[lo]));
+        values[k] = sIndex[lo];
     }
 }
 
 constexpr int kRangesThreads = 256;
 
-__global__ __launch_bounds__(kRangesThreads) void kRanges(const uint64_t* __restrict__ keys,
+template <typename Key>
+__device__ __forceinline__ uint32_t tileOfKey(Key k) {
+    return sizeof(Key) == 8 ? uint32_t(uint64_t(k) >> 32) : uint32_t(k);
+}
+
+template <typename Key>
+__global__ __launch_bounds__(kRangesThreads) void kRanges(const Key* __restrict__ keys,
                                                           const gsr_control* __restrict__ control, uint32_t tileCount,
                                                           uint2* __restrict__ ranges) {
     const uint32_t n = control->entries;
     for (uint32_t k = blockIdx.x * kRangesThreads + threadIdx.x; k < n; k += gridDim.x * kRangesThreads) {
-        const uint32_t tile = uint32_t(keys[k] >> 32);
+        const uint32_t tile = tileOfKey(keys[k]);
         if (tile >= tileCount) {
             if (k > 0) {
-                const uint32_t prev = uint32_t(keys[k - 1] >> 32);
+                const uint32_t prev = tileOfKey(keys[k - 1]);
                 if (prev < tileCount) ranges[prev].y = k;
             }
             continue;
@@ -183,7 +201,7 @@ __global__ __launch_bounds__(kRangesThreads) void kRanges(const uint64_t* __rest
         if (k == 0) {
             ranges[tile].x = 0u;
         } else {
-            const uint32_t prev = uint32_t(keys[k - 1] >> 32);
+            const uint32_t prev = tileOfKey(keys[k - 1]);
             if (prev != tile) {
                 if (prev < tileCount) ranges[prev].y = k;
                 ranges[tile].x = k;
@@ -200,16 +218,37 @@ void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uin
                      uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream) {
     const uint32_t blocks = (count + kDupThreads - 1) / kDupThreads;
     if (blocks == 0) return;
-    hipLaunchKernelGGL(kDuplicate, dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts, blockOffsets,
-                       control, count, bounds, tilesX, tilesY, keys, values, controlRW);
+    hipLaunchKernelGGL((kDuplicate<false, uint64_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
+                       nullptr, blockOffsets, control, count, bounds, tilesX, tilesY, keys, values, controlRW);
+}
+
+void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, const uint32_t* order,
+                            const uint32_t* blockOffsets, const gsr_control* control, uint32_t maxVisible,
+                            uint32_t bounds, uint32_t tilesX, uint32_t tilesY, uint32_t* keys, uint32_t* values,
+                            gsr_control* controlRW, hipStream_t stream) {
+    const uint32_t blocks = (maxVisible + kDupThreads - 1) / kDupThreads;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL((kDuplicate<true, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, sortedCounts,
+                       order, blockOffsets, control, maxVisible, bounds, tilesX, tilesY, keys, values, controlRW);
+}
+
+template <typename Key>
+void launchRangesT(const Key* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount, uint2* ranges,
+                   hipStream_t stream) {
+    uint32_t blocks = (capacity + kRangesThreads - 1) / kRangesThreads;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kRanges<Key>, dim3(blocks), dim3(kRangesThreads), 0, stream, keys, control, tileCount, ranges);
 }
 
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
                   uint2* ranges, hipStream_t stream) {
-    uint32_t blocks = (capacity + kRangesThreads - 1) / kRangesThreads;
-    if (blocks > 2048) blocks = 2048;
-    if (blocks == 0) return;
-    hipLaunchKernelGGL(kRanges, dim3(blocks), dim3(kRangesThreads), 0, stream, keys, control, tileCount, ranges);
+    launchRangesT(keys, control, capacity, tileCount, ranges, stream);
+}
+
+void launchRanges32(const uint32_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
+                    uint2* ranges, hipStream_t stream) {
+    launchRangesT(keys, control, capacity, tileCount, ranges, stream);
 }
 
 }  // namespace gsr

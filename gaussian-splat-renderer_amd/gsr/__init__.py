@@ -37,7 +37,8 @@ BOUNDS = {"rect": 0, "tight": 1, "cutoff-only": 2, "cutoff": 2}
 TARGETS = {"rgba8": 0, "rgba32f": 1}
 SORT_ALGORITHMS = {"reduce-then-scan": 0, "onesweep": 1}
 RENDER_LOOPS = {"predicated": 0, "divergent": 1}
-STAGES = ("preprocess", "scan", "duplicate", "sort", "ranges", "render", "total")
+SORT_MODES = {"binning": 0, "global": 1}
+STAGES = ("preprocess", "scan", "depth_sort", "duplicate", "sort", "ranges", "render", "total")
 CULL_VISIBLE, CULL_NEAR, CULL_SINGULAR, CULL_OFFSCREEN, CULL_LOW_OPACITY = 0, 1, 2, 3, 4
 
 
@@ -96,7 +97,8 @@ class Settings(ctypes.Structure):
         ("render_exact_alpha", ctypes.c_int32),
         ("sort_algorithm", ctypes.c_int32),
         ("render_loop", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("sort_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -349,7 +351,8 @@ class Renderer:
 
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
                  max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
-                 exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "predicated"):
+                 exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "predicated",
+                 sort_mode: str = "binning"):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -360,9 +363,11 @@ class Renderer:
         s.render_exact_alpha = 1 if exact_alpha else 0
         s.sort_algorithm = SORT_ALGORITHMS[sort_algorithm]
         s.render_loop = RENDER_LOOPS[render_loop]
+        s.sort_mode = SORT_MODES[sort_mode]
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
+        self.sort_mode = sort_mode
         self._image: Optional[DeviceBuffer] = None
         self._cam: Optional[Camera] = None
 

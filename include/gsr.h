@@ -117,7 +117,8 @@ typedef struct gsr_scene_desc {
 /* SplatSettings + TileSettings for this path (splat_tile_renderer.h:73-114). */
 typedef struct gsr_settings {
     gsr_bounds bounds;           /* default TIGHT (cli.h:44) */
-    uint32_t budget_mb;          /* list memory, 24.5 B/entry (splat_tile_renderer.cpp:117-124); 0 = 256 */
+    uint32_t budget_mb;          /* list memory: 16.5 (binning) or 24.5 (global) B per entry, in blocks of
+                                    2048 entries (splat_tile_renderer.cpp:117-124); 0 = 256 */
     int32_t max_sh_degree;       /* evaluated bands = min(scene degree, this) */
     gsr_target_format format;    /* output image format */
     int32_t profile_stages;      /* 1: hipEvent pairs around every stage (gsr_read_stage_times) */
@@ -127,8 +128,17 @@ typedef struct gsr_settings {
                                     test before the exponential (same image within 1/255) */
     int32_t sort_algorithm;      /* GSR_SORT_*: radix-sort pass structure (same result, bit for bit) */
     int32_t render_loop;         /* GSR_RENDER_LOOP_*: loop form of the fast one-pixel render */
-    uint32_t reserved[3];
+    int32_t sort_mode;           /* GSR_SORT_MODE_*: how the tile list is built (same list without overflow) */
+    uint32_t reserved[2];
 } gsr_settings;
+
+/* Tile-list construction (TileSortMode, splat_tile_renderer.h:74-78; same values). */
+typedef enum gsr_sort_mode {
+    GSR_SORT_MODE_BINNING = 0, /* depth sort of the visible splats (32-bit keys), duplicate in depth order,
+                                  stable sort of 32-bit tile ids; 16.5 B per list entry (default) */
+    GSR_SORT_MODE_GLOBAL = 1   /* duplicate in index order, one sort of 64-bit (tile, depth) keys;
+                                  24.5 B per list entry */
+} gsr_sort_mode;
 
 /* Loop form of the fast one-pixel tile render (same blend, same image within 1/255). */
 typedef enum gsr_render_loop {
@@ -139,19 +149,21 @@ typedef enum gsr_render_loop {
 /* Radix-sort pass structure. */
 typedef enum gsr_sort_algorithm {
     GSR_SORT_REDUCE_THEN_SCAN = 0, /* per digit: block histograms, scan, ballot-ranked scatter (default) */
-    GSR_SORT_ONESWEEP = 1          /* one histogram pass, then one scatter per digit with decoupled look-back */
+    GSR_SORT_ONESWEEP = 1          /* one histogram pass, then one scatter per digit with decoupled look-back;
+                                      64-bit keys only (global mode, gsr_sort_pairs) */
 } gsr_sort_algorithm;
 
 /* Stage timings of the last profiled frame, ms (the GpuProfiler scopes, profiler.h:1-87). */
 typedef enum gsr_stage {
     GSR_STAGE_PREPROCESS = 0, /* preprocess + cull + tile counts + block sums */
-    GSR_STAGE_SCAN = 1,       /* block-sum scan + control (tile_setup) */
-    GSR_STAGE_DUPLICATE = 2,  /* in-block scan + duplication */
-    GSR_STAGE_SORT = 3,       /* all radix passes */
-    GSR_STAGE_RANGES = 4,     /* tile ranges */
-    GSR_STAGE_RENDER = 5,     /* tile render incl. RGBA8 resolve */
-    GSR_STAGE_TOTAL = 6,
-    GSR_STAGE_COUNT = 7
+    GSR_STAGE_SCAN = 1,       /* block-sum scan + control (tile_setup); binning: + compaction of the visible */
+    GSR_STAGE_DEPTH_SORT = 2, /* binning: depth sort of the visible splats (global: 0) */
+    GSR_STAGE_DUPLICATE = 3,  /* in-block scan + duplication; binning: + count gather and block-sum scan */
+    GSR_STAGE_SORT = 4,       /* all radix passes of the tile-list sort */
+    GSR_STAGE_RANGES = 5,     /* tile ranges */
+    GSR_STAGE_RENDER = 6,     /* tile render incl. RGBA8 resolve */
+    GSR_STAGE_TOTAL = 7,
+    GSR_STAGE_COUNT = 8
 } gsr_stage;
 
 const char* gsr_last_error(void);

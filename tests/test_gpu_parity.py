@@ -3,7 +3,9 @@
 Mirrors the reference's GPU suites (tests/test_tile_raster.cpp, test_tight_bounds.cpp,
 test_gpu_sort.cpp):
   - per-gaussian splats and tile counts: bit-exact (FP32, same operation order);
-  - the sorted (tile, depth, gaussian) list and per-tile ranges: bit-exact, incl. overflow;
+  - the sorted (tile, depth, gaussian) list and per-tile ranges: bit-exact in both sort modes
+    (binning: depth sort then stable tile sort; global: 64-bit keys), incl. overflow, where the
+    two modes drop different entries (binning: the farthest, global: the highest indices);
   - control block counts (N, V, M, entries, overflow, cull counts, coverage mismatch = 0);
   - image: float PSNR > 60 dB (test_tile_raster.cpp:498) and RGBA8 within 1 level per
     channel of the quantised oracle image (>= 58.7 dB);
@@ -45,8 +47,9 @@ def _check_lists(r, o_splats, o_counts, tiles_x, tiles_y, bounds, capacity=None)
     import oracle
 
     keys, depth, vals, ranges, ctl = r.read_back()
+    mode = {"global": 0, "binning": 1}[r.sort_mode]
     hi, lo, ov, orng, dup, mism = oracle.tile_lists(o_splats, o_counts, tiles_x, tiles_y, bounds=bounds,
-                                                    capacity=capacity)
+                                                    capacity=capacity, mode=mode)
     assert ctl.duplicates == dup, (ctl.duplicates, dup)
     assert ctl.entries == len(hi), (ctl.entries, len(hi))
     assert ctl.overflow == dup - len(hi)
@@ -60,14 +63,14 @@ def _check_lists(r, o_splats, o_counts, tiles_x, tiles_y, bounds, capacity=None)
 
 
 def _render_case(gsr_ctx, sc, view, W, H, bounds, fmt="f32", target="rgba32f", budget_mb=256, fov=60.0,
-                 exact_alpha=False, ppt=0, loop="predicated"):
+                 exact_alpha=False, ppt=0, loop="predicated", sort_mode="binning"):
     import gsr
     import oracle
 
     cam = _camera(gsr, view, W, H, fov)
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
     r = gsr.Renderer(gsr_ctx, scene, bounds=bounds, target=target, budget_mb=budget_mb, exact_alpha=exact_alpha,
-                     pixels_per_thread=ppt, render_loop=loop)
+                     pixels_per_thread=ppt, render_loop=loop, sort_mode=sort_mode)
     img = r.render_to_host(cam)
     if fmt == "f16":  # the oracle renders exactly what the GPU decodes (gsplat_ref.cpp:182-190)
         from gsr.scenes import SyntheticScene
@@ -80,16 +83,17 @@ def _render_case(gsr_ctx, sc, view, W, H, bounds, fmt="f32", target="rgba32f", b
     return r, scene, cam, img, sc, ocam, ob, o_splats, o_counts
 
 
+@pytest.mark.parametrize("sort_mode", ["binning", "global"])
 @pytest.mark.parametrize("bounds", ["rect", "tight"])
-def test_100k_720p_splats_lists_image(gsr_ctx, bounds):
+def test_100k_720p_splats_lists_image(gsr_ctx, bounds, sort_mode):
     """The small workload: 100k gaussians, N(0, 2^2), camera at radius 8, 1280x720."""
     import oracle
     from gsr import camera, scenes
 
-    log(f"case 100k-720p bounds={bounds}")
+    log(f"case 100k-720p bounds={bounds} sort_mode={sort_mode}")
     sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
     r, scene, cam, img, sc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, camera.orbit_view(8.0, 0.0, 0.0), 1280,
-                                                              720, bounds)
+                                                              720, bounds, sort_mode=sort_mode)
     gs, gc = r.read_back_splats()
     _check_splats(gs, gc, os_, oc)
     ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
@@ -157,13 +161,13 @@ def test_reference_test_scene_ties_and_overflow(gsr_ctx):
     log("case test-scene 250k 640x360")
     sc = scenes.make_test_scene(250_000, seed=99)
     view = camera.look_at([0.3, 0.5, 6.0], [0.0, 0.0, 0.0])
-    for bounds in ("rect", "tight"):
-        r, scene, cam, img, sc2, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 640, 360, bounds)
+    for bounds, mode in (("rect", "binning"), ("tight", "binning"), ("tight", "global")):
+        r, scene, cam, img, sc2, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 640, 360, bounds, sort_mode=mode)
         gs, gc = r.read_back_splats()
         _check_splats(gs, gc, os_, oc)
         keys, depth, vals, ranges, ctl = r.read_back()
         ties = int(((keys[1:] == keys[:-1]) & (depth[1:] == depth[:-1])).sum())
-        log(f"  {bounds}: entries {ctl.entries}, adjacent equal (tile, depth): {ties}")
+        log(f"  {bounds} {mode}: entries {ctl.entries}, adjacent equal (tile, depth): {ties}")
         assert ties > 0
         _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
         ref, st = oracle.render(sc2, ocam, bounds=ob)
@@ -171,18 +175,45 @@ def test_reference_test_scene_ties_and_overflow(gsr_ctx):
         assert oracle.psnr(img[..., :3], ref) > 60.0
         r.close()
         scene.close()
-    # Overflow: the entries past the capacity are dropped in gaussian-index order.
+    # Overflow: the entries past the capacity are dropped in duplication order: depth order in
+    # binning mode (16.5 B per entry), gaussian-index order in global mode (24.5 B per entry).
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
-    r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", budget_mb=1)
     cam = _camera(gsr, view, 640, 360)
-    r.render(cam)
-    assert r.capacity == 40960
     ocam = oracle.OracleCamera.from_gsr(cam)
     os_, oc = oracle.preprocess(sc, ocam, bounds=oracle.TIGHT)
-    ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, oracle.TIGHT, capacity=r.capacity)
-    log(f"  1 MB budget: duplicates {ctl.duplicates}, entries {ctl.entries}, overflow {ctl.overflow}")
-    assert ctl.overflow > 0 and ctl.entries == r.capacity
-    r.close()
+    for mode, capacity in (("binning", 63488), ("global", 40960)):
+        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", budget_mb=1, sort_mode=mode)
+        r.render(cam)
+        assert r.capacity == capacity
+        ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, oracle.TIGHT, capacity=r.capacity)
+        log(f"  1 MB budget, {mode}: duplicates {ctl.duplicates}, entries {ctl.entries}, overflow {ctl.overflow}")
+        assert ctl.overflow > 0 and ctl.entries == r.capacity
+        r.close()
+    scene.close()
+
+
+def test_binning_lists_equal_global(gsr_ctx):
+    """Without overflow the two sort modes build the same list and the same image, bit for bit
+    (splat_tile_renderer.h:8-26); binning rejects the 64-bit-only one-sweep sort."""
+    import gsr
+    from gsr import camera, scenes
+
+    log("case binning vs global, 100k-720p")
+    sc = scenes.make_scene(100_000, seed=3, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cam = _camera(gsr, camera.orbit_view(8.0, 20.0, 70.0), 1280, 720)
+    out = []
+    for mode in ("binning", "global"):
+        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", sort_mode=mode)
+        img = r.render_to_host(cam).copy()
+        out.append((img, r.read_back()))
+        r.close()
+    assert out[0][1][4].overflow == 0 and out[0][1][4].entries > 0
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1][:4], out[1][1][:4]):
+        assert np.array_equal(a, b)
+    with pytest.raises(gsr.GsrError, match="one-sweep"):
+        gsr.Renderer(gsr_ctx, scene, sort_mode="binning", sort_algorithm="onesweep")
     scene.close()
 
 
@@ -197,7 +228,7 @@ def test_onesweep_lists_equal_reduce_then_scan(gsr_ctx):
     cam = _camera(gsr, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720)
     out = []
     for alg in ("reduce-then-scan", "onesweep"):
-        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", sort_algorithm=alg)
+        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", sort_algorithm=alg, sort_mode="global")
         img = r.render_to_host(cam).copy()
         out.append((img, r.read_back()))
         assert out[-1][1][4].sort_timeouts == 0
