@@ -250,12 +250,30 @@ gsr_status gsr_scene_create(gsr_context* ctx, const gsr_scene_desc* d, gsr_scene
         *dst = buf.ptr;
         return e;
     };
+    // Records wider than 16 B are re-laid out as planes of 16-B quads in library-owned memory
+    // (also for device pointers): the source goes to the device as it is (or is read in
+    // place), and a layout kernel writes the planes.
+    hipStream_t stream = ctx->stream;
+    DeviceBuffer staging;
+    auto planes = [&](DeviceBuffer& buf, const void* src, size_t srcBytes, size_t planeBytes,
+                      const void** devSrc) -> hipError_t {
+        hipError_t e = buf.allocate(planeBytes);
+        if (e != hipSuccess) return e;
+        if (d->device_pointers) {
+            *devSrc = src;
+            return hipSuccess;
+        }
+        if ((e = staging.allocate(srcBytes)) != hipSuccess) return e;
+        *devSrc = staging.ptr;
+        return hipMemcpy(staging.ptr, src, srcBytes, hipMemcpyHostToDevice);
+    };
     SceneRef& r = s->ref;
     r.count = d->count;
     r.K = K;
     r.format = d->format;
     r.degree = d->sh_degree;
     r.colorQuads = colorHalves / 8;
+    r.shQuads = (K * 3 + 3) / 4;
     hipError_t e = hipSuccess;
     const void* p = nullptr;
     if (d->format == GSR_SCENE_F32_SOA) {
@@ -264,12 +282,25 @@ gsr_status gsr_scene_create(gsr_context* ctx, const gsr_scene_desc* d, gsr_scene
             r.scales = static_cast<const float4*>(p);
         if (e == hipSuccess && (e = bind(s->rotations, d->rotations, n * 16, &p)) == hipSuccess)
             r.rotations = static_cast<const float4*>(p);
-        if (e == hipSuccess && (e = bind(s->sh, d->sh, n * K * 3 * 4, &p)) == hipSuccess)
-            r.sh = static_cast<const float*>(p);
-    } else {
-        if ((e = bind(s->geometry, d->geometry, n * 32, &p)) == hipSuccess) r.geometry = static_cast<const uint4*>(p);
-        if (e == hipSuccess && (e = bind(s->color, d->color, n * colorHalves * 2, &p)) == hipSuccess)
-            r.color = static_cast<const uint4*>(p);
+        if (e == hipSuccess && n > 0 && (e = planes(s->sh, d->sh, n * K * 3 * 4, n * r.shQuads * 16, &p)) == hipSuccess) {
+            launchFloatsToPlanes(static_cast<const float*>(p), uint32_t(n), K * 3, r.shQuads, s->sh.as<float4>(), stream);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            r.shPlanes = s->sh.as<float4>();
+        }
+    } else if (n > 0) {
+        if ((e = planes(s->geometry, d->geometry, n * 32, n * 32, &p)) == hipSuccess) {
+            launchQuadsToPlanes(static_cast<const uint4*>(p), uint32_t(n), 2, s->geometry.as<uint4>(), stream);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            r.geometry = s->geometry.as<uint4>();
+        }
+        if (e == hipSuccess && (e = planes(s->color, d->color, n * colorHalves * 2, n * colorHalves * 2, &p)) == hipSuccess) {
+            launchQuadsToPlanes(static_cast<const uint4*>(p), uint32_t(n), r.colorQuads, s->color.as<uint4>(), stream);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            r.color = s->color.as<uint4>();
+        }
     }
     if (e != hipSuccess) {
         delete s;  // frees the partial upload (gpu_scene.cpp:108-114)
@@ -427,6 +458,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (s.sort_mode == GSR_SORT_MODE_BINNING && s.sort_algorithm == GSR_SORT_ONESWEEP)
         return fail(GSR_ERROR_INVALID_ARGUMENT,
                     "gsr_renderer_create: the one-sweep sort takes 64-bit keys (sort_mode GLOBAL)");
+    if (s.render_loop != GSR_RENDER_LOOP_CULLED && s.render_loop != GSR_RENDER_LOOP_DIVERGENT &&
+        s.render_loop != GSR_RENDER_LOOP_PREDICATED)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown render_loop %d", int(s.render_loop));
     if (s.render_pixels_per_thread != 1 && s.render_pixels_per_thread != 2 && s.render_pixels_per_thread != 4)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: render_pixels_per_thread must be 1, 2 or 4");
     GSR_HIP(hipSetDevice(ctx->device));

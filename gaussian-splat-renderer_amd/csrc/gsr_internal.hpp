@@ -38,6 +38,8 @@ struct DevCamera {
     uint32_t width, height, tilesX, tilesY;
 };
 
+// Device view of a scene. Records wider than 16 B are stored as planes of 16-B quads:
+// quad q of gaussian i is at plane[q * count + i].
 struct SceneRef {
     uint32_t count;
     uint32_t K;              // coefficients per channel stored
@@ -46,11 +48,17 @@ struct SceneRef {
     const float4* posAlpha;  // F32
     const float4* scales;
     const float4* rotations;
-    const float* sh;
-    const uint4* geometry;   // F16: 2 x uint4 per gaussian
-    const uint4* color;      // F16: colorQuads x uint4 per gaussian
+    const float4* shPlanes;  // F32: shQuads planes; quad q = floats [4q, 4q + 4) of the K*3 record
+    uint32_t shQuads;
+    const uint4* geometry;   // F16: 2 planes (the two uint4 of the 32-B geometry record)
+    const uint4* color;      // F16: colorQuads planes
     uint32_t colorQuads;     // uint4s per colour record
 };
+
+// Scene layout (scene.hip): record-per-gaussian arrays -> planes of 16-B quads.
+void launchFloatsToPlanes(const float* src, uint32_t count, uint32_t floatsPer, uint32_t quads, float4* planes,
+                          hipStream_t stream);
+void launchQuadsToPlanes(const uint4* src, uint32_t count, uint32_t quads, uint4* planes, hipStream_t stream);
 
 // Sorting: stable LSD radix sort, 8-bit digits, 64-bit keys (tile << 32 | depth bits),
 // one sweep per digit over tiles of 2048 keys.

@@ -63,35 +63,27 @@ template <int FMT>
 __device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, float co[3][16]) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) co[0][k] = co[1][k] = co[2][k] = 0.0f;
+    const size_t plane = s.count;
     if (FMT == GSR_SCENE_F32_SOA) {
-        const float* p = s.sh + size_t(i) * s.K * 3;
-        if ((s.K * 3) % 4 == 0) {
-            const float4* p4 = reinterpret_cast<const float4*>(p);
-            const int quads = (used * 3 + 3) / 4;
+        const int quads = (used * 3 + 3) / 4;
 #pragma unroll
-            for (int q = 0; q < 12; ++q) {
-                if (q < quads) {
-                    const float4 v = p4[q];
-                    const float vv[4] = {v.x, v.y, v.z, v.w};
+        for (int q = 0; q < 12; ++q) {
+            if (q < quads) {
+                const float4 v = s.shPlanes[q * plane + i];
+                const float vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int j = q * 4 + e;
-                        if (j < used * 3) co[j % 3][j / 3] = vv[e];
-                    }
+                for (int e = 0; e < 4; ++e) {
+                    const int j = q * 4 + e;
+                    if (j < used * 3) co[j % 3][j / 3] = vv[e];
                 }
             }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 48; ++j)
-                if (j < used * 3 && uint32_t(j) < s.K * 3) co[j % 3][j / 3] = p[j];
         }
     } else {
-        const uint4* p = s.color + size_t(i) * s.colorQuads;
         const int quads = (used * 3 + 7) / 8;
 #pragma unroll
         for (int q = 0; q < 6; ++q) {
             if (q < quads) {
-                const uint4 v = p[q];
+                const uint4 v = s.color[q * plane + i];
                 const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
@@ -126,8 +118,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
             pz = pa.z;
             alpha = pa.w;
         } else {
-            g0 = scene.geometry[2 * size_t(i)];
-            g1 = scene.geometry[2 * size_t(i) + 1];
+            g0 = scene.geometry[i];
+            g1 = scene.geometry[size_t(scene.count) + i];
             px = __uint_as_float(g0.x);
             py = __uint_as_float(g0.y);
             pz = __uint_as_float(g0.z);

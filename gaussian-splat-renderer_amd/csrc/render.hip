@@ -17,6 +17,11 @@
 //     any exponential, the rest take one v_exp_f32. EXACT keeps the reference's operation
 //     order and expf for every pair (bitwise the CPU expression, up to expf's last ulp);
 //   - whole-tile early exit: the workgroup stops once all pixels are done (__syncthreads_count);
+//   - culled loop (the default): each wave owns an 8x8 quadrant of the tile. At fetch time
+//     each splat gets the bounding box of its alpha >= 1/255 ellipse (padded so that it
+//     also holds for the FP32 expression of q); each wave compacts the batch to the splats
+//     whose box meets its quadrant and walks only those. The pairs it drops all have
+//     q < log2(1/255), so the image is the same, bit for bit, as the uncompacted loop's;
 //   - XCD-aware tile order: workgroup b runs on XCD b % 8; XCD x gets the tile rows x, x + 8,
 //     ... so horizontally neighbouring tiles (which share splats) share an L2 while every
 //     XCD gets an even share of the dense (image-centre) and sparse rows.
@@ -220,6 +225,156 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     }
 }
 
+// Culled predicated loop (FAST, one pixel per thread, 8x8 quadrant per wave).
+template <int FORMAT>
+__global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ splats,
+                                                     const uint32_t* __restrict__ values,
+                                                     const uint2* __restrict__ ranges, uint32_t width,
+                                                     uint32_t height, uint32_t tilesX, uint32_t tileCount,
+                                                     uint32_t gaussianCount, void* __restrict__ out) {
+    // s0 = (cx, cy, A, B), s1 = (C, log2 o), A/B/C the conic scaled by -log2(e)/2 (B: -log2 e).
+    // Slot kBatch is a null splat (q = -inf for every pixel) that pads the wave lists.
+    __shared__ float4 sS0[kBatch + 1];
+    __shared__ float2 sS1[kBatch + 1];
+    __shared__ float4 sColor[kBatch + 1];
+    __shared__ uint32_t sMask[kBatch];                  // bit w: the box meets wave w's quadrant
+    __shared__ __align__(8) uint16_t sList[4][kBatch + 4];  // per wave: batch slots in list order
+
+    const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
+    const uint32_t row = xcd + kXcds * (idx / tilesX);
+    const uint32_t tile = row * tilesX + idx % tilesX;
+    if (tile >= tileCount) return;  // uniform
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t tx0 = (tile % tilesX) * 16, ty0 = (tile / tilesX) * 16;
+    const uint32_t px = tx0 + (wave & 1u) * 8 + (lane & 7u);
+    const uint32_t py = ty0 + (wave >> 1) * 8 + (lane >> 3);
+    const float pxf = float(px), pyf = float(py);
+    // Quadrant corners (pixel coordinates of the first and last pixel centres), per wave bit.
+    const float qx0[2] = {float(tx0), float(tx0 + 8)};
+    const float qy0[2] = {float(ty0), float(ty0 + 8)};
+    if (t == 0) {
+        sS0[kBatch] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        sS1[kBatch] = make_float2(0.0f, -__builtin_inff());
+        sColor[kBatch] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+
+    float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    float thr = (px < width && py < height) ? kLog2MinAlpha : __builtin_inff();
+    const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
+
+    const uint2 range = ranges[tile];
+    for (uint32_t batch = range.x; batch < range.y; batch += kBatch) {
+        // Whole-tile vote; also the barrier that protects the previous batch's LDS.
+        if (__syncthreads_count(thr < __builtin_inff() ? 0 : 1) == 256) break;
+        {
+            const uint32_t e = batch + t;
+            uint32_t mask = 0u;
+            if (e < range.y) {
+                uint32_t g = values[e];
+                g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
+                const float4* sp = splats + 3 * size_t(g);
+                const float4 o0 = sp[0];
+                const float4 co = sp[1];
+                sColor[t] = sp[2];
+                const float logO = __log2f(co.w);
+                sS0[t] = make_float4(o0.x, o0.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+                sS1[t] = make_float2(-0.5f * kLog2e * co.z, logO);
+                // For a positive definite conic, q >= log2(1/255) <=> a dx^2 + 2 b dx dy + c dy^2 <= r2,
+                // r2 = 2 (log2 o - log2(1/255)) / log2 e, so |dx| <= sqrt(r2 c / det) and
+                // |dy| <= sqrt(r2 a / det), det = a c - b^2. The box is widened by 0.1% and one
+                // pixel, far more than the rounding of q. Any other conic keeps every quadrant.
+                const float r2 = 2.0f * (logO - kLog2MinAlpha) * (1.0f / kLog2e);
+                const float det = co.x * co.z - co.y * co.y;
+                mask = 0xFu;
+                if (det > 0.0f && co.x > 0.0f && co.z > 0.0f) {
+                    if (r2 < 0.0f) {
+                        mask = 0u;  // q <= log2 o < log2(1/255) everywhere
+                    } else {
+                        const float ex = sqrtf(r2 * co.z / det) * 1.001f + 1.0f;
+                        const float ey = sqrtf(r2 * co.x / det) * 1.001f + 1.0f;
+                        if (ex < 1e30f && ey < 1e30f) {  // also false for NaN
+                            uint32_t m = 0u;
+#pragma unroll
+                            for (uint32_t w = 0; w < 4; ++w) {
+                                const float x0 = qx0[w & 1u], y0 = qy0[w >> 1];
+                                const bool hit = o0.x + ex >= x0 && o0.x - ex <= x0 + 7.0f && o0.y + ey >= y0 &&
+                                                 o0.y - ey <= y0 + 7.0f;
+                                m |= hit ? (1u << w) : 0u;
+                            }
+                            mask = m;
+                        }
+                    }
+                }
+            }
+            sMask[t] = mask;
+        }
+        __syncthreads();
+        // This wave's list: the batch slots whose box meets its quadrant, in list order
+        // (slot = j * 64 + lane), padded with the null slot to a multiple of 4.
+        uint32_t count = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t slot = j * 64 + lane;
+            const bool hit = (sMask[slot] >> wave) & 1u;
+            const uint64_t b = __builtin_amdgcn_ballot_w64(hit);
+            if (hit) sList[wave][count + uint32_t(__popcll(b & laneMaskLt))] = uint16_t(slot);
+            count += uint32_t(__popcll(b));
+        }
+        const uint32_t count4 = (count + 3u) & ~3u;
+        if (lane < count4 - count) sList[wave][count + lane] = uint16_t(kBatch);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) != 0) {
+            for (uint32_t j = 0; j < count4; j += 4) {
+                const uint2 four = *reinterpret_cast<const uint2*>(&sList[wave][j]);
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t slot = ((u < 2 ? four.x : four.y) >> (16 * (u & 1u))) & 0xFFFFu;
+                    const float4 s0 = sS0[slot];
+                    const float2 s1 = sS1[slot];
+                    const float dx = s0.x - pxf;
+                    const float dy = s0.y - pyf;
+                    const float q = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, fmaf(s1.x * dy, dy, s1.y));
+                    if (__builtin_amdgcn_ballot_w64(q >= thr) == 0) continue;
+                    const float a = fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q));
+                    const float aEff = q >= thr ? a : 0.0f;
+                    const float testT = T * (1.0f - aEff);
+                    const bool term = testT < kMinTransmittance;
+                    const float w = term ? 0.0f : aEff * T;
+                    const float4 col = sColor[slot];
+                    c0 = fmaf(col.x, w, c0);
+                    c1 = fmaf(col.y, w, c1);
+                    c2 = fmaf(col.z, w, c2);
+                    T = term ? T : testT;
+                    thr = term ? __builtin_inff() : thr;
+                }
+                if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
+            }
+        }
+    }
+    if (px < width && py < height) {
+        const size_t p = size_t(py) * width + px;
+        if (FORMAT == GSR_TARGET_RGBA8) {
+            const uint32_t rgba = quantize8(c0) | (quantize8(c1) << 8) | (quantize8(c2) << 16) |
+                                  (quantize8(1.0f - T) << 24);
+            static_cast<uint32_t*>(out)[p] = rgba;
+        } else {
+            static_cast<float4*>(out)[p] = make_float4(c0, c1, c2, 1.0f - T);
+        }
+    }
+}
+
+template <int FORMAT>
+void launchCulled(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+                  uint32_t gaussianCount, void* out, hipStream_t stream) {
+    const uint32_t tileCount = cam.tilesX * cam.tilesY;
+    const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
+    const uint32_t perXcd = rowsPerXcd * cam.tilesX;
+    hipLaunchKernelGGL((kRenderCulled<FORMAT>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats, values, ranges,
+                       cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out);
+}
+
 template <int FORMAT, bool FAST, int PPT, bool PRED>
 void launchOne(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
                uint32_t gaussianCount, void* out, hipStream_t stream) {
@@ -243,8 +398,10 @@ void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uin
         launchOne<FORMAT, true, 2, false>(splats, values, ranges, cam, gaussianCount, out, stream);
     } else if (loop == GSR_RENDER_LOOP_DIVERGENT) {
         launchOne<FORMAT, true, 1, false>(splats, values, ranges, cam, gaussianCount, out, stream);
-    } else {
+    } else if (loop == GSR_RENDER_LOOP_PREDICATED) {
         launchOne<FORMAT, true, 1, true>(splats, values, ranges, cam, gaussianCount, out, stream);
+    } else {
+        launchCulled<FORMAT>(splats, values, ranges, cam, gaussianCount, out, stream);
     }
 }
 

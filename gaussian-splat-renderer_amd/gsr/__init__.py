@@ -36,7 +36,7 @@ SCENE_F32_SOA, SCENE_F16 = 0, 1
 BOUNDS = {"rect": 0, "tight": 1, "cutoff-only": 2, "cutoff": 2}
 TARGETS = {"rgba8": 0, "rgba32f": 1}
 SORT_ALGORITHMS = {"reduce-then-scan": 0, "onesweep": 1}
-RENDER_LOOPS = {"predicated": 0, "divergent": 1}
+RENDER_LOOPS = {"culled": 0, "divergent": 1, "predicated": 2}
 SORT_MODES = {"binning": 0, "global": 1}
 STAGES = ("preprocess", "scan", "depth_sort", "duplicate", "sort", "ranges", "render", "total")
 CULL_VISIBLE, CULL_NEAR, CULL_SINGULAR, CULL_OFFSCREEN, CULL_LOW_OPACITY = 0, 1, 2, 3, 4
@@ -301,11 +301,14 @@ class Scene:
     """GpuScene: the scene buffers on the device, f32 SoA or fp16 records."""
 
     def __init__(self, ctx: Context, pos_alpha=None, scales=None, rotations=None, sh=None, sh_degree: int = 3,
-                 fmt: str = "f32", geometry=None, color=None):
+                 fmt: str = "f32", geometry=None, color=None, device_pointers: bool = False):
+        """device_pointers: the arrays are first copied into device buffers (kept by this
+        object) and the scene is created from those device pointers."""
         desc = SceneDesc()
         desc.sh_degree = sh_degree
         desc.device_pointers = 0
         self._keep = []
+        self._device_arrays = []
         if fmt == "f32":
             arrays = [_f32c(x) for x in (pos_alpha, scales, rotations, sh)]
             n = arrays[0].shape[0]
@@ -329,6 +332,15 @@ class Scene:
             self._keep = [g, c]
         else:
             raise ValueError(f"unknown scene format {fmt!r}")
+        if device_pointers:
+            names = ("pos_alpha", "scales", "rotations", "sh") if fmt == "f32" else ("geometry", "color")
+            for name, arr in zip(names, self._keep):
+                buf = DeviceBuffer(ctx, max(arr.nbytes, 1))
+                if arr.nbytes:
+                    buf.write(arr)
+                setattr(desc, name, buf.ptr)
+                self._device_arrays.append(buf)
+            desc.device_pointers = 1
         h = _vp()
         _check(_scene_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)), "gsr_scene_create")
         self._keep = []  # copied to the device
@@ -338,6 +350,9 @@ class Scene:
         if self.handle:
             _scene_destroy(self.handle)
             self.handle = None
+        for b in self._device_arrays:
+            b.close()
+        self._device_arrays = []
 
     def __del__(self):
         try:
@@ -351,7 +366,7 @@ class Renderer:
 
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
                  max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
-                 exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "predicated",
+                 exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
                  sort_mode: str = "binning"):
         s = Settings()
         s.bounds = BOUNDS[bounds]

@@ -100,7 +100,9 @@ typedef struct gsr_control {
 } gsr_control;
 
 /* Scene description. Host pointers are copied at creation; with device_pointers != 0 the
- * arrays are bound in place (zero copy) and must outlive the scene (gpu_scene.h:4-6). */
+ * 16-B-per-gaussian arrays (pos_alpha, scales, rotations) are bound in place (zero copy) and
+ * must outlive the scene (gpu_scene.h:4-6). The wider records (sh; f16 geometry and colour)
+ * are always re-laid out at creation into library-owned planes of 16-B quads. */
 typedef struct gsr_scene_desc {
     uint32_t count;
     int32_t sh_degree; /* 0..3; K = (sh_degree + 1)^2 coefficients per channel */
@@ -142,8 +144,11 @@ typedef enum gsr_sort_mode {
 
 /* Loop form of the fast one-pixel tile render (same blend, same image within 1/255). */
 typedef enum gsr_render_loop {
-    GSR_RENDER_LOOP_PREDICATED = 0, /* wave-uniform, 4 splats per iteration, branch-free blend (default) */
-    GSR_RENDER_LOOP_DIVERGENT = 1   /* per-lane exits, one splat per iteration */
+    GSR_RENDER_LOOP_CULLED = 0,     /* predicated, one 8x8 quadrant per wave, each wave walks only the splats
+                                       whose alpha >= 1/255 box meets its quadrant (default) */
+    GSR_RENDER_LOOP_DIVERGENT = 1,  /* per-lane exits, one splat per iteration */
+    GSR_RENDER_LOOP_PREDICATED = 2  /* wave-uniform, 4 splats per iteration, branch-free blend, 16x4 rows per
+                                       wave, every splat of the tile */
 } gsr_render_loop;
 
 /* Radix-sort pass structure. */

@@ -63,7 +63,7 @@ def _check_lists(r, o_splats, o_counts, tiles_x, tiles_y, bounds, capacity=None)
 
 
 def _render_case(gsr_ctx, sc, view, W, H, bounds, fmt="f32", target="rgba32f", budget_mb=256, fov=60.0,
-                 exact_alpha=False, ppt=0, loop="predicated", sort_mode="binning"):
+                 exact_alpha=False, ppt=0, loop="culled", sort_mode="binning"):
     import gsr
     import oracle
 
@@ -109,7 +109,7 @@ def test_100k_720p_splats_lists_image(gsr_ctx, bounds, sort_mode):
     scene.close()
 
 
-@pytest.mark.parametrize("exact_alpha,ppt,loop", [(False, 1, "predicated"), (False, 1, "divergent"),
+@pytest.mark.parametrize("exact_alpha,ppt,loop", [(False, 1, "culled"), (False, 1, "predicated"), (False, 1, "divergent"),
                                                   (True, 1, "predicated"), (False, 2, "predicated"),
                                                   (True, 4, "predicated")])
 def test_rgba8_within_one_level(gsr_ctx, exact_alpha, ppt, loop):
@@ -131,6 +131,62 @@ def test_rgba8_within_one_level(gsr_ctx, exact_alpha, ppt, loop):
     assert p >= 58.7
     r.close()
     scene.close()
+
+
+@pytest.mark.parametrize("case", ["100k-720p", "test-scene-640x360"])
+def test_culled_loop_equals_uncompacted_loop(gsr_ctx, case):
+    """The quadrant-culled loop drops only pairs below the alpha cutoff: its RGBA32F image equals
+    the uncompacted predicated loop's bit for bit (same q, same blend order)."""
+    import gsr
+    from gsr import camera, scenes
+
+    log(f"case culled vs predicated, {case}")
+    if case == "100k-720p":
+        sc, view, W, H = scenes.make_scene(100_000, seed=0, mean_sigma=2.0), camera.orbit_view(8.0, 0.0, 0.0), 1280, 720
+    else:
+        sc, view, W, H = scenes.make_test_scene(250_000, seed=99), camera.look_at([0.3, 0.5, 6.0], [0, 0, 0]), 640, 360
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cam = _camera(gsr, view, W, H)
+    imgs = []
+    for loop in ("culled", "predicated"):
+        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", render_loop=loop)
+        imgs.append(r.render_to_host(cam).copy())
+        r.close()
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+    scene.close()
+
+
+@pytest.mark.parametrize("fmt,degree", [("f32", 3), ("f32", 1), ("f32", 0), ("f16", 3), ("f16", 2)])
+def test_device_pointer_scene_and_sh_degrees(gsr_ctx, fmt, degree):
+    """Scenes created from device pointers render exactly what host-pointer scenes do, for the
+    SH degrees whose records are not a whole number of 16-B quads too (K*3 = 3, 12, 27, 48)."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case device pointers fmt={fmt} degree={degree}")
+    sc = scenes.make_scene(30_000, seed=11, mean_sigma=2.0, sh_degree=degree)
+    cam = _camera(gsr, camera.orbit_view(8.0, 10.0, 40.0), 640, 360)
+    imgs = []
+    for dev in (False, True):
+        scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt,
+                          device_pointers=dev)
+        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f")
+        imgs.append(r.render_to_host(cam).copy())
+        if not dev:
+            gs, gc = r.read_back_splats()
+        r.close()
+        scene.close()
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+    ref_scene = sc
+    if fmt == "f16":
+        from gsr.scenes import SyntheticScene
+
+        g, c = oracle.encode_f16(sc)
+        ref_scene = SyntheticScene(*oracle.decode_f16(g, c, sc.sh_degree), sc.sh_degree, sc.seed)
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    os_, oc = oracle.preprocess(ref_scene, ocam, bounds=oracle.TIGHT)
+    _check_splats(gs, gc, os_, oc)
 
 
 def test_f16_scene_bit_exact(gsr_ctx):
