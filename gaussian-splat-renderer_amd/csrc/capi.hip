@@ -624,14 +624,28 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
                         r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
+        // Full key: all 32 + tileBits bits by the radix sort. Otherwise only the tile bits,
+        // then each tile's range by (depth, index) in the tile depth sort.
+        const bool fullKey = r->settings.full_key_sort != 0;
         which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),
-                                r->valuesB.as<uint32_t>(), r->capacity, &control->entries, 32 + tileBits,
-                                &r->sortEpoch, stream);
+                                r->valuesB.as<uint32_t>(), r->capacity, &control->entries, fullKey ? 0u : 32u,
+                                32 + tileBits, &r->sortEpoch, stream);
         GSR_HIP(hipGetLastError());
-        GSR_HIP(mark(5));
-        sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
-        launchRanges(which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>(), control, r->capacity, tiles,
-                     r->ranges.as<uint2>(), stream);
+        uint64_t* keys = which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>();
+        uint32_t* vals = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
+        if (!fullKey) {
+            launchRanges(keys, control, r->capacity, tiles, r->ranges.as<uint2>(), stream);
+            GSR_HIP(hipGetLastError());
+            launchTileDepthSort(keys, vals, which ? r->keysA.as<uint64_t>() : r->keysB.as<uint64_t>(),
+                                which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>(), r->ranges.as<uint2>(),
+                                tiles, stream);
+            GSR_HIP(hipGetLastError());
+            GSR_HIP(mark(5));
+        } else {
+            GSR_HIP(mark(5));
+            launchRanges(keys, control, r->capacity, tiles, r->ranges.as<uint2>(), stream);
+        }
+        sortedValues = vals;
     }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
@@ -751,7 +765,7 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
     SortWork work{int32_t(algorithm), small, small + kMaxSortPasses * kRadix, ctx->sortStatus.as<uint64_t>(),
                   small + kMaxSortPasses * kRadix + kMaxSortPasses,
                   small + kMaxSortPasses * kRadix + kMaxSortPasses + 1};
-    const int which = launchRadixSort(work, keys, values, keysB, valuesB, maxCount, countDevice, keyBits,
+    const int which = launchRadixSort(work, keys, values, keysB, valuesB, maxCount, countDevice, 0u, keyBits,
                                       &ctx->sortEpoch, stream);
     GSR_HIP(hipGetLastError());
     if (which) {  // odd pass count: the first `count` pairs are in the scratch copy

@@ -98,11 +98,18 @@ void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
                      const gsr_control* control, uint32_t count, uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
                      uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream);
-// Returns which buffer (0 = keysA/valuesA, 1 = B) holds the sorted result. *epoch is the
-// caller's status-epoch counter (one per status array), advanced once per pass.
+// Sorts by key bits [firstBit, keyBits) (stable). Returns which buffer (0 = keysA/valuesA,
+// 1 = B) holds the sorted result. *epoch is the caller's status-epoch counter (one per status
+// array), advanced once per pass.
 int launchRadixSort(const SortWork& work, uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB,
-                    uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, uint32_t* epoch,
-                    hipStream_t stream);
+                    uint32_t maxCount, const uint32_t* countDevice, uint32_t firstBit, uint32_t keyBits,
+                    uint32_t* epoch, hipStream_t stream);
+// Per-tile depth order (tilesort.hip): after a stable sort of 64-bit (tile << 32 | depth) keys
+// by their tile bits only, sorts each tile's range by (depth bits, gaussian index) in place;
+// the lists then equal those of the full 64-bit sort. keysTmp/valuesTmp: scratch of the list's
+// size (used for ranges longer than one LDS tile).
+void launchTileDepthSort(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp, const uint2* ranges,
+                         uint32_t tileCount, hipStream_t stream);
 int launchRadixSort32(const SortWork& work, uint32_t* keysA, uint32_t* valuesA, uint32_t* keysB, uint32_t* valuesB,
                       uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, hipStream_t stream);
 void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,

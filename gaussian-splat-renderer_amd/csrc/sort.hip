@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kSortThreads) void kSortInit(uint32_t* __restrict__
 // Global digit histograms of every pass, in one read of the keys.
 __global__ __launch_bounds__(kSortThreads) void kSortUpfront(const uint64_t* __restrict__ keys,
                                                              const uint32_t* __restrict__ countDevice,
-                                                             uint32_t maxCount, uint32_t passes,
+                                                             uint32_t maxCount, uint32_t firstBit, uint32_t passes,
                                                              uint32_t* __restrict__ globalHist) {
     __shared__ uint32_t sHist[kMaxSortPasses][kRadix];
     for (uint32_t i = threadIdx.x; i < kMaxSortPasses * kRadix; i += kSortThreads) (&sHist[0][0])[i] = 0u;
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kSortThreads) void kSortUpfront(const uint64_t* __r
     const uint32_t count = sortCount(countDevice, maxCount);
     for (uint32_t idx = blockIdx.x * kSortThreads + threadIdx.x; idx < count; idx += gridDim.x * kSortThreads) {
         const uint64_t k = keys[idx];
-        for (uint32_t p = 0; p < passes; ++p) atomicAdd(&sHist[p][digitOf(k, p * kRadixBits)], 1u);
+        for (uint32_t p = 0; p < passes; ++p) atomicAdd(&sHist[p][digitOf(k, firstBit + p * kRadixBits)], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < passes * kRadix; i += kSortThreads) {
@@ -427,10 +427,10 @@ int launchRadixSort32(const SortWork& w, uint32_t* keysA, uint32_t* valuesA, uin
 }
 
 int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB,
-                    uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, uint32_t* epoch,
-                    hipStream_t stream) {
-    if (maxCount == 0) return 0;
-    uint32_t passes = (keyBits + kRadixBits - 1) / kRadixBits;
+                    uint32_t maxCount, const uint32_t* countDevice, uint32_t firstBit, uint32_t keyBits,
+                    uint32_t* epoch, hipStream_t stream) {
+    if (maxCount == 0 || keyBits <= firstBit) return 0;
+    uint32_t passes = (keyBits - firstBit + kRadixBits - 1) / kRadixBits;
     if (passes > kMaxSortPasses) passes = kMaxSortPasses;
     uint64_t* kin = keysA;
     uint32_t* vin = valuesA;
@@ -440,7 +440,7 @@ int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint6
         uint32_t blocks = (maxCount + kSortTile - 1) / kSortTile;
         if (blocks > kMaxSortBlocks) blocks = kMaxSortBlocks;
         for (uint32_t p = 0; p < passes; ++p) {
-            const uint32_t shift = p * kRadixBits;
+            const uint32_t shift = firstBit + p * kRadixBits;
             hipLaunchKernelGGL(kSortHistogram<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
                                maxCount, shift, w.blockHist);
             hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks,
@@ -460,13 +460,13 @@ int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint6
     hipLaunchKernelGGL(kSortInit, dim3(1), dim3(kSortThreads), 0, stream, w.globalHist, w.tickets, passes);
     uint32_t upBlocks = (maxCount + kSortThreads * 16 - 1) / (kSortThreads * 16);
     if (upBlocks > 512) upBlocks = 512;
-    hipLaunchKernelGGL(kSortUpfront, dim3(upBlocks), dim3(kSortThreads), 0, stream, keysA, countDevice, maxCount, passes,
-                       w.globalHist);
+    hipLaunchKernelGGL(kSortUpfront, dim3(upBlocks), dim3(kSortThreads), 0, stream, keysA, countDevice, maxCount,
+                       firstBit, passes, w.globalHist);
     for (uint32_t p = 0; p < passes; ++p) {
         *epoch = (*epoch + 1u) & 0x3FFFFFFFu;
         if (*epoch == 0) *epoch = 1;  // 0 is the value of never-written status words
         hipLaunchKernelGGL(kSortOnesweep, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, kout, vout, countDevice,
-                           maxCount, p * kRadixBits, w.globalHist + p * kRadix, w.status, w.tickets + p, *epoch,
+                           maxCount, firstBit + p * kRadixBits, w.globalHist + p * kRadix, w.status, w.tickets + p, *epoch,
                            w.timeouts);
         uint64_t* tk = kin;
         kin = kout;

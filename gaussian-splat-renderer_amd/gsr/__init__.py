@@ -98,7 +98,8 @@ class Settings(ctypes.Structure):
         ("sort_algorithm", ctypes.c_int32),
         ("render_loop", ctypes.c_int32),
         ("sort_mode", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("full_key_sort", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 1),
     ]
 
 
@@ -367,7 +368,7 @@ class Renderer:
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
                  max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
                  exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
-                 sort_mode: str = "binning"):
+                 sort_mode: str = "binning", full_key_sort: bool = False):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -379,6 +380,7 @@ class Renderer:
         s.sort_algorithm = SORT_ALGORITHMS[sort_algorithm]
         s.render_loop = RENDER_LOOPS[render_loop]
         s.sort_mode = SORT_MODES[sort_mode]
+        s.full_key_sort = 1 if full_key_sort else 0
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds

@@ -131,7 +131,10 @@ typedef struct gsr_settings {
     int32_t sort_algorithm;      /* GSR_SORT_*: radix-sort pass structure (same result, bit for bit) */
     int32_t render_loop;         /* GSR_RENDER_LOOP_*: loop form of the fast one-pixel render */
     int32_t sort_mode;           /* GSR_SORT_MODE_*: how the tile list is built (same list without overflow) */
-    uint32_t reserved[2];
+    int32_t full_key_sort;       /* global mode: 0 (default) = the radix sort orders the tile bits only and a
+                                    per-tile sort orders each tile's range by (depth, index); 1 = the radix sort
+                                    orders all (tile, depth) key bits. Same list, bit for bit. */
+    uint32_t reserved[1];
 } gsr_settings;
 
 /* Tile-list construction (TileSortMode, splat_tile_renderer.h:74-78; same values). */
@@ -164,8 +167,9 @@ typedef enum gsr_stage {
     GSR_STAGE_SCAN = 1,       /* block-sum scan + control (tile_setup); binning: + compaction of the visible */
     GSR_STAGE_DEPTH_SORT = 2, /* binning: depth sort of the visible splats (global: 0) */
     GSR_STAGE_DUPLICATE = 3,  /* in-block scan + duplication; binning: + count gather and block-sum scan */
-    GSR_STAGE_SORT = 4,       /* all radix passes of the tile-list sort */
-    GSR_STAGE_RANGES = 5,     /* tile ranges */
+    GSR_STAGE_SORT = 4,       /* all radix passes of the tile-list sort; global mode, full_key_sort 0: + ranges
+                                 + the per-tile depth sort */
+    GSR_STAGE_RANGES = 5,     /* tile ranges (global mode, full_key_sort 0: nothing, counted in SORT) */
     GSR_STAGE_RENDER = 6,     /* tile render incl. RGBA8 resolve */
     GSR_STAGE_TOTAL = 7,
     GSR_STAGE_COUNT = 8

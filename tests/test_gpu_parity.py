@@ -248,6 +248,38 @@ def test_reference_test_scene_ties_and_overflow(gsr_ctx):
     scene.close()
 
 
+def test_global_tile_depth_sort_equals_full_key_sort(gsr_ctx):
+    """Global mode orders the tile bits with the radix sort and each tile's range with the
+    per-tile depth sort; on a dense scene (tiles of up to ~68k entries: LDS chunks plus merge
+    rounds) the lists equal the full 45-bit radix sort's and the oracle's, bit for bit."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log("case global per-tile depth sort vs full-key sort, dense 200k 320x180")
+    sc = scenes.make_scene(200_000, seed=7, mean_sigma=0.5)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cam = _camera(gsr, camera.orbit_view(8.0, 10.0, 20.0), 320, 180)
+    out = []
+    for full in (False, True):
+        r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", sort_mode="global", full_key_sort=full)
+        img = r.render_to_host(cam).copy()
+        lists = r.read_back()
+        out.append((img, lists))
+        if not full:
+            ocam = oracle.OracleCamera.from_gsr(cam)
+            os_, oc = oracle.preprocess(sc, ocam, bounds=oracle.TIGHT)
+            ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, oracle.TIGHT)
+            longest = int((lists[3][:, 1] - lists[3][:, 0]).max())
+            log(f"  entries {ctl.entries}, longest tile range {longest}")
+            assert longest > 4 * 2048
+        r.close()
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    for a, b in zip(out[0][1][:4], out[1][1][:4]):
+        assert np.array_equal(a, b)
+    scene.close()
+
+
 def test_binning_lists_equal_global(gsr_ctx):
     """Without overflow the two sort modes build the same list and the same image, bit for bit
     (splat_tile_renderer.h:8-26); binning rejects the 64-bit-only one-sweep sort."""
