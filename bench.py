@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--count", type=int, default=1_200_000)
     ap.add_argument("--bounds", default="tight")
     ap.add_argument("--sort-mode", default="binning", choices=("binning", "global"))
+    ap.add_argument("--ply", default=None, help="render an INRIA .ply scene instead of the synthetic one")
+    ap.add_argument("--radius", type=float, default=10.0, help="orbit radius")
+    ap.add_argument("--elevation", type=float, default=15.0, help="orbit elevation, degrees")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     return ap.parse_args()
@@ -78,10 +81,22 @@ def main():
 
     W, H = 1920, 1080
     t0 = time.time()
-    sc = scenes.make_scene(args.count, seed=0, mean_sigma=3.0)
-    print(f"# scene generated in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    if args.ply:
+        from gsr.scenes import SyntheticScene
+
+        ply = gsr.load_ply(args.ply)
+        sc = SyntheticScene(ply.pos_alpha, ply.scales, ply.rotations, ply.sh, ply.sh_degree, 0)
+        args.count = sc.count
+        workload = f"ply:{os.path.basename(args.ply)}-1080p-orbit"
+        print(f"# {args.ply}: {sc.count} gaussians, SH degree {sc.sh_degree}, loaded in {time.time() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+    else:
+        sc = scenes.make_scene(args.count, seed=0, mean_sigma=3.0)
+        workload = "1.2M-1080p-orbit" if args.count == 1_200_000 else f"{args.count}-1080p-orbit"
+        print(f"# scene generated in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
     proj = camera.perspective(60.0, W / H)
-    cams = [gsr.camera_from_viewer(v, proj, camera.IDENTITY, W, H) for v in camera.orbit_frames(ORBIT_FRAMES, 10.0, 15.0)]
+    cams = [gsr.camera_from_viewer(v, proj, camera.IDENTITY, W, H)
+            for v in camera.orbit_frames(ORBIT_FRAMES, args.radius, args.elevation)]
 
     ctx = gsr.Context(0)
     out = gsr.DeviceBuffer(ctx, W * H * 4)
@@ -194,7 +209,7 @@ def main():
         maxdiff = int(np.abs(img[..., :3].astype(np.int32) - q.astype(np.int32)).max())
         psnr = oracle.psnr(img[..., :3].astype(np.float64) / 255.0, ref)
         cpu = {"value": round(float(np.median(times)), 2), "unit": "ms/frame", "cores": threads, "kind": "port",
-               "sample": f"{len(times)} renders of orbit frame {args.warmup % ORBIT_FRAMES} (1.2M, 1080p, "
+               "sample": f"{len(times)} renders of orbit frame {args.warmup % ORBIT_FRAMES} ({workload}, "
                          f"{args.bounds} bounds) by the CPU oracle (renderReference restated), median"}
 
     line = {
@@ -209,9 +224,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": "1.2M-1080p-orbit", "gaussians": n, "width": W, "height": H,
-                   "orbit": "300 frames, radius 10, elevation 15 deg", "bounds": args.bounds, "scene_format": "f32",
+        "data": "ply file" if args.ply else "synthetic",
+        "config": {"workload": workload, "gaussians": n, "width": W, "height": H,
+                   "orbit": f"300 frames, radius {args.radius:g}, elevation {args.elevation:g} deg",
+                   "bounds": args.bounds, "scene_format": "f32",
                    "target": "rgba8", "sort_mode": args.sort_mode},
         "fp16_scene_ms_per_frame": round(result[("f16", args.sort_mode)], 4),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},

@@ -30,6 +30,17 @@ gsr_status fail(gsr_status s, const char* fmt, ...) {
     return s;
 }
 
+}  // namespace
+
+namespace gsr {
+gsr_status setError(gsr_status s, const char* message) {
+    g_lastError = message;
+    return s;
+}
+}  // namespace gsr
+
+namespace {
+
 gsr_status hipFail(hipError_t e, const char* what) {
     return fail(e == hipErrorOutOfMemory ? GSR_ERROR_OUT_OF_MEMORY : GSR_ERROR_HIP, "%s: %s", what,
                 hipGetErrorString(e));

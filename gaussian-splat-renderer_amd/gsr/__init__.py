@@ -184,6 +184,35 @@ _read_lists = _sig("gsr_read_lists", _i32, [_vp, _vp, _vp, _vp, _vp, _u32])
 _read_splats = _sig("gsr_read_splats", _i32, [_vp, _vp, _vp])
 _read_stage_times = _sig("gsr_read_stage_times", _i32, [_vp, _vp, _u32])
 _sort_pairs = _sig("gsr_sort_pairs", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _i32, _vp])
+
+
+class PlySceneStruct(ctypes.Structure):
+    """gsr_ply_scene (include/gsr.h)."""
+
+    _fields_ = [
+        ("count", ctypes.c_uint32),
+        ("sh_degree", ctypes.c_int32),
+        ("pos_alpha", ctypes.POINTER(ctypes.c_float)),
+        ("scales", ctypes.POINTER(ctypes.c_float)),
+        ("rotations", ctypes.POINTER(ctypes.c_float)),
+        ("sh", ctypes.POINTER(ctypes.c_float)),
+        ("raw_opacity", ctypes.POINTER(ctypes.c_float)),
+        ("raw_log_scales", ctypes.POINTER(ctypes.c_float)),
+        ("raw_rotations", ctypes.POINTER(ctypes.c_float)),
+        ("non_finite_values", ctypes.c_uint64),
+        ("zero_quaternions", ctypes.c_uint32),
+        ("warning_count", ctypes.c_uint32),
+        ("warnings", ctypes.POINTER(ctypes.c_char_p)),
+        ("comment_count", ctypes.c_uint32),
+        ("comments", ctypes.POINTER(ctypes.c_char_p)),
+        ("internal", ctypes.c_void_p),
+    ]
+
+
+_ply_load = _sig("gsr_ply_load", _i32, [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(PlySceneStruct))])
+_ply_load_memory = _sig("gsr_ply_load_memory", _i32,
+                        [_vp, ctypes.c_uint64, ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(PlySceneStruct))])
+_ply_free = _sig("gsr_ply_free", None, [ctypes.POINTER(PlySceneStruct)])
 _exclusive_scan = _sig("gsr_exclusive_scan", _i32, [_vp, _vp, _vp, _u32, _vp, _vp])
 
 # Every symbol include/gsr.h declares (checked by the CPU tests).
@@ -192,7 +221,8 @@ EXPORTED = (
     "gsr_buffer_create gsr_buffer_device_ptr gsr_buffer_read gsr_buffer_write gsr_buffer_destroy "
     "gsr_scene_create gsr_scene_destroy gsr_encode_f16 gsr_camera_make gsr_camera_from_viewer "
     "gsr_renderer_create gsr_renderer_destroy gsr_renderer_capacity gsr_renderer_total_bytes gsr_render "
-    "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_stage_times gsr_sort_pairs gsr_exclusive_scan"
+    "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_stage_times gsr_sort_pairs gsr_exclusive_scan "
+    "gsr_ply_load gsr_ply_load_memory gsr_ply_free"
 ).split()
 
 
@@ -239,6 +269,63 @@ def encode_f16(pos_alpha, scales, rotations, sh, sh_degree: int):
     color = np.zeros((n, halves), np.uint16)
     _check(_encode_f16(n, sh_degree, _ptr(pa), _ptr(sc), _ptr(ro), _ptr(s), _ptr(geom), _ptr(color)), "gsr_encode_f16")
     return geom, color
+
+
+# --- PLY scenes (host) --------------------------------------------------------------------------
+
+GSR_ERROR_FORMAT, GSR_ERROR_IO = 5, 6
+
+
+class PlyError(GsrError):
+    """A bad .ply file (the reference's PlyError): the message names the problem."""
+
+
+class PlyScene:
+    """loadGaussianPly (scene/ply_loader.h:86-95): the activated f32 SoA arrays (the inputs of
+    Scene(..., fmt="f32")), the raw file values, and the loader's counters, warnings, comments."""
+
+    def __init__(self, p: PlySceneStruct):
+        n, K = int(p.count), (int(p.sh_degree) + 1) ** 2
+
+        def arr(ptr, cols):
+            return np.ctypeslib.as_array(ptr, shape=(n * cols,)).reshape(n, cols).copy() if n else \
+                np.zeros((0, cols), np.float32)
+
+        self.count, self.sh_degree = n, int(p.sh_degree)
+        self.pos_alpha, self.scales, self.rotations = arr(p.pos_alpha, 4), arr(p.scales, 4), arr(p.rotations, 4)
+        self.sh = arr(p.sh, K * 3)
+        self.raw_opacity = arr(p.raw_opacity, 1).reshape(n)
+        self.raw_log_scales, self.raw_rotations = arr(p.raw_log_scales, 3), arr(p.raw_rotations, 4)
+        self.non_finite_values, self.zero_quaternions = int(p.non_finite_values), int(p.zero_quaternions)
+        self.warnings = [p.warnings[i].decode() for i in range(p.warning_count)]
+        self.comments = [p.comments[i].decode() for i in range(p.comment_count)]
+
+    @property
+    def K(self) -> int:
+        return (self.sh_degree + 1) ** 2
+
+
+def _ply_result(status: int, out) -> PlyScene:
+    if status != 0:
+        msg = _lib_last_error().decode()
+        raise (PlyError if status == GSR_ERROR_FORMAT else GsrError)(msg)
+    try:
+        return PlyScene(out.contents)
+    finally:
+        _ply_free(out)
+
+
+def load_ply(path: str) -> PlyScene:
+    """Loads and activates a binary little-endian INRIA 3DGS .ply file (host)."""
+    out = ctypes.POINTER(PlySceneStruct)()
+    return _ply_result(_ply_load(os.fsencode(path), ctypes.byref(out)), out)
+
+
+def load_ply_bytes(data: bytes, name: str = "<memory>") -> PlyScene:
+    """The same from bytes in memory; `name` prefixes the error messages."""
+    buf = np.frombuffer(data, np.uint8) if len(data) else np.zeros(1, np.uint8)
+    out = ctypes.POINTER(PlySceneStruct)()
+    return _ply_result(_ply_load_memory(buf.ctypes.data, len(data), name.encode(), ctypes.byref(out)), out)
 
 
 # --- objects ------------------------------------------------------------------------------------

@@ -35,7 +35,9 @@ typedef enum gsr_status {
     GSR_ERROR_INVALID_ARGUMENT = 1, /* std::runtime_error on bad settings/formats (splat_tile_renderer.cpp:139-163) */
     GSR_ERROR_HIP = 2,              /* a HIP runtime call failed (VK_CHECK, vk_common.h:28-35) */
     GSR_ERROR_OUT_OF_MEMORY = 3,    /* device allocation failed; nothing is left allocated (gpu_scene.cpp:108-114) */
-    GSR_ERROR_UNSUPPORTED = 4       /* a mode this build does not provide */
+    GSR_ERROR_UNSUPPORTED = 4,      /* a mode this build does not provide */
+    GSR_ERROR_FORMAT = 5,           /* a bad scene file (PlyError, scene/ply_loader.h:29-32) */
+    GSR_ERROR_IO = 6                /* a file could not be opened or read (std::runtime_error, ply_loader.h:86-88) */
 } gsr_status;
 
 typedef struct gsr_context gsr_context;
@@ -242,6 +244,38 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
  * *total_device. Asynchronous on `stream`. */
 gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* out, uint32_t count,
                               uint32_t* total_device, void* stream);
+
+/* --- PLY scenes (loadGaussianPly, scene/ply_loader.h:1-12 and 86-95) ---------------------------
+ * A binary little-endian INRIA 3DGS .ply file decoded and activated on the host into the f32
+ * SoA arrays gsr_scene_create takes (format GSR_SCENE_F32_SOA, device_pointers 0):
+ *   pos_alpha  (x, y, z, 1 / (1 + exp(-opacity)))      count x 4
+ *   scales     (exp(scale_0..2), 0)                      count x 4
+ *   rotations  (rot_0..3) / |q|, w first; identity for |q|^2 <= 1e-24 or non-finite
+ *   sh         count x K x 3, coefficient-major RGB triplets (the file's f_rest is channel-major)
+ * plus the raw file values. The arrays belong to the gsr_ply_scene until gsr_ply_free.
+ * Bad files fail with GSR_ERROR_FORMAT and a message naming the problem (the PlyError texts). */
+typedef struct gsr_ply_scene {
+    uint32_t count;
+    int32_t sh_degree;            /* 0..3 from the number of f_rest_* properties (0/9/24/45) */
+    const float* pos_alpha;
+    const float* scales;
+    const float* rotations;
+    const float* sh;
+    const float* raw_opacity;     /* count: the opacity logits */
+    const float* raw_log_scales;  /* count x 3 */
+    const float* raw_rotations;   /* count x 4, as stored */
+    uint64_t non_finite_values;   /* NaN/Inf values read */
+    uint32_t zero_quaternions;    /* replaced by the identity */
+    uint32_t warning_count;
+    const char* const* warnings;  /* skipped elements, trailing bytes, conversions, the counts above */
+    uint32_t comment_count;
+    const char* const* comments;  /* the header's comment / obj_info lines */
+    void* internal;
+} gsr_ply_scene;
+
+gsr_status gsr_ply_load(const char* path, gsr_ply_scene** out);
+gsr_status gsr_ply_load_memory(const void* bytes, uint64_t size, const char* name, gsr_ply_scene** out);
+void gsr_ply_free(gsr_ply_scene* scene);
 
 #ifdef __cplusplus
 }
