@@ -34,12 +34,16 @@ def parse():
     ap.add_argument("--warmup", type=int, default=35)
     ap.add_argument("--count", type=int, default=1_200_000)
     ap.add_argument("--bounds", default="tight")
-    ap.add_argument("--sort-mode", default="binning", choices=("binning", "global"))
+    ap.add_argument("--sort-mode", default="global", choices=("binning", "global"))
+    ap.add_argument("--render-splat", default="full", choices=("full", "p32", "p24"))
     ap.add_argument("--ply", default=None, help="render an INRIA .ply scene instead of the synthetic one")
     ap.add_argument("--radius", type=float, default=10.0, help="orbit radius")
     ap.add_argument("--elevation", type=float, default=15.0, help="orbit elevation, degrees")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
+    ap.add_argument("--profile-frames", type=int, default=0,
+                    help="for rocprofv3 counter runs: render this many orbit frames of the configured mode, then "
+                         "copy 256 MiB device to device twice (the counters' scale reference) and exit")
     return ap.parse_args()
 
 
@@ -101,6 +105,21 @@ def main():
     ctx = gsr.Context(0)
     out = gsr.DeviceBuffer(ctx, W * H * 4)
     timer = HipTimer()
+    if args.profile_frames > 0:
+        scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+        r = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", sort_mode=args.sort_mode,
+                         render_splat=args.render_splat)
+        for i in range(args.profile_frames):
+            r.render(cams[(args.warmup + 9 * i) % ORBIT_FRAMES], out.ptr)
+        ctx.synchronize()
+        print(f"# {args.profile_frames} frames rendered", file=sys.stderr, flush=True)
+        nbytes = 256 << 20
+        a, b = gsr.DeviceBuffer(ctx, nbytes), gsr.DeviceBuffer(ctx, nbytes)
+        for _ in range(2):
+            assert timer.hip.hipMemcpy(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), ctypes.c_size_t(nbytes), 3) == 0
+        ctx.synchronize()
+        print("# 2 x 256 MiB copies done", file=sys.stderr, flush=True)
+        return
 
     def timed_orbit(renderer, first, steps):
         hip_stream = ctypes.c_void_p()
@@ -119,9 +138,11 @@ def main():
 
     def profile_stages(scene, mode):
         """Per-stage ms and list counts: a second renderer with events between the stages."""
-        rp = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", profile_stages=True, sort_mode=mode)
+        rp = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", profile_stages=True, sort_mode=mode,
+                          render_splat=args.render_splat)
         stage_sum = {k: 0.0 for k in gsr.STAGES}
         entries = vis = 0
+        culls = np.zeros(5, np.float64)
         n_prof = min(args.steps, 60)
         for i in range(n_prof):
             cam = cams[(args.warmup + i * (ORBIT_FRAMES // n_prof)) % ORBIT_FRAMES]
@@ -132,23 +153,30 @@ def main():
                 stage_sum[k] += st[k]
             entries += ctl.entries
             vis += ctl.visible
+            culls += np.array(ctl.cull_counts[:5], np.float64)
         rp.close()
-        return {k: v / n_prof for k, v in stage_sum.items()}, entries / n_prof, vis / n_prof
+        return {k: v / n_prof for k, v in stage_sum.items()}, entries / n_prof, vis / n_prof, culls / n_prof
 
     other = "global" if args.sort_mode == "binning" else "binning"
-    result, modes = {}, {}
-    for fmt, mode in (("f32", args.sort_mode), ("f16", args.sort_mode), ("f32", other)):
+    packed = [m for m in ("p32", "p24") if m != args.render_splat]
+    runs = [("f32", args.sort_mode, args.render_splat), ("f16", args.sort_mode, args.render_splat),
+            ("f32", other, args.render_splat)] + [("f32", args.sort_mode, m) for m in packed]
+    result, modes, splat_modes = {}, {}, {}
+    for fmt, mode, rs in runs:
         scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
-        r = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", sort_mode=mode)
+        r = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", sort_mode=mode, render_splat=rs)
         ms = timed_orbit(r, 0, args.steps)
-        result[(fmt, mode)] = ms / args.steps
-        print(f"# {fmt} {mode}: {ms / args.steps:.4f} ms/frame over {args.steps} frames", file=sys.stderr, flush=True)
-        if fmt == "f32":
-            st, ent, vis = profile_stages(scene, mode)
+        result[(fmt, mode, rs)] = ms / args.steps
+        print(f"# {fmt} {mode} {rs}: {ms / args.steps:.4f} ms/frame over {args.steps} frames", file=sys.stderr,
+              flush=True)
+        if rs != args.render_splat:
+            splat_modes[rs] = round(ms / args.steps, 4)
+        elif fmt == "f32":
+            st, ent, vis, culls = profile_stages(scene, mode)
             modes[mode] = {"ms_per_frame": round(ms / args.steps, 4), "stages_ms": {k: round(v, 4) for k, v in st.items()},
                            "mean_entries": int(ent), "mean_visible": int(vis)}
             if mode == args.sort_mode:
-                stages, mean_entries, mean_visible = st, ent, vis
+                stages, mean_entries, mean_visible, mean_culls = st, ent, vis, culls
                 # Quality: one frame (the first timed one) vs the oracle, RGBA8.
                 qcam = cams[args.warmup % ORBIT_FRAMES]
                 r.render(qcam, out.ptr)
@@ -156,35 +184,34 @@ def main():
         r.close()
         scene.close()
 
-    # Roofline of the dominant HBM-bound stage: algorithmic bytes / stage time.
+    # Roofline of the dominant HBM-bound kernel, kPreprocess (f32 scene): algorithmic bytes per
+    # launch / its launch time (hipEvents around it on the render stream, mean of the profiled
+    # frames). Reads: 16 B position + opacity per gaussian, 32 B scale + rotation unless culled at
+    # the near plane or for opacity, 4 x 48 B SH per gaussian that reaches the colour step
+    # (visible). Writes: the 48-B splat + 4-B tile count per gaussian, 8 B per tile range zeroed.
     n = args.count
     tiles = cams[0].tiles
-    tile_bits = int(tiles).bit_length()
+    near, low_opacity = mean_culls[1], mean_culls[4]
+    K = sc.K
+    pre_bytes = n * 16 + (n - near - low_opacity) * 32 + mean_visible * K * 12 + n * 52 + tiles * 8
+    achieved = pre_bytes / (stages["preprocess"] * 1e-3) / 1e9
+    traffic = None
+    traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic_kPreprocess_1.2M-1080p-orbit.md")
+    if os.path.exists(traffic_file) and not args.ply and args.count == 1_200_000:
+        for line in open(traffic_file):
+            if line.startswith("HBM bytes per launch:"):
+                traffic = int(line.split(":")[1].split()[0])
+    roofline = {"bound": "hbm", "kernel": "kPreprocess<f32>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "algorithmic_bytes": int(pre_bytes),
+                "note": "algorithmic bytes per launch / launch time (hipEvents on the render stream); traffic = "
+                        "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, scaled by a plain copy (profiles/)"}
     if args.sort_mode == "binning":
-        tile_passes = (tile_bits + 7) // 8
-        # per pass: key read (histogram) + key/value read + key/value write, 4 + 8 + 8 B
-        sort_bytes = tile_passes * 20.0 * mean_entries
-        depth_sort_bytes = 4 * 20.0 * mean_visible
-        dup_bytes = mean_entries * 8.0 + mean_visible * (48 + 4 + 4)  # list written; splat, order, count read
-        stage_bytes = {"depth_sort": depth_sort_bytes}
-        sorted_keys = mean_entries * tile_passes + mean_visible * 4
-        passes = {"depth": 4, "tile": tile_passes}
+        passes = {"depth_radix": 4, "tile_radix": (int(tiles).bit_length() + 7) // 8}
+        sort_ms = stages["sort"] + stages["depth_sort"]
     else:
-        key_passes = (32 + tile_bits + 7) // 8
-        sort_bytes = key_passes * 32.0 * mean_entries  # 8 + 12 + 12 B per pass
-        dup_bytes = mean_entries * 12.0 + n * 4.0
-        stage_bytes = {}
-        sorted_keys = mean_entries * key_passes
-        passes = {"key": key_passes}
-    pre_bytes = n * (16 + 32 + 192 + 48 + 4)  # upper bound: every gaussian's record read, splat + count written
-    stage_bytes.update({"preprocess": pre_bytes, "sort": sort_bytes, "duplicate": dup_bytes,
-                        "ranges": mean_entries * (4.0 if args.sort_mode == "binning" else 8.0) + tiles * 8.0})
-    hbm_stages = {k: stages[k] for k in stage_bytes}
-    dom = max(hbm_stages, key=hbm_stages.get)
-    achieved = stage_bytes[dom] / (stages[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "note": "algorithmic bytes per launch / stage time (hipEvents on the render stream)"}
+        passes = {"tile_radix": (int(tiles).bit_length() + 7) // 8, "per_tile_depth_sort": 1}
+        sort_ms = stages["sort"]
 
     cpu = None
     psnr = maxdiff = None
@@ -214,12 +241,12 @@ def main():
 
     line = {
         "metric": METRIC,
-        "value": round(result[("f32", args.sort_mode)], 4),
+        "value": round(result[("f32", args.sort_mode, args.render_splat)], 4),
         "unit": "ms/frame",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(result[("f32", args.sort_mode)], 4),
+        "ms_per_step": round(result[("f32", args.sort_mode, args.render_splat)], 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -228,13 +255,14 @@ def main():
         "config": {"workload": workload, "gaussians": n, "width": W, "height": H,
                    "orbit": f"300 frames, radius {args.radius:g}, elevation {args.elevation:g} deg",
                    "bounds": args.bounds, "scene_format": "f32",
-                   "target": "rgba8", "sort_mode": args.sort_mode},
-        "fp16_scene_ms_per_frame": round(result[("f16", args.sort_mode)], 4),
+                   "target": "rgba8", "sort_mode": args.sort_mode, "render_splat": args.render_splat},
+        "fp16_scene_ms_per_frame": round(result[("f16", args.sort_mode, args.render_splat)], 4),
+        "render_splat_ms_per_frame": splat_modes,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "mean_entries": int(mean_entries),
         "mean_visible": int(mean_visible),
         "sort_passes": passes,
-        "sorted_keys_per_s": round(sorted_keys / ((stages["sort"] + stages["depth_sort"]) * 1e-3), 1),
+        "sorted_keys_per_s": round(mean_entries / (sort_ms * 1e-3), 1),
         "sort_modes": modes,
         "psnr_db_vs_oracle": None if psnr is None else round(psnr, 2),
         "max_diff_levels_vs_oracle": maxdiff,

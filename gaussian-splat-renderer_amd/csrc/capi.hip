@@ -121,8 +121,8 @@ struct gsr_renderer {
     gsr_settings settings{};
     uint32_t capacity = 0;
     int32_t degree = 0;
-    // Per gaussian.
-    DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control;
+    // Per gaussian (+ the packed render splats in the p32 / p24 modes).
+    DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control, packed;
     // The list (ping-pong; keys are 32-bit tile ids in binning mode, 64-bit (tile, depth) in
     // global mode) and the sort's work areas.
     DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus;
@@ -464,6 +464,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (s.render_pixels_per_thread == 0) s.render_pixels_per_thread = 1;
     if (s.sort_algorithm != GSR_SORT_REDUCE_THEN_SCAN && s.sort_algorithm != GSR_SORT_ONESWEEP)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown sort_algorithm %d", int(s.sort_algorithm));
+    if (s.render_splat != GSR_RENDER_SPLAT_FULL && s.render_splat != GSR_RENDER_SPLAT_P32 &&
+        s.render_splat != GSR_RENDER_SPLAT_P24)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown render_splat %d", int(s.render_splat));
     if (s.sort_mode != GSR_SORT_MODE_BINNING && s.sort_mode != GSR_SORT_MODE_GLOBAL)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown sort_mode %d", int(s.sort_mode));
     if (s.sort_mode == GSR_SORT_MODE_BINNING && s.sort_algorithm == GSR_SORT_ONESWEEP)
@@ -491,6 +494,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
         if (e == hipSuccess) e = b.allocate(bytes);
     };
     alloc(r->splats, n * 48);
+    if (s.render_splat != GSR_RENDER_SPLAT_FULL) alloc(r->packed, n * (s.render_splat == GSR_RENDER_SPLAT_P32 ? 32 : 24));
     alloc(r->tileCounts, n * 4);
     alloc(r->blockOffsets, gBlocks * 4);
     alloc(r->control, sizeof(gsr_control));
@@ -535,7 +539,7 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
-                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums};
+                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     return s;
@@ -589,7 +593,8 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
 
     GSR_HIP(mark(0));
     launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
-                     r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords, stream);
+                     r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords,
+                     uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(1));
     launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, r->binning,
@@ -660,7 +665,8 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
-    launchRender(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
+    launchRender(r->splats.as<float4>(), r->packed.as<uint32_t>(), r->settings.render_splat, sortedValues,
+                 r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
                  r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0, r->settings.render_loop,
                  outImage, stream);
     GSR_HIP(hipGetLastError());
@@ -729,6 +735,18 @@ gsr_status gsr_read_splats(gsr_renderer* r, gsr_projected_splat* splats, uint32_
     const size_t n = r->scene->count;
     if (n && splats) GSR_HIP(hipMemcpy(splats, r->splats.ptr, n * 48, hipMemcpyDeviceToHost));
     if (n && tileCounts) GSR_HIP(hipMemcpy(tileCounts, r->tileCounts.ptr, n * 4, hipMemcpyDeviceToHost));
+    return GSR_OK;
+}
+
+gsr_status gsr_read_packed_splats(gsr_renderer* r, uint32_t* words) {
+    if (!r || !words) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_packed_splats: null argument");
+    if (r->settings.render_splat == GSR_RENDER_SPLAT_FULL)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_packed_splats: render_splat is FULL (no packed records)");
+    if (!r->haveFrame) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_packed_splats: no frame rendered yet");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    GSR_HIP(hipDeviceSynchronize());
+    const size_t bytes = size_t(r->scene->count) * (r->settings.render_splat == GSR_RENDER_SPLAT_P32 ? 32 : 24);
+    if (bytes) GSR_HIP(hipMemcpy(words, r->packed.ptr, bytes, hipMemcpyDeviceToHost));
     return GSR_OK;
 }
 

@@ -9,6 +9,28 @@
 
 namespace gsr {
 
+// fp16 bits -> float, exactly (integer decode, also for subnormals, Inf and NaN).
+__device__ __forceinline__ float halfBitsToFloat(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16;
+    const uint32_t exponent = (h >> 10) & 0x1Fu;
+    const uint32_t mant = h & 0x3FFu;
+    if (exponent == 0) {
+        const float v = float(mant) * (1.0f / 16777216.0f);
+        return sign ? -v : v;
+    }
+    if (exponent == 31) return __uint_as_float(sign | 0x7F800000u | (mant << 13));
+    return __uint_as_float(sign | ((exponent + 112u) << 23) | (mant << 13));
+}
+
+// float -> fp16 bits, IEEE round to nearest even (v_cvt_f16_f32).
+__device__ __forceinline__ uint32_t halfBits(float v) {
+    return uint32_t(__builtin_bit_cast(uint16_t, static_cast<_Float16>(v)));
+}
+
+// p24 render splats: the conic is stored in fp16 pre-multiplied by 2^12.
+constexpr float kConicScaleP24 = 4096.0f;
+constexpr float kInvConicScaleP24 = 1.0f / 4096.0f;
+
 __device__ __forceinline__ float selMin(float a, float b) { return b < a ? b : a; }   // std::min(a, b)
 __device__ __forceinline__ float selMax(float a, float b) { return a < b ? b : a; }   // std::max(a, b)
 __device__ __forceinline__ float clampSel(float v, float lo, float hi) { return v < lo ? lo : (hi < v ? hi : v); }

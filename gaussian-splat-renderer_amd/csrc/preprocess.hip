@@ -15,18 +15,6 @@ namespace gsr {
 
 namespace {
 
-__device__ __forceinline__ float halfBitsToFloat(uint32_t h) {
-    const uint32_t sign = (h & 0x8000u) << 16;
-    const uint32_t exponent = (h >> 10) & 0x1Fu;
-    const uint32_t mant = h & 0x3FFu;
-    if (exponent == 0) {
-        const float v = float(mant) * (1.0f / 16777216.0f);
-        return sign ? -v : v;
-    }
-    if (exponent == 31) return __uint_as_float(sign | 0x7F800000u | (mant << 13));
-    return __uint_as_float(sign | ((exponent + 112u) << 23) | (mant << 13));
-}
-
 constexpr float kC0 = 0.28209479177387814f;
 constexpr float kC1 = 0.4886025119029199f;
 constexpr float kC2_0 = 1.0925484305920792f, kC2_1 = -1.0925484305920792f, kC2_2 = 0.31539156525252005f,
@@ -102,7 +90,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                                   uint32_t* __restrict__ tileCounts,
                                                                   BlockStats* __restrict__ blockStats,
                                                                   uint32_t* __restrict__ rangesWords,
-                                                                  uint32_t rangeWordCount) {
+                                                                  uint32_t rangeWordCount, uint32_t renderSplat,
+                                                                  uint32_t* __restrict__ packed) {
     const uint32_t i = blockIdx.x * kPreprocessThreads + threadIdx.x;
     if (i < rangeWordCount) rangesWords[i] = 0u;
 
@@ -247,14 +236,40 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                     o2.y = evalShChannel(degree, co[1], x, y, z);
                     o2.z = evalShChannel(degree, co[2], x, y, z);
                     o2.w = radius;
-                    o1 = make_float4(ka, kb, kc, alpha);
+                    // p24 keeps the conic and opacity in fp16: the tiles are counted, duplicated
+                    // and rendered with the values the record gives back.
+                    float ra = ka, rb = kb, rc = kc, ro = alpha;
+                    if (renderSplat == GSR_RENDER_SPLAT_P24) {
+                        ra = halfBitsToFloat(halfBits(ka * kConicScaleP24)) * kInvConicScaleP24;
+                        rb = halfBitsToFloat(halfBits(kb * kConicScaleP24)) * kInvConicScaleP24;
+                        rc = halfBitsToFloat(halfBits(kc * kConicScaleP24)) * kInvConicScaleP24;
+                        ro = halfBitsToFloat(halfBits(alpha));
+                    }
+                    o1 = make_float4(ra, rb, rc, ro);
                     reason = GSR_CULL_VISIBLE;
                     if (bounds == GSR_BOUNDS_RECT) {
                         touched = uint32_t((rect.x1 - rect.x0) * (rect.y1 - rect.y0));
                     } else {
                         const Coverage cv = makeCoverage(bounds, rect, int(cam.tilesX), int(cam.tilesY), ctrX, ctrY,
-                                                         ka, kb, kc, alpha);
+                                                         ra, rb, rc, ro);
                         touched = coverageCount(cv);
+                    }
+                    if (renderSplat != GSR_RENDER_SPLAT_FULL) {
+                        const uint32_t rad = uint32_t(radius < 65535.0f ? radius : 65535.0f);
+                        const uint32_t rg = halfBits(o2.x) | (halfBits(o2.y) << 16);
+                        const uint32_t br = halfBits(o2.z) | (rad << 16);
+                        if (renderSplat == GSR_RENDER_SPLAT_P32) {
+                            uint4* w = reinterpret_cast<uint4*>(packed) + 2 * size_t(i);
+                            w[0] = make_uint4(__float_as_uint(ctrX), __float_as_uint(ctrY), __float_as_uint(ka),
+                                              __float_as_uint(kb));
+                            w[1] = make_uint4(__float_as_uint(kc), __float_as_uint(alpha), rg, br);
+                        } else {
+                            uint2* w = reinterpret_cast<uint2*>(packed) + 3 * size_t(i);
+                            w[0] = make_uint2(__float_as_uint(ctrX), __float_as_uint(ctrY));
+                            w[1] = make_uint2(halfBits(ka * kConicScaleP24) | (halfBits(kb * kConicScaleP24) << 16),
+                                              halfBits(kc * kConicScaleP24) | (halfBits(alpha) << 16));
+                            w[2] = make_uint2(rg, br);
+                        }
                     }
                 }
             }
@@ -297,16 +312,17 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
 
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      hipStream_t stream) {
+                      uint32_t renderSplat, uint32_t* packed, hipStream_t stream) {
     uint32_t n = scene.count > rangeWordCount ? scene.count : rangeWordCount;
     const uint32_t blocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene,
-                           cam, bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount);
+                           cam, bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat,
+                           packed);
     else
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
-                           bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount);
+                           bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat, packed);
 }
 
 }  // namespace gsr

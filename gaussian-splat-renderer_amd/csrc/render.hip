@@ -25,7 +25,7 @@
 //   - XCD-aware tile order: workgroup b runs on XCD b % 8; XCD x gets the tile rows x, x + 8,
 //     ... so horizontally neighbouring tiles (which share splats) share an L2 while every
 //     XCD gets an even share of the dense (image-centre) and sparse rows.
-#include "gsr_internal.hpp"
+#include "gsr_device.hpp"
 
 namespace gsr {
 
@@ -36,14 +36,47 @@ constexpr int kBatch = 256;
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2MinAlpha = -7.994353436858858f;  // log2(1/255)
 
+// The render's view of splat g: centre, (conic a, b, c, opacity), colour. RS = the record the
+// preprocess wrote for the render (the 48-byte splat, or the p32 / p24 words).
+template <int RS>
+__device__ __forceinline__ void fetchSplat(const float4* __restrict__ splats, const uint32_t* __restrict__ packed,
+                                           uint32_t g, float& cx, float& cy, float4& co, float4& col) {
+    if (RS == GSR_RENDER_SPLAT_FULL) {
+        const float4* s = splats + 3 * size_t(g);
+        const float4 o0 = s[0];
+        cx = o0.x;
+        cy = o0.y;
+        co = s[1];
+        col = s[2];
+    } else if (RS == GSR_RENDER_SPLAT_P32) {
+        const uint4* w = reinterpret_cast<const uint4*>(packed) + 2 * size_t(g);
+        const uint4 a = w[0], b = w[1];
+        cx = __uint_as_float(a.x);
+        cy = __uint_as_float(a.y);
+        co = make_float4(__uint_as_float(a.z), __uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y));
+        col = make_float4(halfBitsToFloat(b.z & 0xFFFFu), halfBitsToFloat(b.z >> 16), halfBitsToFloat(b.w & 0xFFFFu),
+                          0.0f);
+    } else {
+        const uint2* w = reinterpret_cast<const uint2*>(packed) + 3 * size_t(g);
+        const uint2 a = w[0], b = w[1], c = w[2];
+        cx = __uint_as_float(a.x);
+        cy = __uint_as_float(a.y);
+        co = make_float4(halfBitsToFloat(b.x & 0xFFFFu) * kInvConicScaleP24, halfBitsToFloat(b.x >> 16) * kInvConicScaleP24,
+                         halfBitsToFloat(b.y & 0xFFFFu) * kInvConicScaleP24, halfBitsToFloat(b.y >> 16));
+        col = make_float4(halfBitsToFloat(c.x & 0xFFFFu), halfBitsToFloat(c.x >> 16), halfBitsToFloat(c.y & 0xFFFFu),
+                          0.0f);
+    }
+}
+
 __device__ __forceinline__ uint32_t quantize8(float v) {
     v = v > 0.0f ? v : 0.0f;  // also NaN -> 0
     v = v < 1.0f ? v : 1.0f;
     return uint32_t(roundf(v * 255.0f));
 }
 
-template <int FORMAT, bool FAST, int PPT, bool PRED>
+template <int FORMAT, bool FAST, int PPT, bool PRED, int RS>
 __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ splats,
+                                                     const uint32_t* __restrict__ packed,
                                                      const uint32_t* __restrict__ values,
                                                      const uint2* __restrict__ ranges, uint32_t width,
                                                      uint32_t height, uint32_t tilesX, uint32_t tileCount,
@@ -98,15 +131,15 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
             if (e < range.y) {
                 uint32_t g = values[e];
                 g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
-                const float4* s = splats + 3 * size_t(g);
-                const float4 o0 = s[0];
-                const float4 co = s[1];
-                sColor[slot] = s[2];
+                float cx, cy;
+                float4 co, col;
+                fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
+                sColor[slot] = col;
                 if (FAST) {
-                    sS0[slot] = make_float4(o0.x, o0.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
+                    sS0[slot] = make_float4(cx, cy, -0.5f * kLog2e * co.x, -kLog2e * co.y);
                     sS1[slot] = make_float2(-0.5f * kLog2e * co.z, __log2f(co.w));
                 } else {
-                    sS0[slot] = make_float4(o0.x, o0.y, co.x, co.y);
+                    sS0[slot] = make_float4(cx, cy, co.x, co.y);
                     sS1[slot] = make_float2(co.z, co.w);
                 }
             }
@@ -226,8 +259,9 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
 }
 
 // Culled predicated loop (FAST, one pixel per thread, 8x8 quadrant per wave).
-template <int FORMAT>
+template <int FORMAT, int RS>
 __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ splats,
+                                                     const uint32_t* __restrict__ packed,
                                                      const uint32_t* __restrict__ values,
                                                      const uint2* __restrict__ ranges, uint32_t width,
                                                      uint32_t height, uint32_t tilesX, uint32_t tileCount,
@@ -272,10 +306,11 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
             if (e < range.y) {
                 uint32_t g = values[e];
                 g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
-                const float4* sp = splats + 3 * size_t(g);
-                const float4 o0 = sp[0];
-                const float4 co = sp[1];
-                sColor[t] = sp[2];
+                float cx, cy;
+                float4 co, col;
+                fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
+                const float2 o0 = make_float2(cx, cy);
+                sColor[t] = col;
                 const float logO = __log2f(co.w);
                 sS0[t] = make_float4(o0.x, o0.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
                 sS1[t] = make_float2(-0.5f * kLog2e * co.z, logO);
@@ -365,58 +400,77 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
     }
 }
 
-template <int FORMAT>
-void launchCulled(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-                  uint32_t gaussianCount, void* out, hipStream_t stream) {
-    const uint32_t tileCount = cam.tilesX * cam.tilesY;
-    const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
-    const uint32_t perXcd = rowsPerXcd * cam.tilesX;
-    hipLaunchKernelGGL((kRenderCulled<FORMAT>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats, values, ranges,
-                       cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out);
-}
-
-template <int FORMAT, bool FAST, int PPT, bool PRED>
-void launchOne(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-               uint32_t gaussianCount, void* out, hipStream_t stream) {
-    const uint32_t tileCount = cam.tilesX * cam.tilesY;
-    const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
-    const uint32_t perXcd = rowsPerXcd * cam.tilesX;
-    hipLaunchKernelGGL((kRender<FORMAT, FAST, PPT, PRED>), dim3(perXcd * kXcds), dim3(256 / PPT), 0, stream, splats,
-                       values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out);
-}
-
-template <int FORMAT>
-void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uint32_t* values, const uint2* ranges,
+template <int FORMAT, int RS>
+void launchCulled(const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
                   const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+    const uint32_t tileCount = cam.tilesX * cam.tilesY;
+    const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
+    const uint32_t perXcd = rowsPerXcd * cam.tilesX;
+    hipLaunchKernelGGL((kRenderCulled<FORMAT, RS>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats, packed, values,
+                       ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out);
+}
+
+template <int FORMAT, bool FAST, int PPT, bool PRED, int RS>
+void launchOne(const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
+               const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+    const uint32_t tileCount = cam.tilesX * cam.tilesY;
+    const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
+    const uint32_t perXcd = rowsPerXcd * cam.tilesX;
+    hipLaunchKernelGGL((kRender<FORMAT, FAST, PPT, PRED, RS>), dim3(perXcd * kXcds), dim3(256 / PPT), 0, stream,
+                       splats, packed, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount,
+                       out);
+}
+
+template <int FORMAT, int RS>
+void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uint32_t* packed, const uint32_t* values,
+                  const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
     if (exact) {
-        if (ppt == 4) launchOne<FORMAT, false, 4, false>(splats, values, ranges, cam, gaussianCount, out, stream);
-        else if (ppt == 2) launchOne<FORMAT, false, 2, false>(splats, values, ranges, cam, gaussianCount, out, stream);
-        else launchOne<FORMAT, false, 1, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+        if (ppt == 4)
+            launchOne<FORMAT, false, 4, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
+        else if (ppt == 2)
+            launchOne<FORMAT, false, 2, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
+        else
+            launchOne<FORMAT, false, 1, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else if (ppt == 4) {
-        launchOne<FORMAT, true, 4, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+        launchOne<FORMAT, true, 4, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else if (ppt == 2) {
-        launchOne<FORMAT, true, 2, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+        launchOne<FORMAT, true, 2, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else if (loop == GSR_RENDER_LOOP_DIVERGENT) {
-        launchOne<FORMAT, true, 1, false>(splats, values, ranges, cam, gaussianCount, out, stream);
+        launchOne<FORMAT, true, 1, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else if (loop == GSR_RENDER_LOOP_PREDICATED) {
-        launchOne<FORMAT, true, 1, true>(splats, values, ranges, cam, gaussianCount, out, stream);
+        launchOne<FORMAT, true, 1, true, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else {
-        launchCulled<FORMAT>(splats, values, ranges, cam, gaussianCount, out, stream);
+        launchCulled<FORMAT, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     }
+}
+
+template <int FORMAT>
+void launchSplat(int renderSplat, int ppt, bool exact, int loop, const float4* splats, const uint32_t* packed,
+                 const uint32_t* values, const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out,
+                 hipStream_t stream) {
+    if (renderSplat == GSR_RENDER_SPLAT_P32)
+        launchFormat<FORMAT, GSR_RENDER_SPLAT_P32>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
+                                                   out, stream);
+    else if (renderSplat == GSR_RENDER_SPLAT_P24)
+        launchFormat<FORMAT, GSR_RENDER_SPLAT_P24>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
+                                                   out, stream);
+    else
+        launchFormat<FORMAT, GSR_RENDER_SPLAT_FULL>(ppt, exact, loop, splats, packed, values, ranges, cam,
+                                                    gaussianCount, out, stream);
 }
 
 }  // namespace
 
-void launchRender(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
-                  uint32_t gaussianCount, gsr_target_format format, int pixelsPerThread, bool exactAlpha, int loop,
-                  void* out, hipStream_t stream) {
+void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
+                  const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
+                  int pixelsPerThread, bool exactAlpha, int loop, void* out, hipStream_t stream) {
     if (cam.tilesX * cam.tilesY == 0) return;
     if (format == GSR_TARGET_RGBA8)
-        launchFormat<GSR_TARGET_RGBA8>(pixelsPerThread, exactAlpha, loop, splats, values, ranges, cam, gaussianCount,
-                                       out, stream);
+        launchSplat<GSR_TARGET_RGBA8>(renderSplat, pixelsPerThread, exactAlpha, loop, splats, packed, values, ranges,
+                                      cam, gaussianCount, out, stream);
     else
-        launchFormat<GSR_TARGET_RGBA32F>(pixelsPerThread, exactAlpha, loop, splats, values, ranges, cam,
-                                         gaussianCount, out, stream);
+        launchSplat<GSR_TARGET_RGBA32F>(renderSplat, pixelsPerThread, exactAlpha, loop, splats, packed, values, ranges,
+                                        cam, gaussianCount, out, stream);
 }
 
 }  // namespace gsr

@@ -38,6 +38,7 @@ TARGETS = {"rgba8": 0, "rgba32f": 1}
 SORT_ALGORITHMS = {"reduce-then-scan": 0, "onesweep": 1}
 RENDER_LOOPS = {"culled": 0, "divergent": 1, "predicated": 2}
 SORT_MODES = {"binning": 0, "global": 1}
+RENDER_SPLATS = {"full": 0, "p32": 1, "p24": 2}
 STAGES = ("preprocess", "scan", "depth_sort", "duplicate", "sort", "ranges", "render", "total")
 CULL_VISIBLE, CULL_NEAR, CULL_SINGULAR, CULL_OFFSCREEN, CULL_LOW_OPACITY = 0, 1, 2, 3, 4
 
@@ -99,7 +100,8 @@ class Settings(ctypes.Structure):
         ("render_loop", ctypes.c_int32),
         ("sort_mode", ctypes.c_int32),
         ("full_key_sort", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 1),
+        ("render_splat", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 3),
     ]
 
 
@@ -183,6 +185,7 @@ _read_control = _sig("gsr_read_control", _i32, [_vp, ctypes.POINTER(Control)])
 _read_lists = _sig("gsr_read_lists", _i32, [_vp, _vp, _vp, _vp, _vp, _u32])
 _read_splats = _sig("gsr_read_splats", _i32, [_vp, _vp, _vp])
 _read_stage_times = _sig("gsr_read_stage_times", _i32, [_vp, _vp, _u32])
+_read_packed = _sig("gsr_read_packed_splats", _i32, [_vp, _vp])
 _sort_pairs = _sig("gsr_sort_pairs", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _i32, _vp])
 
 
@@ -221,7 +224,8 @@ EXPORTED = (
     "gsr_buffer_create gsr_buffer_device_ptr gsr_buffer_read gsr_buffer_write gsr_buffer_destroy "
     "gsr_scene_create gsr_scene_destroy gsr_encode_f16 gsr_camera_make gsr_camera_from_viewer "
     "gsr_renderer_create gsr_renderer_destroy gsr_renderer_capacity gsr_renderer_total_bytes gsr_render "
-    "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_stage_times gsr_sort_pairs gsr_exclusive_scan "
+    "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_sort_pairs "
+    "gsr_exclusive_scan "
     "gsr_ply_load gsr_ply_load_memory gsr_ply_free"
 ).split()
 
@@ -455,7 +459,7 @@ class Renderer:
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
                  max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
                  exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
-                 sort_mode: str = "binning", full_key_sort: bool = False):
+                 sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full"):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -468,10 +472,12 @@ class Renderer:
         s.render_loop = RENDER_LOOPS[render_loop]
         s.sort_mode = SORT_MODES[sort_mode]
         s.full_key_sort = 1 if full_key_sort else 0
+        s.render_splat = RENDER_SPLATS[render_splat]
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
         self.sort_mode = sort_mode
+        self.render_splat = render_splat
         self._image: Optional[DeviceBuffer] = None
         self._cam: Optional[Camera] = None
 
@@ -534,6 +540,12 @@ class Renderer:
         counts = np.empty(n, np.uint32)
         _check(_read_splats(self.handle, _ptr(splats), _ptr(counts)), "gsr_read_splats")
         return splats, counts
+
+    def read_back_packed(self) -> np.ndarray:
+        """readBackPackedSplats: (N, 8) (p32) or (N, 6) (p24) render-splat words."""
+        words = np.empty((self.scene.count, 8 if self.render_splat == "p32" else 6), np.uint32)
+        _check(_read_packed(self.handle, _ptr(words)), "gsr_read_packed_splats")
+        return words
 
     def stage_times(self) -> dict:
         ms = (ctypes.c_float * len(STAGES))()

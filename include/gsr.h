@@ -136,8 +136,22 @@ typedef struct gsr_settings {
     int32_t full_key_sort;       /* global mode: 0 (default) = the radix sort orders the tile bits only and a
                                     per-tile sort orders each tile's range by (depth, index); 1 = the radix sort
                                     orders all (tile, depth) key bits. Same list, bit for bit. */
-    uint32_t reserved[1];
+    int32_t render_splat;        /* GSR_RENDER_SPLAT_*: the record the render pass gathers per list entry */
+    uint32_t reserved[3];
 } gsr_settings;
+
+/* RenderSplatMode (splat_tile_renderer.h:88-92; render_splat.glsl:1-21), same values. The
+ * preprocess writes the packed record next to the 48-byte splat; the render gathers it.
+ *   p32 (8 words): cx, cy, a, b | c, opacity, h(r) | h(g) << 16, h(b) | radius << 16
+ *   p24 (6 words): cx, cy | h(a 2^12) | h(b 2^12) << 16, h(c 2^12) | h(opacity) << 16 | colour as p32
+ * h() = IEEE fp16, round to nearest even; radius = min(radius, 65535) as an integer. In p24 mode
+ * the 48-byte splat holds the conic and opacity as the record gives them back, and the tiles are
+ * counted with those values (splat_preprocess.comp:296-305). */
+typedef enum gsr_render_splat {
+    GSR_RENDER_SPLAT_FULL = 0, /* the 48-byte splat */
+    GSR_RENDER_SPLAT_P32 = 1,  /* 32 bytes: fp32 centre, conic, opacity; fp16 colour */
+    GSR_RENDER_SPLAT_P24 = 2   /* 24 bytes: fp32 centre; fp16 conic (x 2^12), opacity, colour */
+} gsr_render_splat;
 
 /* Tile-list construction (TileSortMode, splat_tile_renderer.h:74-78; same values). */
 typedef enum gsr_sort_mode {
@@ -231,6 +245,10 @@ gsr_status gsr_read_control(gsr_renderer* r, gsr_control* out);
 gsr_status gsr_read_lists(gsr_renderer* r, uint32_t* tile_keys, uint32_t* depth_keys, uint32_t* values,
                           uint32_t* ranges, uint32_t max_entries);
 gsr_status gsr_read_splats(gsr_renderer* r, gsr_projected_splat* splats, uint32_t* tile_counts);
+/* readBackPackedSplats (splat_tile_renderer.h:184-187), packed modes: the render-splat words of
+ * every gaussian (8 or 6 each; culled gaussians' words are not written). The tile rectangles
+ * and depths of the reference's call are in the 48-byte splats (gsr_read_splats). */
+gsr_status gsr_read_packed_splats(gsr_renderer* r, uint32_t* words);
 gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n);
 
 /* GpuRadixSort on its own (gpu_radix_sort.h:38-122, test_gpu_sort.cpp): stable LSD sort of

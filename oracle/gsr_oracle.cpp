@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstring>
 #include <limits>
 
 namespace oracle {
@@ -281,6 +282,53 @@ float halfToFloat(uint16_t h) {
     }
     if (exponent == 31) return bitsf(sign | 0x7F800000u | (mant << 13));
     return bitsf(sign | ((exponent + 112u) << 23) | (mant << 13));
+}
+
+// --- Packed render splats (render_splat.h:29-66) ---------------------------------------------
+
+namespace {
+uint32_t bitsOf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+uint32_t halfPair(float lo, float hi) { return uint32_t(floatToHalf(lo)) | (uint32_t(floatToHalf(hi)) << 16); }
+uint32_t colorRadius(float b, float radius) {
+    return uint32_t(floatToHalf(b)) | (uint32_t(radius < 65535.0f ? radius : 65535.0f) << 16);
+}
+float viaHalf(float v) { return halfToFloat(floatToHalf(v)); }
+}  // namespace
+
+void encodeRenderSplat(uint32_t mode, const Splat& s, uint32_t* w) {
+    const float* co = s.conicOpacity;
+    if (mode == kRenderSplatP32) {
+        w[0] = bitsOf(s.center[0]);
+        w[1] = bitsOf(s.center[1]);
+        w[2] = bitsOf(co[0]);
+        w[3] = bitsOf(co[1]);
+        w[4] = bitsOf(co[2]);
+        w[5] = bitsOf(co[3]);
+        w[6] = halfPair(s.color[0], s.color[1]);
+        w[7] = colorRadius(s.color[2], s.radius);
+    } else {
+        w[0] = bitsOf(s.center[0]);
+        w[1] = bitsOf(s.center[1]);
+        w[2] = halfPair(co[0] * kConicScaleP24, co[1] * kConicScaleP24);
+        w[3] = halfPair(co[2] * kConicScaleP24, co[3]);
+        w[4] = halfPair(s.color[0], s.color[1]);
+        w[5] = colorRadius(s.color[2], s.radius);
+    }
+}
+
+void applyRenderSplat(uint32_t mode, Splat& s, bool colour) {
+    if (mode == kRenderSplatFull) return;
+    if (mode == kRenderSplatP24) {
+        const float inv = 1.0f / kConicScaleP24;
+        for (int a = 0; a < 3; ++a) s.conicOpacity[a] = viaHalf(s.conicOpacity[a] * kConicScaleP24) * inv;
+        s.conicOpacity[3] = viaHalf(s.conicOpacity[3]);
+    }
+    if (colour)
+        for (int c = 0; c < 3; ++c) s.color[c] = viaHalf(s.color[c]);
 }
 
 }  // namespace oracle

@@ -406,4 +406,15 @@ uint32_t radixSortPairs(std::vector<uint32_t>& keys, std::vector<uint32_t>& valu
 uint16_t floatToHalf(float value);
 float halfToFloat(uint16_t h);
 
+// --- Packed render splats (render_splat.h:1-69, render_splat.glsl:1-75) --------------------
+// p32 (8 words): cx, cy, a, b | c, opacity, h(r) | h(g) << 16, h(b) | radius << 16.
+// p24 (6 words): cx, cy | h(a S) | h(b S) << 16, h(c S) | h(opacity) << 16 | colour as p32,
+// S = 2^12; radius = min(radius, 65535) as an integer.
+constexpr uint32_t kRenderSplatFull = 0, kRenderSplatP32 = 1, kRenderSplatP24 = 2;
+constexpr float kConicScaleP24 = 4096.0f;
+void encodeRenderSplat(uint32_t mode, const Splat& s, uint32_t* words);
+// What the packed record gives back: p24's conic and opacity (the preprocess counts tiles and the
+// duplicate and render passes work with these), and with colour = false, the colour untouched.
+void applyRenderSplat(uint32_t mode, Splat& s, bool colour);
+
 }  // namespace oracle

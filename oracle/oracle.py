@@ -30,11 +30,12 @@ _vp, _u32, _i32, _f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes
 SPLAT_DTYPE = np.dtype([("center", np.float32, 2), ("depth", np.float32), ("cull_reason", np.uint32),
                         ("conic_opacity", np.float32, 4), ("color", np.float32, 3), ("radius", np.float32)])
 RECT, TIGHT, CUTOFF = 0, 1, 2
+RENDER_SPLAT = {"full": 0, "p32": 1, "p24": 2}
 
 
 class RenderOptions(ctypes.Structure):
     _fields_ = [("maxShDegree", _i32), ("earlyTermination", _i32), ("bound", _u32), ("bounds", _u32),
-                ("threads", _i32), ("background", _f32 * 3)]
+                ("threads", _i32), ("background", _f32 * 3), ("renderSplat", _u32)]
 
 
 class Stats(ctypes.Structure):
@@ -68,7 +69,8 @@ _tile_rect = _sig("oracle_tile_rect", None, [_f32, _f32, _f32, _u32, _u32, _vp])
 _eval_sh = _sig("oracle_eval_sh", None, [_i32, _vp, _vp, _vp])
 _alpha_at = _sig("oracle_splat_alpha_at", _f32, [_vp, _f32, _f32])
 _preprocess = _sig("oracle_preprocess", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32, _f32, _i32,
-                                               _u32, _vp, _vp, _i32])
+                                               _u32, _u32, _vp, _vp, _i32])
+_encode_render_splats = _sig("oracle_encode_render_splats", None, [_u32, _vp, _u32, _vp])
 _preprocess_f64 = _sig("oracle_preprocess_f64", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32,
                                                        _f32, _i32, _vp, _i32])
 _coverage_tiles = _sig("oracle_coverage_tiles", _u32, [_vp, _u32, _i32, _u32, _u32, _i32, _i32, _vp, _u32, _vp])
@@ -170,16 +172,26 @@ def splat_alpha_at(splat, px, py):
     return float(_alpha_at(_p(s), px, py))
 
 
-def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGHT, threads=None):
-    """preprocessGaussian over the scene: (N,) splats and the per-gaussian tile counts."""
+def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGHT, threads=None, render_splat="full"):
+    """preprocessGaussian over the scene: (N,) splats and the per-gaussian tile counts.
+    render_splat "p24": visible splats hold the conic and opacity as the p24 record gives them
+    back (the tiles are counted with those)."""
     threads = default_threads() if threads is None else threads
     n = scene.count
     out = np.zeros(n, SPLAT_DTYPE)
     counts = np.zeros(n, np.uint32)
     deg = scene.sh_degree if degree is None else min(degree, scene.sh_degree)
     _preprocess(n, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), scene.K, deg, *cam.args(),
-                1 if gpu_cull else 0, bounds, _p(out), _p(counts), threads)
+                1 if gpu_cull else 0, bounds, RENDER_SPLAT[render_splat], _p(out), _p(counts), threads)
     return out, counts
+
+
+def encode_render_splats(splats, mode: str):
+    """encodeRenderSplat (render_splat.h:29-48): (N, 8) words for "p32", (N, 6) for "p24"."""
+    splats = np.ascontiguousarray(splats, SPLAT_DTYPE)
+    words = np.zeros((splats.shape[0], 8 if mode == "p32" else 6), np.uint32)
+    _encode_render_splats(splats.shape[0], _p(splats), RENDER_SPLAT[mode], _p(words))
+    return words
 
 
 def preprocess_f64(scene, cam: OracleCamera, degree=None, gpu_cull=True, threads=None):
@@ -252,11 +264,13 @@ def decode_f16(geom, color, sh_degree):
 
 
 def render(scene, cam: OracleCamera, max_sh_degree=3, early_termination=True, bound=0, bounds=RECT, threads=None,
-           background=(0.0, 0.0, 0.0), want_lists=False):
-    """renderReference: (H, W, 3) float32 image (unclamped) and ReferenceStats (+ per-tile lists)."""
+           background=(0.0, 0.0, 0.0), want_lists=False, render_splat="full"):
+    """renderReference: (H, W, 3) float32 image (unclamped) and ReferenceStats (+ per-tile lists).
+    render_splat "p32"/"p24": every splat as the packed record gives it back (fp16 colour; p24
+    also fp16 conic and opacity)."""
     threads = default_threads() if threads is None else threads
     opt = RenderOptions(max_sh_degree, 1 if early_termination else 0, bound, bounds, threads,
-                        (ctypes.c_float * 3)(*background))
+                        (ctypes.c_float * 3)(*background), RENDER_SPLAT[render_splat])
     rgb = np.zeros((cam.height, cam.width, 3), np.float32)
     st = Stats()
     tiles = cam.tiles_x * cam.tiles_y

@@ -66,6 +66,7 @@ struct oracle_render_options {
     uint32_t bounds;           // 0 rect, 1 tight, 2 cutoff-only
     int32_t threads;           // 0 = hardware_concurrency
     float background[3];
+    uint32_t renderSplat;      // 0 full, 1 p32, 2 p24: render what the packed record holds
 };
 
 struct oracle_stats {  // ReferenceStats (reference_renderer.h:51-62)
@@ -140,8 +141,8 @@ float oracle_splat_alpha_at(const void* splat, float px, float py) {
 // drawn splat under `bounds` (0 if culled), as the GPU's tileCounts[].
 void oracle_preprocess(uint32_t n, const float* posAlpha, const float* scales, const float* rot, const float* sh,
                        uint32_t K, int32_t degree, const float viewInria[16], uint32_t w, uint32_t h, float tanX,
-                       float tanY, int32_t gpuCull, uint32_t bounds, void* outSplats, uint32_t* outCounts,
-                       int32_t threads) {
+                       float tanY, int32_t gpuCull, uint32_t bounds, uint32_t renderSplat, void* outSplats,
+                       uint32_t* outCounts, int32_t threads) {
     const Camera<float> cam = makeCamera<float>(viewInria, w, h, tanX, tanY);
     Splat* out = static_cast<Splat*>(outSplats);
     parallelFor(n, resolveThreads(threads), 4096, [&](uint32_t i) {
@@ -149,6 +150,8 @@ void oracle_preprocess(uint32_t n, const float* posAlpha, const float* scales, c
                                                           rot + size_t(i) * 4, sh + size_t(i) * K * 3, degree, cam,
                                                           gpuCull != 0);
         out[i] = toSplat(s);
+        // p24: the conic and opacity as the packed record holds them (splat_preprocess.comp:296-305).
+        if (s.cullReason == kVisible) applyRenderSplat(renderSplat, out[i], false);
         if (outCounts) {
             uint32_t c = 0;
             if (s.cullReason == kVisible) {
@@ -342,6 +345,13 @@ void oracle_decode_f16(uint32_t n, const uint32_t* geom, const uint16_t* color, 
 // renderReference (reference_renderer.cpp:41-234), without the StopThePop window. rgb:
 // W*H*3 floats, row 0 = top, unclamped. tileStart/tileLists (optional): the per-tile lists
 // in depth order (tiles + 1 and tileInstances entries; pass null to skip).
+// encodeRenderSplat over n splats: 8 (p32) or 6 (p24) words each.
+void oracle_encode_render_splats(uint32_t n, const void* splats, uint32_t mode, uint32_t* words) {
+    const Splat* sp = static_cast<const Splat*>(splats);
+    const uint32_t stride = mode == kRenderSplatP24 ? 6u : 8u;
+    for (uint32_t i = 0; i < n; ++i) encodeRenderSplat(mode, sp[i], words + size_t(i) * stride);
+}
+
 void oracle_render(uint32_t n, const float* posAlpha, const float* scales, const float* rot, const float* sh,
                    uint32_t K, int32_t shDegree, const float viewInria[16], uint32_t w, uint32_t h, float tanX,
                    float tanY, const oracle_render_options* opt, float* rgb, oracle_stats* statsOut,
@@ -360,6 +370,7 @@ void oracle_render(uint32_t n, const float* posAlpha, const float* scales, const
     parallelFor(n, threads, 4096, [&](uint32_t i) {
         splats[i] = toSplat(preprocessGaussian<float>(posAlpha + size_t(i) * 4, scales + size_t(i) * 4,
                                                       rot + size_t(i) * 4, sh + size_t(i) * K * 3, degree, cam, false));
+        if (splats[i].cullReason == kVisible) applyRenderSplat(opt->renderSplat, splats[i], true);
     });
     for (const Splat& s : splats) {
         switch (s.cullReason) {

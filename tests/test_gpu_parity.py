@@ -63,14 +63,14 @@ def _check_lists(r, o_splats, o_counts, tiles_x, tiles_y, bounds, capacity=None)
 
 
 def _render_case(gsr_ctx, sc, view, W, H, bounds, fmt="f32", target="rgba32f", budget_mb=256, fov=60.0,
-                 exact_alpha=False, ppt=0, loop="culled", sort_mode="binning"):
+                 exact_alpha=False, ppt=0, loop="culled", sort_mode="binning", render_splat="full"):
     import gsr
     import oracle
 
     cam = _camera(gsr, view, W, H, fov)
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
     r = gsr.Renderer(gsr_ctx, scene, bounds=bounds, target=target, budget_mb=budget_mb, exact_alpha=exact_alpha,
-                     pixels_per_thread=ppt, render_loop=loop, sort_mode=sort_mode)
+                     pixels_per_thread=ppt, render_loop=loop, sort_mode=sort_mode, render_splat=render_splat)
     img = r.render_to_host(cam)
     if fmt == "f16":  # the oracle renders exactly what the GPU decodes (gsplat_ref.cpp:182-190)
         from gsr.scenes import SyntheticScene
@@ -79,7 +79,7 @@ def _render_case(gsr_ctx, sc, view, W, H, bounds, fmt="f32", target="rgba32f", b
         sc = SyntheticScene(*oracle.decode_f16(g, c, sc.sh_degree), sc.sh_degree, sc.seed)
     ocam = oracle.OracleCamera.from_gsr(cam)
     ob = {"rect": oracle.RECT, "tight": oracle.TIGHT, "cutoff-only": oracle.CUTOFF}[bounds]
-    o_splats, o_counts = oracle.preprocess(sc, ocam, bounds=ob)
+    o_splats, o_counts = oracle.preprocess(sc, ocam, bounds=ob, render_splat=render_splat)
     return r, scene, cam, img, sc, ocam, ob, o_splats, o_counts
 
 
@@ -222,6 +222,39 @@ def test_ply_scene_end_to_end(gsr_ctx, tmp_path):
     _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
     ref, _ = oracle.render(loaded, ocam, bounds=ob)
     assert oracle.psnr(img[..., :3], ref) > 60.0
+    r.close()
+    scene.close()
+
+
+@pytest.mark.parametrize("mode", ["p32", "p24"])
+def test_packed_render_splats(gsr_ctx, mode):
+    """p32 / p24 render splats (test_scene_formats.cpp:248-316): the 48-byte splats (p24: decoded
+    conic and opacity) and tile lists bit-exact, the packed words equal to the oracle's encoder
+    for every visible splat, the image > 60 dB vs the oracle rendering the same records and
+    > 75 / 70 dB vs the full-precision GPU image."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case packed render splats {mode}, 100k-720p")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    view = camera.orbit_view(8.0, 0.0, 0.0)
+    r, scene, cam, img, sc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 1280, 720, "tight", render_splat=mode)
+    gs, gc = r.read_back_splats()
+    _check_splats(gs, gc, os_, oc)
+    _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+    words = r.read_back_packed()
+    vis = oc > 0
+    assert np.array_equal(words[vis], oracle.encode_render_splats(os_[vis], mode))
+    ref, _ = oracle.render(sc, ocam, bounds=ob, render_splat=mode)
+    p_ref = oracle.psnr(img[..., :3], ref)
+    rf = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f")
+    full = rf.render_to_host(cam).copy()
+    rf.close()
+    p_full = oracle.psnr(img[..., :3], full[..., :3])
+    log(f"  PSNR vs oracle({mode}) {p_ref:.2f} dB, vs full GPU image {p_full:.2f} dB")
+    assert p_ref > 60.0
+    assert p_full > (75.0 if mode == "p32" else 70.0)
     r.close()
     scene.close()
 
