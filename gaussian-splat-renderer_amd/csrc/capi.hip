@@ -123,6 +123,7 @@ struct gsr_renderer {
     int32_t degree = 0;
     // Per gaussian (+ the packed render splats in the p32 / p24 modes).
     DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control, packed;
+    DeviceBuffer rayDepth;  // StopThePop: 3 float4 per gaussian
     // The list (ping-pong; keys are 32-bit tile ids in binning mode, 64-bit (tile, depth) in
     // global mode) and the sort's work areas.
     DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus;
@@ -467,6 +468,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (s.render_splat != GSR_RENDER_SPLAT_FULL && s.render_splat != GSR_RENDER_SPLAT_P32 &&
         s.render_splat != GSR_RENDER_SPLAT_P24)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown render_splat %d", int(s.render_splat));
+    if (s.sort_window > uint32_t(kMaxSortWindow))
+        return fail(GSR_ERROR_UNSUPPORTED, "gsr_renderer_create: sort_window %u (this build compiles windows 1..%d)",
+                    s.sort_window, kMaxSortWindow);
     if (s.sort_mode != GSR_SORT_MODE_BINNING && s.sort_mode != GSR_SORT_MODE_GLOBAL)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown sort_mode %d", int(s.sort_mode));
     if (s.sort_mode == GSR_SORT_MODE_BINNING && s.sort_algorithm == GSR_SORT_ONESWEEP)
@@ -495,6 +499,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     };
     alloc(r->splats, n * 48);
     if (s.render_splat != GSR_RENDER_SPLAT_FULL) alloc(r->packed, n * (s.render_splat == GSR_RENDER_SPLAT_P32 ? 32 : 24));
+    if (s.sort_window > 0) alloc(r->rayDepth, n * 48);
     alloc(r->tileCounts, n * 4);
     alloc(r->blockOffsets, gBlocks * 4);
     alloc(r->control, sizeof(gsr_control));
@@ -539,7 +544,7 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
-                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed};
+                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     return s;
@@ -594,7 +599,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     GSR_HIP(mark(0));
     launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
                      r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords,
-                     uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), stream);
+                     uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), r->rayDepth.as<float4>(), stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(1));
     launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, r->binning,
@@ -668,7 +673,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     launchRender(r->splats.as<float4>(), r->packed.as<uint32_t>(), r->settings.render_splat, sortedValues,
                  r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
                  r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0, r->settings.render_loop,
-                 outImage, stream);
+                 int(r->settings.sort_window), r->rayDepth.as<float4>(), outImage, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(7));
     r->lastCam = *cam;

@@ -16,6 +16,7 @@ constexpr float kFrustumClampFactor = 1.3f;
 constexpr float kMaxAlpha = 0.99f;
 constexpr float kMinAlpha = 1.0f / 255.0f;
 constexpr float kMinTransmittance = 1e-4f;
+constexpr int kMaxSortWindow = 16;  // StopThePop: largest per-pixel window this build compiles
 constexpr int kTileSize = 16;
 constexpr uint32_t kNoGaussian = 0xFFFFFFFFu;
 
@@ -90,7 +91,7 @@ size_t sortStatusBytes(uint32_t maxCount);
 // Launch wrappers (each in its .hip file).
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, hipStream_t stream);
+                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, hipStream_t stream);
 // scanVisible = false: blockOffsets = exclusive scan of the blocks' tile sums (global mode);
 // true: of their visible (non-zero count) gaussians (binning mode's compaction).
 void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
@@ -132,9 +133,11 @@ void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, 
                             const uint32_t* blockOffsets, const gsr_control* control, uint32_t maxVisible,
                             uint32_t bounds, uint32_t tilesX, uint32_t tilesY, uint32_t* keys, uint32_t* values,
                             gsr_control* controlRW, hipStream_t stream);
+// sortWindow > 0: StopThePop with that window (rayDepth: 3 float4 per gaussian), reference alpha.
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
-                  int pixelsPerThread, bool exactAlpha, int loop, void* out, hipStream_t stream);
+                  int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
+                  hipStream_t stream);
 
 // Stand-alone scan (GpuScan): reduce, block-sum scan, downsweep.
 constexpr int kScanThreads = 256;

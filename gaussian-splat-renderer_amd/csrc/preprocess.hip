@@ -91,7 +91,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                                   BlockStats* __restrict__ blockStats,
                                                                   uint32_t* __restrict__ rangesWords,
                                                                   uint32_t rangeWordCount, uint32_t renderSplat,
-                                                                  uint32_t* __restrict__ packed) {
+                                                                  uint32_t* __restrict__ packed,
+                                                                  float4* __restrict__ rayDepth) {
     const uint32_t i = blockIdx.x * kPreprocessThreads + threadIdx.x;
     if (i < rangeWordCount) rangesWords[i] = 0u;
 
@@ -254,6 +255,42 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                          ra, rb, rc, ro);
                         touched = coverageCount(cv);
                     }
+                    if (rayDepth) {
+                        // StopThePop (computeRayDepthParams, splat_math.cpp:195-205): P = Sigma^-1 in
+                        // camera space = sum_k w_k M_k M_k^T with M = mat3(view) R, w = 1 / s^2,
+                        // s = max(sigma, max(1e-4 smax, 1e-12)); b = P t.
+                        const float Rm[3][3] = {{R00, R01, R02}, {R10, R11, R12}, {R20, R21, R22}};
+                        float Mc[3][3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+#pragma unroll
+                            for (int r = 0; r < 3; ++r)
+                                Mc[c][r] = v[0 * 4 + r] * Rm[c][0] + v[1 * 4 + r] * Rm[c][1] + v[2 * 4 + r] * Rm[c][2];
+                        const float smax = selMax(sx, selMax(sy, sz));
+                        const float floorS = selMax(1e-4f * smax, 1e-12f);
+                        const float sig[3] = {sx, sy, sz};
+                        float w[3];
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            const float sk = sig[k] < floorS ? floorS : sig[k];
+                            w[k] = 1.0f / (sk * sk);
+                        }
+                        float P[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+#pragma unroll
+                        for (int k = 0; k < 3; ++k)
+#pragma unroll
+                            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                                for (int b = 0; b < 3; ++b) P[a][b] = P[a][b] + w[k] * (Mc[k][b] * Mc[k][a]);
+                        const float tc[3] = {t0, t1, t2};
+                        float bv[3];
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) bv[r] = P[0][r] * tc[0] + P[1][r] * tc[1] + P[2][r] * tc[2];
+                        float4* rd = rayDepth + 3 * size_t(i);
+                        rd[0] = make_float4(P[0][0], P[0][1], P[0][2], P[1][1]);
+                        rd[1] = make_float4(P[1][2], P[2][2], bv[0], bv[1]);
+                        rd[2] = make_float4(bv[2], 0.0f, 0.0f, 0.0f);
+                    }
                     if (renderSplat != GSR_RENDER_SPLAT_FULL) {
                         const uint32_t rad = uint32_t(radius < 65535.0f ? radius : 65535.0f);
                         const uint32_t rg = halfBits(o2.x) | (halfBits(o2.y) << 16);
@@ -312,17 +349,18 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
 
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, hipStream_t stream) {
+                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, hipStream_t stream) {
     uint32_t n = scene.count > rangeWordCount ? scene.count : rangeWordCount;
     const uint32_t blocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene,
                            cam, bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat,
-                           packed);
+                           packed, rayDepth);
     else
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
-                           bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat, packed);
+                           bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat, packed,
+                           rayDepth);
 }
 
 }  // namespace gsr

@@ -400,6 +400,208 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
     }
 }
 
+// StopThePop (tile_render.comp:201-279, GSR_STOPTHEPOP; CPU twin reference_renderer.cpp:143-196):
+// the reference's alpha (expf, its operation order) and blend rule, applied in the order of a
+// per-pixel window of K slots sorted by the depth of each splat's maximum along the pixel's ray,
+// t* = (v . b) / (v^T P v), v = ((px - (W - 1) / 2) / fx, (py - (H - 1) / 2) / fy, 1). When the
+// window is full the nearest of (window + new) is blended and leaves; at the end the window is
+// blended nearest first. The window lives in registers (constant indices, K a template).
+template <int K>
+struct StpWindow {
+    float depth[K], alpha[K], c0[K], c1[K], c2[K];
+};
+
+template <int K>
+__device__ __forceinline__ void stpBlend(float a, float r, float g, float b, float& T, float& C0, float& C1, float& C2,
+                                         bool& done) {
+    const float testT = T * (1.0f - a);
+    if (testT < kMinTransmittance) {
+        done = true;
+        return;
+    }
+    C0 += r * a * T;
+    C1 += g * a * T;
+    C2 += b * a * T;
+    T = testT;
+}
+
+template <int K>
+__device__ __forceinline__ void stpPush(StpWindow<K>& w, float depth, float a, float r, float g, float b, float& T,
+                                        float& C0, float& C1, float& C2, bool& done) {
+    const float kEmpty = __builtin_inff();
+    if (!(depth < 3.0e38f)) depth = 3.0e38f;  // finite (NaN/Inf from a degenerate splat)
+    if (w.depth[K - 1] == kEmpty) {  // not full: insertion from the back
+        bool placed = false;
+#pragma unroll
+        for (int i = K - 1; i > 0; --i) {
+            if (!placed) {
+                if (w.depth[i - 1] > depth) {
+                    w.depth[i] = w.depth[i - 1];
+                    w.alpha[i] = w.alpha[i - 1];
+                    w.c0[i] = w.c0[i - 1];
+                    w.c1[i] = w.c1[i - 1];
+                    w.c2[i] = w.c2[i - 1];
+                } else {
+                    w.depth[i] = depth;
+                    w.alpha[i] = a;
+                    w.c0[i] = r;
+                    w.c1[i] = g;
+                    w.c2[i] = b;
+                    placed = true;
+                }
+            }
+        }
+        if (!placed) {
+            w.depth[0] = depth;
+            w.alpha[0] = a;
+            w.c0[0] = r;
+            w.c1[0] = g;
+            w.c2[0] = b;
+        }
+        return;
+    }
+    if (depth < w.depth[0]) {  // full: the new splat is the nearest
+        stpBlend<K>(a, r, g, b, T, C0, C1, C2, done);
+        return;
+    }
+    stpBlend<K>(w.alpha[0], w.c0[0], w.c1[0], w.c2[0], T, C0, C1, C2, done);
+    bool placed = false;
+#pragma unroll
+    for (int i = 0; i + 1 < K; ++i) {
+        if (!placed) {
+            if (w.depth[i + 1] <= depth) {
+                w.depth[i] = w.depth[i + 1];
+                w.alpha[i] = w.alpha[i + 1];
+                w.c0[i] = w.c0[i + 1];
+                w.c1[i] = w.c1[i + 1];
+                w.c2[i] = w.c2[i + 1];
+            } else {
+                w.depth[i] = depth;
+                w.alpha[i] = a;
+                w.c0[i] = r;
+                w.c1[i] = g;
+                w.c2[i] = b;
+                placed = true;
+            }
+        }
+    }
+    if (!placed) {
+        w.depth[K - 1] = depth;
+        w.alpha[K - 1] = a;
+        w.c0[K - 1] = r;
+        w.c1[K - 1] = g;
+        w.c2[K - 1] = b;
+    }
+}
+
+template <int FORMAT, int RS, int K>
+__global__ __launch_bounds__(256) void kRenderStp(const float4* __restrict__ splats, const uint32_t* __restrict__ packed,
+                                                  const float4* __restrict__ rayDepth,
+                                                  const uint32_t* __restrict__ values, const uint2* __restrict__ ranges,
+                                                  uint32_t width, uint32_t height, uint32_t tilesX, uint32_t tileCount,
+                                                  uint32_t gaussianCount, float focalX, float focalY,
+                                                  void* __restrict__ out) {
+    __shared__ float4 sS0[kBatch];  // cx, cy, a, b
+    __shared__ float2 sS1[kBatch];  // c, o
+    __shared__ float4 sColor[kBatch];
+    __shared__ float4 sRayA[kBatch];  // p00, p01, p02, p11
+    __shared__ float4 sRayB[kBatch];  // p12, p22, b0, b1
+    __shared__ float sRayC[kBatch];   // b2
+
+    const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
+    const uint32_t row = xcd + kXcds * (idx / tilesX);
+    const uint32_t tile = row * tilesX + idx % tilesX;
+    if (tile >= tileCount) return;  // uniform
+    const uint32_t t = threadIdx.x;
+    const uint32_t px = (tile % tilesX) * 16 + (t % 16);
+    const uint32_t py = (tile / tilesX) * 16 + (t / 16);
+    const float pxf = float(px), pyf = float(py);
+    const float rx = (pxf - 0.5f * (float(width) - 1.0f)) / focalX;
+    const float ry = (pyf - 0.5f * (float(height) - 1.0f)) / focalY;
+    bool done = !(px < width && py < height);
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+    StpWindow<K> w;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        w.depth[i] = __builtin_inff();
+        w.alpha[i] = w.c0[i] = w.c1[i] = w.c2[i] = 0.0f;
+    }
+
+    const uint2 range = ranges[tile];
+    for (uint32_t batch = range.x; batch < range.y; batch += kBatch) {
+        if (__syncthreads_count(done ? 1 : 0) == 256) break;
+        const uint32_t e = batch + t;
+        if (e < range.y) {
+            uint32_t g = values[e];
+            g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
+            float cx, cy;
+            float4 co, col;
+            fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
+            sS0[t] = make_float4(cx, cy, co.x, co.y);
+            sS1[t] = make_float2(co.z, co.w);
+            sColor[t] = col;
+            const float4* rd = rayDepth + 3 * size_t(g);
+            sRayA[t] = rd[0];
+            sRayB[t] = rd[1];
+            sRayC[t] = rd[2].x;
+        }
+        __syncthreads();
+        const uint32_t n = range.y - batch < uint32_t(kBatch) ? range.y - batch : uint32_t(kBatch);
+        for (uint32_t j = 0; !done && j < n; ++j) {
+            const float4 s0 = sS0[j];
+            const float2 s1 = sS1[j];
+            const float dx = s0.x - pxf, dy = s0.y - pyf;
+            const float power = -0.5f * (s0.z * dx * dx + s1.x * dy * dy) - s0.w * dx * dy;
+            if (power > 0.0f) continue;
+            const float ex = s1.y * expf(power);
+            const float a = kMaxAlpha < ex ? kMaxAlpha : ex;
+            if (a < kMinAlpha) continue;
+            const float4 pa = sRayA[j], pb = sRayB[j];
+            const float vPv = pa.x * rx * rx + 2.0f * pa.y * rx * ry + 2.0f * pa.z * rx + pa.w * ry * ry +
+                              2.0f * pb.x * ry + pb.y;
+            const float vb = pb.z * rx + pb.w * ry + sRayC[j];
+            const float4 col = sColor[j];
+            stpPush<K>(w, vb / vPv, a, col.x, col.y, col.z, T, C0, C1, C2, done);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+        if (!done && w.depth[i] != __builtin_inff()) stpBlend<K>(w.alpha[i], w.c0[i], w.c1[i], w.c2[i], T, C0, C1, C2, done);
+    if (px < width && py < height) {
+        const size_t p = size_t(py) * width + px;
+        if (FORMAT == GSR_TARGET_RGBA8) {
+            const uint32_t rgba = quantize8(C0) | (quantize8(C1) << 8) | (quantize8(C2) << 16) |
+                                  (quantize8(1.0f - T) << 24);
+            static_cast<uint32_t*>(out)[p] = rgba;
+        } else {
+            static_cast<float4*>(out)[p] = make_float4(C0, C1, C2, 1.0f - T);
+        }
+    }
+}
+
+template <int FORMAT, int RS, int K>
+void launchStpK(const float4* splats, const uint32_t* packed, const float4* rayDepth, const uint32_t* values,
+                const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+    const uint32_t tileCount = cam.tilesX * cam.tilesY;
+    const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
+    hipLaunchKernelGGL((kRenderStp<FORMAT, RS, K>), dim3(rowsPerXcd * cam.tilesX * kXcds), dim3(256), 0, stream, splats,
+                       packed, rayDepth, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount,
+                       cam.focalX, cam.focalY, out);
+}
+
+template <int FORMAT, int RS, int K = 1>
+void launchStp(int window, const float4* splats, const uint32_t* packed, const float4* rayDepth,
+               const uint32_t* values, const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out,
+               hipStream_t stream) {
+    if constexpr (K <= kMaxSortWindow) {
+        if (window == K)
+            launchStpK<FORMAT, RS, K>(splats, packed, rayDepth, values, ranges, cam, gaussianCount, out, stream);
+        else
+            launchStp<FORMAT, RS, K + 1>(window, splats, packed, rayDepth, values, ranges, cam, gaussianCount, out,
+                                         stream);
+    }
+}
+
 template <int FORMAT, int RS>
 void launchCulled(const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
                   const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
@@ -445,9 +647,21 @@ void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uin
 }
 
 template <int FORMAT>
-void launchSplat(int renderSplat, int ppt, bool exact, int loop, const float4* splats, const uint32_t* packed,
-                 const uint32_t* values, const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out,
-                 hipStream_t stream) {
+void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow, const float4* rayDepth,
+                 const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
+                 const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+    if (sortWindow > 0) {
+        if (renderSplat == GSR_RENDER_SPLAT_P32)
+            launchStp<FORMAT, GSR_RENDER_SPLAT_P32>(sortWindow, splats, packed, rayDepth, values, ranges, cam,
+                                                    gaussianCount, out, stream);
+        else if (renderSplat == GSR_RENDER_SPLAT_P24)
+            launchStp<FORMAT, GSR_RENDER_SPLAT_P24>(sortWindow, splats, packed, rayDepth, values, ranges, cam,
+                                                    gaussianCount, out, stream);
+        else
+            launchStp<FORMAT, GSR_RENDER_SPLAT_FULL>(sortWindow, splats, packed, rayDepth, values, ranges, cam,
+                                                     gaussianCount, out, stream);
+        return;
+    }
     if (renderSplat == GSR_RENDER_SPLAT_P32)
         launchFormat<FORMAT, GSR_RENDER_SPLAT_P32>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
                                                    out, stream);
@@ -463,14 +677,15 @@ void launchSplat(int renderSplat, int ppt, bool exact, int loop, const float4* s
 
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
-                  int pixelsPerThread, bool exactAlpha, int loop, void* out, hipStream_t stream) {
+                  int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
+                  hipStream_t stream) {
     if (cam.tilesX * cam.tilesY == 0) return;
     if (format == GSR_TARGET_RGBA8)
-        launchSplat<GSR_TARGET_RGBA8>(renderSplat, pixelsPerThread, exactAlpha, loop, splats, packed, values, ranges,
-                                      cam, gaussianCount, out, stream);
+        launchSplat<GSR_TARGET_RGBA8>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
+                                      packed, values, ranges, cam, gaussianCount, out, stream);
     else
-        launchSplat<GSR_TARGET_RGBA32F>(renderSplat, pixelsPerThread, exactAlpha, loop, splats, packed, values, ranges,
-                                        cam, gaussianCount, out, stream);
+        launchSplat<GSR_TARGET_RGBA32F>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
+                                        packed, values, ranges, cam, gaussianCount, out, stream);
 }
 
 }  // namespace gsr

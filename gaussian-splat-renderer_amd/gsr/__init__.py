@@ -101,7 +101,8 @@ class Settings(ctypes.Structure):
         ("sort_mode", ctypes.c_int32),
         ("full_key_sort", ctypes.c_int32),
         ("render_splat", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("sort_window", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -459,7 +460,8 @@ class Renderer:
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
                  max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
                  exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
-                 sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full"):
+                 sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full",
+                 sort_window: int = 0):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -473,6 +475,7 @@ class Renderer:
         s.sort_mode = SORT_MODES[sort_mode]
         s.full_key_sort = 1 if full_key_sort else 0
         s.render_splat = RENDER_SPLATS[render_splat]
+        s.sort_window = sort_window
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds

@@ -137,7 +137,12 @@ typedef struct gsr_settings {
                                     per-tile sort orders each tile's range by (depth, index); 1 = the radix sort
                                     orders all (tile, depth) key bits. Same list, bit for bit. */
     int32_t render_splat;        /* GSR_RENDER_SPLAT_*: the record the render pass gathers per list entry */
-    uint32_t reserved[3];
+    uint32_t sort_window;        /* StopThePop per-pixel re-sorting window K (TileSettings::sortWindow,
+                                    splat_tile_renderer.h:108): 0 = INRIA order; 1..16 = blend in the order
+                                    of a K-slot window sorted by the depth of each splat's maximum along the
+                                    pixel's ray (tile_render.comp:201-279), reference alpha. K > 16:
+                                    GSR_ERROR_UNSUPPORTED (the reference allows 64) */
+    uint32_t reserved[2];
 } gsr_settings;
 
 /* RenderSplatMode (splat_tile_renderer.h:88-92; render_splat.glsl:1-21), same values. The

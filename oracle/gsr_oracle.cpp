@@ -331,4 +331,38 @@ void applyRenderSplat(uint32_t mode, Splat& s, bool colour) {
         for (int c = 0; c < 3; ++c) s.color[c] = viaHalf(s.color[c]);
 }
 
+// --- StopThePop ray depth (splat_math.cpp:195-205) ----------------------------------------------
+
+RayDepth computeRayDepth(const float sigma[3], const float q[4], const float tCam[3], const float view[4][4]) {
+    const M3<float> R = quaternionToMatrix<float>(q);
+    float M[3][3];  // M[c][r]: column c = the gaussian's axis c in camera space
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r)
+            M[c][r] = view[0][r] * R.m[c][0] + view[1][r] * R.m[c][1] + view[2][r] * R.m[c][2];
+    const float smax = std::max(sigma[0], std::max(sigma[1], sigma[2]));
+    const float floorS = std::max(1e-4f * smax, 1e-12f);
+    float w[3];
+    for (int k = 0; k < 3; ++k) {
+        const float sk = sigma[k] < floorS ? floorS : sigma[k];
+        w[k] = 1.0f / (sk * sk);
+    }
+    float P[3][3] = {};
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) P[i][j] = P[i][j] + w[k] * (M[k][j] * M[k][i]);
+    float b[3];
+    for (int r = 0; r < 3; ++r) b[r] = P[0][r] * tCam[0] + P[1][r] * tCam[1] + P[2][r] * tCam[2];
+    RayDepth d;
+    d.p00 = P[0][0];
+    d.p01 = P[0][1];
+    d.p02 = P[0][2];
+    d.p11 = P[1][1];
+    d.p12 = P[1][2];
+    d.p22 = P[2][2];
+    d.b0 = b[0];
+    d.b1 = b[1];
+    d.b2 = b[2];
+    return d;
+}
+
 }  // namespace oracle

@@ -406,6 +406,20 @@ uint32_t radixSortPairs(std::vector<uint32_t>& keys, std::vector<uint32_t>& valu
 uint16_t floatToHalf(float value);
 float halfToFloat(uint16_t h);
 
+// --- StopThePop ray depth (splat_math.h:134-143, splat_math.cpp:195-205) --------------------
+// P = Sigma^-1 in INRIA camera space = sum_k w_k M_k M_k^T, M = mat3(view) R (columns: the
+// gaussian's axes), w = 1 / s^2 with s = max(sigma, max(1e-4 smax, 1e-12)); b = P t. The depth of
+// the gaussian's maximum along the ray t (vx, vy, 1) is (v . b) / (v^T P v).
+struct RayDepth {
+    float p00 = 0, p01 = 0, p02 = 0, p11 = 0, p12 = 0, p22 = 0, b0 = 0, b1 = 0, b2 = 0;
+    float depthAt(float vx, float vy) const {
+        const float vPv = p00 * vx * vx + 2.0f * p01 * vx * vy + 2.0f * p02 * vx + p11 * vy * vy + 2.0f * p12 * vy + p22;
+        const float vb = b0 * vx + b1 * vy + b2;
+        return vb / vPv;
+    }
+};
+RayDepth computeRayDepth(const float sigma[3], const float q[4], const float tCam[3], const float view[4][4]);
+
 // --- Packed render splats (render_splat.h:1-69, render_splat.glsl:1-75) --------------------
 // p32 (8 words): cx, cy, a, b | c, opacity, h(r) | h(g) << 16, h(b) | radius << 16.
 // p24 (6 words): cx, cy | h(a S) | h(b S) << 16, h(c S) | h(opacity) << 16 | colour as p32,

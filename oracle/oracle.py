@@ -31,11 +31,12 @@ SPLAT_DTYPE = np.dtype([("center", np.float32, 2), ("depth", np.float32), ("cull
                         ("conic_opacity", np.float32, 4), ("color", np.float32, 3), ("radius", np.float32)])
 RECT, TIGHT, CUTOFF = 0, 1, 2
 RENDER_SPLAT = {"full": 0, "p32": 1, "p24": 2}
+EXACT_PIXEL_SORT = 0xFFFFFFFF
 
 
 class RenderOptions(ctypes.Structure):
     _fields_ = [("maxShDegree", _i32), ("earlyTermination", _i32), ("bound", _u32), ("bounds", _u32),
-                ("threads", _i32), ("background", _f32 * 3), ("renderSplat", _u32)]
+                ("threads", _i32), ("background", _f32 * 3), ("renderSplat", _u32), ("sortWindow", _u32)]
 
 
 class Stats(ctypes.Structure):
@@ -71,6 +72,7 @@ _alpha_at = _sig("oracle_splat_alpha_at", _f32, [_vp, _f32, _f32])
 _preprocess = _sig("oracle_preprocess", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32, _f32, _i32,
                                                _u32, _u32, _vp, _vp, _i32])
 _encode_render_splats = _sig("oracle_encode_render_splats", None, [_u32, _vp, _u32, _vp])
+_ray_depth = _sig("oracle_ray_depth", None, [_vp, _vp, _vp, _vp, _vp])
 _preprocess_f64 = _sig("oracle_preprocess_f64", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32,
                                                        _f32, _i32, _vp, _i32])
 _coverage_tiles = _sig("oracle_coverage_tiles", _u32, [_vp, _u32, _i32, _u32, _u32, _i32, _i32, _vp, _u32, _vp])
@@ -186,6 +188,14 @@ def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGH
     return out, counts
 
 
+def ray_depth(sigma, q, t_cam, view_inria):
+    """computeRayDepthParams: (p00, p01, p02, p11, p12, p22, b0, b1, b2) as float32."""
+    a = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in (sigma, q, t_cam, view_inria)]
+    out = np.zeros(9, np.float32)
+    _ray_depth(*(_p(x) for x in a), _p(out))
+    return out
+
+
 def encode_render_splats(splats, mode: str):
     """encodeRenderSplat (render_splat.h:29-48): (N, 8) words for "p32", (N, 6) for "p24"."""
     splats = np.ascontiguousarray(splats, SPLAT_DTYPE)
@@ -264,13 +274,14 @@ def decode_f16(geom, color, sh_degree):
 
 
 def render(scene, cam: OracleCamera, max_sh_degree=3, early_termination=True, bound=0, bounds=RECT, threads=None,
-           background=(0.0, 0.0, 0.0), want_lists=False, render_splat="full"):
+           background=(0.0, 0.0, 0.0), want_lists=False, render_splat="full", sort_window=0):
     """renderReference: (H, W, 3) float32 image (unclamped) and ReferenceStats (+ per-tile lists).
     render_splat "p32"/"p24": every splat as the packed record gives it back (fp16 colour; p24
-    also fp16 conic and opacity)."""
+    also fp16 conic and opacity). sort_window K > 0: StopThePop's per-pixel window of K splats
+    ordered by ray depth; EXACT_PIXEL_SORT: every pixel sorts all its splats."""
     threads = default_threads() if threads is None else threads
     opt = RenderOptions(max_sh_degree, 1 if early_termination else 0, bound, bounds, threads,
-                        (ctypes.c_float * 3)(*background), RENDER_SPLAT[render_splat])
+                        (ctypes.c_float * 3)(*background), RENDER_SPLAT[render_splat], sort_window)
     rgb = np.zeros((cam.height, cam.width, 3), np.float32)
     st = Stats()
     tiles = cam.tiles_x * cam.tiles_y

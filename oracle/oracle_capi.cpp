@@ -67,6 +67,7 @@ struct oracle_render_options {
     int32_t threads;           // 0 = hardware_concurrency
     float background[3];
     uint32_t renderSplat;      // 0 full, 1 p32, 2 p24: render what the packed record holds
+    uint32_t sortWindow;       // StopThePop window K (0 = INRIA order; 0xFFFFFFFF = exact per-pixel sort)
 };
 
 struct oracle_stats {  // ReferenceStats (reference_renderer.h:51-62)
@@ -345,6 +346,17 @@ void oracle_decode_f16(uint32_t n, const uint32_t* geom, const uint16_t* color, 
 // renderReference (reference_renderer.cpp:41-234), without the StopThePop window. rgb:
 // W*H*3 floats, row 0 = top, unclamped. tileStart/tileLists (optional): the per-tile lists
 // in depth order (tiles + 1 and tileInstances entries; pass null to skip).
+// computeRayDepthParams for one gaussian: out = p00, p01, p02, p11, p12, p22, b0, b1, b2.
+void oracle_ray_depth(const float sigma[3], const float q[4], const float tCam[3], const float viewInria[16],
+                      float out[9]) {
+    float view[4][4];
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) view[c][r] = viewInria[c * 4 + r];
+    const RayDepth d = computeRayDepth(sigma, q, tCam, view);
+    const float v[9] = {d.p00, d.p01, d.p02, d.p11, d.p12, d.p22, d.b0, d.b1, d.b2};
+    for (int i = 0; i < 9; ++i) out[i] = v[i];
+}
+
 // encodeRenderSplat over n splats: 8 (p32) or 6 (p24) words each.
 void oracle_encode_render_splats(uint32_t n, const void* splats, uint32_t mode, uint32_t* words) {
     const Splat* sp = static_cast<const Splat*>(splats);
@@ -380,6 +392,19 @@ void oracle_render(uint32_t n, const float* posAlpha, const float* scales, const
             case kOffScreen: stats.culledOffScreen++; break;
             default: break;
         }
+    }
+    // StopThePop: ray-depth parameters of every visible gaussian (reference_renderer.cpp:61-70).
+    std::vector<RayDepth> rayDepth;
+    if (opt->sortWindow > 0) {
+        rayDepth.resize(n);
+        parallelFor(n, threads, 4096, [&](uint32_t i) {
+            if (splats[i].cullReason != kVisible) return;
+            const float* m = posAlpha + size_t(i) * 4;
+            float t[3];
+            for (int r = 0; r < 3; ++r)
+                t[r] = cam.view[0][r] * m[0] + cam.view[1][r] * m[1] + cam.view[2][r] * m[2] + cam.view[3][r];
+            rayDepth[i] = computeRayDepth(scales + size_t(i) * 4, rot + size_t(i) * 4, t, cam.view);
+        });
     }
     const auto tPre = Clock::now();
 
@@ -429,7 +454,7 @@ void oracle_render(uint32_t n, const float* posAlpha, const float* scales, const
     }
     const auto tBin = Clock::now();
 
-    // 4) Rasterize, one tile per task (reference_renderer.cpp:118-222, sortWindow = 0).
+    // 4) Rasterize, one tile per task (reference_renderer.cpp:118-222).
     std::atomic<uint64_t> evaluated{0}, blended{0}, terminated{0};
     const bool radiusBound = opt->bound == kBoundRadius;
     parallelFor(tileCount, threads, 1, [&](uint32_t tile) {
@@ -437,6 +462,13 @@ void oracle_render(uint32_t n, const float* posAlpha, const float* scales, const
         std::vector<Splat> local;
         local.reserve(tileStart[tile + 1] - tileStart[tile]);
         for (uint32_t j = tileStart[tile]; j < tileStart[tile + 1]; ++j) local.push_back(splats[tileLists[j]]);
+        std::vector<RayDepth> localDepth;
+        if (opt->sortWindow > 0)
+            for (uint32_t j = tileStart[tile]; j < tileStart[tile + 1]; ++j) localDepth.push_back(rayDepth[tileLists[j]]);
+        struct Pending {
+            float depth, alpha, color[3];
+        };
+        std::vector<Pending> window;
         uint64_t tEval = 0, tBlend = 0, tTerm = 0;
         const uint32_t x0 = tx * kTileSize, y0 = ty * kTileSize;
         const uint32_t x1 = std::min(cam.width, x0 + kTileSize), y1 = std::min(cam.height, y0 + kTileSize);
@@ -445,6 +477,65 @@ void oracle_render(uint32_t n, const float* posAlpha, const float* scales, const
                 const float px = float(x), py = float(y);
                 float T = 1.0f;
                 float C[3] = {0.0f, 0.0f, 0.0f};
+                if (opt->sortWindow > 0) {
+                    // StopThePop (reference_renderer.cpp:143-196): the same blend rule, in the
+                    // order of a K-slot window sorted by the depth of each splat's maximum along
+                    // the pixel's ray; the nearest of (window + new) leaves the window.
+                    const float vx = (px - 0.5f * (float(cam.width) - 1.0f)) / cam.focalX;
+                    const float vy = (py - 0.5f * (float(cam.height) - 1.0f)) / cam.focalY;
+                    bool done = false;
+                    auto blend = [&](float alpha, const float* col) {
+                        const float testT = T * (1.0f - alpha);
+                        if (testT < kMinTransmittance) {
+                            done = true;
+                            ++tTerm;
+                            return;
+                        }
+                        for (int c = 0; c < 3; ++c) C[c] += col[c] * alpha * T;
+                        T = testT;
+                        ++tBlend;
+                    };
+                    const bool exact = opt->sortWindow == 0xFFFFFFFFu;
+                    const size_t K = opt->sortWindow;
+                    window.clear();
+                    for (size_t j = 0; j < local.size() && !done; ++j) {
+                        const Splat& s = local[j];
+                        if (radiusBound) {
+                            const float fx = px + 0.5f, fy = py + 0.5f;
+                            const float cx = s.center[0] + 0.5f, cy = s.center[1] + 0.5f;
+                            if (!(fx >= cx - s.radius && fx < cx + s.radius && fy >= cy - s.radius && fy < cy + s.radius))
+                                continue;
+                        }
+                        ++tEval;
+                        const float alpha = splatAlphaAt(s, px, py);
+                        if (alpha == 0.0f) continue;
+                        const Pending e{localDepth[j].depthAt(vx, vy), alpha, {s.color[0], s.color[1], s.color[2]}};
+                        if (exact) {
+                            window.push_back(e);
+                        } else if (window.size() == K) {
+                            if (e.depth < window[0].depth) {
+                                blend(e.alpha, e.color);
+                                continue;
+                            }
+                            blend(window[0].alpha, window[0].color);
+                            size_t i = 0;
+                            for (; i + 1 < K && window[i + 1].depth <= e.depth; ++i) window[i] = window[i + 1];
+                            window[i] = e;
+                        } else {
+                            size_t i = window.size();
+                            window.push_back(e);
+                            for (; i > 0 && window[i - 1].depth > e.depth; --i) window[i] = window[i - 1];
+                            window[i] = e;
+                        }
+                    }
+                    if (exact)
+                        std::stable_sort(window.begin(), window.end(),
+                                         [](const Pending& a, const Pending& b) { return a.depth < b.depth; });
+                    for (size_t i = 0; i < window.size() && !done; ++i) blend(window[i].alpha, window[i].color);
+                    float* dst = &rgb[(size_t(y) * cam.width + x) * 3];
+                    for (int c = 0; c < 3; ++c) dst[c] = C[c] + T * opt->background[c];
+                    continue;
+                }
                 for (const Splat& s : local) {
                     if (radiusBound) {
                         // insideRadiusSquare (splat_math.h:166-170)

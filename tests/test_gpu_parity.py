@@ -259,6 +259,35 @@ def test_packed_render_splats(gsr_ctx, mode):
     scene.close()
 
 
+@pytest.mark.parametrize("window", [1, 4, 16])
+def test_stopthepop_window(gsr_ctx, window):
+    """StopThePop window K (test_tight_bounds.cpp:472-481): the GPU image against the oracle with
+    the same window > 60 dB (the lists are the INRIA-order lists, bit-exact); K > 16 is refused."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case StopThePop K={window}, test scene 250k 640x360")
+    sc = scenes.make_test_scene(250_000, seed=99)
+    view = camera.look_at([0.3, 0.5, 6.0], [0.0, 0.0, 0.0])
+    cam = _camera(gsr, view, 640, 360)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", sort_window=window)
+    img = r.render_to_host(cam).copy()
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    os_, oc = oracle.preprocess(sc, ocam, bounds=oracle.TIGHT)
+    _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, oracle.TIGHT)
+    ref, _ = oracle.render(sc, ocam, bounds=oracle.TIGHT, sort_window=window)
+    inria, _ = oracle.render(sc, ocam, bounds=oracle.TIGHT)
+    p, p_inria = oracle.psnr(img[..., :3], ref), oracle.psnr(img[..., :3], inria)
+    log(f"  vs oracle same window {p:.2f} dB (vs INRIA order {p_inria:.2f} dB)")
+    assert p > 60.0 and p > p_inria
+    r.close()
+    with pytest.raises(gsr.GsrError, match="sort_window"):
+        gsr.Renderer(gsr_ctx, scene, sort_window=17)
+    scene.close()
+
+
 def test_f16_scene_bit_exact(gsr_ctx):
     """The fp16 records: the GPU decode equals decodeGaussian, so splats and lists stay bit-exact."""
     import oracle
