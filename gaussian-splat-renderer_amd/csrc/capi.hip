@@ -78,6 +78,25 @@ struct DeviceBuffer {
     }
     // Grows (contents not kept).
     hipError_t ensure(size_t n) { return bytes >= n && ptr ? hipSuccess : allocate(n); }
+    // Stream-ordered allocation and release (the adaptive list's buffers): usable by work
+    // enqueued after the allocation on `stream`, freed after the work enqueued before the release.
+    hipError_t allocateAsync(size_t n, hipStream_t stream) {
+        release();
+        const size_t b = std::max<size_t>(n, 256);
+        hipError_t e = hipMallocAsync(&ptr, b, stream);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        bytes = b;
+        return hipSuccess;
+    }
+    hipError_t releaseAsync(hipStream_t stream) {
+        hipError_t e = ptr ? hipFreeAsync(ptr, stream) : hipSuccess;
+        ptr = nullptr;
+        bytes = 0;
+        return e;
+    }
     template <typename T>
     T* as() const {
         return static_cast<T*>(ptr);
@@ -132,6 +151,20 @@ struct gsr_renderer {
     DeviceBuffer depthKeysA, depthKeysB, depthValuesA, depthValuesB, sortedCounts, visibleSums;
     bool binning = false;
     uint32_t sortEpoch = 0;
+    // Adaptive list (splat_tile_renderer.cpp:444-486) and the per-frame statistics read back
+    // without stalling: each frame's control block is copied to pinned host memory and an event
+    // marks it; the next frames collect the completed copies.
+    uint32_t maxCapacity = 0, listResizes = 0;
+    uint64_t frameIndex = 0;
+    static constexpr int kStatSlots = 4;
+    gsr_control* statHost = nullptr;  // kStatSlots pinned control blocks
+    hipEvent_t statEvent[kStatSlots] = {};
+    uint64_t statFrame[kStatSlots] = {};
+    bool statPending[kStatSlots] = {};
+    gsr_control lastStats{};
+    uint64_t lastStatsFrame = 0, lastAdaptedFrame = ~uint64_t(0);
+    bool lastStatsValid = false;
+    uint32_t shrinkFrames = 0, shrinkWindowMax = 0;
     // Per tile (sized for the largest camera seen).
     DeviceBuffer ranges;
     uint32_t rangesTiles = 0;
@@ -447,6 +480,89 @@ static uint32_t capacityForBytes(double bytes, bool binning) {
     return e - e % 2048u;
 }
 
+static uint32_t roundUpToBlock(uint64_t n) {
+    const uint64_t b = kSortTile;
+    return uint32_t(std::min<uint64_t>((n + b - 1) / b * b, 0xFFFFF800ull));
+}
+
+// The list buffers for `capacity` entries: keys and values ping-pong (+ the one-sweep status
+// words), zeroed so that the lists start as valid keys and gaussian indices. Adaptive lists are
+// stream-ordered allocations, so a resize never waits for the frames in flight.
+static hipError_t allocateList(gsr_renderer* r, uint32_t capacity, hipStream_t stream) {
+    const size_t keyBytes = r->binning ? 4 : 8;
+    const bool async = r->settings.adaptive_list != 0;
+    hipError_t e = hipSuccess;
+    for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB, &r->sortStatus}) {
+        const size_t bytes = b == &r->sortStatus ? sortStatusBytes(capacity)
+                             : (b == &r->keysA || b == &r->keysB) ? size_t(capacity) * keyBytes
+                                                                  : size_t(capacity) * 4;
+        if (e == hipSuccess) e = async ? b->allocateAsync(bytes, stream) : b->allocate(bytes);
+        if (e == hipSuccess) e = async ? hipMemsetAsync(b->ptr, 0, b->bytes, stream) : hipMemset(b->ptr, 0, b->bytes);
+    }
+    if (e == hipSuccess) r->capacity = capacity;
+    return e;
+}
+
+// Collects the frame statistics whose copies have completed (lastStats = the newest).
+static void collectStats(gsr_renderer* r) {
+    for (int s = 0; s < gsr_renderer::kStatSlots; ++s) {
+        if (!r->statPending[s] || hipEventQuery(r->statEvent[s]) != hipSuccess) continue;
+        r->statPending[s] = false;
+        if (r->lastStatsValid && r->statFrame[s] < r->lastStatsFrame) continue;
+        r->lastStats = r->statHost[s];
+        r->lastStatsFrame = r->statFrame[s];
+        r->lastStatsValid = true;
+    }
+}
+
+// adaptCapacity (splat_tile_renderer.cpp:444-486): grow at once to 1.5 M when M > 0.9 capacity
+// (M is the unclamped entry count, the true need even for a frame that overflowed); shrink to
+// 1.5 x the window's largest M after 120 consecutive frames below 0.4 capacity; never below
+// min(256k, max), never above the budget, and not for a shrink of less than a quarter.
+static hipError_t adaptCapacity(gsr_renderer* r, hipStream_t stream) {
+    if (!r->settings.adaptive_list || !r->lastStatsValid || r->lastStatsFrame == r->lastAdaptedFrame) return hipSuccess;
+    r->lastAdaptedFrame = r->lastStatsFrame;
+    const uint32_t m = r->lastStats.duplicates;
+    const uint32_t cap = r->capacity;
+    uint64_t target = 0;
+    if (double(m) > 0.9 * cap) {
+        target = uint64_t(1.5 * m);
+        r->shrinkFrames = 0;
+        r->shrinkWindowMax = 0;
+    } else if (double(m) < 0.4 * cap) {
+        r->shrinkWindowMax = std::max(r->shrinkWindowMax, m);
+        if (++r->shrinkFrames >= 120) {
+            target = uint64_t(1.5 * r->shrinkWindowMax);
+            r->shrinkFrames = 0;
+            r->shrinkWindowMax = 0;
+        }
+    } else {
+        r->shrinkFrames = 0;
+        r->shrinkWindowMax = 0;
+    }
+    if (target == 0) return hipSuccess;
+    const uint32_t minCap = std::min<uint32_t>(256u * 1024u, r->maxCapacity);
+    const uint32_t newCap = std::clamp(roundUpToBlock(target), minCap, r->maxCapacity);
+    if (newCap == cap || (newCap < cap && double(newCap) > 0.75 * cap)) return hipSuccess;
+    // The frames in flight may still read the old list: its release is ordered after them.
+    if (r->lastStream && r->lastStream != stream) {
+        hipEvent_t ev;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+        e = hipEventRecord(ev, r->lastStream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev, 0);
+        (void)hipEventDestroy(ev);
+        if (e != hipSuccess) return e;
+    }
+    for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB, &r->sortStatus}) {
+        const hipError_t e = b->releaseAsync(stream);
+        if (e != hipSuccess) return e;
+    }
+    const hipError_t e = allocateList(r, newCap, stream);
+    if (e == hipSuccess) ++r->listResizes;
+    return e;
+}
+
 gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const gsr_settings* settings,
                                gsr_renderer** out) {
     if (!ctx || !scene || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: null argument");
@@ -489,8 +605,14 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     r->settings = s;
     r->degree = std::min(scene->shDegree, std::max(0, std::min(3, s.max_sh_degree)));
     r->binning = s.sort_mode == GSR_SORT_MODE_BINNING;
-    r->capacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0, r->binning);
-    const size_t keyBytes = r->binning ? 4 : 8;
+    r->maxCapacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0, r->binning);
+    uint32_t initialCapacity = r->maxCapacity;
+    if (s.adaptive_list) {
+        // A first guess (4 entries per gaussian); the measured count corrects it a few frames later.
+        const uint64_t guess = s.adaptive_initial_entries ? uint64_t(s.adaptive_initial_entries)
+                                                          : std::max<uint64_t>(4ull * scene->count, 1u << 20);
+        initialCapacity = std::min(r->maxCapacity, roundUpToBlock(guess));
+    }
     const size_t n = std::max<size_t>(scene->count, 1);
     const size_t gBlocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     hipError_t e = hipSuccess;
@@ -503,23 +625,23 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     alloc(r->tileCounts, n * 4);
     alloc(r->blockOffsets, gBlocks * 4);
     alloc(r->control, sizeof(gsr_control));
-    alloc(r->keysA, size_t(r->capacity) * keyBytes);
-    alloc(r->keysB, size_t(r->capacity) * keyBytes);
     if (r->binning) {
         for (DeviceBuffer* b : {&r->depthKeysA, &r->depthKeysB, &r->depthValuesA, &r->depthValuesB, &r->sortedCounts})
             alloc(*b, n * 4);
         alloc(r->visibleSums, gBlocks * 4);
     }
-    alloc(r->valuesA, size_t(r->capacity) * 4);
-    alloc(r->valuesB, size_t(r->capacity) * 4);
     alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses + size_t(kRadix) * kMaxSortBlocks) * 4);
-    alloc(r->sortStatus, sortStatusBytes(r->capacity));
-    if (e == hipSuccess) e = hipMemset(r->sortStatus.ptr, 0, r->sortStatus.bytes);
+    if (e == hipSuccess) e = allocateList(r, initialCapacity, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
-    // Lists start as valid (zero) keys and gaussian indices.
-    for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB})
-        if (e == hipSuccess) e = hipMemset(b->ptr, 0, b->bytes);
     for (int i = 0; i < GSR_STAGE_COUNT && e == hipSuccess; ++i) e = hipEventCreate(&r->events[i]);
+    if (s.adaptive_list || s.frame_stats) {
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&r->statHost), sizeof(gsr_control) * gsr_renderer::kStatSlots,
+                              hipHostMallocDefault);
+        for (int i = 0; i < gsr_renderer::kStatSlots && e == hipSuccess; ++i)
+            e = hipEventCreateWithFlags(&r->statEvent[i], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         gsr_renderer_destroy(r);
         return hipFail(e, "gsr_renderer_create: allocation");
@@ -534,10 +656,28 @@ void gsr_renderer_destroy(gsr_renderer* r) {
     (void)hipDeviceSynchronize();
     for (hipEvent_t& ev : r->events)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t& ev : r->statEvent)
+        if (ev) (void)hipEventDestroy(ev);
+    if (r->statHost) (void)hipHostFree(r->statHost);
     delete r;
 }
 
 uint32_t gsr_renderer_capacity(const gsr_renderer* r) { return r ? r->capacity : 0; }
+uint32_t gsr_renderer_max_capacity(const gsr_renderer* r) { return r ? r->maxCapacity : 0; }
+uint32_t gsr_renderer_list_resizes(const gsr_renderer* r) { return r ? r->listResizes : 0; }
+
+gsr_status gsr_renderer_last_stats(gsr_renderer* r, gsr_control* out, uint64_t* frameIndex) {
+    if (!r || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_last_stats: null argument");
+    if (!r->statHost)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_last_stats: needs frame_stats or adaptive_list");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    collectStats(r);
+    if (!r->lastStatsValid)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_last_stats: no frame has completed yet");
+    *out = r->lastStats;
+    if (frameIndex) *frameIndex = r->lastStatsFrame;
+    return GSR_OK;
+}
 
 uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     if (!r) return 0;
@@ -586,6 +726,10 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     SceneRef ref = sc->ref;
     ref.degree = r->degree;
 
+    if (r->statHost) {
+        collectStats(r);
+        GSR_HIP(adaptCapacity(r, stream));
+    }
     const bool prof = r->settings.profile_stages != 0;
     auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(r->events[i], stream) : hipSuccess; };
     gsr_control* control = r->control.as<gsr_control>();
@@ -676,6 +820,23 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
                  int(r->settings.sort_window), r->rayDepth.as<float4>(), outImage, stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(7));
+    if (r->statHost) {  // this frame's control block, read back without a stall
+        int slot = -1;
+        for (int s = 0; s < gsr_renderer::kStatSlots && slot < 0; ++s)
+            if (!r->statPending[s]) slot = s;
+        if (slot < 0) {  // every slot in flight: wait for the oldest
+            uint64_t oldest = ~uint64_t(0);
+            for (int s = 0; s < gsr_renderer::kStatSlots; ++s)
+                if (r->statFrame[s] < oldest) oldest = r->statFrame[s], slot = s;
+            GSR_HIP(hipEventSynchronize(r->statEvent[slot]));
+            collectStats(r);
+        }
+        GSR_HIP(hipMemcpyAsync(&r->statHost[slot], control, sizeof(gsr_control), hipMemcpyDeviceToHost, stream));
+        GSR_HIP(hipEventRecord(r->statEvent[slot], stream));
+        r->statFrame[slot] = r->frameIndex;
+        r->statPending[slot] = true;
+    }
+    ++r->frameIndex;
     r->lastCam = *cam;
     r->haveFrame = true;
     r->sortedIn = which;

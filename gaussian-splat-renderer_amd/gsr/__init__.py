@@ -102,7 +102,10 @@ class Settings(ctypes.Structure):
         ("full_key_sort", ctypes.c_int32),
         ("render_splat", ctypes.c_int32),
         ("sort_window", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("adaptive_list", ctypes.c_int32),
+        ("adaptive_initial_entries", ctypes.c_uint32),
+        ("frame_stats", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 3),
     ]
 
 
@@ -187,6 +190,9 @@ _read_lists = _sig("gsr_read_lists", _i32, [_vp, _vp, _vp, _vp, _vp, _u32])
 _read_splats = _sig("gsr_read_splats", _i32, [_vp, _vp, _vp])
 _read_stage_times = _sig("gsr_read_stage_times", _i32, [_vp, _vp, _u32])
 _read_packed = _sig("gsr_read_packed_splats", _i32, [_vp, _vp])
+_renderer_max_capacity = _sig("gsr_renderer_max_capacity", _u32, [_vp])
+_renderer_list_resizes = _sig("gsr_renderer_list_resizes", _u32, [_vp])
+_renderer_last_stats = _sig("gsr_renderer_last_stats", _i32, [_vp, _vp, _vp])
 _sort_pairs = _sig("gsr_sort_pairs", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _i32, _vp])
 
 
@@ -225,6 +231,7 @@ EXPORTED = (
     "gsr_buffer_create gsr_buffer_device_ptr gsr_buffer_read gsr_buffer_write gsr_buffer_destroy "
     "gsr_scene_create gsr_scene_destroy gsr_encode_f16 gsr_camera_make gsr_camera_from_viewer "
     "gsr_renderer_create gsr_renderer_destroy gsr_renderer_capacity gsr_renderer_total_bytes gsr_render "
+    "gsr_renderer_max_capacity gsr_renderer_list_resizes gsr_renderer_last_stats "
     "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_sort_pairs "
     "gsr_exclusive_scan "
     "gsr_ply_load gsr_ply_load_memory gsr_ply_free"
@@ -461,7 +468,8 @@ class Renderer:
                  max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
                  exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
                  sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full",
-                 sort_window: int = 0):
+                 sort_window: int = 0, adaptive_list: bool = False, adaptive_initial_entries: int = 0,
+                 frame_stats: bool = False):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -476,6 +484,9 @@ class Renderer:
         s.full_key_sort = 1 if full_key_sort else 0
         s.render_splat = RENDER_SPLATS[render_splat]
         s.sort_window = sort_window
+        s.adaptive_list = 1 if adaptive_list else 0
+        s.adaptive_initial_entries = adaptive_initial_entries
+        s.frame_stats = 1 if frame_stats else 0
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
@@ -491,6 +502,22 @@ class Renderer:
     @property
     def total_bytes(self) -> int:
         return int(_renderer_total_bytes(self.handle))
+
+    @property
+    def max_capacity(self) -> int:
+        return int(_renderer_max_capacity(self.handle))
+
+    @property
+    def list_resizes(self) -> int:
+        return int(_renderer_list_resizes(self.handle))
+
+    def last_stats(self):
+        """lastStats: (control, frame index) of the newest completed frame, or None."""
+        c = Control()
+        idx = ctypes.c_uint64()
+        if _renderer_last_stats(self.handle, ctypes.byref(c), ctypes.byref(idx)) != 0:
+            return None
+        return c, int(idx.value)
 
     def image_bytes(self, cam: Camera) -> int:
         return int(cam.width) * int(cam.height) * (4 if self.target == "rgba8" else 16)

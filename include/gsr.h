@@ -142,7 +142,13 @@ typedef struct gsr_settings {
                                     of a K-slot window sorted by the depth of each splat's maximum along the
                                     pixel's ray (tile_render.comp:201-279), reference alpha. K > 16:
                                     GSR_ERROR_UNSUPPORTED (the reference allows 64) */
-    uint32_t reserved[2];
+    int32_t adaptive_list;       /* TileSettings::adaptiveList (splat_tile_renderer.h:104): size the list from the
+                                    measured entry count (adaptCapacity, splat_tile_renderer.cpp:444-486), within
+                                    the budget; resizes never stall the frames in flight */
+    uint32_t adaptive_initial_entries; /* first guess (0 = max(4 N, 2^20)) */
+    int32_t frame_stats;         /* 1: read every frame's control block back asynchronously
+                                    (gsr_renderer_last_stats); implied by adaptive_list */
+    uint32_t reserved[3];
 } gsr_settings;
 
 /* RenderSplatMode (splat_tile_renderer.h:88-92; render_splat.glsl:1-21), same values. The
@@ -234,6 +240,12 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
                                gsr_renderer** out);
 void gsr_renderer_destroy(gsr_renderer* r);
 uint32_t gsr_renderer_capacity(const gsr_renderer* r);     /* list capacity, entries */
+uint32_t gsr_renderer_max_capacity(const gsr_renderer* r); /* capacity of the full budget (maxCapacity) */
+uint32_t gsr_renderer_list_resizes(const gsr_renderer* r); /* adaptive list resizes so far (listResizes) */
+/* lastStats (splat_tile_renderer.h:167-176): the control block of the newest frame whose
+ * asynchronous read-back has completed, and its index (0 = the first frame rendered). Does not
+ * wait for the GPU; fails while no frame has completed. Needs frame_stats or adaptive_list. */
+gsr_status gsr_renderer_last_stats(gsr_renderer* r, gsr_control* out, uint64_t* frame_index);
 uint64_t gsr_renderer_total_bytes(const gsr_renderer* r);  /* device memory owned by the renderer */
 
 /* SplatTileRenderer::render: the whole frame on `stream` (null = the context's stream),

@@ -288,6 +288,58 @@ def test_stopthepop_window(gsr_ctx, window):
     scene.close()
 
 
+def test_adaptive_list_capacity(gsr_ctx):
+    """adaptCapacity (splat_tile_renderer.cpp:444-486): a first frame that overflows a small guess
+    grows the list to 1.5 M (then the lists are bit-exact against the oracle); 120 frames below
+    0.4 capacity shrink it, never below min(256k, budget); the frame statistics arrive without a
+    stall and name their frame."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log("case adaptive list, 300k 1920x1080")
+    sc = scenes.make_scene(300_000, seed=2, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", adaptive_list=True, adaptive_initial_entries=4096)
+    assert r.capacity == 4096 and r.max_capacity == 16267264  # 256 MB / 16.5 B, in blocks of 2048
+    cam = _camera(gsr, camera.orbit_view(6.0, 10.0, 30.0), 1920, 1080)
+    assert r.last_stats() is None
+    r.render(cam)
+    gsr_ctx.synchronize()
+    ctl0 = r.read_control()
+    assert ctl0.overflow > 0 and ctl0.entries == 4096
+    r.render(cam)  # collects frame 0's statistics and grows before recording frame 1
+    gsr_ctx.synchronize()
+    st = r.last_stats()
+    assert st is not None and st[1] in (0, 1) and st[0].duplicates == ctl0.duplicates
+    m = ctl0.duplicates
+    assert r.list_resizes == 1 and r.capacity == max(-(-int(1.5 * m) // 2048) * 2048, 256 * 1024)
+    ctl1 = r.read_control()
+    assert ctl1.overflow == 0 and ctl1.duplicates == m
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    os_, oc = oracle.preprocess(sc, ocam, bounds=oracle.TIGHT)
+    _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, oracle.TIGHT)
+    log(f"  M {m}: capacity 4096 -> {r.capacity}")
+    grown = r.capacity
+    # a far view with few entries: 120 frames below 0.4 capacity shrink the list to 1.5 x their M
+    far = _camera(gsr, camera.orbit_view(60.0, 10.0, 30.0), 1920, 1080)
+    for _ in range(125):
+        r.render(far)
+        gsr_ctx.synchronize()
+    m_far = r.read_control().duplicates
+    want = max(-(-int(1.5 * m_far) // 2048) * 2048, 256 * 1024)
+    log(f"  after 125 far frames (M {m_far}): capacity {r.capacity}, resizes {r.list_resizes}")
+    assert 0 < m_far < 0.4 * grown
+    assert r.list_resizes == 2 and r.capacity == want < grown
+    st = r.last_stats()
+    assert st is not None and st[1] >= 120
+    r.render(cam)
+    gsr_ctx.synchronize()
+    assert r.read_control().duplicates == m  # the list still holds what fits
+    r.close()
+    scene.close()
+
+
 def test_f16_scene_bit_exact(gsr_ctx):
     """The fp16 records: the GPU decode equals decodeGaussian, so splats and lists stay bit-exact."""
     import oracle
