@@ -68,6 +68,56 @@ __device__ __forceinline__ void fetchSplat(const float4* __restrict__ splats, co
     }
 }
 
+// The predicated loops' per-splat record: q = log2(o exp(power)) at the pixel (x0 + X, y0 + Y)
+// of the tile with origin (x0, y0), as a quadratic in the tile-local X, Y:
+//   q = F + D X + E Y + A X^2 + B X Y + C Y^2,
+// A, B, C = the conic scaled by -log2(e)/2 (B: -log2 e), D = -2 A dx - B dy, E = -B dx - 2 C dy,
+// F = q at the tile origin, dx, dy = the centre relative to the tile origin. s0 = (A, B, C, D),
+// s1 = (E, F): a pair then costs 5 FMAs against per-lane constants X, Y, X^2, XY, Y^2.
+__device__ __forceinline__ void tileQuadratic(float cx, float cy, float4 co, float x0, float y0, float4& s0,
+                                              float2& s1) {
+    const float A = -0.5f * kLog2e * co.x, B = -kLog2e * co.y, C = -0.5f * kLog2e * co.z;
+    const float L = __log2f(co.w);
+    const float dx = cx - x0, dy = cy - y0;
+    const float D = -2.0f * A * dx - B * dy;
+    const float E = -B * dx - 2.0f * C * dy;
+    const float F = fmaf(fmaf(A, dx, B * dy), dx, fmaf(C * dy, dy, L));
+    s0 = make_float4(A, B, C, D);
+    s1 = make_float2(E, F);
+}
+
+struct PixelQ {
+    float X, Y, X2, XY, Y2;
+};
+
+__device__ __forceinline__ PixelQ pixelQ(uint32_t lx, uint32_t ly) {
+    const float X = float(lx), Y = float(ly);
+    return PixelQ{X, Y, X * X, X * Y, Y * Y};
+}
+
+__device__ __forceinline__ float evalQ(const float4& s0, const float2& s1, const PixelQ& p) {
+    return fmaf(s0.x, p.X2, fmaf(s0.y, p.XY, fmaf(s0.z, p.Y2, fmaf(s0.w, p.X, fmaf(s1.x, p.Y, s1.y)))));
+}
+
+// One splat into one pixel, branch-free: the lane's state is its transmittance T, colour and
+// cutoff thr (log2(1/255) while alive, +inf once done or outside the image). A pair below the
+// cutoff changes nothing (aEff = 0); the splat that would take T below 1e-4 is not blended and
+// ends the pixel (test-before-accept). The weight is T - T', which is exactly 0 in both cases.
+__device__ __forceinline__ void blendPredicated(float q, float4 col, float& T, float& c0, float& c1, float& c2,
+                                                float& thr) {
+    const float a = fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q));
+    const float aEff = q >= thr ? a : 0.0f;
+    const float testT = T * (1.0f - aEff);
+    const bool term = testT < kMinTransmittance;
+    const float Tn = term ? T : testT;
+    const float w = T - Tn;
+    c0 = fmaf(col.x, w, c0);
+    c1 = fmaf(col.y, w, c1);
+    c2 = fmaf(col.z, w, c2);
+    T = Tn;
+    thr = term ? __builtin_inff() : thr;
+}
+
 __device__ __forceinline__ uint32_t quantize8(float v) {
     v = v > 0.0f ? v : 0.0f;  // also NaN -> 0
     v = v < 1.0f ? v : 1.0f;
@@ -84,8 +134,8 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
                                                      void* __restrict__ out) {
     constexpr int kThreads = 256 / PPT;
     constexpr int kRowStep = 16 / PPT;
-    // FAST: s0 = (cx, cy, A, B), s1 = (C, log2 o), A/B/C the conic scaled by -log2(e)/2 (B: -log2 e).
-    // EXACT: s0 = (cx, cy, a, b), s1 = (c, o).
+    // PRED: the tile-local quadratic (tileQuadratic). FAST: s0 = (cx, cy, A, B), s1 = (C, log2 o),
+    // A/B/C the conic scaled by -log2(e)/2 (B: -log2 e). EXACT: s0 = (cx, cy, a, b), s1 = (c, o).
     __shared__ float4 sS0[kBatch];
     __shared__ float2 sS1[kBatch];
     __shared__ float4 sColor[kBatch];
@@ -98,8 +148,9 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     const uint32_t tile = row * tilesX + idx % tilesX;
     if (tile >= tileCount) return;  // uniform
     const uint32_t t = threadIdx.x;
-    const uint32_t px = (tile % tilesX) * 16 + (t % 16);
-    const uint32_t py0 = (tile / tilesX) * 16 + (t / 16);
+    const uint32_t tx0 = (tile % tilesX) * 16, ty0 = (tile / tilesX) * 16;
+    const uint32_t px = tx0 + (t % 16);
+    const uint32_t py0 = ty0 + (t / 16);
     const float pxf = float(px);
 
     float T[PPT], C0[PPT], C1[PPT], C2[PPT], pyf[PPT];
@@ -135,7 +186,9 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
                 float4 co, col;
                 fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
                 sColor[slot] = col;
-                if (FAST) {
+                if (PRED) {
+                    tileQuadratic(cx, cy, co, float(tx0), float(ty0), sS0[slot], sS1[slot]);
+                } else if (FAST) {
                     sS0[slot] = make_float4(cx, cy, -0.5f * kLog2e * co.x, -kLog2e * co.y);
                     sS1[slot] = make_float2(-0.5f * kLog2e * co.z, __log2f(co.w));
                 } else {
@@ -152,33 +205,18 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
             // (log2(1/255) while alive, +inf once done or outside the image), so every branch is
             // one vector compare: the exponential and the blend run only when some lane of the
             // wave passes its cutoff, and the blend itself is branch-free (selects).
-            float& Tl = T[0];
-            float& c0 = C0[0];
-            float& c1 = C1[0];
-            float& c2 = C2[0];
-            const float py = pyf[0];
+            const PixelQ pq = pixelQ(t % 16, t / 16);
             float thr = done[0] ? __builtin_inff() : kLog2MinAlpha;
             const uint32_t n4 = (n + 3u) & ~3u;
             for (uint32_t j = 0; j < n4; j += 4) {
+                // The four q first (their LDS reads in flight together), then the blends in order.
+                float q[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) q[u] = evalQ(sS0[j + u], sS1[j + u], pq);
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
-                    const float4 s0 = sS0[j + u];
-                    const float2 s1 = sS1[j + u];
-                    const float dx = s0.x - pxf;
-                    const float dy = s0.y - py;
-                    const float q = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, fmaf(s1.x * dy, dy, s1.y));
-                    if (__builtin_amdgcn_ballot_w64(q >= thr) == 0) continue;
-                    const float a = fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q));
-                    const float aEff = q >= thr ? a : 0.0f;
-                    const float testT = Tl * (1.0f - aEff);
-                    const bool term = testT < kMinTransmittance;
-                    const float w = term ? 0.0f : aEff * Tl;
-                    const float4 col = sColor[j + u];
-                    c0 = fmaf(col.x, w, c0);
-                    c1 = fmaf(col.y, w, c1);
-                    c2 = fmaf(col.z, w, c2);
-                    Tl = term ? Tl : testT;
-                    thr = term ? __builtin_inff() : thr;
+                    if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;
+                    blendPredicated(q[u], sColor[j + u], T[0], C0[0], C1[0], C2[0], thr);
                 }
                 if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
             }
@@ -266,8 +304,8 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
                                                      const uint2* __restrict__ ranges, uint32_t width,
                                                      uint32_t height, uint32_t tilesX, uint32_t tileCount,
                                                      uint32_t gaussianCount, void* __restrict__ out) {
-    // s0 = (cx, cy, A, B), s1 = (C, log2 o), A/B/C the conic scaled by -log2(e)/2 (B: -log2 e).
-    // Slot kBatch is a null splat (q = -inf for every pixel) that pads the wave lists.
+    // s0, s1: the tile-local quadratic of q (tileQuadratic). Slot kBatch is a null splat (q = -inf
+    // for every pixel) that pads the wave lists.
     __shared__ float4 sS0[kBatch + 1];
     __shared__ float2 sS1[kBatch + 1];
     __shared__ float4 sColor[kBatch + 1];
@@ -282,7 +320,7 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
     const uint32_t tx0 = (tile % tilesX) * 16, ty0 = (tile / tilesX) * 16;
     const uint32_t px = tx0 + (wave & 1u) * 8 + (lane & 7u);
     const uint32_t py = ty0 + (wave >> 1) * 8 + (lane >> 3);
-    const float pxf = float(px), pyf = float(py);
+    const PixelQ pq = pixelQ(px - tx0, py - ty0);
     // Quadrant corners (pixel coordinates of the first and last pixel centres), per wave bit.
     const float qx0[2] = {float(tx0), float(tx0 + 8)};
     const float qy0[2] = {float(ty0), float(ty0 + 8)};
@@ -312,8 +350,7 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
                 const float2 o0 = make_float2(cx, cy);
                 sColor[t] = col;
                 const float logO = __log2f(co.w);
-                sS0[t] = make_float4(o0.x, o0.y, -0.5f * kLog2e * co.x, -kLog2e * co.y);
-                sS1[t] = make_float2(-0.5f * kLog2e * co.z, logO);
+                tileQuadratic(o0.x, o0.y, co, float(tx0), float(ty0), sS0[t], sS1[t]);
                 // For a positive definite conic, q >= log2(1/255) <=> a dx^2 + 2 b dx dy + c dy^2 <= r2,
                 // r2 = 2 (log2 o - log2(1/255)) / log2 e, so |dx| <= sqrt(r2 c / det) and
                 // |dy| <= sqrt(r2 a / det), det = a c - b^2. The box is widened by 0.1% and one
@@ -363,26 +400,18 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
         if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) != 0) {
             for (uint32_t j = 0; j < count4; j += 4) {
                 const uint2 four = *reinterpret_cast<const uint2*>(&sList[wave][j]);
+                // The four q first (their LDS reads in flight together), then the blends in order.
+                uint32_t slot[4];
+                float q[4];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
-                    const uint32_t slot = ((u < 2 ? four.x : four.y) >> (16 * (u & 1u))) & 0xFFFFu;
-                    const float4 s0 = sS0[slot];
-                    const float2 s1 = sS1[slot];
-                    const float dx = s0.x - pxf;
-                    const float dy = s0.y - pyf;
-                    const float q = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, fmaf(s1.x * dy, dy, s1.y));
-                    if (__builtin_amdgcn_ballot_w64(q >= thr) == 0) continue;
-                    const float a = fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q));
-                    const float aEff = q >= thr ? a : 0.0f;
-                    const float testT = T * (1.0f - aEff);
-                    const bool term = testT < kMinTransmittance;
-                    const float w = term ? 0.0f : aEff * T;
-                    const float4 col = sColor[slot];
-                    c0 = fmaf(col.x, w, c0);
-                    c1 = fmaf(col.y, w, c1);
-                    c2 = fmaf(col.z, w, c2);
-                    T = term ? T : testT;
-                    thr = term ? __builtin_inff() : thr;
+                    slot[u] = ((u < 2 ? four.x : four.y) >> (16 * (u & 1u))) & 0xFFFFu;
+                    q[u] = evalQ(sS0[slot[u]], sS1[slot[u]], pq);
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;
+                    blendPredicated(q[u], sColor[slot[u]], T, c0, c1, c2, thr);
                 }
                 if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
             }
