@@ -4,13 +4,31 @@
 // gfx950 design: the global radix sort only orders the list by its tile bits (2 passes for a
 // 13-bit tile id instead of 6 for the full 45-bit key). Within a tile the entries are then in
 // duplication (gaussian-index) order, and this kernel orders each tile's range by
-// (depth bits, gaussian index), which is the stable (tile, depth) order of the full sort:
-//   - one workgroup per tile; ranges of up to kChunk entries are sorted in LDS with a bitonic
-//     network over 64-bit (depth << 32 | index) words (unique within a tile: a gaussian
-//     appears at most once per tile), written back in place;
-//   - longer ranges are sorted chunk by chunk the same way, then merged in rounds of doubling
-//     run width through the scratch buffers (merge path: every thread finds its output slice
-//     by a binary search, then merges it sequentially).
+// (depth bits, gaussian index), which is the stable (tile, depth, index) order of the full sort.
+//
+// One wave per tile, no LDS and no workgroup barrier (the 4 waves of a workgroup work on 4
+// different tiles and never wait for each other):
+//   - a range of n <= 2048 entries is loaded into registers, E = 1, 2, 4, ..., 32 64-bit
+//     (depth << 32 | index) words per lane (the smallest E with 64 E >= n; padding words sort
+//     last), sorted by a bitonic network and written back. The words are unique within a tile
+//     (a gaussian appears at most once per tile), so the order is total and deterministic.
+//     Network positions are i = lane * E + r: partners less than E apart are in the lane's own
+//     registers, partners further apart are one 64-bit lane shuffle per word. The network is
+//     the flip formulation (the first stage of every merge level pairs i with i ^ (k - 1),
+//     every later stage pairs i with i ^ j, and the lower position always keeps the minimum),
+//     so no per-element direction bits are needed. The stage loops stay rolled (only the
+//     in-register stages unroll): the code stays small enough for the instruction cache when
+//     waves of every size class run side by side;
+//   - a longer range (rare: a tile covered by more than 2048 splats) is sorted afterwards by
+//     the whole workgroup: chunks of 2048 by an LDS bitonic network, then merge rounds of
+//     doubling run width through the scratch buffers (merge path: every thread finds its
+//     output slice by a binary search, then merges it sequentially). Every thread reads the
+//     workgroup's four range lengths first, so the choice is uniform: a workgroup without a
+//     long range never waits at a barrier. That path has no loop around the register sort
+//     (such a loop makes the compiler keep the 32 words' addresses live across iterations,
+//     doubling the kernel's registers).
+// The loads are striped (word r of lane l is input entry r * 64 + l: coalesced; the input order
+// of a sort is arbitrary), the stores follow the network positions.
 #include "gsr_internal.hpp"
 
 namespace gsr {
@@ -18,12 +36,134 @@ namespace gsr {
 namespace {
 
 constexpr int kTsThreads = 256;
-constexpr uint32_t kChunk = 2048;  // entries sorted in LDS at once (16 KB)
+constexpr int kTsWaves = kTsThreads / 64;
+constexpr uint32_t kWaveChunk = 2048;  // entries one wave sorts in registers (E = 32)
+constexpr uint64_t kPad = ~uint64_t(0);
 
 __device__ __forceinline__ uint64_t orderWord(uint64_t key, uint32_t value) {
     return (key << 32) | value;  // (depth bits, gaussian index); the tile bits shift out
 }
 
+__device__ __forceinline__ uint64_t shflXor64(uint64_t v, uint32_t mask) {
+    const uint32_t lo = __shfl_xor(uint32_t(v), int(mask), 64);
+    const uint32_t hi = __shfl_xor(uint32_t(v >> 32), int(mask), 64);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Compare-exchange of two registers: minimum to the first.
+__device__ __forceinline__ void exchange(uint64_t& a, uint64_t& b) {
+    const uint64_t x = a, y = b;
+    a = __builtin_elementwise_min(x, y);
+    b = __builtin_elementwise_max(x, y);
+}
+
+// The lower position keeps the minimum of the pair, the upper one the maximum.
+__device__ __forceinline__ uint64_t keep(bool lower, uint64_t mine, uint64_t partner) {
+    return (lower == (mine < partner)) ? mine : partner;
+}
+
+// In-lane stages (partners r and r ^ M of the same lane; M < E).
+template <int E, int M>
+__device__ __forceinline__ void registerStage(uint64_t (&w)[E]) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int p = r ^ M;
+        if (p > r) exchange(w[r], w[p]);
+    }
+}
+template <int E, int J>
+__device__ __forceinline__ void registerHalves(uint64_t (&w)[E]) {  // stages J, J / 2, ..., 1
+    if constexpr (J >= 1) {
+        registerStage<E, J>(w);
+        registerHalves<E, J / 2>(w);
+    }
+}
+// Merge levels k = 2 .. E: every lane sorts its own E registers.
+template <int E, int K = 2>
+__device__ __forceinline__ void registerLevels(uint64_t (&w)[E]) {
+    if constexpr (K <= E) {
+        registerStage<E, K - 1>(w);  // flip
+        registerHalves<E, K / 4>(w);
+        registerLevels<E, K * 2>(w);
+    }
+}
+
+// Lane stages (partners m lanes apart, m = j / E): same register.
+template <int E>
+__device__ __forceinline__ void laneHalf(uint64_t (&w)[E], uint32_t lane, uint32_t m) {
+    const bool lower = (lane & m) == 0u;
+#pragma unroll
+    for (int r = 0; r < E; ++r) w[r] = keep(lower, w[r], shflXor64(w[r], m));
+}
+// Flip of a level k > E: position lane * E + r pairs with (lane ^ m) * E + (E - 1 - r),
+// m = k / E - 1; the lower position is the one whose lane has bit k / (2 E) clear.
+template <int E>
+__device__ __forceinline__ void laneFlip(uint64_t (&w)[E], uint32_t lane, uint32_t m, uint32_t half) {
+    const bool lower = (lane & half) == 0u;
+    if constexpr (E == 1) {
+        w[0] = keep(lower, w[0], shflXor64(w[0], m));
+    } else {
+#pragma unroll
+        for (int r = 0; r < E / 2; ++r) {
+            const uint64_t ya = shflXor64(w[E - 1 - r], m);
+            const uint64_t yb = shflXor64(w[r], m);
+            w[r] = keep(lower, w[r], ya);
+            w[E - 1 - r] = keep(lower, w[E - 1 - r], yb);
+        }
+    }
+}
+
+// Ascending bitonic sort of the wave's 64 E words (positions lane * E + r).
+template <int E>
+__device__ __forceinline__ void waveSort(uint64_t (&w)[E], uint32_t lane) {
+    constexpr uint32_t N = 64u * E;
+    registerLevels<E>(w);
+#pragma unroll 1
+    for (uint32_t k = 2u * E; k <= N; k <<= 1) {
+        laneFlip<E>(w, lane, k / E - 1u, k / (2u * E));
+#pragma unroll 1
+        for (uint32_t j = k >> 2; j >= uint32_t(E); j >>= 1) laneHalf<E>(w, lane, j / E);
+        registerHalves<E, E / 2>(w);
+    }
+}
+
+// The wave sorts entries [at, at + n) in place (n <= 64 E); tileHi is the tile's key bits.
+template <int E>
+__device__ __forceinline__ void waveSortRange(uint64_t* keys, uint32_t* values, uint64_t tileHi, uint32_t at,
+                                              uint32_t n, uint32_t lane) {
+    uint64_t w[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const uint32_t e = uint32_t(r) * 64u + lane;
+        w[r] = e < n ? orderWord(keys[at + e], values[at + e]) : kPad;
+    }
+    waveSort<E>(w, lane);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const uint32_t e = lane * E + uint32_t(r);
+        if (e < n) {
+            keys[at + e] = tileHi | (w[r] >> 32);
+            values[at + e] = uint32_t(w[r]);
+        }
+    }
+}
+
+__device__ __forceinline__ void waveSortAny(uint64_t* keys, uint32_t* values, uint64_t tileHi, uint32_t at,
+                                            uint32_t n, uint32_t lane) {
+    if (n <= 64u) waveSortRange<1>(keys, values, tileHi, at, n, lane);  // uniform per wave
+    else if (n <= 128u) waveSortRange<2>(keys, values, tileHi, at, n, lane);
+    else if (n <= 256u) waveSortRange<4>(keys, values, tileHi, at, n, lane);
+    else if (n <= 512u) waveSortRange<8>(keys, values, tileHi, at, n, lane);
+    else if (n <= 1024u) waveSortRange<16>(keys, values, tileHi, at, n, lane);
+    else waveSortRange<32>(keys, values, tileHi, at, n, lane);
+}
+
+// ---- long ranges: the whole workgroup ----
+
+
+constexpr uint32_t kLdsChunk = 2048;  // entries sorted in LDS at once (16 KB)
+
+// Ascending bitonic sort of s[0, n2) (n2 a power of two) by the workgroup.
 __device__ __forceinline__ void bitonicSortLds(uint64_t* s, uint32_t n2) {
     for (uint32_t k = 2; k <= n2; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
@@ -54,26 +194,19 @@ __device__ __forceinline__ uint32_t coRank(const uint64_t* ka, const uint32_t* v
     return lo;
 }
 
-__global__ __launch_bounds__(kTsThreads) void kTileDepthSort(uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
-                                                             uint64_t* __restrict__ keysTmp,
-                                                             uint32_t* __restrict__ valuesTmp,
-                                                             const uint2* __restrict__ ranges, uint32_t tileCount) {
-    __shared__ uint64_t sW[kChunk];
-    const uint32_t tile = blockIdx.x;
-    if (tile >= tileCount) return;
-    const uint2 range = ranges[tile];
+__device__ void groupSortLong(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp,
+                              uint32_t tile, uint2 range, uint64_t* sW) {
     const uint32_t n = range.y - range.x;
-    if (n <= 1u) return;  // uniform
     const uint64_t tileHi = uint64_t(tile) << 32;
 
-    // 1. Chunks of up to kChunk entries, sorted in LDS, in place.
-    for (uint32_t c0 = 0; c0 < n; c0 += kChunk) {
-        const uint32_t m = n - c0 < kChunk ? n - c0 : kChunk;
+    // 1. Chunks of up to 2048 entries, sorted in LDS, in place.
+    for (uint32_t c0 = 0; c0 < n; c0 += kLdsChunk) {
+        const uint32_t m = n - c0 < kLdsChunk ? n - c0 : kLdsChunk;
         uint32_t n2 = 2;
         while (n2 < m) n2 <<= 1;
         const uint32_t base = range.x + c0;
         for (uint32_t i = threadIdx.x; i < n2; i += kTsThreads)
-            sW[i] = i < m ? orderWord(keys[base + i], values[base + i]) : ~uint64_t(0);
+            sW[i] = i < m ? orderWord(keys[base + i], values[base + i]) : kPad;
         __syncthreads();
         bitonicSortLds(sW, n2);
         for (uint32_t i = threadIdx.x; i < m; i += kTsThreads) {
@@ -83,14 +216,13 @@ __global__ __launch_bounds__(kTsThreads) void kTileDepthSort(uint64_t* __restric
         }
         __syncthreads();
     }
-    if (n <= kChunk) return;
 
     // 2. Merge rounds: runs of width w -> 2w, alternating between the list and the scratch.
     uint64_t* ks = keys + range.x;
     uint32_t* vs = values + range.x;
     uint64_t* kd = keysTmp + range.x;
     uint32_t* vd = valuesTmp + range.x;
-    for (uint32_t w = kChunk; w < n; w <<= 1) {
+    for (uint32_t w = kLdsChunk; w < n; w <<= 1) {
         for (uint32_t lo = 0; lo < n; lo += 2 * w) {
             const uint32_t mid = lo + w < n ? lo + w : n;
             const uint32_t hi = lo + 2 * w < n ? lo + 2 * w : n;
@@ -130,6 +262,37 @@ __global__ __launch_bounds__(kTsThreads) void kTileDepthSort(uint64_t* __restric
             values[range.x + i] = vs[i];
         }
     }
+    __syncthreads();  // before the next long range reuses the LDS
+}
+
+// 5 workgroups per CU: 96 VGPRs, the most the 32-word sort fits in without spills.
+__global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
+                                                             uint64_t* __restrict__ keysTmp,
+                                                             uint32_t* __restrict__ valuesTmp,
+                                                             const uint2* __restrict__ ranges, uint32_t tileCount) {
+    __shared__ uint64_t sW[kLdsChunk];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t first = blockIdx.x * kTsWaves;
+    uint32_t longMask = 0;  // the workgroup's ranges longer than 2048 (same in every thread)
+#pragma unroll
+    for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v) {
+        if (first + v < tileCount) {
+            const uint2 rg = ranges[first + v];
+            if (rg.y - rg.x > kWaveChunk) longMask |= 1u << v;
+        }
+    }
+    // 1. One range of up to 2048 entries per wave.
+    const uint32_t tile = first + wave;
+    if (tile < tileCount) {
+        const uint2 range = ranges[tile];
+        const uint32_t n = range.y - range.x;
+        if (n > 1u && n <= kWaveChunk) waveSortAny(keys, values, uint64_t(tile) << 32, range.x, n, lane);
+    }
+    if (longMask == 0u) return;  // uniform per workgroup
+    // 2. The longer ones, one at a time by the whole workgroup.
+    __syncthreads();
+    for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v)
+        if (longMask & (1u << v)) groupSortLong(keys, values, keysTmp, valuesTmp, first + v, ranges[first + v], sW);
 }
 
 }  // namespace
@@ -137,7 +300,8 @@ __global__ __launch_bounds__(kTsThreads) void kTileDepthSort(uint64_t* __restric
 void launchTileDepthSort(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp, const uint2* ranges,
                          uint32_t tileCount, hipStream_t stream) {
     if (tileCount == 0) return;
-    hipLaunchKernelGGL(kTileDepthSort, dim3(tileCount), dim3(kTsThreads), 0, stream, keys, values, keysTmp, valuesTmp,
+    const uint32_t blocks = (tileCount + kTsWaves - 1) / kTsWaves;
+    hipLaunchKernelGGL(kTileDepthSort, dim3(blocks), dim3(kTsThreads), 0, stream, keys, values, keysTmp, valuesTmp,
                        ranges, tileCount);
 }
 
