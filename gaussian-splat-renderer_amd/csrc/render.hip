@@ -90,6 +90,14 @@ struct PixelQ {
     float X, Y, X2, XY, Y2;
 };
 
+// A batch slot of the culled loop: the quadratic (s0, s1) and the colour, 48 B.
+struct __align__(16) SplatRecord {
+    float4 s0;
+    float2 s1;
+    float2 pad;
+    float4 col;
+};
+
 __device__ __forceinline__ PixelQ pixelQ(uint32_t lx, uint32_t ly) {
     const float X = float(lx), Y = float(ly);
     return PixelQ{X, Y, X * X, X * Y, Y * Y};
@@ -304,13 +312,12 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
                                                      const uint2* __restrict__ ranges, uint32_t width,
                                                      uint32_t height, uint32_t tilesX, uint32_t tileCount,
                                                      uint32_t gaussianCount, void* __restrict__ out) {
-    // s0, s1: the tile-local quadratic of q (tileQuadratic). Slot kBatch is a null splat (q = -inf
-    // for every pixel) that pads the wave lists.
-    __shared__ float4 sS0[kBatch + 1];
-    __shared__ float2 sS1[kBatch + 1];
-    __shared__ float4 sColor[kBatch + 1];
-    __shared__ uint32_t sMask[kBatch];                  // bit w: the box meets wave w's quadrant
-    __shared__ __align__(8) uint16_t sList[4][kBatch + 4];  // per wave: batch slots in list order
+    // One 48-B record per batch slot: the tile-local quadratic of q (tileQuadratic) and the
+    // colour. Slot kBatch is a null splat (q = -inf for every pixel) that pads the wave lists.
+    // The lists hold the records' byte offsets, so a splat's reads need no address arithmetic.
+    __shared__ SplatRecord sRec[kBatch + 1];
+    __shared__ uint32_t sMask[kBatch];                     // bit w: the box meets wave w's quadrant
+    __shared__ __align__(16) uint32_t sList[4][kBatch + 4];  // per wave: record offsets in list order
 
     const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
     const uint32_t row = xcd + kXcds * (idx / tilesX);
@@ -325,10 +332,11 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
     const float qx0[2] = {float(tx0), float(tx0 + 8)};
     const float qy0[2] = {float(ty0), float(ty0 + 8)};
     if (t == 0) {
-        sS0[kBatch] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        sS1[kBatch] = make_float2(0.0f, -__builtin_inff());
-        sColor[kBatch] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        sRec[kBatch].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        sRec[kBatch].s1 = make_float2(0.0f, -__builtin_inff());
+        sRec[kBatch].col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    const char* recBase = reinterpret_cast<const char*>(sRec);
 
     float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
     float thr = (px < width && py < height) ? kLog2MinAlpha : __builtin_inff();
@@ -348,9 +356,9 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
                 float4 co, col;
                 fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
                 const float2 o0 = make_float2(cx, cy);
-                sColor[t] = col;
+                sRec[t].col = col;
                 const float logO = __log2f(co.w);
-                tileQuadratic(o0.x, o0.y, co, float(tx0), float(ty0), sS0[t], sS1[t]);
+                tileQuadratic(o0.x, o0.y, co, float(tx0), float(ty0), sRec[t].s0, sRec[t].s1);
                 // For a positive definite conic, q >= log2(1/255) <=> a dx^2 + 2 b dx dy + c dy^2 <= r2,
                 // r2 = 2 (log2 o - log2(1/255)) / log2 e, so |dx| <= sqrt(r2 c / det) and
                 // |dy| <= sqrt(r2 a / det), det = a c - b^2. The box is widened by 0.1% and one
@@ -389,29 +397,29 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
             const uint32_t slot = j * 64 + lane;
             const bool hit = (sMask[slot] >> wave) & 1u;
             const uint64_t b = __builtin_amdgcn_ballot_w64(hit);
-            if (hit) sList[wave][count + uint32_t(__popcll(b & laneMaskLt))] = uint16_t(slot);
+            if (hit) sList[wave][count + uint32_t(__popcll(b & laneMaskLt))] = slot * uint32_t(sizeof(SplatRecord));
             count += uint32_t(__popcll(b));
         }
         const uint32_t count4 = (count + 3u) & ~3u;
-        if (lane < count4 - count) sList[wave][count + lane] = uint16_t(kBatch);
+        if (lane < count4 - count) sList[wave][count + lane] = kBatch * uint32_t(sizeof(SplatRecord));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) != 0) {
             for (uint32_t j = 0; j < count4; j += 4) {
-                const uint2 four = *reinterpret_cast<const uint2*>(&sList[wave][j]);
+                const uint4 four = *reinterpret_cast<const uint4*>(&sList[wave][j]);
                 // The four q first (their LDS reads in flight together), then the blends in order.
-                uint32_t slot[4];
+                const SplatRecord* rec[4] = {reinterpret_cast<const SplatRecord*>(recBase + four.x),
+                                             reinterpret_cast<const SplatRecord*>(recBase + four.y),
+                                             reinterpret_cast<const SplatRecord*>(recBase + four.z),
+                                             reinterpret_cast<const SplatRecord*>(recBase + four.w)};
                 float q[4];
 #pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                    slot[u] = ((u < 2 ? four.x : four.y) >> (16 * (u & 1u))) & 0xFFFFu;
-                    q[u] = evalQ(sS0[slot[u]], sS1[slot[u]], pq);
-                }
+                for (uint32_t u = 0; u < 4; ++u) q[u] = evalQ(rec[u]->s0, rec[u]->s1, pq);
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;
-                    blendPredicated(q[u], sColor[slot[u]], T, c0, c1, c2, thr);
+                    blendPredicated(q[u], rec[u]->col, T, c0, c1, c2, thr);
                 }
                 if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
             }
