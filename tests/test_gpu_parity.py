@@ -109,6 +109,44 @@ def test_100k_720p_splats_lists_image(gsr_ctx, bounds, sort_mode):
     scene.close()
 
 
+@pytest.mark.parametrize("bounds,sort_mode,W,H,radius", [
+    ("cutoff-only", "global", 640, 360, 6.0),
+    ("cutoff-only", "binning", 640, 360, 6.0),
+    ("tight", "global", 1000, 563, 8.0),
+    ("tight", "binning", 17, 9, 8.0),
+    ("rect", "global", 333, 211, 3.0),
+    ("tight", "global", 640, 360, 1.2),
+    ("tight", "binning", 640, 360, 1.2),
+])
+def test_bounds_sizes_and_close_cameras(gsr_ctx, bounds, sort_mode, W, H, radius):
+    """cutoff-only bounds, image sizes that are not multiples of the tile, a 17x9 image (two
+    tiles, ~30k entries in one), and a camera inside a dense cloud of larger splats (thousands of
+    near-plane culls, 1.5M entries, tiles over 2048 entries): splats, lists and ranges
+    bit-exact, image > 60 dB."""
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case 30k {W}x{H} radius={radius} bounds={bounds} sort_mode={sort_mode}")
+    if radius < 2.0:  # inside the cloud
+        sc = scenes.make_scene(30_000, seed=4, mean_sigma=1.0, log_scale_mean=-2.5)
+    else:
+        sc = scenes.make_scene(30_000, seed=4, mean_sigma=2.0)
+    r, scene, cam, img, sc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, camera.orbit_view(radius, 10.0, 30.0), W, H,
+                                                              bounds, sort_mode=sort_mode)
+    gs, gc = r.read_back_splats()
+    _check_splats(gs, gc, os_, oc)
+    ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+    _, _, _, ranges, _ = r.read_back()
+    longest = int((ranges[:, 1] - ranges[:, 0]).max()) if ranges.size else 0
+    ref, _ = oracle.render(sc, ocam, bounds=ob)
+    p = oracle.psnr(img[..., :3], ref)
+    log(f"  entries {ctl.entries}, visible {ctl.visible}, near-culled {ctl.cull_counts[1]}, longest tile "
+        f"{longest}, PSNR {p:.2f} dB")
+    assert p > 60.0
+    r.close()
+    scene.close()
+
+
 @pytest.mark.parametrize("exact_alpha,ppt,loop", [(False, 1, "culled"), (False, 1, "predicated"), (False, 1, "divergent"),
                                                   (True, 1, "predicated"), (False, 2, "predicated"),
                                                   (True, 4, "predicated")])
