@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -168,6 +169,10 @@ struct gsr_renderer {
     // Per tile (sized for the largest camera seen).
     DeviceBuffer ranges;
     uint32_t rangesTiles = 0;
+    // The tile depth sort's tile order for the current grid: centre-out (row distance from the
+    // middle row, then column distance from the middle column).
+    DeviceBuffer tileOrder;
+    uint32_t orderTilesX = 0, orderTilesY = 0;
     // Last frame.
     gsr_camera lastCam{};
     bool haveFrame = false;
@@ -684,7 +689,8 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
-                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth};
+                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth,
+                                  &r->tileOrder};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     return s;
@@ -705,6 +711,22 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         GSR_HIP(hipStreamSynchronize(stream));
         GSR_HIP(r->ranges.allocate(size_t(tiles) * 8));
         r->rangesTiles = tiles;
+    }
+    if (!r->binning && !r->settings.full_key_sort &&
+        (cam->tiles_x != r->orderTilesX || cam->tiles_y != r->orderTilesY)) {
+        std::vector<uint32_t> order(tiles);
+        for (uint32_t t = 0; t < tiles; ++t) order[t] = t;
+        const int64_t tx = cam->tiles_x, ty = cam->tiles_y;
+        auto centreKey = [&](uint32_t t) {
+            const int64_t row = t / tx, col = t % tx;
+            return std::llabs(2 * row - (ty - 1)) * (2 * tx + 1) + std::llabs(2 * col - (tx - 1));
+        };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return centreKey(a) < centreKey(b); });
+        GSR_HIP(hipStreamSynchronize(stream));
+        GSR_HIP(r->tileOrder.allocate(size_t(tiles) * 4));
+        GSR_HIP(hipMemcpy(r->tileOrder.ptr, order.data(), size_t(tiles) * 4, hipMemcpyHostToDevice));
+        r->orderTilesX = cam->tiles_x;
+        r->orderTilesY = cam->tiles_y;
     }
     const uint32_t rangeWords = 2 * tiles;
     const uint32_t preBlocks = (std::max(sc->count, rangeWords) + kPreprocessThreads - 1) / kPreprocessThreads;
@@ -803,7 +825,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
             GSR_HIP(hipGetLastError());
             launchTileDepthSort(keys, vals, which ? r->keysA.as<uint64_t>() : r->keysB.as<uint64_t>(),
                                 which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>(), r->ranges.as<uint2>(),
-                                tiles, stream);
+                                r->tileOrder.as<uint32_t>(), tiles, stream);
             GSR_HIP(hipGetLastError());
             GSR_HIP(mark(5));
         } else {

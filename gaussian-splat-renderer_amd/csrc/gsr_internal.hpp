@@ -108,9 +108,10 @@ int launchRadixSort(const SortWork& work, uint64_t* keysA, uint32_t* valuesA, ui
 // Per-tile depth order (tilesort.hip): after a stable sort of 64-bit (tile << 32 | depth) keys
 // by their tile bits only, sorts each tile's range by (depth bits, gaussian index) in place;
 // the lists then equal those of the full 64-bit sort. keysTmp/valuesTmp: scratch of the list's
-// size (used for ranges longer than 2048 entries).
+// size (used for ranges longer than 2048 entries). order: a permutation of the tile ids, the
+// order in which the tiles are taken (scheduling only; the lists do not depend on it).
 void launchTileDepthSort(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp, const uint2* ranges,
-                         uint32_t tileCount, hipStream_t stream);
+                         const uint32_t* order, uint32_t tileCount, hipStream_t stream);
 int launchRadixSort32(const SortWork& work, uint32_t* keysA, uint32_t* valuesA, uint32_t* keysB, uint32_t* valuesB,
                       uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, hipStream_t stream);
 void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,

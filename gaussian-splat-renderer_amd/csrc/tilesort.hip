@@ -28,7 +28,9 @@
 //     (such a loop makes the compiler keep the 32 words' addresses live across iterations,
 //     doubling the kernel's registers).
 // The loads are striped (word r of lane l is input entry r * 64 + l: coalesced; the input order
-// of a sort is arbitrary), the stores follow the network positions.
+// of a sort is arbitrary), the stores follow the network positions. Tiles are taken in the
+// order the caller gives (the frame passes the tiles centre-out: dense tiles, the long sorts,
+// tend to be central and so start first rather than last; the lists do not depend on it).
 #include "gsr_internal.hpp"
 
 namespace gsr {
@@ -267,22 +269,26 @@ __device__ void groupSortLong(uint64_t* keys, uint32_t* values, uint64_t* keysTm
 
 // 5 workgroups per CU: 96 VGPRs, the most the 32-word sort fits in without spills.
 __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
-                                                             uint64_t* __restrict__ keysTmp,
-                                                             uint32_t* __restrict__ valuesTmp,
-                                                             const uint2* __restrict__ ranges, uint32_t tileCount) {
+                                                                uint64_t* __restrict__ keysTmp,
+                                                                uint32_t* __restrict__ valuesTmp,
+                                                                const uint2* __restrict__ ranges,
+                                                                const uint32_t* __restrict__ order,
+                                                                uint32_t tileCount) {
     __shared__ uint64_t sW[kLdsChunk];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t first = blockIdx.x * kTsWaves;
     uint32_t longMask = 0;  // the workgroup's ranges longer than 2048 (same in every thread)
+    uint32_t tiles[kTsWaves];
 #pragma unroll
     for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v) {
-        if (first + v < tileCount) {
-            const uint2 rg = ranges[first + v];
+        tiles[v] = first + v < tileCount ? order[first + v] : tileCount;
+        if (tiles[v] < tileCount) {
+            const uint2 rg = ranges[tiles[v]];
             if (rg.y - rg.x > kWaveChunk) longMask |= 1u << v;
         }
     }
     // 1. One range of up to 2048 entries per wave.
-    const uint32_t tile = first + wave;
+    const uint32_t tile = tiles[wave];
     if (tile < tileCount) {
         const uint2 range = ranges[tile];
         const uint32_t n = range.y - range.x;
@@ -292,17 +298,17 @@ __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __rest
     // 2. The longer ones, one at a time by the whole workgroup.
     __syncthreads();
     for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v)
-        if (longMask & (1u << v)) groupSortLong(keys, values, keysTmp, valuesTmp, first + v, ranges[first + v], sW);
+        if (longMask & (1u << v)) groupSortLong(keys, values, keysTmp, valuesTmp, tiles[v], ranges[tiles[v]], sW);
 }
 
 }  // namespace
 
 void launchTileDepthSort(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp, const uint2* ranges,
-                         uint32_t tileCount, hipStream_t stream) {
+                         const uint32_t* order, uint32_t tileCount, hipStream_t stream) {
     if (tileCount == 0) return;
     const uint32_t blocks = (tileCount + kTsWaves - 1) / kTsWaves;
     hipLaunchKernelGGL(kTileDepthSort, dim3(blocks), dim3(kTsThreads), 0, stream, keys, values, keysTmp, valuesTmp,
-                       ranges, tileCount);
+                       ranges, order, tileCount);
 }
 
 }  // namespace gsr
