@@ -32,6 +32,15 @@ namespace gsr {
 namespace {
 
 constexpr uint32_t kXcds = 8;
+
+// Workgroup b runs on XCD b % 8, which renders tile rows xcd, xcd + 8, ... (neighbouring tiles
+// share that XCD's L2; every XCD gets an even share of dense rows). An XCD takes its rows
+// centre-out: slot s of n is row k = m, m + 1, m - 1, m + 2, ... (m = (n - 1) / 2), so the dense
+// middle of the image starts first and the sparse edges fill the tail.
+__device__ __forceinline__ uint32_t centreOut(uint32_t s, uint32_t n) {
+    const uint32_t m = (n - 1u) / 2u;
+    return (s & 1u) ? m + (s + 1u) / 2u : m - s / 2u;
+}
 constexpr int kBatch = 256;
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLog2MinAlpha = -7.994353436858858f;  // log2(1/255)
@@ -152,7 +161,7 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     // Row-interleaved XCD mapping: workgroup b runs on XCD b % 8; XCD x renders tile rows
     // x, x + 8, x + 16, ... in row-major order.
     const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
-    const uint32_t row = xcd + kXcds * (idx / tilesX);
+    const uint32_t row = xcd + kXcds * centreOut(idx / tilesX, gridDim.x / kXcds / tilesX);
     const uint32_t tile = row * tilesX + idx % tilesX;
     if (tile >= tileCount) return;  // uniform
     const uint32_t t = threadIdx.x;
@@ -320,7 +329,7 @@ __global__ __launch_bounds__(256) void kRenderCulled(const float4* __restrict__ 
     __shared__ __align__(16) uint32_t sList[4][kBatch + 4];  // per wave: record offsets in list order
 
     const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
-    const uint32_t row = xcd + kXcds * (idx / tilesX);
+    const uint32_t row = xcd + kXcds * centreOut(idx / tilesX, gridDim.x / kXcds / tilesX);
     const uint32_t tile = row * tilesX + idx % tilesX;
     if (tile >= tileCount) return;  // uniform
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
@@ -546,7 +555,7 @@ __global__ __launch_bounds__(256) void kRenderStp(const float4* __restrict__ spl
     __shared__ float sRayC[kBatch];   // b2
 
     const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
-    const uint32_t row = xcd + kXcds * (idx / tilesX);
+    const uint32_t row = xcd + kXcds * centreOut(idx / tilesX, gridDim.x / kXcds / tilesX);
     const uint32_t tile = row * tilesX + idx % tilesX;
     if (tile >= tileCount) return;  // uniform
     const uint32_t t = threadIdx.x;
