@@ -564,7 +564,16 @@ static hipError_t adaptCapacity(gsr_renderer* r, hipStream_t stream) {
         if (e != hipSuccess) return e;
     }
     const hipError_t e = allocateList(r, newCap, stream);
-    if (e == hipSuccess) ++r->listResizes;
+    if (e == hipSuccess) {
+        ++r->listResizes;
+        return e;
+    }
+    // The new size did not fit: back to the old one (the frame reports the failure); if even
+    // that fails, the renderer has no lists and every later frame reports OUT_OF_MEMORY.
+    (void)hipGetLastError();
+    for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB, &r->sortStatus}) (void)b->releaseAsync(stream);
+    r->capacity = 0;
+    (void)allocateList(r, cap, stream);
     return e;
 }
 
@@ -709,6 +718,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     // Per-size buffers (onRenderTargetsChanged, splat_tile_renderer.cpp:537-570).
     if (tiles > r->rangesTiles) {
         GSR_HIP(hipStreamSynchronize(stream));
+        r->rangesTiles = 0;  // no buffer until the allocation succeeds (a failed one is retried)
         GSR_HIP(r->ranges.allocate(size_t(tiles) * 8));
         r->rangesTiles = tiles;
     }
@@ -723,6 +733,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         };
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return centreKey(a) < centreKey(b); });
         GSR_HIP(hipStreamSynchronize(stream));
+        r->orderTilesX = r->orderTilesY = 0;  // no table until it is uploaded (a failure is retried)
         GSR_HIP(r->tileOrder.allocate(size_t(tiles) * 4));
         GSR_HIP(hipMemcpy(r->tileOrder.ptr, order.data(), size_t(tiles) * 4, hipMemcpyHostToDevice));
         r->orderTilesX = cam->tiles_x;
@@ -752,6 +763,8 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         collectStats(r);
         GSR_HIP(adaptCapacity(r, stream));
     }
+    if (r->capacity == 0)
+        return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_render: the entry lists could not be reallocated after a failed resize");
     const bool prof = r->settings.profile_stages != 0;
     auto mark = [&](int i) -> hipError_t { return prof ? hipEventRecord(r->events[i], stream) : hipSuccess; };
     gsr_control* control = r->control.as<gsr_control>();

@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -445,9 +446,13 @@ class Scene:
         _check(_scene_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)), "gsr_scene_create")
         self._keep = []  # copied to the device
         self.handle, self.count, self.sh_degree, self.format, self.ctx = h, int(desc.count), sh_degree, fmt, ctx
+        self._renderers = weakref.WeakSet()  # open renderers bound to this scene
 
     def close(self) -> None:
         if self.handle:
+            if any(r.handle for r in list(self._renderers)):
+                raise GsrError("Scene.close: a renderer bound to this scene is still open (the scene must outlive "
+                               "its renderers; close them first)")
             _scene_destroy(self.handle)
             self.handle = None
         for b in self._device_arrays:
@@ -490,6 +495,7 @@ class Renderer:
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
+        scene._renderers.add(self)
         self.sort_mode = sort_mode
         self.render_splat = render_splat
         self._image: Optional[DeviceBuffer] = None
