@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -169,10 +170,18 @@ struct gsr_renderer {
     // Per tile (sized for the largest camera seen).
     DeviceBuffer ranges;
     uint32_t rangesTiles = 0;
-    // The tile depth sort's tile order for the current grid: centre-out (row distance from the
-    // middle row, then column distance from the middle column).
-    DeviceBuffer tileOrder;
-    uint32_t orderTilesX = 0, orderTilesY = 0;
+    // The tile depth sort's tile order, one table per tile grid rendered so far: centre-out (row
+    // distance from the middle row, then column distance from the middle column).
+    struct TileOrder {
+        uint32_t tilesX = 0, tilesY = 0;
+        DeviceBuffer table;
+    };
+    std::vector<std::unique_ptr<TileOrder>> tileOrders;
+    const TileOrder* findOrder(uint32_t tx, uint32_t ty) const {
+        for (const auto& o : tileOrders)
+            if (o->tilesX == tx && o->tilesY == ty) return o.get();
+        return nullptr;
+    }
     // Last frame.
     gsr_camera lastCam{};
     bool haveFrame = false;
@@ -698,10 +707,10 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
-                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth,
-                                  &r->tileOrder};
+                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
+    for (const auto& o : r->tileOrders) s += o->table.bytes;
     return s;
 }
 
@@ -722,22 +731,28 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         GSR_HIP(r->ranges.allocate(size_t(tiles) * 8));
         r->rangesTiles = tiles;
     }
-    if (!r->binning && !r->settings.full_key_sort &&
-        (cam->tiles_x != r->orderTilesX || cam->tiles_y != r->orderTilesY)) {
-        std::vector<uint32_t> order(tiles);
-        for (uint32_t t = 0; t < tiles; ++t) order[t] = t;
-        const int64_t tx = cam->tiles_x, ty = cam->tiles_y;
-        auto centreKey = [&](uint32_t t) {
-            const int64_t row = t / tx, col = t % tx;
-            return std::llabs(2 * row - (ty - 1)) * (2 * tx + 1) + std::llabs(2 * col - (tx - 1));
-        };
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return centreKey(a) < centreKey(b); });
-        GSR_HIP(hipStreamSynchronize(stream));
-        r->orderTilesX = r->orderTilesY = 0;  // no table until it is uploaded (a failure is retried)
-        GSR_HIP(r->tileOrder.allocate(size_t(tiles) * 4));
-        GSR_HIP(hipMemcpy(r->tileOrder.ptr, order.data(), size_t(tiles) * 4, hipMemcpyHostToDevice));
-        r->orderTilesX = cam->tiles_x;
-        r->orderTilesY = cam->tiles_y;
+    const uint32_t* tileOrder = nullptr;
+    if (!r->binning && !r->settings.full_key_sort) {
+        const gsr_renderer::TileOrder* o = r->findOrder(cam->tiles_x, cam->tiles_y);
+        if (!o) {  // a new grid: its table, kept for later frames of the size
+            std::vector<uint32_t> order(tiles);
+            for (uint32_t t = 0; t < tiles; ++t) order[t] = t;
+            const int64_t tx = cam->tiles_x, ty = cam->tiles_y;
+            auto centreKey = [&](uint32_t t) {
+                const int64_t row = t / tx, col = t % tx;
+                return std::llabs(2 * row - (ty - 1)) * (2 * tx + 1) + std::llabs(2 * col - (tx - 1));
+            };
+            std::stable_sort(order.begin(), order.end(),
+                             [&](uint32_t a, uint32_t b) { return centreKey(a) < centreKey(b); });
+            auto entry = std::make_unique<gsr_renderer::TileOrder>();
+            GSR_HIP(entry->table.allocate(size_t(tiles) * 4));
+            GSR_HIP(hipMemcpy(entry->table.ptr, order.data(), size_t(tiles) * 4, hipMemcpyHostToDevice));
+            entry->tilesX = cam->tiles_x;
+            entry->tilesY = cam->tiles_y;
+            r->tileOrders.push_back(std::move(entry));
+            o = r->tileOrders.back().get();
+        }
+        tileOrder = o->table.as<uint32_t>();
     }
     const uint32_t rangeWords = 2 * tiles;
     const uint32_t preBlocks = (std::max(sc->count, rangeWords) + kPreprocessThreads - 1) / kPreprocessThreads;
@@ -838,7 +853,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
             GSR_HIP(hipGetLastError());
             launchTileDepthSort(keys, vals, which ? r->keysA.as<uint64_t>() : r->keysB.as<uint64_t>(),
                                 which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>(), r->ranges.as<uint2>(),
-                                r->tileOrder.as<uint32_t>(), tiles, stream);
+                                tileOrder, tiles, stream);
             GSR_HIP(hipGetLastError());
             GSR_HIP(mark(5));
         } else {
@@ -1018,6 +1033,177 @@ gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* ou
     }
     launchExclusiveScan(in, out, count, ctx->scratch.as<uint32_t>(), totalDevice, stream);
     GSR_HIP(hipGetLastError());
+    return GSR_OK;
+}
+
+}  // extern "C"
+
+// --- Batched multi-view -------------------------------------------------------------------------
+// Views go round-robin to `branches` streams, each with its own renderer; the branches are forked
+// from the caller's stream and joined back with events, so the batch is stream-ordered like one
+// gsr_render, and the same calls on a capturing stream record one graph with parallel branches.
+// Small views do not fill 256 CUs (a 640x360 view is 920 tiles); side by side they do.
+struct gsr_batch {
+    gsr_context* ctx = nullptr;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
+    hipStream_t captureStream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void releaseGraph(gsr_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+gsr_status checkBatchArgs(gsr_batch* b, gsr_renderer* const* renderers, uint32_t viewCount, const gsr_camera* cams,
+                          void* const* outImages, const char* what) {
+    if (!b || !renderers || (viewCount && (!cams || !outImages)))
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: null argument", what);
+    const uint32_t used = std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
+    for (uint32_t i = 0; i < used; ++i) {
+        if (!renderers[i]) return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: renderer %u is null", what, i);
+        if (renderers[i]->ctx != b->ctx)
+            return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: renderer %u belongs to another context", what, i);
+        for (uint32_t j = 0; j < i; ++j)
+            if (renderers[j] == renderers[i])
+                return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: renderers %u and %u are the same (one per branch)", what,
+                            j, i);
+    }
+    return GSR_OK;
+}
+
+// Fork, render view v on branch v % branches, join. Every fork is joined, also after a failure.
+gsr_status batchRecord(gsr_batch* b, gsr_renderer* const* renderers, uint32_t viewCount, const gsr_camera* cams,
+                       void* const* outImages, hipStream_t stream) {
+    const uint32_t branches = std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
+    if (branches == 0) return GSR_OK;
+    GSR_HIP(hipEventRecord(b->fork, stream));
+    for (uint32_t i = 0; i < branches; ++i) GSR_HIP(hipStreamWaitEvent(b->streams[i], b->fork, 0));
+    gsr_status st = GSR_OK;
+    for (uint32_t v = 0; v < viewCount && st == GSR_OK; ++v)
+        st = gsr_render(renderers[v % branches], &cams[v], outImages[v], b->streams[v % branches]);
+    const std::string err = st == GSR_OK ? std::string() : g_lastError;
+    for (uint32_t i = 0; i < branches; ++i) {
+        GSR_HIP(hipEventRecord(b->joins[i], b->streams[i]));
+        GSR_HIP(hipStreamWaitEvent(stream, b->joins[i], 0));
+    }
+    if (st != GSR_OK) g_lastError = err;
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+gsr_status gsr_batch_create(gsr_context* ctx, uint32_t branches, gsr_batch** out) {
+    if (!ctx || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_create: null argument");
+    *out = nullptr;
+    if (branches == 0 || branches > 16)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_create: branches must be 1..16, got %u", branches);
+    GSR_HIP(hipSetDevice(ctx->device));
+    gsr_batch* b = new (std::nothrow) gsr_batch();
+    if (!b) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_batch_create: host allocation");
+    b->ctx = ctx;
+    hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+    for (uint32_t i = 0; i < branches && e == hipSuccess; ++i) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);  // default priority
+        if (e == hipSuccess) b->streams.push_back(st);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e == hipSuccess) b->joins.push_back(ev);
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->captureStream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        gsr_batch_destroy(b);
+        return hipFail(e, "gsr_batch_create");
+    }
+    *out = b;
+    return GSR_OK;
+}
+
+void gsr_batch_destroy(gsr_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    releaseGraph(b);
+    for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->captureStream) (void)hipStreamDestroy(b->captureStream);
+    delete b;
+}
+
+uint32_t gsr_batch_branches(const gsr_batch* b) { return b ? uint32_t(b->streams.size()) : 0; }
+
+gsr_status gsr_batch_render(gsr_batch* b, gsr_renderer* const* renderers, uint32_t viewCount, const gsr_camera* cams,
+                            void* const* outImages, void* streamArg) {
+    gsr_status st = checkBatchArgs(b, renderers, viewCount, cams, outImages, "gsr_batch_render");
+    if (st != GSR_OK) return st;
+    GSR_HIP(hipSetDevice(b->ctx->device));
+    return batchRecord(b, renderers, viewCount, cams, outImages,
+                       streamArg ? static_cast<hipStream_t>(streamArg) : b->ctx->stream);
+}
+
+gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint32_t viewCount,
+                             const gsr_camera* cams, void* const* outImages) {
+    gsr_status st = checkBatchArgs(b, renderers, viewCount, cams, outImages, "gsr_batch_capture");
+    if (st != GSR_OK) return st;
+    const uint32_t used = std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
+    for (uint32_t i = 0; i < used; ++i) {
+        const gsr_renderer* r = renderers[i];
+        if (r->statHost || r->settings.profile_stages)
+            return fail(GSR_ERROR_INVALID_ARGUMENT,
+                        "gsr_batch_capture: renderer %u reads back statistics or stage times (adaptive_list, "
+                        "frame_stats, profile_stages); a captured batch takes plain renderers", i);
+    }
+    // Every view's size must have been rendered by its branch's renderer before (a new size
+    // allocates and synchronises, which a capture cannot record).
+    for (uint32_t v = 0; v < viewCount; ++v) {
+        const gsr_renderer* r = renderers[v % used];
+        const uint32_t tiles = cams[v].tiles_x * cams[v].tiles_y;
+        const bool orderReady =
+            r->binning || r->settings.full_key_sort || r->findOrder(cams[v].tiles_x, cams[v].tiles_y) != nullptr;
+        if (tiles > r->rangesTiles || !orderReady || !r->haveFrame)
+            return fail(GSR_ERROR_INVALID_ARGUMENT,
+                        "gsr_batch_capture: view %u (%ux%u) is a size its renderer has not rendered yet: render the "
+                        "batch once (gsr_batch_render) before capturing it", v, cams[v].width, cams[v].height);
+    }
+    GSR_HIP(hipSetDevice(b->ctx->device));
+    releaseGraph(b);
+    GSR_HIP(hipStreamBeginCapture(b->captureStream, hipStreamCaptureModeThreadLocal));
+    st = batchRecord(b, renderers, viewCount, cams, outImages, b->captureStream);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->captureStream, &graph);
+    if (st != GSR_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hipFail(e, "gsr_batch_capture: hipStreamEndCapture");
+    }
+    b->graph = graph;
+    const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        releaseGraph(b);
+        return hipFail(ei, "gsr_batch_capture: hipGraphInstantiate");
+    }
+    return GSR_OK;
+}
+
+gsr_status gsr_batch_replay(gsr_batch* b, void* streamArg) {
+    if (!b) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_replay: null batch");
+    if (!b->exec) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_replay: nothing captured");
+    GSR_HIP(hipSetDevice(b->ctx->device));
+    GSR_HIP(hipGraphLaunch(b->exec, streamArg ? static_cast<hipStream_t>(streamArg) : b->ctx->stream));
     return GSR_OK;
 }
 

@@ -225,6 +225,12 @@ _ply_load_memory = _sig("gsr_ply_load_memory", _i32,
                         [_vp, ctypes.c_uint64, ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(PlySceneStruct))])
 _ply_free = _sig("gsr_ply_free", None, [ctypes.POINTER(PlySceneStruct)])
 _exclusive_scan = _sig("gsr_exclusive_scan", _i32, [_vp, _vp, _vp, _u32, _vp, _vp])
+_batch_create = _sig("gsr_batch_create", _i32, [_vp, _u32, ctypes.POINTER(_vp)])
+_batch_destroy = _sig("gsr_batch_destroy", None, [_vp])
+_batch_branches = _sig("gsr_batch_branches", _u32, [_vp])
+_batch_render = _sig("gsr_batch_render", _i32, [_vp, _vp, _u32, _vp, _vp, _vp])
+_batch_capture = _sig("gsr_batch_capture", _i32, [_vp, _vp, _u32, _vp, _vp])
+_batch_replay = _sig("gsr_batch_replay", _i32, [_vp, _vp])
 
 # Every symbol include/gsr.h declares (checked by the CPU tests).
 EXPORTED = (
@@ -235,6 +241,7 @@ EXPORTED = (
     "gsr_renderer_max_capacity gsr_renderer_list_resizes gsr_renderer_last_stats "
     "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_sort_pairs "
     "gsr_exclusive_scan "
+    "gsr_batch_create gsr_batch_destroy gsr_batch_branches gsr_batch_render gsr_batch_capture gsr_batch_replay "
     "gsr_ply_load gsr_ply_load_memory gsr_ply_free"
 ).split()
 
@@ -601,6 +608,84 @@ class Renderer:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many views of one scene on `branches` streams at once (gsr_batch_*): view v is rendered by
+    renderers[v % branches] on branch stream v % branches, forked from and joined back to the
+    caller's stream; capture() records the batch into a HIP graph, replay() launches it."""
+
+    def __init__(self, ctx: Context, branches: int):
+        h = _vp()
+        _check(_batch_create(ctx.handle, int(branches), ctypes.byref(h)), "gsr_batch_create")
+        self.handle, self.ctx, self.branches = h, ctx, int(branches)
+        self._captured = None  # keeps the captured arrays and objects alive
+
+    @staticmethod
+    def _arrays(renderers, cams, outs):
+        rs = (_vp * len(renderers))(*[r.handle for r in renderers])
+        cs = (Camera * len(cams))(*cams)
+        os_ = (_vp * len(outs))(*[int(o) for o in outs])
+        return rs, cs, os_
+
+    def render(self, renderers, cams, outs, stream: Optional[int] = None) -> None:
+        """cams: Camera per view; outs: device pointers (one image per view, the renderer's format)."""
+        if len(cams) != len(outs):
+            raise GsrError("Batch.render: one output per camera")
+        rs, cs, os_ = self._arrays(renderers, cams, outs)
+        _check(_batch_render(self.handle, rs, len(cams), cs, os_, stream), "gsr_batch_render")
+
+    def capture(self, renderers, cams, outs) -> None:
+        if len(cams) != len(outs):
+            raise GsrError("Batch.capture: one output per camera")
+        rs, cs, os_ = self._arrays(renderers, cams, outs)
+        _check(_batch_capture(self.handle, rs, len(cams), cs, os_), "gsr_batch_capture")
+        self._captured = (list(renderers), rs, cs, os_)
+
+    def replay(self, stream: Optional[int] = None) -> None:
+        _check(_batch_replay(self.handle, stream), "gsr_batch_replay")
+
+    def close(self) -> None:
+        if self.handle:
+            _batch_destroy(self.handle)
+            self.handle = None
+        self._captured = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_batch(ctx: Context, make_renderer, cams, outs, candidates=(1, 2, 4, 8), replays: int = 5, timer=None):
+    """The batch's autotuner: renders the views with each branch count in `candidates` (a
+    captured graph, `replays` timed replays after one warm-up) and returns (best branch count,
+    {branches: ms per replay}). make_renderer() makes one renderer of the scene; timer(fn) times
+    fn on the context's stream and returns ms (hipEvents)."""
+    results = {}
+    pool = []
+    for b in candidates:
+        while len(pool) < b:
+            pool.append(make_renderer())
+        batch = Batch(ctx, b)
+        try:
+            batch.render(pool[:b], cams, outs)  # every size once per renderer: its buffers
+            ctx.synchronize()
+            batch.capture(pool[:b], cams, outs)
+            batch.replay()
+            ctx.synchronize()
+
+            def run():
+                for _ in range(replays):
+                    batch.replay()
+            results[b] = timer(run) / replays
+        finally:
+            batch.close()
+    best = min(results, key=results.get)
+    for r in pool:
+        r.close()
+    return best, results
 
 
 def sort_pairs(ctx: Context, keys: np.ndarray, values: np.ndarray, key_bits: int = 64, count: Optional[int] = None,

@@ -280,6 +280,29 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
 gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* out, uint32_t count,
                               uint32_t* total_device, void* stream);
 
+/* --- Batched multi-view ------------------------------------------------------------------------
+ * No counterpart in the reference (its benchmark renders one view per frame, application.cpp:
+ * 1042-1057); for callers that render many cameras of one scene at once (dataset views, light
+ * probes, multi-camera rigs). A batch owns `branches` HIP streams (1..16, default priority).
+ * gsr_batch_render renders view v with renderers[v % b] on stream v % b, b = min(branches,
+ * view_count), each branch's views in order, forked from `stream` and joined back to it:
+ * asynchronous and stream-ordered like gsr_render, and the images equal those of one
+ * gsr_render per view. Each branch needs its own renderer of the scene (lists are per
+ * renderer); renderers[0 .. b-1] must be distinct. gsr_batch_capture records the same work into
+ * a HIP graph held by the batch (it replaces the previous one); gsr_batch_replay launches it on
+ * `stream`, one launch for the whole batch, with the cameras, images and renderers of the
+ * capture. A capture needs every view's size rendered before by its branch's renderer (render
+ * the batch once first) and plain renderers (no adaptive_list, frame_stats or profile_stages). */
+typedef struct gsr_batch gsr_batch;
+gsr_status gsr_batch_create(gsr_context* ctx, uint32_t branches, gsr_batch** out);
+void gsr_batch_destroy(gsr_batch* batch);
+uint32_t gsr_batch_branches(const gsr_batch* batch);
+gsr_status gsr_batch_render(gsr_batch* batch, gsr_renderer* const* renderers, uint32_t view_count,
+                            const gsr_camera* cams, void* const* out_images, void* stream);
+gsr_status gsr_batch_capture(gsr_batch* batch, gsr_renderer* const* renderers, uint32_t view_count,
+                             const gsr_camera* cams, void* const* out_images);
+gsr_status gsr_batch_replay(gsr_batch* batch, void* stream);
+
 /* --- PLY scenes (loadGaussianPly, scene/ply_loader.h:1-12 and 86-95) ---------------------------
  * A binary little-endian INRIA 3DGS .ply file decoded and activated on the host into the f32
  * SoA arrays gsr_scene_create takes (format GSR_SCENE_F32_SOA, device_pointers 0):
