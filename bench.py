@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--radius", type=float, default=10.0, help="orbit radius")
     ap.add_argument("--elevation", type=float, default=15.0, help="orbit elevation, degrees")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     ap.add_argument("--profile-frames", type=int, default=0,
                     help="for rocprofv3 counter runs: render this many orbit frames of the configured mode, then "
@@ -70,6 +71,67 @@ class HipTimer:
         ms = ctypes.c_float()
         assert self.hip.hipEventElapsedTime(ctypes.byref(ms), self.start, self.stop) == 0
         return ms.value
+
+
+def multiview_bench(ctx, views=32, count=1_000_000, reps=5):
+    """Batched multi-view: `views` cameras of mixed sizes ({640x360, 1280x720, 1920x1080}, uniform,
+    seeded) on a shell of radius 8..12 around a 1M-gaussian scene of the same distributions,
+    looking at the origin. Sequential = one renderer, one gsr_render per view on one stream;
+    batched = gsr_batch with the branch count the autotuner picks (1, 2, 4 or 8 streams), the
+    whole batch captured into one HIP graph and replayed. Both: `reps` timed batches after a
+    warm-up, host clock around them with the device synchronised."""
+    import numpy as np
+
+    import gsr
+    from gsr import camera, scenes
+
+    rng = np.random.default_rng(5)
+    sizes = [(640, 360), (1280, 720), (1920, 1080)]
+    cams = []
+    for _ in range(views):
+        W, H = sizes[int(rng.integers(0, 3))]
+        d = rng.normal(size=3)
+        eye = d / np.linalg.norm(d) * rng.uniform(8.0, 12.0)
+        cams.append(gsr.camera_from_viewer(camera.look_at(eye, [0.0, 0.0, 0.0]), camera.perspective(60.0, W / H),
+                                           camera.IDENTITY, W, H))
+    pixels = sum(int(c.width) * int(c.height) for c in cams)
+    sc = scenes.make_scene(count, seed=0, mean_sigma=3.0)
+    scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    outs = [gsr.DeviceBuffer(ctx, int(c.width) * int(c.height) * 4) for c in cams]
+
+    def make():
+        return gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+
+    def timed(fn):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    seq = make()
+    for c, o in zip(cams, outs):  # warm-up: every size once
+        seq.render(c, o.ptr)
+
+    def sequential():
+        for _ in range(reps):
+            for c, o in zip(cams, outs):
+                seq.render(c, o.ptr)
+    seq_ms = timed(sequential) / reps
+    seq.close()
+    best, by = gsr.tune_batch(ctx, make, cams, [o.ptr for o in outs], candidates=(1, 2, 4, 8), replays=reps,
+                              timer=timed)
+    for o in outs:
+        o.close()
+    scene.close()
+    print(f"# multiview: {views} views, {pixels / 1e6:.1f} Mpixel: sequential {seq_ms:.3f} ms, batched "
+          f"{by[best]:.3f} ms with {best} branches ({', '.join(f'{k}: {v:.3f}' for k, v in by.items())})",
+          file=sys.stderr, flush=True)
+    return {"views": views, "gaussians": count, "sizes": "uniform of 640x360, 1280x720, 1920x1080",
+            "megapixels": round(pixels / 1e6, 2), "sequential_ms": round(seq_ms, 3),
+            "sequential_mpix_per_s": round(pixels / 1e6 / (seq_ms * 1e-3), 1), "batched_ms": round(by[best], 3),
+            "batched_mpix_per_s": round(pixels / 1e6 / (by[best] * 1e-3), 1), "branches": best,
+            "batched_ms_by_branches": {str(k): round(v, 3) for k, v in by.items()}, "graph": True}
 
 
 def main():
@@ -217,6 +279,34 @@ def main():
         passes = {"tile_radix": (int(tiles).bit_length() + 7) // 8, "per_tile_depth_sort": 1}
         sort_ms = stages["sort"]
 
+    multiview = None if args.no_multiview or args.ply else multiview_bench(ctx)
+
+    # Frames in flight (the reference viewer keeps two, application.h:55): the orbit rendered by
+    # k renderers on k streams (gsr_batch, frame i on branch i % k, each branch its own image),
+    # so one frame's latency-bound kernels overlap the next's. Throughput only: the headline
+    # `value` stays the serial per-frame time.
+    in_flight = {}
+    scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    for k in (2, 4):
+        rs = [gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", sort_mode=args.sort_mode,
+                           render_splat=args.render_splat) for _ in range(k)]
+        imgs = [gsr.DeviceBuffer(ctx, W * H * 4) for _ in range(k)]
+        batch = gsr.Batch(ctx, k)
+        orbit = [cams[(args.warmup + i) % ORBIT_FRAMES] for i in range(args.steps)]
+        targets = [imgs[i % k].ptr for i in range(args.steps)]
+        batch.render(rs, orbit[:k * 4], targets[:k * 4])  # warm-up
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        batch.render(rs, orbit, targets)
+        ctx.synchronize()
+        in_flight[str(k)] = round((time.perf_counter() - t0) * 1e3 / args.steps, 4)
+        batch.close()
+        for x in rs + imgs:
+            x.close()
+        print(f"# {k} frames in flight: {in_flight[str(k)]:.4f} ms/frame over {args.steps} frames", file=sys.stderr,
+              flush=True)
+    scene.close()
+
     cpu = None
     psnr = maxdiff = None
     if not args.no_cpu_baseline:
@@ -273,6 +363,8 @@ def main():
         "max_diff_levels_vs_oracle": maxdiff,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "multiview": multiview,
+        "frames_in_flight_ms_per_frame": in_flight,
     }
     print(json.dumps(line), flush=True)
 
