@@ -10,7 +10,9 @@ frames are timed with HIP events on the render stream.
 
 Prints ONE JSON line: ms/frame (value), per-stage ms, sorted keys/s, the roofline of the
 dominant kernel, the fp16-scene and other-sort-mode frame times, the image quality of one
-frame against the CPU oracle, and the oracle's own CPU time on the same frame (cpu_baseline).
+frame against the CPU oracle, and the oracle's own CPU time on the same frame (cpu_baseline);
+also the batched multi-view throughput (32 cameras of mixed sizes on a 1M-gaussian scene) and
+the orbit's throughput with 2 and 4 frames in flight.
 """
 import argparse
 import ctypes
