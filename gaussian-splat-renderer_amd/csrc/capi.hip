@@ -136,6 +136,22 @@ struct gsr_scene {
     SceneRef ref{};
 };
 
+// Learned colour: the MLP's parameters and the per-gaussian latents (fp32 for training, bf16 for
+// rendering), each with Adam moments and gradients, the packed bf16 weight tiles and the fallback
+// marks.
+struct gsr_colour_model {
+    gsr_context* ctx = nullptr;
+    const gsr_scene* scene = nullptr;
+    uint32_t count = 0;
+    uint32_t seed = 0;
+    DeviceBuffer params, gradParams, mParams, vParams, packs;
+    DeviceBuffer latents, gradLatents, mLatents, vLatents, latentsBf16;
+    DeviceBuffer useMlp, errors, sse;
+    uint32_t step = 0;        // Adam steps since creation or the last set
+    uint32_t streamIndex = 0; // training sample streams used
+    uint32_t mlpCount = 0;    // gaussians marked by the last fallback pass
+};
+
 struct gsr_renderer {
     gsr_context* ctx = nullptr;
     const gsr_scene* scene = nullptr;
@@ -189,6 +205,7 @@ struct gsr_renderer {
     hipStream_t lastStream = nullptr;
     hipEvent_t events[GSR_STAGE_COUNT] = {};
     bool eventsRecorded = false;
+    gsr_colour_model* colour = nullptr;  // learned colour, or null (SH)
 };
 
 extern "C" {
@@ -793,8 +810,16 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     GSR_HIP(mark(0));
     launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
                      r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords,
-                     uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), r->rayDepth.as<float4>(), stream);
+                     uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), r->rayDepth.as<float4>(),
+                     r->colour ? r->colour->useMlp.as<uint8_t>() : nullptr, stream);
     GSR_HIP(hipGetLastError());
+    if (r->colour && r->colour->mlpCount > 0) {
+        const gsr_colour_model* cm = r->colour;
+        launchColourMlp(ref, dc, cm->params.as<float>(), cm->packs.as<uint2>(), cm->latentsBf16.as<uint16_t>(),
+                        cm->useMlp.as<uint8_t>(), r->splats.as<float4>(), uint32_t(r->settings.render_splat),
+                        r->packed.as<uint32_t>(), stream);
+        GSR_HIP(hipGetLastError());
+    }
     GSR_HIP(mark(1));
     launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, r->binning,
                      r->blockOffsets.as<uint32_t>(), control, stream);
@@ -1204,6 +1229,275 @@ gsr_status gsr_batch_replay(gsr_batch* b, void* streamArg) {
     if (!b->exec) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_replay: nothing captured");
     GSR_HIP(hipSetDevice(b->ctx->device));
     GSR_HIP(hipGraphLaunch(b->exec, streamArg ? static_cast<hipStream_t>(streamArg) : b->ctx->stream));
+    return GSR_OK;
+}
+
+/* --- Learned colour --------------------------------------------------------------------------- */
+
+uint32_t gsr_colour_model_param_count(void) { return uint32_t(mlpParamCount()); }
+
+}  // extern "C"
+
+namespace {
+
+// Every model call starts after all work on the device (frames in flight may read the model) and
+// ends with its stream drained.
+hipError_t modelBegin(gsr_colour_model* m) {
+    hipError_t e = hipSetDevice(m->ctx->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e;
+}
+
+SceneRef modelScene(const gsr_colour_model* m) {
+    SceneRef ref = m->scene->ref;
+    ref.degree = m->scene->shDegree;  // the target: the scene's full SH colour
+    return ref;
+}
+
+// After a parameter or latent change: the bf16 weight tiles and latents, and no gaussian marked
+// until the next fallback pass.
+hipError_t modelRefresh(gsr_colour_model* m, hipStream_t stream) {
+    launchMlpPack(m->params.as<float>(), m->packs.as<uint2>(), stream);
+    launchToBf16(m->latents.as<float>(), m->latentsBf16.as<uint16_t>(), m->count * 16, stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemsetAsync(m->useMlp.ptr, 0, std::max<size_t>(m->count, 1), stream);
+    m->mlpCount = 0;
+    return e;
+}
+
+uint32_t trainSeed(const gsr_colour_model* m, uint32_t index) {
+    uint32_t x = m->seed ^ (index * 0x9E3779B9u + 0x632BE5ABu);
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+gsr_status gsr_colour_model_create(gsr_context* ctx, const gsr_scene* scene, uint32_t seed, gsr_colour_model** out) {
+    if (!ctx || !scene || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_create: null argument");
+    *out = nullptr;
+    if (scene->count == 0) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_create: empty scene");
+    if (scene->count > (0xFFFFFFFFu / 16u))
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_create: %u gaussians (at most %u)", scene->count,
+                    0xFFFFFFFFu / 16u);
+    GSR_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<gsr_colour_model> m(new (std::nothrow) gsr_colour_model());
+    if (!m) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_colour_model_create: host allocation");
+    m->ctx = ctx;
+    m->scene = scene;
+    m->count = scene->count;
+    m->seed = seed;
+    const size_t P = size_t(mlpParamCount()), L = size_t(scene->count) * 16;
+    hipError_t e = hipSuccess;
+    for (DeviceBuffer* b : {&m->params, &m->gradParams, &m->mParams, &m->vParams}) {
+        if (e == hipSuccess) e = b->allocate(P * 4);
+        if (e == hipSuccess) e = hipMemset(b->ptr, 0, b->bytes);
+    }
+    for (DeviceBuffer* b : {&m->latents, &m->gradLatents, &m->mLatents, &m->vLatents}) {
+        if (e == hipSuccess) e = b->allocate(L * 4);
+        if (e == hipSuccess) e = hipMemset(b->ptr, 0, b->bytes);
+    }
+    if (e == hipSuccess) e = m->latentsBf16.allocate(L * 2);
+    if (e == hipSuccess) e = m->packs.allocate(size_t(mlpPackedTiles()) * 64 * 8);
+    if (e == hipSuccess) e = m->useMlp.allocate(scene->count);
+    if (e == hipSuccess) e = m->errors.allocate(size_t(scene->count) * 4);
+    if (e == hipSuccess) e = m->sse.allocate(4);
+    if (e != hipSuccess) return hipFail(e, "gsr_colour_model_create: allocation");
+    // He-normal weights (fan-in 19 for layer 1: latent + direction; 64 for layers 2 and 3, the
+    // last scaled by 0.5), zero biases; latents uniform in [-0.5, 0.5).
+    std::vector<float> p(P, 0.0f);
+    uint64_t state = 0x853C49E6748FEA9Bull ^ (uint64_t(seed) << 1);
+    auto uniform = [&]() {
+        state = state * 6364136223846793005ull + 1442695040888963407ull;
+        return (double((state >> 11) & ((1ull << 53) - 1)) + 0.5) / double(1ull << 53);
+    };
+    auto normal = [&]() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(6.283185307179586 * uniform()); };
+    const float s1 = std::sqrt(2.0f / 19.0f), s2 = std::sqrt(2.0f / 64.0f), s3 = 0.5f * std::sqrt(1.0f / 64.0f);
+    for (int r = 0; r < 64; ++r)
+        for (int c = 0; c < 19; ++c) p[size_t(r) * 32 + c] = s1 * float(normal());
+    const size_t w2 = 64 * 32, w3 = w2 + 64 * 64 + 64;
+    for (size_t i = 0; i < 64 * 64; ++i) p[w2 + i] = s2 * float(normal());
+    for (size_t i = 0; i < 3 * 64; ++i) p[w3 + i] = s3 * float(normal());
+    hipStream_t stream = ctx->stream;
+    if ((e = hipMemcpy(m->params.ptr, p.data(), P * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hipFail(e, "gsr_colour_model_create: upload");
+    launchInitLatents(m->latents.as<float>(), uint32_t(L), seed, 0.5f, stream);
+    if ((e = modelRefresh(m.get(), stream)) != hipSuccess) return hipFail(e, "gsr_colour_model_create: init");
+    GSR_HIP(hipStreamSynchronize(stream));
+    *out = m.release();
+    return GSR_OK;
+}
+
+void gsr_colour_model_destroy(gsr_colour_model* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete m;
+}
+
+gsr_status gsr_colour_model_train(gsr_colour_model* m, uint32_t steps, uint32_t samples, float lrWeights,
+                                  float lrLatents, float* lastLoss) {
+    if (!m) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_train: null model");
+    if (samples == 0 || samples % 256 != 0)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_train: samples_per_step %u (a positive multiple of 256)",
+                    samples);
+    if (!(lrWeights >= 0.0f) || !(lrLatents >= 0.0f))
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_train: negative or NaN learning rate");
+    GSR_HIP(modelBegin(m));
+    hipStream_t stream = m->ctx->stream;
+    const SceneRef ref = modelScene(m);
+    const uint32_t P = uint32_t(mlpParamCount()), L = m->count * 16;
+    for (uint32_t i = 0; i < steps; ++i) {
+        ++m->step;
+        GSR_HIP(hipMemsetAsync(m->sse.ptr, 0, 4, stream));
+        launchMlpTrain(ref, m->params.as<float>(), m->packs.as<uint2>(), m->latents.as<float>(),
+                       m->gradParams.as<float>(), m->gradLatents.as<float>(), m->sse.as<float>(),
+                       trainSeed(m, m->streamIndex++), samples, true, stream);
+        launchAdam(m->params.as<float>(), m->gradParams.as<float>(), m->mParams.as<float>(), m->vParams.as<float>(), P,
+                   lrWeights, m->step, stream);
+        launchAdam(m->latents.as<float>(), m->gradLatents.as<float>(), m->mLatents.as<float>(),
+                   m->vLatents.as<float>(), L, lrLatents, m->step, stream);
+        launchMlpPack(m->params.as<float>(), m->packs.as<uint2>(), stream);
+        GSR_HIP(hipGetLastError());
+    }
+    GSR_HIP(modelRefresh(m, stream));
+    float sse = 0.0f;
+    GSR_HIP(hipMemcpyAsync(&sse, m->sse.ptr, 4, hipMemcpyDeviceToHost, stream));
+    GSR_HIP(hipStreamSynchronize(stream));
+    if (lastLoss) *lastLoss = steps ? sse / (3.0f * float(samples)) : 0.0f;
+    return GSR_OK;
+}
+
+gsr_status gsr_colour_model_loss(gsr_colour_model* m, uint32_t seed, uint32_t samples, float* loss) {
+    if (!m || !loss) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_loss: null argument");
+    if (samples == 0 || samples % 256 != 0)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_loss: samples %u (a positive multiple of 256)", samples);
+    GSR_HIP(modelBegin(m));
+    hipStream_t stream = m->ctx->stream;
+    GSR_HIP(hipMemsetAsync(m->sse.ptr, 0, 4, stream));
+    launchMlpTrain(modelScene(m), m->params.as<float>(), m->packs.as<uint2>(), m->latents.as<float>(), nullptr, nullptr,
+                   m->sse.as<float>(), seed, samples, false, stream);
+    GSR_HIP(hipGetLastError());
+    float sse = 0.0f;
+    GSR_HIP(hipMemcpyAsync(&sse, m->sse.ptr, 4, hipMemcpyDeviceToHost, stream));
+    GSR_HIP(hipStreamSynchronize(stream));
+    *loss = sse / (3.0f * float(samples));
+    return GSR_OK;
+}
+
+gsr_status gsr_colour_model_gradients(gsr_colour_model* m, uint32_t seed, uint32_t samples, float* gradParams,
+                                      float* gradLatents, float* loss, uint32_t* gaussians, float* dirs) {
+    if (!m) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_gradients: null model");
+    if (samples == 0 || samples % 256 != 0)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_gradients: samples %u (a positive multiple of 256)",
+                    samples);
+    GSR_HIP(modelBegin(m));
+    hipStream_t stream = m->ctx->stream;
+    const size_t P = size_t(mlpParamCount()), L = size_t(m->count) * 16;
+    // The gradient buffers are zero between training steps (Adam clears them); clear them anyway.
+    GSR_HIP(hipMemsetAsync(m->gradParams.ptr, 0, P * 4, stream));
+    GSR_HIP(hipMemsetAsync(m->gradLatents.ptr, 0, L * 4, stream));
+    GSR_HIP(hipMemsetAsync(m->sse.ptr, 0, 4, stream));
+    launchMlpTrain(modelScene(m), m->params.as<float>(), m->packs.as<uint2>(), m->latents.as<float>(),
+                   m->gradParams.as<float>(), m->gradLatents.as<float>(), m->sse.as<float>(), seed, samples, true,
+                   stream);
+    GSR_HIP(hipGetLastError());
+    if (gradParams) GSR_HIP(hipMemcpyAsync(gradParams, m->gradParams.ptr, P * 4, hipMemcpyDeviceToHost, stream));
+    if (gradLatents) GSR_HIP(hipMemcpyAsync(gradLatents, m->gradLatents.ptr, L * 4, hipMemcpyDeviceToHost, stream));
+    float sse = 0.0f;
+    GSR_HIP(hipMemcpyAsync(&sse, m->sse.ptr, 4, hipMemcpyDeviceToHost, stream));
+    if (gaussians || dirs) {
+        DeviceBuffer g, d;
+        GSR_HIP(g.allocate(size_t(samples) * 4));
+        GSR_HIP(d.allocate(size_t(samples) * 12));
+        launchMlpSamples(seed, 0, samples, m->count, g.as<uint32_t>(), d.as<float>(), stream);
+        GSR_HIP(hipGetLastError());
+        if (gaussians) GSR_HIP(hipMemcpyAsync(gaussians, g.ptr, size_t(samples) * 4, hipMemcpyDeviceToHost, stream));
+        if (dirs) GSR_HIP(hipMemcpyAsync(dirs, d.ptr, size_t(samples) * 12, hipMemcpyDeviceToHost, stream));
+        GSR_HIP(hipStreamSynchronize(stream));
+    }
+    GSR_HIP(hipMemsetAsync(m->gradParams.ptr, 0, P * 4, stream));
+    GSR_HIP(hipMemsetAsync(m->gradLatents.ptr, 0, L * 4, stream));
+    GSR_HIP(hipStreamSynchronize(stream));
+    if (loss) *loss = sse / (3.0f * float(samples));
+    return GSR_OK;
+}
+
+gsr_status gsr_colour_model_get(gsr_colour_model* m, float* params, float* latents) {
+    if (!m) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_get: null model");
+    GSR_HIP(modelBegin(m));
+    if (params) GSR_HIP(hipMemcpy(params, m->params.ptr, size_t(mlpParamCount()) * 4, hipMemcpyDeviceToHost));
+    if (latents) GSR_HIP(hipMemcpy(latents, m->latents.ptr, size_t(m->count) * 64, hipMemcpyDeviceToHost));
+    return GSR_OK;
+}
+
+gsr_status gsr_colour_model_set(gsr_colour_model* m, const float* params, const float* latents) {
+    if (!m) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_set: null model");
+    GSR_HIP(modelBegin(m));
+    hipStream_t stream = m->ctx->stream;
+    if (params) GSR_HIP(hipMemcpy(m->params.ptr, params, size_t(mlpParamCount()) * 4, hipMemcpyHostToDevice));
+    if (latents) GSR_HIP(hipMemcpy(m->latents.ptr, latents, size_t(m->count) * 64, hipMemcpyHostToDevice));
+    // A new starting point: fresh Adam moments.
+    for (DeviceBuffer* b : {&m->mParams, &m->vParams, &m->mLatents, &m->vLatents, &m->gradParams, &m->gradLatents})
+        GSR_HIP(hipMemsetAsync(b->ptr, 0, b->bytes, stream));
+    m->step = 0;
+    GSR_HIP(modelRefresh(m, stream));
+    GSR_HIP(hipStreamSynchronize(stream));
+    return GSR_OK;
+}
+
+gsr_status gsr_colour_model_evaluate(gsr_colour_model* m, const uint32_t* gaussians, const float* dirs, uint32_t n,
+                                     float* rgb) {
+    if (!m || (n && (!gaussians || !dirs || !rgb)))
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_evaluate: null argument");
+    for (uint32_t i = 0; i < n; ++i)
+        if (gaussians[i] >= m->count)
+            return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_evaluate: gaussian %u of %u", gaussians[i],
+                        m->count);
+    if (n == 0) return GSR_OK;
+    GSR_HIP(modelBegin(m));
+    hipStream_t stream = m->ctx->stream;
+    DeviceBuffer g, d, out;
+    GSR_HIP(g.allocate(size_t(n) * 4));
+    GSR_HIP(d.allocate(size_t(n) * 12));
+    GSR_HIP(out.allocate(size_t(n) * 12));
+    GSR_HIP(hipMemcpy(g.ptr, gaussians, size_t(n) * 4, hipMemcpyHostToDevice));
+    GSR_HIP(hipMemcpy(d.ptr, dirs, size_t(n) * 12, hipMemcpyHostToDevice));
+    launchMlpEvaluate(m->params.as<float>(), m->packs.as<uint2>(), m->latents.as<float>(), g.as<uint32_t>(),
+                      d.as<float>(), n, out.as<float>(), stream);
+    GSR_HIP(hipGetLastError());
+    GSR_HIP(hipMemcpyAsync(rgb, out.ptr, size_t(n) * 12, hipMemcpyDeviceToHost, stream));
+    GSR_HIP(hipStreamSynchronize(stream));
+    return GSR_OK;
+}
+
+gsr_status gsr_colour_model_fallback(gsr_colour_model* m, float threshold, uint32_t* mlpCount, float* errors) {
+    if (!m) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_fallback: null model");
+    GSR_HIP(modelBegin(m));
+    hipStream_t stream = m->ctx->stream;
+    launchMlpFallback(modelScene(m), m->params.as<float>(), m->packs.as<uint2>(), m->latentsBf16.as<uint16_t>(),
+                      threshold, m->useMlp.as<uint8_t>(), m->errors.as<float>(), stream);
+    GSR_HIP(hipGetLastError());
+    std::vector<uint8_t> marks(m->count);
+    GSR_HIP(hipMemcpyAsync(marks.data(), m->useMlp.ptr, m->count, hipMemcpyDeviceToHost, stream));
+    if (errors) GSR_HIP(hipMemcpyAsync(errors, m->errors.ptr, size_t(m->count) * 4, hipMemcpyDeviceToHost, stream));
+    GSR_HIP(hipStreamSynchronize(stream));
+    uint32_t c = 0;
+    for (uint8_t v : marks) c += v ? 1u : 0u;
+    m->mlpCount = c;
+    if (mlpCount) *mlpCount = c;
+    return GSR_OK;
+}
+
+gsr_status gsr_renderer_set_colour_model(gsr_renderer* r, gsr_colour_model* m) {
+    if (!r) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_set_colour_model: null renderer");
+    if (m && m->scene != r->scene)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_set_colour_model: the model belongs to another scene");
+    r->colour = m;
     return GSR_OK;
 }
 

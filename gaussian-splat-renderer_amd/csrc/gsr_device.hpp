@@ -232,4 +232,80 @@ __device__ __forceinline__ uint32_t coverageCount(const Coverage& cv) {
     return n;
 }
 
+// --- SH colour (shared by the preprocess and the learned-colour training targets) ------------
+
+constexpr float kC0 = 0.28209479177387814f;
+constexpr float kC1 = 0.4886025119029199f;
+constexpr float kC2_0 = 1.0925484305920792f, kC2_1 = -1.0925484305920792f, kC2_2 = 0.31539156525252005f,
+                kC2_3 = -1.0925484305920792f, kC2_4 = 0.5462742152960396f;
+constexpr float kC3_0 = -0.5900435899266435f, kC3_1 = 2.890611442640554f, kC3_2 = -0.4570457994644658f,
+                kC3_3 = 0.3731763325901154f, kC3_4 = -0.4570457994644658f, kC3_5 = 1.445305721320277f,
+                kC3_6 = -0.5900435899266435f;
+
+// evalSH (splat_math.cpp:129-151), one channel; c = the 16 coefficients of that channel.
+// evalSH before its final clamp: the sum + 0.5.
+__device__ __forceinline__ float evalShRaw(int degree, const float* c, float x, float y, float z) {
+    float result = kC0 * c[0];
+    if (degree > 0) {
+        result = result - kC1 * y * c[1] + kC1 * z * c[2] - kC1 * x * c[3];
+        if (degree > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z;
+            const float xy = x * y, yz = y * z, xz = x * z;
+            result = result + kC2_0 * xy * c[4] + kC2_1 * yz * c[5] + kC2_2 * (2.0f * zz - xx - yy) * c[6] +
+                     kC2_3 * xz * c[7] + kC2_4 * (xx - yy) * c[8];
+            if (degree > 2) {
+                result = result + kC3_0 * y * (3.0f * xx - yy) * c[9] + kC3_1 * xy * z * c[10] +
+                         kC3_2 * y * (4.0f * zz - xx - yy) * c[11] +
+                         kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * c[12] +
+                         kC3_4 * x * (4.0f * zz - xx - yy) * c[13] + kC3_5 * z * (xx - yy) * c[14] +
+                         kC3_6 * x * (xx - 3.0f * yy) * c[15];
+            }
+        }
+    }
+    return result + 0.5f;
+}
+
+__device__ __forceinline__ float evalShChannel(int degree, const float* c, float x, float y, float z) {
+    const float result = evalShRaw(degree, c, x, y, z);
+    return result < 0.0f ? 0.0f : result;
+}
+
+// The coefficients of the evaluated bands, channel-major: co[ch][k].
+template <int FMT>
+__device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, float co[3][16]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) co[0][k] = co[1][k] = co[2][k] = 0.0f;
+    const size_t plane = s.count;
+    if (FMT == GSR_SCENE_F32_SOA) {
+        const int quads = (used * 3 + 3) / 4;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            if (q < quads) {
+                const float4 v = s.shPlanes[q * plane + i];
+                const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int j = q * 4 + e;
+                    if (j < used * 3) co[j % 3][j / 3] = vv[e];
+                }
+            }
+        }
+    } else {
+        const int quads = (used * 3 + 7) / 8;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            if (q < quads) {
+                const uint4 v = s.color[q * plane + i];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int j = q * 8 + e;
+                    const uint32_t h = (e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xFFFFu);
+                    if (j < used * 3) co[j % 3][j / 3] = halfBitsToFloat(h);
+                }
+            }
+        }
+    }
+}
+
 }  // namespace gsr

@@ -91,7 +91,8 @@ size_t sortStatusBytes(uint32_t maxCount);
 // Launch wrappers (each in its .hip file).
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, hipStream_t stream);
+                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, const uint8_t* useMlp,
+                      hipStream_t stream);
 // scanVisible = false: blockOffsets = exclusive scan of the blocks' tile sums (global mode);
 // true: of their visible (non-zero count) gaussians (binning mode's compaction).
 void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
@@ -139,6 +140,33 @@ void launchRender(const float4* splats, const uint32_t* packed, int renderSplat,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
                   hipStream_t stream);
+
+// Learned colour (mlp.hip): a 16-value latent per gaussian and one shared MLP. Parameters:
+// mlpParamCount() floats; packs: mlpPackedTiles() x 64 uint2 (the bf16 A-operand tiles, rebuilt
+// by launchMlpPack after every parameter change).
+int mlpParamCount();
+int mlpPackedTiles();
+void launchMlpPack(const float* params, uint2* packs, hipStream_t stream);
+// Samples 0 .. samples - 1 (samples % 16 == 0) of stream `seed`: adds the gradient of the mean
+// squared error into gradParams / gradLatents (backward) and the squared-error sum into *sse.
+void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, const float* latents,
+                    float* gradParams, float* gradLatents, float* sse, uint32_t seed, uint32_t samples, bool backward,
+                    hipStream_t stream);
+// One Adam step (step >= 1) on n values; clears the gradients.
+void launchAdam(float* p, float* grad, float* m, float* v, uint32_t n, float lr, uint32_t step, hipStream_t stream);
+void launchToBf16(const float* in, uint16_t* out, uint32_t n, hipStream_t stream);
+void launchInitLatents(float* out, uint32_t n, uint32_t seed, float scale, hipStream_t stream);
+void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count, uint32_t* gaussians, float* dirs,
+                      hipStream_t stream);
+void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* packs, const uint16_t* latentsBf16,
+                       float threshold, uint8_t* useMlp, float* err, hipStream_t stream);
+// After the preprocess (run with the same useMlp): the MLP colour of the visible gaussians with
+// useMlp set, into the splats (and the packed records in the p32 / p24 modes).
+void launchColourMlp(const SceneRef& scene, const DevCamera& cam, const float* params, const uint2* packs,
+                     const uint16_t* latentsBf16, const uint8_t* useMlp, float4* splats, uint32_t renderSplat,
+                     uint32_t* packed, hipStream_t stream);
+void launchMlpEvaluate(const float* params, const uint2* packs, const float* latents, const uint32_t* gaussians,
+                       const float* dirs, uint32_t n, float* rgb, hipStream_t stream);
 
 // Stand-alone scan (GpuScan): reduce, block-sum scan, downsweep.
 constexpr int kScanThreads = 256;

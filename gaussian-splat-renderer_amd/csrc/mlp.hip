@@ -1,0 +1,642 @@
+// Learned colour (no counterpart in the reference): a 16-dim latent per gaussian and one MLP shared
+// by all gaussians, [latent(16), view direction(3), 1] -> 64 -> ReLU -> 64 -> ReLU -> 3, stand in
+// for the degree-3 SH colour (48 floats per gaussian): colour = max(y + 0.5, 0), the transform the
+// SH evaluation ends with. The training targets are the scene's own SH colours (evalSH,
+// splat_math.cpp:129-151) for random (gaussian, direction) samples; gaussians whose learned
+// colour misses the SH colour by more than a threshold in any of 32 test directions keep SH.
+//
+// gfx950 design: every product is v_mfma_f32_16x16x16_bf16 (bf16 operands, fp32 accumulation) in
+// the "weights x activations" form, with the samples along N: a layer's output tile (lane l holds
+// rows 4 (l / 16) + j of column l % 16) is already the next layer's B operand, so the forward pass
+// moves no data between layers. The weights are packed once per update into the A-operand lane
+// layout (lane l holds W[l % 16][4 (l / 16) + j]: one 8-byte load per lane per tile). The
+// backward pass needs the activations and output gradients with the samples along K instead
+// (dW = dY X^T): those tiles are transposed through a 1 KB per-wave LDS buffer.
+#include <algorithm>
+
+#include "gsr_device.hpp"
+
+namespace gsr {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// fp32 parameter block: W1 [64][32] (input 19 = the constant 1: layer 1's bias), W2 [64][64],
+// b2 [64], W3 [3][64], b3 [3].
+constexpr int kW1 = 0;
+constexpr int kW2 = kW1 + 64 * 32;
+constexpr int kB2 = kW2 + 64 * 64;
+constexpr int kW3 = kB2 + 64;
+constexpr int kB3 = kW3 + 3 * 64;
+// Packed A tiles (64 lanes x 4 bf16 each): the forward's W1 (4 x 2 tiles), W2 (4 x 4), W3 (rows
+// padded to 16: 1 x 4), and the backward's W3^T (4 x 1), W2^T (4 x 4), W1^T rows 0..15 (1 x 4).
+constexpr int kTA1 = 0, kTA2 = 8, kTA3 = 24, kTA3T = 28, kTA2T = 32, kTA1T = 48;
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kTestDirs = 32;  // fallback test directions per gaussian
+
+__device__ __forceinline__ uint16_t bf16Bits(float f) {  // round to nearest even (finite values)
+    const uint32_t u = __float_as_uint(f);
+    return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ bf16x4 toBf16(f32x4 v) {
+    bf16x4 r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = short(bf16Bits(v[j]));
+    return r;
+}
+__device__ __forceinline__ bf16x4 tileA(const uint2* packs, int tile, uint32_t lane) {
+    const uint2 u = packs[tile * 64 + lane];
+    bf16x4 r;
+    r[0] = short(u.x & 0xFFFFu);
+    r[1] = short(u.x >> 16);
+    r[2] = short(u.y & 0xFFFFu);
+    r[3] = short(u.y >> 16);
+    return r;
+}
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 relu(f32x4 v) {
+    f32x4 r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r[j] = v[j] > 0.0f ? v[j] : 0.0f;
+    return r;
+}
+
+// Value j of lane `lane` in packed tile `tile` (the A-operand layout: W[l % 16][4 (l / 16) + j]).
+__device__ float packedValue(const float* P, int tile, uint32_t lane, int j) {
+    const int r = int(lane % 16u), k = int(4 * (lane / 16u)) + j;
+    if (tile < kTA2) {  // W1: mi = tile / 2, ki = tile % 2
+        return P[kW1 + (16 * (tile / 2) + r) * 32 + 16 * (tile % 2) + k];
+    } else if (tile < kTA3) {  // W2
+        const int t = tile - kTA2;
+        return P[kW2 + (16 * (t / 4) + r) * 64 + 16 * (t % 4) + k];
+    } else if (tile < kTA3T) {  // W3, rows 0..2 of 16
+        return r < 3 ? P[kW3 + r * 64 + 16 * (tile - kTA3) + k] : 0.0f;
+    } else if (tile < kTA2T) {  // W3^T: rows = hidden 16 mi + r, cols = output k
+        return k < 3 ? P[kW3 + k * 64 + 16 * (tile - kTA3T) + r] : 0.0f;
+    } else if (tile < kTA1T) {  // W2^T: rows = layer-1 unit 16 mi + r, cols = layer-2 unit 16 ki + k
+        const int t = tile - kTA2T;
+        return P[kW2 + (16 * (t % 4) + k) * 64 + 16 * (t / 4) + r];
+    } else {  // W1^T rows 0..15: rows = input feature r, cols = layer-1 unit 16 ki + k
+        return P[kW1 + (16 * (tile - kTA1T) + k) * 32 + r];
+    }
+}
+
+__global__ __launch_bounds__(64) void kPack(const float* __restrict__ params, uint2* __restrict__ packs) {
+    const int tile = int(blockIdx.x);
+    const uint32_t lane = threadIdx.x;
+    uint16_t h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = bf16Bits(packedValue(params, tile, lane, j));
+    packs[tile * 64 + lane] = make_uint2(uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16));
+}
+
+struct Forward {
+    f32x4 h1[4], h2[4], y;  // post-ReLU hidden tiles and the output tile (rows = units, cols = samples)
+};
+
+// x0: input features 0..15 (the latent), x1: features 16..31 ((dir, 1) in lanes 0..15, else 0),
+// both as D-layout tiles of the input matrix (rows = features, cols = the 16 samples).
+__device__ __forceinline__ void forward(const uint2* packs, const float* params, f32x4 x0, f32x4 x1, uint32_t lane,
+                                        Forward& f) {
+    const bf16x4 b0 = toBf16(x0), b1 = toBf16(x1);
+    bf16x4 b[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = mfma(tileA(packs, kTA1 + 2 * mi, lane), b0, acc);
+        acc = mfma(tileA(packs, kTA1 + 2 * mi + 1, lane), b1, acc);
+        f.h1[mi] = relu(acc);
+        b[mi] = toBf16(f.h1[mi]);
+    }
+    const int rq = int(4 * (lane / 16u));
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) {
+        f32x4 acc;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = params[kB2 + 16 * mi + rq + j];
+#pragma unroll
+        for (int ki = 0; ki < 4; ++ki) acc = mfma(tileA(packs, kTA2 + 4 * mi + ki, lane), b[ki], acc);
+        f.h2[mi] = relu(acc);
+    }
+    bf16x4 c[4];
+#pragma unroll
+    for (int ki = 0; ki < 4; ++ki) c[ki] = toBf16(f.h2[ki]);
+    f32x4 acc;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] = rq + j < 3 ? params[kB3 + rq + j] : 0.0f;
+#pragma unroll
+    for (int ki = 0; ki < 4; ++ki) acc = mfma(tileA(packs, kTA3 + ki, lane), c[ki], acc);
+    f.y = acc;
+}
+
+// A D-layout tile of a 16 x 16 matrix M (lane l: M[4 (l / 16) + j][l % 16]) re-read in the
+// A pattern (lane l: M[l % 16][4 (l / 16) + j]) through the wave's LDS buffer (16 x 17 floats).
+__device__ __forceinline__ bf16x4 transposeTile(f32x4 d, float* lds, uint32_t lane) {
+    const uint32_t rq = 4 * (lane / 16u), c = lane % 16u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) lds[(rq + j) * 17 + c] = d[j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    f32x4 a;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = lds[c * 17 + rq + j];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return toBf16(a);
+}
+
+__device__ __forceinline__ float4 bf16Quad(const uint16_t* p) {  // 4 bf16 values, 8-byte aligned
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xFFFF0000u), __uint_as_float(u.y << 16),
+                       __uint_as_float(u.y & 0xFFFF0000u));
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {  // a 32-bit integer hash (splitmix-style)
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ float unit01(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
+
+// Sample s of a stream: a gaussian and a direction uniform on the sphere.
+__device__ __forceinline__ void sampleOf(uint32_t seed, uint32_t s, uint32_t count, uint32_t* g, float dir[3]) {
+    const uint32_t h0 = mix32(s * 0x9E3779B9u ^ seed), h1 = mix32(h0 ^ 0x85EBCA6Bu), h2 = mix32(h1 ^ 0xC2B2AE35u);
+    *g = uint32_t((uint64_t(h0) * count) >> 32);
+    const float z = 2.0f * unit01(h1) - 1.0f, phi = 6.28318530718f * unit01(h2);
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    dir[0] = r * cosf(phi);
+    dir[1] = r * sinf(phi);
+    dir[2] = z;
+}
+
+// Test direction d of kTestDirs (a Fibonacci sphere).
+__device__ __forceinline__ void testDir(uint32_t d, float dir[3]) {
+    const float z = 1.0f - (2.0f * float(d) + 1.0f) / float(kTestDirs);
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z)), phi = 2.39996322973f * float(d);
+    dir[0] = r * cosf(phi);
+    dir[1] = r * sinf(phi);
+    dir[2] = z;
+}
+
+template <int FMT>
+__device__ __forceinline__ void shColour(const SceneRef& scene, uint32_t g, const float dir[3], float out[3]) {
+    const int used = (scene.degree + 1) * (scene.degree + 1);
+    float co[3][16];
+    loadSh<FMT>(scene, g, used, co);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = evalShChannel(scene.degree, co[c], dir[0], dir[1], dir[2]);  // clamped
+}
+// The unclamped SH value (y + 0.5 is trained towards it; both are clamped at 0 when used).
+template <int FMT>
+__device__ __forceinline__ void shTarget(const SceneRef& scene, uint32_t g, const float dir[3], float out[3]) {
+    const int used = (scene.degree + 1) * (scene.degree + 1);
+    float co[3][16];
+    loadSh<FMT>(scene, g, used, co);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = evalShRaw(scene.degree, co[c], dir[0], dir[1], dir[2]);
+}
+
+// The inputs of one 16-sample tile as D-layout tiles: x0 = latent features 4 (l / 16) .. + 3 of
+// sample l % 16 (latQuad), x1 = (dir, 1) in lanes 0..15.
+__device__ __forceinline__ void inputTiles(float4 latQuad, const float dir[3], uint32_t lane, f32x4& x0, f32x4& x1) {
+    x0 = f32x4{latQuad.x, latQuad.y, latQuad.z, latQuad.w};
+    const bool first = lane < 16u;
+    x1[0] = first ? dir[0] : 0.0f;
+    x1[1] = first ? dir[1] : 0.0f;
+    x1[2] = first ? dir[2] : 0.0f;
+    x1[3] = first ? 1.0f : 0.0f;
+}
+
+// Training pass over samples 0 .. total - 1 of stream `seed`, `perWave` consecutive ones (a
+// multiple of 16) per wave: accumulates the gradient of mean((y + 0.5 - sh)^2) over the total x 3
+// values into gradParams / gradLatents and the summed squared error into *sse. BACKWARD = false:
+// only the error.
+template <int FMT, bool BACKWARD>
+__global__ __launch_bounds__(kThreads) void kTrain(SceneRef scene, const float* __restrict__ params,
+                                                   const uint2* __restrict__ packs, const float* __restrict__ latents,
+                                                   float* __restrict__ gradParams, float* __restrict__ gradLatents,
+                                                   float* __restrict__ sse, uint32_t seed, uint32_t perWave,
+                                                   uint32_t total) {
+    __shared__ float sT[kWaves][16 * 17];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    float* lds = sT[wave];
+    const uint32_t rq = 4 * (lane / 16u), col = lane % 16u;
+    const float scale = 2.0f / (3.0f * float(total));
+    f32x4 dW1[4][2], dW2[4][4], dW3[4];
+    float db2[4][4], db3[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        dW3[a] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int b = 0; b < 2; ++b) dW1[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) dW2[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) db2[a][j] = 0.0f;
+        db3[a] = 0.0f;
+    }
+    float err = 0.0f;
+    const uint32_t base = (blockIdx.x * kWaves + wave) * perWave;
+    for (uint32_t t0 = 0; t0 < perWave && base + t0 < total; t0 += 16) {  // whole tiles: total % 16 == 0
+        uint32_t g;
+        float dir[3];
+        sampleOf(seed, base + t0 + col, scene.count, &g, dir);
+        f32x4 x0, x1;
+        inputTiles(*reinterpret_cast<const float4*>(latents + size_t(g) * 16 + rq), dir, lane, x0, x1);
+        Forward f;
+        forward(packs, params, x0, x1, lane, f);
+        f32x4 dy = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (lane < 16u) {
+            float tgt[3];
+            shTarget<FMT>(scene, g, dir, tgt);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float e = f.y[c] + 0.5f - tgt[c];
+                err += e * e;
+                dy[c] = scale * e;
+            }
+        }
+        if (!BACKWARD) continue;
+        // dW3 += dY H2^T; dH2 = W3^T dY (masked by ReLU).
+        const bf16x4 aDy = transposeTile(dy, lds, lane);
+        bf16x4 bH2[4], bH1[4];
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+            bH2[ni] = transposeTile(f.h2[ni], lds, lane);
+            dW3[ni] = mfma(aDy, bH2[ni], dW3[ni]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) db3[j] += dy[j];
+        const bf16x4 bDy = toBf16(dy);
+        f32x4 dH2[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) {
+            f32x4 d = mfma(tileA(packs, kTA3T + mi, lane), bDy, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                d[j] = f.h2[mi][j] > 0.0f ? d[j] : 0.0f;
+                db2[mi][j] += d[j];
+            }
+            dH2[mi] = d;
+        }
+        // dW2 += dH2 H1^T; dH1 = W2^T dH2 (masked).
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) bH1[ni] = transposeTile(f.h1[ni], lds, lane);
+        bf16x4 bDH2[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) {
+            const bf16x4 a = transposeTile(dH2[mi], lds, lane);
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni) dW2[mi][ni] = mfma(a, bH1[ni], dW2[mi][ni]);
+            bDH2[mi] = toBf16(dH2[mi]);
+        }
+        f32x4 dH1[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) {
+            f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ki = 0; ki < 4; ++ki) d = mfma(tileA(packs, kTA2T + 4 * mi + ki, lane), bDH2[ki], d);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) d[j] = f.h1[mi][j] > 0.0f ? d[j] : 0.0f;
+            dH1[mi] = d;
+        }
+        // dW1 += dH1 X^T; dLatent = W1^T[0..15] dH1.
+        const bf16x4 bX0 = transposeTile(x0, lds, lane), bX1 = transposeTile(x1, lds, lane);
+        bf16x4 bDH1[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) {
+            const bf16x4 a = transposeTile(dH1[mi], lds, lane);
+            dW1[mi][0] = mfma(a, bX0, dW1[mi][0]);
+            dW1[mi][1] = mfma(a, bX1, dW1[mi][1]);
+            bDH1[mi] = toBf16(dH1[mi]);
+        }
+        f32x4 dl = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ki = 0; ki < 4; ++ki) dl = mfma(tileA(packs, kTA1T + ki, lane), bDH1[ki], dl);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) atomicAdd(&gradLatents[size_t(g) * 16 + rq + j], dl[j]);
+    }
+    // The wave's sums: one atomic per parameter entry it holds.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) err += __shfl_xor(err, off, 64);
+    if (lane == 0) atomicAdd(sse, err);
+    if (!BACKWARD) return;
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = 16 * mi + int(rq) + j;
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) atomicAdd(&gradParams[kW1 + row * 32 + 16 * ni + int(col)], dW1[mi][ni][j]);
+#pragma unroll
+            for (int ni = 0; ni < 4; ++ni) atomicAdd(&gradParams[kW2 + row * 64 + 16 * ni + int(col)], dW2[mi][ni][j]);
+        }
+#pragma unroll
+    for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (int(rq) + j < 3) atomicAdd(&gradParams[kW3 + (int(rq) + j) * 64 + 16 * ni + int(col)], dW3[ni][j]);
+    // Bias sums over the 16 sample columns (lanes with the same l / 16).
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float v = db2[mi][j];
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (col == 0u) atomicAdd(&gradParams[kB2 + 16 * mi + int(rq) + j], v);
+        }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        float v = db3[j];
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0u) atomicAdd(&gradParams[kB3 + j], v);
+    }
+}
+
+// Adam on n values; the gradients are cleared for the next step.
+__global__ __launch_bounds__(256) void kAdam(float* __restrict__ p, float* __restrict__ grad, float* __restrict__ m,
+                                             float* __restrict__ v, uint32_t n, float lr, float bc1, float bc2) {
+    constexpr float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float gi = grad[i];
+        grad[i] = 0.0f;
+        const float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
+}
+
+// The latents the renderer reads: bf16 (rounded as the MFMA operands are), 32 bytes per gaussian.
+__global__ __launch_bounds__(256) void kToBf16(const float* __restrict__ in, uint16_t* __restrict__ out, uint32_t n) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) out[i] = bf16Bits(in[i]);
+}
+
+// Latent initialisation: uniform in [-scale, scale) from a hash of (seed, index).
+__global__ __launch_bounds__(256) void kInitLatents(float* __restrict__ out, uint32_t n, uint32_t seed, float scale) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        out[i] = scale * (2.0f * unit01(mix32(i * 0x9E3779B9u ^ mix32(seed))) - 1.0f);
+}
+
+// The (gaussian, direction) pairs of samples first .. first + n - 1 of stream `seed`.
+__global__ __launch_bounds__(256) void kSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count,
+                                                uint32_t* __restrict__ gaussians, float* __restrict__ dirs) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t g;
+    float dir[3];
+    sampleOf(seed, first + i, count, &g, dir);
+    gaussians[i] = g;
+    dirs[3 * size_t(i)] = dir[0];
+    dirs[3 * size_t(i) + 1] = dir[1];
+    dirs[3 * size_t(i) + 2] = dir[2];
+}
+
+// Fallback: per gaussian, the largest channel difference between max(y + 0.5, 0) (with the bf16
+// latent the renderer uses) and the clamped SH colour over the kTestDirs test directions;
+// useMlp[g] = 1 when it is at most `threshold`, and err[g] holds the difference. A wave takes
+// two gaussians per round (2 x 32 directions = 4 tiles).
+template <int FMT>
+__global__ __launch_bounds__(kThreads) void kFallback(SceneRef scene, const float* __restrict__ params,
+                                                      const uint2* __restrict__ packs,
+                                                      const uint16_t* __restrict__ latentsBf16, float threshold,
+                                                      uint8_t* __restrict__ useMlp, float* __restrict__ errOut) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t col = lane % 16u, rq = 4 * (lane / 16u);
+    const uint32_t waves = gridDim.x * kWaves;
+    for (uint32_t g0 = (blockIdx.x * kWaves + wave) * 2; g0 < scene.count; g0 += waves * 2) {
+        float worst[2] = {0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t g = g0 + uint32_t(t / 2);
+            const uint32_t gs = g < scene.count ? g : scene.count - 1;
+            float dir[3];
+            testDir(uint32_t(t % 2) * 16 + col, dir);
+            f32x4 x0, x1;
+            inputTiles(bf16Quad(latentsBf16 + size_t(gs) * 16 + rq), dir, lane, x0, x1);
+            Forward f;
+            forward(packs, params, x0, x1, lane, f);
+            float e = 0.0f;
+            if (lane < 16u) {
+                float sh[3];
+                shColour<FMT>(scene, gs, dir, sh);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float y = fmaxf(f.y[c] + 0.5f, 0.0f);
+                    e = fmaxf(e, fabsf(y - sh[c]));
+                }
+            }
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) e = fmaxf(e, __shfl_xor(e, off, 64));
+            worst[t / 2] = fmaxf(worst[t / 2], e);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const uint32_t g = g0 + uint32_t(k);
+                if (g < scene.count) {
+                    useMlp[g] = worst[k] <= threshold ? 1u : 0u;
+                    errOut[g] = worst[k];
+                }
+            }
+        }
+    }
+}
+
+// The render's colour for the visible gaussians that use the MLP (after the preprocess, which left
+// their colour at 0): dir = normalize(mean - camPos) as for SH; writes the splat's o2.xyz and, in
+// the packed render-splat modes, the fp16 colour of the packed record. A wave takes 64 gaussians
+// (4 tiles) per round and skips a tile none of whose gaussians needs the MLP.
+template <int FMT>
+__global__ __launch_bounds__(kThreads) void kColourMlp(SceneRef scene, DevCamera cam, const float* __restrict__ params,
+                                                       const uint2* __restrict__ packs,
+                                                       const uint16_t* __restrict__ latentsBf16,
+                                                       const uint8_t* __restrict__ useMlp, float4* __restrict__ splats,
+                                                       uint32_t renderSplat, uint32_t* __restrict__ packed) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t col = lane % 16u, rq = 4 * (lane / 16u);
+    const uint32_t waves = gridDim.x * kWaves;
+    for (uint32_t g0 = (blockIdx.x * kWaves + wave) * 64; g0 < scene.count; g0 += waves * 64) {
+#pragma unroll 1
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t g = g0 + uint32_t(t) * 16 + col;
+            const bool need = g < scene.count && useMlp[g] &&
+                              __float_as_uint(splats[3 * size_t(g)].w) == GSR_CULL_VISIBLE;
+            if (!__any(need)) continue;  // wave-uniform
+            const uint32_t gs = g < scene.count ? g : scene.count - 1;
+            float px, py, pz;
+            if (FMT == GSR_SCENE_F32_SOA) {
+                const float4 pa = scene.posAlpha[gs];
+                px = pa.x;
+                py = pa.y;
+                pz = pa.z;
+            } else {
+                const uint4 g0v = scene.geometry[gs];
+                px = __uint_as_float(g0v.x);
+                py = __uint_as_float(g0v.y);
+                pz = __uint_as_float(g0v.z);
+            }
+            const float dx = px - cam.pos[0], dy = py - cam.pos[1], dz = pz - cam.pos[2];
+            const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+            const float dir[3] = {dx * inv, dy * inv, dz * inv};
+            f32x4 x0, x1;
+            inputTiles(bf16Quad(latentsBf16 + size_t(gs) * 16 + rq), dir, lane, x0, x1);
+            Forward f;
+            forward(packs, params, x0, x1, lane, f);  // every lane: the MFMAs need the whole wave
+            if (lane < 16u && need) {
+                const float r = fmaxf(f.y[0] + 0.5f, 0.0f), gr = fmaxf(f.y[1] + 0.5f, 0.0f),
+                            b = fmaxf(f.y[2] + 0.5f, 0.0f);
+                *reinterpret_cast<float3*>(splats + 3 * size_t(g) + 2) = make_float3(r, gr, b);
+                const uint32_t rg = halfBits(r) | (halfBits(gr) << 16);
+                if (renderSplat == GSR_RENDER_SPLAT_P32) {
+                    uint32_t* w = packed + 8 * size_t(g);
+                    w[6] = rg;
+                    w[7] = (w[7] & 0xFFFF0000u) | halfBits(b);  // the radius stays in the high half
+                } else if (renderSplat == GSR_RENDER_SPLAT_P24) {
+                    uint32_t* w = packed + 6 * size_t(g);
+                    w[4] = rg;
+                    w[5] = (w[5] & 0xFFFF0000u) | halfBits(b);
+                }
+            }
+        }
+    }
+}
+
+// Evaluation at given (gaussian, direction) pairs: y + 0.5 unclamped (tests).
+__global__ __launch_bounds__(kThreads) void kEvaluate(const float* __restrict__ params, const uint2* __restrict__ packs,
+                                                      const float* __restrict__ latents,
+                                                      const uint32_t* __restrict__ gaussians,
+                                                      const float* __restrict__ dirs, uint32_t n,
+                                                      float* __restrict__ rgb) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t col = lane % 16u;
+    const uint32_t t0 = (blockIdx.x * kWaves + wave) * 16;
+    if (t0 >= n) return;  // whole waves
+    const uint32_t s = t0 + col < n ? t0 + col : n - 1;
+    const float dir[3] = {dirs[3 * s], dirs[3 * s + 1], dirs[3 * s + 2]};
+    f32x4 x0, x1;
+    inputTiles(*reinterpret_cast<const float4*>(latents + size_t(gaussians[s]) * 16 + 4 * (lane / 16u)), dir, lane,
+               x0, x1);
+    Forward f;
+    forward(packs, params, x0, x1, lane, f);
+    if (lane < 16u && t0 + col < n) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rgb[3 * size_t(s) + c] = f.y[c] + 0.5f;
+    }
+}
+
+uint32_t gridFor(uint32_t n, uint32_t perBlock, uint32_t cap) {
+    const uint32_t b = (n + perBlock - 1) / perBlock;
+    return b > cap ? cap : b;
+}
+
+}  // namespace
+
+int mlpParamCount() { return kB3 + 3; }
+int mlpPackedTiles() { return kTA1T + 4; }
+
+void launchMlpPack(const float* params, uint2* packs, hipStream_t stream) {
+    hipLaunchKernelGGL(kPack, dim3(mlpPackedTiles()), dim3(64), 0, stream, params, packs);
+}
+
+void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, const float* latents,
+                    float* gradParams, float* gradLatents, float* sse, uint32_t seed, uint32_t samples, bool backward,
+                    hipStream_t stream) {
+    // Each wave takes perWave consecutive samples (a multiple of 16, at least 64); at most 1024
+    // waves, so that the per-wave weight-gradient atomics (6.4 K per wave) stay few. samples is a
+    // multiple of 16.
+    const uint32_t tiles = samples / 16;
+    if (tiles == 0 || scene.count == 0) return;
+    const uint32_t tilesPerWave = std::max(4u, (tiles + 1023) / 1024);
+    const uint32_t waves = (tiles + tilesPerWave - 1) / tilesPerWave;
+    const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+    const uint32_t perWave = 16 * tilesPerWave;
+    if (scene.format == GSR_SCENE_F32_SOA) {
+        if (backward)
+            hipLaunchKernelGGL((kTrain<GSR_SCENE_F32_SOA, true>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
+                               packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
+        else
+            hipLaunchKernelGGL((kTrain<GSR_SCENE_F32_SOA, false>), dim3(blocks), dim3(kThreads), 0, stream, scene,
+                               params, packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
+    } else {
+        if (backward)
+            hipLaunchKernelGGL((kTrain<GSR_SCENE_F16, true>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
+                               packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
+        else
+            hipLaunchKernelGGL((kTrain<GSR_SCENE_F16, false>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
+                               packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
+    }
+}
+
+void launchAdam(float* p, float* grad, float* m, float* v, uint32_t n, float lr, uint32_t step, hipStream_t stream) {
+    const float bc1 = 1.0f - powf(0.9f, float(step)), bc2 = 1.0f - powf(0.999f, float(step));
+    const uint32_t blocks = gridFor(n, 256, 4096);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kAdam, dim3(blocks), dim3(256), 0, stream, p, grad, m, v, n, lr, bc1, bc2);
+}
+
+void launchToBf16(const float* in, uint16_t* out, uint32_t n, hipStream_t stream) {
+    const uint32_t blocks = gridFor(n, 256, 4096);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kToBf16, dim3(blocks), dim3(256), 0, stream, in, out, n);
+}
+
+void launchInitLatents(float* out, uint32_t n, uint32_t seed, float scale, hipStream_t stream) {
+    const uint32_t blocks = gridFor(n, 256, 4096);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kInitLatents, dim3(blocks), dim3(256), 0, stream, out, n, seed, scale);
+}
+
+void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count, uint32_t* gaussians, float* dirs,
+                      hipStream_t stream) {
+    const uint32_t blocks = (n + 255) / 256;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kSamples, dim3(blocks), dim3(256), 0, stream, seed, first, n, count, gaussians, dirs);
+}
+
+void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* packs, const uint16_t* latentsBf16,
+                       float threshold, uint8_t* useMlp, float* err, hipStream_t stream) {
+    const uint32_t blocks = gridFor(scene.count, 2 * kWaves, 8192);
+    if (blocks == 0) return;
+    if (scene.format == GSR_SCENE_F32_SOA)
+        hipLaunchKernelGGL(kFallback<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kThreads), 0, stream, scene, params, packs,
+                           latentsBf16, threshold, useMlp, err);
+    else
+        hipLaunchKernelGGL(kFallback<GSR_SCENE_F16>, dim3(blocks), dim3(kThreads), 0, stream, scene, params, packs,
+                           latentsBf16, threshold, useMlp, err);
+}
+
+void launchColourMlp(const SceneRef& scene, const DevCamera& cam, const float* params, const uint2* packs,
+                     const uint16_t* latentsBf16, const uint8_t* useMlp, float4* splats, uint32_t renderSplat,
+                     uint32_t* packed, hipStream_t stream) {
+    const uint32_t blocks = gridFor(scene.count, 64 * kWaves, 8192);
+    if (blocks == 0) return;
+    if (scene.format == GSR_SCENE_F32_SOA)
+        hipLaunchKernelGGL(kColourMlp<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kThreads), 0, stream, scene, cam, params,
+                           packs, latentsBf16, useMlp, splats, renderSplat, packed);
+    else
+        hipLaunchKernelGGL(kColourMlp<GSR_SCENE_F16>, dim3(blocks), dim3(kThreads), 0, stream, scene, cam, params,
+                           packs, latentsBf16, useMlp, splats, renderSplat, packed);
+}
+
+void launchMlpEvaluate(const float* params, const uint2* packs, const float* latents, const uint32_t* gaussians,
+                       const float* dirs, uint32_t n, float* rgb, hipStream_t stream) {
+    const uint32_t blocks = (n + 16 * kWaves - 1) / (16 * kWaves);
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kEvaluate, dim3(blocks), dim3(kThreads), 0, stream, params, packs, latents, gaussians, dirs, n,
+                       rgb);
+}
+
+}  // namespace gsr

@@ -15,75 +15,6 @@ namespace gsr {
 
 namespace {
 
-constexpr float kC0 = 0.28209479177387814f;
-constexpr float kC1 = 0.4886025119029199f;
-constexpr float kC2_0 = 1.0925484305920792f, kC2_1 = -1.0925484305920792f, kC2_2 = 0.31539156525252005f,
-                kC2_3 = -1.0925484305920792f, kC2_4 = 0.5462742152960396f;
-constexpr float kC3_0 = -0.5900435899266435f, kC3_1 = 2.890611442640554f, kC3_2 = -0.4570457994644658f,
-                kC3_3 = 0.3731763325901154f, kC3_4 = -0.4570457994644658f, kC3_5 = 1.445305721320277f,
-                kC3_6 = -0.5900435899266435f;
-
-// evalSH (splat_math.cpp:129-151), one channel; c = the 16 coefficients of that channel.
-__device__ __forceinline__ float evalShChannel(int degree, const float* c, float x, float y, float z) {
-    float result = kC0 * c[0];
-    if (degree > 0) {
-        result = result - kC1 * y * c[1] + kC1 * z * c[2] - kC1 * x * c[3];
-        if (degree > 1) {
-            const float xx = x * x, yy = y * y, zz = z * z;
-            const float xy = x * y, yz = y * z, xz = x * z;
-            result = result + kC2_0 * xy * c[4] + kC2_1 * yz * c[5] + kC2_2 * (2.0f * zz - xx - yy) * c[6] +
-                     kC2_3 * xz * c[7] + kC2_4 * (xx - yy) * c[8];
-            if (degree > 2) {
-                result = result + kC3_0 * y * (3.0f * xx - yy) * c[9] + kC3_1 * xy * z * c[10] +
-                         kC3_2 * y * (4.0f * zz - xx - yy) * c[11] +
-                         kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * c[12] +
-                         kC3_4 * x * (4.0f * zz - xx - yy) * c[13] + kC3_5 * z * (xx - yy) * c[14] +
-                         kC3_6 * x * (xx - 3.0f * yy) * c[15];
-            }
-        }
-    }
-    result = result + 0.5f;
-    return result < 0.0f ? 0.0f : result;
-}
-
-// The coefficients of the evaluated bands, channel-major: co[ch][k].
-template <int FMT>
-__device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, float co[3][16]) {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) co[0][k] = co[1][k] = co[2][k] = 0.0f;
-    const size_t plane = s.count;
-    if (FMT == GSR_SCENE_F32_SOA) {
-        const int quads = (used * 3 + 3) / 4;
-#pragma unroll
-        for (int q = 0; q < 12; ++q) {
-            if (q < quads) {
-                const float4 v = s.shPlanes[q * plane + i];
-                const float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int j = q * 4 + e;
-                    if (j < used * 3) co[j % 3][j / 3] = vv[e];
-                }
-            }
-        }
-    } else {
-        const int quads = (used * 3 + 7) / 8;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            if (q < quads) {
-                const uint4 v = s.color[q * plane + i];
-                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int j = q * 8 + e;
-                    const uint32_t h = (e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xFFFFu);
-                    if (j < used * 3) co[j % 3][j / 3] = halfBitsToFloat(h);
-                }
-            }
-        }
-    }
-}
-
 template <int FMT>
 __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene, DevCamera cam, uint32_t bounds,
                                                                   float4* __restrict__ splats,
@@ -92,7 +23,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                                   uint32_t* __restrict__ rangesWords,
                                                                   uint32_t rangeWordCount, uint32_t renderSplat,
                                                                   uint32_t* __restrict__ packed,
-                                                                  float4* __restrict__ rayDepth) {
+                                                                  float4* __restrict__ rayDepth,
+                                                                  const uint8_t* __restrict__ useMlp) {
     const uint32_t i = blockIdx.x * kPreprocessThreads + threadIdx.x;
     if (i < rangeWordCount) rangesWords[i] = 0u;
 
@@ -225,17 +157,21 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                 if (rect.x1 <= rect.x0 || rect.y1 <= rect.y0) {
                     reason = GSR_CULL_OFF_SCREEN;
                 } else {
-                    // SH colour, dir = normalize(mean - camPos) in data space.
-                    const int degree = scene.degree;
-                    const int used = (degree + 1) * (degree + 1);
-                    float co[3][16];
-                    loadSh<FMT>(scene, i, used, co);
-                    const float dx = px - cam.pos[0], dy = py - cam.pos[1], dz = pz - cam.pos[2];
-                    const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-                    const float x = dx * inv, y = dy * inv, z = dz * inv;
-                    o2.x = evalShChannel(degree, co[0], x, y, z);
-                    o2.y = evalShChannel(degree, co[1], x, y, z);
-                    o2.z = evalShChannel(degree, co[2], x, y, z);
+                    if (useMlp && useMlp[i]) {
+                        o2.x = o2.y = o2.z = 0.0f;  // the learned colour pass writes it
+                    } else {
+                        // SH colour, dir = normalize(mean - camPos) in data space.
+                        const int degree = scene.degree;
+                        const int used = (degree + 1) * (degree + 1);
+                        float co[3][16];
+                        loadSh<FMT>(scene, i, used, co);
+                        const float dx = px - cam.pos[0], dy = py - cam.pos[1], dz = pz - cam.pos[2];
+                        const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+                        const float x = dx * inv, y = dy * inv, z = dz * inv;
+                        o2.x = evalShChannel(degree, co[0], x, y, z);
+                        o2.y = evalShChannel(degree, co[1], x, y, z);
+                        o2.z = evalShChannel(degree, co[2], x, y, z);
+                    }
                     o2.w = radius;
                     // p24 keeps the conic and opacity in fp16: the tiles are counted, duplicated
                     // and rendered with the values the record gives back.
@@ -349,18 +285,19 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
 
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, hipStream_t stream) {
+                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, const uint8_t* useMlp,
+                      hipStream_t stream) {
     uint32_t n = scene.count > rangeWordCount ? scene.count : rangeWordCount;
     const uint32_t blocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene,
                            cam, bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat,
-                           packed, rayDepth);
+                           packed, rayDepth, useMlp);
     else
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
                            bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat, packed,
-                           rayDepth);
+                           rayDepth, useMlp);
 }
 
 }  // namespace gsr

@@ -231,6 +231,17 @@ _batch_branches = _sig("gsr_batch_branches", _u32, [_vp])
 _batch_render = _sig("gsr_batch_render", _i32, [_vp, _vp, _u32, _vp, _vp, _vp])
 _batch_capture = _sig("gsr_batch_capture", _i32, [_vp, _vp, _u32, _vp, _vp])
 _batch_replay = _sig("gsr_batch_replay", _i32, [_vp, _vp])
+_cm_param_count = _sig("gsr_colour_model_param_count", _u32, [])
+_cm_create = _sig("gsr_colour_model_create", _i32, [_vp, _vp, _u32, ctypes.POINTER(_vp)])
+_cm_destroy = _sig("gsr_colour_model_destroy", None, [_vp])
+_cm_train = _sig("gsr_colour_model_train", _i32, [_vp, _u32, _u32, _f32, _f32, _vp])
+_cm_loss = _sig("gsr_colour_model_loss", _i32, [_vp, _u32, _u32, _vp])
+_cm_gradients = _sig("gsr_colour_model_gradients", _i32, [_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp])
+_cm_get = _sig("gsr_colour_model_get", _i32, [_vp, _vp, _vp])
+_cm_set = _sig("gsr_colour_model_set", _i32, [_vp, _vp, _vp])
+_cm_evaluate = _sig("gsr_colour_model_evaluate", _i32, [_vp, _vp, _vp, _u32, _vp])
+_cm_fallback = _sig("gsr_colour_model_fallback", _i32, [_vp, _f32, _vp, _vp])
+_renderer_set_colour_model = _sig("gsr_renderer_set_colour_model", _i32, [_vp, _vp])
 
 # Every symbol include/gsr.h declares (checked by the CPU tests).
 EXPORTED = (
@@ -242,6 +253,9 @@ EXPORTED = (
     "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_sort_pairs "
     "gsr_exclusive_scan "
     "gsr_batch_create gsr_batch_destroy gsr_batch_branches gsr_batch_render gsr_batch_capture gsr_batch_replay "
+    "gsr_colour_model_param_count gsr_colour_model_create gsr_colour_model_destroy gsr_colour_model_train "
+    "gsr_colour_model_loss gsr_colour_model_gradients gsr_colour_model_get gsr_colour_model_set "
+    "gsr_colour_model_evaluate gsr_colour_model_fallback gsr_renderer_set_colour_model "
     "gsr_ply_load gsr_ply_load_memory gsr_ply_free"
 ).split()
 
@@ -505,6 +519,7 @@ class Renderer:
         scene._renderers.add(self)
         self.sort_mode = sort_mode
         self.render_splat = render_splat
+        self._colour: Optional["ColourModel"] = None
         self._image: Optional[DeviceBuffer] = None
         self._cam: Optional[Camera] = None
 
@@ -534,6 +549,14 @@ class Renderer:
 
     def image_bytes(self, cam: Camera) -> int:
         return int(cam.width) * int(cam.height) * (4 if self.target == "rgba8" else 16)
+
+    def set_colour_model(self, model: Optional["ColourModel"]) -> None:
+        """Learned colour for the gaussians the model's last fallback pass marked (None: SH only)."""
+        _check(_renderer_set_colour_model(self.handle, None if model is None else model.handle),
+               "gsr_renderer_set_colour_model")
+        self._colour = model
+        if model is not None:
+            model._bound.add(self)
 
     def render(self, cam: Camera, out_ptr: Optional[int] = None, stream: Optional[int] = None) -> int:
         """Record one frame (asynchronous). Returns the device pointer written."""
@@ -650,6 +673,95 @@ class Batch:
             _batch_destroy(self.handle)
             self.handle = None
         self._captured = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ColourModel:
+    """Learned colour (gsr_colour_model_*): a 16-value latent per gaussian and one shared MLP
+    ([latent, direction, 1] -> 64 -> ReLU -> 64 -> ReLU -> 3, bf16 MFMA), trained towards the
+    scene's SH colours; fallback() marks the gaussians it reproduces within a threshold, and
+    renderers bound with Renderer.set_colour_model colour those from the MLP."""
+
+    LATENT = 16
+
+    def __init__(self, ctx: Context, scene: Scene, seed: int = 0):
+        h = _vp()
+        _check(_cm_create(ctx.handle, scene.handle, int(seed) & 0xFFFFFFFF, ctypes.byref(h)), "gsr_colour_model_create")
+        self.handle, self.ctx, self.scene, self.count = h, ctx, scene, scene.count
+        scene._renderers.add(self)  # the scene must outlive the model
+        self._bound = weakref.WeakSet()
+
+    @staticmethod
+    def param_count() -> int:
+        return int(_cm_param_count())
+
+    def train(self, steps: int, samples_per_step: int = 1 << 20, lr_weights: float = 1e-3,
+              lr_latents: float = 1e-2) -> float:
+        """Adam steps; returns the last step's mean squared error (per channel)."""
+        loss = ctypes.c_float()
+        _check(_cm_train(self.handle, int(steps), int(samples_per_step), float(lr_weights), float(lr_latents),
+                         ctypes.byref(loss)), "gsr_colour_model_train")
+        return float(loss.value)
+
+    def loss(self, seed: int, samples: int) -> float:
+        out = ctypes.c_float()
+        _check(_cm_loss(self.handle, int(seed) & 0xFFFFFFFF, int(samples), ctypes.byref(out)), "gsr_colour_model_loss")
+        return float(out.value)
+
+    def gradients(self, seed: int, samples: int):
+        """(grad_params, grad_latents (N, 16), loss, gaussians, dirs (samples, 3)) of one batch."""
+        gp = np.zeros(self.param_count(), np.float32)
+        gl = np.zeros((self.count, self.LATENT), np.float32)
+        g = np.zeros(samples, np.uint32)
+        d = np.zeros((samples, 3), np.float32)
+        loss = ctypes.c_float()
+        _check(_cm_gradients(self.handle, int(seed) & 0xFFFFFFFF, int(samples), _ptr(gp), _ptr(gl),
+                             ctypes.byref(loss), _ptr(g), _ptr(d)), "gsr_colour_model_gradients")
+        return gp, gl, float(loss.value), g, d
+
+    def get(self):
+        p = np.zeros(self.param_count(), np.float32)
+        lat = np.zeros((self.count, self.LATENT), np.float32)
+        _check(_cm_get(self.handle, _ptr(p), _ptr(lat)), "gsr_colour_model_get")
+        return p, lat
+
+    def set(self, params=None, latents=None) -> None:
+        p = None if params is None else _f32c(params).reshape(-1)
+        lat = None if latents is None else _f32c(latents).reshape(-1)
+        if p is not None and p.size != self.param_count():
+            raise ValueError(f"params must hold {self.param_count()} floats")
+        if lat is not None and lat.size != self.count * self.LATENT:
+            raise ValueError(f"latents must hold {self.count * self.LATENT} floats")
+        _check(_cm_set(self.handle, _ptr(p), _ptr(lat)), "gsr_colour_model_set")
+
+    def evaluate(self, gaussians, dirs) -> np.ndarray:
+        """y + 0.5 (unclamped) at (gaussian, direction) pairs: (n, 3)."""
+        g = np.ascontiguousarray(gaussians, np.uint32).reshape(-1)
+        d = _f32c(dirs).reshape(-1, 3)
+        if d.shape[0] != g.size:
+            raise ValueError("one direction per gaussian index")
+        out = np.zeros((g.size, 3), np.float32)
+        _check(_cm_evaluate(self.handle, _ptr(g), _ptr(d), g.size, _ptr(out)), "gsr_colour_model_evaluate")
+        return out
+
+    def fallback(self, threshold: float):
+        """Marks the gaussians within `threshold` of their SH colour; returns (count, errors)."""
+        n = ctypes.c_uint32()
+        err = np.zeros(self.count, np.float32)
+        _check(_cm_fallback(self.handle, float(threshold), ctypes.byref(n), _ptr(err)), "gsr_colour_model_fallback")
+        return int(n.value), err
+
+    def close(self) -> None:
+        if self.handle:
+            if any(r.handle for r in list(self._bound) if getattr(r, "_colour", None) is self):
+                raise GsrError("ColourModel.close: a renderer bound to this model is still open (unbind or close it)")
+            _cm_destroy(self.handle)
+            self.handle = None
 
     def __del__(self):
         try:

@@ -303,6 +303,45 @@ gsr_status gsr_batch_capture(gsr_batch* batch, gsr_renderer* const* renderers, u
                              const gsr_camera* cams, void* const* out_images);
 gsr_status gsr_batch_replay(gsr_batch* batch, void* stream);
 
+/* --- Learned colour ----------------------------------------------------------------------------
+ * No counterpart in the reference (its colour is always the SH evaluation, evalSH,
+ * splat_math.cpp:129-151). A colour model gives every gaussian of a scene a 16-value latent and
+ * shares one MLP among them: [latent, view direction, 1] -> 64 -> ReLU -> 64 -> ReLU -> 3,
+ * colour = max(y + 0.5, 0), computed with bf16 MFMA products and fp32 accumulation. It is
+ * trained (Adam) towards the scene's own SH colours on random (gaussian, direction) samples; a
+ * fallback pass then marks the gaussians whose learned colour is within `threshold` of the SH
+ * colour in all 32 test directions, and a renderer bound to the model colours those from the
+ * MLP and the rest from SH. Parameters (gsr_colour_model_param_count floats): W1 [64][32]
+ * (columns 0..15 the latent, 16..18 the direction, 19 the bias, 20..31 zero), W2 [64][64],
+ * b2 [64], W3 [3][64], b3 [3]. Latents: count x 16 floats (rendered through fp16 copies).
+ * Host pointers throughout; calls are synchronous unless they say otherwise. The model must
+ * outlive every renderer bound to it. */
+typedef struct gsr_colour_model gsr_colour_model;
+uint32_t gsr_colour_model_param_count(void);
+gsr_status gsr_colour_model_create(gsr_context* ctx, const gsr_scene* scene, uint32_t seed, gsr_colour_model** out);
+void gsr_colour_model_destroy(gsr_colour_model* model);
+/* `steps` Adam steps of `samples_per_step` samples each (a multiple of 256); *last_loss (may be
+ * null) = the mean squared error of the last step, per channel, before its update. */
+gsr_status gsr_colour_model_train(gsr_colour_model* model, uint32_t steps, uint32_t samples_per_step,
+                                  float lr_weights, float lr_latents, float* last_loss);
+/* The mean squared error over `samples` samples (a multiple of 256) of stream `seed`. */
+gsr_status gsr_colour_model_loss(gsr_colour_model* model, uint32_t seed, uint32_t samples, float* loss);
+/* The gradient of that error with respect to the parameters and latents (no update), and the
+ * samples it used (gaussians: samples values, dirs: samples x 3; either may be null). */
+gsr_status gsr_colour_model_gradients(gsr_colour_model* model, uint32_t seed, uint32_t samples, float* grad_params,
+                                      float* grad_latents, float* loss, uint32_t* gaussians, float* dirs);
+gsr_status gsr_colour_model_get(gsr_colour_model* model, float* params, float* latents);
+gsr_status gsr_colour_model_set(gsr_colour_model* model, const float* params, const float* latents);
+/* y + 0.5 (unclamped) for n (gaussian, direction) pairs; rgb: n x 3. */
+gsr_status gsr_colour_model_evaluate(gsr_colour_model* model, const uint32_t* gaussians, const float* dirs, uint32_t n,
+                                     float* rgb);
+/* Marks the gaussians that use the MLP (error <= threshold; a negative threshold marks none);
+ * *mlp_count (may be null) = how many; errors (may be null): count floats, the largest channel
+ * difference of each gaussian over the test directions. */
+gsr_status gsr_colour_model_fallback(gsr_colour_model* model, float threshold, uint32_t* mlp_count, float* errors);
+/* Binds a model of the renderer's scene (null: plain SH colour again). */
+gsr_status gsr_renderer_set_colour_model(gsr_renderer* r, gsr_colour_model* model);
+
 /* --- PLY scenes (loadGaussianPly, scene/ply_loader.h:1-12 and 86-95) ---------------------------
  * A binary little-endian INRIA 3DGS .ply file decoded and activated on the host into the f32
  * SoA arrays gsr_scene_create takes (format GSR_SCENE_F32_SOA, device_pointers 0):

@@ -1,0 +1,340 @@
+"""Learned colour (gsr_colour_model_*): a 16-value latent per gaussian and one shared MLP
+([latent, direction, 1] -> 64 -> ReLU -> 64 -> ReLU -> 3) in bf16 MFMA products with fp32
+accumulation, trained towards the scene's SH colours. No counterpart in the reference: the
+checks pin it against
+  - a numpy restatement that rounds the same operands to bf16 (round to nearest even) as the
+    kernels: forward and every gradient to fp32 accumulation-order noise;
+  - float32 autograd (torch, CPU) without any bf16 rounding: the gradients' direction (a wrong
+    mask, sign, index or transposed tile fails both);
+  - the oracle's renderer on a degree-0 scene whose colours are the expected per-gaussian
+    colours (MLP colour where the fallback pass marked the gaussian, SH colour elsewhere):
+    RGBA8 within 1 level per channel; with no gaussian marked, bit-exact to plain SH.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+C0 = np.float32(0.28209479177387814)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bf16(a):
+    """float32 -> bf16 (round to nearest even) -> float32."""
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    return r.view(np.float32).reshape(a.shape)
+
+
+def unpack(p):
+    return (p[0:2048].reshape(64, 32), p[2048:6144].reshape(64, 64), p[6144:6208], p[6208:6400].reshape(3, 64),
+            p[6400:6403])
+
+
+def inputs(lat, g, d):
+    """(32, S): latent rows 0..15, direction 16..18, the constant 1 at 19."""
+    S = len(g)
+    X = np.zeros((32, S), np.float32)
+    X[:16] = lat[g].T
+    X[16:19] = d.T
+    X[19] = 1.0
+    return X
+
+
+def f32(x):
+    return np.asarray(x, np.float64).astype(np.float32)
+
+
+def forward_bf16(p, lat, g, d):
+    W1, W2, b2, W3, b3 = unpack(p)
+    X = bf16(inputs(lat, g, d))
+    h1 = np.maximum(f32(bf16(W1).astype(np.float64) @ X), 0)
+    h2 = np.maximum(f32(bf16(W2).astype(np.float64) @ bf16(h1) + b2[:, None]), 0)
+    y = f32(bf16(W3).astype(np.float64) @ bf16(h2) + b3[:, None])
+    return X, h1, h2, y
+
+
+def backward_bf16(p, lat, g, d, tgt, total):
+    """The kernels' backward: the operands of every product rounded to bf16, fp32 sums."""
+    W1, W2, b2, W3, b3 = unpack(p)
+    X, h1, h2, y = forward_bf16(p, lat, g, d)
+    e = f32(y + np.float32(0.5) - tgt.T)
+    loss = float((e.astype(np.float64) ** 2).sum() / (3 * total))
+    dy = f32(np.float32(2.0) / (np.float32(3.0) * np.float32(total)) * e)
+    D = lambda a, b: f32(np.asarray(a, np.float64) @ np.asarray(b, np.float64))  # noqa: E731
+    dW3 = D(bf16(dy), bf16(h2).T)
+    db3 = dy.astype(np.float64).sum(1)
+    dH2 = D(bf16(W3).T, bf16(dy)) * (h2 > 0)
+    db2 = dH2.astype(np.float64).sum(1)
+    dW2 = D(bf16(dH2), bf16(h1).T)
+    dH1 = D(bf16(W2).T, bf16(dH2)) * (h1 > 0)
+    dW1 = D(bf16(dH1), X.T)
+    dlat_s = D(bf16(W1[:, :16]).T, bf16(dH1))
+    gl = np.zeros_like(lat, dtype=np.float64)
+    np.add.at(gl, g, dlat_s.T)
+    gp = np.concatenate([dW1.reshape(-1), dW2.reshape(-1), db2, dW3.reshape(-1), db3])
+    return gp, gl, loss
+
+
+def autograd_fp32(p, lat, g, d, tgt, total):
+    """float32 autograd of the same loss, no bf16 rounding."""
+    import torch
+
+    P = torch.tensor(p, requires_grad=True)
+    L = torch.tensor(lat, requires_grad=True)
+    W1, W2, b2, W3, b3 = P[0:2048].view(64, 32), P[2048:6144].view(64, 64), P[6144:6208], P[6208:6400].view(3, 64), \
+        P[6400:6403]
+    S = len(g)
+    x = torch.cat([L[torch.tensor(g.astype(np.int64))], torch.tensor(d), torch.ones(S, 1), torch.zeros(S, 12)], 1)
+    h1 = torch.relu(x @ W1.T)
+    h2 = torch.relu(h1 @ W2.T + b2)
+    y = h2 @ W3.T + b3
+    loss = ((y + 0.5 - torch.tensor(tgt)) ** 2).sum() / (3 * total)
+    loss.backward()
+    return P.grad.numpy(), L.grad.numpy(), float(loss.detach())
+
+
+def sh_colours(oracle, sc, g, d):
+    return np.stack([oracle.eval_sh(sc.sh_degree, sc.sh[gi], di) for gi, di in zip(g, d)]).astype(np.float32)
+
+
+def _scene(gsr, ctx, n, seed, **kw):
+    from gsr import scenes
+
+    sc = scenes.make_scene(n, seed=seed, mean_sigma=2.0, **kw)
+    return sc, gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+
+
+def test_forward_matches_bf16_restatement(gsr_ctx):
+    import gsr
+
+    log("case colour forward 3000 gaussians, 1000 pairs")
+    sc, scene = _scene(gsr, gsr_ctx, 3000, 21)
+    m = gsr.ColourModel(gsr_ctx, scene, seed=5)
+    p, lat = m.get()
+    assert np.isfinite(p).all() and np.isfinite(lat).all()
+    assert np.all(p[0:2048].reshape(64, 32)[:, 19:] == 0), "layer-1 bias and padding columns start at zero"
+    rng = np.random.default_rng(0)
+    n = 1000  # not a multiple of 16: a partial last tile
+    g = rng.integers(0, 3000, n).astype(np.uint32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out = m.evaluate(g, d)
+    _, _, _, y = forward_bf16(p, lat, g, d)
+    want = y.T + np.float32(0.5)
+    err = np.abs(out - want).max()
+    log(f"  max |gpu - bf16 restatement| = {err:.2e}, spread of the outputs {want.std():.3f}")
+    assert err <= 2e-5 * max(1.0, np.abs(want).max())
+    # Without the bf16 roundings the outputs move by the bf16 error only.
+    W1, W2, b2, W3, b3 = unpack(p.astype(np.float64))
+    X = inputs(lat, g, d).astype(np.float64)
+    y64 = W3 @ np.maximum(W2 @ np.maximum(W1 @ X, 0) + b2[:, None], 0) + b3[:, None]
+    assert np.abs(out - (y64.T + 0.5)).max() < 0.05
+    # Trained parameters and latents set from the host are used as given.
+    p2 = p.copy()
+    p2[6400:6403] += np.float32(0.25)
+    m.set(p2, lat * np.float32(0.5))
+    _, _, _, y2 = forward_bf16(p2, lat * np.float32(0.5), g, d)
+    assert np.abs(m.evaluate(g, d) - (y2.T + 0.5)).max() <= 2e-5 * max(1.0, np.abs(y2).max())
+    q, lat2 = m.get()
+    assert np.array_equal(q, p2) and np.array_equal(lat2, lat * np.float32(0.5))
+    m.close()
+    scene.close()
+
+
+@pytest.mark.parametrize("fmt", ["f32", "f16"])
+def test_gradients_match_restatement_and_autograd(gsr_ctx, oracle, fmt):
+    import gsr
+    from gsr.scenes import SyntheticScene
+
+    log(f"case colour gradients 2000 gaussians, 1024 samples, scene {fmt}")
+    from gsr import scenes
+
+    # Colours well inside (0, 1): the clamped SH colour is the raw one the training targets.
+    sc = scenes.make_scene(2000, seed=4, mean_sigma=2.0, dc_sigma=0.2, rest_sigma=0.03)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
+    if fmt == "f16":  # the targets come from what the GPU decodes
+        gg, cc = oracle.encode_f16(sc)
+        sc = SyntheticScene(*oracle.decode_f16(gg, cc, sc.sh_degree), sc.sh_degree, sc.seed)
+    m = gsr.ColourModel(gsr_ctx, scene, seed=9)
+    m.train(3, 4096)  # away from the initial point (Adam steps): non-trivial ReLU masks and latents
+    p, lat = m.get()
+    S = 1024
+    gp, gl, loss, g, d = m.gradients(1234, S)
+    assert g.max() < 2000 and np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)
+    assert len(np.unique(g)) > 600, "samples spread over the gaussians"
+    tgt = sh_colours(oracle, sc, g, d)
+    assert tgt.min() > 0.0
+    rp, rl, rloss = backward_bf16(p, lat, g, d, tgt, S)
+    log(f"  loss gpu {loss:.6f} restatement {rloss:.6f}")
+    assert abs(loss - rloss) <= 1e-4 * rloss
+    for name, a, b in (("params", gp, rp), ("latents", gl.reshape(-1), rl.reshape(-1))):
+        err = np.abs(a - b).max()
+        log(f"  d{name}: max |gpu - restatement| {err:.2e} of max {np.abs(b).max():.2e}")
+        assert err <= 2e-3 * np.abs(b).max() + 1e-9, name
+    # Every block of the parameter vector gets gradient (none is stuck at zero).
+    for lo, hi in ((0, 2048), (2048, 6144), (6144, 6208), (6208, 6400), (6400, 6403)):
+        assert np.abs(gp[lo:hi]).max() > 0, (lo, hi)
+    assert np.all(gp[0:2048].reshape(64, 32)[:, 20:] == 0), "padding columns get no gradient"
+    ap, al, aloss = autograd_fp32(p, lat, g, d, tgt, S)
+    # bf16 bounds: the weight gradients average ~1000 samples (0.6% measured); a latent's comes
+    # from its own 1-2 samples through three chained bf16 products whose sums cancel (6%).
+    for name, a, b, min_cos, max_rel in (("params", gp, ap, 0.999, 0.03),
+                                         ("latents", gl.reshape(-1), al.reshape(-1), 0.995, 0.12)):
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+        rel = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        log(f"  d{name} vs fp32 autograd: cosine {cos:.5f}, relative error {rel:.3e}")
+        assert cos > min_cos and rel < max_rel, name
+    assert abs(loss - aloss) < 0.02 * aloss
+    # The gradient buffers are left clear: a training step after this call is unaffected.
+    gp2, _, loss2, _, _ = m.gradients(1234, S)
+    assert np.array_equal(gp2, gp) or np.abs(gp2 - gp).max() <= 1e-6 * np.abs(gp).max()
+    assert loss2 == pytest.approx(loss, rel=1e-6)
+    m.close()
+    scene.close()
+
+
+def test_training_lowers_the_error(gsr_ctx):
+    import gsr
+
+    log("case colour training 20k gaussians")
+    sc, scene = _scene(gsr, gsr_ctx, 20_000, 7)
+    m = gsr.ColourModel(gsr_ctx, scene, seed=1)
+    before = m.loss(99, 1 << 16)
+    last = m.train(400, 1 << 16, lr_weights=2e-3, lr_latents=2e-2)
+    after = m.loss(99, 1 << 16)
+    log(f"  MSE before {before:.5f}, after {after:.5f} (last step {last:.5f})")
+    assert after < 0.2 * before
+    p, lat = m.get()
+    assert np.isfinite(p).all() and np.isfinite(lat).all()
+    with pytest.raises(gsr.GsrError, match="multiple of 256"):
+        m.train(1, 1000)
+    with pytest.raises(gsr.GsrError, match="gaussian"):
+        m.evaluate(np.array([20_000], np.uint32), np.zeros((1, 3), np.float32))
+    m.close()
+    scene.close()
+
+
+def _expected_colours(oracle, m, sc, cam, marks):
+    """The colour each gaussian should be drawn with from this camera: the MLP colour where
+    marked, the SH colour elsewhere (directions as the preprocess computes them, in fp32)."""
+    pos = sc.pos_alpha[:, :3]
+    camp = np.asarray(cam.position[:3], np.float32)
+    dx, dy, dz = (pos[:, k] - camp[k] for k in range(3))
+    inv = np.float32(1.0) / np.sqrt((dx * dx + dy * dy) + dz * dz)
+    d = np.stack([dx * inv, dy * inv, dz * inv], 1).astype(np.float32)
+    col = np.zeros((sc.count, 3), np.float32)
+    idx = np.nonzero(marks)[0]
+    if idx.size:
+        col[idx] = np.maximum(m.evaluate(idx.astype(np.uint32), d[idx]), 0)
+    rest = np.nonzero(~marks)[0]
+    if rest.size:
+        col[rest] = sh_colours(oracle, sc, rest, d[rest])
+    return col
+
+
+def _degree0(sc, col):
+    from gsr.scenes import SyntheticScene
+
+    dc = ((col - np.float32(0.5)) / C0).astype(np.float32)
+    return SyntheticScene(sc.pos_alpha, sc.scales, sc.rotations, np.ascontiguousarray(dc), 0, sc.seed)
+
+
+@pytest.mark.parametrize("sort_mode,render_splat", [("binning", "full"), ("global", "full"), ("global", "p24"),
+                                                    ("binning", "p32")])
+def test_render_with_learned_colour_matches_oracle(gsr_ctx, oracle, sort_mode, render_splat):
+    import gsr
+    from gsr import camera
+
+    log(f"case colour render 30k sort_mode={sort_mode} render_splat={render_splat}")
+    sc, scene = _scene(gsr, gsr_ctx, 30_000, 12)
+    m = gsr.ColourModel(gsr_ctx, scene, seed=3)
+    m.train(150, 1 << 16, lr_weights=2e-3, lr_latents=2e-2)
+    W, H = 640, 360
+    cam = gsr.camera_from_viewer(camera.orbit_view(7.0, 15.0, 40.0), camera.perspective(60.0, W / H),
+                                 camera.IDENTITY, W, H)
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    plain = gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode=sort_mode, render_splat=render_splat)
+    want_plain = plain.render_to_host(cam)
+    r = gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode=sort_mode, render_splat=render_splat)
+    r.set_colour_model(m)
+    # No gaussian marked: exactly the SH render.
+    n0, err = m.fallback(-1.0)
+    assert n0 == 0 and np.all(err >= 0)
+    assert np.array_equal(r.render_to_host(cam), want_plain)
+    for threshold in (float(np.median(err)), 1e9):
+        n, err2 = m.fallback(threshold)
+        assert np.array_equal(err2, err), "the errors do not depend on the threshold"
+        marks = err <= np.float32(threshold)
+        assert n == int(marks.sum())
+        img = r.render_to_host(cam)
+        col = _expected_colours(oracle, m, sc, cam, marks)
+        if render_splat == "full":
+            # The drawn colours: the MLP's exactly where the direction is the same float.
+            gs, _ = r.read_back_splats()
+            vis = gs["cull_reason"] == 0
+            got = gs["color"][vis]
+            diff = np.abs(got - col[vis]).max(axis=1)
+            log(f"  threshold {threshold:.3g}: {n} marked, splat colours exact for "
+                f"{(diff == 0).mean() * 100:.2f}% of visible, max diff {diff.max():.2e}")
+            assert (diff == 0).mean() > 0.99 and diff.max() < 2e-2
+        ref, _ = oracle.render(_degree0(sc, col), ocam, bounds=oracle.TIGHT, render_splat=render_splat)
+        q = oracle.quantize8(ref)
+        d8 = np.abs(img[..., :3].astype(np.int32) - q.astype(np.int32))
+        p = oracle.psnr(img[..., :3] / 255.0, q / 255.0)
+        log(f"  threshold {threshold:.3g}: max level diff {d8.max()}, PSNR vs oracle {p:.2f} dB")
+        assert d8.max() <= 1 and p >= 58.7
+    # Everything learned vs SH: the learned image is close (the fallback error bounds the colours).
+    pl = oracle.psnr(img[..., :3] / 255.0, want_plain[..., :3] / 255.0)
+    log(f"  all learned vs SH: PSNR {pl:.2f} dB")
+    # A new parameter set clears the marks: SH again until the next fallback pass.
+    p_, lat_ = m.get()
+    m.set(p_, lat_)
+    assert np.array_equal(r.render_to_host(cam), want_plain)
+    r.set_colour_model(None)
+    assert np.array_equal(r.render_to_host(cam), want_plain)
+    _, other = _scene(gsr, gsr_ctx, 100, 1)
+    ro = gsr.Renderer(gsr_ctx, other, target="rgba8")
+    with pytest.raises(gsr.GsrError, match="another scene"):
+        ro.set_colour_model(m)
+    ro.close()
+    other.close()
+    for x in (r, plain):
+        x.close()
+    m.close()
+    scene.close()
+
+
+def test_fallback_errors_match_restatement(gsr_ctx, oracle):
+    import gsr
+
+    log("case colour fallback errors 500 gaussians")
+    sc, scene = _scene(gsr, gsr_ctx, 500, 13)
+    m = gsr.ColourModel(gsr_ctx, scene, seed=2)
+    m.train(50, 1 << 14)
+    n, err = m.fallback(0.05)
+    assert n == int((err <= np.float32(0.05)).sum())
+    # 32 test directions on a Fibonacci sphere; the error is the largest channel difference
+    # between the clamped MLP colour and the clamped SH colour.
+    k = np.arange(32, dtype=np.float32)
+    z = np.float32(1.0) - (np.float32(2.0) * k + np.float32(1.0)) / np.float32(32.0)
+    rr = np.sqrt(np.maximum(np.float32(0.0), np.float32(1.0) - z * z))
+    phi = np.float32(2.39996322973) * k
+    dirs = np.stack([rr * np.cos(phi), rr * np.sin(phi), z], 1).astype(np.float32)
+    gi = np.repeat(np.arange(500, dtype=np.uint32), 32)
+    di = np.tile(dirs, (500, 1))
+    mlp = np.maximum(m.evaluate(gi, di), 0)
+    sh = sh_colours(oracle, sc, gi, di)
+    want = np.abs(mlp - sh).max(axis=1).reshape(500, 32).max(axis=1)
+    diff = np.abs(err - want)
+    log(f"  {n} of 500 within 0.05; |gpu - restatement| median {np.median(diff):.2e} max {diff.max():.2e}")
+    assert np.median(diff) < 1e-5 and diff.max() < 1e-2
+    m.close()
+    scene.close()
