@@ -816,7 +816,8 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     if (r->colour && r->colour->mlpCount > 0) {
         const gsr_colour_model* cm = r->colour;
         launchColourMlp(ref, dc, cm->params.as<float>(), cm->packs.as<uint2>(), cm->latentsBf16.as<uint16_t>(),
-                        cm->useMlp.as<uint8_t>(), r->splats.as<float4>(), uint32_t(r->settings.render_splat),
+                        cm->useMlp.as<uint8_t>(), r->tileCounts.as<uint32_t>(), r->splats.as<float4>(),
+                        uint32_t(r->settings.render_splat),
                         r->packed.as<uint32_t>(), stream);
         GSR_HIP(hipGetLastError());
     }

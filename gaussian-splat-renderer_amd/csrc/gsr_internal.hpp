@@ -160,11 +160,11 @@ void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count,
                       hipStream_t stream);
 void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* packs, const uint16_t* latentsBf16,
                        float threshold, uint8_t* useMlp, float* err, hipStream_t stream);
-// After the preprocess (run with the same useMlp): the MLP colour of the visible gaussians with
-// useMlp set, into the splats (and the packed records in the p32 / p24 modes).
+// After the preprocess (run with the same useMlp): the MLP colour of the gaussians with useMlp set
+// and a non-zero tile count, into the splats (and the packed records in the p32 / p24 modes).
 void launchColourMlp(const SceneRef& scene, const DevCamera& cam, const float* params, const uint2* packs,
-                     const uint16_t* latentsBf16, const uint8_t* useMlp, float4* splats, uint32_t renderSplat,
-                     uint32_t* packed, hipStream_t stream);
+                     const uint16_t* latentsBf16, const uint8_t* useMlp, const uint32_t* tileCounts, float4* splats,
+                     uint32_t renderSplat, uint32_t* packed, hipStream_t stream);
 void launchMlpEvaluate(const float* params, const uint2* packs, const float* latents, const uint32_t* gaussians,
                        const float* dirs, uint32_t n, float* rgb, hipStream_t stream);
 

@@ -32,20 +32,38 @@ constexpr int kW3 = kB2 + 64;
 constexpr int kB3 = kW3 + 3 * 64;
 // Packed A tiles (64 lanes x 4 bf16 each): the forward's W1 (4 x 2 tiles), W2 (4 x 4), W3 (rows
 // padded to 16: 1 x 4), and the backward's W3^T (4 x 1), W2^T (4 x 4), W1^T rows 0..15 (1 x 4).
-constexpr int kTA1 = 0, kTA2 = 8, kTA3 = 24, kTA3T = 28, kTA2T = 32, kTA1T = 48;
+constexpr int kTA1 = 0, kTA2 = 8, kTA3 = 24, kTA3T = 28, kTA2T = 32, kTA1T = 48, kTACount = 52;
+// The render-time forward's tiles for v_mfma_f32_16x16x32_bf16 (64 lanes x 8 bf16 each, after the
+// 16x16x16 tiles): W1 (4 x 1), W2 (4 x 2), W3 (1 x 2). Lane l, slot j holds W[l % 16][k] with
+// k = kb + 4 (l / 16) + j for j < 4 and kb + 16 + 4 (l / 16) + j - 4 for j >= 4 (kb = 32 x the
+// tile's K index): the two D-layout tiles of rows kb .. kb + 15 and kb + 16 .. kb + 31 of the
+// previous layer are then, side by side, the B operand with the same k in every slot. The sum
+// over k does not depend on which k the hardware puts in a slot, as long as A and B agree.
+constexpr int kTB1 = 0, kTB2 = 4, kTB3 = 12, kTBCount = 14;
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kTestDirs = 32;  // fallback test directions per gaussian
 
-__device__ __forceinline__ uint16_t bf16Bits(float f) {  // round to nearest even (finite values)
-    const uint32_t u = __float_as_uint(f);
-    return uint16_t((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+// fp32 -> bf16, round to nearest even (gfx950's v_cvt_pk_bf16_f32, two values per instruction).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t bf16Bits(float f) { return __builtin_bit_cast(uint16_t, __bf16(f)); }
+__device__ __forceinline__ uint32_t bf16Pair(float a, float b) {  // one v_cvt_pk_bf16_f32
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
-__device__ __forceinline__ bf16x4 toBf16(f32x4 v) {
-    bf16x4 r;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) r[j] = short(bf16Bits(v[j]));
-    return r;
+__device__ __forceinline__ bf16x4 fromPairs(uint32_t lo, uint32_t hi) {
+    return __builtin_bit_cast(bf16x4, make_uint2(lo, hi));
+}
+__device__ __forceinline__ bf16x4 toBf16(f32x4 v) { return fromPairs(bf16Pair(v[0], v[1]), bf16Pair(v[2], v[3])); }
+// bf16(max(v, 0)) = max(bf16(v), 0): on the bf16 bit patterns as signed 16-bit integers, every
+// negative value (and -0) is below 0 and the non-negative ones keep their order.
+__device__ __forceinline__ uint32_t reluPair(uint32_t p) {
+    const i16x2 v = __builtin_elementwise_max(__builtin_bit_cast(i16x2, p), i16x2{0, 0});
+    return __builtin_bit_cast(uint32_t, v);
+}
+__device__ __forceinline__ bf16x4 reluBf16(f32x4 v) {
+    return fromPairs(reluPair(bf16Pair(v[0], v[1])), reluPair(bf16Pair(v[2], v[3])));
 }
 __device__ __forceinline__ bf16x4 tileA(const uint2* packs, int tile, uint32_t lane) {
     const uint2 u = packs[tile * 64 + lane];
@@ -86,13 +104,37 @@ __device__ float packedValue(const float* P, int tile, uint32_t lane, int j) {
     }
 }
 
+// Value j of lane `lane` in K=32 tile `tile`.
+__device__ float packedValue32(const float* P, int tile, uint32_t lane, int j) {
+    const int r = int(lane % 16u), q = int(4 * (lane / 16u));
+    const int k = j < 4 ? q + j : 16 + q + j - 4;
+    if (tile < kTB2) return P[kW1 + (16 * tile + r) * 32 + k];  // W1: mi = tile
+    if (tile < kTB3) {                                          // W2: mi = t / 2, K half t % 2
+        const int t = tile - kTB2;
+        return P[kW2 + (16 * (t / 2) + r) * 64 + 32 * (t % 2) + k];
+    }
+    return r < 3 ? P[kW3 + r * 64 + 32 * (tile - kTB3) + k] : 0.0f;  // W3, rows 0..2 of 16
+}
+
+// Blocks 0 .. kTACount - 1: the 16x16x16 tiles; then the kTBCount 16x16x32 tiles.
 __global__ __launch_bounds__(64) void kPack(const float* __restrict__ params, uint2* __restrict__ packs) {
     const int tile = int(blockIdx.x);
     const uint32_t lane = threadIdx.x;
-    uint16_t h[4];
+    if (tile < kTACount) {
+        uint16_t h[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) h[j] = bf16Bits(packedValue(params, tile, lane, j));
-    packs[tile * 64 + lane] = make_uint2(uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16));
+        for (int j = 0; j < 4; ++j) h[j] = bf16Bits(packedValue(params, tile, lane, j));
+        packs[tile * 64 + lane] =
+            make_uint2(uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16));
+    } else {
+        const int t = tile - kTACount;
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            w[j] = uint32_t(bf16Bits(packedValue32(params, t, lane, 2 * j))) |
+                   (uint32_t(bf16Bits(packedValue32(params, t, lane, 2 * j + 1))) << 16);
+        reinterpret_cast<uint4*>(packs + kTACount * 64)[t * 64 + lane] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
 }
 
 struct Forward {
@@ -132,6 +174,51 @@ __device__ __forceinline__ void forward(const uint2* packs, const float* params,
 #pragma unroll
     for (int ki = 0; ki < 4; ++ki) acc = mfma(tileA(packs, kTA3 + ki, lane), c[ki], acc);
     f.y = acc;
+}
+
+// The render-time forward (v_mfma_f32_16x16x32_bf16, 14 per 16 samples) with its weights held in
+// registers for a wave's lifetime (56 VGPRs of A tiles + 20 of biases): the kernels that use it
+// read no weight memory per tile.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+struct Weights {
+    bf16x8 w1[4], w2[8], w3[2];
+    f32x4 b2[4], b3;
+};
+__device__ __forceinline__ bf16x8 concat(bf16x4 lo, bf16x4 hi) {
+    return __builtin_bit_cast(bf16x8, make_uint4(__builtin_bit_cast(uint2, lo).x, __builtin_bit_cast(uint2, lo).y,
+                                                 __builtin_bit_cast(uint2, hi).x, __builtin_bit_cast(uint2, hi).y));
+}
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ void loadWeights(const uint2* packs, const float* params, uint32_t lane, Weights& w) {
+    const uint4* t32 = reinterpret_cast<const uint4*>(packs + kTACount * 64);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w.w1[t] = __builtin_bit_cast(bf16x8, t32[(kTB1 + t) * 64 + lane]);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) w.w2[t] = __builtin_bit_cast(bf16x8, t32[(kTB2 + t) * 64 + lane]);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) w.w3[t] = __builtin_bit_cast(bf16x8, t32[(kTB3 + t) * 64 + lane]);
+    const int rq = int(4 * (lane / 16u));
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) w.b2[mi][j] = params[kB2 + 16 * mi + rq + j];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w.b3[j] = rq + j < 3 ? params[kB3 + rq + j] : 0.0f;
+}
+// y (lane l < 16: the 3 outputs of sample l in y[0..2]). The same bf16 operands as forward(); the
+// fp32 sums may round differently (32 products per instruction instead of 16).
+__device__ __forceinline__ f32x4 forwardY(const Weights& w, f32x4 x0, f32x4 x1) {
+    const bf16x8 bx = concat(toBf16(x0), toBf16(x1));
+    bf16x4 h[4], c[4];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) h[mi] = reluBf16(mfma32(w.w1[mi], bx, f32x4{0.0f, 0.0f, 0.0f, 0.0f}));
+    const bf16x8 bh0 = concat(h[0], h[1]), bh1 = concat(h[2], h[3]);
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi) c[mi] = reluBf16(mfma32(w.w2[2 * mi + 1], bh1, mfma32(w.w2[2 * mi], bh0, w.b2[mi])));
+    const bf16x8 bc0 = concat(c[0], c[1]), bc1 = concat(c[2], c[3]);
+    return mfma32(w.w3[1], bc1, mfma32(w.w3[0], bc0, w.b3));
 }
 
 // A D-layout tile of a 16 x 16 matrix M (lane l: M[4 (l / 16) + j][l % 16]) re-read in the
@@ -416,6 +503,8 @@ __global__ __launch_bounds__(kThreads) void kFallback(SceneRef scene, const floa
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t col = lane % 16u, rq = 4 * (lane / 16u);
     const uint32_t waves = gridDim.x * kWaves;
+    Weights w;
+    loadWeights(packs, params, lane, w);
     for (uint32_t g0 = (blockIdx.x * kWaves + wave) * 2; g0 < scene.count; g0 += waves * 2) {
         float worst[2] = {0.0f, 0.0f};
 #pragma unroll
@@ -426,17 +515,13 @@ __global__ __launch_bounds__(kThreads) void kFallback(SceneRef scene, const floa
             testDir(uint32_t(t % 2) * 16 + col, dir);
             f32x4 x0, x1;
             inputTiles(bf16Quad(latentsBf16 + size_t(gs) * 16 + rq), dir, lane, x0, x1);
-            Forward f;
-            forward(packs, params, x0, x1, lane, f);
+            const f32x4 y = forwardY(w, x0, x1);
             float e = 0.0f;
             if (lane < 16u) {
                 float sh[3];
                 shColour<FMT>(scene, gs, dir, sh);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const float y = fmaxf(f.y[c] + 0.5f, 0.0f);
-                    e = fmaxf(e, fabsf(y - sh[c]));
-                }
+                for (int c = 0; c < 3; ++c) e = fmaxf(e, fabsf(fmaxf(y[c] + 0.5f, 0.0f) - sh[c]));
             }
 #pragma unroll
             for (int off = 8; off > 0; off >>= 1) e = fmaxf(e, __shfl_xor(e, off, 64));
@@ -455,59 +540,79 @@ __global__ __launch_bounds__(kThreads) void kFallback(SceneRef scene, const floa
     }
 }
 
-// The render's colour for the visible gaussians that use the MLP (after the preprocess, which left
-// their colour at 0): dir = normalize(mean - camPos) as for SH; writes the splat's o2.xyz and, in
-// the packed render-splat modes, the fp16 colour of the packed record. A wave takes 64 gaussians
-// (4 tiles) per round and skips a tile none of whose gaussians needs the MLP.
+// The render's colour for the gaussians that use the MLP and cover a tile (after the preprocess,
+// which left their colour at 0): dir = normalize(mean - camPos) as for SH; writes the splat's
+// o2.xyz and, in the packed render-splat modes, the fp16 colour of the packed record. Persistent
+// waves with the weights in registers; a wave takes 4 tiles of 16 consecutive gaussians per
+// round, issues all their loads first (the round is latency-bound otherwise) and skips the MFMAs
+// of a tile none of whose gaussians needs the MLP.
+constexpr int kColourTiles = 4;
 template <int FMT>
 __global__ __launch_bounds__(kThreads) void kColourMlp(SceneRef scene, DevCamera cam, const float* __restrict__ params,
                                                        const uint2* __restrict__ packs,
                                                        const uint16_t* __restrict__ latentsBf16,
-                                                       const uint8_t* __restrict__ useMlp, float4* __restrict__ splats,
-                                                       uint32_t renderSplat, uint32_t* __restrict__ packed) {
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+                                                       const uint8_t* __restrict__ useMlp,
+                                                       const uint32_t* __restrict__ tileCounts,
+                                                       float4* __restrict__ splats, uint32_t renderSplat,
+                                                       uint32_t* __restrict__ packed) {
+    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t col = lane % 16u, rq = 4 * (lane / 16u);
-    const uint32_t waves = gridDim.x * kWaves;
-    for (uint32_t g0 = (blockIdx.x * kWaves + wave) * 64; g0 < scene.count; g0 += waves * 64) {
-#pragma unroll 1
-        for (int t = 0; t < 4; ++t) {
-            const uint32_t g = g0 + uint32_t(t) * 16 + col;
-            const bool need = g < scene.count && useMlp[g] &&
-                              __float_as_uint(splats[3 * size_t(g)].w) == GSR_CULL_VISIBLE;
-            if (!__any(need)) continue;  // wave-uniform
+    const uint32_t waves = gridDim.x * kWaves, rounds = (scene.count + 16 * kColourTiles - 1) / (16 * kColourTiles);
+    Weights w;
+    loadWeights(packs, params, lane, w);
+    for (uint32_t round = blockIdx.x * kWaves + (threadIdx.x >> 6); round < rounds; round += waves) {
+        // Every load of the round first, unconditionally (no load waits on another's value), and
+        // all of them in registers before the one skip test: one memory round trip per round.
+        uint32_t mark[kColourTiles], cover[kColourTiles];
+        float3 pos[kColourTiles];
+        float4 lat[kColourTiles];
+#pragma unroll
+        for (int u = 0; u < kColourTiles; ++u) {
+            const uint32_t g = (round * kColourTiles + uint32_t(u)) * 16 + col;
             const uint32_t gs = g < scene.count ? g : scene.count - 1;
-            float px, py, pz;
+            mark[u] = useMlp[gs];
+            cover[u] = tileCounts[gs];
             if (FMT == GSR_SCENE_F32_SOA) {
                 const float4 pa = scene.posAlpha[gs];
-                px = pa.x;
-                py = pa.y;
-                pz = pa.z;
+                pos[u] = make_float3(pa.x, pa.y, pa.z);
             } else {
-                const uint4 g0v = scene.geometry[gs];
-                px = __uint_as_float(g0v.x);
-                py = __uint_as_float(g0v.y);
-                pz = __uint_as_float(g0v.z);
+                const uint4 gv = scene.geometry[gs];
+                pos[u] = make_float3(__uint_as_float(gv.x), __uint_as_float(gv.y), __uint_as_float(gv.z));
             }
-            const float dx = px - cam.pos[0], dy = py - cam.pos[1], dz = pz - cam.pos[2];
+            lat[u] = bf16Quad(latentsBf16 + size_t(gs) * 16 + rq);
+        }
+        bool need[kColourTiles], any = false;
+#pragma unroll
+        for (int u = 0; u < kColourTiles; ++u) {
+            const uint32_t g = (round * kColourTiles + uint32_t(u)) * 16 + col;
+            need[u] = (g < scene.count) & (mark[u] != 0u) & (cover[u] != 0u);
+            any |= need[u];
+            asm volatile("" ::"v"(pos[u].x), "v"(pos[u].y), "v"(pos[u].z), "v"(lat[u].x), "v"(lat[u].y),
+                         "v"(lat[u].z), "v"(lat[u].w));
+        }
+        if (!__any(any)) continue;  // wave-uniform: no gaussian of the round needs the MLP
+#pragma unroll
+        for (int u = 0; u < kColourTiles; ++u) {
+            const uint32_t g = (round * kColourTiles + uint32_t(u)) * 16 + col;
+            const float dx = pos[u].x - cam.pos[0], dy = pos[u].y - cam.pos[1], dz = pos[u].z - cam.pos[2];
             const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
             const float dir[3] = {dx * inv, dy * inv, dz * inv};
             f32x4 x0, x1;
-            inputTiles(bf16Quad(latentsBf16 + size_t(gs) * 16 + rq), dir, lane, x0, x1);
-            Forward f;
-            forward(packs, params, x0, x1, lane, f);  // every lane: the MFMAs need the whole wave
-            if (lane < 16u && need) {
-                const float r = fmaxf(f.y[0] + 0.5f, 0.0f), gr = fmaxf(f.y[1] + 0.5f, 0.0f),
-                            b = fmaxf(f.y[2] + 0.5f, 0.0f);
+            inputTiles(lat[u], dir, lane, x0, x1);
+            const f32x4 y = forwardY(w, x0, x1);  // every lane: the MFMAs need the whole wave
+            if (lane < 16u && need[u]) {
+                const float r = fmaxf(y[0] + 0.5f, 0.0f), gr = fmaxf(y[1] + 0.5f, 0.0f),
+                            b = fmaxf(y[2] + 0.5f, 0.0f);
                 *reinterpret_cast<float3*>(splats + 3 * size_t(g) + 2) = make_float3(r, gr, b);
                 const uint32_t rg = halfBits(r) | (halfBits(gr) << 16);
                 if (renderSplat == GSR_RENDER_SPLAT_P32) {
-                    uint32_t* w = packed + 8 * size_t(g);
-                    w[6] = rg;
-                    w[7] = (w[7] & 0xFFFF0000u) | halfBits(b);  // the radius stays in the high half
+                    uint32_t* pw = packed + 8 * size_t(g);
+                    pw[6] = rg;
+                    pw[7] = (pw[7] & 0xFFFF0000u) | halfBits(b);  // the radius stays in the high half
                 } else if (renderSplat == GSR_RENDER_SPLAT_P24) {
-                    uint32_t* w = packed + 6 * size_t(g);
-                    w[4] = rg;
-                    w[5] = (w[5] & 0xFFFF0000u) | halfBits(b);
+                    uint32_t* pw = packed + 6 * size_t(g);
+                    pw[4] = rg;
+                    pw[5] = (pw[5] & 0xFFFF0000u) | halfBits(b);
                 }
             }
         }
@@ -529,11 +634,12 @@ __global__ __launch_bounds__(kThreads) void kEvaluate(const float* __restrict__ 
     f32x4 x0, x1;
     inputTiles(*reinterpret_cast<const float4*>(latents + size_t(gaussians[s]) * 16 + 4 * (lane / 16u)), dir, lane,
                x0, x1);
-    Forward f;
-    forward(packs, params, x0, x1, lane, f);
+    Weights w;
+    loadWeights(packs, params, lane, w);
+    const f32x4 y = forwardY(w, x0, x1);
     if (lane < 16u && t0 + col < n) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) rgb[3 * size_t(s) + c] = f.y[c] + 0.5f;
+        for (int c = 0; c < 3; ++c) rgb[3 * size_t(s) + c] = y[c] + 0.5f;
     }
 }
 
@@ -545,10 +651,10 @@ uint32_t gridFor(uint32_t n, uint32_t perBlock, uint32_t cap) {
 }  // namespace
 
 int mlpParamCount() { return kB3 + 3; }
-int mlpPackedTiles() { return kTA1T + 4; }
+int mlpPackedTiles() { return kTACount + 2 * kTBCount; }  // units of 64 x uint2
 
 void launchMlpPack(const float* params, uint2* packs, hipStream_t stream) {
-    hipLaunchKernelGGL(kPack, dim3(mlpPackedTiles()), dim3(64), 0, stream, params, packs);
+    hipLaunchKernelGGL(kPack, dim3(kTACount + kTBCount), dim3(64), 0, stream, params, packs);
 }
 
 void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, const float* latents,
@@ -608,7 +714,7 @@ void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count,
 
 void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* packs, const uint16_t* latentsBf16,
                        float threshold, uint8_t* useMlp, float* err, hipStream_t stream) {
-    const uint32_t blocks = gridFor(scene.count, 2 * kWaves, 8192);
+    const uint32_t blocks = gridFor(scene.count, 2 * kWaves, 2048);
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kFallback<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kThreads), 0, stream, scene, params, packs,
@@ -619,16 +725,17 @@ void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* 
 }
 
 void launchColourMlp(const SceneRef& scene, const DevCamera& cam, const float* params, const uint2* packs,
-                     const uint16_t* latentsBf16, const uint8_t* useMlp, float4* splats, uint32_t renderSplat,
-                     uint32_t* packed, hipStream_t stream) {
-    const uint32_t blocks = gridFor(scene.count, 64 * kWaves, 8192);
+                     const uint16_t* latentsBf16, const uint8_t* useMlp, const uint32_t* tileCounts, float4* splats,
+                     uint32_t renderSplat, uint32_t* packed, hipStream_t stream) {
+    // Persistent: at most 4 waves per SIMD (each loads the weights once), 64 gaussians per round.
+    const uint32_t blocks = gridFor(scene.count, 16 * kColourTiles * kWaves, 1024);
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kColourMlp<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kThreads), 0, stream, scene, cam, params,
-                           packs, latentsBf16, useMlp, splats, renderSplat, packed);
+                           packs, latentsBf16, useMlp, tileCounts, splats, renderSplat, packed);
     else
         hipLaunchKernelGGL(kColourMlp<GSR_SCENE_F16>, dim3(blocks), dim3(kThreads), 0, stream, scene, cam, params,
-                           packs, latentsBf16, useMlp, splats, renderSplat, packed);
+                           packs, latentsBf16, useMlp, tileCounts, splats, renderSplat, packed);
 }
 
 void launchMlpEvaluate(const float* params, const uint2* packs, const float* latents, const uint32_t* gaussians,

@@ -277,9 +277,10 @@ def test_render_with_learned_colour_matches_oracle(gsr_ctx, oracle, sort_mode, r
         img = r.render_to_host(cam)
         col = _expected_colours(oracle, m, sc, cam, marks)
         if render_splat == "full":
-            # The drawn colours: the MLP's exactly where the direction is the same float.
-            gs, _ = r.read_back_splats()
-            vis = gs["cull_reason"] == 0
+            # The drawn colours (splats that cover a tile): the MLP's exactly where the direction
+            # is the same float.
+            gs, gc = r.read_back_splats()
+            vis = (gs["cull_reason"] == 0) & (gc > 0)
             got = gs["color"][vis]
             diff = np.abs(got - col[vis]).max(axis=1)
             log(f"  threshold {threshold:.3g}: {n} marked, splat colours exact for "
