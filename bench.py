@@ -11,8 +11,9 @@ frames are timed with HIP events on the render stream.
 Prints ONE JSON line: ms/frame (value), per-stage ms, sorted keys/s, the roofline of the
 dominant kernel, the fp16-scene and other-sort-mode frame times, the image quality of one
 frame against the CPU oracle, and the oracle's own CPU time on the same frame (cpu_baseline);
-also the batched multi-view throughput (32 cameras of mixed sizes on a 1M-gaussian scene) and
-the orbit's throughput with 2 and 4 frames in flight.
+also the batched multi-view throughput (32 cameras of mixed sizes on a 1M-gaussian scene), the
+orbit's throughput with 2 and 4 frames in flight, and the learned colour on a 3M-gaussian scene
+(training time and error, fallback fraction at 2/255, ms/frame and PSNR against plain SH).
 """
 import argparse
 import ctypes
@@ -43,7 +44,11 @@ def parse():
     ap.add_argument("--elevation", type=float, default=15.0, help="orbit elevation, degrees")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-colour measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
+    ap.add_argument("--profile-learned", type=int, default=0,
+                    help="for rocprofv3 runs of the learned colour: render this many orbit frames of a 3M-gaussian "
+                         "scene plain, then with a colour model marking every gaussian, and exit")
     ap.add_argument("--profile-frames", type=int, default=0,
                     help="for rocprofv3 counter runs: render this many orbit frames of the configured mode, then "
                          "copy 256 MiB device to device twice (the counters' scale reference) and exit")
@@ -136,6 +141,79 @@ def multiview_bench(ctx, views=32, count=1_000_000, reps=5):
             "batched_ms_by_branches": {str(k): round(v, 3) for k, v in by.items()}, "graph": True}
 
 
+def learned_bench(ctx, cams, frames, count=3_000_000, samples=1 << 24):
+    """Learned colour: a 3M-gaussian scene of the same distributions (seed 0, means ~ N(0, 3^2 I),
+    SH degree 3) with a colour model trained on `samples` (gaussian, direction, SH RGB) samples
+    (512 Adam steps), then the 1080p orbit (global sort mode, full records) rendered plain and with
+    the model, after fallback passes at 2/255 and with every gaussian learned: ms/frame (wall
+    clock around `frames` synchronised frames), PSNR of frame 50 against the plain SH render, and
+    the colour bytes per gaussian."""
+    import numpy as np
+
+    import gsr
+    from gsr import scenes
+
+    sc = scenes.make_scene(count, seed=0, mean_sigma=3.0)
+    scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    del sc
+    W, H = int(cams[0].width), int(cams[0].height)
+    out = gsr.DeviceBuffer(ctx, W * H * 4)
+
+    def orbit(r):
+        for i in range(10):
+            r.render(cams[i], out.ptr)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for i in range(frames):
+            r.render(cams[(35 + i) % ORBIT_FRAMES], out.ptr)
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / frames
+
+    def psnr(a, b):
+        mse = float(np.mean((a[..., :3].astype(np.float64) / 255.0 - b[..., :3].astype(np.float64) / 255.0) ** 2))
+        return None if mse == 0 else round(10.0 * np.log10(1.0 / mse), 2)
+
+    plain = gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+    plain_ms = orbit(plain)
+    ref = plain.render_to_host(cams[50])
+    model = gsr.ColourModel(ctx, scene, seed=0)
+    step_samples = 1 << 15
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    last = model.train(samples // step_samples, step_samples, lr_weights=2e-3, lr_latents=2e-2)
+    train_ms = (time.perf_counter() - t0) * 1e3
+    mse = model.loss(0x5EED, 1 << 20)
+    r = gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+    r.set_colour_model(model)
+    forms = {}
+    for name, threshold in (("fallback_2_255", 2.0 / 255.0), ("all_learned", float("inf"))):
+        t0 = time.perf_counter()
+        n, err = model.fallback(threshold)
+        fallback_ms = (time.perf_counter() - t0) * 1e3
+        ms = orbit(r)
+        img = r.render_to_host(cams[50])
+        forms[name] = {"learned_fraction": round(n / count, 4), "ms_per_frame": round(ms, 4),
+                       "throughput_vs_plain": round(plain_ms / ms, 3), "psnr_db_vs_plain_sh": psnr(img, ref),
+                       "colour_bytes_per_gaussian": round(32.0 + 192.0 * (1.0 - n / count), 1),
+                       "fallback_pass_ms": round(fallback_ms, 2)}
+    res = {"gaussians": count, "training_samples": samples, "training_ms": round(train_ms, 1),
+           "train_last_mse": round(last, 6), "eval_mse": round(mse, 6),
+           "median_worst_error": round(float(np.median(err)), 4), "plain_ms_per_frame": round(plain_ms, 4),
+           "plain_colour_bytes_per_gaussian": 192, "shared_mlp_bytes": 4 * gsr.ColourModel.param_count(),
+           "forms": forms}
+    print(f"# learned colour ({count} gaussians): trained 2^{int(np.log2(samples))} samples in {train_ms:.0f} ms, "
+          f"eval MSE {mse:.5f}; plain {plain_ms:.3f} ms/frame; "
+          + "; ".join(f"{k}: {v['learned_fraction'] * 100:.1f}% learned, {v['ms_per_frame']:.3f} ms/frame, "
+                      f"PSNR vs plain {v['psnr_db_vs_plain_sh']}" for k, v in forms.items()),
+          file=sys.stderr, flush=True)
+    r.close()
+    plain.close()
+    model.close()
+    out.close()
+    scene.close()
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -183,6 +261,30 @@ def main():
             assert timer.hip.hipMemcpy(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), ctypes.c_size_t(nbytes), 3) == 0
         ctx.synchronize()
         print("# 2 x 256 MiB copies done", file=sys.stderr, flush=True)
+        return
+
+    if args.profile_learned > 0:
+        big = scenes.make_scene(3_000_000, seed=0, mean_sigma=3.0)
+        scene = gsr.Scene(ctx, big.pos_alpha, big.scales, big.rotations, big.sh, big.sh_degree)
+        del big
+        plain = gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+        for i in range(args.profile_learned):
+            plain.render(cams[(args.warmup + i) % ORBIT_FRAMES], out.ptr)
+        ctx.synchronize()
+        model = gsr.ColourModel(ctx, scene, seed=0)
+        model.train(4, 1 << 16)
+        marked, _ = model.fallback(float("inf"))
+        r = gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+        r.set_colour_model(model)
+        for i in range(args.profile_learned):
+            r.render(cams[(args.warmup + i) % ORBIT_FRAMES], out.ptr)
+        ctx.synchronize()
+        print(f"# {args.profile_learned} plain and {args.profile_learned} learned frames rendered "
+              f"({marked} gaussians learned)", file=sys.stderr, flush=True)
+        r.close()
+        plain.close()
+        model.close()
+        scene.close()
         return
 
     def timed_orbit(renderer, first, steps):
@@ -282,6 +384,7 @@ def main():
         sort_ms = stages["sort"]
 
     multiview = None if args.no_multiview or args.ply else multiview_bench(ctx)
+    learned = None if args.no_learned or args.ply else learned_bench(ctx, cams, min(args.steps, 100))
 
     # Frames in flight (the reference viewer keeps two, application.h:55): the orbit rendered by
     # k renderers on k streams (gsr_batch, frame i on branch i % k, each branch its own image),
@@ -367,6 +470,7 @@ def main():
         "cpu_baseline": cpu,
         "multiview": multiview,
         "frames_in_flight_ms_per_frame": in_flight,
+        "learned_colour": learned,
     }
     print(json.dumps(line), flush=True)
 
