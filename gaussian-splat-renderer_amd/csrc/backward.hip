@@ -1,0 +1,455 @@
+// Differentiable rendering (no counterpart in the reference, which renders forward only): a
+// training forward that keeps each pixel's final transmittance and last contributor, the render
+// backward (per tile, the list walked back to front, transmittance recovered by division, the
+// per-gaussian gradients reduced per wave and in LDS before one global atomic per list entry and
+// component), and the preprocess backward (per gaussian, the chain rule through the conic, the
+// EWA projection, the 3D covariance, the quaternion, the projection to pixels, the view direction
+// and the SH evaluation of preprocess.hip).
+//
+// The image is out = sum_i c_i a_i T_i + T_N bg (bg = 0), a_i = min(0.99, o_i exp(p_i)),
+// p_i = -0.5 (A dx^2 + C dy^2) - B dx dy with (dx, dy) = centre - pixel; a splat is skipped where
+// p > 0 or a < 1/255, and the walk ends before the splat whose T (1 - a) falls below 1e-4. The
+// gradient is that of this function where it is differentiable (a clamped alpha passes none).
+#include "gsr_device.hpp"
+
+namespace gsr {
+
+namespace {
+
+constexpr int kBatch = 256;  // list entries staged in LDS per round (one per thread)
+// Per-gaussian screen-space gradient planes (grad2D + v * N): centre x, y; conic A, B, C;
+// opacity; colour r, g, b.
+constexpr int kGrad2D = 9;
+
+// Training forward: one workgroup per 16x16 tile, one pixel per thread, exact alpha in the
+// reference's operation order. out = (C, 1 - T); state = (T bits, entries of the tile's range up
+// to and including the last contributor).
+__global__ __launch_bounds__(256) void kRenderTrain(const float4* __restrict__ splats, const uint32_t* __restrict__ values,
+                                                    const uint2* __restrict__ ranges, DevCamera cam,
+                                                    float4* __restrict__ out, uint2* __restrict__ state) {
+    __shared__ float4 sA[kBatch];  // centre x, y, conic A, B
+    __shared__ float4 sB[kBatch];  // conic C, opacity, r, g
+    __shared__ float sC[kBatch];   // b
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+    const uint32_t px = tx * kTileSize + threadIdx.x % kTileSize, py = ty * kTileSize + threadIdx.x / kTileSize;
+    const bool inside = px < cam.width && py < cam.height;
+    const float pxf = float(px), pyf = float(py);
+    const uint2 range = ranges[tile];
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+    uint32_t last = 0;
+    bool done = !inside;
+    for (uint32_t base = range.x; base < range.y; base += kBatch) {
+        if (__syncthreads_count(!done) == 0) break;
+        const uint32_t j = base + threadIdx.x;
+        if (j < range.y) {
+            const uint32_t g = values[j];
+            const float4 s0 = splats[3 * size_t(g)], s1 = splats[3 * size_t(g) + 1], s2 = splats[3 * size_t(g) + 2];
+            sA[threadIdx.x] = make_float4(s0.x, s0.y, s1.x, s1.y);
+            sB[threadIdx.x] = make_float4(s1.z, s1.w, s2.x, s2.y);
+            sC[threadIdx.x] = s2.z;
+        }
+        __syncthreads();
+        const uint32_t n = range.y - base < uint32_t(kBatch) ? range.y - base : uint32_t(kBatch);
+        for (uint32_t k = 0; k < n && !done; ++k) {
+            const float4 a = sA[k], b = sB[k];
+            const float dx = a.x - pxf, dy = a.y - pyf;
+            const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+            if (power > 0.0f) continue;
+            const float e = b.y * expf(power);
+            const float alpha = kMaxAlpha < e ? kMaxAlpha : e;
+            if (alpha < kMinAlpha) continue;
+            const float testT = T * (1.0f - alpha);
+            if (testT < kMinTransmittance) {
+                done = true;
+                break;
+            }
+            C0 += b.z * alpha * T;
+            C1 += b.w * alpha * T;
+            C2 += sC[k] * alpha * T;
+            T = testT;
+            last = base + k - range.x + 1;
+        }
+    }
+    if (inside) {
+        const size_t p = size_t(py) * cam.width + px;
+        out[p] = make_float4(C0, C1, C2, 1.0f - T);
+        state[p] = make_uint2(__float_as_uint(T), last);
+    }
+}
+
+__device__ __forceinline__ float waveSum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Render backward: one workgroup per tile. Each pixel walks its entries from its last contributor
+// back to the first, recovering T_i = T_{i+1} / (1 - a_i) and the colour behind,
+// B_{i-1} = c_i a_i + (1 - a_i) B_i (B = bg = 0 behind the last); dout/da_i = T_i (c_i - B_i),
+// dout/dc_i = a_i T_i. The 9 gradient values of an entry are summed over the wave's 64 pixels
+// (shuffles), the waves' sums over the tile in LDS, and each entry adds its tile's sums to the
+// gaussian's planes with one atomic per component.
+__global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict__ splats,
+                                                       const uint32_t* __restrict__ values,
+                                                       const uint2* __restrict__ ranges, DevCamera cam,
+                                                       const float4* __restrict__ dOut, const uint2* __restrict__ state,
+                                                       float* __restrict__ grad2D, uint32_t count) {
+    __shared__ float4 sA[kBatch];
+    __shared__ float4 sB[kBatch];
+    __shared__ float sC[kBatch];
+    __shared__ float sG[kGrad2D][kBatch];
+    __shared__ uint32_t sMax[4];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+    const uint32_t px = tx * kTileSize + threadIdx.x % kTileSize, py = ty * kTileSize + threadIdx.x / kTileSize;
+    const bool inside = px < cam.width && py < cam.height;
+    const float pxf = float(px), pyf = float(py);
+    const uint2 range = ranges[tile];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    float T = 1.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    uint32_t last = 0;
+    if (inside) {
+        const size_t p = size_t(py) * cam.width + px;
+        const uint2 st = state[p];
+        T = __uint_as_float(st.x);
+        last = st.y;
+        const float4 d = dOut[p];
+        g0 = d.x;
+        g1 = d.y;
+        g2 = d.z;
+    }
+    float B0 = 0.0f, B1 = 0.0f, B2 = 0.0f;
+    // The tile's largest last-contributor count bounds the walk.
+    uint32_t m = last;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), off, 64)));
+    if (lane == 0) sMax[wave] = m;
+    __syncthreads();
+    const uint32_t maxLast = max(max(sMax[0], sMax[1]), max(sMax[2], sMax[3]));
+    for (uint32_t end = maxLast; end > 0;) {
+        const uint32_t begin = end > uint32_t(kBatch) ? end - uint32_t(kBatch) : 0u;
+        const uint32_t n = end - begin;
+        __syncthreads();  // the previous round's LDS reads are done
+        if (threadIdx.x < n) {
+            const uint32_t g = values[range.x + begin + threadIdx.x];
+            const float4 s0 = splats[3 * size_t(g)], s1 = splats[3 * size_t(g) + 1], s2 = splats[3 * size_t(g) + 2];
+            sA[threadIdx.x] = make_float4(s0.x, s0.y, s1.x, s1.y);
+            sB[threadIdx.x] = make_float4(s1.z, s1.w, s2.x, s2.y);
+            sC[threadIdx.x] = s2.z;
+        }
+#pragma unroll
+        for (int v = 0; v < kGrad2D; ++v) sG[v][threadIdx.x] = 0.0f;
+        __syncthreads();
+        for (int k = int(n) - 1; k >= 0; --k) {
+            const uint32_t j = begin + uint32_t(k);
+            const float4 a = sA[k], b = sB[k];
+            const float c2 = sC[k];
+            float v[kGrad2D] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            bool act = j < last;
+            if (act) {
+                const float dx = a.x - pxf, dy = a.y - pyf;
+                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
+                const float G = power > 0.0f ? 0.0f : expf(power);
+                const float e = b.y * G;
+                const float alpha = kMaxAlpha < e ? kMaxAlpha : e;
+                act = power <= 0.0f && alpha >= kMinAlpha;
+                if (act) {
+                    T = T / (1.0f - alpha);  // T before this splat
+                    const float dAlpha = T * ((b.z - B0) * g0 + (b.w - B1) * g1 + (c2 - B2) * g2);
+                    v[6] = alpha * T * g0;
+                    v[7] = alpha * T * g1;
+                    v[8] = alpha * T * g2;
+                    B0 = b.z * alpha + (1.0f - alpha) * B0;
+                    B1 = b.w * alpha + (1.0f - alpha) * B1;
+                    B2 = c2 * alpha + (1.0f - alpha) * B2;
+                    if (e < kMaxAlpha) {  // a = o G: d/do = G, d/dp = a
+                        const float gp = dAlpha * alpha;
+                        v[0] = -gp * (a.z * dx + a.w * dy);
+                        v[1] = -gp * (a.w * dx + b.x * dy);
+                        v[2] = -0.5f * gp * dx * dx;
+                        v[3] = -gp * dx * dy;
+                        v[4] = -0.5f * gp * dy * dy;
+                        v[5] = dAlpha * G;
+                    }
+                }
+            }
+            if (__any(act)) {  // wave-uniform
+#pragma unroll
+                for (int q = 0; q < kGrad2D; ++q) {
+                    const float s = waveSum(v[q]);
+                    if (lane == 0) atomicAdd(&sG[q][k], s);
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < n) {
+            const uint32_t g = values[range.x + begin + threadIdx.x];
+#pragma unroll
+            for (int q = 0; q < kGrad2D; ++q) {
+                const float s = sG[q][threadIdx.x];
+                if (s != 0.0f) atomicAdd(&grad2D[size_t(q) * count + g], s);
+            }
+        }
+        end = begin;
+    }
+}
+
+// d(raw SH colour)/d(direction) for one channel's coefficients c[0..15], degree <= 3; also the
+// basis values Y[k] (the derivative of the colour by coefficient k).
+__device__ __forceinline__ void shBackward(int degree, const float* c, float x, float y, float z, float Y[16],
+                                           float grad[3]) {
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) Y[k] = 0.0f;
+    Y[0] = kC0;
+    if (degree > 0) {
+        Y[1] = -kC1 * y;
+        Y[2] = kC1 * z;
+        Y[3] = -kC1 * x;
+        gy += -kC1 * c[1];
+        gz += kC1 * c[2];
+        gx += -kC1 * c[3];
+        if (degree > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            Y[4] = kC2_0 * xy;
+            Y[5] = kC2_1 * yz;
+            Y[6] = kC2_2 * (2.0f * zz - xx - yy);
+            Y[7] = kC2_3 * xz;
+            Y[8] = kC2_4 * (xx - yy);
+            gx += kC2_0 * y * c[4] - 2.0f * kC2_2 * x * c[6] + kC2_3 * z * c[7] + 2.0f * kC2_4 * x * c[8];
+            gy += kC2_0 * x * c[4] + kC2_1 * z * c[5] - 2.0f * kC2_2 * y * c[6] - 2.0f * kC2_4 * y * c[8];
+            gz += kC2_1 * y * c[5] + 4.0f * kC2_2 * z * c[6] + kC2_3 * x * c[7];
+            if (degree > 2) {
+                Y[9] = kC3_0 * y * (3.0f * xx - yy);
+                Y[10] = kC3_1 * xy * z;
+                Y[11] = kC3_2 * y * (4.0f * zz - xx - yy);
+                Y[12] = kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+                Y[13] = kC3_4 * x * (4.0f * zz - xx - yy);
+                Y[14] = kC3_5 * z * (xx - yy);
+                Y[15] = kC3_6 * x * (xx - 3.0f * yy);
+                gx += kC3_0 * 6.0f * xy * c[9] + kC3_1 * yz * c[10] - kC3_2 * 2.0f * xy * c[11] -
+                      kC3_3 * 6.0f * xz * c[12] + kC3_4 * (4.0f * zz - 3.0f * xx - yy) * c[13] +
+                      kC3_5 * 2.0f * xz * c[14] + kC3_6 * (3.0f * xx - 3.0f * yy) * c[15];
+                gy += kC3_0 * (3.0f * xx - 3.0f * yy) * c[9] + kC3_1 * xz * c[10] +
+                      kC3_2 * (4.0f * zz - xx - 3.0f * yy) * c[11] - kC3_3 * 6.0f * yz * c[12] -
+                      kC3_4 * 2.0f * xy * c[13] - kC3_5 * 2.0f * yz * c[14] - kC3_6 * 6.0f * xy * c[15];
+                gz += kC3_1 * xy * c[10] + kC3_2 * 8.0f * yz * c[11] + kC3_3 * (6.0f * zz - 3.0f * xx - 3.0f * yy) * c[12] +
+                      kC3_4 * 8.0f * xz * c[13] + kC3_5 * (xx - yy) * c[14];
+            }
+        }
+    }
+    grad[0] = gx;
+    grad[1] = gy;
+    grad[2] = gz;
+}
+
+// Preprocess backward, one thread per gaussian (f32 scenes). From the screen-space gradient
+// planes to the gradients of the mean, log-scales (s = exp), quaternion (as stored, used
+// unnormalised), opacity logit (a = sigmoid) and SH coefficients (K x 3, coefficient-major RGB).
+// Culled gaussians get zeros.
+__global__ __launch_bounds__(256) void kPreprocessBackward(SceneRef scene, DevCamera cam,
+                                                           const float4* __restrict__ splats,
+                                                           const float* __restrict__ grad2D, float* __restrict__ dMean,
+                                                           float* __restrict__ dLogScale, float* __restrict__ dQuat,
+                                                           float* __restrict__ dLogit, float* __restrict__ dSh) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= scene.count) return;
+    const size_t N = scene.count;
+    const bool visible = __float_as_uint(splats[3 * size_t(i)].w) == GSR_CULL_VISIBLE;
+    float gM[3] = {0.0f, 0.0f, 0.0f}, gS[3] = {0.0f, 0.0f, 0.0f}, gQ[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gL = 0.0f;
+    const int K = int(scene.K);
+    if (!visible) {
+        if (dMean)
+            for (int c = 0; c < 3; ++c) dMean[3 * i + c] = 0.0f;
+        if (dLogScale)
+            for (int c = 0; c < 3; ++c) dLogScale[3 * i + c] = 0.0f;
+        if (dQuat)
+            for (int c = 0; c < 4; ++c) dQuat[4 * i + c] = 0.0f;
+        if (dLogit) dLogit[i] = 0.0f;
+        if (dSh)
+            for (int k = 0; k < 3 * K; ++k) dSh[size_t(i) * 3 * K + k] = 0.0f;
+        return;
+    }
+    float g2[kGrad2D];
+#pragma unroll
+    for (int q = 0; q < kGrad2D; ++q) g2[q] = grad2D[q * N + i];
+    const float4 pa = scene.posAlpha[i], sc = scene.scales[i], qv = scene.rotations[i];
+    const float mean[3] = {pa.x, pa.y, pa.z};
+    const float* v = cam.view;
+    float t[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) t[r] = v[r] * mean[0] + v[4 + r] * mean[1] + v[8 + r] * mean[2] + v[12 + r];
+    // Opacity.
+    gL = g2[5] * pa.w * (1.0f - pa.w);
+    // Forward of the covariance (as preprocess.hip).
+    const float s[3] = {sc.x, sc.y, sc.z};
+    const float qw = qv.x, qx = qv.y, qy = qv.z, qz = qv.w;
+    float R[3][3];
+    R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz);
+    R[0][1] = 2.0f * (qx * qy + qw * qz);
+    R[0][2] = 2.0f * (qx * qz - qw * qy);
+    R[1][0] = 2.0f * (qx * qy - qw * qz);
+    R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz);
+    R[1][2] = 2.0f * (qy * qz + qw * qx);
+    R[2][0] = 2.0f * (qx * qz + qw * qy);
+    R[2][1] = 2.0f * (qy * qz - qw * qx);
+    R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+    float M[3][3], S[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) M[c][r] = R[c][r] * s[c];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) S[j][k] = M[0][k] * M[0][j] + M[1][k] * M[1][j] + M[2][k] * M[2][j];
+    const float limX = kFrustumClampFactor * cam.tanX, limY = kFrustumClampFactor * cam.tanY;
+    const float txtz = t[0] / t[2], tytz = t[1] / t[2];
+    const bool clampX = txtz < -limX || txtz > limX, clampY = tytz < -limY || tytz > limY;
+    const float tx = (clampX ? (txtz < 0.0f ? -limX : limX) : txtz) * t[2];
+    const float ty = (clampY ? (tytz < 0.0f ? -limY : limY) : tytz) * t[2];
+    const float tz = t[2];
+    const float j00 = cam.focalX / tz, j20 = -(cam.focalX * tx) / (tz * tz);
+    const float j11 = cam.focalY / tz, j21 = -(cam.focalY * ty) / (tz * tz);
+    float T[3][2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T[j][0] = j00 * v[j * 4 + 0] + j20 * v[j * 4 + 2];
+        T[j][1] = j11 * v[j * 4 + 1] + j21 * v[j * 4 + 2];
+    }
+    float ST[3][2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) ST[j][k] = S[j][0] * T[0][k] + S[j][1] * T[1][k] + S[j][2] * T[2][k];
+    const float c00 = ST[0][0] * T[0][0] + ST[1][0] * T[1][0] + ST[2][0] * T[2][0];
+    const float c01 = ST[0][1] * T[0][0] + ST[1][1] * T[1][0] + ST[2][1] * T[2][0];
+    const float c11 = ST[0][1] * T[0][1] + ST[1][1] * T[1][1] + ST[2][1] * T[2][1];
+    const float ca = c00 + kLowPassFilter, cb = c01, cc = c11 + kLowPassFilter;
+    const float detInv = 1.0f / (ca * cc - cb * cb);
+    const float ka = cc * detInv, kb = -cb * detInv, kc = ca * detInv;
+    // Conic: K = C'^-1, dL/dC' = -K G K with G = [[gA, gB / 2], [gB / 2, gC]].
+    const float G00 = g2[2], G01 = 0.5f * g2[3], G11 = g2[4];
+    const float KG00 = ka * G00 + kb * G01, KG01 = ka * G01 + kb * G11;
+    const float KG10 = kb * G00 + kc * G01, KG11 = kb * G01 + kc * G11;
+    const float H00 = -(KG00 * ka + KG01 * kb), H01 = -(KG00 * kb + KG01 * kc), H11 = -(KG10 * kb + KG11 * kc);
+    // cov2D = T^T S T: dL/dS = T H T^T, dL/dT = 2 S T H.
+    float gSig[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            gSig[a][b] = T[a][0] * (H00 * T[b][0] + H01 * T[b][1]) + T[a][1] * (H01 * T[b][0] + H11 * T[b][1]);
+    float gT[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        gT[a][0] = 2.0f * (ST[a][0] * H00 + ST[a][1] * H01);
+        gT[a][1] = 2.0f * (ST[a][0] * H01 + ST[a][1] * H11);
+    }
+    // T = J W.
+    float gj00 = 0.0f, gj20 = 0.0f, gj11 = 0.0f, gj21 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        gj00 += gT[j][0] * v[j * 4 + 0];
+        gj20 += gT[j][0] * v[j * 4 + 2];
+        gj11 += gT[j][1] * v[j * 4 + 1];
+        gj21 += gT[j][1] * v[j * 4 + 2];
+    }
+    float gt[3] = {0.0f, 0.0f, 0.0f};
+    const float tz2 = tz * tz, tz3 = tz2 * tz;
+    gt[2] += gj00 * (-cam.focalX / tz2) + gj20 * (2.0f * cam.focalX * tx / tz3) + gj11 * (-cam.focalY / tz2) +
+             gj21 * (2.0f * cam.focalY * ty / tz3);
+    const float gtx = gj20 * (-cam.focalX / tz2), gty = gj21 * (-cam.focalY / tz2);
+    if (clampX) gt[2] += gtx * (txtz < 0.0f ? -limX : limX);
+    else gt[0] += gtx;
+    if (clampY) gt[2] += gty * (tytz < 0.0f ? -limY : limY);
+    else gt[1] += gty;
+    // Centre: ctr = ((t0 / (t2 tan) + 1) W - 1) / 2.
+    const float hx = 0.5f * float(cam.width) / cam.tanX, hy = 0.5f * float(cam.height) / cam.tanY;
+    gt[0] += g2[0] * hx / tz;
+    gt[1] += g2[1] * hy / tz;
+    gt[2] += -(g2[0] * hx * t[0] + g2[1] * hy * t[1]) / tz2;
+    // Sigma = M^T M: dL/dM = 2 M gSig; M = diag(s) R.
+    float gR[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float gs = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const float gMcr = 2.0f * (M[c][0] * gSig[0][r] + M[c][1] * gSig[1][r] + M[c][2] * gSig[2][r]);
+            gs += gMcr * R[c][r];
+            gR[c][r] = gMcr * s[c];
+        }
+        gS[c] = gs * s[c];  // log-scale
+    }
+    gQ[0] = 2.0f * (gR[0][1] * qz - gR[0][2] * qy - gR[1][0] * qz + gR[1][2] * qx + gR[2][0] * qy - gR[2][1] * qx);
+    gQ[1] = 2.0f * (gR[0][1] * qy + gR[0][2] * qz + gR[1][0] * qy - 2.0f * gR[1][1] * qx + gR[1][2] * qw +
+                    gR[2][0] * qz - gR[2][1] * qw - 2.0f * gR[2][2] * qx);
+    gQ[2] = 2.0f * (-2.0f * gR[0][0] * qy + gR[0][1] * qx - gR[0][2] * qw + gR[1][0] * qx + gR[1][2] * qz +
+                    gR[2][0] * qw + gR[2][1] * qz - 2.0f * gR[2][2] * qy);
+    gQ[3] = 2.0f * (-2.0f * gR[0][0] * qz + gR[0][1] * qw + gR[0][2] * qx - gR[1][0] * qw - 2.0f * gR[1][1] * qz +
+                    gR[1][2] * qy + gR[2][0] * qx + gR[2][1] * qy);
+    // Colour: raw = evalSH(dir) + 0.5, colour = max(raw, 0); dir = d / |d|, d = mean - camPos.
+    const int degree = scene.degree;
+    const int used = (degree + 1) * (degree + 1);
+    float co[3][16];
+    loadSh<GSR_SCENE_F32_SOA>(scene, i, used, co);
+    const float dx = mean[0] - cam.pos[0], dy = mean[1] - cam.pos[1], dz = mean[2] - cam.pos[2];
+    const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    const float dir[3] = {dx * inv, dy * inv, dz * inv};
+    float gdir[3] = {0.0f, 0.0f, 0.0f};
+    for (int ch = 0; ch < 3; ++ch) {
+        const float raw = evalShRaw(degree, co[ch], dir[0], dir[1], dir[2]);
+        const float gc = raw < 0.0f ? 0.0f : g2[6 + ch];
+        float Y[16], gd[3];
+        shBackward(degree, co[ch], dir[0], dir[1], dir[2], Y, gd);
+        if (dSh)
+            for (int k = 0; k < K; ++k) dSh[(size_t(i) * K + k) * 3 + ch] = k < used ? gc * Y[k] : 0.0f;
+        gdir[0] += gc * gd[0];
+        gdir[1] += gc * gd[1];
+        gdir[2] += gc * gd[2];
+    }
+    const float dd = dir[0] * gdir[0] + dir[1] * gdir[1] + dir[2] * gdir[2];
+    // Mean: through t (view rotation) and the direction.
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        gM[c] = v[c * 4 + 0] * gt[0] + v[c * 4 + 1] * gt[1] + v[c * 4 + 2] * gt[2] + (gdir[c] - dir[c] * dd) * inv;
+    if (dMean)
+        for (int c = 0; c < 3; ++c) dMean[3 * i + c] = gM[c];
+    if (dLogScale)
+        for (int c = 0; c < 3; ++c) dLogScale[3 * i + c] = gS[c];
+    if (dQuat)
+        for (int c = 0; c < 4; ++c) dQuat[4 * i + c] = gQ[c];
+    if (dLogit) dLogit[i] = gL;
+}
+
+}  // namespace
+
+int grad2DComponents() { return kGrad2D; }
+
+void launchRenderTrain(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+                       float4* out, uint2* state, hipStream_t stream) {
+    const uint32_t tiles = cam.tilesX * cam.tilesY;
+    if (tiles == 0) return;
+    hipLaunchKernelGGL(kRenderTrain, dim3(tiles), dim3(256), 0, stream, splats, values, ranges, cam, out, state);
+}
+
+void launchRenderBackward(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+                          const float4* dOut, const uint2* state, float* grad2D, uint32_t count, hipStream_t stream) {
+    const uint32_t tiles = cam.tilesX * cam.tilesY;
+    if (tiles == 0) return;
+    hipLaunchKernelGGL(kRenderBackward, dim3(tiles), dim3(256), 0, stream, splats, values, ranges, cam, dOut, state,
+                       grad2D, count);
+}
+
+void launchPreprocessBackward(const SceneRef& scene, const DevCamera& cam, const float4* splats, const float* grad2D,
+                              float* dMean, float* dLogScale, float* dQuat, float* dLogit, float* dSh,
+                              hipStream_t stream) {
+    const uint32_t blocks = (scene.count + 255) / 256;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(kPreprocessBackward, dim3(blocks), dim3(256), 0, stream, scene, cam, splats, grad2D, dMean,
+                       dLogScale, dQuat, dLogit, dSh);
+}
+
+}  // namespace gsr

@@ -161,6 +161,10 @@ struct gsr_renderer {
     // Per gaussian (+ the packed render splats in the p32 / p24 modes).
     DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control, packed;
     DeviceBuffer rayDepth;  // StopThePop: 3 float4 per gaussian
+    // Differentiable renderers: per pixel (T bits, last contributor) of the last frame, sized for
+    // the largest camera seen; per gaussian the screen-space gradient planes of gsr_backward.
+    DeviceBuffer pixelState, grad2D;
+    size_t pixelStatePixels = 0;
     // The list (ping-pong; keys are 32-bit tile ids in binning mode, 64-bit (tile, depth) in
     // global mode) and the sort's work areas.
     DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus;
@@ -637,6 +641,12 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown render_loop %d", int(s.render_loop));
     if (s.render_pixels_per_thread != 1 && s.render_pixels_per_thread != 2 && s.render_pixels_per_thread != 4)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: render_pixels_per_thread must be 1, 2 or 4");
+    if (s.differentiable &&
+        (s.format != GSR_TARGET_RGBA32F || s.render_splat != GSR_RENDER_SPLAT_FULL || s.sort_window != 0 ||
+         scene->format != GSR_SCENE_F32_SOA))
+        return fail(GSR_ERROR_UNSUPPORTED,
+                    "gsr_renderer_create: differentiable needs format RGBA32F, render_splat FULL, sort_window 0 "
+                    "and an f32 scene");
     GSR_HIP(hipSetDevice(ctx->device));
     gsr_renderer* r = new (std::nothrow) gsr_renderer();
     if (!r) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_renderer_create: host allocation");
@@ -662,6 +672,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     alloc(r->splats, n * 48);
     if (s.render_splat != GSR_RENDER_SPLAT_FULL) alloc(r->packed, n * (s.render_splat == GSR_RENDER_SPLAT_P32 ? 32 : 24));
     if (s.sort_window > 0) alloc(r->rayDepth, n * 48);
+    if (s.differentiable) alloc(r->grad2D, n * 4 * size_t(grad2DComponents()));
     alloc(r->tileCounts, n * 4);
     alloc(r->blockOffsets, gBlocks * 4);
     alloc(r->control, sizeof(gsr_control));
@@ -724,7 +735,8 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     const DeviceBuffer* bufs[] = {&r->splats, &r->tileCounts, &r->blockOffsets, &r->blockStats, &r->control,
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
-                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth};
+                                  &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth,
+                                  &r->pixelState, &r->grad2D};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     for (const auto& o : r->tileOrders) s += o->table.bytes;
@@ -747,6 +759,12 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         r->rangesTiles = 0;  // no buffer until the allocation succeeds (a failed one is retried)
         GSR_HIP(r->ranges.allocate(size_t(tiles) * 8));
         r->rangesTiles = tiles;
+    }
+    if (r->settings.differentiable && size_t(cam->width) * cam->height > r->pixelStatePixels) {
+        GSR_HIP(hipStreamSynchronize(stream));
+        r->pixelStatePixels = 0;
+        GSR_HIP(r->pixelState.allocate(size_t(cam->width) * cam->height * 8));
+        r->pixelStatePixels = size_t(cam->width) * cam->height;
     }
     const uint32_t* tileOrder = nullptr;
     if (!r->binning && !r->settings.full_key_sort) {
@@ -890,10 +908,15 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
-    launchRender(r->splats.as<float4>(), r->packed.as<uint32_t>(), r->settings.render_splat, sortedValues,
-                 r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
-                 r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0, r->settings.render_loop,
-                 int(r->settings.sort_window), r->rayDepth.as<float4>(), outImage, stream);
+    if (r->settings.differentiable)
+        launchRenderTrain(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc,
+                          static_cast<float4*>(outImage), r->pixelState.as<uint2>(), stream);
+    else
+        launchRender(r->splats.as<float4>(), r->packed.as<uint32_t>(), r->settings.render_splat, sortedValues,
+                     r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
+                     r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0,
+                     r->settings.render_loop, int(r->settings.sort_window), r->rayDepth.as<float4>(), outImage,
+                     stream);
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(7));
     if (r->statHost) {  // this frame's control block, read back without a stall
@@ -1498,7 +1521,51 @@ gsr_status gsr_renderer_set_colour_model(gsr_renderer* r, gsr_colour_model* m) {
     if (!r) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_set_colour_model: null renderer");
     if (m && m->scene != r->scene)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_set_colour_model: the model belongs to another scene");
+    if (m && r->settings.differentiable)
+        return fail(GSR_ERROR_UNSUPPORTED, "gsr_renderer_set_colour_model: a differentiable renderer colours from SH");
     r->colour = m;
+    return GSR_OK;
+}
+
+gsr_status gsr_backward(gsr_renderer* r, const void* gradImage, const gsr_gradients* out, void* streamArg) {
+    if (!r || !gradImage || !out) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_backward: null argument");
+    if (!r->settings.differentiable)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_backward: the renderer was not created differentiable");
+    if (!r->haveFrame) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_backward: no frame rendered yet");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : r->ctx->stream;
+    const gsr_scene* sc = r->scene;
+    const gsr_camera& cam = r->lastCam;
+    DevCamera dc{};
+    std::memcpy(dc.view, cam.view_inria, sizeof(dc.view));
+    std::memcpy(dc.pos, cam.position, sizeof(dc.pos));
+    dc.tanX = cam.tan_half_fov_x;
+    dc.tanY = cam.tan_half_fov_y;
+    dc.focalX = cam.focal_x;
+    dc.focalY = cam.focal_y;
+    dc.width = cam.width;
+    dc.height = cam.height;
+    dc.tilesX = cam.tiles_x;
+    dc.tilesY = cam.tiles_y;
+    SceneRef ref = sc->ref;
+    ref.degree = r->degree;
+    // The frame's stream may differ from this one: order after it.
+    if (r->lastStream && r->lastStream != stream) {
+        hipEvent_t ev;
+        GSR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        hipError_t e = hipEventRecord(ev, r->lastStream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev, 0);
+        (void)hipEventDestroy(ev);
+        GSR_HIP(e);
+    }
+    GSR_HIP(hipMemsetAsync(r->grad2D.ptr, 0, size_t(sc->count) * 4 * size_t(grad2DComponents()), stream));
+    const uint32_t* values = r->sortedIn ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
+    launchRenderBackward(r->splats.as<float4>(), values, r->ranges.as<uint2>(), dc, static_cast<const float4*>(gradImage),
+                         r->pixelState.as<uint2>(), r->grad2D.as<float>(), sc->count, stream);
+    GSR_HIP(hipGetLastError());
+    launchPreprocessBackward(ref, dc, r->splats.as<float4>(), r->grad2D.as<float>(), out->means, out->log_scales,
+                             out->quats, out->opacity_logits, out->sh, stream);
+    GSR_HIP(hipGetLastError());
     return GSR_OK;
 }
 

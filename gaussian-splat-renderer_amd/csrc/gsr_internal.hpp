@@ -168,6 +168,21 @@ void launchColourMlp(const SceneRef& scene, const DevCamera& cam, const float* p
 void launchMlpEvaluate(const float* params, const uint2* packs, const float* latents, const uint32_t* gaussians,
                        const float* dirs, uint32_t n, float* rgb, hipStream_t stream);
 
+// Differentiable rendering (backward.hip). The training forward (full records, RGBA32F out, exact
+// alpha) keeps per pixel (T bits, entries up to the last contributor); the render backward adds
+// each gaussian's screen-space gradient into grad2D (grad2DComponents() planes of N floats:
+// centre x, y, conic A, B, C, opacity, colour r, g, b; cleared by the caller); the preprocess
+// backward turns them into the gradients of the mean, log-scales, quaternion, opacity logit and
+// SH coefficients (f32 scenes; each output may be null).
+int grad2DComponents();
+void launchRenderTrain(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+                       float4* out, uint2* state, hipStream_t stream);
+void launchRenderBackward(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
+                          const float4* dOut, const uint2* state, float* grad2D, uint32_t count, hipStream_t stream);
+void launchPreprocessBackward(const SceneRef& scene, const DevCamera& cam, const float4* splats, const float* grad2D,
+                              float* dMean, float* dLogScale, float* dQuat, float* dLogit, float* dSh,
+                              hipStream_t stream);
+
 // Stand-alone scan (GpuScan): reduce, block-sum scan, downsweep.
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;

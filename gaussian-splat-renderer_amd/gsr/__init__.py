@@ -106,7 +106,8 @@ class Settings(ctypes.Structure):
         ("adaptive_list", ctypes.c_int32),
         ("adaptive_initial_entries", ctypes.c_uint32),
         ("frame_stats", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("differentiable", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -243,6 +244,15 @@ _cm_evaluate = _sig("gsr_colour_model_evaluate", _i32, [_vp, _vp, _vp, _u32, _vp
 _cm_fallback = _sig("gsr_colour_model_fallback", _i32, [_vp, _f32, _vp, _vp])
 _renderer_set_colour_model = _sig("gsr_renderer_set_colour_model", _i32, [_vp, _vp])
 
+
+class Gradients(ctypes.Structure):
+    """gsr_gradients: device pointers (float32), each may be null."""
+
+    _fields_ = [("means", _vp), ("log_scales", _vp), ("quats", _vp), ("opacity_logits", _vp), ("sh", _vp)]
+
+
+_backward = _sig("gsr_backward", _i32, [_vp, _vp, ctypes.POINTER(Gradients), _vp])
+
 # Every symbol include/gsr.h declares (checked by the CPU tests).
 EXPORTED = (
     "gsr_last_error gsr_version gsr_context_create gsr_context_synchronize gsr_context_destroy "
@@ -255,7 +265,7 @@ EXPORTED = (
     "gsr_batch_create gsr_batch_destroy gsr_batch_branches gsr_batch_render gsr_batch_capture gsr_batch_replay "
     "gsr_colour_model_param_count gsr_colour_model_create gsr_colour_model_destroy gsr_colour_model_train "
     "gsr_colour_model_loss gsr_colour_model_gradients gsr_colour_model_get gsr_colour_model_set "
-    "gsr_colour_model_evaluate gsr_colour_model_fallback gsr_renderer_set_colour_model "
+    "gsr_colour_model_evaluate gsr_colour_model_fallback gsr_renderer_set_colour_model gsr_backward "
     "gsr_ply_load gsr_ply_load_memory gsr_ply_free"
 ).split()
 
@@ -495,7 +505,7 @@ class Renderer:
                  exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
                  sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full",
                  sort_window: int = 0, adaptive_list: bool = False, adaptive_initial_entries: int = 0,
-                 frame_stats: bool = False):
+                 frame_stats: bool = False, differentiable: bool = False):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -513,6 +523,7 @@ class Renderer:
         s.adaptive_list = 1 if adaptive_list else 0
         s.adaptive_initial_entries = adaptive_initial_entries
         s.frame_stats = 1 if frame_stats else 0
+        s.differentiable = 1 if differentiable else 0
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
@@ -549,6 +560,38 @@ class Renderer:
 
     def image_bytes(self, cam: Camera) -> int:
         return int(cam.width) * int(cam.height) * (4 if self.target == "rgba8" else 16)
+
+    def backward(self, grad_image_ptr: int, means: Optional[int] = None, log_scales: Optional[int] = None,
+                 quats: Optional[int] = None, opacity_logits: Optional[int] = None, sh: Optional[int] = None,
+                 stream: Optional[int] = None) -> None:
+        """gsr_backward on device pointers: dL/d(image) of the last frame (RGBA32F) -> the gradients
+        of the scene parameters (asynchronous)."""
+        g = Gradients(means, log_scales, quats, opacity_logits, sh)
+        _check(_backward(self.handle, grad_image_ptr, ctypes.byref(g), stream), "gsr_backward")
+
+    def backward_to_host(self, grad_image: np.ndarray):
+        """gsr_backward with host arrays: grad_image (H, W, 3 or 4) float32 -> dict of numpy
+        gradients (means, log_scales, quats, opacity_logits, sh)."""
+        cam = self._cam
+        if cam is None:
+            raise GsrError("Renderer.backward_to_host: no frame rendered yet")
+        H, W = int(cam.height), int(cam.width)
+        gi = np.zeros((H, W, 4), np.float32)
+        gi[..., :3] = np.asarray(grad_image, np.float32).reshape(H, W, -1)[..., :3]
+        n = self.scene.count
+        K = (self.scene.sh_degree + 1) ** 2
+        shapes = {"means": (n, 3), "log_scales": (n, 3), "quats": (n, 4), "opacity_logits": (n,), "sh": (n, K * 3)}
+        bufs = {k: DeviceBuffer(self.ctx, max(4 * int(np.prod(v)), 4)) for k, v in shapes.items()}
+        gbuf = DeviceBuffer(self.ctx, gi.nbytes)
+        try:
+            gbuf.write(gi)
+            self.backward(gbuf.ptr, **{k: b.ptr for k, b in bufs.items()})
+            self.ctx.synchronize()
+            return {k: bufs[k].read(np.empty(shapes[k], np.float32)) for k in shapes}
+        finally:
+            gbuf.close()
+            for b in bufs.values():
+                b.close()
 
     def set_colour_model(self, model: Optional["ColourModel"]) -> None:
         """Learned colour for the gaussians the model's last fallback pass marked (None: SH only)."""

@@ -148,7 +148,11 @@ typedef struct gsr_settings {
     uint32_t adaptive_initial_entries; /* first guess (0 = max(4 N, 2^20)) */
     int32_t frame_stats;         /* 1: read every frame's control block back asynchronously
                                     (gsr_renderer_last_stats); implied by adaptive_list */
-    uint32_t reserved[3];
+    int32_t differentiable;      /* 1: frames keep what gsr_backward needs (no counterpart in the reference):
+                                    the render pass is the training forward (exact alpha, per-pixel final
+                                    transmittance and last contributor). Needs format RGBA32F, render_splat
+                                    FULL, sort_window 0 and an f32 scene */
+    uint32_t reserved[2];
 } gsr_settings;
 
 /* RenderSplatMode (splat_tile_renderer.h:88-92; render_splat.glsl:1-21), same values. The
@@ -341,6 +345,28 @@ gsr_status gsr_colour_model_evaluate(gsr_colour_model* model, const uint32_t* ga
 gsr_status gsr_colour_model_fallback(gsr_colour_model* model, float threshold, uint32_t* mlp_count, float* errors);
 /* Binds a model of the renderer's scene (null: plain SH colour again). */
 gsr_status gsr_renderer_set_colour_model(gsr_renderer* r, gsr_colour_model* model);
+
+/* --- Differentiable rendering ------------------------------------------------------------------
+ * No counterpart in the reference (it renders forward only). For a renderer created with
+ * `differentiable`, gsr_backward takes dL/d(image) of its last frame (device, RGBA32F layout of
+ * that frame's size; the alpha channel is ignored) and writes the gradients of L with respect to
+ * the scene's parameters, as seen through the image out = sum_i c_i a_i T_i (black background):
+ *   means           N x 3
+ *   log_scales      N x 3   (the scene's scales are exp(log_scales))
+ *   quats           N x 4   (w, x, y, z as stored; the f32 path uses them unnormalised)
+ *   opacity_logits  N       (the scene's opacities are sigmoid(logits))
+ *   sh              N x K x 3 (the scene's layout; bands above the evaluated degree get 0)
+ * Device pointers, float32, each may be null; overwritten, not accumulated; culled gaussians get
+ * zeros. The alpha clamp at 0.99 passes no gradient, as min() does. Asynchronous on `stream` (or
+ * the context's), ordered after the frame when on the frame's stream. */
+typedef struct gsr_gradients {
+    float* means;
+    float* log_scales;
+    float* quats;
+    float* opacity_logits;
+    float* sh;
+} gsr_gradients;
+gsr_status gsr_backward(gsr_renderer* r, const void* grad_image, const gsr_gradients* out, void* stream);
 
 /* --- PLY scenes (loadGaussianPly, scene/ply_loader.h:1-12 and 86-95) ---------------------------
  * A binary little-endian INRIA 3DGS .ply file decoded and activated on the host into the f32

@@ -300,6 +300,54 @@ def render(scene, cam: OracleCamera, max_sh_degree=3, early_termination=True, bo
     return rgb, st.as_dict()
 
 
+_diff_render = _sig("oracle_diff_render_f64", ctypes.c_double,
+                    [ctypes.c_uint32] + [ctypes.c_void_p] * 5 + [ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                                                 ctypes.c_uint32, ctypes.c_uint32, _f32, _f32,
+                                                                 ctypes.c_void_p, ctypes.c_void_p])
+_diff_fd = _sig("oracle_diff_fd_f64", None,
+                [ctypes.c_uint32] + [ctypes.c_void_p] * 5 + [ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                                             ctypes.c_uint32, ctypes.c_uint32, _f32, _f32,
+                                                             ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                                             ctypes.c_double, ctypes.c_void_p, ctypes.c_int32])
+
+
+class RawParams:
+    """Raw parameters of the differentiable render (float64): means (N, 3), log_scales (N, 3),
+    quats (N, 4) w-first, used unnormalised, logits (N,), sh (N, K * 3) coefficient-major RGB."""
+
+    def __init__(self, means, log_scales, quats, logits, sh, sh_degree):
+        f = lambda a: np.ascontiguousarray(a, np.float64)  # noqa: E731
+        self.means, self.log_scales, self.quats, self.logits, self.sh = (f(a) for a in
+                                                                          (means, log_scales, quats, logits, sh))
+        self.sh_degree = int(sh_degree)
+        self.count = self.means.shape[0]
+        self.K = (self.sh_degree + 1) ** 2
+
+    def _args(self):
+        return [self.count, _p(self.means), _p(self.log_scales), _p(self.quats), _p(self.logits), _p(self.sh),
+                self.K, self.sh_degree]
+
+
+def diff_render_f64(params: RawParams, cam: OracleCamera, target=None):
+    """FP64 render of the raw parameters (rect bounds, black background): (loss, (H, W, 3) image);
+    loss = mean |image - target| (0 without a target)."""
+    img = np.zeros((cam.height, cam.width, 3), np.float64)
+    t = None if target is None else np.ascontiguousarray(np.asarray(target, np.float32)[..., :3])
+    loss = _diff_render(*params._args(), *cam.args(), None if t is None else _p(t), _p(img))
+    return float(loss), img
+
+
+def diff_fd_f64(params: RawParams, cam: OracleCamera, target, which, h=1e-4, threads=None):
+    """Central differences of the L1 loss for the gaussians `which`: (len(which), 11 + 3K) in the
+    order mean (3), log-scale (3), quaternion (4), logit (1), SH (K x 3)."""
+    threads = default_threads() if threads is None else threads
+    w = np.ascontiguousarray(which, np.uint32)
+    t = np.ascontiguousarray(np.asarray(target, np.float32)[..., :3])
+    out = np.zeros((w.size, 11 + 3 * params.K), np.float64)
+    _diff_fd(*params._args(), *cam.args(), _p(t), w.size, _p(w), float(h), _p(out), int(threads))
+    return out
+
+
 def quantize8(rgb):
     """writePngRgb (image_io.cpp:51-57): clamp to [0, 1], NaN -> 0, lround(v * 255)."""
     v = np.nan_to_num(np.clip(np.asarray(rgb, np.float32), 0.0, 1.0), nan=0.0).astype(np.float32)
