@@ -479,6 +479,26 @@ class Scene:
         self.handle, self.count, self.sh_degree, self.format, self.ctx = h, int(desc.count), sh_degree, fmt, ctx
         self._renderers = weakref.WeakSet()  # open renderers bound to this scene
 
+    @classmethod
+    def from_device(cls, ctx: Context, count: int, pos_alpha_ptr: int, scales_ptr: int, rotations_ptr: int,
+                    sh_ptr: int, sh_degree: int) -> "Scene":
+        """An f32 scene bound to caller-owned device arrays (pos_alpha, scales, rotations: N x 4
+        floats, bound in place and read by every frame; sh: N x K x 3, copied at creation). The
+        caller keeps the arrays alive and unchanged while the scene is in use."""
+        desc = SceneDesc()
+        desc.sh_degree = sh_degree
+        desc.format = SCENE_F32_SOA
+        desc.count = int(count)
+        desc.device_pointers = 1
+        desc.pos_alpha, desc.scales, desc.rotations, desc.sh = pos_alpha_ptr, scales_ptr, rotations_ptr, sh_ptr
+        h = _vp()
+        _check(_scene_create(ctx.handle, ctypes.byref(desc), ctypes.byref(h)), "gsr_scene_create")
+        self = cls.__new__(cls)
+        self._keep, self._device_arrays = [], []
+        self.handle, self.count, self.sh_degree, self.format, self.ctx = h, int(count), sh_degree, "f32", ctx
+        self._renderers = weakref.WeakSet()
+        return self
+
     def close(self) -> None:
         if self.handle:
             if any(r.handle for r in list(self._renderers)):
