@@ -12,8 +12,9 @@ Prints ONE JSON line: ms/frame (value), per-stage ms, sorted keys/s, the rooflin
 dominant kernel, the fp16-scene and other-sort-mode frame times, the image quality of one
 frame against the CPU oracle, and the oracle's own CPU time on the same frame (cpu_baseline);
 also the batched multi-view throughput (32 cameras of mixed sizes on a 1M-gaussian scene), the
-orbit's throughput with 2 and 4 frames in flight, and the learned colour on a 3M-gaussian scene
-(training time and error, fallback fraction at 2/255, ms/frame and PSNR against plain SH).
+orbit's throughput with 2 and 4 frames in flight, the learned colour on a 3M-gaussian scene
+(training time and error, fallback fraction at 2/255, ms/frame and PSNR against plain SH), and
+the differentiable render (500k gaussians, 1280x720, L1 loss: forward and backward ms).
 """
 import argparse
 import ctypes
@@ -45,6 +46,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-colour measurement")
+    ap.add_argument("--no-backward", action="store_true", help="skip the differentiable-rendering measurement")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound of the CPU baseline sample")
     ap.add_argument("--profile-learned", type=int, default=0,
                     help="for rocprofv3 runs of the learned colour: render this many orbit frames of a 3M-gaussian "
@@ -210,6 +212,81 @@ def learned_bench(ctx, cams, frames, count=3_000_000, samples=1 << 24):
     plain.close()
     model.close()
     out.close()
+    scene.close()
+    return res
+
+
+def backward_bench(ctx, count=500_000, W=1280, H=720, reps=20):
+    """Differentiable rendering: `count` gaussians of the small workload's distributions (seed 0,
+    means ~ N(0, 2^2 I)), 1280x720 camera at radius 8 looking at the origin, L1 loss against the
+    render of the same scene with means moved by N(0, 0.01^2) and opacity logits by N(0, 0.1^2).
+    Times (wall clock around `reps` synchronised iterations) the differentiable forward alone,
+    gsr_backward alone and both, with dL/d(image) of the frame resident on the device (its
+    element-wise sign is not timed)."""
+    import numpy as np
+
+    import gsr
+    from gsr import camera, scenes
+
+    sc = scenes.make_scene(count, seed=0, mean_sigma=2.0)
+    cam = gsr.camera_from_viewer(camera.orbit_view(8.0, 0.0, 0.0), camera.perspective(60.0, W / H),
+                                 camera.IDENTITY, W, H)
+    rng = np.random.default_rng(0)
+    alpha = sc.pos_alpha[:, 3].astype(np.float64)
+    logit = np.log(alpha / (1.0 - alpha)) + rng.normal(0.0, 0.1, count)
+    moved = sc.pos_alpha.copy()
+    moved[:, :3] += rng.normal(0.0, 0.01, (count, 3)).astype(np.float32)
+    moved[:, 3] = (1.0 / (1.0 + np.exp(-logit))).astype(np.float32)
+    tscene = gsr.Scene(ctx, moved, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    tr = gsr.Renderer(ctx, tscene, target="rgba32f", sort_mode="global", differentiable=True)
+    target = tr.render_to_host(cam)[..., :3]
+    tr.close()
+    tscene.close()
+    scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    r = gsr.Renderer(ctx, scene, target="rgba32f", sort_mode="global", differentiable=True)
+    img = r.render_to_host(cam)[..., :3]
+    loss = float(np.abs(img - target).mean())
+    gimg = np.zeros((H, W, 4), np.float32)
+    gimg[..., :3] = np.sign(img - target) / np.float32(img.size)
+    gbuf = gsr.DeviceBuffer(ctx, gimg.nbytes)
+    gbuf.write(gimg)
+    out = gsr.DeviceBuffer(ctx, W * H * 16)
+    K = (sc.sh_degree + 1) ** 2
+    grads = {k: gsr.DeviceBuffer(ctx, count * 4 * w) for k, w in
+             (("means", 3), ("log_scales", 3), ("quats", 4), ("opacity_logits", 1), ("sh", 3 * K))}
+    ptrs = {k: b.ptr for k, b in grads.items()}
+
+    def timed(fn):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    fwd = timed(lambda: r.render(cam, out.ptr))
+    r.render(cam, out.ptr)
+    bwd = timed(lambda: r.backward(gbuf.ptr, **ptrs))
+
+    def both():
+        r.render(cam, out.ptr)
+        r.backward(gbuf.ptr, **ptrs)
+
+    total = timed(both)
+    plain = gsr.Renderer(ctx, scene, target="rgba32f", sort_mode="global")
+    plain_ms = timed(lambda: plain.render(cam, out.ptr))
+    g_means = grads["means"].read(np.empty((count, 3), np.float32))
+    res = {"gaussians": count, "width": W, "height": H, "l1_loss": round(loss, 6),
+           "forward_ms": round(fwd, 4), "backward_ms": round(bwd, 4), "forward_backward_ms": round(total, 4),
+           "plain_forward_ms": round(plain_ms, 4),
+           "gaussians_with_mean_gradient": int((np.abs(g_means).sum(1) > 0).sum())}
+    print(f"# backward ({count} gaussians, {W}x{H}): forward {fwd:.3f} ms, backward {bwd:.3f} ms, both "
+          f"{total:.3f} ms (plain forward {plain_ms:.3f} ms); L1 {loss:.5f}", file=sys.stderr, flush=True)
+    for b in list(grads.values()) + [gbuf, out]:
+        b.close()
+    plain.close()
+    r.close()
     scene.close()
     return res
 
@@ -385,6 +462,7 @@ def main():
 
     multiview = None if args.no_multiview or args.ply else multiview_bench(ctx)
     learned = None if args.no_learned or args.ply else learned_bench(ctx, cams, min(args.steps, 100))
+    backward = None if args.no_backward or args.ply else backward_bench(ctx)
 
     # Frames in flight (the reference viewer keeps two, application.h:55): the orbit rendered by
     # k renderers on k streams (gsr_batch, frame i on branch i % k, each branch its own image),
@@ -471,6 +549,7 @@ def main():
         "multiview": multiview,
         "frames_in_flight_ms_per_frame": in_flight,
         "learned_colour": learned,
+        "backward": backward,
     }
     print(json.dumps(line), flush=True)
 
