@@ -21,18 +21,120 @@ constexpr int kBatch = 256;  // list entries staged in LDS per round (one per th
 // opacity; colour r, g, b.
 constexpr int kGrad2D = 9;
 
-// Training forward: one workgroup per 16x16 tile, one pixel per thread, exact alpha in the
-// reference's operation order. out = (C, 1 - T); state = (T bits, entries of the tile's range up
-// to and including the last contributor).
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLog2MinAlpha = -7.994353436858858f;  // log2(1/255)
+
+// One list entry as staged in LDS: the conic scaled into the log2 domain and log2(opacity),
+// s0 = (cx, cy, -log2(e) A / 2, -log2(e) B), s1 = (-log2(e) C / 2, log2 o), its colour, and the
+// mask of the tile's 8x8 quadrants (bit w: quadrant (w % 2, w / 2)) that the box around its
+// alpha >= 1/255 ellipse meets. The ellipse is d^T K d <= r^2, r^2 = 2 ln(255 o), with
+// half-extents r sqrt(C / det K) and r sqrt(A / det K), widened by 0.1 % and one pixel against
+// rounding; any other conic keeps every quadrant.
+__device__ __forceinline__ uint32_t stageEntry(const float4* splats, uint32_t g, float tx0, float ty0, float4& s0,
+                                               float2& s1, float4& col) {
+    const float4 a = splats[3 * size_t(g)], b = splats[3 * size_t(g) + 1], c = splats[3 * size_t(g) + 2];
+    s0 = make_float4(a.x, a.y, -0.5f * kLog2e * b.x, -kLog2e * b.y);
+    s1 = make_float2(-0.5f * kLog2e * b.z, __log2f(b.w));
+    col = c;
+    const float det = b.x * b.z - b.y * b.y;
+    const float r2 = 2.0f * (s1.y - kLog2MinAlpha) * (1.0f / kLog2e);
+    if (!(det > 0.0f && b.x > 0.0f && b.z > 0.0f)) return 0xFu;
+    if (r2 < 0.0f) return 0u;  // alpha < 1/255 everywhere
+    const float inv = __builtin_amdgcn_rcpf(det);  // hardware rcp and sqrt: inside the widening
+    const float ex = __builtin_amdgcn_sqrtf(r2 * b.z * inv) * 1.001f + 1.0f;
+    const float ey = __builtin_amdgcn_sqrtf(r2 * b.x * inv) * 1.001f + 1.0f;
+    if (!(ex < 1e30f && ey < 1e30f)) return 0xFu;
+    uint32_t m = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+        const float x0 = tx0 + 8.0f * float(w & 1u), y0 = ty0 + 8.0f * float(w >> 1);
+        const bool hit = a.x + ex >= x0 && a.x - ex <= x0 + 7.0f && a.y + ey >= y0 && a.y - ey <= y0 + 7.0f;
+        m |= hit ? (1u << w) : 0u;
+    }
+    return m;
+}
+
+// The wave's list: the batch slots whose mask has the wave's bit, in slot order; returns the count.
+__device__ __forceinline__ uint32_t waveList(const uint32_t* mask, uint32_t* list, uint32_t wave, uint32_t lane) {
+    const uint64_t lt = (uint64_t(1) << lane) - 1u;
+    uint32_t count = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kBatch / 64; ++j) {
+        const uint32_t slot = j * 64 + lane;
+        const bool hit = (mask[slot] >> wave) & 1u;
+        const uint64_t b = __builtin_amdgcn_ballot_w64(hit);
+        if (hit) list[count + uint32_t(__popcll(b & lt))] = slot;
+        count += uint32_t(__popcll(b));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return count;
+}
+
+// alpha = min(0.99, o exp(power)) as the fast render takes it: q = log2(o) + log2(e) power, the
+// cutoff tested on q before the exponential; 0 where the splat is skipped (power > 0 or
+// alpha < 1/255). The training forward and the backward take every decision from this one
+// expression, so the backward retraces the forward's walk exactly. *ex = o exp(power) (before the
+// 0.99 clamp).
+__device__ __forceinline__ float alphaOf(float4 s0, float2 s1, float dx, float dy, float* ex) {
+    const float p = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, s1.x * dy * dy);
+    const float q = p + s1.y;
+    *ex = 0.0f;
+    if (!(q >= kLog2MinAlpha && p <= 0.0f)) return 0.0f;
+    const float e = __builtin_amdgcn_exp2f(q);
+    *ex = e;
+    const float a = kMaxAlpha < e ? kMaxAlpha : e;
+    return a >= kMinAlpha ? a : 0.0f;
+}
+
+// Tile order: workgroup b runs on XCD b % 8, and XCD x takes tile rows x, x + 8, x + 16, ... from
+// the middle row outwards (the heavy central tiles first, the light ones fill the tail); the grid
+// is ceil(tilesY / 8) x tilesX x 8 workgroups, those past the last row return at once.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t centreOut(uint32_t s, uint32_t n) {
+    const uint32_t m = (n - 1u) / 2u;
+    return (s & 1u) ? m + (s + 1u) / 2u : m - s / 2u;
+}
+__device__ __forceinline__ uint32_t scheduledTile(uint32_t tilesX, uint32_t tileCount) {
+    const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
+    const uint32_t row = xcd + kXcds * centreOut(idx / tilesX, gridDim.x / kXcds / tilesX);
+    const uint32_t tile = row * tilesX + idx % tilesX;
+    return tile < tileCount ? tile : 0xFFFFFFFFu;
+}
+uint32_t scheduledGrid(const DevCamera& cam) { return (cam.tilesY + kXcds - 1) / kXcds * cam.tilesX * kXcds; }
+
+// Wave w of a tile's workgroup takes the 8x8 quadrant (w % 2, w / 2): lane l at (l % 8, l / 8).
+__device__ __forceinline__ void quadrantPixel(uint32_t tile, uint32_t tilesX, uint32_t* px, uint32_t* py, float* qx,
+                                              float* qy) {
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    const uint32_t x0 = (tile % tilesX) * kTileSize + 8 * (w & 1u), y0 = (tile / tilesX) * kTileSize + 8 * (w >> 1);
+    *px = x0 + (l & 7u);
+    *py = y0 + (l >> 3);
+    *qx = float(x0);
+    *qy = float(y0);
+}
+
+// Training forward: one workgroup per 16x16 tile, one pixel per thread, wave w on the 8x8
+// quadrant (w % 2, w / 2), walking the batch entries whose box meets its quadrant four at a time
+// (the four alphas first, their LDS reads in flight together, then the blends in list order).
+// out = (C, 1 - T); state = (T bits, entries of the tile's range up to and including the last
+// contributor).
 __global__ __launch_bounds__(256) void kRenderTrain(const float4* __restrict__ splats, const uint32_t* __restrict__ values,
                                                     const uint2* __restrict__ ranges, DevCamera cam,
                                                     float4* __restrict__ out, uint2* __restrict__ state) {
-    __shared__ float4 sA[kBatch];  // centre x, y, conic A, B
-    __shared__ float4 sB[kBatch];  // conic C, opacity, r, g
-    __shared__ float sC[kBatch];   // b
-    const uint32_t tile = blockIdx.x;
-    const uint32_t tx = tile % cam.tilesX, ty = tile / cam.tilesX;
-    const uint32_t px = tx * kTileSize + threadIdx.x % kTileSize, py = ty * kTileSize + threadIdx.x / kTileSize;
+    __shared__ float4 sS0[kBatch];
+    __shared__ float2 sS1[kBatch];
+    __shared__ float4 sCol[kBatch];
+    __shared__ uint32_t sMask[kBatch];
+    __shared__ uint32_t sList[4][kBatch];
+    const uint32_t tile = scheduledTile(cam.tilesX, cam.tilesX * cam.tilesY);
+    if (tile == 0xFFFFFFFFu) return;  // whole workgroup
+    uint32_t px, py;
+    float qx, qy;
+    quadrantPixel(tile, cam.tilesX, &px, &py, &qx, &qy);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const float tx0 = float((tile % cam.tilesX) * kTileSize), ty0 = float((tile / cam.tilesX) * kTileSize);
     const bool inside = px < cam.width && py < cam.height;
     const float pxf = float(px), pyf = float(py);
     const uint2 range = ranges[tile];
@@ -42,33 +144,38 @@ __global__ __launch_bounds__(256) void kRenderTrain(const float4* __restrict__ s
     for (uint32_t base = range.x; base < range.y; base += kBatch) {
         if (__syncthreads_count(!done) == 0) break;
         const uint32_t j = base + threadIdx.x;
-        if (j < range.y) {
-            const uint32_t g = values[j];
-            const float4 s0 = splats[3 * size_t(g)], s1 = splats[3 * size_t(g) + 1], s2 = splats[3 * size_t(g) + 2];
-            sA[threadIdx.x] = make_float4(s0.x, s0.y, s1.x, s1.y);
-            sB[threadIdx.x] = make_float4(s1.z, s1.w, s2.x, s2.y);
-            sC[threadIdx.x] = s2.z;
-        }
+        sMask[threadIdx.x] = j < range.y ? stageEntry(splats, values[j], tx0, ty0, sS0[threadIdx.x], sS1[threadIdx.x],
+                                                      sCol[threadIdx.x])
+                                         : 0u;
         __syncthreads();
-        const uint32_t n = range.y - base < uint32_t(kBatch) ? range.y - base : uint32_t(kBatch);
-        for (uint32_t k = 0; k < n && !done; ++k) {
-            const float4 a = sA[k], b = sB[k];
-            const float dx = a.x - pxf, dy = a.y - pyf;
-            const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-            if (power > 0.0f) continue;
-            const float e = b.y * expf(power);
-            const float alpha = kMaxAlpha < e ? kMaxAlpha : e;
-            if (alpha < kMinAlpha) continue;
-            const float testT = T * (1.0f - alpha);
-            if (testT < kMinTransmittance) {
-                done = true;
-                break;
+        const uint32_t count = waveList(sMask, sList[wave], wave, lane);
+        for (uint32_t i = 0; i < count; i += 4) {
+            if (__builtin_amdgcn_ballot_w64(!done) == 0) break;
+            uint32_t slot[4];
+            float alpha[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                slot[u] = sList[wave][i + u < count ? i + u : i];
+                const float4 s0 = sS0[slot[u]];
+                float ex;
+                alpha[u] = i + u < count ? alphaOf(s0, sS1[slot[u]], s0.x - pxf, s0.y - pyf, &ex) : 0.0f;
             }
-            C0 += b.z * alpha * T;
-            C1 += b.w * alpha * T;
-            C2 += sC[k] * alpha * T;
-            T = testT;
-            last = base + k - range.x + 1;
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                if (done || alpha[u] == 0.0f) continue;
+                const float testT = T * (1.0f - alpha[u]);
+                if (testT < kMinTransmittance) {
+                    done = true;
+                    continue;
+                }
+                const float4 col = sCol[slot[u]];
+                const float w = alpha[u] * T;
+                C0 = fmaf(col.x, w, C0);
+                C1 = fmaf(col.y, w, C1);
+                C2 = fmaf(col.z, w, C2);
+                T = testT;
+                last = base + slot[u] - range.x + 1;
+            }
         }
     }
     if (inside) {
@@ -84,7 +191,31 @@ __device__ __forceinline__ float waveSum(float v) {
     return v;
 }
 
-// Render backward: one workgroup per tile. Each pixel walks its entries from its last contributor
+// Sums v[0..7] over the wave with 10 shuffles instead of 48: at each of the first three halvings
+// a lane keeps the half of its values its lane bit selects and receives its partner's partial
+// sums of that half. Afterwards every lane holds the wave's sum of value
+// 4 b5 + 2 b4 + b3 (b = the lane's bits).
+__device__ __forceinline__ float waveSum8(const float v[8], uint32_t lane) {
+    const bool h5 = (lane & 32u) != 0, h4 = (lane & 16u) != 0, h3 = (lane & 8u) != 0;
+    float a[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float keep = h5 ? v[j + 4] : v[j], give = h5 ? v[j] : v[j + 4];
+        a[j] = keep + __shfl_xor(give, 32, 64);
+    }
+    float b[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const float keep = h4 ? a[j + 2] : a[j], give = h4 ? a[j] : a[j + 2];
+        b[j] = keep + __shfl_xor(give, 16, 64);
+    }
+    float c = (h3 ? b[1] : b[0]) + __shfl_xor(h3 ? b[0] : b[1], 8, 64);
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    return c;
+}
+
+// Render backward: one workgroup per tile (waves on 8x8 quadrants). Each pixel walks its entries from its last contributor
 // back to the first, recovering T_i = T_{i+1} / (1 - a_i) and the colour behind,
 // B_{i-1} = c_i a_i + (1 - a_i) B_i (B = bg = 0 behind the last); dout/da_i = T_i (c_i - B_i),
 // dout/dc_i = a_i T_i. The 9 gradient values of an entry are summed over the wave's 64 pixels
@@ -94,15 +225,20 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
                                                        const uint32_t* __restrict__ values,
                                                        const uint2* __restrict__ ranges, DevCamera cam,
                                                        const float4* __restrict__ dOut, const uint2* __restrict__ state,
-                                                       float* __restrict__ grad2D, uint32_t count) {
-    __shared__ float4 sA[kBatch];
-    __shared__ float4 sB[kBatch];
-    __shared__ float sC[kBatch];
+                                                       float* __restrict__ grad2D, uint32_t gaussianCount) {
+    __shared__ float4 sS0[kBatch];
+    __shared__ float2 sS1[kBatch];
+    __shared__ float4 sCol[kBatch];
+    __shared__ uint32_t sMask[kBatch];
+    __shared__ uint32_t sList[4][kBatch];
     __shared__ float sG[kGrad2D][kBatch];
     __shared__ uint32_t sMax[4];
-    const uint32_t tile = blockIdx.x;
-    const uint32_t tx = tile % cam.tilesX, ty = tile / cam.tilesX;
-    const uint32_t px = tx * kTileSize + threadIdx.x % kTileSize, py = ty * kTileSize + threadIdx.x / kTileSize;
+    const uint32_t tile = scheduledTile(cam.tilesX, cam.tilesX * cam.tilesY);
+    if (tile == 0xFFFFFFFFu) return;  // whole workgroup
+    uint32_t px, py;
+    float qx, qy;
+    quadrantPixel(tile, cam.tilesX, &px, &py, &qx, &qy);
+    const float tx0 = float((tile % cam.tilesX) * kTileSize), ty0 = float((tile / cam.tilesX) * kTileSize);
     const bool inside = px < cam.width && py < cam.height;
     const float pxf = float(px), pyf = float(py);
     const uint2 range = ranges[tile];
@@ -131,55 +267,54 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
         const uint32_t begin = end > uint32_t(kBatch) ? end - uint32_t(kBatch) : 0u;
         const uint32_t n = end - begin;
         __syncthreads();  // the previous round's LDS reads are done
-        if (threadIdx.x < n) {
-            const uint32_t g = values[range.x + begin + threadIdx.x];
-            const float4 s0 = splats[3 * size_t(g)], s1 = splats[3 * size_t(g) + 1], s2 = splats[3 * size_t(g) + 2];
-            sA[threadIdx.x] = make_float4(s0.x, s0.y, s1.x, s1.y);
-            sB[threadIdx.x] = make_float4(s1.z, s1.w, s2.x, s2.y);
-            sC[threadIdx.x] = s2.z;
-        }
+        sMask[threadIdx.x] = threadIdx.x < n ? stageEntry(splats, values[range.x + begin + threadIdx.x], tx0, ty0,
+                                                          sS0[threadIdx.x], sS1[threadIdx.x], sCol[threadIdx.x])
+                                             : 0u;
 #pragma unroll
         for (int v = 0; v < kGrad2D; ++v) sG[v][threadIdx.x] = 0.0f;
         __syncthreads();
-        for (int k = int(n) - 1; k >= 0; --k) {
-            const uint32_t j = begin + uint32_t(k);
-            const float4 a = sA[k], b = sB[k];
-            const float c2 = sC[k];
+        const uint32_t listCount = waveList(sMask, sList[wave], wave, lane);
+        for (int i = int(listCount) - 1; i >= 0; --i) {
+            const uint32_t slot = sList[wave][i];
+            const uint32_t j = begin + slot;  // the entry's position in the tile's range
+            const float4 s0 = sS0[slot];
             float v[kGrad2D] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            bool act = j < last;
-            if (act) {
-                const float dx = a.x - pxf, dy = a.y - pyf;
-                const float power = -0.5f * (a.z * dx * dx + b.x * dy * dy) - a.w * dx * dy;
-                const float G = power > 0.0f ? 0.0f : expf(power);
-                const float e = b.y * G;
-                const float alpha = kMaxAlpha < e ? kMaxAlpha : e;
-                act = power <= 0.0f && alpha >= kMinAlpha;
+            bool act = false;
+            if (j < last) {
+                const float dx = s0.x - pxf, dy = s0.y - pyf;
+                const float2 s1 = sS1[slot];
+                float ex;
+                const float alpha = alphaOf(s0, s1, dx, dy, &ex);
+                act = alpha != 0.0f;
                 if (act) {
-                    T = T / (1.0f - alpha);  // T before this splat
-                    const float dAlpha = T * ((b.z - B0) * g0 + (b.w - B1) * g1 + (c2 - B2) * g2);
+                    const float4 col = sCol[slot];
+                    T = T * __builtin_amdgcn_rcpf(1.0f - alpha);  // T before this splat (1 - alpha >= 0.01)
+                    const float dAlpha = T * ((col.x - B0) * g0 + (col.y - B1) * g1 + (col.z - B2) * g2);
                     v[6] = alpha * T * g0;
                     v[7] = alpha * T * g1;
                     v[8] = alpha * T * g2;
-                    B0 = b.z * alpha + (1.0f - alpha) * B0;
-                    B1 = b.w * alpha + (1.0f - alpha) * B1;
-                    B2 = c2 * alpha + (1.0f - alpha) * B2;
-                    if (e < kMaxAlpha) {  // a = o G: d/do = G, d/dp = a
+                    B0 = col.x * alpha + (1.0f - alpha) * B0;
+                    B1 = col.y * alpha + (1.0f - alpha) * B1;
+                    B2 = col.z * alpha + (1.0f - alpha) * B2;
+                    if (ex < kMaxAlpha) {  // alpha = o exp(power): d/dpower = alpha, d/do = alpha / o
+                        // The unscaled conic: A = -2 s0.z / log2(e), B = -s0.w / log2(e), C = -2 s1.x / log2(e).
+                        const float A = -2.0f * s0.z * (1.0f / kLog2e), Bc = -s0.w * (1.0f / kLog2e);
+                        const float Cc = -2.0f * s1.x * (1.0f / kLog2e);
                         const float gp = dAlpha * alpha;
-                        v[0] = -gp * (a.z * dx + a.w * dy);
-                        v[1] = -gp * (a.w * dx + b.x * dy);
+                        v[0] = -gp * (A * dx + Bc * dy);
+                        v[1] = -gp * (Bc * dx + Cc * dy);
                         v[2] = -0.5f * gp * dx * dx;
                         v[3] = -gp * dx * dy;
                         v[4] = -0.5f * gp * dy * dy;
-                        v[5] = dAlpha * G;
+                        v[5] = gp * __builtin_amdgcn_exp2f(-s1.y);  // dAlpha alpha / o
                     }
                 }
             }
             if (__any(act)) {  // wave-uniform
-#pragma unroll
-                for (int q = 0; q < kGrad2D; ++q) {
-                    const float s = waveSum(v[q]);
-                    if (lane == 0) atomicAdd(&sG[q][k], s);
-                }
+                const float s8 = waveSum8(v, lane);
+                if ((lane & 7u) == 0) atomicAdd(&sG[(lane >> 3) & 7u][slot], s8);
+                const float s9 = waveSum(v[8]);
+                if (lane == 0) atomicAdd(&sG[8][slot], s9);
             }
         }
         __syncthreads();
@@ -187,8 +322,8 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
             const uint32_t g = values[range.x + begin + threadIdx.x];
 #pragma unroll
             for (int q = 0; q < kGrad2D; ++q) {
-                const float s = sG[q][threadIdx.x];
-                if (s != 0.0f) atomicAdd(&grad2D[size_t(q) * count + g], s);
+                const float sum = sG[q][threadIdx.x];
+                if (sum != 0.0f) atomicAdd(&grad2D[size_t(q) * gaussianCount + g], sum);
             }
         }
         end = begin;
@@ -430,17 +565,16 @@ int grad2DComponents() { return kGrad2D; }
 
 void launchRenderTrain(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
                        float4* out, uint2* state, hipStream_t stream) {
-    const uint32_t tiles = cam.tilesX * cam.tilesY;
-    if (tiles == 0) return;
-    hipLaunchKernelGGL(kRenderTrain, dim3(tiles), dim3(256), 0, stream, splats, values, ranges, cam, out, state);
+    if (cam.tilesX * cam.tilesY == 0) return;
+    hipLaunchKernelGGL(kRenderTrain, dim3(scheduledGrid(cam)), dim3(256), 0, stream, splats, values, ranges, cam, out,
+                       state);
 }
 
 void launchRenderBackward(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
                           const float4* dOut, const uint2* state, float* grad2D, uint32_t count, hipStream_t stream) {
-    const uint32_t tiles = cam.tilesX * cam.tilesY;
-    if (tiles == 0) return;
-    hipLaunchKernelGGL(kRenderBackward, dim3(tiles), dim3(256), 0, stream, splats, values, ranges, cam, dOut, state,
-                       grad2D, count);
+    if (cam.tilesX * cam.tilesY == 0) return;
+    hipLaunchKernelGGL(kRenderBackward, dim3(scheduledGrid(cam)), dim3(256), 0, stream, splats, values, ranges, cam,
+                       dOut, state, grad2D, count);
 }
 
 void launchPreprocessBackward(const SceneRef& scene, const DevCamera& cam, const float4* splats, const float* grad2D,

@@ -58,9 +58,12 @@ def test_backward_matches_fp64_finite_differences(gsr_ctx, oracle, sort_mode):
     d = np.abs(img - img64)
     log(f"  forward vs FP64: max {d.max():.2e}, median {np.median(d):.2e}")
     assert d.max() < 1e-3
-    # The exact-alpha render of a plain renderer is the same image.
+    # The exact-alpha render of a plain renderer is the same image up to the exponential's
+    # rounding (the training forward takes exp as v_exp_f32 of power log2(e)).
     plain = gsr.Renderer(gsr_ctx, scene, bounds="rect", target="rgba32f", sort_mode=sort_mode, exact_alpha=True)
-    assert np.abs(plain.render_to_host(cam)[..., :3] - img).max() <= 1e-6
+    dp = np.abs(plain.render_to_host(cam)[..., :3] - img)
+    log(f"  training forward vs exact-alpha render: max {dp.max():.2e}")
+    assert dp.max() <= 1e-4
     plain.close()
     r.render(cam)
     H, W = img.shape[:2]
