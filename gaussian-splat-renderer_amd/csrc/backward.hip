@@ -185,42 +185,25 @@ __global__ __launch_bounds__(256) void kRenderTrain(const float4* __restrict__ s
     }
 }
 
-__device__ __forceinline__ float waveSum(float v) {
+// One halving step of a transposing wave sum over 2M values: a lane keeps the half its lane bit
+// `off` selects and adds its partner's partial sums of that half; v[j] then stands for value
+// j + M (lane bit set) of the previous array.
+template <int M>
+__device__ __forceinline__ void halve(float* v, uint32_t lane, int off) {
+    const bool h = (lane & uint32_t(off)) != 0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Sums v[0..7] over the wave with 10 shuffles instead of 48: at each of the first three halvings
-// a lane keeps the half of its values its lane bit selects and receives its partner's partial
-// sums of that half. Afterwards every lane holds the wave's sum of value
-// 4 b5 + 2 b4 + b3 (b = the lane's bits).
-__device__ __forceinline__ float waveSum8(const float v[8], uint32_t lane) {
-    const bool h5 = (lane & 32u) != 0, h4 = (lane & 16u) != 0, h3 = (lane & 8u) != 0;
-    float a[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const float keep = h5 ? v[j + 4] : v[j], give = h5 ? v[j] : v[j + 4];
-        a[j] = keep + __shfl_xor(give, 32, 64);
+    for (int j = 0; j < M; ++j) {
+        const float keep = h ? v[j + M] : v[j], give = h ? v[j] : v[j + M];
+        v[j] = keep + __shfl_xor(give, off, 64);
     }
-    float b[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const float keep = h4 ? a[j + 2] : a[j], give = h4 ? a[j] : a[j + 2];
-        b[j] = keep + __shfl_xor(give, 16, 64);
-    }
-    float c = (h3 ? b[1] : b[0]) + __shfl_xor(h3 ? b[0] : b[1], 8, 64);
-#pragma unroll
-    for (int off = 4; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-    return c;
 }
 
 // Render backward: one workgroup per tile (waves on 8x8 quadrants). Each pixel walks its entries from its last contributor
 // back to the first, recovering T_i = T_{i+1} / (1 - a_i) and the colour behind,
 // B_{i-1} = c_i a_i + (1 - a_i) B_i (B = bg = 0 behind the last); dout/da_i = T_i (c_i - B_i),
 // dout/dc_i = a_i T_i. The 9 gradient values of an entry are summed over the wave's 64 pixels
-// (shuffles), the waves' sums over the tile in LDS, and each entry adds its tile's sums to the
-// gaussian's planes with one atomic per component.
+// (shuffles, four entries together), the waves' sums over the tile in LDS, and each entry adds
+// its tile's sums to the gaussian's planes with one atomic per component.
 __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict__ splats,
                                                        const uint32_t* __restrict__ values,
                                                        const uint2* __restrict__ ranges, DevCamera cam,
@@ -274,47 +257,67 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
         for (int v = 0; v < kGrad2D; ++v) sG[v][threadIdx.x] = 0.0f;
         __syncthreads();
         const uint32_t listCount = waveList(sMask, sList[wave], wave, lane);
-        for (int i = int(listCount) - 1; i >= 0; --i) {
-            const uint32_t slot = sList[wave][i];
-            const uint32_t j = begin + slot;  // the entry's position in the tile's range
-            const float4 s0 = sS0[slot];
-            float v[kGrad2D] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            bool act = false;
-            if (j < last) {
-                const float dx = s0.x - pxf, dy = s0.y - pyf;
+        // Four entries at a time (list positions i, i - 1, i - 2, i - 3): the pixel's state is
+        // updated entry by entry, the 4 x 9 gradient terms are summed over the wave together
+        // (32 terms in 32 shuffles, leaving each lane pair one sum; the 4 colour-b terms in 7).
+        for (int i = int(listCount) - 1; i >= 0; i -= 4) {
+            float v[36];  // v[8 u + q] for the terms q < 8 of entry u, v[32 + u] for term 8
+#pragma unroll
+            for (int q = 0; q < 36; ++q) v[q] = 0.0f;
+            bool act4 = false;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (i - u < 0) break;  // uniform
+                const uint32_t slot = sList[wave][i - u];
+                const uint32_t j = begin + slot;  // the entry's position in the tile's range
+                if (j >= last) continue;
+                const float4 s0 = sS0[slot];
                 const float2 s1 = sS1[slot];
+                const float dx = s0.x - pxf, dy = s0.y - pyf;
                 float ex;
                 const float alpha = alphaOf(s0, s1, dx, dy, &ex);
-                act = alpha != 0.0f;
-                if (act) {
-                    const float4 col = sCol[slot];
-                    T = T * __builtin_amdgcn_rcpf(1.0f - alpha);  // T before this splat (1 - alpha >= 0.01)
-                    const float dAlpha = T * ((col.x - B0) * g0 + (col.y - B1) * g1 + (col.z - B2) * g2);
-                    v[6] = alpha * T * g0;
-                    v[7] = alpha * T * g1;
-                    v[8] = alpha * T * g2;
-                    B0 = col.x * alpha + (1.0f - alpha) * B0;
-                    B1 = col.y * alpha + (1.0f - alpha) * B1;
-                    B2 = col.z * alpha + (1.0f - alpha) * B2;
-                    if (ex < kMaxAlpha) {  // alpha = o exp(power): d/dpower = alpha, d/do = alpha / o
-                        // The unscaled conic: A = -2 s0.z / log2(e), B = -s0.w / log2(e), C = -2 s1.x / log2(e).
-                        const float A = -2.0f * s0.z * (1.0f / kLog2e), Bc = -s0.w * (1.0f / kLog2e);
-                        const float Cc = -2.0f * s1.x * (1.0f / kLog2e);
-                        const float gp = dAlpha * alpha;
-                        v[0] = -gp * (A * dx + Bc * dy);
-                        v[1] = -gp * (Bc * dx + Cc * dy);
-                        v[2] = -0.5f * gp * dx * dx;
-                        v[3] = -gp * dx * dy;
-                        v[4] = -0.5f * gp * dy * dy;
-                        v[5] = gp * __builtin_amdgcn_exp2f(-s1.y);  // dAlpha alpha / o
-                    }
+                if (alpha == 0.0f) continue;
+                act4 = true;
+                const float4 col = sCol[slot];
+                T = T * __builtin_amdgcn_rcpf(1.0f - alpha);  // T before this splat (1 - alpha >= 0.01)
+                const float dAlpha = T * ((col.x - B0) * g0 + (col.y - B1) * g1 + (col.z - B2) * g2);
+                v[8 * u + 6] = alpha * T * g0;
+                v[8 * u + 7] = alpha * T * g1;
+                v[32 + u] = alpha * T * g2;
+                B0 = col.x * alpha + (1.0f - alpha) * B0;
+                B1 = col.y * alpha + (1.0f - alpha) * B1;
+                B2 = col.z * alpha + (1.0f - alpha) * B2;
+                if (ex < kMaxAlpha) {  // alpha = o exp(power): d/dpower = alpha, d/do = alpha / o
+                    // The unscaled conic: A = -2 s0.z / log2(e), B = -s0.w / log2(e), C = -2 s1.x / log2(e).
+                    const float A = -2.0f * s0.z * (1.0f / kLog2e), Bc = -s0.w * (1.0f / kLog2e);
+                    const float Cc = -2.0f * s1.x * (1.0f / kLog2e);
+                    const float gp = dAlpha * alpha;
+                    v[8 * u + 0] = -gp * (A * dx + Bc * dy);
+                    v[8 * u + 1] = -gp * (Bc * dx + Cc * dy);
+                    v[8 * u + 2] = -0.5f * gp * dx * dx;
+                    v[8 * u + 3] = -gp * dx * dy;
+                    v[8 * u + 4] = -0.5f * gp * dy * dy;
+                    v[8 * u + 5] = gp * __builtin_amdgcn_exp2f(-s1.y);  // dAlpha alpha / o
                 }
             }
-            if (__any(act)) {  // wave-uniform
-                const float s8 = waveSum8(v, lane);
-                if ((lane & 7u) == 0) atomicAdd(&sG[(lane >> 3) & 7u][slot], s8);
-                const float s9 = waveSum(v[8]);
-                if (lane == 0) atomicAdd(&sG[8][slot], s9);
+            if (__any(act4)) {  // wave-uniform
+                // 32 terms: after the halvings at 32, 16, 8, 4, 2 lane L holds term (L >> 1) & 31.
+                halve<16>(v, lane, 32);
+                halve<8>(v, lane, 16);
+                halve<4>(v, lane, 8);
+                halve<2>(v, lane, 4);
+                halve<1>(v, lane, 2);
+                const float s32 = v[0] + __shfl_xor(v[0], 1, 64);
+                // The 4 colour-b terms: after the halvings at 32, 16 lane L holds term (L >> 4) & 3.
+                float w[4] = {v[32], v[33], v[34], v[35]};
+                halve<2>(w, lane, 32);
+                halve<1>(w, lane, 16);
+                float s4 = w[0];
+#pragma unroll
+                for (int off = 8; off > 0; off >>= 1) s4 += __shfl_xor(s4, off, 64);
+                const uint32_t t32 = (lane >> 1) & 31u, u32 = t32 >> 3, u4 = (lane >> 4) & 3u;
+                if ((lane & 1u) == 0 && int(u32) <= i) atomicAdd(&sG[t32 & 7u][sList[wave][i - int(u32)]], s32);
+                if ((lane & 15u) == 0 && int(u4) <= i) atomicAdd(&sG[8][sList[wave][i - int(u4)]], s4);
             }
         }
         __syncthreads();
