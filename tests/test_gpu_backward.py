@@ -5,8 +5,9 @@ logits and SH, against FP64 central finite differences (h = 1e-4) of the FP64 re
 visible gaussians. The target is the render of the scene with means moved by N(0, 0.01^2) and
 opacity logits by N(0, 0.1^2). The forward is discontinuous where a pixel's alpha crosses 1/255
 or a splat's tile rect changes, and a central difference across such a point is not a
-derivative: the bar is a robust one (most parameters within 2 % relative, cosine of every
-group > 0.99), stated with the measured figures."""
+derivative (it then changes with the step): the comparison takes the parameters whose
+differences at h = 1e-4 and 2e-5 agree within 1 % (at least 75 % of them), and on those asks
+for 98 % within 2 % relative and a cosine above 0.999 per parameter group."""
 import sys
 
 import numpy as np
@@ -40,17 +41,19 @@ def _target(oracle, raw, ocam, seed=9):
 GROUPS = [("means", 0, 3), ("log_scales", 3, 6), ("quats", 6, 10), ("opacity_logits", 10, 11), ("sh", 11, 59)]
 
 
-@pytest.mark.parametrize("sort_mode,bounds", [("global", "rect"), ("binning", "rect"), ("global", "tight")])
-def test_backward_matches_fp64_finite_differences(gsr_ctx, oracle, sort_mode, bounds):
+@pytest.mark.parametrize("sort_mode,bounds,W,H", [("global", "rect", 96, 64), ("binning", "rect", 96, 64),
+                                                  ("global", "tight", 96, 64), ("global", "tight", 90, 50)])
+def test_backward_matches_fp64_finite_differences(gsr_ctx, oracle, sort_mode, bounds, W, H):
     """The FP64 reference reaches a splat's rect tiles; tight bounds leave out only tiles where its
-    alpha stays below 1/255, so the same reference holds for them."""
+    alpha stays below 1/255, so the same reference holds for them. 90x50: partial tiles and
+    quadrants past the image edge."""
     import gsr
 
-    log(f"case backward 120 gaussians 96x64 sort_mode={sort_mode} bounds={bounds}")
+    log(f"case backward 120 gaussians {W}x{H} sort_mode={sort_mode} bounds={bounds}")
     raw = raw_scene()
     sc = f32_scene(raw)
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, 3)
-    cam = _camera(gsr)
+    cam = _camera(gsr, W, H)
     ocam = oracle.OracleCamera.from_gsr(cam)
     target = _target(oracle, raw, ocam)
     r = gsr.Renderer(gsr_ctx, scene, bounds=bounds, target="rgba32f", sort_mode=sort_mode, differentiable=True)
@@ -79,19 +82,44 @@ def test_backward_matches_fp64_finite_differences(gsr_ctx, oracle, sort_mode, bo
         assert np.all(g[name][culled] == 0), f"culled gaussians get no {name} gradient"
     which = np.random.default_rng(3).choice(visible, 64, replace=False)
     fd = oracle.diff_fd_f64(p, ocam, target, which, h=1e-4)
+    # Where a central difference crosses a discontinuity it is not a derivative, and it then
+    # changes with the step: compare where the differences at h and h / 5 agree within 1 %.
+    fd5 = oracle.diff_fd_f64(p, ocam, target, which, h=2e-5)
     gpu = np.concatenate([g["means"][which], g["log_scales"][which], g["quats"][which],
                           g["opacity_logits"][which][:, None], g["sh"][which]], 1).astype(np.float64)
     scale = np.abs(fd).max()
     for name, lo, hi in GROUPS:
-        a, b = gpu[:, lo:hi].ravel(), fd[:, lo:hi].ravel()
+        a, b, b5 = gpu[:, lo:hi].ravel(), fd[:, lo:hi].ravel(), fd5[:, lo:hi].ravel()
         keep = np.abs(b) > 1e-4 * scale  # parameters with a gradient to compare
-        rel = np.abs(a[keep] - b[keep]) / np.abs(b[keep])
-        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
-        within = float((rel < 0.02).mean()) if keep.any() else 1.0
-        log(f"  {name:15s}: cosine {cos:.5f}, {within * 100:5.1f}% of {keep.sum()} within 2%, "
-            f"median rel {np.median(rel) if keep.any() else 0:.2e}")
-        assert cos > 0.99, name
-        assert within > 0.85, name
+        stable = keep & (np.abs(b - b5) <= 0.01 * np.abs(b))
+        rel = np.abs(a[stable] - b[stable]) / np.abs(b[stable])
+        cos = float(a[stable] @ b[stable] / (np.linalg.norm(a[stable]) * np.linalg.norm(b[stable])))
+        within = float((rel < 0.02).mean())
+        log(f"  {name:15s}: {stable.sum()} of {keep.sum()} differences stable; on them cosine {cos:.6f}, "
+            f"{within * 100:5.1f}% within 2%, median rel {np.median(rel):.2e}")
+        assert stable.sum() >= 0.75 * keep.sum(), name
+        assert cos > 0.999 and within >= 0.98, name
+    r.close()
+    scene.close()
+
+
+def test_backward_nothing_visible(gsr_ctx):
+    """A frame with every gaussian culled (the camera looks away): a black image and all-zero
+    gradients."""
+    import gsr
+    from gsr import camera
+
+    means, log_scales, q, logits, sh = raw_scene()
+    sc = f32_scene((means + np.array([0.0, 0.0, 60.0]), log_scales, q, logits, sh))  # behind the camera
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, 3)
+    cam = gsr.camera_from_viewer(camera.orbit_view(4.0, 0.0, 0.0), camera.perspective(60.0, 1.5), camera.IDENTITY,
+                                 96, 64)
+    r = gsr.Renderer(gsr_ctx, scene, target="rgba32f", differentiable=True)
+    img = r.render_to_host(cam)
+    assert np.all(img == 0)
+    g = r.backward_to_host(np.ones((64, 96, 3), np.float32))
+    for name, arr in g.items():
+        assert np.all(arr == 0), name
     r.close()
     scene.close()
 
