@@ -333,11 +333,9 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
     }
 }
 
-// d(raw SH colour)/d(direction) for one channel's coefficients c[0..15], degree <= 3; also the
-// basis values Y[k] (the derivative of the colour by coefficient k).
-__device__ __forceinline__ void shBackward(int degree, const float* c, float x, float y, float z, float Y[16],
-                                           float grad[3]) {
-    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+// The SH basis Y[k] at a direction (degree <= 3; Y[k] = 0 past the degree), and
+// sum_k e[k] grad Y_k (the direction gradient of a colour with coefficients e).
+__device__ __forceinline__ void shBasis(int degree, float x, float y, float z, float Y[16]) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) Y[k] = 0.0f;
     Y[0] = kC0;
@@ -345,9 +343,6 @@ __device__ __forceinline__ void shBackward(int degree, const float* c, float x, 
         Y[1] = -kC1 * y;
         Y[2] = kC1 * z;
         Y[3] = -kC1 * x;
-        gy += -kC1 * c[1];
-        gz += kC1 * c[2];
-        gx += -kC1 * c[3];
         if (degree > 1) {
             const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
             Y[4] = kC2_0 * xy;
@@ -355,9 +350,6 @@ __device__ __forceinline__ void shBackward(int degree, const float* c, float x, 
             Y[6] = kC2_2 * (2.0f * zz - xx - yy);
             Y[7] = kC2_3 * xz;
             Y[8] = kC2_4 * (xx - yy);
-            gx += kC2_0 * y * c[4] - 2.0f * kC2_2 * x * c[6] + kC2_3 * z * c[7] + 2.0f * kC2_4 * x * c[8];
-            gy += kC2_0 * x * c[4] + kC2_1 * z * c[5] - 2.0f * kC2_2 * y * c[6] - 2.0f * kC2_4 * y * c[8];
-            gz += kC2_1 * y * c[5] + 4.0f * kC2_2 * z * c[6] + kC2_3 * x * c[7];
             if (degree > 2) {
                 Y[9] = kC3_0 * y * (3.0f * xx - yy);
                 Y[10] = kC3_1 * xy * z;
@@ -366,6 +358,22 @@ __device__ __forceinline__ void shBackward(int degree, const float* c, float x, 
                 Y[13] = kC3_4 * x * (4.0f * zz - xx - yy);
                 Y[14] = kC3_5 * z * (xx - yy);
                 Y[15] = kC3_6 * x * (xx - 3.0f * yy);
+            }
+        }
+    }
+}
+__device__ __forceinline__ void shGradDir(int degree, const float* c, float x, float y, float z, float grad[3]) {
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    if (degree > 0) {
+        gy += -kC1 * c[1];
+        gz += kC1 * c[2];
+        gx += -kC1 * c[3];
+        if (degree > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            gx += kC2_0 * y * c[4] - 2.0f * kC2_2 * x * c[6] + kC2_3 * z * c[7] + 2.0f * kC2_4 * x * c[8];
+            gy += kC2_0 * x * c[4] + kC2_1 * z * c[5] - 2.0f * kC2_2 * y * c[6] - 2.0f * kC2_4 * y * c[8];
+            gz += kC2_1 * y * c[5] + 4.0f * kC2_2 * z * c[6] + kC2_3 * x * c[7];
+            if (degree > 2) {
                 gx += kC3_0 * 6.0f * xy * c[9] + kC3_1 * yz * c[10] - kC3_2 * 2.0f * xy * c[11] -
                       kC3_3 * 6.0f * xz * c[12] + kC3_4 * (4.0f * zz - 3.0f * xx - yy) * c[13] +
                       kC3_5 * 2.0f * xz * c[14] + kC3_6 * (3.0f * xx - 3.0f * yy) * c[15];
@@ -536,17 +544,33 @@ __global__ __launch_bounds__(256) void kPreprocessBackward(SceneRef scene, DevCa
     const float dx = mean[0] - cam.pos[0], dy = mean[1] - cam.pos[1], dz = mean[2] - cam.pos[2];
     const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
     const float dir[3] = {dx * inv, dy * inv, dz * inv};
-    float gdir[3] = {0.0f, 0.0f, 0.0f};
+    // The channels' colour gradients (0 where the colour is clamped), folded into one set of
+    // coefficients e[k] = sum_ch gc[ch] co[ch][k]: one basis and one direction gradient.
+    float gc[3], e[16];
+#pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
         const float raw = evalShRaw(degree, co[ch], dir[0], dir[1], dir[2]);
-        const float gc = raw < 0.0f ? 0.0f : g2[6 + ch];
-        float Y[16], gd[3];
-        shBackward(degree, co[ch], dir[0], dir[1], dir[2], Y, gd);
-        if (dSh)
-            for (int k = 0; k < K; ++k) dSh[(size_t(i) * K + k) * 3 + ch] = k < used ? gc * Y[k] : 0.0f;
-        gdir[0] += gc * gd[0];
-        gdir[1] += gc * gd[1];
-        gdir[2] += gc * gd[2];
+        gc[ch] = raw < 0.0f ? 0.0f : g2[6 + ch];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) e[k] = gc[0] * co[0][k] + gc[1] * co[1][k] + gc[2] * co[2][k];
+    float Y[16], gdir[3];
+    shBasis(degree, dir[0], dir[1], dir[2], Y);
+    shGradDir(degree, e, dir[0], dir[1], dir[2], gdir);
+    if (dSh) {
+        float* out = dSh + size_t(i) * K * 3;
+        if (K == 16 && (reinterpret_cast<uintptr_t>(dSh) & 15u) == 0) {  // 48 floats per gaussian: 12 vector stores
+#pragma unroll
+            for (int q = 0; q < 12; ++q) {
+                float w[4];
+#pragma unroll
+                for (int l = 0; l < 4; ++l) w[l] = gc[(4 * q + l) % 3] * Y[(4 * q + l) / 3];
+                reinterpret_cast<float4*>(out)[q] = make_float4(w[0], w[1], w[2], w[3]);
+            }
+        } else {
+            for (int k = 0; k < K; ++k)
+                for (int ch = 0; ch < 3; ++ch) out[k * 3 + ch] = gc[ch] * Y[k];
+        }
     }
     const float dd = dir[0] * gdir[0] + dir[1] * gdir[1] + dir[2] * gdir[2];
     // Mean: through t (view rotation) and the direction.
