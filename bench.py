@@ -294,8 +294,8 @@ def backward_bench(ctx, count=500_000, W=1280, H=720, reps=20):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print(f"bench.py: this renderer runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 

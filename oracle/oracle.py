@@ -86,8 +86,8 @@ _render = _sig("oracle_render", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp
 
 
 def default_threads() -> int:
-    """Host threads for the oracle: OMP_NUM_THREADS when set (the CPU share of a GPU box), else
-    every core up to 16."""
+    """Host threads for the oracle: OMP_NUM_THREADS when set (the process's share of the host's
+    cores), else every core up to 16."""
     env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     return env if env > 0 else min(os.cpu_count() or 1, 16)
 
