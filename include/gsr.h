@@ -35,7 +35,7 @@ typedef enum gsr_status {
     GSR_ERROR_INVALID_ARGUMENT = 1, /* std::runtime_error on bad settings/formats (splat_tile_renderer.cpp:139-163) */
     GSR_ERROR_HIP = 2,              /* a HIP runtime call failed (VK_CHECK, vk_common.h:28-35) */
     GSR_ERROR_OUT_OF_MEMORY = 3,    /* device allocation failed; nothing is left allocated (gpu_scene.cpp:108-114) */
-    GSR_ERROR_UNSUPPORTED = 4,      /* a mode this build does not provide */
+    GSR_ERROR_UNSUPPORTED = 4,      /* a mode the library does not provide */
     GSR_ERROR_FORMAT = 5,           /* a bad scene file (PlyError, scene/ply_loader.h:29-32) */
     GSR_ERROR_IO = 6                /* a file could not be opened or read (std::runtime_error, ply_loader.h:86-88) */
 } gsr_status;
