@@ -72,20 +72,24 @@ __device__ __forceinline__ uint32_t waveList(const uint32_t* mask, uint32_t* lis
     return count;
 }
 
-// alpha = min(0.99, o exp(power)) as the fast render takes it: q = log2(o) + log2(e) power, the
-// cutoff tested on q before the exponential; 0 where the splat is skipped (power > 0 or
-// alpha < 1/255). The training forward and the backward take every decision from this one
-// expression, so the backward retraces the forward's walk exactly. *ex = o exp(power) (before the
-// 0.99 clamp).
-__device__ __forceinline__ float alphaOf(float4 s0, float2 s1, float dx, float dy, float* ex) {
+// q = log2(o exp(power)) at a pixel (dx, dy = centre - pixel), -inf where power > 0 (only for
+// a conic that is not positive definite).
+__device__ __forceinline__ float qOf(float4 s0, float2 s1, float dx, float dy) {
     const float p = fmaf(fmaf(s0.z, dx, s0.w * dy), dx, s1.x * dy * dy);
-    const float q = p + s1.y;
+    return p <= 0.0f ? p + s1.y : -__builtin_inff();
+}
+
+// A splat takes part at a pixel when q >= log2(1/255) (alpha >= 1/255 up to the rounding of q);
+// then alpha = min(0.99, 2^q). The training forward and the backward take every decision from
+// these two expressions, so the backward retraces the forward's walk exactly. *ex = 2^q (before
+// the 0.99 clamp); 0 where the splat does not take part.
+__device__ __forceinline__ float alphaOf(float4 s0, float2 s1, float dx, float dy, float* ex) {
+    const float q = qOf(s0, s1, dx, dy);
     *ex = 0.0f;
-    if (!(q >= kLog2MinAlpha && p <= 0.0f)) return 0.0f;
+    if (!(q >= kLog2MinAlpha)) return 0.0f;
     const float e = __builtin_amdgcn_exp2f(q);
     *ex = e;
-    const float a = kMaxAlpha < e ? kMaxAlpha : e;
-    return a >= kMinAlpha ? a : 0.0f;
+    return kMaxAlpha < e ? kMaxAlpha : e;
 }
 
 // Tile order: workgroup b runs on XCD b % 8, and XCD x takes tile rows x, x + 8, x + 16, ... from
@@ -138,11 +142,14 @@ __global__ __launch_bounds__(256) void kRenderTrain(const float4* __restrict__ s
     const bool inside = px < cam.width && py < cam.height;
     const float pxf = float(px), pyf = float(py);
     const uint2 range = ranges[tile];
+    // The lane's state is its transmittance, colour, last contributor and cutoff thr
+    // (log2(1/255) while the pixel is open, +inf once done or outside the image): every blend is
+    // branch-free, and a splat that no lane of the wave takes is skipped.
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
     uint32_t last = 0;
-    bool done = !inside;
+    float thr = inside ? kLog2MinAlpha : __builtin_inff();
     for (uint32_t base = range.x; base < range.y; base += kBatch) {
-        if (__syncthreads_count(!done) == 0) break;
+        if (__syncthreads_count(thr < __builtin_inff()) == 0) break;
         const uint32_t j = base + threadIdx.x;
         sMask[threadIdx.x] = j < range.y ? stageEntry(splats, values[j], tx0, ty0, sS0[threadIdx.x], sS1[threadIdx.x],
                                                       sCol[threadIdx.x])
@@ -150,31 +157,31 @@ __global__ __launch_bounds__(256) void kRenderTrain(const float4* __restrict__ s
         __syncthreads();
         const uint32_t count = waveList(sMask, sList[wave], wave, lane);
         for (uint32_t i = 0; i < count; i += 4) {
-            if (__builtin_amdgcn_ballot_w64(!done) == 0) break;
+            if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
             uint32_t slot[4];
-            float alpha[4];
+            float q[4];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
+            for (uint32_t u = 0; u < 4; ++u) {  // the four q first, their LDS reads in flight together
                 slot[u] = sList[wave][i + u < count ? i + u : i];
                 const float4 s0 = sS0[slot[u]];
-                float ex;
-                alpha[u] = i + u < count ? alphaOf(s0, sS1[slot[u]], s0.x - pxf, s0.y - pyf, &ex) : 0.0f;
+                q[u] = i + u < count ? qOf(s0, sS1[slot[u]], s0.x - pxf, s0.y - pyf) : -__builtin_inff();
             }
 #pragma unroll
             for (uint32_t u = 0; u < 4; ++u) {
-                if (done || alpha[u] == 0.0f) continue;
-                const float testT = T * (1.0f - alpha[u]);
-                if (testT < kMinTransmittance) {
-                    done = true;
-                    continue;
-                }
+                if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;
+                const bool take = q[u] >= thr;
+                const float a = fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q[u]));
+                const float testT = T * (1.0f - (take ? a : 0.0f));
+                const bool term = take && testT < kMinTransmittance;  // not blended; the pixel ends
+                const bool blend = take && !term;
                 const float4 col = sCol[slot[u]];
-                const float w = alpha[u] * T;
+                const float w = blend ? a * T : 0.0f;
                 C0 = fmaf(col.x, w, C0);
                 C1 = fmaf(col.y, w, C1);
                 C2 = fmaf(col.z, w, C2);
-                T = testT;
-                last = base + slot[u] - range.x + 1;
+                T = blend ? testT : T;
+                last = blend ? base + slot[u] - range.x + 1 : last;
+                thr = term ? __builtin_inff() : thr;
             }
         }
     }
