@@ -192,17 +192,28 @@ __global__ __launch_bounds__(256) void kRenderTrain(const float4* __restrict__ s
     }
 }
 
-// One halving step of a transposing wave sum over 2M values: a lane keeps the half its lane bit
-// `off` selects and adds its partner's partial sums of that half; v[j] then stands for value
-// j + M (lane bit set) of the previous array.
-template <int M>
-__device__ __forceinline__ void halve(float* v, uint32_t lane, int off) {
-    const bool h = (lane & uint32_t(off)) != 0;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-        const float keep = h ? v[j + M] : v[j], give = h ? v[j] : v[j + M];
-        v[j] = keep + __shfl_xor(give, off, 64);
-    }
+// Lane exchanges for the wave sums, all VALU: v_permlane32_swap / v_permlane16_swap exchange
+// half-waves / neighbouring rows of two registers at once, DPP moves permute within a row.
+__device__ __forceinline__ float swapSum32(float a, float b) {  // a of lanes 0-31 paired with b of 32-63
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swapSum16(float a, float b) {  // rows 0/1 and 2/3 likewise
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kDppRor8 = 0x128;        // row_ror:8 (lane ^ 8 within a row of 16)
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror (lane -> 7 - lane within 8)
+constexpr int kDppXor1 = 0xB1;         // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;         // quad_perm [2, 3, 0, 1]
+__device__ __forceinline__ float sum8Lanes(float v) {  // over each group of 8 lanes (lane bits 0-2)
+    v += dpp<kDppHalfMirror>(v);
+    v += dpp<kDppXor1>(v);
+    return v + dpp<kDppXor2>(v);
 }
 
 // Render backward: one workgroup per tile (waves on 8x8 quadrants). Each pixel walks its entries from its last contributor
@@ -308,23 +319,34 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
                 }
             }
             if (__any(act4)) {  // wave-uniform
-                // 32 terms: after the halvings at 32, 16, 8, 4, 2 lane L holds term (L >> 1) & 31.
-                halve<16>(v, lane, 32);
-                halve<8>(v, lane, 16);
-                halve<4>(v, lane, 8);
-                halve<2>(v, lane, 4);
-                halve<1>(v, lane, 2);
-                const float s32 = v[0] + __shfl_xor(v[0], 1, 64);
-                // The 4 colour-b terms: after the halvings at 32, 16 lane L holds term (L >> 4) & 3.
-                float w[4] = {v[32], v[33], v[34], v[35]};
-                halve<2>(w, lane, 32);
-                halve<1>(w, lane, 16);
-                float s4 = w[0];
+                // The 32 terms v[8 u + q]: swap-sums at lane distances 32 (16 pairs) and 16 (8 pairs)
+                // leave v[j] = term j + 8 b4 + 16 b5 (b = lane bits); a halving at distance 8
+                // leaves v[j] = term j + 4 b3 + 8 b4 + 16 b5 (j < 4), summed over the lane groups
+                // of 8: every lane then holds its group's four totals.
 #pragma unroll
-                for (int off = 8; off > 0; off >>= 1) s4 += __shfl_xor(s4, off, 64);
-                const uint32_t t32 = (lane >> 1) & 31u, u32 = t32 >> 3, u4 = (lane >> 4) & 3u;
-                if ((lane & 1u) == 0 && int(u32) <= i) atomicAdd(&sG[t32 & 7u][sList[wave][i - int(u32)]], s32);
-                if ((lane & 15u) == 0 && int(u4) <= i) atomicAdd(&sG[8][sList[wave][i - int(u4)]], s4);
+                for (int j = 0; j < 16; ++j) v[j] = swapSum32(v[j], v[j + 16]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = swapSum16(v[j], v[j + 8]);
+                const bool h3 = (lane & 8u) != 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float keep = h3 ? v[j + 4] : v[j], give = h3 ? v[j] : v[j + 4];
+                    v[j] = sum8Lanes(keep + dpp<kDppRor8>(give));
+                }
+                // The 4 colour-b terms: swap-sums at 32 and 16 leave term b4 + 2 b5, then the row sum.
+                float w0 = swapSum32(v[32], v[34]), w1 = swapSum32(v[33], v[35]);
+                w0 = swapSum16(w0, w1);
+                w0 = sum8Lanes(w0 + dpp<kDppRor8>(w0));
+                const uint32_t jl = lane & 7u;
+                if (jl < 4u) {  // lane j of a group adds term j + 4 b3 + 8 b4 + 16 b5
+                    const float val = jl == 0u ? v[0] : jl == 1u ? v[1] : jl == 2u ? v[2] : v[3];
+                    const uint32_t t = jl + 4u * ((lane >> 3) & 7u), u = t >> 3;
+                    if (int(u) <= i) atomicAdd(&sG[t & 7u][sList[wave][i - int(u)]], val);
+                }
+                if ((lane & 15u) == 0u) {
+                    const uint32_t u = (lane >> 4) & 3u;
+                    if (int(u) <= i) atomicAdd(&sG[8][sList[wave][i - int(u)]], w0);
+                }
             }
         }
         __syncthreads();
