@@ -306,16 +306,16 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
                 B1 = col.y * alpha + (1.0f - alpha) * B1;
                 B2 = col.z * alpha + (1.0f - alpha) * B2;
                 if (ex < kMaxAlpha) {  // alpha = o exp(power): d/dpower = alpha, d/do = alpha / o
-                    // The unscaled conic: A = -2 s0.z / log2(e), B = -s0.w / log2(e), C = -2 s1.x / log2(e).
-                    const float A = -2.0f * s0.z * (1.0f / kLog2e), Bc = -s0.w * (1.0f / kLog2e);
-                    const float Cc = -2.0f * s1.x * (1.0f / kLog2e);
+                    // The pixel's moments of g = dL/dpower; the per-gaussian factors (conic,
+                    // 1 / o) are applied once per entry when the tile's sums are flushed.
                     const float gp = dAlpha * alpha;
-                    v[8 * u + 0] = -gp * (A * dx + Bc * dy);
-                    v[8 * u + 1] = -gp * (Bc * dx + Cc * dy);
-                    v[8 * u + 2] = -0.5f * gp * dx * dx;
-                    v[8 * u + 3] = -gp * dx * dy;
-                    v[8 * u + 4] = -0.5f * gp * dy * dy;
-                    v[8 * u + 5] = gp * __builtin_amdgcn_exp2f(-s1.y);  // dAlpha alpha / o
+                    const float gx = gp * dx, gy = gp * dy;
+                    v[8 * u + 0] = gx;
+                    v[8 * u + 1] = gy;
+                    v[8 * u + 2] = gx * dx;
+                    v[8 * u + 3] = gx * dy;
+                    v[8 * u + 4] = gy * dy;
+                    v[8 * u + 5] = gp;
                 }
             }
             if (__any(act4)) {  // wave-uniform
@@ -351,12 +351,27 @@ __global__ __launch_bounds__(256) void kRenderBackward(const float4* __restrict_
         }
         __syncthreads();
         if (threadIdx.x < n) {
+            // The tile's sums of the moments -> the screen-space gradient: with power =
+            // -(A dx^2 + 2 B dx dy + C dy^2) / 2, d/dcentre = -(A m_x + B m_y, B m_x + C m_y),
+            // d/d(A, B, C) = -(m_xx / 2, m_xy, m_yy / 2), d/do = (sum of g) / o.
             const uint32_t g = values[range.x + begin + threadIdx.x];
+            const float4 s0 = sS0[threadIdx.x];
+            const float2 s1 = sS1[threadIdx.x];
+            const float A = -2.0f * s0.z * (1.0f / kLog2e), Bc = -s0.w * (1.0f / kLog2e);
+            const float Cc = -2.0f * s1.x * (1.0f / kLog2e);
+            const float mx = sG[0][threadIdx.x], my = sG[1][threadIdx.x];
+            float out[kGrad2D];
+            out[0] = -(A * mx + Bc * my);
+            out[1] = -(Bc * mx + Cc * my);
+            out[2] = -0.5f * sG[2][threadIdx.x];
+            out[3] = -sG[3][threadIdx.x];
+            out[4] = -0.5f * sG[4][threadIdx.x];
+            out[5] = sG[5][threadIdx.x] * __builtin_amdgcn_exp2f(-s1.y);
 #pragma unroll
-            for (int q = 0; q < kGrad2D; ++q) {
-                const float sum = sG[q][threadIdx.x];
-                if (sum != 0.0f) atomicAdd(&grad2D[size_t(q) * gaussianCount + g], sum);
-            }
+            for (int q = 6; q < kGrad2D; ++q) out[q] = sG[q][threadIdx.x];
+#pragma unroll
+            for (int q = 0; q < kGrad2D; ++q)
+                if (out[q] != 0.0f) atomicAdd(&grad2D[size_t(q) * gaussianCount + g], out[q]);
         }
         end = begin;
     }
