@@ -247,14 +247,19 @@ def _degree0(sc, col):
     return SyntheticScene(sc.pos_alpha, sc.scales, sc.rotations, np.ascontiguousarray(dc), 0, sc.seed)
 
 
-@pytest.mark.parametrize("sort_mode,render_splat", [("binning", "full"), ("global", "full"), ("global", "p24"),
-                                                    ("binning", "p32")])
-def test_render_with_learned_colour_matches_oracle(gsr_ctx, oracle, sort_mode, render_splat):
+@pytest.mark.parametrize("sort_mode,render_splat,fmt", [("binning", "full", "f32"), ("global", "full", "f32"),
+                                                        ("global", "p24", "f32"), ("binning", "p32", "f32"),
+                                                        ("global", "full", "f16")])
+def test_render_with_learned_colour_matches_oracle(gsr_ctx, oracle, sort_mode, render_splat, fmt):
     import gsr
-    from gsr import camera
+    from gsr import camera, scenes
+    from gsr.scenes import SyntheticScene
 
-    log(f"case colour render 30k sort_mode={sort_mode} render_splat={render_splat}")
-    sc, scene = _scene(gsr, gsr_ctx, 30_000, 12)
+    log(f"case colour render 30k sort_mode={sort_mode} render_splat={render_splat} scene={fmt}")
+    sc = scenes.make_scene(30_000, seed=12, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)
+    if fmt == "f16":  # the GPU renders (and the model trains on) what the fp16 records decode to
+        sc = SyntheticScene(*oracle.decode_f16(*oracle.encode_f16(sc), sc.sh_degree), sc.sh_degree, sc.seed)
     m = gsr.ColourModel(gsr_ctx, scene, seed=3)
     m.train(150, 1 << 16, lr_weights=2e-3, lr_latents=2e-2)
     W, H = 640, 360
@@ -337,5 +342,47 @@ def test_fallback_errors_match_restatement(gsr_ctx, oracle):
     diff = np.abs(err - want)
     log(f"  {n} of 500 within 0.05; |gpu - restatement| median {np.median(diff):.2e} max {diff.max():.2e}")
     assert np.median(diff) < 1e-5 and diff.max() < 1e-2
+    m.close()
+    scene.close()
+
+
+def test_batch_graph_with_learned_colour(gsr_ctx):
+    """Renderers bound to a colour model inside a captured batch graph (the colour pass is one of
+    the graph's kernels): every view and every replay equals a single render with the model."""
+    import gsr
+    from gsr import camera
+
+    log("case colour batch graph 20k gaussians, 6 views")
+    sc, scene = _scene(gsr, gsr_ctx, 20_000, 5)
+    m = gsr.ColourModel(gsr_ctx, scene, seed=1)
+    m.train(20, 1 << 14)
+    n, _ = m.fallback(float(np.float32(0.3)))
+    assert 0 < n < 20_000  # a mix of learned and SH colours
+    cams = [gsr.camera_from_viewer(camera.orbit_view(6.0, 10.0, 60.0 * k), camera.perspective(60.0, W / H),
+                                   camera.IDENTITY, W, H) for k, (W, H) in enumerate([(320, 180), (200, 120)] * 3)]
+    single = gsr.Renderer(gsr_ctx, scene, target="rgba8")
+    single.set_colour_model(m)
+    want = [single.render_to_host(c) for c in cams]
+    rs = [gsr.Renderer(gsr_ctx, scene, target="rgba8") for _ in range(2)]
+    for r in rs:
+        r.set_colour_model(m)
+    outs = [gsr.DeviceBuffer(gsr_ctx, int(c.width) * int(c.height) * 4) for c in cams]
+    batch = gsr.Batch(gsr_ctx, 2)
+    batch.render(rs, cams, [o.ptr for o in outs])
+    gsr_ctx.synchronize()
+    batch.capture(rs, cams, [o.ptr for o in outs])
+    for rep in range(2):
+        for o in outs:
+            o.write(np.zeros(o.nbytes, np.uint8))
+        batch.replay()
+        gsr_ctx.synchronize()
+        for i, (o, c) in enumerate(zip(outs, cams)):
+            img = o.read(np.empty((int(c.height), int(c.width), 4), np.uint8))
+            assert np.array_equal(img, want[i]), f"replay {rep} view {i}"
+    batch.close()
+    for r in rs + [single]:
+        r.close()
+    for o in outs:
+        o.close()
     m.close()
     scene.close()
