@@ -310,6 +310,11 @@ _diff_fd = _sig("oracle_diff_fd_f64", None,
                                                              ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                                              ctypes.c_double, ctypes.c_void_p, ctypes.c_int32])
 
+_diff_grad = _sig("oracle_diff_grad_f64", None,
+                  [ctypes.c_uint32] + [ctypes.c_void_p] * 5 + [ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                                               ctypes.c_uint32, ctypes.c_uint32, _f32, _f32,
+                                                               ctypes.c_void_p, ctypes.c_void_p])
+
 
 class RawParams:
     """Raw parameters of the differentiable render (float64): means (N, 3), log_scales (N, 3),
@@ -345,6 +350,16 @@ def diff_fd_f64(params: RawParams, cam: OracleCamera, target, which, h=1e-4, thr
     t = np.ascontiguousarray(np.asarray(target, np.float32)[..., :3])
     out = np.zeros((w.size, 11 + 3 * params.K), np.float64)
     _diff_fd(*params._args(), *cam.args(), _p(t), w.size, _p(w), float(h), _p(out), int(threads))
+    return out
+
+
+def diff_grad_f64(params: RawParams, cam: OracleCamera, grad_image):
+    """The exact gradient of a loss L whose derivative in the image is grad_image (H, W, 3):
+    (N, 11 + 3K) in the order of diff_fd_f64; zero rows for culled gaussians."""
+    g = np.ascontiguousarray(np.asarray(grad_image, np.float32)[..., :3])
+    assert g.shape == (cam.height, cam.width, 3)
+    out = np.zeros((params.count, 11 + 3 * params.K), np.float64)
+    _diff_grad(*params._args(), *cam.args(), _p(g), _p(out))
     return out
 
 

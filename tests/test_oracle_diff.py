@@ -6,7 +6,11 @@ backward. Pinned here, on the CPU:
   - its central differences against two calls of the FP64 render with one parameter moved by
     +-h (the parameter order and the difference formula);
   - a culled gaussian has zero differences; a quadratic-free parameter (an SH coefficient enters
-    the loss piecewise linearly) gives the same difference for h and h / 10."""
+    the loss piecewise linearly) gives the same difference for h and h / 10;
+  - its exact gradient (dual-number Jacobians of the per-gaussian math, the composite's
+    derivative per pixel) against the central differences wherever those are derivatives (the
+    differences at h and h / 5 agree), is linear in the image gradient, and is zero for a culled
+    gaussian."""
 import numpy as np
 
 
@@ -112,3 +116,48 @@ def test_fd_culled_zero_and_linear_sh(oracle):
     fd2 = oracle.diff_fd_f64(p, ocam, target, [2], h=1e-5, threads=4)
     # the DC coefficients enter linearly between kinks of |.|: the same difference for both h
     assert np.allclose(fd[1, 11:14], fd2[0, 11:14], rtol=1e-4, atol=1e-9)
+
+
+def test_exact_gradient_matches_fd(oracle):
+    ocam = _camera(oracle)
+    raw = raw_scene()
+    means = raw[0].copy()
+    means[7] = [0.0, 0.0, 40.0]  # culled
+    raw = (means,) + raw[1:]
+    p = params_of(oracle, raw)
+    _, target = oracle.diff_render_f64(params_of(oracle, raw_scene(seed=6)), ocam)
+    target = target.astype(np.float32)
+    _, img = oracle.diff_render_f64(p, ocam, target)
+    G = (np.sign(img - target) / (3 * ocam.width * ocam.height)).astype(np.float32)
+    exact = oracle.diff_grad_f64(p, ocam, G)
+    assert exact.shape == (120, 59)
+    assert np.all(exact[7] == 0)
+    which = [g for g in range(120) if g != 7][:24]
+    fd = oracle.diff_fd_f64(p, ocam, target, which, h=1e-4, threads=4)
+    fd5 = oracle.diff_fd_f64(p, ocam, target, which, h=2e-5, threads=4)
+    a = exact[which]
+    scale = np.abs(fd).max()
+    for name, lo, hi in [("means", 0, 3), ("log_scales", 3, 6), ("quats", 6, 10), ("logits", 10, 11),
+                         ("sh", 11, 59)]:
+        A, B, B5 = a[:, lo:hi].ravel(), fd[:, lo:hi].ravel(), fd5[:, lo:hi].ravel()
+        keep = np.abs(B) > 1e-6 * scale
+        stable = keep & (np.abs(B - B5) <= 0.01 * np.abs(B))
+        rel = np.abs(A[stable] - B[stable]) / np.abs(B[stable])
+        print(f"{name:10s}: {stable.sum()} of {keep.sum()} stable, median rel {np.median(rel):.1e}, "
+              f"max {rel.max():.1e}")
+        assert stable.sum() >= 0.9 * keep.sum(), name
+        assert np.median(rel) < 1e-6 and np.quantile(rel, 0.95) < 1e-4 and rel.max() < 1e-2, name
+
+
+def test_exact_gradient_is_linear_in_image_gradient(oracle):
+    ocam = _camera(oracle)
+    p = params_of(oracle, raw_scene())
+    rng = np.random.default_rng(2)
+    shape = (ocam.height, ocam.width, 3)
+    # small integers: G1 + 2 G2 is exact in float32, so only the FP64 sums' rounding remains
+    G1 = rng.integers(-8, 9, size=shape).astype(np.float32)
+    G2 = rng.integers(-8, 9, size=shape).astype(np.float32)
+    g1, g2 = oracle.diff_grad_f64(p, ocam, G1), oracle.diff_grad_f64(p, ocam, G2)
+    g12 = oracle.diff_grad_f64(p, ocam, G1 + 2 * G2)
+    assert np.abs(g12 - (g1 + 2 * g2)).max() <= 1e-12 * np.abs(g12).max()
+    assert np.all(oracle.diff_grad_f64(p, ocam, np.zeros(shape, np.float32)) == 0)
