@@ -13,7 +13,9 @@
 //     last), sorted by a bitonic network and written back. The words are unique within a tile
 //     (a gaussian appears at most once per tile), so the order is total and deterministic.
 //     Network positions are i = lane * E + r: partners less than E apart are in the lane's own
-//     registers, partners further apart are one 64-bit lane shuffle per word. The network is
+//     registers, partners further apart are in another lane: for E <= 16, fewer than 16 lanes
+//     apart (17 of the 21 lane stages) a DPP move within the row per 32-bit half, VALU only;
+//     16 or more lanes apart, and every lane stage of E = 32, a lane shuffle (ds_bpermute). The network is
 //     the flip formulation (the first stage of every merge level pairs i with i ^ (k - 1),
 //     every later stage pairs i with i ^ j, and the lower position always keeps the minimum),
 //     so no per-element direction bits are needed. The stage loops stay rolled (only the
@@ -50,6 +52,31 @@ __device__ __forceinline__ uint64_t shflXor64(uint64_t v, uint32_t mask) {
     const uint32_t lo = __shfl_xor(uint32_t(v), int(mask), 64);
     const uint32_t hi = __shfl_xor(uint32_t(v >> 32), int(mask), 64);
     return (uint64_t(hi) << 32) | lo;
+}
+
+// The word of lane ^ M for a constant M < 16: DPP moves within a row of 16 lanes, VALU only
+// (quad_perm for xor 1, 2, 3; row_half_mirror = xor 7; row_ror:8 = xor 8; row_mirror = xor 15;
+// xor 4 = the quad mirror of the half-row mirror).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, 0xF, 0xF, false));
+}
+template <uint32_t M>
+__device__ __forceinline__ uint32_t rowXor(uint32_t v) {
+    if constexpr (M == 1) return dpp<0xB1>(v);
+    else if constexpr (M == 2) return dpp<0x4E>(v);
+    else if constexpr (M == 3) return dpp<0x1B>(v);
+    else if constexpr (M == 4) return dpp<0x1B>(dpp<0x141>(v));
+    else if constexpr (M == 7) return dpp<0x141>(v);
+    else if constexpr (M == 8) return dpp<0x128>(v);
+    else {
+        static_assert(M == 15, "xor masks with a DPP form");
+        return dpp<0x140>(v);
+    }
+}
+template <uint32_t M>
+__device__ __forceinline__ uint64_t rowXor64(uint64_t v) {
+    return (uint64_t(rowXor<M>(uint32_t(v >> 32))) << 32) | rowXor<M>(uint32_t(v));
 }
 
 // Compare-exchange of two registers: minimum to the first.
@@ -90,28 +117,72 @@ __device__ __forceinline__ void registerLevels(uint64_t (&w)[E]) {
     }
 }
 
-// Lane stages (partners m lanes apart, m = j / E): same register.
+// The 32-word sort keeps lane shuffles for every lane stage: in the DPP form it spills within
+// the 96 VGPRs of 5 workgroups per CU, and at 4 workgroups per CU (128 VGPRs) the kernel took
+// 71.9 us per headline frame against 58.5 us with this split (64.0 us with shuffles only).
+template <int E>
+constexpr bool kDppStages = E <= 16;
+
+// Lane stages (partners m lanes apart, m = j / E): same register. M > 0: the constant lane
+// distance, exchanged by DPP; M == 0: the run-time distance m, by lane shuffles.
+template <int E, uint32_t M>
+__device__ __forceinline__ void laneHalfBy(uint64_t (&w)[E], bool lower, uint32_t m) {
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        uint64_t y;
+        if constexpr (M == 0) y = shflXor64(w[r], m);
+        else y = rowXor64<M>(w[r]);
+        w[r] = keep(lower, w[r], y);
+    }
+}
 template <int E>
 __device__ __forceinline__ void laneHalf(uint64_t (&w)[E], uint32_t lane, uint32_t m) {
     const bool lower = (lane & m) == 0u;
-#pragma unroll
-    for (int r = 0; r < E; ++r) w[r] = keep(lower, w[r], shflXor64(w[r], m));
+    if constexpr (!kDppStages<E>) {
+        laneHalfBy<E, 0>(w, lower, m);
+        return;
+    }
+    switch (m) {  // uniform per wave
+    case 1: laneHalfBy<E, 1>(w, lower, m); break;
+    case 2: laneHalfBy<E, 2>(w, lower, m); break;
+    case 4: laneHalfBy<E, 4>(w, lower, m); break;
+    case 8: laneHalfBy<E, 8>(w, lower, m); break;
+    default: laneHalfBy<E, 0>(w, lower, m); break;  // 16, 32
+    }
 }
 // Flip of a level k > E: position lane * E + r pairs with (lane ^ m) * E + (E - 1 - r),
 // m = k / E - 1; the lower position is the one whose lane has bit k / (2 E) clear.
-template <int E>
-__device__ __forceinline__ void laneFlip(uint64_t (&w)[E], uint32_t lane, uint32_t m, uint32_t half) {
-    const bool lower = (lane & half) == 0u;
+template <int E, uint32_t M>
+__device__ __forceinline__ void laneFlipBy(uint64_t (&w)[E], bool lower, uint32_t m) {
+    auto partner = [&](uint64_t v) {
+        if constexpr (M == 0) return shflXor64(v, m);
+        else return rowXor64<M>(v);
+    };
     if constexpr (E == 1) {
-        w[0] = keep(lower, w[0], shflXor64(w[0], m));
+        w[0] = keep(lower, w[0], partner(w[0]));
     } else {
 #pragma unroll
         for (int r = 0; r < E / 2; ++r) {
-            const uint64_t ya = shflXor64(w[E - 1 - r], m);
-            const uint64_t yb = shflXor64(w[r], m);
+            const uint64_t ya = partner(w[E - 1 - r]);
+            const uint64_t yb = partner(w[r]);
             w[r] = keep(lower, w[r], ya);
             w[E - 1 - r] = keep(lower, w[E - 1 - r], yb);
         }
+    }
+}
+template <int E>
+__device__ __forceinline__ void laneFlip(uint64_t (&w)[E], uint32_t lane, uint32_t m, uint32_t half) {
+    const bool lower = (lane & half) == 0u;
+    if constexpr (!kDppStages<E>) {
+        laneFlipBy<E, 0>(w, lower, m);
+        return;
+    }
+    switch (m) {  // uniform per wave
+    case 1: laneFlipBy<E, 1>(w, lower, m); break;
+    case 3: laneFlipBy<E, 3>(w, lower, m); break;
+    case 7: laneFlipBy<E, 7>(w, lower, m); break;
+    case 15: laneFlipBy<E, 15>(w, lower, m); break;
+    default: laneFlipBy<E, 0>(w, lower, m); break;  // 31, 63
     }
 }
 
@@ -287,8 +358,9 @@ __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __rest
             if (rg.y - rg.x > kWaveChunk) longMask |= 1u << v;
         }
     }
-    // 1. One range of up to 2048 entries per wave.
-    const uint32_t tile = tiles[wave];
+    // 1. One range of up to 2048 entries per wave (the tile, its range and size class are
+    // uniform per wave: scalar registers).
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(tiles[wave]);
     if (tile < tileCount) {
         const uint2 range = ranges[tile];
         const uint32_t n = range.y - range.x;
