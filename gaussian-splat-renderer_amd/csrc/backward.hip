@@ -51,7 +51,7 @@ __device__ __forceinline__ uint32_t stageEntry(const float4* splats, uint32_t g,
         const bool hit = a.x + ex >= x0 && a.x - ex <= x0 + 7.0f && a.y + ey >= y0 && a.y - ey <= y0 + 7.0f;
         m |= hit ? (1u << w) : 0u;
     }
-    return m;
+    return exactQuadrants(m, a.x, a.y, b.x, b.y, b.z, r2, tx0, ty0);
 }
 
 // The wave's list: the batch slots whose mask has the wave's bit, in slot order; returns the count.

@@ -308,4 +308,36 @@ __device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, 
     }
 }
 
+// Exact test of a splat against the four 8x8-pixel quadrants of the 16x16 tile at (tx0, ty0)
+// (bit w: quadrant (w & 1, w >> 1)), for the quadrants whose bits are set in `box`. A splat with
+// the positive definite conic (a, b, c) reaches a pixel (q >= log2(1/255)) iff
+// Q = a dx^2 + 2 b dx dy + c dy^2 <= r2 there, r2 = 2 (log2 o - log2(1/255)) / log2 e. The
+// minimum of Q over a quadrant's pixel-centre rectangle is on the edge facing the centre (or 0
+// inside): at x = clamp(cx) with the y minimising Q along that line (dy = -b dx / c), or at
+// y = clamp(cy) likewise. A quadrant stays while that minimum is <= r2 (1 + 2e-3) + 4e-6 M,
+// M the size of the terms of the tile-local quadratic (several times the FP32 rounding of Q
+// and of the kernels' q). NaN keeps a quadrant.
+__device__ __forceinline__ uint32_t exactQuadrants(uint32_t box, float cx, float cy, float a, float b, float c,
+                                                   float r2, float tx0, float ty0) {
+    const float kb = b * __builtin_amdgcn_rcpf(c);
+    const float kc = b * __builtin_amdgcn_rcpf(a);
+    const float Dx = fabsf(cx - tx0) + 16.0f, Dy = fabsf(cy - ty0) + 16.0f;
+    const float M = a * Dx * Dx + 2.0f * fabsf(b) * Dx * Dy + c * Dy * Dy;
+    const float lim = r2 * 1.002f + 4e-6f * M;
+    auto form = [&](float dx, float dy) { return fmaf(dx, fmaf(a, dx, 2.0f * b * dy), c * dy * dy); };
+    uint32_t m = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+        const float x0 = tx0 + 8.0f * float(w & 1u), y0 = ty0 + 8.0f * float(w >> 1);
+        const float x1 = x0 + 7.0f, y1 = y0 + 7.0f;
+        const float dx1 = fminf(fmaxf(cx, x0), x1) - cx;
+        const float dy1 = fminf(fmaxf(fmaf(-kb, dx1, cy), y0), y1) - cy;
+        const float dy2 = fminf(fmaxf(cy, y0), y1) - cy;
+        const float dx2 = fminf(fmaxf(fmaf(-kc, dy2, cx), x0), x1) - cx;
+        const bool far = fminf(form(dx1, dy1), form(dx2, dy2)) > lim;
+        m |= ((box >> w) & 1u) && !far ? (1u << w) : 0u;
+    }
+    return m;
+}
+
 }  // namespace gsr

@@ -382,35 +382,17 @@ __global__ __launch_bounds__(256, 7) void kRenderCulled(const float4* __restrict
                         const float ex = sqrtf(r2 * co.z / det) * 1.001f + 1.0f;
                         const float ey = sqrtf(r2 * co.x / det) * 1.001f + 1.0f;
                         if (ex < 1e30f && ey < 1e30f) {  // also false for NaN
-                            // Then the ellipse itself against the quadrant's pixel-centre rectangle:
-                            // the minimum of Q = a dx^2 + 2 b dx dy + c dy^2 over it is on the edge
-                            // facing the centre (or 0 inside), at x = clamp(cx) with y minimising Q
-                            // along that line (dy = -b dx / c), or at y = clamp(cy) likewise; the
-                            // quadrant is kept while that minimum is <= r2. Q and the render's q
-                            // both round relative to the tile-local quadratic's terms, whose size is
-                            // M: the limit adds 4e-6 M, several times their FP32 rounding.
-                            const float kb = co.y * __builtin_amdgcn_rcpf(co.z);
-                            const float kc = co.y * __builtin_amdgcn_rcpf(co.x);
-                            const float Dx = fabsf(o0.x - float(tx0)) + 16.0f, Dy = fabsf(o0.y - float(ty0)) + 16.0f;
-                            const float M = co.x * Dx * Dx + 2.0f * fabsf(co.y) * Dx * Dy + co.z * Dy * Dy;
-                            const float lim = r2 * 1.002f + 4e-6f * M;
-                            auto form = [&](float dx, float dy) {
-                                return fmaf(dx, fmaf(co.x, dx, 2.0f * co.y * dy), co.z * dy * dy);
-                            };
+                            // Then the ellipse itself (exactQuadrants: the minimum of the form over
+                            // each quadrant's pixel-centre rectangle against r2).
                             uint32_t m = 0u;
 #pragma unroll
                             for (uint32_t w = 0; w < 4; ++w) {
                                 const float x0 = qx0[w & 1u], y0 = qy0[w >> 1];
-                                const float x1 = x0 + 7.0f, y1 = y0 + 7.0f;
-                                const bool hit = o0.x + ex >= x0 && o0.x - ex <= x1 && o0.y + ey >= y0 &&
-                                                 o0.y - ey <= y1;
-                                const float dx1 = fminf(fmaxf(o0.x, x0), x1) - o0.x;
-                                const float dy1 = fminf(fmaxf(fmaf(-kb, dx1, o0.y), y0), y1) - o0.y;
-                                const float dy2 = fminf(fmaxf(o0.y, y0), y1) - o0.y;
-                                const float dx2 = fminf(fmaxf(fmaf(-kc, dy2, o0.x), x0), x1) - o0.x;
-                                const bool far = fminf(form(dx1, dy1), form(dx2, dy2)) > lim;  // NaN: kept
-                                m |= hit && !far ? (1u << w) : 0u;
+                                const bool hit = o0.x + ex >= x0 && o0.x - ex <= x0 + 7.0f && o0.y + ey >= y0 &&
+                                                 o0.y - ey <= y0 + 7.0f;
+                                m |= hit ? (1u << w) : 0u;
                             }
+                            m = exactQuadrants(m, o0.x, o0.y, co.x, co.y, co.z, r2, float(tx0), float(ty0));
                             mask = m;
                         }
                     }
