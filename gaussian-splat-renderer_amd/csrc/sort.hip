@@ -35,6 +35,11 @@ __device__ __forceinline__ uint32_t sortCount(const uint32_t* countDevice, uint3
 
 template <typename Key>
 __device__ __forceinline__ uint32_t digitOf(Key key, uint32_t shift) { return uint32_t(key >> shift) & 0xFFu; }
+// A digit of `bits` <= 8 bits (the reduce-then-scan passes split a key's bits evenly).
+template <typename Key>
+__device__ __forceinline__ uint32_t digitOf(Key key, uint32_t shift, uint32_t bits) {
+    return uint32_t(key >> shift) & ((1u << bits) - 1u);
+}
 
 __device__ __forceinline__ uint64_t packStatus(uint32_t epoch, uint64_t flag, uint32_t value) {
     return (uint64_t(epoch) << 34) | (flag << 32) | uint64_t(value);
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(kSortThreads) void kSortOnesweep(const uint64_t* __
 template <typename Key>
 __global__ __launch_bounds__(kSortThreads) void kSortHistogram(const Key* __restrict__ keys,
                                                                const uint32_t* __restrict__ countDevice,
-                                                               uint32_t maxCount, uint32_t shift,
+                                                               uint32_t maxCount, uint32_t shift, uint32_t bits,
                                                                uint32_t* __restrict__ hist) {
     __shared__ uint32_t sHist[kSortWaves][kRadix];
     const uint32_t t = threadIdx.x, wave = t >> 6;
@@ -229,7 +234,8 @@ __global__ __launch_bounds__(kSortThreads) void kSortHistogram(const Key* __rest
     const uint32_t chunk = (count + G - 1) / G;
     const uint32_t begin = blockIdx.x * chunk;
     const uint32_t end = begin + chunk < count ? begin + chunk : count;
-    for (uint32_t idx = begin + t; idx < end; idx += kSortThreads) atomicAdd(&sHist[wave][digitOf(keys[idx], shift)], 1u);
+    for (uint32_t idx = begin + t; idx < end; idx += kSortThreads)
+        atomicAdd(&sHist[wave][digitOf(keys[idx], shift, bits)], 1u);
     __syncthreads();
     for (uint32_t d = t; d < kRadix; d += kSortThreads) {
         uint32_t s = 0;
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const Key* __restri
                                                              Key* __restrict__ keysOut,
                                                              uint32_t* __restrict__ valsOut,
                                                              const uint32_t* __restrict__ countDevice,
-                                                             uint32_t maxCount, uint32_t shift,
+                                                             uint32_t maxCount, uint32_t shift, uint32_t bits,
                                                              const uint32_t* __restrict__ hist,
                                                              const uint32_t* __restrict__ digitTotals) {
     __shared__ Key sKeys[kSortTile];
@@ -312,12 +318,13 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const Key* __restri
         for (int r = 0; r < kSortKeysPerLane; ++r) {
             const uint32_t pos = wave * (kSortTile / kSortWaves) + r * 64 + lane;
             const bool valid = pos < tileN;
-            const uint32_t d = digitOf(key[r], shift);
+            const uint32_t d = digitOf(key[r], shift, bits);
             dig[r] = d;
-            // Lanes holding the same digit: 8 ballots, one per digit bit.
+            // Lanes holding the same digit: one ballot per digit bit.
             uint64_t peers = __ballot(valid);
 #pragma unroll
             for (int b = 0; b < kRadixBits; ++b) {
+                if (uint32_t(b) >= bits) break;  // uniform
                 const bool bit = (d >> b) & 1u;
                 const uint64_t bal = __ballot(bit);
                 peers &= bit ? bal : ~bal;
@@ -355,7 +362,7 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const Key* __restri
         __syncthreads();
         for (uint32_t i = t; i < tileN; i += kSortThreads) {
             const Key k = sKeys[i];
-            const uint32_t d = digitOf(k, shift);
+            const uint32_t d = digitOf(k, shift, bits);
             const uint32_t dst = sGlobalBase[d] + (i - sLocalStart[d]);
             keysOut[dst] = k;
             valsOut[dst] = sVals[i];
@@ -383,6 +390,12 @@ __global__ __launch_bounds__(kSortThreads) void kCopyPairs(const Key* __restrict
 
 }  // namespace
 
+// Digit width of pass p when `total` key bits are sorted in `passes` passes: the bits split as
+// evenly as the passes allow, the wider digits first (13 bits: 7 + 6; 32: 8 + 8 + 8 + 8). Narrower
+// digits give the scatter longer runs per digit (coalesced stores); any split gives the same
+// stable order.
+static uint32_t passBits(uint32_t total, uint32_t passes, uint32_t p) { return (total + passes - 1u - p) / passes; }
+
 size_t sortStatusBytes(uint32_t maxCount) {
     const size_t tiles = (size_t(maxCount) + kSortTile - 1) / kSortTile;
     return (tiles > 0 ? tiles : 1) * kRadix * sizeof(uint64_t);
@@ -401,21 +414,23 @@ void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint3
 int launchRadixSort32(const SortWork& w, uint32_t* keysA, uint32_t* valuesA, uint32_t* keysB, uint32_t* valuesB,
                       uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, hipStream_t stream) {
     if (maxCount == 0) return 0;
-    uint32_t passes = (keyBits + kRadixBits - 1) / kRadixBits;
-    if (passes > 4) passes = 4;
+    if (keyBits > 32) keyBits = 32;
+    const uint32_t passes = (keyBits + kRadixBits - 1) / kRadixBits;
     uint32_t blocks = (maxCount + kSortTile - 1) / kSortTile;
     if (blocks > kMaxSortBlocks) blocks = kMaxSortBlocks;
     uint32_t* kin = keysA;
     uint32_t* vin = valuesA;
     uint32_t* kout = keysB;
     uint32_t* vout = valuesB;
+    uint32_t shift = 0;
     for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t shift = p * kRadixBits;
+        const uint32_t bits = passBits(keyBits, passes, p);
         hipLaunchKernelGGL(kSortHistogram<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
-                           maxCount, shift, w.blockHist);
+                           maxCount, shift, bits, w.blockHist);
         hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks, w.globalHist);
         hipLaunchKernelGGL(kSortScatter<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout, vout,
-                           countDevice, maxCount, shift, w.blockHist, w.globalHist);
+                           countDevice, maxCount, shift, bits, w.blockHist, w.globalHist);
+        shift += bits;
         uint32_t* tk = kin;
         kin = kout;
         kout = tk;
@@ -439,14 +454,17 @@ int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint6
     if (w.algorithm == kSortAlgReduceThenScan) {
         uint32_t blocks = (maxCount + kSortTile - 1) / kSortTile;
         if (blocks > kMaxSortBlocks) blocks = kMaxSortBlocks;
+        const uint32_t bitsTotal = passes * kRadixBits < keyBits - firstBit ? passes * kRadixBits : keyBits - firstBit;
+        uint32_t shift = firstBit;
         for (uint32_t p = 0; p < passes; ++p) {
-            const uint32_t shift = firstBit + p * kRadixBits;
+            const uint32_t bits = passBits(bitsTotal, passes, p);
             hipLaunchKernelGGL(kSortHistogram<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
-                               maxCount, shift, w.blockHist);
+                               maxCount, shift, bits, w.blockHist);
             hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks,
                                w.globalHist);
             hipLaunchKernelGGL(kSortScatter<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout,
-                               vout, countDevice, maxCount, shift, w.blockHist, w.globalHist);
+                               vout, countDevice, maxCount, shift, bits, w.blockHist, w.globalHist);
+            shift += bits;
             uint64_t* tk = kin;
             kin = kout;
             kout = tk;
