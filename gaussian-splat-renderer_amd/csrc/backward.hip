@@ -26,32 +26,17 @@ constexpr float kLog2MinAlpha = -7.994353436858858f;  // log2(1/255)
 
 // One list entry as staged in LDS: the conic scaled into the log2 domain and log2(opacity),
 // s0 = (cx, cy, -log2(e) A / 2, -log2(e) B), s1 = (-log2(e) C / 2, log2 o), its colour, and the
-// mask of the tile's 8x8 quadrants (bit w: quadrant (w % 2, w / 2)) that the box around its
-// alpha >= 1/255 ellipse meets. The ellipse is d^T K d <= r^2, r^2 = 2 ln(255 o), with
-// half-extents r sqrt(C / det K) and r sqrt(A / det K), widened by 0.1 % and one pixel against
-// rounding; any other conic keeps every quadrant.
+// mask of the tile's 8x8 quadrants (bit w: quadrant (w % 2, w / 2)) in which its alpha can reach
+// 1/255 (quadrantMask: the ellipse's box, then the exact ellipse-vs-quadrant test, both with a
+// margin sized to the rounding of q).
 __device__ __forceinline__ uint32_t stageEntry(const float4* splats, uint32_t g, float tx0, float ty0, float4& s0,
                                                float2& s1, float4& col) {
     const float4 a = splats[3 * size_t(g)], b = splats[3 * size_t(g) + 1], c = splats[3 * size_t(g) + 2];
     s0 = make_float4(a.x, a.y, -0.5f * kLog2e * b.x, -kLog2e * b.y);
     s1 = make_float2(-0.5f * kLog2e * b.z, __log2f(b.w));
     col = c;
-    const float det = b.x * b.z - b.y * b.y;
     const float r2 = 2.0f * (s1.y - kLog2MinAlpha) * (1.0f / kLog2e);
-    if (!(det > 0.0f && b.x > 0.0f && b.z > 0.0f)) return 0xFu;
-    if (r2 < 0.0f) return 0u;  // alpha < 1/255 everywhere
-    const float inv = __builtin_amdgcn_rcpf(det);  // hardware rcp and sqrt: inside the widening
-    const float ex = __builtin_amdgcn_sqrtf(r2 * b.z * inv) * 1.001f + 1.0f;
-    const float ey = __builtin_amdgcn_sqrtf(r2 * b.x * inv) * 1.001f + 1.0f;
-    if (!(ex < 1e30f && ey < 1e30f)) return 0xFu;
-    uint32_t m = 0u;
-#pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) {
-        const float x0 = tx0 + 8.0f * float(w & 1u), y0 = ty0 + 8.0f * float(w >> 1);
-        const bool hit = a.x + ex >= x0 && a.x - ex <= x0 + 7.0f && a.y + ey >= y0 && a.y - ey <= y0 + 7.0f;
-        m |= hit ? (1u << w) : 0u;
-    }
-    return exactQuadrants(m, a.x, a.y, b.x, b.y, b.z, r2, tx0, ty0);
+    return quadrantMask(a.x, a.y, b.x, b.y, b.z, r2, tx0, ty0);
 }
 
 // The wave's list: the batch slots whose mask has the wave's bit, in slot order; returns the count.

@@ -308,34 +308,48 @@ __device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, 
     }
 }
 
-// Exact test of a splat against the four 8x8-pixel quadrants of the 16x16 tile at (tx0, ty0)
-// (bit w: quadrant (w & 1, w >> 1)), for the quadrants whose bits are set in `box`. A splat with
-// the positive definite conic (a, b, c) reaches a pixel (q >= log2(1/255)) iff
-// Q = a dx^2 + 2 b dx dy + c dy^2 <= r2 there, r2 = 2 (log2 o - log2(1/255)) / log2 e. The
-// minimum of Q over a quadrant's pixel-centre rectangle is on the edge facing the centre (or 0
-// inside): at x = clamp(cx) with the y minimising Q along that line (dy = -b dx / c), or at
-// y = clamp(cy) likewise. A quadrant stays while that minimum is <= r2 (1 + 2e-3) + 4e-6 M,
-// M the size of the terms of the tile-local quadratic (several times the FP32 rounding of Q
-// and of the kernels' q). NaN keeps a quadrant.
-__device__ __forceinline__ uint32_t exactQuadrants(uint32_t box, float cx, float cy, float a, float b, float c,
-                                                   float r2, float tx0, float ty0) {
-    const float kb = b * __builtin_amdgcn_rcpf(c);
-    const float kc = b * __builtin_amdgcn_rcpf(a);
+// The quadrants (8x8 pixel blocks of the 16x16 tile at (tx0, ty0); bit w: quadrant
+// (w & 1, w >> 1)) in which a splat can reach a pixel. With the conic (a, b, c) the render takes a
+// pixel iff its q >= log2(1/255), i.e. iff Q = a dx^2 + 2 b dx dy + c dy^2 <= r2,
+// r2 = 2 (log2 o - log2(1/255)) / log2 e, up to the FP32 rounding of q. That rounding is relative
+// to the size of the terms of the tile-local quadratic the render evaluates, M (centres far from
+// the tile and nearly singular conics make those terms large and cancelling), so every test here
+// is against lim = r2 + 2e-3 |r2| + 1e-4 + 4e-6 M, several times that rounding: a quadrant is
+// dropped only when no pixel centre in it has Q <= lim. The tests:
+//   - lim < 0: no pixel anywhere;
+//   - the ellipse's bounding box (half-extents sqrt(lim c / det), sqrt(lim a / det), widened by
+//     0.1 % and one pixel; det = a c - b^2 from an exact product so that it does not cancel);
+//   - the minimum of Q over the quadrant's pixel-centre rectangle, which is on the edge facing
+//     the centre (or 0 inside): at x = clamp(cx) with the y minimising Q along that line
+//     (dy = -b dx / c), or at y = clamp(cy) likewise.
+// A conic that is not positive definite, and NaN anywhere, keep every quadrant.
+__device__ __forceinline__ uint32_t quadrantMask(float cx, float cy, float a, float b, float c, float r2, float tx0,
+                                                 float ty0) {
+    const float bb = b * b;
+    const float det = fmaf(a, c, -bb) - fmaf(b, b, -bb);  // a c - b^2: b^2 = bb + the exact rounding error
+    if (!(det > 0.0f && a > 0.0f && c > 0.0f)) return 0xFu;
     const float Dx = fabsf(cx - tx0) + 16.0f, Dy = fabsf(cy - ty0) + 16.0f;
     const float M = a * Dx * Dx + 2.0f * fabsf(b) * Dx * Dy + c * Dy * Dy;
-    const float lim = r2 * 1.002f + 4e-6f * M;
+    const float lim = r2 + 2e-3f * fabsf(r2) + 1e-4f + 4e-6f * M;
+    if (lim < 0.0f) return 0u;
+    const float ex = sqrtf(lim * c / det) * 1.001f + 1.0f;
+    const float ey = sqrtf(lim * a / det) * 1.001f + 1.0f;
+    if (!(ex < 1e30f && ey < 1e30f && lim < 1e30f)) return 0xFu;  // also false for NaN
+    const float kb = b * __builtin_amdgcn_rcpf(c);
+    const float kc = b * __builtin_amdgcn_rcpf(a);
     auto form = [&](float dx, float dy) { return fmaf(dx, fmaf(a, dx, 2.0f * b * dy), c * dy * dy); };
     uint32_t m = 0u;
 #pragma unroll
     for (uint32_t w = 0; w < 4; ++w) {
         const float x0 = tx0 + 8.0f * float(w & 1u), y0 = ty0 + 8.0f * float(w >> 1);
         const float x1 = x0 + 7.0f, y1 = y0 + 7.0f;
+        const bool box = cx + ex >= x0 && cx - ex <= x1 && cy + ey >= y0 && cy - ey <= y1;
         const float dx1 = fminf(fmaxf(cx, x0), x1) - cx;
         const float dy1 = fminf(fmaxf(fmaf(-kb, dx1, cy), y0), y1) - cy;
         const float dy2 = fminf(fmaxf(cy, y0), y1) - cy;
         const float dx2 = fminf(fmaxf(fmaf(-kc, dy2, cx), x0), x1) - cx;
-        const bool far = fminf(form(dx1, dy1), form(dx2, dy2)) > lim;
-        m |= ((box >> w) & 1u) && !far ? (1u << w) : 0u;
+        const bool far = fminf(form(dx1, dy1), form(dx2, dy2)) > lim;  // NaN: kept
+        m |= box && !far ? (1u << w) : 0u;
     }
     return m;
 }

@@ -337,9 +337,6 @@ __global__ __launch_bounds__(256, 7) void kRenderCulled(const float4* __restrict
     const uint32_t px = tx0 + (wave & 1u) * 8 + (lane & 7u);
     const uint32_t py = ty0 + (wave >> 1) * 8 + (lane >> 3);
     const PixelQ pq = pixelQ(px - tx0, py - ty0);
-    // Quadrant corners (pixel coordinates of the first and last pixel centres), per wave bit.
-    const float qx0[2] = {float(tx0), float(tx0 + 8)};
-    const float qy0[2] = {float(ty0), float(ty0 + 8)};
     if (t == 0) {
         sRec[kBatch].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         sRec[kBatch].s1 = make_float2(0.0f, -__builtin_inff());
@@ -368,35 +365,10 @@ __global__ __launch_bounds__(256, 7) void kRenderCulled(const float4* __restrict
                 sRec[t].col = col;
                 const float logO = __log2f(co.w);
                 tileQuadratic(o0.x, o0.y, co, float(tx0), float(ty0), sRec[t].s0, sRec[t].s1);
-                // For a positive definite conic, q >= log2(1/255) <=> a dx^2 + 2 b dx dy + c dy^2 <= r2,
-                // r2 = 2 (log2 o - log2(1/255)) / log2 e, so |dx| <= sqrt(r2 c / det) and
-                // |dy| <= sqrt(r2 a / det), det = a c - b^2. The box is widened by 0.1% and one
-                // pixel, far more than the rounding of q. Any other conic keeps every quadrant.
+                // The quadrants the splat can reach (quadrantMask: box, then the exact
+                // ellipse-vs-quadrant test, both with a margin sized to the rounding of q).
                 const float r2 = 2.0f * (logO - kLog2MinAlpha) * (1.0f / kLog2e);
-                const float det = co.x * co.z - co.y * co.y;
-                mask = 0xFu;
-                if (det > 0.0f && co.x > 0.0f && co.z > 0.0f) {
-                    if (r2 < 0.0f) {
-                        mask = 0u;  // q <= log2 o < log2(1/255) everywhere
-                    } else {
-                        const float ex = sqrtf(r2 * co.z / det) * 1.001f + 1.0f;
-                        const float ey = sqrtf(r2 * co.x / det) * 1.001f + 1.0f;
-                        if (ex < 1e30f && ey < 1e30f) {  // also false for NaN
-                            // Then the ellipse itself (exactQuadrants: the minimum of the form over
-                            // each quadrant's pixel-centre rectangle against r2).
-                            uint32_t m = 0u;
-#pragma unroll
-                            for (uint32_t w = 0; w < 4; ++w) {
-                                const float x0 = qx0[w & 1u], y0 = qy0[w >> 1];
-                                const bool hit = o0.x + ex >= x0 && o0.x - ex <= x0 + 7.0f && o0.y + ey >= y0 &&
-                                                 o0.y - ey <= y0 + 7.0f;
-                                m |= hit ? (1u << w) : 0u;
-                            }
-                            m = exactQuadrants(m, o0.x, o0.y, co.x, co.y, co.z, r2, float(tx0), float(ty0));
-                            mask = m;
-                        }
-                    }
-                }
+                mask = quadrantMask(o0.x, o0.y, co.x, co.y, co.z, r2, float(tx0), float(ty0));
             }
             sMask[t] = mask;
         }

@@ -171,16 +171,31 @@ def test_rgba8_within_one_level(gsr_ctx, exact_alpha, ppt, loop):
     scene.close()
 
 
-@pytest.mark.parametrize("case", ["100k-720p", "test-scene-640x360"])
+@pytest.mark.parametrize("case", ["100k-720p", "test-scene-640x360", "needles-giants-1280x720"])
 def test_culled_loop_equals_uncompacted_loop(gsr_ctx, case):
-    """The quadrant-culled loop drops only pairs below the alpha cutoff: its RGBA32F image equals
-    the uncompacted predicated loop's bit for bit (same q, same blend order)."""
+    """The quadrant-culled loop drops only pairs below the alpha cutoff (a box, then the exact
+    ellipse-vs-quadrant test with a margin sized to the FP32 rounding of q): its RGBA32F image
+    equals the uncompacted predicated loop's bit for bit (same q, same blend order). The
+    needles-and-giants scene stresses that margin: one long axis (up to e^0.5) and two tiny ones
+    (e^-7) at every rotation, so the projected conics are nearly singular and the quadratic's terms
+    cancel, 1 % large isotropic splats near the camera, opacities mostly near 1."""
     import gsr
     from gsr import camera, scenes
 
     log(f"case culled vs predicated, {case}")
     if case == "100k-720p":
         sc, view, W, H = scenes.make_scene(100_000, seed=0, mean_sigma=2.0), camera.orbit_view(8.0, 0.0, 0.0), 1280, 720
+    elif case == "needles-giants-1280x720":
+        sc = scenes.make_scene(20_000, seed=3, mean_sigma=1.5)
+        rng = np.random.default_rng(3)
+        n = sc.count
+        sig = np.full((n, 3), np.exp(-7.0), np.float32)
+        sig[:, 0] = np.exp(rng.uniform(-2.0, 0.5, n))
+        giant = rng.random(n) < 0.01
+        sig[giant] = np.exp(rng.uniform(-1.0, 0.7, (int(giant.sum()), 1))).astype(np.float32)
+        sc.scales[:, :3] = sig
+        sc.pos_alpha[:, 3] = (1.0 / (1.0 + np.exp(-rng.normal(3.0, 1.0, n)))).astype(np.float32)
+        view, W, H = camera.orbit_view(6.0, 10.0, 30.0), 1280, 720
     else:
         sc, view, W, H = scenes.make_test_scene(250_000, seed=99), camera.look_at([0.3, 0.5, 6.0], [0, 0, 0]), 640, 360
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
@@ -590,6 +605,22 @@ def test_sort_pairs_stable(gsr_ctx, n, algorithm):
         order = np.argsort(keys, kind="stable")
         assert np.array_equal(ok, keys[order]), f"{name}: keys"
         assert np.array_equal(ov, vals[order]), f"{name}: values (stability)"
+
+
+@pytest.mark.parametrize("key_bits", [9, 13, 21, 33])
+def test_sort_pairs_uneven_digits(gsr_ctx, key_bits):
+    """The reduce-then-scan passes split the key bits evenly (9: 5 + 4, 13: 7 + 6, 21: 7 + 7 + 7,
+    33: 7 + 7 + 7 + 6 + 6): a stable sort whatever the split, keys using every bit."""
+    import gsr
+
+    n = 100_003
+    rng = np.random.default_rng(key_bits)
+    keys = rng.integers(0, 1 << key_bits, n, dtype=np.uint64)
+    keys[:16] = (1 << key_bits) - 1  # the top digit value of every pass
+    vals = np.arange(n, dtype=np.uint32)
+    ok, ov = gsr.sort_pairs(gsr_ctx, keys, vals, key_bits=key_bits, algorithm="reduce-then-scan")
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(ok, keys[order]) and np.array_equal(ov, vals[order])
 
 
 @pytest.mark.parametrize("algorithm", ["reduce-then-scan", "onesweep"])
