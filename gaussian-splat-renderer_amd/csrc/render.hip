@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
 
 // Culled predicated loop (FAST, one pixel per thread, 8x8 quadrant per wave).
 template <int FORMAT, int RS>
-__global__ __launch_bounds__(256, 7) void kRenderCulled(const float4* __restrict__ splats,
+__global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict__ splats,
                                                      const uint32_t* __restrict__ packed,
                                                      const uint32_t* __restrict__ values,
                                                      const uint2* __restrict__ ranges, uint32_t width,
