@@ -36,14 +36,17 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     __shared__ uint32_t sScan[kDupThreads];
     __shared__ uint32_t sWave[kDupThreads / 64];
     // Per gaussian: first row, row count, depth bits, and either the rectangle's x0/width
-    // (one span for every row) or the first kCachedRows row spans (x0 | width << 16); an entry
-    // in a row past them rebuilds the gaussian's coverage from its splat (rare: long splats),
-    // which keeps the LDS small enough for 10 workgroups per CU.
+    // (one span for every row) or the first kCachedRows row spans (x0 | width << 16). For the
+    // rows past them (long splats) the binning form also keeps the ellipse: its blocks hold
+    // depth-ordered splats, so the near, long ones come together and rebuilding their coverage
+    // per entry costs more than the LDS. The global form (blocks in index order, long splats
+    // spread out) rebuilds it from the splat, which keeps its LDS small enough for 8 waves/SIMD.
     __shared__ int sRow0[kDupThreads];
     __shared__ int sRows[kDupThreads];
     __shared__ uint32_t sDepth[kDupThreads];
     __shared__ uint32_t sSpan[kCachedRows][kDupThreads];
     __shared__ int sBinX[2][kDupThreads];
+    __shared__ float sEl[BINNING ? 9 : 1][BINNING ? kDupThreads : 1];  // cx cy A B C D t exPad eyPad
 
     const uint32_t t = threadIdx.x;
     const uint32_t j = blockIdx.x * kDupThreads + t;
@@ -89,6 +92,19 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
             sBinX[1][t] = cv.bins.x1 - cv.bins.x0;
             sRows[t] = -(r1 - r0) - 1;  // negative: a rectangle
         } else {
+            if constexpr (BINNING) {
+                sEl[0][t] = cv.e.cx;
+                sEl[1][t] = cv.e.cy;
+                sEl[2][t] = cv.e.A;
+                sEl[3][t] = cv.e.B;
+                sEl[4][t] = cv.e.C;
+                sEl[5][t] = cv.e.D;
+                sEl[6][t] = cv.e.t;
+                sEl[7][t] = cv.e.exPad;
+                sEl[8][t] = cv.e.eyPad;
+                sBinX[0][t] = cv.bins.x0;
+                sBinX[1][t] = cv.bins.x1;
+            }
             // Walk the rows once: cache the first spans, and check the count/write
             // agreement (tile_duplicate.comp:375-376) from the same walk.
             uint32_t sum = 0;
@@ -135,12 +151,28 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                 before += w;
             }
             if (tile == gridTiles && row == kCachedRows) {  // a long splat: walk the remaining rows
-                // of its coverage, rebuilt from the splat as above (same inputs, same bits)
-                const float4* s = splats + 3 * size_t(sIndex[lo]);
-                const float4 o0 = s[0], o1 = s[1], o2 = s[2];
-                const Rect rect = tileRect(o0.x, o0.y, o2.w, tilesX, tilesY);
-                const Coverage cv = makeCoverage(bounds, rect, int(tilesX), int(tilesY), o0.x, o0.y, o1.x, o1.y,
-                                                 o1.z, o1.w);
+                // of its coverage (binning: the cached ellipse; global: rebuilt from the splat as
+                // above, the same inputs and so the same bits)
+                Coverage cv;
+                if constexpr (BINNING) {
+                    cv.bins = Rect{sBinX[0][lo], 0, sBinX[1][lo], 0};
+                    cv.useEllipse = true;
+                    cv.e.cx = sEl[0][lo];
+                    cv.e.cy = sEl[1][lo];
+                    cv.e.A = sEl[2][lo];
+                    cv.e.B = sEl[3][lo];
+                    cv.e.C = sEl[4][lo];
+                    cv.e.D = sEl[5][lo];
+                    cv.e.t = sEl[6][lo];
+                    cv.e.exPad = sEl[7][lo];
+                    cv.e.eyPad = sEl[8][lo];
+                    cv.e.valid = true;
+                } else {
+                    const float4* s = splats + 3 * size_t(sIndex[lo]);
+                    const float4 o0 = s[0], o1 = s[1], o2 = s[2];
+                    const Rect rect = tileRect(o0.x, o0.y, o2.w, tilesX, tilesY);
+                    cv = makeCoverage(bounds, rect, int(tilesX), int(tilesY), o0.x, o0.y, o1.x, o1.y, o1.z, o1.w);
+                }
                 for (int rr = r0 + kCachedRows; rr < r0 + rows; ++rr) {
                     int x0, x1;
                     coverageRowSpan(cv, rr, &x0, &x1);
