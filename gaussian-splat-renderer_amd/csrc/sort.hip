@@ -234,8 +234,18 @@ __global__ __launch_bounds__(kSortThreads) void kSortHistogram(const Key* __rest
     const uint32_t chunk = (count + G - 1) / G;
     const uint32_t begin = blockIdx.x * chunk;
     const uint32_t end = begin + chunk < count ? begin + chunk : count;
-    for (uint32_t idx = begin + t; idx < end; idx += kSortThreads)
-        atomicAdd(&sHist[wave][digitOf(keys[idx], shift, bits)], 1u);
+    // four keys in flight per thread, then their counts
+    for (uint32_t idx = begin + t; idx < end; idx += 4 * kSortThreads) {
+        Key k[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t e = idx + uint32_t(u) * kSortThreads;
+            k[u] = e < end ? keys[e] : Key(0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (idx + uint32_t(u) * kSortThreads < end) atomicAdd(&sHist[wave][digitOf(k[u], shift, bits)], 1u);
+    }
     __syncthreads();
     for (uint32_t d = t; d < kRadix; d += kSortThreads) {
         uint32_t s = 0;
