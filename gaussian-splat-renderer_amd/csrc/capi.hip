@@ -11,9 +11,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 using namespace gsr;
@@ -207,10 +210,30 @@ struct gsr_renderer {
     bool haveFrame = false;
     int sortedIn = 0;  // 0 = A, 1 = B
     hipStream_t lastStream = nullptr;
+    hipEvent_t order = nullptr;  // marks the last frame on lastStream (frames on another stream wait on it)
     hipEvent_t events[GSR_STAGE_COUNT] = {};
     bool eventsRecorded = false;
     gsr_colour_model* colour = nullptr;  // learned colour, or null (SH)
+    // A captured batch holds raw pointers to this renderer's buffers: `id` names the renderer for
+    // its lifetime (never reused) and `generation` goes up whenever a buffer a frame writes or
+    // reads is reallocated or the colour binding changes, so a replay can refuse stale graphs.
+    uint64_t id = 0;
+    uint64_t generation = 0;
 };
+
+namespace {
+// Live renderers (id by address), so that a replay can tell a destroyed renderer from a live one,
+// even when a new renderer was allocated at the same address.
+std::mutex g_liveMutex;
+std::unordered_map<const gsr_renderer*, uint64_t> g_liveRenderers;
+std::atomic<uint64_t> g_nextRendererId{1};
+
+bool rendererLive(const gsr_renderer* r, uint64_t id) {
+    std::lock_guard<std::mutex> lock(g_liveMutex);
+    const auto it = g_liveRenderers.find(r);
+    return it != g_liveRenderers.end() && it->second == id;
+}
+}  // namespace
 
 extern "C" {
 
@@ -538,6 +561,15 @@ static hipError_t allocateList(gsr_renderer* r, uint32_t capacity, hipStream_t s
     return e;
 }
 
+// Makes `stream` wait for the renderer's last frame when that frame ran on another stream. The
+// frame's end is an event recorded on its own stream (never a handle to that stream, which may be
+// gone by now). Frames recorded into a capture record no event, so a replayed graph is ordered by
+// the stream it is launched on.
+static hipError_t orderAfterLastFrame(gsr_renderer* r, hipStream_t stream, bool capturing) {
+    if (capturing || !r->lastStream || r->lastStream == stream) return hipSuccess;
+    return hipStreamWaitEvent(stream, r->order, 0);
+}
+
 // Collects the frame statistics whose copies have completed (lastStats = the newest).
 static void collectStats(gsr_renderer* r) {
     for (int s = 0; s < gsr_renderer::kStatSlots; ++s) {
@@ -579,20 +611,13 @@ static hipError_t adaptCapacity(gsr_renderer* r, hipStream_t stream) {
     const uint32_t minCap = std::min<uint32_t>(256u * 1024u, r->maxCapacity);
     const uint32_t newCap = std::clamp(roundUpToBlock(target), minCap, r->maxCapacity);
     if (newCap == cap || (newCap < cap && double(newCap) > 0.75 * cap)) return hipSuccess;
-    // The frames in flight may still read the old list: its release is ordered after them.
-    if (r->lastStream && r->lastStream != stream) {
-        hipEvent_t ev;
-        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-        if (e != hipSuccess) return e;
-        e = hipEventRecord(ev, r->lastStream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev, 0);
-        (void)hipEventDestroy(ev);
-        if (e != hipSuccess) return e;
-    }
+    // The frames in flight may still read the old list: its release is ordered after them
+    // (gsr_render has made `stream` wait for the renderer's last frame).
     for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB, &r->sortStatus}) {
         const hipError_t e = b->releaseAsync(stream);
         if (e != hipSuccess) return e;
     }
+    ++r->generation;
     const hipError_t e = allocateList(r, newCap, stream);
     if (e == hipSuccess) {
         ++r->listResizes;
@@ -686,12 +711,18 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
     for (int i = 0; i < GSR_STAGE_COUNT && e == hipSuccess; ++i) e = hipEventCreate(&r->events[i]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->order, hipEventDisableTiming);
     if (s.adaptive_list || s.frame_stats) {
         if (e == hipSuccess)
             e = hipHostMalloc(reinterpret_cast<void**>(&r->statHost), sizeof(gsr_control) * gsr_renderer::kStatSlots,
                               hipHostMallocDefault);
         for (int i = 0; i < gsr_renderer::kStatSlots && e == hipSuccess; ++i)
             e = hipEventCreateWithFlags(&r->statEvent[i], hipEventDisableTiming);
+    }
+    r->id = g_nextRendererId.fetch_add(1);
+    {
+        std::lock_guard<std::mutex> lock(g_liveMutex);
+        g_liveRenderers[r] = r->id;
     }
     if (e != hipSuccess) {
         gsr_renderer_destroy(r);
@@ -703,8 +734,13 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
 
 void gsr_renderer_destroy(gsr_renderer* r) {
     if (!r) return;
+    {
+        std::lock_guard<std::mutex> lock(g_liveMutex);
+        g_liveRenderers.erase(r);
+    }
     (void)hipSetDevice(r->ctx->device);
     (void)hipDeviceSynchronize();
+    if (r->order) (void)hipEventDestroy(r->order);
     for (hipEvent_t& ev : r->events)
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t& ev : r->statEvent)
@@ -751,17 +787,25 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_render: camera tiles do not match its size (use gsr_camera_make)");
     GSR_HIP(hipSetDevice(r->ctx->device));
     hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : r->ctx->stream;
+    hipStreamCaptureStatus captureStatus = hipStreamCaptureStatusNone;
+    GSR_HIP(hipStreamIsCapturing(stream, &captureStatus));
+    const bool capturing = captureStatus != hipStreamCaptureStatusNone;
+    // A frame reuses the previous frame's buffers: on another stream it waits for that frame
+    // (a captured batch is ordered by its own fork event instead).
+    GSR_HIP(orderAfterLastFrame(r, stream, capturing));
     const gsr_scene* sc = r->scene;
     const uint32_t tiles = cam->tiles_x * cam->tiles_y;
     // Per-size buffers (onRenderTargetsChanged, splat_tile_renderer.cpp:537-570).
     if (tiles > r->rangesTiles) {
         GSR_HIP(hipStreamSynchronize(stream));
+        ++r->generation;
         r->rangesTiles = 0;  // no buffer until the allocation succeeds (a failed one is retried)
         GSR_HIP(r->ranges.allocate(size_t(tiles) * 8));
         r->rangesTiles = tiles;
     }
     if (r->settings.differentiable && size_t(cam->width) * cam->height > r->pixelStatePixels) {
         GSR_HIP(hipStreamSynchronize(stream));
+        ++r->generation;
         r->pixelStatePixels = 0;
         GSR_HIP(r->pixelState.allocate(size_t(cam->width) * cam->height * 8));
         r->pixelStatePixels = size_t(cam->width) * cam->height;
@@ -793,6 +837,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     const uint32_t preBlocks = (std::max(sc->count, rangeWords) + kPreprocessThreads - 1) / kPreprocessThreads;
     if (r->blockStats.bytes < size_t(preBlocks) * sizeof(BlockStats)) {
         GSR_HIP(hipStreamSynchronize(stream));
+        ++r->generation;
         GSR_HIP(r->blockStats.allocate(size_t(preBlocks) * sizeof(BlockStats)));
     }
     DevCamera dc{};
@@ -935,6 +980,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         r->statFrame[slot] = r->frameIndex;
         r->statPending[slot] = true;
     }
+    if (!capturing) GSR_HIP(hipEventRecord(r->order, stream));
     ++r->frameIndex;
     r->lastCam = *cam;
     r->haveFrame = true;
@@ -1100,6 +1146,15 @@ struct gsr_batch {
     hipStream_t captureStream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // What the graph was recorded against: each branch renderer, its id and generation, and
+    // whether its frames ran the learned-colour pass. A replay refuses when any of them changed.
+    struct Captured {
+        const gsr_renderer* renderer;
+        uint64_t id, generation;
+        const gsr_colour_model* colour;
+        bool colourPass;
+    };
+    std::vector<Captured> captured;
 };
 
 namespace {
@@ -1109,7 +1164,10 @@ void releaseGraph(gsr_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+    b->captured.clear();
 }
+
+bool colourPassOn(const gsr_renderer* r) { return r->colour && r->colour->mlpCount > 0; }
 
 gsr_status checkBatchArgs(gsr_batch* b, gsr_renderer* const* renderers, uint32_t viewCount, const gsr_camera* cams,
                           void* const* outImages, const char* what) {
@@ -1212,6 +1270,12 @@ gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint3
             return fail(GSR_ERROR_INVALID_ARGUMENT,
                         "gsr_batch_capture: renderer %u reads back statistics or stage times (adaptive_list, "
                         "frame_stats, profile_stages); a captured batch takes plain renderers", i);
+        // The one-sweep sort tags its look-back status words with a host-side epoch, which a
+        // graph would freeze: a replay could read the previous replay's words as current.
+        if (!r->binning && r->settings.sort_algorithm == GSR_SORT_ONESWEEP)
+            return fail(GSR_ERROR_INVALID_ARGUMENT,
+                        "gsr_batch_capture: renderer %u uses the one-sweep sort, whose status epochs cannot be "
+                        "captured; use the reduce-then-scan sort", i);
     }
     // Every view's size must have been rendered by its branch's renderer before (a new size
     // allocates and synchronises, which a capture cannot record).
@@ -1245,12 +1309,28 @@ gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint3
         releaseGraph(b);
         return hipFail(ei, "gsr_batch_capture: hipGraphInstantiate");
     }
+    for (uint32_t i = 0; i < used; ++i) {
+        const gsr_renderer* r = renderers[i];
+        b->captured.push_back({r, r->id, r->generation, r->colour, colourPassOn(r)});
+    }
     return GSR_OK;
 }
 
 gsr_status gsr_batch_replay(gsr_batch* b, void* streamArg) {
     if (!b) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_replay: null batch");
     if (!b->exec) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_replay: nothing captured");
+    // The graph holds raw pointers into each renderer: refuse (before launching anything) when a
+    // renderer was destroyed, reallocated a buffer, or changed its colour binding since the capture.
+    for (size_t i = 0; i < b->captured.size(); ++i) {
+        const gsr_batch::Captured& c = b->captured[i];
+        if (!rendererLive(c.renderer, c.id))
+            return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_replay: renderer %zu of the capture was destroyed", i);
+        if (c.renderer->generation != c.generation || c.renderer->colour != c.colour ||
+            colourPassOn(c.renderer) != c.colourPass)
+            return fail(GSR_ERROR_INVALID_ARGUMENT,
+                        "gsr_batch_replay: renderer %zu changed since the capture (a larger view reallocated its "
+                        "buffers, or its colour model or fallback marks changed): capture the batch again", i);
+    }
     GSR_HIP(hipSetDevice(b->ctx->device));
     GSR_HIP(hipGraphLaunch(b->exec, streamArg ? static_cast<hipStream_t>(streamArg) : b->ctx->stream));
     return GSR_OK;
@@ -1523,6 +1603,7 @@ gsr_status gsr_renderer_set_colour_model(gsr_renderer* r, gsr_colour_model* m) {
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_set_colour_model: the model belongs to another scene");
     if (m && r->settings.differentiable)
         return fail(GSR_ERROR_UNSUPPORTED, "gsr_renderer_set_colour_model: a differentiable renderer colours from SH");
+    if (r->colour != m) ++r->generation;
     r->colour = m;
     return GSR_OK;
 }
@@ -1550,14 +1631,7 @@ gsr_status gsr_backward(gsr_renderer* r, const void* gradImage, const gsr_gradie
     SceneRef ref = sc->ref;
     ref.degree = r->degree;
     // The frame's stream may differ from this one: order after it.
-    if (r->lastStream && r->lastStream != stream) {
-        hipEvent_t ev;
-        GSR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        hipError_t e = hipEventRecord(ev, r->lastStream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev, 0);
-        (void)hipEventDestroy(ev);
-        GSR_HIP(e);
-    }
+    GSR_HIP(orderAfterLastFrame(r, stream, false));
     GSR_HIP(hipMemsetAsync(r->grad2D.ptr, 0, size_t(sc->count) * 4 * size_t(grad2DComponents()), stream));
     const uint32_t* values = r->sortedIn ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
     launchRenderBackward(r->splats.as<float4>(), values, r->ranges.as<uint2>(), dc, static_cast<const float4*>(gradImage),

@@ -2,21 +2,27 @@
 // radix_histogram.comp, radix_scan.comp, radix_scatter_reorder.comp): stable LSD sort of
 // (64-bit key, u32 value) pairs by 8-bit digits.
 //
-// gfx950 design, one sweep per digit:
-//   - the key count lives on the device (the frame's entries = min(M, capacity)): every
-//     kernel reads it there; grids are sized for the capacity and idle blocks exit;
-//   - one up-front pass reads the keys once and builds the global histograms of every
-//     digit position (LDS, then one atomic per block and bin);
-//   - each digit pass is ONE kernel over tiles of 2048 keys. Each wave64 owns 512
-//     consecutive keys of the tile and ranks them with 8 64-lane ballots per key (the lanes
-//     holding the same digit) and a running per-wave digit counter in LDS, so ranks follow
-//     the input order (stable) with no workgroup barrier inside the wave's rounds. The tile's
-//     per-digit offset among all earlier tiles comes from a decoupled look-back over
-//     per-(tile, digit) status words (aggregate / inclusive prefix, tagged with the pass's
-//     epoch, so the status array never needs clearing). Tiles take their index from an
-//     atomic ticket, so a tile only waits on tiles that are already running; the wait is
-//     bounded and a timeout is counted, never hung on. The tile is then reordered by digit
-//     in LDS and written out as contiguous per-digit runs (coalesced stores).
+// gfx950 design. The key count lives on the device (the frame's entries = min(M, capacity)):
+// every kernel reads it there; grids are sized for the capacity and idle blocks exit.
+//
+// Reduce-then-scan (the default, GSR_SORT_REDUCE_THEN_SCAN): per digit of at most 8 bits (a
+// key's bits split evenly over the passes) three kernels:
+//   - kSortHistogram: each block counts the digits of its contiguous chunk of keys in LDS and
+//     writes one count per (digit, block);
+//   - kSortScan: one workgroup per digit scans that digit's block counts;
+//   - kSortScatter: each wave64 ranks its keys with one 64-lane ballot per digit bit (the
+//     lanes holding the same digit) and a running per-wave digit counter in LDS, so ranks
+//     follow the input order (stable); the block's tile of 2048 keys is reordered by digit in
+//     LDS and written out as contiguous per-digit runs (coalesced stores).
+//
+// One-sweep (GSR_SORT_ONESWEEP, 64-bit keys only): one up-front pass builds the global
+// histograms of every digit position; then ONE kernel per digit over tiles of 2048 keys ranks
+// as above and takes the tile's per-digit offset among the earlier tiles from a decoupled
+// look-back over per-(tile, digit) status words (aggregate / inclusive prefix, tagged with the
+// pass's epoch, so the status array never needs clearing). Tiles take their index from an
+// atomic ticket, so a tile only waits on tiles that are already running; the wait is bounded
+// and a timeout is counted, never hung on. Slower here than reduce-then-scan: the look-back
+// crosses the XCDs' separate L2s.
 #include "gsr_internal.hpp"
 
 namespace gsr {

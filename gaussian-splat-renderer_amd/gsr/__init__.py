@@ -696,6 +696,52 @@ class Renderer:
             pass
 
 
+_HIP_RUNTIME = None
+
+
+def hip_runtime():
+    """The HIP runtime libgsr.so runs on (its SONAME, already loaded by the library), for callers
+    that need their own streams or events next to the library's."""
+    global _HIP_RUNTIME
+    if _HIP_RUNTIME is None:
+        try:
+            _HIP_RUNTIME = ctypes.CDLL("libamdhip64.so.7")
+        except OSError:
+            _HIP_RUNTIME = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+    return _HIP_RUNTIME
+
+
+class Stream:
+    """A non-blocking HIP stream of default priority on the context's device (frames on several
+    streams, frames in flight). `handle` is what Renderer.render and Batch take as `stream`."""
+
+    def __init__(self, ctx: Context):
+        hip = hip_runtime()
+        if hip.hipSetDevice(ctx.device) != 0:
+            raise GsrError("hipSetDevice failed")
+        h = ctypes.c_void_p()
+        if hip.hipStreamCreateWithFlags(ctypes.byref(h), 1) != 0:
+            raise GsrError("hipStreamCreateWithFlags failed")
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        if hip_runtime().hipStreamSynchronize(ctypes.c_void_p(self.handle)) != 0:
+            raise GsrError("hipStreamSynchronize failed")
+
+    def close(self) -> None:
+        if self.handle:
+            hip = hip_runtime()
+            hip.hipStreamSynchronize(ctypes.c_void_p(self.handle))
+            hip.hipStreamDestroy(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Many views of one scene on `branches` streams at once (gsr_batch_*): view v is rendered by
     renderers[v % branches] on branch stream v % branches, forked from and joined back to the

@@ -255,7 +255,9 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r);  /* device memory owne
 /* SplatTileRenderer::render: the whole frame on `stream` (null = the context's stream),
  * no host synchronisation. out_image: device pointer, width*height pixels of
  * settings.format (RGBA8: 4 bytes; RGBA32F: 16 bytes), row 0 = top, tightly packed. Every
- * pixel is written: C + T * background (black), alpha = 1 - T. */
+ * pixel is written: C + T * background (black), alpha = 1 - T. A frame on another stream than
+ * the renderer's previous frame waits for that frame (the two share the renderer's buffers);
+ * frames recorded into a graph capture are ordered by the stream the graph is launched on. */
 gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* out_image, void* stream);
 
 /* Test/debug readbacks of the last rendered frame (blocking; the GPU is synchronised).
@@ -296,7 +298,12 @@ gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* ou
  * a HIP graph held by the batch (it replaces the previous one); gsr_batch_replay launches it on
  * `stream`, one launch for the whole batch, with the cameras, images and renderers of the
  * capture. A capture needs every view's size rendered before by its branch's renderer (render
- * the batch once first) and plain renderers (no adaptive_list, frame_stats or profile_stages). */
+ * the batch once first) and plain renderers (no adaptive_list, frame_stats or profile_stages;
+ * the reduce-then-scan sort). gsr_batch_replay fails with GSR_ERROR_INVALID_ARGUMENT, launching
+ * nothing, when a renderer of the capture has been destroyed, has reallocated a buffer (a view
+ * larger than it had rendered) or has changed its colour model or fallback marks since the
+ * capture: capture again. Replays and other work on the same renderers must be stream-ordered
+ * by the caller (replays on one stream are). */
 typedef struct gsr_batch gsr_batch;
 gsr_status gsr_batch_create(gsr_context* ctx, uint32_t branches, gsr_batch** out);
 void gsr_batch_destroy(gsr_batch* batch);

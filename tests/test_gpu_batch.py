@@ -104,3 +104,80 @@ def test_batch_arguments(gsr_ctx):
     for o in outs:
         o.close()
     scene.close()
+
+
+@pytest.mark.gpu
+def test_replay_refuses_changed_or_destroyed_renderers(gsr_ctx):
+    """A captured graph holds raw pointers into its renderers: after a renderer reallocates (a larger
+    view), changes its colour binding or is destroyed, gsr_batch_replay refuses on the host before
+    launching anything; a fresh capture works again. The one-sweep sort is refused at capture."""
+    import gsr
+    from gsr import camera, scenes
+
+    log("case replay refusals")
+    sc = scenes.make_scene(5_000, seed=4, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cams = [gsr.camera_from_viewer(camera.orbit_view(6.0, 10.0, a), camera.perspective(60.0, 320 / 180),
+                                   camera.IDENTITY, 320, 180) for a in (0.0, 90.0)]
+    outs = [gsr.DeviceBuffer(gsr_ctx, 320 * 180 * 4) for _ in cams]
+    rs = [gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode="global") for _ in range(2)]
+    want = [rs[0].render_to_host(c).copy() for c in cams]
+    batch = gsr.Batch(gsr_ctx, 2)
+    batch.render(rs, cams, [o.ptr for o in outs])
+    batch.capture(rs, cams, [o.ptr for o in outs])
+    batch.replay()
+    gsr_ctx.synchronize()
+    assert np.array_equal(_read(gsr, outs[1], cams[1]), want[1])
+    # a larger view reallocates renderer 1's per-tile buffers
+    big = gsr.camera_from_viewer(camera.orbit_view(6.0, 10.0, 0.0), camera.perspective(60.0, 1280 / 720),
+                                 camera.IDENTITY, 1280, 720)
+    rs[1].render_to_host(big)
+    with pytest.raises(gsr.GsrError, match="changed since the capture"):
+        batch.replay()
+    batch.capture(rs, cams, [o.ptr for o in outs])  # a new capture sees the new buffers
+    batch.replay()
+    gsr_ctx.synchronize()
+    assert np.array_equal(_read(gsr, outs[1], cams[1]), want[1])
+    rs[1].close()
+    with pytest.raises(gsr.GsrError, match="destroyed"):
+        batch.replay()
+    one = gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode="global", sort_algorithm="onesweep")
+    one.render_to_host(cams[0])
+    with pytest.raises(gsr.GsrError, match="one-sweep"):
+        batch.capture([rs[0], one], cams, [o.ptr for o in outs])
+    batch.close()
+    for x in (rs[0], one):
+        x.close()
+    for o in outs:
+        o.close()
+    scene.close()
+
+
+@pytest.mark.gpu
+def test_frames_on_alternating_streams(gsr_ctx):
+    """Frames of one renderer on different streams are ordered after each other by the library:
+    alternating two streams with no caller synchronisation gives every frame's own image."""
+    import gsr
+    from gsr import camera, scenes
+
+    log("case alternating streams")
+    sc = scenes.make_scene(60_000, seed=6, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    W, H = 640, 360
+    cams = [gsr.camera_from_viewer(camera.orbit_view(7.0, 5.0, 30.0 * i), camera.perspective(60.0, W / H),
+                                   camera.IDENTITY, W, H) for i in range(6)]
+    r = gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode="global")
+    want = [r.render_to_host(c).copy() for c in cams]
+    outs = [gsr.DeviceBuffer(gsr_ctx, W * H * 4) for _ in cams]
+    streams = [gsr.Stream(gsr_ctx) for _ in range(2)]
+    for i, c in enumerate(cams):
+        r.render(c, outs[i].ptr, streams[i % 2].handle)
+    gsr_ctx.synchronize()
+    for i, c in enumerate(cams):
+        assert np.array_equal(_read(gsr, outs[i], c), want[i]), f"frame {i} differs"
+    for s in streams:
+        s.close()
+    r.close()
+    for o in outs:
+        o.close()
+    scene.close()
