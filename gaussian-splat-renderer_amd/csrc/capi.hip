@@ -116,10 +116,47 @@ uint32_t bitsFor(uint32_t v) {  // bits needed to hold values 0..v
 
 }  // namespace
 
+// --- Fixed scratch of the allocation-free sort (gsr_sort_pairs_noalloc) ---------------------------
+// For kNoAllocSortMaxKeys keys: the ping-pong copy (8-B keys, then 4-B values) and the
+// reduce-then-scan work area (digit totals and per-block digit histograms), in one file-scope
+// device array. Every GPU the library runs on holds its own copy until the process ends. In host
+// code the array's name is a host-side stand-in, not a device address: host code uses only the
+// address hipGetSymbolAddress gives for the current device (resolved once per device at
+// gsr_context_create). Nothing in it must start at zero: the histogram pass writes every
+// (digit, block) count and the scan every digit total before they are read.
+namespace {
+constexpr uint32_t kNoAllocSortMaxKeys = 1u << 20;
+constexpr size_t kNoAllocValuesOffset = size_t(kNoAllocSortMaxKeys) * 8;
+constexpr size_t kNoAllocWorkOffset = size_t(kNoAllocSortMaxKeys) * 12;
+constexpr size_t kNoAllocWorkWords = size_t(kMaxSortPasses) * kRadix + kMaxSortPasses + 1 + size_t(kRadix) * kMaxSortBlocks;
+constexpr size_t kNoAllocScratchBytes = kNoAllocWorkOffset + kNoAllocWorkWords * 4;
+}  // namespace
+static __device__ __align__(256) uint64_t scratch[(kNoAllocScratchBytes + 7) / 8];
+
+namespace {
+std::mutex g_scratchMutex;
+std::vector<void*> g_scratchByDevice;  // device ordinal -> that device's address of `scratch`
+
+// With `device` current: its address of `scratch`, looked up once per device.
+hipError_t resolveScratch(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchByDevice.size() <= size_t(device)) g_scratchByDevice.resize(size_t(device) + 1, nullptr);
+    if (!g_scratchByDevice[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchByDevice[device] = p;
+    }
+    *out = g_scratchByDevice[device];
+    return hipSuccess;
+}
+}  // namespace
+
 struct gsr_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    DeviceBuffer scratch;     // stand-alone sort/scan scratch
+    DeviceBuffer work;        // stand-alone sort/scan work area
+    void* noAllocScratch = nullptr;  // this device's copy of `scratch` (gsr_sort_pairs_noalloc)
     DeviceBuffer sortStatus;  // stand-alone sort: look-back status words (epoch-tagged, zeroed once)
     uint32_t sortEpoch = 0;
 };
@@ -250,7 +287,12 @@ gsr_status gsr_context_create(int32_t device, gsr_context** out) {
     gsr_context* c = new (std::nothrow) gsr_context();
     if (!c) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_context_create: host allocation");
     c->device = device;
-    const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    hipError_t e = resolveScratch(device, &c->noAllocScratch);
+    if (e != hipSuccess) {
+        delete c;
+        return hipFail(e, "gsr_context_create: hipGetSymbolAddress(scratch)");
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
         return hipFail(e, "hipStreamCreateWithFlags");
@@ -270,7 +312,7 @@ void gsr_context_destroy(gsr_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    ctx->scratch.release();
+    ctx->work.release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1088,9 +1130,9 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
     const size_t n = maxCount;
     const size_t smallBytes = (kMaxSortPasses * kRadix + kMaxSortPasses + 1 + size_t(kRadix) * kMaxSortBlocks) * 4;
     const size_t need = n * 8 + n * 4 + smallBytes + 256;
-    if (ctx->scratch.bytes < need) {
+    if (ctx->work.bytes < need) {
         GSR_HIP(hipDeviceSynchronize());
-        GSR_HIP(ctx->scratch.allocate(need));
+        GSR_HIP(ctx->work.allocate(need));
     }
     if (ctx->sortStatus.bytes < sortStatusBytes(maxCount)) {
         GSR_HIP(hipDeviceSynchronize());
@@ -1098,7 +1140,7 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
         GSR_HIP(hipMemset(ctx->sortStatus.ptr, 0, ctx->sortStatus.bytes));
         ctx->sortEpoch = 0;
     }
-    char* base = ctx->scratch.as<char>();
+    char* base = ctx->work.as<char>();
     uint64_t* keysB = reinterpret_cast<uint64_t*>(base);
     uint32_t* valuesB = reinterpret_cast<uint32_t*>(base + n * 8);
     uint32_t* small = reinterpret_cast<uint32_t*>(base + n * 12);
@@ -1115,6 +1157,36 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
     return GSR_OK;
 }
 
+gsr_status gsr_sort_pairs_noalloc(gsr_context* ctx, uint64_t* keys, uint32_t* values, uint32_t maxCount,
+                                  const uint32_t* countDevice, uint32_t keyBits, void* streamArg) {
+    if (!ctx || (maxCount && (!keys || !values)))
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_sort_pairs_noalloc: null argument");
+    if (keyBits == 0 || keyBits > 64) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_sort_pairs_noalloc: key_bits %u", keyBits);
+    if (maxCount > kNoAllocSortMaxKeys)
+        return fail(GSR_ERROR_INVALID_ARGUMENT,
+                    "gsr_sort_pairs_noalloc: max_count %u exceeds the fixed scratch of %u keys (use gsr_sort_pairs)",
+                    maxCount, kNoAllocSortMaxKeys);
+    if (maxCount == 0) return GSR_OK;
+    if (!ctx->noAllocScratch) return fail(GSR_ERROR_HIP, "gsr_sort_pairs_noalloc: the device scratch was not resolved");
+    GSR_HIP(hipSetDevice(ctx->device));
+    hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : ctx->stream;
+    char* base = static_cast<char*>(ctx->noAllocScratch);
+    uint64_t* keysB = reinterpret_cast<uint64_t*>(base);
+    uint32_t* valuesB = reinterpret_cast<uint32_t*>(base + kNoAllocValuesOffset);
+    uint32_t* small = reinterpret_cast<uint32_t*>(base + kNoAllocWorkOffset);
+    SortWork work{kSortAlgReduceThenScan, small, nullptr, nullptr, nullptr,
+                  small + kMaxSortPasses * kRadix + kMaxSortPasses + 1};
+    uint32_t epoch = 0;  // unused by the reduce-then-scan passes
+    const int which = launchRadixSort(work, keys, values, keysB, valuesB, maxCount, countDevice, 0u, keyBits, &epoch,
+                                      stream);
+    GSR_HIP(hipGetLastError());
+    if (which) {  // odd pass count: the first `count` pairs are in the scratch copy
+        launchCopyPairs(keysB, valuesB, keys, values, countDevice, maxCount, stream);
+        GSR_HIP(hipGetLastError());
+    }
+    return GSR_OK;
+}
+
 gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* out, uint32_t count,
                               uint32_t* totalDevice, void* streamArg) {
     if (!ctx || (count && (!in || !out))) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_exclusive_scan: null argument");
@@ -1122,11 +1194,11 @@ gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* ou
     hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : ctx->stream;
     const size_t blocks = (size_t(count) + kScanBlock - 1) / kScanBlock;
     const size_t need = (blocks + 1) * 4;
-    if (ctx->scratch.bytes < need) {
+    if (ctx->work.bytes < need) {
         GSR_HIP(hipDeviceSynchronize());
-        GSR_HIP(ctx->scratch.allocate(need));
+        GSR_HIP(ctx->work.allocate(need));
     }
-    launchExclusiveScan(in, out, count, ctx->scratch.as<uint32_t>(), totalDevice, stream);
+    launchExclusiveScan(in, out, count, ctx->work.as<uint32_t>(), totalDevice, stream);
     GSR_HIP(hipGetLastError());
     return GSR_OK;
 }

@@ -196,6 +196,7 @@ _renderer_max_capacity = _sig("gsr_renderer_max_capacity", _u32, [_vp])
 _renderer_list_resizes = _sig("gsr_renderer_list_resizes", _u32, [_vp])
 _renderer_last_stats = _sig("gsr_renderer_last_stats", _i32, [_vp, _vp, _vp])
 _sort_pairs = _sig("gsr_sort_pairs", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _i32, _vp])
+_sort_pairs_noalloc = _sig("gsr_sort_pairs_noalloc", _i32, [_vp, _vp, _vp, _u32, _vp, _u32, _vp])
 
 
 class PlySceneStruct(ctypes.Structure):
@@ -261,7 +262,7 @@ EXPORTED = (
     "gsr_renderer_create gsr_renderer_destroy gsr_renderer_capacity gsr_renderer_total_bytes gsr_render "
     "gsr_renderer_max_capacity gsr_renderer_list_resizes gsr_renderer_last_stats "
     "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_sort_pairs "
-    "gsr_exclusive_scan "
+    "gsr_sort_pairs_noalloc gsr_exclusive_scan "
     "gsr_batch_create gsr_batch_destroy gsr_batch_branches gsr_batch_render gsr_batch_capture gsr_batch_replay "
     "gsr_colour_model_param_count gsr_colour_model_create gsr_colour_model_destroy gsr_colour_model_train "
     "gsr_colour_model_loss gsr_colour_model_gradients gsr_colour_model_get gsr_colour_model_set "
@@ -907,6 +908,24 @@ def tune_batch(ctx: Context, make_renderer, cams, outs, candidates=(1, 2, 4, 8),
     for r in pool:
         r.close()
     return best, results
+
+
+NOALLOC_SORT_MAX_KEYS = 1 << 20  # the fixed scratch of gsr_sort_pairs_noalloc
+
+
+def sort_pairs_device(ctx: Context, keys_ptr: int, values_ptr: int, max_count: int, key_bits: int = 64,
+                      count_ptr: Optional[int] = None, algorithm: str = "reduce-then-scan", noalloc: bool = False,
+                      stream: Optional[int] = None) -> None:
+    """GpuRadixSort in place on device pointers (asynchronous). noalloc: gsr_sort_pairs_noalloc (no
+    allocation; at most NOALLOC_SORT_MAX_KEYS keys, reduce-then-scan only)."""
+    if noalloc:
+        if algorithm != "reduce-then-scan":
+            raise GsrError("sort_pairs_device: the allocation-free sort is reduce-then-scan")
+        _check(_sort_pairs_noalloc(ctx.handle, keys_ptr, values_ptr, int(max_count), count_ptr, int(key_bits), stream),
+               "gsr_sort_pairs_noalloc")
+    else:
+        _check(_sort_pairs(ctx.handle, keys_ptr, values_ptr, int(max_count), count_ptr, int(key_bits),
+                           SORT_ALGORITHMS[algorithm], stream), "gsr_sort_pairs")
 
 
 def sort_pairs(ctx: Context, keys: np.ndarray, values: np.ndarray, key_bits: int = 64, count: Optional[int] = None,

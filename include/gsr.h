@@ -281,6 +281,18 @@ gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n);
 gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, uint32_t max_count,
                           const uint32_t* count_device, uint32_t key_bits, gsr_sort_algorithm algorithm, void* stream);
 
+/* gsr_sort_pairs without any allocation. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, result and stability as gsr_sort_pairs with the
+ * reduce-then-scan algorithm, but the ping-pong copy and the histograms live in a fixed
+ * library-owned device array (about 13 MiB, held by every GPU the library runs on until the
+ * process ends), so max_count is at most 1,048,576 keys (GSR_ERROR_INVALID_ARGUMENT above that)
+ * and the call makes no hipMalloc, hipFree or other allocating call. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams, other threads) race;
+ * order them. Asynchronous on `stream` (null = the context's). */
+gsr_status gsr_sort_pairs_noalloc(gsr_context* ctx, uint64_t* keys, uint32_t* values, uint32_t max_count,
+                                  const uint32_t* count_device, uint32_t key_bits, void* stream);
+
 /* GpuScan on its own (gpu_scan.h:25-76): exclusive prefix sum of u32 values, the total in
  * *total_device. Asynchronous on `stream`. */
 gsr_status gsr_exclusive_scan(gsr_context* ctx, const uint32_t* in, uint32_t* out, uint32_t count,
