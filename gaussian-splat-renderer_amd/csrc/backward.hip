@@ -624,6 +624,7 @@ void launchRenderTrain(const float4* splats, const uint32_t* values, const uint2
     if (cam.tilesX * cam.tilesY == 0) return;
     hipLaunchKernelGGL(kRenderTrain, dim3(scheduledGrid(cam)), dim3(256), 0, stream, splats, values, ranges, cam, out,
                        state);
+    markKernel("kRenderTrain", stream);
 }
 
 void launchRenderBackward(const float4* splats, const uint32_t* values, const uint2* ranges, const DevCamera& cam,
@@ -631,6 +632,7 @@ void launchRenderBackward(const float4* splats, const uint32_t* values, const ui
     if (cam.tilesX * cam.tilesY == 0) return;
     hipLaunchKernelGGL(kRenderBackward, dim3(scheduledGrid(cam)), dim3(256), 0, stream, splats, values, ranges, cam,
                        dOut, state, grad2D, count);
+    markKernel("kRenderBackward", stream);
 }
 
 void launchPreprocessBackward(const SceneRef& scene, const DevCamera& cam, const float4* splats, const float* grad2D,
@@ -640,6 +642,7 @@ void launchPreprocessBackward(const SceneRef& scene, const DevCamera& cam, const
     if (blocks == 0) return;
     hipLaunchKernelGGL(kPreprocessBackward, dim3(blocks), dim3(256), 0, stream, scene, cam, splats, grad2D, dMean,
                        dLogScale, dQuat, dLogit, dSh);
+    markKernel("kPreprocessBackward", stream);
 }
 
 }  // namespace gsr

@@ -71,6 +71,7 @@ void launchCompactVisible(const float4* splats, const uint32_t* tileCounts, cons
     static_assert(kBinThreads == kPreprocessThreads, "compaction blocks are the preprocess blocks");
     hipLaunchKernelGGL(kCompactVisible, dim3(blocks), dim3(kBinThreads), 0, stream, splats, tileCounts, blockOffsets,
                        count, depthKeys, depthValues);
+    markKernel("kCompactVisible", stream);
 }
 
 void launchGatherCounts(const uint32_t* tileCounts, const uint32_t* order, const gsr_control* control,
@@ -79,6 +80,7 @@ void launchGatherCounts(const uint32_t* tileCounts, const uint32_t* order, const
     if (blocks == 0) return;
     hipLaunchKernelGGL(kGatherCounts, dim3(blocks), dim3(kBinThreads), 0, stream, tileCounts, order, control,
                        sortedCounts, blockSums);
+    markKernel("kGatherCounts", stream);
 }
 
 }  // namespace gsr

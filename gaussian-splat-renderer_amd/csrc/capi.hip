@@ -38,6 +38,8 @@ gsr_status fail(gsr_status s, const char* fmt, ...) {
 }  // namespace
 
 namespace gsr {
+thread_local KernelMarks* t_kernelMarks = nullptr;
+
 gsr_status setError(gsr_status s, const char* message) {
     g_lastError = message;
     return s;
@@ -250,6 +252,11 @@ struct gsr_renderer {
     hipEvent_t order = nullptr;  // marks the last frame on lastStream (frames on another stream wait on it)
     hipEvent_t events[GSR_STAGE_COUNT] = {};
     bool eventsRecorded = false;
+    std::unique_ptr<KernelMarks> marks;  // profile_stages = 2: an event after every kernel
+    bool marksRecorded = false;
+    // tile_stats: kTileStatCount counters per tile of the last frame (sized for the largest grid).
+    DeviceBuffer tileStats;
+    uint32_t tileStatsTiles = 0;
     gsr_colour_model* colour = nullptr;  // learned colour, or null (SH)
     // A captured batch holds raw pointers to this renderer's buffers: `id` names the renderer for
     // its lifetime (never reused) and `generation` goes up whenever a buffer a frame writes or
@@ -714,6 +721,14 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
         return fail(GSR_ERROR_UNSUPPORTED,
                     "gsr_renderer_create: differentiable needs format RGBA32F, render_splat FULL, sort_window 0 "
                     "and an f32 scene");
+    if (s.profile_stages < 0 || s.profile_stages > 2)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: profile_stages must be 0, 1 or 2");
+    if (s.tile_stats &&
+        (s.render_loop != GSR_RENDER_LOOP_CULLED || s.render_pixels_per_thread != 1 || s.render_exact_alpha ||
+         s.sort_window != 0 || s.differentiable))
+        return fail(GSR_ERROR_UNSUPPORTED,
+                    "gsr_renderer_create: tile_stats counts the culled render loop (render_loop CULLED, one pixel per "
+                    "thread, fast alpha, sort_window 0, not differentiable)");
     GSR_HIP(hipSetDevice(ctx->device));
     gsr_renderer* r = new (std::nothrow) gsr_renderer();
     if (!r) return fail(GSR_ERROR_OUT_OF_MEMORY, "gsr_renderer_create: host allocation");
@@ -753,6 +768,11 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
     for (int i = 0; i < GSR_STAGE_COUNT && e == hipSuccess; ++i) e = hipEventCreate(&r->events[i]);
+    if (s.profile_stages == 2 && e == hipSuccess) {
+        r->marks.reset(new (std::nothrow) KernelMarks());
+        if (!r->marks) e = hipErrorOutOfMemory;
+        for (int i = 0; i <= KernelMarks::kMax && e == hipSuccess; ++i) e = hipEventCreate(&r->marks->events[i]);
+    }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&r->order, hipEventDisableTiming);
     if (s.adaptive_list || s.frame_stats) {
         if (e == hipSuccess)
@@ -787,6 +807,9 @@ void gsr_renderer_destroy(gsr_renderer* r) {
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t& ev : r->statEvent)
         if (ev) (void)hipEventDestroy(ev);
+    if (r->marks)
+        for (hipEvent_t& ev : r->marks->events)
+            if (ev) (void)hipEventDestroy(ev);
     if (r->statHost) (void)hipHostFree(r->statHost);
     delete r;
 }
@@ -814,7 +837,7 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
                                   &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth,
-                                  &r->pixelState, &r->grad2D};
+                                  &r->pixelState, &r->grad2D, &r->tileStats};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     for (const auto& o : r->tileOrders) s += o->table.bytes;
@@ -851,6 +874,13 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         r->pixelStatePixels = 0;
         GSR_HIP(r->pixelState.allocate(size_t(cam->width) * cam->height * 8));
         r->pixelStatePixels = size_t(cam->width) * cam->height;
+    }
+    if (r->settings.tile_stats && tiles > r->tileStatsTiles) {
+        GSR_HIP(hipStreamSynchronize(stream));
+        ++r->generation;
+        r->tileStatsTiles = 0;
+        GSR_HIP(r->tileStats.allocate(size_t(tiles) * kTileStatCount * 4));
+        r->tileStatsTiles = tiles;
     }
     const uint32_t* tileOrder = nullptr;
     if (!r->binning && !r->settings.full_key_sort) {
@@ -913,6 +943,16 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     const uint32_t bounds = uint32_t(r->settings.bounds);
 
     GSR_HIP(mark(0));
+    // Per-kernel marks: the launch wrappers record them while t_kernelMarks is set (cleared on
+    // every return path by the guard).
+    struct MarksGuard {
+        ~MarksGuard() { t_kernelMarks = nullptr; }
+    } marksGuard;
+    if (r->marks && !capturing) {
+        r->marks->count = 0;
+        GSR_HIP(hipEventRecord(r->marks->events[0], stream));
+        t_kernelMarks = r->marks.get();
+    }
     launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
                      r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords,
                      uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), r->rayDepth.as<float4>(),
@@ -998,14 +1038,22 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     if (r->settings.differentiable)
         launchRenderTrain(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc,
                           static_cast<float4*>(outImage), r->pixelState.as<uint2>(), stream);
-    else
+    else {
+        uint32_t* tileStats = nullptr;
+        if (r->settings.tile_stats) {
+            tileStats = r->tileStats.as<uint32_t>();
+            GSR_HIP(hipMemsetAsync(tileStats, 0, size_t(tiles) * kTileStatCount * 4, stream));
+        }
         launchRender(r->splats.as<float4>(), r->packed.as<uint32_t>(), r->settings.render_splat, sortedValues,
                      r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
                      r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0,
                      r->settings.render_loop, int(r->settings.sort_window), r->rayDepth.as<float4>(), outImage,
-                     stream);
+                     tileStats, stream);
+    }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(7));
+    r->marksRecorded = t_kernelMarks != nullptr;
+    t_kernelMarks = nullptr;
     if (r->statHost) {  // this frame's control block, read back without a stall
         int slot = -1;
         for (int s = 0; s < gsr_renderer::kStatSlots && slot < 0; ++s)
@@ -1112,6 +1160,33 @@ gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n) {
     for (int i = 0; i < GSR_STAGE_COUNT - 1; ++i) GSR_HIP(hipEventElapsedTime(&t[i], r->events[i], r->events[i + 1]));
     GSR_HIP(hipEventElapsedTime(&t[GSR_STAGE_TOTAL], r->events[0], r->events[GSR_STAGE_COUNT - 1]));
     for (uint32_t i = 0; i < n && i < uint32_t(GSR_STAGE_COUNT); ++i) ms[i] = t[i];
+    return GSR_OK;
+}
+
+gsr_status gsr_read_kernel_times(gsr_renderer* r, const char** names, float* ms, uint32_t max, uint32_t* count) {
+    if (!r) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_kernel_times: null renderer");
+    if (!r->marks || !r->marksRecorded)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_kernel_times: no frame with kernel marks (profile_stages = 2?)");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    const KernelMarks& m = *r->marks;
+    if (m.count > 0) GSR_HIP(hipEventSynchronize(m.events[m.count]));
+    for (int i = 0; i < m.count && uint32_t(i) < max; ++i) {
+        if (names) names[i] = m.names[i];
+        if (ms) GSR_HIP(hipEventElapsedTime(&ms[i], m.events[i], m.events[i + 1]));
+    }
+    if (count) *count = uint32_t(m.count);
+    return GSR_OK;
+}
+
+gsr_status gsr_read_tile_stats(gsr_renderer* r, uint32_t* stats, uint32_t maxTiles) {
+    if (!r || !stats) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_tile_stats: null argument");
+    if (!r->settings.tile_stats)
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_tile_stats: the renderer was created without tile_stats");
+    if (!r->haveFrame) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_tile_stats: no frame rendered yet");
+    GSR_HIP(hipSetDevice(r->ctx->device));
+    GSR_HIP(hipDeviceSynchronize());
+    const uint32_t tiles = std::min(r->lastCam.tiles_x * r->lastCam.tiles_y, maxTiles);
+    if (tiles) GSR_HIP(hipMemcpy(stats, r->tileStats.ptr, size_t(tiles) * kTileStatCount * 4, hipMemcpyDeviceToHost));
     return GSR_OK;
 }
 
@@ -1338,10 +1413,10 @@ gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint3
     const uint32_t used = std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
     for (uint32_t i = 0; i < used; ++i) {
         const gsr_renderer* r = renderers[i];
-        if (r->statHost || r->settings.profile_stages)
+        if (r->statHost || r->settings.profile_stages || r->settings.tile_stats)
             return fail(GSR_ERROR_INVALID_ARGUMENT,
                         "gsr_batch_capture: renderer %u reads back statistics or stage times (adaptive_list, "
-                        "frame_stats, profile_stages); a captured batch takes plain renderers", i);
+                        "frame_stats, profile_stages, tile_stats); a captured batch takes plain renderers", i);
         // The one-sweep sort tags its look-back status words with a host-side epoch, which a
         // graph would freeze: a replay could read the previous replay's words as current.
         if (!r->binning && r->settings.sort_algorithm == GSR_SORT_ONESWEEP)

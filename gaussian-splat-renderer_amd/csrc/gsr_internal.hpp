@@ -88,6 +88,25 @@ struct SortWork {
 };
 size_t sortStatusBytes(uint32_t maxCount);
 
+// Per-kernel timing (profile_stages = 2). While a frame is recorded with marks set for the calling
+// thread, every launch wrapper records an event on its stream after its kernel, with the
+// kernel's name; kernel i took elapsed(events[i], events[i + 1]) (events[0] is recorded before
+// the frame's first kernel).
+struct KernelMarks {
+    static constexpr int kMax = 96;
+    hipEvent_t events[kMax + 1] = {};
+    const char* names[kMax] = {};
+    int count = 0;
+};
+extern thread_local KernelMarks* t_kernelMarks;
+inline void markKernel(const char* name, hipStream_t stream) {
+    KernelMarks* m = t_kernelMarks;
+    if (m && m->count < KernelMarks::kMax) {
+        (void)hipEventRecord(m->events[m->count + 1], stream);
+        m->names[m->count++] = name;
+    }
+}
+
 // Launch wrappers (each in its .hip file).
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
@@ -136,10 +155,13 @@ void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, 
                             uint32_t bounds, uint32_t tilesX, uint32_t tilesY, uint32_t* keys, uint32_t* values,
                             gsr_control* controlRW, hipStream_t stream);
 // sortWindow > 0: StopThePop with that window (rayDepth: 3 float4 per gaussian), reference alpha.
+// tileStats (culled loop only; null = none): 8 counters per tile, zeroed by the caller (the
+// reference's TileStat, splat_tile_renderer.h:130-141).
+constexpr int kTileStatCount = 8;
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
-                  hipStream_t stream);
+                  uint32_t* tileStats, hipStream_t stream);
 
 // Learned colour (mlp.hip): a 16-value latent per gaussian and one shared MLP. Parameters:
 // mlpParamCount() floats; packs: mlpPackedTiles() x 64 uint2 (the bf16 A-operand tiles, rebuilt

@@ -655,6 +655,7 @@ int mlpPackedTiles() { return kTACount + 2 * kTBCount; }  // units of 64 x uint2
 
 void launchMlpPack(const float* params, uint2* packs, hipStream_t stream) {
     hipLaunchKernelGGL(kPack, dim3(kTACount + kTBCount), dim3(64), 0, stream, params, packs);
+    markKernel("kPack", stream);
 }
 
 void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, const float* latents,
@@ -676,6 +677,7 @@ void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* pac
         else
             hipLaunchKernelGGL((kTrain<GSR_SCENE_F32_SOA, false>), dim3(blocks), dim3(kThreads), 0, stream, scene,
                                params, packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
+            markKernel("kTrain", stream);
     } else {
         if (backward)
             hipLaunchKernelGGL((kTrain<GSR_SCENE_F16, true>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
@@ -683,6 +685,7 @@ void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* pac
         else
             hipLaunchKernelGGL((kTrain<GSR_SCENE_F16, false>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
                                packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
+            markKernel("kTrain", stream);
     }
 }
 
@@ -691,18 +694,21 @@ void launchAdam(float* p, float* grad, float* m, float* v, uint32_t n, float lr,
     const uint32_t blocks = gridFor(n, 256, 4096);
     if (blocks == 0) return;
     hipLaunchKernelGGL(kAdam, dim3(blocks), dim3(256), 0, stream, p, grad, m, v, n, lr, bc1, bc2);
+    markKernel("kAdam", stream);
 }
 
 void launchToBf16(const float* in, uint16_t* out, uint32_t n, hipStream_t stream) {
     const uint32_t blocks = gridFor(n, 256, 4096);
     if (blocks == 0) return;
     hipLaunchKernelGGL(kToBf16, dim3(blocks), dim3(256), 0, stream, in, out, n);
+    markKernel("kToBf16", stream);
 }
 
 void launchInitLatents(float* out, uint32_t n, uint32_t seed, float scale, hipStream_t stream) {
     const uint32_t blocks = gridFor(n, 256, 4096);
     if (blocks == 0) return;
     hipLaunchKernelGGL(kInitLatents, dim3(blocks), dim3(256), 0, stream, out, n, seed, scale);
+    markKernel("kInitLatents", stream);
 }
 
 void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count, uint32_t* gaussians, float* dirs,
@@ -710,6 +716,7 @@ void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count,
     const uint32_t blocks = (n + 255) / 256;
     if (blocks == 0) return;
     hipLaunchKernelGGL(kSamples, dim3(blocks), dim3(256), 0, stream, seed, first, n, count, gaussians, dirs);
+    markKernel("kSamples", stream);
 }
 
 void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* packs, const uint16_t* latentsBf16,
@@ -722,6 +729,7 @@ void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* 
     else
         hipLaunchKernelGGL(kFallback<GSR_SCENE_F16>, dim3(blocks), dim3(kThreads), 0, stream, scene, params, packs,
                            latentsBf16, threshold, useMlp, err);
+    markKernel("kFallback", stream);
 }
 
 void launchColourMlp(const SceneRef& scene, const DevCamera& cam, const float* params, const uint2* packs,
@@ -736,6 +744,7 @@ void launchColourMlp(const SceneRef& scene, const DevCamera& cam, const float* p
     else
         hipLaunchKernelGGL(kColourMlp<GSR_SCENE_F16>, dim3(blocks), dim3(kThreads), 0, stream, scene, cam, params,
                            packs, latentsBf16, useMlp, tileCounts, splats, renderSplat, packed);
+    markKernel("kColourMlp", stream);
 }
 
 void launchMlpEvaluate(const float* params, const uint2* packs, const float* latents, const uint32_t* gaussians,
@@ -744,6 +753,7 @@ void launchMlpEvaluate(const float* params, const uint2* packs, const float* lat
     if (blocks == 0) return;
     hipLaunchKernelGGL(kEvaluate, dim3(blocks), dim3(kThreads), 0, stream, params, packs, latents, gaussians, dirs, n,
                        rgb);
+    markKernel("kEvaluate", stream);
 }
 
 }  // namespace gsr

@@ -298,6 +298,7 @@ void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t boun
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
                            bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat, packed,
                            rayDepth, useMlp);
+    markKernel("kPreprocess", stream);
 }
 
 }  // namespace gsr

@@ -313,14 +313,36 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     }
 }
 
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t waveMax(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
 // Culled predicated loop (FAST, one pixel per thread, 8x8 quadrant per wave).
-template <int FORMAT, int RS>
+// STATS: the same render, also counting per tile the reference's tile statistics (TileStat,
+// splat_tile_renderer.h:130-141; tile_render.comp:78-90, 380-399) into tileStats[tile * 8 + k]
+// (zeroed by the caller): k = 0 list length, 1 entries fetched before the tile stopped,
+// 2 (pixel, entry) pairs evaluated in the reference's loop (a pixel walks every fetched entry
+// until it terminates, the terminating one included), 3 pairs blended, 4 pixels that
+// terminated (T < 1e-4), 5 lane slots this kernel spent in its blend loop (per wave and batch,
+// the list slots walked x 64), 6 the largest per-pixel evaluated count, 7 pixels in the image.
+template <int FORMAT, int RS, bool STATS = false>
 __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict__ splats,
                                                      const uint32_t* __restrict__ packed,
                                                      const uint32_t* __restrict__ values,
                                                      const uint2* __restrict__ ranges, uint32_t width,
                                                      uint32_t height, uint32_t tilesX, uint32_t tileCount,
-                                                     uint32_t gaussianCount, void* __restrict__ out) {
+                                                     uint32_t gaussianCount, void* __restrict__ out,
+                                                     uint32_t* __restrict__ tileStats = nullptr) {
     // One 48-B record per batch slot: the tile-local quadratic of q (tileQuadratic) and the
     // colour. Slot kBatch is a null splat (q = -inf for every pixel) that pads the wave lists.
     // The lists hold the records' byte offsets, so a splat's reads need no address arithmetic.
@@ -349,9 +371,11 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
     const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
 
     const uint2 range = ranges[tile];
+    uint32_t statFetched = 0, statTermEntry = 0xFFFFFFFFu, statBlended = 0, statSlots = 0;
     for (uint32_t batch = range.x; batch < range.y; batch += kBatch) {
         // Whole-tile vote; also the barrier that protects the previous batch's LDS.
         if (__syncthreads_count(thr < __builtin_inff() ? 0 : 1) == 256) break;
+        if (STATS) statFetched += range.y - batch < uint32_t(kBatch) ? range.y - batch : uint32_t(kBatch);
         {
             const uint32_t e = batch + t;
             uint32_t mask = 0u;
@@ -403,13 +427,23 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;
+                    const float thrBefore = thr;
                     blendPredicated(q[u], rec[u]->col, T, c0, c1, c2, thr);
+                    if (STATS) {
+                        const bool ended = thrBefore < __builtin_inff() && !(thr < __builtin_inff());
+                        const uint32_t slot = uint32_t(reinterpret_cast<const char*>(rec[u]) - recBase) /
+                                              uint32_t(sizeof(SplatRecord));
+                        if (ended) statTermEntry = batch + slot;
+                        statBlended += (q[u] >= thrBefore && !ended) ? 1u : 0u;
+                    }
                 }
+                if (STATS) statSlots += 4u * 64u;
                 if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
             }
         }
     }
-    if (px < width && py < height) {
+    const bool inside = px < width && py < height;
+    if (inside) {
         const size_t p = size_t(py) * width + px;
         if (FORMAT == GSR_TARGET_RGBA8) {
             const uint32_t rgba = quantize8(c0) | (quantize8(c1) << 8) | (quantize8(c2) << 16) |
@@ -417,6 +451,26 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
             static_cast<uint32_t*>(out)[p] = rgba;
         } else {
             static_cast<float4*>(out)[p] = make_float4(c0, c1, c2, 1.0f - T);
+        }
+    }
+    if (STATS) {
+        const bool terminated = inside && statTermEntry != 0xFFFFFFFFu;
+        const uint32_t evaluated = !inside ? 0u : terminated ? statTermEntry - range.x + 1u : statFetched;
+        const uint32_t sumEval = waveSum(evaluated), sumBlend = waveSum(inside ? statBlended : 0u);
+        const uint32_t sumTerm = waveSum(terminated ? 1u : 0u), sumInside = waveSum(inside ? 1u : 0u);
+        const uint32_t maxEval = waveMax(evaluated);
+        uint32_t* st = tileStats + size_t(tile) * 8u;
+        if (lane == 0) {
+            atomicAdd(&st[2], sumEval);
+            atomicAdd(&st[3], sumBlend);
+            atomicAdd(&st[4], sumTerm);
+            atomicAdd(&st[5], statSlots);
+            atomicMax(&st[6], maxEval);
+            atomicAdd(&st[7], sumInside);
+        }
+        if (t == 0) {
+            st[0] = range.y - range.x;
+            st[1] = statFetched;
         }
     }
 }
@@ -608,6 +662,7 @@ void launchStpK(const float4* splats, const uint32_t* packed, const float4* rayD
     hipLaunchKernelGGL((kRenderStp<FORMAT, RS, K>), dim3(rowsPerXcd * cam.tilesX * kXcds), dim3(256), 0, stream, splats,
                        packed, rayDepth, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount,
                        cam.focalX, cam.focalY, out);
+    markKernel("kRenderStp", stream);
 }
 
 template <int FORMAT, int RS, int K = 1>
@@ -625,12 +680,19 @@ void launchStp(int window, const float4* splats, const uint32_t* packed, const f
 
 template <int FORMAT, int RS>
 void launchCulled(const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
-                  const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+                  const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, hipStream_t stream) {
     const uint32_t tileCount = cam.tilesX * cam.tilesY;
     const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
     const uint32_t perXcd = rowsPerXcd * cam.tilesX;
-    hipLaunchKernelGGL((kRenderCulled<FORMAT, RS>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats, packed, values,
-                       ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out);
+    if (tileStats)
+        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, true>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats,
+                           packed, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out,
+                           tileStats);
+    else
+        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, false>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats,
+                           packed, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out,
+                           nullptr);
+    markKernel("kRenderCulled", stream);
 }
 
 template <int FORMAT, bool FAST, int PPT, bool PRED, int RS>
@@ -642,11 +704,13 @@ void launchOne(const float4* splats, const uint32_t* packed, const uint32_t* val
     hipLaunchKernelGGL((kRender<FORMAT, FAST, PPT, PRED, RS>), dim3(perXcd * kXcds), dim3(256 / PPT), 0, stream,
                        splats, packed, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount,
                        out);
+    markKernel("kRender", stream);
 }
 
 template <int FORMAT, int RS>
 void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uint32_t* packed, const uint32_t* values,
-                  const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+                  const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats,
+                  hipStream_t stream) {
     if (exact) {
         if (ppt == 4)
             launchOne<FORMAT, false, 4, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
@@ -663,14 +727,14 @@ void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uin
     } else if (loop == GSR_RENDER_LOOP_PREDICATED) {
         launchOne<FORMAT, true, 1, true, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else {
-        launchCulled<FORMAT, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
+        launchCulled<FORMAT, RS>(splats, packed, values, ranges, cam, gaussianCount, out, tileStats, stream);
     }
 }
 
 template <int FORMAT>
 void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow, const float4* rayDepth,
                  const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
-                 const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
+                 const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, hipStream_t stream) {
     if (sortWindow > 0) {
         if (renderSplat == GSR_RENDER_SPLAT_P32)
             launchStp<FORMAT, GSR_RENDER_SPLAT_P32>(sortWindow, splats, packed, rayDepth, values, ranges, cam,
@@ -685,13 +749,13 @@ void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow,
     }
     if (renderSplat == GSR_RENDER_SPLAT_P32)
         launchFormat<FORMAT, GSR_RENDER_SPLAT_P32>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
-                                                   out, stream);
+                                                   out, tileStats, stream);
     else if (renderSplat == GSR_RENDER_SPLAT_P24)
         launchFormat<FORMAT, GSR_RENDER_SPLAT_P24>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
-                                                   out, stream);
+                                                   out, tileStats, stream);
     else
         launchFormat<FORMAT, GSR_RENDER_SPLAT_FULL>(ppt, exact, loop, splats, packed, values, ranges, cam,
-                                                    gaussianCount, out, stream);
+                                                    gaussianCount, out, tileStats, stream);
 }
 
 }  // namespace
@@ -699,14 +763,14 @@ void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow,
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
-                  hipStream_t stream) {
+                  uint32_t* tileStats, hipStream_t stream) {
     if (cam.tilesX * cam.tilesY == 0) return;
     if (format == GSR_TARGET_RGBA8)
         launchSplat<GSR_TARGET_RGBA8>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
-                                      packed, values, ranges, cam, gaussianCount, out, stream);
+                                      packed, values, ranges, cam, gaussianCount, out, tileStats, stream);
     else
         launchSplat<GSR_TARGET_RGBA32F>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
-                                        packed, values, ranges, cam, gaussianCount, out, stream);
+                                        packed, values, ranges, cam, gaussianCount, out, tileStats, stream);
 }
 
 }  // namespace gsr

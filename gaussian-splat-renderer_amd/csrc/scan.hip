@@ -196,10 +196,12 @@ void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t
                       bool scanVisible, uint32_t* blockOffsets, gsr_control* control, hipStream_t stream) {
     hipLaunchKernelGGL(kFrameSetup, dim3(1), dim3(kSetupThreads), 0, stream, blockStats, numBlocks, gaussianCount,
                        capacity, scanVisible ? 1u : 0u, blockOffsets, control);
+    markKernel("kFrameSetup", stream);
 }
 
 void launchScanVisibleBlocks(uint32_t* blockSums, const gsr_control* control, hipStream_t stream) {
     hipLaunchKernelGGL(kScanVisibleBlocks, dim3(1), dim3(kSetupThreads), 0, stream, blockSums, control);
+    markKernel("kScanVisibleBlocks", stream);
 }
 
 void launchExclusiveScan(const uint32_t* in, uint32_t* out, uint32_t count, uint32_t* blockSums, uint32_t* total,
@@ -207,8 +209,10 @@ void launchExclusiveScan(const uint32_t* in, uint32_t* out, uint32_t count, uint
     const uint32_t blocks = (count + kScanBlock - 1) / kScanBlock;
     if (blocks > 0) hipLaunchKernelGGL(kScanReduce, dim3(blocks), dim3(kScanThreads), 0, stream, in, count, blockSums);
     hipLaunchKernelGGL(kScanBlockSums, dim3(1), dim3(kSetupThreads), 0, stream, blockSums, blocks, total);
+    markKernel("kScanBlockSums", stream);
     if (blocks > 0)
         hipLaunchKernelGGL(kScanDownsweep, dim3(blocks), dim3(kScanThreads), 0, stream, in, out, count, blockSums);
+        markKernel("kScanDownsweep", stream);
 }
 
 }  // namespace gsr

@@ -424,6 +424,7 @@ void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint3
     if (blocks == 0) return;
     hipLaunchKernelGGL(kCopyPairs<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, ks, vs, kd, vd, countDevice,
                        maxCount);
+    markKernel("kCopyPairs", stream);
 }
 
 // Reduce-then-scan LSD sort of 32-bit keys (depth keys, tile ids): 3 kernels per digit.
@@ -443,9 +444,12 @@ int launchRadixSort32(const SortWork& w, uint32_t* keysA, uint32_t* valuesA, uin
         const uint32_t bits = passBits(keyBits, passes, p);
         hipLaunchKernelGGL(kSortHistogram<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
                            maxCount, shift, bits, w.blockHist);
+        markKernel("kSortHistogram", stream);
         hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks, w.globalHist);
+        markKernel("kSortScan", stream);
         hipLaunchKernelGGL(kSortScatter<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout, vout,
                            countDevice, maxCount, shift, bits, w.blockHist, w.globalHist);
+        markKernel("kSortScatter", stream);
         shift += bits;
         uint32_t* tk = kin;
         kin = kout;
@@ -476,10 +480,13 @@ int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint6
             const uint32_t bits = passBits(bitsTotal, passes, p);
             hipLaunchKernelGGL(kSortHistogram<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
                                maxCount, shift, bits, w.blockHist);
+            markKernel("kSortHistogram", stream);
             hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks,
                                w.globalHist);
+            markKernel("kSortScan", stream);
             hipLaunchKernelGGL(kSortScatter<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout,
                                vout, countDevice, maxCount, shift, bits, w.blockHist, w.globalHist);
+            markKernel("kSortScatter", stream);
             shift += bits;
             uint64_t* tk = kin;
             kin = kout;
@@ -492,16 +499,19 @@ int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint6
     }
     const uint32_t tiles = (maxCount + kSortTile - 1) / kSortTile;
     hipLaunchKernelGGL(kSortInit, dim3(1), dim3(kSortThreads), 0, stream, w.globalHist, w.tickets, passes);
+    markKernel("kSortInit", stream);
     uint32_t upBlocks = (maxCount + kSortThreads * 16 - 1) / (kSortThreads * 16);
     if (upBlocks > 512) upBlocks = 512;
     hipLaunchKernelGGL(kSortUpfront, dim3(upBlocks), dim3(kSortThreads), 0, stream, keysA, countDevice, maxCount,
                        firstBit, passes, w.globalHist);
+    markKernel("kSortUpfront", stream);
     for (uint32_t p = 0; p < passes; ++p) {
         *epoch = (*epoch + 1u) & 0x3FFFFFFFu;
         if (*epoch == 0) *epoch = 1;  // 0 is the value of never-written status words
         hipLaunchKernelGGL(kSortOnesweep, dim3(tiles), dim3(kSortThreads), 0, stream, kin, vin, kout, vout, countDevice,
                            maxCount, firstBit + p * kRadixBits, w.globalHist + p * kRadix, w.status, w.tickets + p, *epoch,
                            w.timeouts);
+        markKernel("kSortOnesweep", stream);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;

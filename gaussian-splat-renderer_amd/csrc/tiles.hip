@@ -234,6 +234,7 @@ void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uin
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<false, uint64_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
                        nullptr, blockOffsets, control, count, bounds, tilesX, tilesY, keys, values, controlRW);
+    markKernel("kDuplicate", stream);
 }
 
 void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, const uint32_t* order,
@@ -244,6 +245,7 @@ void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, 
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<true, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, sortedCounts,
                        order, blockOffsets, control, maxVisible, bounds, tilesX, tilesY, keys, values, controlRW);
+    markKernel("kDuplicate", stream);
 }
 
 template <typename Key>
@@ -253,6 +255,7 @@ void launchRangesT(const Key* keys, const gsr_control* control, uint32_t capacit
     if (blocks > 2048) blocks = 2048;
     if (blocks == 0) return;
     hipLaunchKernelGGL(kRanges<Key>, dim3(blocks), dim3(kRangesThreads), 0, stream, keys, control, tileCount, ranges);
+    markKernel("kRanges", stream);
 }
 
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,

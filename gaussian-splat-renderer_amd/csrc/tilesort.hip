@@ -381,6 +381,7 @@ void launchTileDepthSort(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, ui
     const uint32_t blocks = (tileCount + kTsWaves - 1) / kTsWaves;
     hipLaunchKernelGGL(kTileDepthSort, dim3(blocks), dim3(kTsThreads), 0, stream, keys, values, keysTmp, valuesTmp,
                        ranges, order, tileCount);
+    markKernel("kTileDepthSort", stream);
 }
 
 }  // namespace gsr

@@ -40,6 +40,7 @@ SORT_ALGORITHMS = {"reduce-then-scan": 0, "onesweep": 1}
 RENDER_LOOPS = {"culled": 0, "divergent": 1, "predicated": 2}
 SORT_MODES = {"binning": 0, "global": 1}
 RENDER_SPLATS = {"full": 0, "p32": 1, "p24": 2}
+TILE_STATS = ("list_length", "fetched", "evaluated", "blended", "terminated", "lane_slots", "max_depth", "pixels")
 STAGES = ("preprocess", "scan", "depth_sort", "duplicate", "sort", "ranges", "render", "total")
 CULL_VISIBLE, CULL_NEAR, CULL_SINGULAR, CULL_OFFSCREEN, CULL_LOW_OPACITY = 0, 1, 2, 3, 4
 
@@ -107,7 +108,8 @@ class Settings(ctypes.Structure):
         ("adaptive_initial_entries", ctypes.c_uint32),
         ("frame_stats", ctypes.c_int32),
         ("differentiable", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("tile_stats", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 1),
     ]
 
 
@@ -191,6 +193,8 @@ _read_control = _sig("gsr_read_control", _i32, [_vp, ctypes.POINTER(Control)])
 _read_lists = _sig("gsr_read_lists", _i32, [_vp, _vp, _vp, _vp, _vp, _u32])
 _read_splats = _sig("gsr_read_splats", _i32, [_vp, _vp, _vp])
 _read_stage_times = _sig("gsr_read_stage_times", _i32, [_vp, _vp, _u32])
+_read_tile_stats = _sig("gsr_read_tile_stats", _i32, [_vp, _vp, _u32])
+_read_kernel_times = _sig("gsr_read_kernel_times", _i32, [_vp, _vp, _vp, _u32, _vp])
 _read_packed = _sig("gsr_read_packed_splats", _i32, [_vp, _vp])
 _renderer_max_capacity = _sig("gsr_renderer_max_capacity", _u32, [_vp])
 _renderer_list_resizes = _sig("gsr_renderer_list_resizes", _u32, [_vp])
@@ -261,7 +265,7 @@ EXPORTED = (
     "gsr_scene_create gsr_scene_destroy gsr_encode_f16 gsr_camera_make gsr_camera_from_viewer "
     "gsr_renderer_create gsr_renderer_destroy gsr_renderer_capacity gsr_renderer_total_bytes gsr_render "
     "gsr_renderer_max_capacity gsr_renderer_list_resizes gsr_renderer_last_stats "
-    "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_sort_pairs "
+    "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_read_tile_stats gsr_read_kernel_times gsr_sort_pairs "
     "gsr_sort_pairs_noalloc gsr_exclusive_scan "
     "gsr_batch_create gsr_batch_destroy gsr_batch_branches gsr_batch_render gsr_batch_capture gsr_batch_replay "
     "gsr_colour_model_param_count gsr_colour_model_create gsr_colour_model_destroy gsr_colour_model_train "
@@ -522,17 +526,17 @@ class Renderer:
     """SplatTileRenderer for one scene (init + render + the test readbacks)."""
 
     def __init__(self, ctx: Context, scene: Scene, bounds: str = "tight", target: str = "rgba8", budget_mb: int = 256,
-                 max_sh_degree: int = 3, profile_stages: bool = False, pixels_per_thread: int = 0,
+                 max_sh_degree: int = 3, profile_stages: int = 0, pixels_per_thread: int = 0,
                  exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
                  sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full",
                  sort_window: int = 0, adaptive_list: bool = False, adaptive_initial_entries: int = 0,
-                 frame_stats: bool = False, differentiable: bool = False):
+                 frame_stats: bool = False, differentiable: bool = False, tile_stats: bool = False):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
         s.max_sh_degree = max_sh_degree
         s.format = TARGETS[target]
-        s.profile_stages = 1 if profile_stages else 0
+        s.profile_stages = int(profile_stages)  # False/True = 0/1; 2 = also per-kernel events
         s.render_pixels_per_thread = pixels_per_thread
         s.render_exact_alpha = 1 if exact_alpha else 0
         s.sort_algorithm = SORT_ALGORITHMS[sort_algorithm]
@@ -545,6 +549,7 @@ class Renderer:
         s.adaptive_initial_entries = adaptive_initial_entries
         s.frame_stats = 1 if frame_stats else 0
         s.differentiable = 1 if differentiable else 0
+        s.tile_stats = 1 if tile_stats else 0
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
@@ -681,6 +686,24 @@ class Renderer:
         ms = (ctypes.c_float * len(STAGES))()
         _check(_read_stage_times(self.handle, ctypes.cast(ms, _vp), len(STAGES)), "gsr_read_stage_times")
         return {k: float(v) for k, v in zip(STAGES, ms)}
+
+    def kernel_times(self):
+        """profile_stages=2: [(kernel name, ms)] of the last frame, in launch order."""
+        n = 96
+        names = (ctypes.c_char_p * n)()
+        ms = (ctypes.c_float * n)()
+        cnt = ctypes.c_uint32()
+        _check(_read_kernel_times(self.handle, ctypes.cast(names, _vp), ctypes.cast(ms, _vp), n, ctypes.byref(cnt)),
+               "gsr_read_kernel_times")
+        return [(names[i].decode(), float(ms[i])) for i in range(min(cnt.value, n))]
+
+    def tile_stats(self) -> np.ndarray:
+        """tile_stats=True: (tiles, 8) uint32 per-tile counters of the last frame (TILE_STATS order)."""
+        cam = self._cam
+        tiles = int(cam.tiles_x) * int(cam.tiles_y)
+        out = np.zeros((tiles, len(TILE_STATS)), np.uint32)
+        _check(_read_tile_stats(self.handle, _ptr(out), tiles), "gsr_read_tile_stats")
+        return out
 
     def close(self) -> None:
         if self.handle:

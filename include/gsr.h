@@ -125,7 +125,8 @@ typedef struct gsr_settings {
                                     2048 entries (splat_tile_renderer.cpp:117-124); 0 = 256 */
     int32_t max_sh_degree;       /* evaluated bands = min(scene degree, this) */
     gsr_target_format format;    /* output image format */
-    int32_t profile_stages;      /* 1: hipEvent pairs around every stage (gsr_read_stage_times) */
+    int32_t profile_stages;      /* 1: hipEvent pairs around every stage (gsr_read_stage_times); 2: also an
+                                    event after every kernel (gsr_read_kernel_times) */
     int32_t render_pixels_per_thread; /* tile_render: 1, 2 or 4 pixels per thread (0 = default 1) */
     int32_t render_exact_alpha;  /* 1: alpha = min(0.99, o expf(power)) in the reference's operation order for
                                     every pair; 0 (default): log2-domain alpha with v_exp_f32 and a cutoff
@@ -152,7 +153,10 @@ typedef struct gsr_settings {
                                     the render pass is the training forward (exact alpha, per-pixel final
                                     transmittance and last contributor). Needs format RGBA32F, render_splat
                                     FULL, sort_window 0 and an f32 scene */
-    uint32_t reserved[2];
+    int32_t tile_stats;          /* 1: every frame also counts the per-tile render statistics of the reference's
+                                    stats build (TileStat, splat_tile_renderer.h:130-141; gsr_read_tile_stats).
+                                    Culled render loop only; the counting slows the render pass */
+    uint32_t reserved[1];
 } gsr_settings;
 
 /* RenderSplatMode (splat_tile_renderer.h:88-92; render_splat.glsl:1-21), same values. The
@@ -273,6 +277,27 @@ gsr_status gsr_read_splats(gsr_renderer* r, gsr_projected_splat* splats, uint32_
  * and depths of the reference's call are in the 48-byte splats (gsr_read_splats). */
 gsr_status gsr_read_packed_splats(gsr_renderer* r, uint32_t* words);
 gsr_status gsr_read_stage_times(gsr_renderer* r, float* ms, uint32_t n);
+/* The reference's per-tile render statistics of the last frame (tile_stats = 1; TileStat,
+ * splat_tile_renderer.h:130-141, counted as tile_render.comp:78-90, 380-399): GSR_TILE_STAT_COUNT
+ * counters per tile, tile-major, for min(tiles, max_tiles) tiles. */
+typedef enum gsr_tile_stat {
+    GSR_TILE_STAT_LIST_LENGTH = 0, /* entries in the tile's range */
+    GSR_TILE_STAT_FETCHED = 1,     /* entries loaded before the whole tile stopped */
+    GSR_TILE_STAT_EVALUATED = 2,   /* (pixel, entry) pairs of the reference's loop: a pixel walks every fetched
+                                      entry until it terminates (the terminating one included) */
+    GSR_TILE_STAT_BLENDED = 3,     /* pairs blended */
+    GSR_TILE_STAT_TERMINATED = 4,  /* pixels that reached T < 1e-4 */
+    GSR_TILE_STAT_LANE_SLOTS = 5,  /* lane slots this library's render loop spent evaluating pairs (per wave and
+                                      batch: the list slots it walked x 64) */
+    GSR_TILE_STAT_MAX_DEPTH = 6,   /* the largest evaluated count of one pixel */
+    GSR_TILE_STAT_PIXELS = 7,      /* pixels of the tile inside the image */
+    GSR_TILE_STAT_COUNT = 8
+} gsr_tile_stat;
+gsr_status gsr_read_tile_stats(gsr_renderer* r, uint32_t* stats, uint32_t max_tiles);
+/* profile_stages = 2: the kernels of the last frame in launch order (static names) and the time of
+ * each in ms (hipEvents recorded between consecutive launches on the frame's stream). Fills
+ * min(kernels, max) entries (either array may be null); *count (may be null) = the kernels. */
+gsr_status gsr_read_kernel_times(gsr_renderer* r, const char** names, float* ms, uint32_t max, uint32_t* count);
 
 /* GpuRadixSort on its own (gpu_radix_sort.h:38-122, test_gpu_sort.cpp): stable LSD sort of
  * (key, value) pairs by the low `key_bits` bits of 64-bit keys, in place, on device

@@ -648,3 +648,71 @@ def test_exclusive_scan(gsr_ctx, n):
     ref = np.concatenate([[0], np.cumsum(v, dtype=np.uint64)[:-1]]).astype(np.uint32) if n else v
     assert np.array_equal(out, ref)
     assert total == int(v.astype(np.uint64).sum())
+
+
+@pytest.mark.parametrize("sort_mode", ["binning", "global"])
+def test_tile_stats_match_oracle_counts(gsr_ctx, sort_mode):
+    """The stats build of the render (tile_stats) on the small workload: per-tile list lengths and
+    pixel counts exact; the evaluated / blended pair and terminated pixel totals equal those of the
+    oracle's renderReference (same definitions: a pixel walks its tile's list until it terminates)
+    up to the alpha cutoff's rounding; the image is the plain render's, bit for bit."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case tile stats 100k-720p sort_mode={sort_mode}")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cam = _camera(gsr, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720)
+    r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", sort_mode=sort_mode, tile_stats=True)
+    plain = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", sort_mode=sort_mode)
+    img = r.render_to_host(cam).copy()
+    assert np.array_equal(img, plain.render_to_host(cam)), "the stats build changed the image"
+    st = r.tile_stats().astype(np.int64)
+    _, _, _, ranges, _ = r.read_back()
+    ranges = ranges.reshape(-1, 2).astype(np.int64)
+    assert np.array_equal(st[:, 0], ranges[:, 1] - ranges[:, 0])
+    tx, ty = np.meshgrid(np.arange(cam.tiles_x), np.arange(cam.tiles_y))
+    w = np.minimum(16, 1280 - 16 * tx).ravel()
+    h = np.minimum(16, 720 - 16 * ty).ravel()
+    assert np.array_equal(st[:, 7], w * h)
+    assert (st[:, 1] <= st[:, 0]).all() and (st[:, 4] <= st[:, 7]).all()
+    assert (st[st[:, 4] < st[:, 7], 1] == st[st[:, 4] < st[:, 7], 0]).all(), "a tile stopped with live pixels"
+    assert (st[:, 5] % 256 == 0).all() and (st[:, 3] <= st[:, 2]).all() and (st[:, 6] <= st[:, 1]).all()
+    _, ost = oracle.render(sc, oracle.OracleCamera.from_gsr(cam), bounds=oracle.TIGHT)
+    ev, bl, te = st[:, 2].sum(), st[:, 3].sum(), st[:, 4].sum()
+    log(f"  evaluated {ev} (oracle {ost['evaluatedFragments']}), blended {bl} ({ost['blendedFragments']}), "
+        f"terminated {te} ({ost['terminatedPixels']}), lane slots {st[:, 5].sum()}")
+    assert abs(ev - ost["evaluatedFragments"]) <= 1e-4 * ost["evaluatedFragments"]
+    assert abs(bl - ost["blendedFragments"]) <= 1e-4 * ost["blendedFragments"]
+    assert abs(te - ost["terminatedPixels"]) <= 1e-3 * max(ost["terminatedPixels"], 1)
+    r.close()
+    plain.close()
+    scene.close()
+
+
+def test_kernel_times(gsr_ctx):
+    """profile_stages=2: one event after every kernel of the frame; the kernels appear in launch
+    order and their times add up to the frame's stage total."""
+    import gsr
+    from gsr import camera, scenes
+
+    log("case kernel times")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cam = _camera(gsr, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720)
+    for mode, expect in (("global", ["kPreprocess", "kFrameSetup", "kDuplicate", "kSortHistogram", "kSortScan",
+                                     "kSortScatter", "kRanges", "kTileDepthSort", "kRenderCulled"]),
+                         ("binning", ["kPreprocess", "kFrameSetup", "kCompactVisible", "kGatherCounts",
+                                      "kDuplicate", "kRanges", "kRenderCulled"])):
+        r = gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode=mode, profile_stages=2)
+        r.render_to_host(cam)
+        kt = r.kernel_times()
+        names = [n for n, _ in kt]
+        pos = [names.index(k) for k in expect]
+        assert pos == sorted(pos), names
+        assert all(ms >= 0.0 for _, ms in kt)
+        total = r.stage_times()["total"]
+        assert abs(sum(ms for _, ms in kt) - total) <= 0.05 * total + 0.01, (sum(ms for _, ms in kt), total)
+        r.close()
+    scene.close()
