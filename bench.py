@@ -28,6 +28,7 @@ sys.path.insert(0, os.path.join(ROOT, "gaussian-splat-renderer_amd"))
 
 METRIC = "1080p ms/frame @1.2M splats at >=58.7 dB vs CPU ref; learned vs plain SH"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 ORBIT_FRAMES = 300
 
 
@@ -379,26 +380,51 @@ def main():
         assert timer.hip.hipStreamDestroy(hip_stream) == 0
         return ms
 
-    def profile_stages(scene, mode):
-        """Per-stage ms and list counts: a second renderer with events between the stages."""
-        rp = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", profile_stages=True, sort_mode=mode,
-                          render_splat=args.render_splat)
+    n_prof = min(args.steps, 60)
+    prof_cams = [cams[(args.warmup + i * (ORBIT_FRAMES // n_prof)) % ORBIT_FRAMES] for i in range(n_prof)]
+
+    def profile_stages(scene, mode, full_key_sort=False):
+        """Per-stage and per-kernel ms and list counts over n_prof frames spread over the orbit: a
+        second renderer with events between the stages and after every kernel (profile_stages=2).
+        Kernels launched several times per frame (the radix passes) are summed per frame, with
+        their launch count."""
+        rp = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", profile_stages=2, sort_mode=mode,
+                          render_splat=args.render_splat, full_key_sort=full_key_sort)
         stage_sum = {k: 0.0 for k in gsr.STAGES}
+        kern_sum, kern_calls = {}, {}
         entries = vis = 0
         culls = np.zeros(5, np.float64)
-        n_prof = min(args.steps, 60)
-        for i in range(n_prof):
-            cam = cams[(args.warmup + i * (ORBIT_FRAMES // n_prof)) % ORBIT_FRAMES]
+        for cam in prof_cams:
             rp.render(cam, out.ptr)
             st = rp.stage_times()
             ctl = rp.read_control()
             for k in stage_sum:
                 stage_sum[k] += st[k]
+            calls = {}
+            for name, ms in rp.kernel_times():
+                kern_sum[name] = kern_sum.get(name, 0.0) + ms
+                calls[name] = calls.get(name, 0) + 1
+            kern_calls.update(calls)
             entries += ctl.entries
             vis += ctl.visible
             culls += np.array(ctl.cull_counts[:5], np.float64)
         rp.close()
-        return {k: v / n_prof for k, v in stage_sum.items()}, entries / n_prof, vis / n_prof, culls / n_prof
+        kernels = {k: (v / n_prof, kern_calls[k]) for k, v in kern_sum.items()}
+        return ({k: v / n_prof for k, v in stage_sum.items()}, entries / n_prof, vis / n_prof, culls / n_prof,
+                kernels)
+
+    def render_pairs(scene, mode):
+        """The render's per-tile statistics (tile_stats build) summed per frame, mean over the same
+        frames: list entries, fetched entries, evaluated and blended (pixel, splat) pairs of the
+        reference's loop, terminated pixels, and the lane slots of this library's loop."""
+        rs = gsr.Renderer(ctx, scene, bounds=args.bounds, target="rgba8", sort_mode=mode,
+                          render_splat=args.render_splat, tile_stats=True)
+        tot = np.zeros(len(gsr.TILE_STATS), np.float64)
+        for cam in prof_cams:
+            rs.render(cam, out.ptr)
+            tot += rs.tile_stats().astype(np.float64).sum(axis=0)
+        rs.close()
+        return {k: v / n_prof for k, v in zip(gsr.TILE_STATS, tot)}
 
     other = "global" if args.sort_mode == "binning" else "binning"
     packed = [m for m in ("p32", "p24") if m != args.render_splat]
@@ -419,46 +445,108 @@ def main():
         elif rs != args.render_splat:
             splat_modes[rs] = round(ms / args.steps, 4)
         elif fmt == "f32":
-            st, ent, vis, culls = profile_stages(scene, mode)
+            st, ent, vis, culls, kern = profile_stages(scene, mode)
             modes[mode] = {"ms_per_frame": round(ms / args.steps, 4), "stages_ms": {k: round(v, 4) for k, v in st.items()},
                            "mean_entries": int(ent), "mean_visible": int(vis)}
             if mode == args.sort_mode:
-                stages, mean_entries, mean_visible, mean_culls = st, ent, vis, culls
+                stages, mean_entries, mean_visible, mean_culls, kernels = st, ent, vis, culls, kern
+                pairs = render_pairs(scene, mode)
+                if mode == "global":  # the full 64-bit (tile, depth) key radix sort, for its keys/s
+                    fst, fent, _, _, fkern = profile_stages(scene, mode, full_key_sort=True)
+                    full_sort = {"ms": fst["sort"], "entries": fent, "kernels": fkern}
                 # Quality: one frame (the first timed one) vs the oracle, RGBA8.
                 qcam = cams[args.warmup % ORBIT_FRAMES]
                 r.render(qcam, out.ptr)
                 img = out.read(np.empty((H, W, 4), np.uint8))
+                qsplats, _ = r.read_back_splats()
         r.close()
         scene.close()
 
-    # Roofline of the dominant HBM-bound kernel, kPreprocess (f32 scene): algorithmic bytes per
-    # launch / its launch time (hipEvents around it on the render stream, mean of the profiled
-    # frames). Reads: 16 B position + opacity per gaussian, 32 B scale + rotation unless culled at
-    # the near plane or for opacity, 4 x 48 B SH per gaussian that reaches the colour step
-    # (visible). Writes: the 48-B splat + 4-B tile count per gaussian, 8 B per tile range zeroed.
+    # Per-kernel algorithmic bytes (one launch; the frame's mean counts). N gaussians, V visible,
+    # M list entries, T tiles, G sort blocks. kPreprocess reads 16 B position + opacity per gaussian,
+    # 32 B scale + rotation unless culled at the near plane or for opacity and K x 12 B of SH per
+    # visible gaussian, and writes the 48-B splat + 4-B tile count per gaussian and 8 B per tile
+    # range zeroed; kFrameSetup reads the 32-B block records; kDuplicate reads 4 B count per gaussian
+    # and the 48-B splat per visible one and writes 8 + 4 B per entry; a radix pass reads 8-B keys
+    # (histogram), then 12 B per entry and writes 12 B per entry (scatter); kRanges reads the keys
+    # and writes 8 B per tile; the tile depth sort reads and writes 12 B per entry; the render
+    # reads 4 B value + 48 B splat per fetched entry and 8 B per tile range and writes 4 B per pixel.
     n = args.count
     tiles = cams[0].tiles
     near, low_opacity = mean_culls[1], mean_culls[4]
     K = sc.K
-    pre_bytes = n * 16 + (n - near - low_opacity) * 32 + mean_visible * K * 12 + n * 52 + tiles * 8
-    achieved = pre_bytes / (stages["preprocess"] * 1e-3) / 1e9
-    traffic = None
-    traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic_kPreprocess_1.2M-1080p-orbit.md")
-    if os.path.exists(traffic_file) and not args.ply and args.count == 1_200_000:
-        for line in open(traffic_file):
-            if line.startswith("HBM bytes per launch:"):
-                traffic = int(line.split(":")[1].split()[0])
-    roofline = {"bound": "hbm", "kernel": "kPreprocess<f32>", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "algorithmic_bytes": int(pre_bytes),
-                "note": "algorithmic bytes per launch / launch time (hipEvents on the render stream); traffic = "
-                        "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, scaled by a plain copy (profiles/)"}
+    M, V = mean_entries, mean_visible
+    blocks = min((int(M) + 2047) // 2048, 1024)
+    key_b = 8 if args.sort_mode == "global" else 4
+    pre_bytes = n * 16 + (n - near - low_opacity) * 32 + V * K * 12 + n * 52 + tiles * 8
+    algo = {"kPreprocess": pre_bytes,
+            "kFrameSetup": (n + 255) // 256 * 32 + (n + 255) // 256 * 4 + 80,
+            "kDuplicate": n * 4 + V * 48 + M * (key_b + 4),
+            "kSortHistogram": M * key_b + blocks * 256 * 4,
+            "kSortScan": 2 * blocks * 256 * 4,
+            "kSortScatter": 2 * M * (key_b + 4) + blocks * 256 * 4,
+            "kRanges": M * key_b + tiles * 8,
+            "kTileDepthSort": 2 * M * 12,
+            "kRenderCulled": pairs["fetched"] * 52 + tiles * 8 + W * H * 4}
+    traffic_by_kernel = {}
+    traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic_1.2M-1080p-orbit.md")
+    if os.path.exists(traffic_file) and not args.ply and args.count == 1_200_000 and args.sort_mode == "global":
+        for ln in open(traffic_file):
+            cells = [c.strip() for c in ln.split("|")]
+            if len(cells) > 5 and cells[1].startswith("k") and cells[4].replace(".", "").isdigit():
+                traffic_by_kernel[cells[1].split("<")[0].split(" ")[0]] = int(float(cells[4]))
+    kernel_table = {}
+    for name, (ms, calls) in sorted(kernels.items(), key=lambda kv: -kv[1][0]):
+        entry = {"us_per_frame": round(ms * 1e3, 2), "launches": calls}
+        if name in algo:
+            b = algo[name] * calls
+            entry.update({"algorithmic_bytes": int(b), "gb_per_s": round(b / (ms * 1e-3) / 1e9, 1),
+                          "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)})
+            if name in traffic_by_kernel:
+                entry["traffic_bytes"] = traffic_by_kernel[name] * calls
+        kernel_table[name] = entry
+
+    # Roofline of the dominant kernel, the render (kRenderCulled): it is bound by VALU issue, so its
+    # figure is the reference algorithm's arithmetic per launch over its launch time against the FP32
+    # peak. Per evaluated (pixel, splat) pair the reference's loop (tile_render.comp:335-352) does
+    # 16 operations (d = centre - pixel: 2; power: 9; power > 0: 1; exp: 1; o * exp: 1; min: 1;
+    # alpha < 1/255: 1) and per blended pair 10 more (T (1 - alpha): 2; the test: 1; alpha T and
+    # three colour FMAs: 7). Pair counts: the tile statistics of the same frames.
+    render_ms = kernels["kRenderCulled"][0]
+    ops = 16.0 * pairs["evaluated"] + 10.0 * pairs["blended"]
+    achieved = ops / (render_ms * 1e-3) / 1e12
+    render_bytes = algo["kRenderCulled"]
+    roofline = {"bound": "valu", "kernel": "kRenderCulled<rgba8, " + args.render_splat + ">",
+                "achieved": round(achieved, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": traffic_by_kernel.get("kRenderCulled"),
+                "algorithmic_bytes": int(render_bytes),
+                "hbm_gb_per_s": round(render_bytes / (render_ms * 1e-3) / 1e9, 1),
+                "hbm_frac": round(render_bytes / (render_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "launch_us": round(render_ms * 1e3, 2), "ops_per_launch": int(ops),
+                "pairs_per_launch": {k: int(v) for k, v in pairs.items() if k not in ("max_depth",)},
+                "note": "achieved = (16 ops x evaluated + 10 ops x blended pairs of the reference's tile loop, "
+                        "counted by the tile-stats build on the same frames) / launch time (hipEvents on the render "
+                        "stream); peak = MI355X FP32 157.3 TFLOP/s; traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                        "launch scaled by a plain copy (profiles/); lane_slots = (pixel, splat) evaluations this "
+                        "kernel's loop issued"}
     if args.sort_mode == "binning":
         passes = {"depth_radix": 4, "tile_radix": (int(tiles).bit_length() + 7) // 8}
         sort_ms = stages["sort"] + stages["depth_sort"]
     else:
         passes = {"tile_radix": (int(tiles).bit_length() + 7) // 8, "per_tile_depth_sort": 1}
         sort_ms = stages["sort"]
+    full_key = None
+    if args.sort_mode == "global":
+        fpasses = (32 + int(tiles).bit_length() + 7) // 8
+        fm = full_sort["entries"]
+        full_key = {"ms": round(full_sort["ms"], 4), "entries": int(fm), "passes": fpasses,
+                    "sorted_keys_per_s": round(fm / (full_sort["ms"] * 1e-3), 1),
+                    "key_passes_per_s": round(fm * fpasses / (full_sort["ms"] * 1e-3), 1),
+                    "gb_per_s": round(fm * fpasses * (8 + 24) / (full_sort["ms"] * 1e-3) / 1e9, 1),
+                    "note": "the 64-bit (tile, depth) key LSD radix sort of the frame's list, all key bits, "
+                            "reduce-then-scan; bytes = passes x (8 B histogram read + 12 B read + 12 B written) "
+                            "per entry"}
 
     multiview = None if args.no_multiview or args.ply else multiview_bench(ctx)
     learned = None if args.no_learned or args.ply else learned_bench(ctx, cams, min(args.steps, 100))
@@ -491,7 +579,7 @@ def main():
     scene.close()
 
     cpu = None
-    psnr = maxdiff = None
+    psnr = maxdiff = fp64_error = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
@@ -499,12 +587,13 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
         ocam = oracle.OracleCamera.from_gsr(qcam)
         ob = {"rect": oracle.RECT, "tight": oracle.TIGHT}[args.bounds]
-        times = []
+        times, phase_runs = [], []
         t_start = time.time()
         ref = None
         while True:
             rgb, st = oracle.render(sc, ocam, bounds=ob, threads=threads)
             times.append(st["totalMs"])
+            phase_runs.append(st)
             if ref is None:
                 ref = rgb
             if time.time() - t_start > args.cpu_seconds or len(times) >= 5:
@@ -512,9 +601,46 @@ def main():
         q = oracle.quantize8(ref)
         maxdiff = int(np.abs(img[..., :3].astype(np.int32) - q.astype(np.int32)).max())
         psnr = oracle.psnr(img[..., :3].astype(np.float64) / 255.0, ref)
+        phases = {k: round(float(np.median([p[k] for p in phase_runs])), 2)
+                  for k in ("preprocessMs", "sortMs", "binMs", "rasterMs")}
+        # Single-thread DepthSorter::radixSortPairs (depth_sort.cpp:44-75) on 2^22 positive-float depth keys.
+        rng = np.random.default_rng(0)
+        dkeys = rng.uniform(0.2, 30.0, 1 << 22).astype(np.float32).view(np.uint32)
+        t1 = time.perf_counter()
+        _, _, rpasses = oracle.radix_sort_pairs(dkeys, np.arange(dkeys.size, dtype=np.uint32))
+        radix_s = time.perf_counter() - t1
+        cpu_model = "unknown"
+        try:
+            for ln in open("/proc/cpuinfo"):
+                if ln.startswith("model name"):
+                    cpu_model = ln.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         cpu = {"value": round(float(np.median(times)), 2), "unit": "ms/frame", "cores": threads, "kind": "port",
                "sample": f"{len(times)} renders of orbit frame {args.warmup % ORBIT_FRAMES} ({workload}, "
-                         f"{args.bounds} bounds) by the CPU oracle (renderReference restated), median"}
+                         f"{args.bounds} bounds) by the CPU oracle (renderReference restated), median",
+               "phases_ms": {"preprocess": phases["preprocessMs"], "sort": phases["sortMs"], "bin": phases["binMs"],
+                             "raster": phases["rasterMs"]},
+               "cpu_model": cpu_model,
+               "radix_sort_pairs_1thread": {"keys": int(dkeys.size), "passes": rpasses,
+                                            "keys_per_s": round(dkeys.size / radix_s, 1)}}
+        # FP32 preprocess error of this GPU frame's splats against the FP64 twin of the oracle.
+        s64 = oracle.preprocess_f64(sc, ocam, threads=threads)
+        vis = (qsplats["cull_reason"] == 0) & (s64[:, 3] == 0)
+
+        def stat(a):
+            return {"median": float(f"{np.median(a):.3e}"), "max": float(f"{np.max(a):.3e}")}
+        fp64_error = {
+            "visible": int(vis.sum()),
+            "cull_mismatches": int((qsplats["cull_reason"] != s64[:, 3]).sum()),
+            "centre_px": stat(np.abs(qsplats["center"][vis] - s64[vis, 0:2]).max(axis=1)),
+            "conic_rel": stat((np.abs(qsplats["conic_opacity"][vis, :3] - s64[vis, 4:7]) /
+                               np.abs(s64[vis, 4:7]).max(axis=1, keepdims=True)).max(axis=1)),
+            "colour_abs": stat(np.abs(qsplats["color"][vis] - s64[vis, 8:11]).max(axis=1)),
+            "depth_rel": stat(np.abs(qsplats["depth"][vis] - s64[vis, 2]) / s64[vis, 2]),
+            "radius_px": stat(np.abs(qsplats["radius"][vis] - s64[vis, 11])),
+            "frame": args.warmup % ORBIT_FRAMES}
 
     line = {
         "metric": METRIC,
@@ -541,10 +667,13 @@ def main():
         "mean_visible": int(mean_visible),
         "sort_passes": passes,
         "sorted_keys_per_s": round(mean_entries / (sort_ms * 1e-3), 1),
+        "full_key_sort": full_key,
+        "kernels": kernel_table,
         "sort_modes": modes,
         "psnr_db_vs_oracle": None if psnr is None else round(psnr, 2),
         "max_diff_levels_vs_oracle": maxdiff,
         "roofline": roofline,
+        "preprocess_fp32_error_vs_fp64": fp64_error,
         "cpu_baseline": cpu,
         "multiview": multiview,
         "frames_in_flight_ms_per_frame": in_flight,
