@@ -308,6 +308,17 @@ __device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, 
     }
 }
 
+// Whether the conic (a, b, c) is positive definite (det = a c - b^2 from an exact product, so
+// that it does not cancel). The preprocess's conics are (the inverse of a positive definite
+// covariance); p24 render records round a, b, c to fp16 separately, which can make the conic of a
+// thin splat indefinite, and then the reference's `power > 0` test (tile_render.comp:340) drops
+// the pixels along its open directions.
+__device__ __forceinline__ bool conicPositiveDefinite(float a, float b, float c) {
+    const float bb = b * b;
+    const float det = fmaf(a, c, -bb) - fmaf(b, b, -bb);
+    return det > 0.0f && a > 0.0f && c > 0.0f;
+}
+
 // The quadrants (8x8 pixel blocks of the 16x16 tile at (tx0, ty0); bit w: quadrant
 // (w & 1, w >> 1)) in which a splat can reach a pixel. With the conic (a, b, c) the render takes a
 // pixel iff its q >= log2(1/255), i.e. iff Q = a dx^2 + 2 b dx dy + c dy^2 <= r2,

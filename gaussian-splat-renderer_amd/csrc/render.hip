@@ -99,11 +99,13 @@ struct PixelQ {
     float X, Y, X2, XY, Y2;
 };
 
-// A batch slot of the culled loop: the quadratic (s0, s1) and the colour, 48 B.
+// A batch slot of the culled loop: the quadratic (s0, s1), qmax and the colour, 48 B. qmax =
+// log2(o) for a splat whose conic is not positive definite (a pair with q > qmax has power > 0
+// and is not drawn), +inf otherwise (power <= 0 everywhere).
 struct __align__(16) SplatRecord {
     float4 s0;
     float2 s1;
-    float2 pad;
+    float qmax, pad;
     float4 col;
 };
 
@@ -156,6 +158,7 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
     __shared__ float4 sS0[kBatch];
     __shared__ float2 sS1[kBatch];
     __shared__ float4 sColor[kBatch];
+    __shared__ float sQmax[PRED ? kBatch : 1];  // PRED: qmax as in the culled loop's records
     static_assert(!PRED || (FAST && PPT == 1), "the predicated loop is the fast one-pixel form");
 
     // Row-interleaved XCD mapping: workgroup b runs on XCD b % 8; XCD x renders tile rows
@@ -195,6 +198,7 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
                 // Padding up to a multiple of 4 for the unrolled loop: q = -inf never passes.
                 sS0[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 sS1[slot] = make_float2(0.0f, -__builtin_inff());
+                sQmax[slot] = __builtin_inff();
             }
             if (e < range.y) {
                 uint32_t g = values[e];
@@ -205,6 +209,7 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
                 sColor[slot] = col;
                 if (PRED) {
                     tileQuadratic(cx, cy, co, float(tx0), float(ty0), sS0[slot], sS1[slot]);
+                    sQmax[slot] = conicPositiveDefinite(co.x, co.y, co.z) ? __builtin_inff() : __log2f(co.w);
                 } else if (FAST) {
                     sS0[slot] = make_float4(cx, cy, -0.5f * kLog2e * co.x, -kLog2e * co.y);
                     sS1[slot] = make_float2(-0.5f * kLog2e * co.z, __log2f(co.w));
@@ -229,7 +234,10 @@ __global__ __launch_bounds__(256 / PPT) void kRender(const float4* __restrict__ 
                 // The four q first (their LDS reads in flight together), then the blends in order.
                 float q[4];
 #pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) q[u] = evalQ(sS0[j + u], sS1[j + u], pq);
+                for (uint32_t u = 0; u < 4; ++u) {
+                    q[u] = evalQ(sS0[j + u], sS1[j + u], pq);
+                    q[u] = q[u] > sQmax[j + u] ? -__builtin_inff() : q[u];  // power > 0: not drawn
+                }
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;
@@ -362,6 +370,7 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
     if (t == 0) {
         sRec[kBatch].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         sRec[kBatch].s1 = make_float2(0.0f, -__builtin_inff());
+        sRec[kBatch].qmax = __builtin_inff();
         sRec[kBatch].col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     const char* recBase = reinterpret_cast<const char*>(sRec);
@@ -376,6 +385,7 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
         // Whole-tile vote; also the barrier that protects the previous batch's LDS.
         if (__syncthreads_count(thr < __builtin_inff() ? 0 : 1) == 256) break;
         if (STATS) statFetched += range.y - batch < uint32_t(kBatch) ? range.y - batch : uint32_t(kBatch);
+        bool indefinite = false;
         {
             const uint32_t e = batch + t;
             uint32_t mask = 0u;
@@ -389,6 +399,8 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
                 sRec[t].col = col;
                 const float logO = __log2f(co.w);
                 tileQuadratic(o0.x, o0.y, co, float(tx0), float(ty0), sRec[t].s0, sRec[t].s1);
+                indefinite = !conicPositiveDefinite(co.x, co.y, co.z);
+                sRec[t].qmax = indefinite ? logO : __builtin_inff();
                 // The quadrants the splat can reach (quadrantMask: box, then the exact
                 // ellipse-vs-quadrant test, both with a margin sized to the rounding of q).
                 const float r2 = 2.0f * (logO - kLog2MinAlpha) * (1.0f / kLog2e);
@@ -396,7 +408,9 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
             }
             sMask[t] = mask;
         }
-        __syncthreads();
+        // Also tells every wave whether the batch holds a splat with an indefinite conic (only
+        // p24 records of thin splats): only such batches test power <= 0 per pair.
+        const bool anyIndefinite = __syncthreads_or(indefinite ? 1 : 0) != 0;
         // This wave's list: the batch slots whose box meets its quadrant, in list order
         // (slot = j * 64 + lane), padded with the null slot to a multiple of 4.
         uint32_t count = 0;
@@ -424,6 +438,10 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
                 float q[4];
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) q[u] = evalQ(rec[u]->s0, rec[u]->s1, pq);
+                if (anyIndefinite) {  // uniform: a batch with an indefinite conic drops power > 0 pairs
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) q[u] = q[u] > rec[u]->qmax ? -__builtin_inff() : q[u];
+                }
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u) {
                     if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;

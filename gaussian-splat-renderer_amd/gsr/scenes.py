@@ -130,6 +130,42 @@ def make_test_scene(count: int, seed: int = 99, sh_degree: int = 1) -> Synthetic
                           np.ascontiguousarray(q.astype(np.float32)), np.ascontiguousarray(shv), sh_degree, seed)
 
 
+def make_needle_scene(count: int, seed: int = 77, sh_degree: int = 1) -> SyntheticScene:
+    """The tight-bounds test scene of test_tight_bounds.cpp:181-211 (150,000 gaussians there),
+    restated with this module's generator: the renderer test scene's 16^3 box, alpha uniform with
+    every 7th scaled by 0.01 (many below 1/255) and every 3rd in [0.95, 1), exact depth ties (every
+    4099th at z = -1), sigma = exp(U * 4 - 5.5), every 5th gaussian a needle with axes
+    (3 s, 0.01 s, 0.02 s) (the others (s, s (0.2 + U), s (0.2 + U))), arbitrary rotations
+    normalize(U4 - 0.5), SH DC in [-1, 1) and the other coefficients in [-0.25, 0.25)."""
+    n = int(count)
+    K = (sh_degree + 1) ** 2
+    u = uniform01(seed, 17, n * 8).reshape(n, 8).astype(np.float32)
+    p = u[:, 0:3] * np.float32(16.0) - np.float32(8.0)
+    i = np.arange(n)
+    alpha = u[:, 3].copy()
+    alpha[i % 7 == 0] *= np.float32(0.01)
+    opaque = i % 3 == 0
+    alpha[opaque] = np.float32(0.95) + np.float32(0.05) * u[opaque, 4]
+    p[i % 4099 == 0, 2] = np.float32(-1.0)
+    s = np.exp(u[:, 5] * np.float32(4.0) - np.float32(5.5)).astype(np.float32)
+    needle = i % 5 == 0
+    sx = s.copy()
+    sy = s * (np.float32(0.2) + u[:, 6])
+    sz = s * (np.float32(0.2) + u[:, 7])
+    sx[needle] = s[needle] * np.float32(3.0)
+    sy[needle] = s[needle] * np.float32(0.01)
+    sz[needle] = s[needle] * np.float32(0.02)
+    scales = np.stack([sx, sy, sz, np.zeros(n, np.float32)], axis=1).astype(np.float32)
+    q = (uniform01(seed, 18, n * 4).reshape(n, 4) - 0.5).astype(np.float32)
+    q /= np.sqrt((q * q).sum(axis=1, dtype=np.float32))[:, None]
+    shv = (uniform01(seed, 19, n * K * 3).reshape(n, K * 3) - 0.5).astype(np.float32)
+    shv[:, :3] *= np.float32(2.0)
+    shv[:, 3:] *= np.float32(0.5)
+    pos_alpha = np.concatenate([p, alpha[:, None]], axis=1).astype(np.float32)
+    return SyntheticScene(np.ascontiguousarray(pos_alpha), np.ascontiguousarray(scales),
+                          np.ascontiguousarray(q.astype(np.float32)), np.ascontiguousarray(shv), sh_degree, seed)
+
+
 # The benchmark workloads (names carry the sizes).
 WORKLOADS = {
     "100k-720p": dict(count=100_000, mean_sigma=2.0, width=1280, height=720),

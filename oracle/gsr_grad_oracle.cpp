@@ -334,9 +334,291 @@ void gradOf(const Params& p, const Camera<double>& cam, const float* G, double* 
     }
 }
 
+// --- Tiled form (large scenes) ------------------------------------------------------------------
+// The same forward, composite and gradient as lossOf / gradOf, but each pixel walks only the
+// splats whose tile rect holds its tile (per-tile lists in (depth, index) order), so the cost is
+// that of a render instead of pixels x gaussians. The central differences re-render only the
+// tiles the moved gaussian reaches before or after the move: every other pixel keeps its value,
+// so the change of the loss is the change over those tiles.
+
+struct Tiled {
+    std::vector<DSplat> splats;    // visible, stable-sorted by depth (index order within a depth)
+    std::vector<uint32_t> start;   // tiles + 1
+    std::vector<uint32_t> list;    // positions in splats, per tile in depth order
+};
+
+Tiled buildTiled(const Params& p, const Camera<double>& cam) {
+    Tiled t;
+    t.splats.reserve(p.n);
+    for (uint32_t i = 0; i < p.n; ++i) {
+        DSplat s;
+        if (splatOf(p, i, cam, &s)) t.splats.push_back(s);
+    }
+    std::stable_sort(t.splats.begin(), t.splats.end(),
+                     [](const DSplat& a, const DSplat& b) { return a.depth < b.depth; });
+    const uint32_t tiles = cam.tilesX * cam.tilesY;
+    t.start.assign(tiles + 1, 0);
+    for (const DSplat& s : t.splats)
+        for (int ty = s.rect.minY; ty < s.rect.maxY; ++ty)
+            for (int tx = s.rect.minX; tx < s.rect.maxX; ++tx) t.start[size_t(ty) * cam.tilesX + tx + 1]++;
+    for (uint32_t i = 0; i < tiles; ++i) t.start[i + 1] += t.start[i];
+    t.list.resize(t.start[tiles]);
+    std::vector<uint32_t> cursor(t.start.begin(), t.start.end() - 1);
+    for (uint32_t k = 0; k < t.splats.size(); ++k) {
+        const DSplat& s = t.splats[k];
+        for (int ty = s.rect.minY; ty < s.rect.maxY; ++ty)
+            for (int tx = s.rect.minX; tx < s.rect.maxX; ++tx) t.list[cursor[size_t(ty) * cam.tilesX + tx]++] = k;
+    }
+    return t;
+}
+
+// The composite of one pixel over splats in order (lossOf's loop).
+void shadePixel(const DSplat* const* seq, size_t count, uint32_t x, uint32_t y, double C[3]) {
+    double T = 1;
+    C[0] = C[1] = C[2] = 0;
+    for (size_t j = 0; j < count; ++j) {
+        const DSplat& s = *seq[j];
+        const double dx = s.cx - double(x), dy = s.cy - double(y);
+        const double power = -0.5 * (s.a * dx * dx + s.c * dy * dy) - s.b * dx * dy;
+        if (power > 0) continue;
+        const double e = s.o * std::exp(power);
+        const double alpha = e < double(kMaxAlpha) ? e : double(kMaxAlpha);
+        if (alpha < double(kMinAlpha)) continue;
+        const double testT = T * (1 - alpha);
+        if (testT < double(kMinTransmittance)) break;
+        for (int c = 0; c < 3; ++c) C[c] += s.col[c] * alpha * T;
+        T = testT;
+    }
+}
+
+// Sum over the pixels of tile `tile` of |C - target| (image: written if not null).
+double tileLoss(const Camera<double>& cam, uint32_t tile, const DSplat* const* seq, size_t count, const float* target,
+                double* image) {
+    const uint32_t tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+    const uint32_t x0 = tx * kTileSize, y0 = ty * kTileSize;
+    const uint32_t x1 = std::min(cam.width, x0 + kTileSize), y1 = std::min(cam.height, y0 + kTileSize);
+    double sum = 0;
+    for (uint32_t y = y0; y < y1; ++y)
+        for (uint32_t x = x0; x < x1; ++x) {
+            double C[3];
+            shadePixel(seq, count, x, y, C);
+            const size_t px = (size_t(y) * cam.width + x) * 3;
+            for (int c = 0; c < 3; ++c) {
+                if (image) image[px + c] = C[c];
+                if (target) sum += std::fabs(C[c] - double(target[px + c]));
+            }
+        }
+    return sum;
+}
+
+template <typename Fn>
+void parallelTasks(uint64_t tasks, unsigned threads, Fn&& fn) {
+    std::atomic<uint64_t> next{0};
+    auto work = [&](unsigned worker) {
+        for (uint64_t t = next++; t < tasks; t = next++) fn(t, worker);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < threads; ++i) pool.emplace_back(work, i);
+    work(0);
+    for (std::thread& th : pool) th.join();
+}
+
+// The tile's splats as pointers, with gaussian g's splat left out and `moved` (if not null and
+// its rect holds the tile) put in its (depth, index) place.
+void tileSeq(const Tiled& t, const Camera<double>& cam, uint32_t tile, uint32_t g, const DSplat* moved,
+             std::vector<const DSplat*>& out) {
+    out.clear();
+    const uint32_t tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+    const bool in = moved && int(tx) >= moved->rect.minX && int(tx) < moved->rect.maxX && int(ty) >= moved->rect.minY &&
+                    int(ty) < moved->rect.maxY;
+    bool placed = !in;
+    for (uint32_t j = t.start[tile]; j < t.start[tile + 1]; ++j) {
+        const DSplat& s = t.splats[t.list[j]];
+        if (s.index == g) continue;
+        if (!placed && (s.depth > moved->depth || (s.depth == moved->depth && s.index > g))) {
+            out.push_back(moved);
+            placed = true;
+        }
+        out.push_back(&s);
+    }
+    if (!placed) out.push_back(moved);
+}
+
 }  // namespace
 
 extern "C" {
+
+// The tiled FP64 references for large scenes (same forward as oracle_diff_render_f64, each pixel
+// walking only its tile's list). image (H x W x 3) and *loss (L1 against target) if not null;
+// for the `count` gaussians `which`: grad (count x (11 + 3K)) = rows of oracle_diff_grad_f64 for
+// dL/dimage = gradImage, and fd (count x (11 + 3K)) = oracle_diff_fd_f64's central differences
+// with step h, each from a re-render of the tiles the gaussian reaches at x, x + h or x - h.
+void oracle_diff_local_f64(uint32_t n, const double* means, const double* logScales, const double* quats,
+                           const double* logits, const double* sh, uint32_t K, int32_t degree,
+                           const float viewInria[16], uint32_t width, uint32_t height, float tanX, float tanY,
+                           const float* target, const float* gradImage, uint32_t count, const uint32_t* which,
+                           double h, double* image, double* lossOut, double* grad, double* fd, int32_t threadsArg) {
+    const Camera<double> cam = makeCamera<double>(viewInria, width, height, tanX, tanY);
+    const Params p{n, means, logScales, quats, logits, sh, K, degree};
+    const unsigned threads = threadsArg > 0 ? unsigned(threadsArg) : std::max(1u, std::thread::hardware_concurrency());
+    const Tiled t = buildTiled(p, cam);
+    const uint32_t tiles = cam.tilesX * cam.tilesY;
+    const uint32_t P = 11 + 3 * K;
+    auto seqOf = [&](uint32_t tile, std::vector<const DSplat*>& out) {
+        out.clear();
+        for (uint32_t j = t.start[tile]; j < t.start[tile + 1]; ++j) out.push_back(&t.splats[t.list[j]]);
+    };
+    if (image || lossOut) {
+        std::vector<double> partial(tiles, 0.0);
+        parallelTasks(tiles, threads, [&](uint64_t tile, unsigned) {
+            std::vector<const DSplat*> seq;
+            seqOf(uint32_t(tile), seq);
+            partial[tile] = tileLoss(cam, uint32_t(tile), seq.data(), seq.size(), target, image);
+        });
+        double sum = 0;
+        for (double v : partial) sum += v;
+        if (lossOut) *lossOut = target ? sum / (3.0 * double(width) * double(height)) : 0.0;
+    }
+    std::vector<int32_t> slot(n, -1);
+    for (uint32_t k = 0; k < count; ++k) slot[which[k]] = int32_t(k);
+    if (grad && gradImage) {
+        // gradOf's per-pixel reverse pass over the tile lists, accumulated only for `which`.
+        std::vector<std::vector<double>> local(threads, std::vector<double>(size_t(count) * 9, 0.0));
+        parallelTasks(tiles, threads, [&](uint64_t tileIdx, unsigned worker) {
+            const uint32_t tile = uint32_t(tileIdx);
+            std::vector<double>& gs = local[worker];
+            struct Hit {
+                const DSplat* s;
+                double alpha, T, e, dx, dy;
+                bool clamped;
+            };
+            std::vector<Hit> hits;
+            const uint32_t tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+            const uint32_t x0 = tx * kTileSize, y0 = ty * kTileSize;
+            const uint32_t x1 = std::min(cam.width, x0 + kTileSize), y1 = std::min(cam.height, y0 + kTileSize);
+            for (uint32_t y = y0; y < y1; ++y)
+                for (uint32_t x = x0; x < x1; ++x) {
+                    hits.clear();
+                    double T = 1;
+                    for (uint32_t j = t.start[tile]; j < t.start[tile + 1]; ++j) {
+                        const DSplat& s = t.splats[t.list[j]];
+                        const double dx = s.cx - double(x), dy = s.cy - double(y);
+                        const double power = -0.5 * (s.a * dx * dx + s.c * dy * dy) - s.b * dx * dy;
+                        if (power > 0) continue;
+                        const double e = std::exp(power);
+                        const double oe = s.o * e;
+                        const double alpha = oe < double(kMaxAlpha) ? oe : double(kMaxAlpha);
+                        if (alpha < double(kMinAlpha)) continue;
+                        const double testT = T * (1 - alpha);
+                        if (testT < double(kMinTransmittance)) break;
+                        hits.push_back({&s, alpha, T, e, dx, dy, !(oe < double(kMaxAlpha))});
+                        T = testT;
+                    }
+                    const float* gi = gradImage + (size_t(y) * cam.width + x) * 3;
+                    double B[3] = {0, 0, 0};
+                    for (size_t hh = hits.size(); hh-- > 0;) {
+                        const Hit& hit = hits[hh];
+                        const DSplat& s = *hit.s;
+                        const int32_t k = slot[s.index];
+                        double dAlpha = 0;
+                        for (int c = 0; c < 3; ++c) {
+                            if (k >= 0) gs[size_t(k) * 9 + 6 + c] += double(gi[c]) * hit.alpha * hit.T;
+                            dAlpha += double(gi[c]) * (s.col[c] * hit.T - B[c] / (1 - hit.alpha));
+                            B[c] += s.col[c] * hit.alpha * hit.T;
+                        }
+                        if (k < 0 || hit.clamped) continue;
+                        double* gsp = &gs[size_t(k) * 9];
+                        const double dPower = dAlpha * hit.alpha;
+                        gsp[0] += -dPower * (s.a * hit.dx + s.b * hit.dy);
+                        gsp[1] += -dPower * (s.c * hit.dy + s.b * hit.dx);
+                        gsp[2] += -0.5 * dPower * hit.dx * hit.dx;
+                        gsp[3] += -dPower * hit.dx * hit.dy;
+                        gsp[4] += -0.5 * dPower * hit.dy * hit.dy;
+                        gsp[5] += dAlpha * hit.e;
+                    }
+                }
+        });
+        std::vector<double> gs(size_t(count) * 9, 0.0);
+        for (const auto& l : local)
+            for (size_t i = 0; i < gs.size(); ++i) gs[i] += l[i];
+        Camera<Dual> camD;
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r) camD.view[c][r] = cam.view[c][r];
+        for (int i = 0; i < 3; ++i) camD.pos[i] = cam.pos[i];
+        camD.width = cam.width;
+        camD.height = cam.height;
+        camD.tanHalfFovX = cam.tanHalfFovX;
+        camD.tanHalfFovY = cam.tanHalfFovY;
+        camD.focalX = cam.focalX;
+        camD.focalY = cam.focalY;
+        camD.tilesX = cam.tilesX;
+        camD.tilesY = cam.tilesY;
+        std::fill(grad, grad + size_t(count) * P, 0.0);
+        for (uint32_t k = 0; k < count; ++k) {
+            DSplat probe;
+            if (!splatOf(p, which[k], cam, &probe)) continue;  // culled: zeros
+            const double* gsp = &gs[size_t(k) * 9];
+            const GSplat d = dualSplat(p, which[k], camD);
+            const Dual* q[6] = {&d.cx, &d.cy, &d.a, &d.b, &d.c, &d.o};
+            double* o = grad + size_t(k) * P;
+            for (int j = 0; j < 11; ++j) {
+                double v = 0;
+                for (int m = 0; m < 6; ++m) v += gsp[m] * q[m]->d[j];
+                for (int c = 0; c < 3; ++c) v += gsp[6 + c] * d.col[c].d[j];
+                o[j] = v;
+            }
+            for (uint32_t kk = 0; kk < K; ++kk)
+                for (int c = 0; c < 3; ++c) o[11 + kk * 3 + c] = gsp[6 + c] * d.col[c].d[11 + kk];
+        }
+    }
+    if (fd && target) {
+        const double norm = 3.0 * double(width) * double(height);
+        parallelTasks(uint64_t(count) * P, threads, [&](uint64_t task, unsigned) {
+            const uint32_t g = which[task / P], k = uint32_t(task % P);
+            std::vector<double> m(means + 3 * size_t(g), means + 3 * size_t(g) + 3);
+            std::vector<double> ls(logScales + 3 * size_t(g), logScales + 3 * size_t(g) + 3);
+            std::vector<double> q(quats + 4 * size_t(g), quats + 4 * size_t(g) + 4);
+            double lg = logits[g];
+            std::vector<double> shg(sh + size_t(g) * K * 3, sh + size_t(g + 1) * K * 3);
+            // A one-gaussian parameter set addressing gaussian g's own (moved) values.
+            auto splatAt = [&](double delta, DSplat* out) {
+                double* x;
+                if (k < 3) x = &m[k];
+                else if (k < 6) x = &ls[k - 3];
+                else if (k < 10) x = &q[k - 6];
+                else if (k < 11) x = &lg;
+                else x = &shg[k - 11];
+                const double x0 = *x;
+                *x = x0 + delta;
+                const Params one{1, m.data(), ls.data(), q.data(), &lg, shg.data(), K, degree};
+                const bool vis = splatOf(one, 0, cam, out);
+                *x = x0;
+                if (vis) out->index = g;
+                return vis;
+            };
+            DSplat plus, minus, base;
+            const bool vp = splatAt(h, &plus), vm = splatAt(-h, &minus), v0 = splatAt(0.0, &base);
+            // The tiles any of the three reaches.
+            std::vector<uint32_t> touched;
+            for (const DSplat* s : {v0 ? &base : nullptr, vp ? &plus : nullptr, vm ? &minus : nullptr}) {
+                if (!s) continue;
+                for (int ty = s->rect.minY; ty < s->rect.maxY; ++ty)
+                    for (int tx = s->rect.minX; tx < s->rect.maxX; ++tx) touched.push_back(uint32_t(ty) * cam.tilesX + tx);
+            }
+            std::sort(touched.begin(), touched.end());
+            touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+            std::vector<const DSplat*> seq;
+            double lp = 0, lm = 0;
+            for (uint32_t tile : touched) {
+                tileSeq(t, cam, tile, g, vp ? &plus : nullptr, seq);
+                lp += tileLoss(cam, tile, seq.data(), seq.size(), target, nullptr);
+                tileSeq(t, cam, tile, g, vm ? &minus : nullptr, seq);
+                lm += tileLoss(cam, tile, seq.data(), seq.size(), target, nullptr);
+            }
+            fd[task] = (lp - lm) / norm / (2 * h);
+        });
+    }
+}
 
 // The FP64 image (H x W x 3) and L1 loss against target (may be null: loss 0).
 double oracle_diff_render_f64(uint32_t n, const double* means, const double* logScales, const double* quats,

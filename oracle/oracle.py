@@ -363,6 +363,37 @@ def diff_grad_f64(params: RawParams, cam: OracleCamera, grad_image):
     return out
 
 
+_diff_local = _sig("oracle_diff_local_f64", None,
+                   [ctypes.c_uint32] + [ctypes.c_void_p] * 5 + [ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p,
+                                                                ctypes.c_uint32, ctypes.c_uint32, _f32, _f32,
+                                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                                ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                                ctypes.c_int32])
+
+
+def diff_local_f64(params: RawParams, cam: OracleCamera, target=None, grad_image=None, which=(), h=1e-4,
+                   want_image=False, want_fd=True, threads=None):
+    """The tiled FP64 references for large scenes (each pixel walks only its tile's splats): dict of
+    loss (L1 against target) and image (if want_image), and for the gaussians `which` the exact
+    gradient rows for dL/dimage = grad_image ("grad") and the central differences with step h
+    ("fd", from re-renders of the tiles the moved gaussian reaches), both (len(which), 11 + 3K)
+    in the order of diff_fd_f64."""
+    threads = default_threads() if threads is None else threads
+    w = np.ascontiguousarray(which, np.uint32)
+    P = 11 + 3 * params.K
+    t = None if target is None else np.ascontiguousarray(np.asarray(target, np.float32)[..., :3])
+    g = None if grad_image is None else np.ascontiguousarray(np.asarray(grad_image, np.float32)[..., :3])
+    img = np.zeros((cam.height, cam.width, 3), np.float64) if want_image else None
+    grad = np.zeros((w.size, P), np.float64) if g is not None and w.size else None
+    fd = np.zeros((w.size, P), np.float64) if want_fd and t is not None and w.size else None
+    loss = ctypes.c_double(0.0)
+    _diff_local(*params._args(), *cam.args(), None if t is None else _p(t), None if g is None else _p(g), w.size,
+                _p(w) if w.size else None, float(h), None if img is None else _p(img), ctypes.byref(loss),
+                None if grad is None else _p(grad), None if fd is None else _p(fd), int(threads))
+    return {"loss": float(loss.value), "image": img, "grad": grad, "fd": fd}
+
+
 def quantize8(rgb):
     """writePngRgb (image_io.cpp:51-57): clamp to [0, 1], NaN -> 0, lround(v * 255)."""
     v = np.nan_to_num(np.clip(np.asarray(rgb, np.float32), 0.0, 1.0), nan=0.0).astype(np.float32)

@@ -386,3 +386,66 @@ def test_batch_graph_with_learned_colour(gsr_ctx):
         o.close()
     m.close()
     scene.close()
+
+
+def test_learned_colour_3m_1080p_workload(gsr_ctx, oracle):
+    """The learned-colour workload at its own size: 3M gaussians (means ~ N(0, 3^2 I), seed 0, SH
+    degree 3), the model trained on 2^24 (gaussian, direction, SH colour) samples (512 steps of
+    32768), the fallback pass at 2/255 (its marks = the gaussians whose error is within it), and the
+    1080p orbit frame 50 (radius 10, elevation 15) rendered with half of the gaussians learned (the
+    fallback at the median error): the splats drawn with the expected colours (MLP where marked, SH
+    elsewhere) and the RGBA8 image within one level of the oracle's render of a degree-0 scene with
+    those colours; with no gaussian marked the render is the plain SH render, bit for bit."""
+    import gsr
+    from gsr import camera, scenes
+
+    log("case learned colour 3M-1080p")
+    sc = scenes.make_scene(3_000_000, seed=0, mean_sigma=3.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    m = gsr.ColourModel(gsr_ctx, scene, seed=0)
+    last = m.train(512, 1 << 15, lr_weights=2e-3, lr_latents=2e-2)
+    held = m.loss(0x5EED, 1 << 20)
+    log(f"  trained 2^24 samples: last step MSE {last:.5f}, held-out MSE {held:.5f}")
+    assert held < 0.02 and abs(held - last) < 0.2 * held
+    W, H = 1920, 1080
+    cam = gsr.camera_from_viewer(camera.orbit_view(10.0, 15.0, 360.0 * 50 / 300), camera.perspective(60.0, W / H),
+                                 camera.IDENTITY, W, H)
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    plain = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+    want_plain = plain.render_to_host(cam).copy()
+    r = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+    r.set_colour_model(m)
+    n, err = m.fallback(2.0 / 255.0)
+    assert n == int((err <= np.float32(2.0 / 255.0)).sum())
+    log(f"  fallback at 2/255: {n} of {sc.count} learned; median error {np.median(err):.4f}")
+    if n == 0:
+        assert np.array_equal(r.render_to_host(cam), want_plain)
+    thr = float(np.median(err))
+    n, _ = m.fallback(thr)
+    marks = err <= np.float32(thr)
+    assert n == int(marks.sum()) and 0.4 * sc.count < n < 0.6 * sc.count
+    img = r.render_to_host(cam)
+    o_splats, _ = oracle.preprocess(sc, ocam, bounds=oracle.TIGHT)
+    col = o_splats["color"].copy()  # SH colours (the oracle's evalSH) of the gaussians not marked
+    pos = sc.pos_alpha[:, :3]
+    camp = np.asarray(cam.position[:3], np.float32)
+    dx, dy, dz = (pos[:, k] - camp[k] for k in range(3))
+    inv = np.float32(1.0) / np.sqrt((dx * dx + dy * dy) + dz * dz)
+    d = np.stack([dx * inv, dy * inv, dz * inv], 1).astype(np.float32)
+    gs, gc = r.read_back_splats()
+    drawn = (gs["cull_reason"] == 0) & (gc > 0)
+    idx = np.nonzero(marks & drawn)[0]
+    col[idx] = np.maximum(m.evaluate(idx.astype(np.uint32), d[idx]), 0)
+    diff = np.abs(gs["color"][drawn] - col[drawn]).max(axis=1)
+    log(f"  {drawn.sum()} splats drawn ({idx.size} learned): colours exact for {(diff == 0).mean() * 100:.2f}%, "
+        f"max diff {diff.max():.2e}")
+    assert (diff == 0).mean() > 0.99 and diff.max() < 2e-2
+    ref, _ = oracle.render(_degree0(sc, col), ocam, bounds=oracle.TIGHT)
+    d8 = np.abs(img[..., :3].astype(np.int32) - oracle.quantize8(ref).astype(np.int32))
+    pl = oracle.psnr(img[..., :3] / 255.0, want_plain[..., :3] / 255.0)
+    log(f"  half learned: max level diff vs oracle {d8.max()}; PSNR vs plain SH {pl:.2f} dB")
+    assert d8.max() <= 1
+    for x in (r, plain):
+        x.close()
+    m.close()
+    scene.close()

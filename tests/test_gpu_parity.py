@@ -531,22 +531,77 @@ def test_onesweep_lists_equal_reduce_then_scan(gsr_ctx):
     scene.close()
 
 
-def test_1_2m_1080p_orbit_frame(gsr_ctx):
-    """The headline workload at full size: 1.2M gaussians, 1920x1080, orbit frame 40 (48 deg)."""
+@pytest.mark.parametrize("sort_mode", ["global", "binning"])
+@pytest.mark.parametrize("fmt", ["f32", "f16"])
+def test_1_2m_1080p_orbit_frame(gsr_ctx, sort_mode, fmt):
+    """The headline workload at full size: 1.2M gaussians (means ~ N(0, 3^2 I), seed 0), 1920x1080,
+    orbit frame 40 (48 deg, radius 10, elevation 15), tight bounds, in both sort modes and with the
+    f32 and fp16 scene storage: splats, tile lists, ranges and control counts bit-exact, the float
+    image > 60 dB and the RGBA8 image within one level of the quantised oracle image."""
+    import gsr
     import oracle
     from gsr import camera, scenes
 
-    log("case 1.2M-1080p orbit frame 40, tight")
+    log(f"case 1.2M-1080p orbit frame 40, tight, sort_mode={sort_mode}, scene {fmt}")
     sc = scenes.make_scene(1_200_000, seed=0, mean_sigma=3.0)
     view = camera.orbit_view(10.0, 15.0, 360.0 * 40 / 300)
-    r, scene, cam, img, sc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 1920, 1080, "tight")
+    r, scene, cam, img, osc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 1920, 1080, "tight", fmt=fmt,
+                                                              sort_mode=sort_mode)
     gs, gc = r.read_back_splats()
     _check_splats(gs, gc, os_, oc)
     ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
-    ref, st = oracle.render(sc, ocam, bounds=ob)
+    ref, st = oracle.render(osc, ocam, bounds=ob)
     p = oracle.psnr(img[..., :3], ref)
-    log(f"  V {ctl.visible}, M {ctl.duplicates}, PSNR {p:.2f} dB")
+    r8 = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba8", sort_mode=sort_mode)
+    img8 = r8.render_to_host(cam)
+    diff = np.abs(img8[..., :3].astype(np.int32) - oracle.quantize8(ref).astype(np.int32))
+    p8 = oracle.psnr(img8[..., :3].astype(np.float64) / 255.0, ref)
+    log(f"  V {ctl.visible}, M {ctl.duplicates}, float PSNR {p:.2f} dB, RGBA8 {p8:.2f} dB, max {diff.max()} level")
+    assert p > 60.0 and diff.max() <= 1 and p8 >= 58.7
+    assert ctl.duplicates == st["tileInstances"]
+    r8.close()
+    r.close()
+    scene.close()
+
+
+@pytest.mark.parametrize("bounds,sort_mode,render_splat", [("tight", "binning", "full"), ("tight", "global", "full"),
+                                                           ("rect", "binning", "full"), ("tight", "binning", "p32"),
+                                                           ("tight", "global", "p24"),
+                                                           ("cutoff-only", "binning", "full")])
+def test_needle_scene_lists_and_image(gsr_ctx, bounds, sort_mode, render_splat):
+    """The reference's tight-bounds scene (test_tight_bounds.cpp:181-211, 150k gaussians: every 5th a
+    needle with axes (3 s, 0.01 s, 0.02 s), arbitrary rotations, every 7th alpha below 1/255 x 2.55,
+    near-opaque and tied-depth gaussians) seen from (0.3, 0.5, 6) at 640x360 (:218-225), checked
+    against the oracle as test_tight_bounds.cpp:343-398 checks the GPU against its CPU
+    construction, but with nothing left out: splats and tile counts bit-exact, lists and ranges
+    bit-exact (no borderline exclusions), no overflow or coverage mismatch, and the tight and
+    cutoff-only images against the oracle render with the same bounds; tight bounds drop entries
+    and leave the image of rect bounds unchanged (tight_bounds.cpp:433-447)."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case needle scene {bounds} {sort_mode} {render_splat}")
+    sc = scenes.make_needle_scene(150_000, seed=77)
+    view = camera.look_at([0.3, 0.5, 6.0], [0.0, 0.0, 0.0])
+    r, scene, cam, img, osc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 640, 360, bounds, sort_mode=sort_mode,
+                                                              render_splat=render_splat)
+    gs, gc = r.read_back_splats()
+    _check_splats(gs, gc, os_, oc)
+    ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+    assert ctl.overflow == 0 and ctl.coverage_mismatch == 0
+    ref, st = oracle.render(osc, ocam, bounds=ob, render_splat=render_splat)
+    p = oracle.psnr(img[..., :3], ref)
+    rect = gsr.Renderer(gsr_ctx, scene, bounds="rect", target="rgba32f", sort_mode=sort_mode, render_splat=render_splat)
+    rimg = rect.render_to_host(cam)
+    rctl = rect.read_control()
+    log(f"  entries {ctl.entries} (rect {rctl.entries}), needles visible "
+        f"{int(((gs['cull_reason'] == 0) & (np.arange(sc.count) % 5 == 0)).sum())}, PSNR {p:.2f} dB")
     assert p > 60.0
+    if bounds == "tight":
+        assert ctl.entries < rctl.entries
+        assert np.array_equal(img, rimg), "tight bounds changed the image"
+    rect.close()
     r.close()
     scene.close()
 

@@ -161,3 +161,32 @@ def test_exact_gradient_is_linear_in_image_gradient(oracle):
     g12 = oracle.diff_grad_f64(p, ocam, G1 + 2 * G2)
     assert np.abs(g12 - (g1 + 2 * g2)).max() <= 1e-12 * np.abs(g12).max()
     assert np.all(oracle.diff_grad_f64(p, ocam, np.zeros(shape, np.float32)) == 0)
+
+
+def test_tiled_fp64_references_equal_the_full_ones(oracle):
+    """The tiled FP64 references (per-tile lists; central differences from re-renders of the tiles the
+    moved gaussian reaches) give the full references' image, loss, gradient rows and central
+    differences: the same arithmetic per pixel, only the splats that cannot reach the pixel's tile
+    (and the pixels a move cannot change) are skipped."""
+    raw = list(raw_scene(n=300, seed=21))
+    cam = _camera(oracle, 128, 80)
+    raw[0] = raw[0].copy()
+    raw[0][:2] = 3.0 * np.asarray(cam.position, np.float64)  # behind the camera: culled at the near plane
+    p = params_of(oracle, raw)
+    rng = np.random.default_rng(4)
+    target = rng.uniform(0, 0.6, (cam.height, cam.width, 3)).astype(np.float32)
+    G = (rng.normal(size=(cam.height, cam.width, 3)) / (3 * cam.width * cam.height)).astype(np.float32)
+    loss, img = oracle.diff_render_f64(p, cam, target)
+    exact = oracle.diff_grad_f64(p, cam, G)
+    which = np.nonzero(np.abs(exact).sum(1) > 0)[0][:24]
+    s32, _ = oracle.preprocess(f32_scene(raw), cam)
+    assert np.all(s32["cull_reason"][:2] == 1)
+    which = np.concatenate([which[which > 1], [0, 1]])  # the two culled ones too
+    fd = oracle.diff_fd_f64(p, cam, target, which, h=1e-4)
+    loc = oracle.diff_local_f64(p, cam, target, G, which, h=1e-4, want_image=True)
+    assert np.array_equal(loc["image"], img)
+    assert abs(loc["loss"] - loss) <= 1e-12 * loss
+    assert np.allclose(loc["grad"], exact[which], rtol=1e-10, atol=1e-14 * np.abs(exact).max())
+    # the full differences subtract two sums over every pixel (rounding ~1e-9 of the largest)
+    assert np.allclose(loc["fd"], fd, rtol=1e-7, atol=1e-8 * np.abs(fd).max())
+    assert np.all(loc["grad"][-2:] == 0) and np.all(loc["fd"][-2:] == 0)
