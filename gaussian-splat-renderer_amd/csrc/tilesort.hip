@@ -233,6 +233,92 @@ __device__ __forceinline__ void waveSortAny(uint64_t* keys, uint32_t* values, ui
 
 // ---- long ranges: the whole workgroup ----
 
+constexpr uint32_t kGroupChunk = 4u * kWaveChunk;  // ranges up to 8192: the register form below
+
+// Storage slot in sW of chunk position q = lane * 32 + r (slot r * 64 + lane: a wave's accesses
+// to one register index hit 64 consecutive words).
+__device__ __forceinline__ uint32_t chunkSlot(uint32_t lane, uint32_t r) { return r * 64u + lane; }
+
+// Cross-wave compare-exchange of two 2048-word chunks held in registers by waves a (the lower
+// positions) and b: position q of a pairs with position q of b, or with 2047 - q when `flip`.
+// a keeps the minimum, b the maximum, exchanged in place through sW: b writes its words, a
+// reads each partner, keeps the minimum and writes the maximum into the same slot, b reads it
+// back. Every thread of the workgroup calls it (barriers).
+__device__ __forceinline__ void crossExchange(uint64_t (&w)[32], uint32_t wave, uint32_t lane, uint32_t a, uint32_t b,
+                                              bool flip, uint64_t* sW) {
+    if (wave == b) {
+#pragma unroll
+        for (uint32_t r = 0; r < 32u; ++r) sW[chunkSlot(lane, r)] = w[r];
+    }
+    __syncthreads();
+    if (wave == a) {
+#pragma unroll
+        for (uint32_t r = 0; r < 32u; ++r) {
+            const uint32_t slot = flip ? chunkSlot(63u - lane, 31u - r) : chunkSlot(lane, r);
+            const uint64_t y = sW[slot];
+            sW[slot] = __builtin_elementwise_max(w[r], y);
+            w[r] = __builtin_elementwise_min(w[r], y);
+        }
+    }
+    __syncthreads();
+    if (wave == b) {
+#pragma unroll
+        for (uint32_t r = 0; r < 32u; ++r) w[r] = sW[chunkSlot(lane, r)];
+    }
+    __syncthreads();
+}
+
+// The stages of a merge level below the chunk size (j = 1024 .. 1), within each wave's chunk.
+__device__ __forceinline__ void mergeChunk(uint64_t (&w)[32], uint32_t lane) {
+#pragma unroll 1
+    for (uint32_t j = kWaveChunk / 2u; j >= 32u; j >>= 1) laneHalf<32>(w, lane, j / 32u);
+    registerHalves<32, 16>(w);
+}
+
+// A range of 2048 < n <= 8192 entries sorted by W = 2 or 4 waves of the workgroup: positions
+// v * 2048 + lane * 32 + r (wave v), each wave's chunk sorted in registers, then the bitonic
+// merge levels 4096 (and 8192): the stages whose partners are in another wave through sW, the
+// rest in registers.
+__device__ __attribute__((noinline)) void groupSortRegisters(uint64_t* keys, uint32_t* values, uint32_t tile, uint2 range, uint32_t wave,
+                                   uint32_t lane, uint64_t* sW) {
+    const uint32_t n = range.y - range.x;
+    const uint32_t W = n <= 2u * kWaveChunk ? 2u : 4u;
+    const bool active = wave < W;
+    uint64_t w[32];
+    if (active) {
+#pragma unroll
+        for (uint32_t r = 0; r < 32u; ++r) {
+            const uint32_t e = wave * kWaveChunk + r * 64u + lane;
+            w[r] = e < n ? orderWord(keys[range.x + e], values[range.x + e]) : kPad;
+        }
+        waveSort<32>(w, lane);
+    }
+    // Level 4096: chunks (0, 1) and (2, 3), flip.
+    crossExchange(w, wave, lane, 0u, 1u, true, sW);
+    if (W == 4u) crossExchange(w, wave, lane, 2u, 3u, true, sW);
+    if (active) mergeChunk(w, lane);
+    if (W == 4u) {
+        // Level 8192: the flip pairs chunk v with chunk 3 - v (position q with 2047 - q), then the
+        // stage 2048 apart pairs chunk v with chunk v ^ 1 (same position).
+        crossExchange(w, wave, lane, 0u, 3u, true, sW);
+        crossExchange(w, wave, lane, 1u, 2u, true, sW);
+        crossExchange(w, wave, lane, 0u, 1u, false, sW);
+        crossExchange(w, wave, lane, 2u, 3u, false, sW);
+        if (active) mergeChunk(w, lane);
+    }
+    if (active) {
+        const uint64_t tileHi = uint64_t(tile) << 32;
+#pragma unroll
+        for (uint32_t r = 0; r < 32u; ++r) {
+            const uint32_t e = wave * kWaveChunk + lane * 32u + r;
+            if (e < n) {
+                keys[range.x + e] = tileHi | (w[r] >> 32);
+                values[range.x + e] = uint32_t(w[r]);
+            }
+        }
+    }
+}
+
 
 constexpr uint32_t kLdsChunk = 2048;  // entries sorted in LDS at once (16 KB)
 
@@ -367,10 +453,15 @@ __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __rest
         if (n > 1u && n <= kWaveChunk) waveSortAny(keys, values, uint64_t(tile) << 32, range.x, n, lane);
     }
     if (longMask == 0u) return;  // uniform per workgroup
-    // 2. The longer ones, one at a time by the whole workgroup.
+    // 2. The longer ones, one at a time by the whole workgroup: up to 8192 entries in registers,
+    // longer ones by LDS chunks and merge rounds.
     __syncthreads();
-    for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v)
-        if (longMask & (1u << v)) groupSortLong(keys, values, keysTmp, valuesTmp, tiles[v], ranges[tiles[v]], sW);
+    for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v) {
+        if (!(longMask & (1u << v))) continue;
+        const uint2 rg = ranges[tiles[v]];
+        if (rg.y - rg.x <= kGroupChunk) groupSortRegisters(keys, values, tiles[v], rg, wave, lane, sW);
+        else groupSortLong(keys, values, keysTmp, valuesTmp, tiles[v], rg, sW);
+    }
 }
 
 }  // namespace

@@ -771,3 +771,33 @@ def test_kernel_times(gsr_ctx):
         assert abs(sum(ms for _, ms in kt) - total) <= 0.05 * total + 0.01, (sum(ms for _, ms in kt), total)
         r.close()
     scene.close()
+
+
+def test_tile_depth_sort_long_ranges(gsr_ctx):
+    """The per-tile depth sort of global mode on every size class of tile range: a 600k-gaussian
+    cloud (means ~ N(0, 1.5^2 I)) at 640x360 from radius 7 has ranges up to 2048 entries (one wave,
+    registers), 2049..4096 and 4097..8192 (2 or 4 waves, registers with cross-wave stages through
+    LDS) and over 8192 (LDS chunks and merge rounds): lists and ranges bit-exact against the oracle
+    and equal to the full 64-bit key sort."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log("case tile depth sort: long ranges")
+    sc = scenes.make_scene(600_000, seed=4, mean_sigma=1.5)
+    view = camera.orbit_view(7.0, 10.0, 30.0)
+    r, scene, cam, img, osc, ocam, ob, os_, oc = _render_case(gsr_ctx, sc, view, 640, 360, "tight", sort_mode="global")
+    ctl = _check_lists(r, os_, oc, cam.tiles_x, cam.tiles_y, ob)
+    _, _, _, ranges, _ = r.read_back()
+    L = ranges.reshape(-1, 2)[:, 1].astype(np.int64) - ranges.reshape(-1, 2)[:, 0]
+    bands = [int(((L > a) & (L <= b)).sum()) for a, b in ((0, 2048), (2048, 4096), (4096, 8192), (8192, 1 << 30))]
+    log(f"  {ctl.entries} entries; tiles per range band <=2048 / <=4096 / <=8192 / more: {bands}")
+    assert all(b > 0 for b in bands)
+    full = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", sort_mode="global", full_key_sort=True)
+    fimg = full.render_to_host(cam)
+    assert np.array_equal(fimg, img)
+    for a, b in zip(full.read_back()[:4], r.read_back()[:4]):
+        assert np.array_equal(a, b)
+    full.close()
+    r.close()
+    scene.close()
