@@ -430,7 +430,7 @@ def main():
     packed = [m for m in ("p32", "p24") if m != args.render_splat]
     runs = [("f32", args.sort_mode, args.render_splat, 0), ("f16", args.sort_mode, args.render_splat, 0),
             ("f32", other, args.render_splat, 0)] + [("f32", args.sort_mode, m, 0) for m in packed] + \
-        [("f32", args.sort_mode, args.render_splat, k) for k in (4, 16)]
+        [("f32", args.sort_mode, args.render_splat, k) for k in (4, 16, 64)]
     result, modes, splat_modes, windows = {}, {}, {}, {}
     for fmt, mode, rs, window in runs:
         scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree, fmt=fmt)

@@ -702,9 +702,9 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (s.render_splat != GSR_RENDER_SPLAT_FULL && s.render_splat != GSR_RENDER_SPLAT_P32 &&
         s.render_splat != GSR_RENDER_SPLAT_P24)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown render_splat %d", int(s.render_splat));
-    if (s.sort_window > uint32_t(kMaxSortWindow))
-        return fail(GSR_ERROR_UNSUPPORTED, "gsr_renderer_create: sort_window %u (the library compiles windows 1..%d)",
-                    s.sort_window, kMaxSortWindow);
+    if (s.sort_window > uint32_t(kMaxSortWindowHeap))
+        return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: sort_window %u must be <= %d", s.sort_window,
+                    kMaxSortWindowHeap);
     if (s.sort_mode != GSR_SORT_MODE_BINNING && s.sort_mode != GSR_SORT_MODE_GLOBAL)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: unknown sort_mode %d", int(s.sort_mode));
     if (s.sort_mode == GSR_SORT_MODE_BINNING && s.sort_algorithm == GSR_SORT_ONESWEEP)

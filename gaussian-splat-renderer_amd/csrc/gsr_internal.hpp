@@ -16,7 +16,8 @@ constexpr float kFrustumClampFactor = 1.3f;
 constexpr float kMaxAlpha = 0.99f;
 constexpr float kMinAlpha = 1.0f / 255.0f;
 constexpr float kMinTransmittance = 1e-4f;
-constexpr int kMaxSortWindow = 16;  // StopThePop: largest per-pixel window the library compiles
+constexpr int kMaxSortWindow = 16;      // StopThePop: largest window held in registers
+constexpr int kMaxSortWindowHeap = 64;  // StopThePop: largest window (LDS heaps beyond kMaxSortWindow)
 constexpr int kTileSize = 16;
 constexpr uint32_t kNoGaussian = 0xFFFFFFFFu;
 

@@ -672,6 +672,170 @@ __global__ __launch_bounds__(256) void kRenderStp(const float4* __restrict__ spl
     }
 }
 
+// StopThePop windows beyond the register form (K = 17..64; splat_tile_renderer.cpp:142 allows 64):
+// the same blend order, with the window a binary min-heap per pixel in LDS. The register form
+// keeps the window sorted by (ray depth, arrival) (an arrival is placed after the equal depths
+// already there, tile_render.comp:221-279); a min-heap keyed by (ray depth, list position) pops in
+// exactly that order, so the splats are blended in the same order: when the window is full the
+// new splat is blended at once if it is strictly nearer than the heap's top, else the top is
+// blended and replaced by the new splat; at the end the heap is emptied nearest first. A push or a
+// pop costs O(log K) LDS moves instead of the O(K) of a sorted insertion, and the heap needs no
+// registers per slot: one wave (64 threads) renders one 8x8 quadrant of a tile, and each of its
+// pixels keeps K slots of 12 B (ordered depth, list position, gaussian) in LDS, slot-major so that
+// the lanes' accesses to one slot index are conflict-free. Alpha and colour of a slot are fetched
+// again from its gaussian when it is blended (the same expression on the same inputs).
+__device__ __forceinline__ uint32_t depthOrder(float d) {
+    const uint32_t u = __float_as_uint(d + 0.0f);  // -0 -> +0 (they compare equal)
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+template <int FORMAT, int RS>
+__global__ __launch_bounds__(64) void kRenderStpHeap(const float4* __restrict__ splats,
+                                                     const uint32_t* __restrict__ packed,
+                                                     const float4* __restrict__ rayDepth,
+                                                     const uint32_t* __restrict__ values,
+                                                     const uint2* __restrict__ ranges, uint32_t width,
+                                                     uint32_t height, uint32_t tilesX, uint32_t tileCount,
+                                                     uint32_t gaussianCount, float focalX, float focalY, uint32_t K,
+                                                     void* __restrict__ out) {
+    extern __shared__ uint32_t sHeap[];  // [3][K][64]: ordered depth, list position, gaussian
+    __shared__ float4 sS0[64];           // cx, cy, a, b
+    __shared__ float2 sS1[64];           // c, o
+    __shared__ float4 sColor[64];
+    __shared__ float4 sRayA[64], sRayB[64];
+    __shared__ float sRayC[64];
+    __shared__ uint32_t sG[64];
+    uint32_t* hKey = sHeap;
+    uint32_t* hSeq = sHeap + K * 64u;
+    uint32_t* hG = sHeap + 2u * K * 64u;
+
+    const uint32_t tile = blockIdx.x >> 2, quadrant = blockIdx.x & 3u;
+    if (tile >= tileCount) return;  // uniform
+    const uint32_t lane = threadIdx.x;
+    const uint32_t px = (tile % tilesX) * 16 + (quadrant & 1u) * 8 + (lane & 7u);
+    const uint32_t py = (tile / tilesX) * 16 + (quadrant >> 1) * 8 + (lane >> 3);
+    const float pxf = float(px), pyf = float(py);
+    const float rx = (pxf - 0.5f * (float(width) - 1.0f)) / focalX;
+    const float ry = (pyf - 0.5f * (float(height) - 1.0f)) / focalY;
+    bool done = !(px < width && py < height);
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
+    uint32_t size = 0;
+
+    // alpha and colour of gaussian g at this pixel (the push-time expression).
+    auto alphaOf = [&](float cx, float cy, float4 co) {
+        const float dx = cx - pxf, dy = cy - pyf;
+        const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+        if (power > 0.0f) return 0.0f;
+        const float ex = co.w * expf(power);
+        const float a = kMaxAlpha < ex ? kMaxAlpha : ex;
+        return a < kMinAlpha ? 0.0f : a;
+    };
+    auto blendG = [&](uint32_t g) {
+        float cx, cy;
+        float4 co, col;
+        fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
+        stpBlend<1>(alphaOf(cx, cy, co), col.x, col.y, col.z, T, C0, C1, C2, done);
+    };
+    auto less = [](uint32_t ka, uint32_t sa, uint32_t kb, uint32_t sb) { return ka < kb || (ka == kb && sa < sb); };
+    auto put = [&](uint32_t i, uint32_t k, uint32_t q, uint32_t g) {
+        hKey[i * 64u + lane] = k;
+        hSeq[i * 64u + lane] = q;
+        hG[i * 64u + lane] = g;
+    };
+    // Sift (k, q, g) down from the root of a heap of `size` slots.
+    auto siftDown = [&](uint32_t k, uint32_t q, uint32_t g) {
+        uint32_t i = 0;
+        for (;;) {
+            uint32_t c = 2u * i + 1u;
+            if (c >= size) break;
+            uint32_t ck = hKey[c * 64u + lane], cq = hSeq[c * 64u + lane];
+            if (c + 1u < size) {
+                const uint32_t rk = hKey[(c + 1u) * 64u + lane], rq = hSeq[(c + 1u) * 64u + lane];
+                if (less(rk, rq, ck, cq)) {
+                    ++c;
+                    ck = rk;
+                    cq = rq;
+                }
+            }
+            if (!less(ck, cq, k, q)) break;
+            put(i, ck, cq, hG[c * 64u + lane]);
+            i = c;
+        }
+        put(i, k, q, g);
+    };
+
+    const uint2 range = ranges[tile];
+    for (uint32_t batch = range.x; batch < range.y; batch += 64u) {
+        if (__builtin_amdgcn_ballot_w64(!done) == 0) break;  // uniform: the quadrant is done
+        const uint32_t e = batch + lane;
+        __syncthreads();  // the previous batch is no longer read
+        if (e < range.y) {
+            uint32_t g = values[e];
+            g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
+            float cx, cy;
+            float4 co, col;
+            fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
+            sS0[lane] = make_float4(cx, cy, co.x, co.y);
+            sS1[lane] = make_float2(co.z, co.w);
+            sColor[lane] = col;
+            const float4* rd = rayDepth + 3 * size_t(g);
+            sRayA[lane] = rd[0];
+            sRayB[lane] = rd[1];
+            sRayC[lane] = rd[2].x;
+            sG[lane] = g;
+        }
+        __syncthreads();
+        const uint32_t n = range.y - batch < 64u ? range.y - batch : 64u;
+        for (uint32_t j = 0; !done && j < n; ++j) {
+            const float4 s0 = sS0[j];
+            const float2 s1 = sS1[j];
+            const float a = alphaOf(s0.x, s0.y, make_float4(s0.z, s0.w, s1.x, s1.y));
+            if (a == 0.0f) continue;
+            const float4 pa = sRayA[j], pb = sRayB[j];
+            const float vPv = pa.x * rx * rx + 2.0f * pa.y * rx * ry + 2.0f * pa.z * rx + pa.w * ry * ry +
+                              2.0f * pb.x * ry + pb.y;
+            const float vb = pb.z * rx + pb.w * ry + sRayC[j];
+            float depth = vb / vPv;
+            if (!(depth < 3.0e38f)) depth = 3.0e38f;  // finite (NaN/Inf from a degenerate splat)
+            const uint32_t key = depthOrder(depth), seq = batch + j, g = sG[j];
+            if (size < K) {  // not full: push (sift up)
+                uint32_t i = size++;
+                while (i > 0u) {
+                    const uint32_t p = (i - 1u) >> 1;
+                    const uint32_t pk = hKey[p * 64u + lane], pq = hSeq[p * 64u + lane];
+                    if (!less(key, seq, pk, pq)) break;
+                    put(i, pk, pq, hG[p * 64u + lane]);
+                    i = p;
+                }
+                put(i, key, seq, g);
+                continue;
+            }
+            if (less(key, seq, hKey[lane], hSeq[lane])) {  // nearer than the whole window
+                const float4 col = sColor[j];
+                stpBlend<1>(a, col.x, col.y, col.z, T, C0, C1, C2, done);
+                continue;
+            }
+            blendG(hG[lane]);  // the window's nearest leaves, the new splat takes its place
+            siftDown(key, seq, g);
+        }
+    }
+    while (size > 0u && !done) {  // the rest, nearest first
+        blendG(hG[lane]);
+        --size;
+        siftDown(hKey[size * 64u + lane], hSeq[size * 64u + lane], hG[size * 64u + lane]);
+    }
+    if (px < width && py < height) {
+        const size_t p = size_t(py) * width + px;
+        if (FORMAT == GSR_TARGET_RGBA8) {
+            const uint32_t rgba = quantize8(C0) | (quantize8(C1) << 8) | (quantize8(C2) << 16) |
+                                  (quantize8(1.0f - T) << 24);
+            static_cast<uint32_t*>(out)[p] = rgba;
+        } else {
+            static_cast<float4*>(out)[p] = make_float4(C0, C1, C2, 1.0f - T);
+        }
+    }
+}
+
 template <int FORMAT, int RS, int K>
 void launchStpK(const float4* splats, const uint32_t* packed, const float4* rayDepth, const uint32_t* values,
                 const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out, hipStream_t stream) {
@@ -687,6 +851,15 @@ template <int FORMAT, int RS, int K = 1>
 void launchStp(int window, const float4* splats, const uint32_t* packed, const float4* rayDepth,
                const uint32_t* values, const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out,
                hipStream_t stream) {
+    if (K == 1 && window > kMaxSortWindow) {  // the LDS heap form
+        const uint32_t tileCount = cam.tilesX * cam.tilesY;
+        const size_t lds = size_t(window) * 64u * 12u;
+        hipLaunchKernelGGL((kRenderStpHeap<FORMAT, RS>), dim3(tileCount * 4u), dim3(64), lds, stream, splats, packed,
+                           rayDepth, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount,
+                           cam.focalX, cam.focalY, uint32_t(window), out);
+        markKernel("kRenderStpHeap", stream);
+        return;
+    }
     if constexpr (K <= kMaxSortWindow) {
         if (window == K)
             launchStpK<FORMAT, RS, K>(splats, packed, rayDepth, values, ranges, cam, gaussianCount, out, stream);

@@ -139,10 +139,11 @@ typedef struct gsr_settings {
                                     orders all (tile, depth) key bits. Same list, bit for bit. */
     int32_t render_splat;        /* GSR_RENDER_SPLAT_*: the record the render pass gathers per list entry */
     uint32_t sort_window;        /* StopThePop per-pixel re-sorting window K (TileSettings::sortWindow,
-                                    splat_tile_renderer.h:108): 0 = INRIA order; 1..16 = blend in the order
+                                    splat_tile_renderer.h:108): 0 = INRIA order; 1..64 = blend in the order
                                     of a K-slot window sorted by the depth of each splat's maximum along the
-                                    pixel's ray (tile_render.comp:201-279), reference alpha. K > 16:
-                                    GSR_ERROR_UNSUPPORTED (the reference allows 64) */
+                                    pixel's ray (tile_render.comp:201-279), reference alpha; K <= 16 in
+                                    registers, 17..64 in per-pixel LDS heaps (same order); K > 64:
+                                    GSR_ERROR_INVALID_ARGUMENT (splat_tile_renderer.cpp:142) */
     int32_t adaptive_list;       /* TileSettings::adaptiveList (splat_tile_renderer.h:104): size the list from the
                                     measured entry count (adaptCapacity, splat_tile_renderer.cpp:444-486), within
                                     the budget; resizes never stall the frames in flight */

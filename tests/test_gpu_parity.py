@@ -312,10 +312,12 @@ def test_packed_render_splats(gsr_ctx, mode):
     scene.close()
 
 
-@pytest.mark.parametrize("window", [1, 4, 16])
+@pytest.mark.parametrize("window", [1, 4, 16, 17, 40, 64])
 def test_stopthepop_window(gsr_ctx, window):
     """StopThePop window K (test_tight_bounds.cpp:472-481): the GPU image against the oracle with
-    the same window > 60 dB (the lists are the INRIA-order lists, bit-exact); K > 16 is refused."""
+    the same window > 60 dB (the lists are the INRIA-order lists, bit-exact), for windows held in
+    registers (K <= 16) and in per-pixel LDS heaps (17..64, splat_tile_renderer.cpp:142);
+    K > 64 is refused."""
     import gsr
     import oracle
     from gsr import camera, scenes
@@ -337,7 +339,7 @@ def test_stopthepop_window(gsr_ctx, window):
     assert p > 60.0 and p > p_inria
     r.close()
     with pytest.raises(gsr.GsrError, match="sort_window"):
-        gsr.Renderer(gsr_ctx, scene, sort_window=17)
+        gsr.Renderer(gsr_ctx, scene, sort_window=65)
     scene.close()
 
 
