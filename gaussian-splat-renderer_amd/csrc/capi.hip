@@ -254,6 +254,10 @@ struct gsr_renderer {
     bool eventsRecorded = false;
     std::unique_ptr<KernelMarks> marks;  // profile_stages = 2: an event after every kernel
     bool marksRecorded = false;
+    // The lists' bins: TileSettings::tileSize (16 or 8 px).
+    uint32_t binSize = 16;
+    uint32_t binsX(const gsr_camera& c) const { return (c.width + binSize - 1) / binSize; }
+    uint32_t binsY(const gsr_camera& c) const { return (c.height + binSize - 1) / binSize; }
     // tile_stats: kTileStatCount counters per tile of the last frame (sized for the largest grid).
     DeviceBuffer tileStats;
     uint32_t tileStatsTiles = 0;
@@ -723,6 +727,17 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
                     "and an f32 scene");
     if (s.profile_stages < 0 || s.profile_stages > 2)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_renderer_create: profile_stages must be 0, 1 or 2");
+    if (s.tile_size == 0) s.tile_size = 16;
+    if (s.tile_size != 8 && s.tile_size != 16)
+        return fail(GSR_ERROR_INVALID_ARGUMENT,
+                    "gsr_renderer_create: tile_size must be 8 or 16 (a divisor of INRIA's 16-px tiles), got %d",
+                    s.tile_size);
+    if (s.tile_size == 8 &&
+        (s.render_loop != GSR_RENDER_LOOP_CULLED || s.render_pixels_per_thread != 1 || s.render_exact_alpha ||
+         s.sort_window != 0 || s.differentiable || s.tile_stats))
+        return fail(GSR_ERROR_UNSUPPORTED,
+                    "gsr_renderer_create: tile_size 8 renders with the culled loop only (render_loop CULLED, one "
+                    "pixel per thread, fast alpha, sort_window 0, not differentiable, no tile_stats)");
     if (s.tile_stats &&
         (s.render_loop != GSR_RENDER_LOOP_CULLED || s.render_pixels_per_thread != 1 || s.render_exact_alpha ||
          s.sort_window != 0 || s.differentiable))
@@ -735,6 +750,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     r->ctx = ctx;
     r->scene = scene;
     r->settings = s;
+    r->binSize = uint32_t(s.tile_size);
     r->degree = std::min(scene->shDegree, std::max(0, std::min(3, s.max_sh_degree)));
     r->binning = s.sort_mode == GSR_SORT_MODE_BINNING;
     r->maxCapacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0, r->binning);
@@ -859,7 +875,9 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     // (a captured batch is ordered by its own fork event instead).
     GSR_HIP(orderAfterLastFrame(r, stream, capturing));
     const gsr_scene* sc = r->scene;
-    const uint32_t tiles = cam->tiles_x * cam->tiles_y;
+    // The lists' grid: bins of binSize px (the camera's tiles_x/y are INRIA's 16-px tiles).
+    const uint32_t binsX = r->binsX(*cam), binsY = r->binsY(*cam);
+    const uint32_t tiles = binsX * binsY;
     // Per-size buffers (onRenderTargetsChanged, splat_tile_renderer.cpp:537-570).
     if (tiles > r->rangesTiles) {
         GSR_HIP(hipStreamSynchronize(stream));
@@ -884,11 +902,11 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     }
     const uint32_t* tileOrder = nullptr;
     if (!r->binning && !r->settings.full_key_sort) {
-        const gsr_renderer::TileOrder* o = r->findOrder(cam->tiles_x, cam->tiles_y);
+        const gsr_renderer::TileOrder* o = r->findOrder(binsX, binsY);
         if (!o) {  // a new grid: its table, kept for later frames of the size
             std::vector<uint32_t> order(tiles);
             for (uint32_t t = 0; t < tiles; ++t) order[t] = t;
-            const int64_t tx = cam->tiles_x, ty = cam->tiles_y;
+            const int64_t tx = binsX, ty = binsY;
             auto centreKey = [&](uint32_t t) {
                 const int64_t row = t / tx, col = t % tx;
                 return std::llabs(2 * row - (ty - 1)) * (2 * tx + 1) + std::llabs(2 * col - (tx - 1));
@@ -898,8 +916,8 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
             auto entry = std::make_unique<gsr_renderer::TileOrder>();
             GSR_HIP(entry->table.allocate(size_t(tiles) * 4));
             GSR_HIP(hipMemcpy(entry->table.ptr, order.data(), size_t(tiles) * 4, hipMemcpyHostToDevice));
-            entry->tilesX = cam->tiles_x;
-            entry->tilesY = cam->tiles_y;
+            entry->tilesX = binsX;
+            entry->tilesY = binsY;
             r->tileOrders.push_back(std::move(entry));
             o = r->tileOrders.back().get();
         }
@@ -923,6 +941,10 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     dc.height = cam->height;
     dc.tilesX = cam->tiles_x;
     dc.tilesY = cam->tiles_y;
+    dc.binSize = r->binSize;
+    dc.binsX = binsX;
+    dc.binsY = binsY;
+    const BinGrid grid{cam->tiles_x, cam->tiles_y, r->binSize, binsX, binsY};
     SceneRef ref = sc->ref;
     ref.degree = r->degree;
 
@@ -991,7 +1013,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         launchScanVisibleBlocks(r->visibleSums.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         launchDuplicateBinning(r->splats.as<float4>(), r->sortedCounts.as<uint32_t>(), order,
-                               r->visibleSums.as<uint32_t>(), control, sc->count, bounds, cam->tiles_x, cam->tiles_y,
+                               r->visibleSums.as<uint32_t>(), control, sc->count, bounds, grid,
                                r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
@@ -1006,7 +1028,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
         GSR_HIP(mark(2));
         GSR_HIP(mark(3));
         launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(), control,
-                        sc->count, bounds, cam->tiles_x, cam->tiles_y, r->keysA.as<uint64_t>(),
+                        sc->count, bounds, grid, r->keysA.as<uint64_t>(),
                         r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
@@ -1123,7 +1145,7 @@ gsr_status gsr_read_lists(gsr_renderer* r, uint32_t* tileKeys, uint32_t* depthKe
         }
     }
     if (ranges)
-        GSR_HIP(hipMemcpy(ranges, r->ranges.ptr, size_t(r->lastCam.tiles_x) * r->lastCam.tiles_y * 8,
+        GSR_HIP(hipMemcpy(ranges, r->ranges.ptr, size_t(r->binsX(r->lastCam)) * r->binsY(r->lastCam) * 8,
                           hipMemcpyDeviceToHost));
     return GSR_OK;
 }
@@ -1185,7 +1207,7 @@ gsr_status gsr_read_tile_stats(gsr_renderer* r, uint32_t* stats, uint32_t maxTil
     if (!r->haveFrame) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_read_tile_stats: no frame rendered yet");
     GSR_HIP(hipSetDevice(r->ctx->device));
     GSR_HIP(hipDeviceSynchronize());
-    const uint32_t tiles = std::min(r->lastCam.tiles_x * r->lastCam.tiles_y, maxTiles);
+    const uint32_t tiles = std::min(r->binsX(r->lastCam) * r->binsY(r->lastCam), maxTiles);
     if (tiles) GSR_HIP(hipMemcpy(stats, r->tileStats.ptr, size_t(tiles) * kTileStatCount * 4, hipMemcpyDeviceToHost));
     return GSR_OK;
 }
@@ -1428,9 +1450,9 @@ gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint3
     // allocates and synchronises, which a capture cannot record).
     for (uint32_t v = 0; v < viewCount; ++v) {
         const gsr_renderer* r = renderers[v % used];
-        const uint32_t tiles = cams[v].tiles_x * cams[v].tiles_y;
+        const uint32_t tiles = r->binsX(cams[v]) * r->binsY(cams[v]);
         const bool orderReady =
-            r->binning || r->settings.full_key_sort || r->findOrder(cams[v].tiles_x, cams[v].tiles_y) != nullptr;
+            r->binning || r->settings.full_key_sort || r->findOrder(r->binsX(cams[v]), r->binsY(cams[v])) != nullptr;
         if (tiles > r->rangesTiles || !orderReady || !r->haveFrame)
             return fail(GSR_ERROR_INVALID_ARGUMENT,
                         "gsr_batch_capture: view %u (%ux%u) is a size its renderer has not rendered yet: render the "
@@ -1775,6 +1797,9 @@ gsr_status gsr_backward(gsr_renderer* r, const void* gradImage, const gsr_gradie
     dc.height = cam.height;
     dc.tilesX = cam.tiles_x;
     dc.tilesY = cam.tiles_y;
+    dc.binSize = r->binSize;
+    dc.binsX = r->binsX(cam);
+    dc.binsY = r->binsY(cam);
     SceneRef ref = sc->ref;
     ref.degree = r->degree;
     // The frame's stream may differ from this one: order after it.

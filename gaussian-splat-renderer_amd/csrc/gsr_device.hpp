@@ -125,25 +125,35 @@ __device__ __forceinline__ Ellipse makeEllipse(float cx, float cy, float A, floa
     return e;
 }
 
+// floor(v / S) for the bin sizes S = 8 and 16, v clamped: the division by a power of two is the
+// exact scaling ldexp(v, -log2 S).
 __device__ __forceinline__ int floorDivClamped(float v, int S) {
-    return int(floorf(clampSel(v, -65536.0f, 1048576.0f) / float(S)));
+    return int(floorf(ldexpf(clampSel(v, -65536.0f, 1048576.0f), S == 8 ? -3 : -4)));
 }
 
-// Coverage of one splat in 16-px tiles: either the rectangle (useEllipse = false) or the
-// rectangle intersected with the cutoff ellipse, one contiguous run per tile row.
+// Coverage of one splat in bins of S = 16 or 8 pixels (tile_coverage.cpp:24-164): either the
+// getRect rectangle (on 16-px tiles) in bins (useEllipse = false) or that rectangle intersected
+// with the cutoff ellipse, one contiguous run per bin row.
 struct Coverage {
     Rect bins;
     Ellipse e;
     bool useEllipse;
 };
 
-__device__ __forceinline__ Coverage makeCoverage(uint32_t mode, const Rect& rect, int gridX, int gridY, float cx,
-                                                 float cy, float A, float B, float C, float opacity) {
+// rectToBins: the 16-px rectangle in bins of S pixels, clipped to the bin grid.
+__device__ __forceinline__ Rect rectToBins(const Rect& r, int S, int gridX, int gridY) {
+    const int f = 16 / S;
+    const int x1 = r.x1 * f, y1 = r.y1 * f;
+    return Rect{r.x0 * f, r.y0 * f, x1 < gridX ? x1 : gridX, y1 < gridY ? y1 : gridY};
+}
+
+__device__ __forceinline__ Coverage makeCoverage(uint32_t mode, const Rect& rect, int S, int gridX, int gridY,
+                                                 float cx, float cy, float A, float B, float C, float opacity) {
     Coverage cv;
     if (mode == GSR_BOUNDS_CUTOFF_ONLY) {
         cv.bins = Rect{0, 0, gridX, gridY};
     } else {
-        cv.bins = Rect{rect.x0, rect.y0, rect.x1 < gridX ? rect.x1 : gridX, rect.y1 < gridY ? rect.y1 : gridY};
+        cv.bins = rectToBins(rect, S, gridX, gridY);
     }
     cv.useEllipse = false;
     cv.e.valid = false;
@@ -151,13 +161,12 @@ __device__ __forceinline__ Coverage makeCoverage(uint32_t mode, const Rect& rect
     if (mode != GSR_BOUNDS_RECT) {
         cv.e = makeEllipse(cx, cy, A, B, C, opacity);
         cv.useEllipse = cv.e.valid;
-        if (!cv.e.valid && mode == GSR_BOUNDS_CUTOFF_ONLY)
-            cv.bins = Rect{rect.x0, rect.y0, rect.x1 < gridX ? rect.x1 : gridX, rect.y1 < gridY ? rect.y1 : gridY};
+        if (!cv.e.valid && mode == GSR_BOUNDS_CUTOFF_ONLY) cv.bins = rectToBins(rect, S, gridX, gridY);
     }
     return cv;
 }
 
-__device__ __forceinline__ void coverageRows(const Coverage& cv, int* r0, int* r1) {
+__device__ __forceinline__ void coverageRows(const Coverage& cv, int S, int* r0, int* r1) {
     if (!cv.useEllipse) {
         *r0 = cv.bins.y0;
         *r1 = cv.bins.y1 > cv.bins.y0 ? cv.bins.y1 : cv.bins.y0;
@@ -170,8 +179,8 @@ __device__ __forceinline__ void coverageRows(const Coverage& cv, int* r0, int* r
     }
     const float y0 = ceilf(cv.e.cy - cv.e.eyPad);
     const float y1 = floorf(cv.e.cy + cv.e.eyPad);
-    const int a = floorDivClamped(y0, kTileSize);
-    const int b = floorDivClamped(y1, kTileSize) + 1;
+    const int a = floorDivClamped(y0, S);
+    const int b = floorDivClamped(y1, S) + 1;
     const int lo = cv.bins.y0 > a ? cv.bins.y0 : a;
     const int hi = cv.bins.y1 < b ? cv.bins.y1 : b;
     *r0 = lo;
@@ -179,14 +188,13 @@ __device__ __forceinline__ void coverageRows(const Coverage& cv, int* r0, int* r
 }
 
 // ellipseRowSpan (tile_coverage.cpp:135-161): the run [x0, x1) of tile row `row`.
-__device__ __forceinline__ void coverageRowSpan(const Coverage& cv, int row, int* x0, int* x1) {
+__device__ __forceinline__ void coverageRowSpan(const Coverage& cv, int row, int S, int* x0, int* x1) {
     if (!cv.useEllipse) {
         *x0 = cv.bins.x0;
         *x1 = cv.bins.x1 > cv.bins.x0 ? cv.bins.x1 : cv.bins.x0;
         return;
     }
     const Ellipse& e = cv.e;
-    const int S = kTileSize;
     const float ey0 = float(row * S) - e.cy;
     const float ey1 = float(row * S + S - 1) - e.cy;
     const float nep = -e.eyPad;
@@ -216,9 +224,9 @@ __device__ __forceinline__ void coverageRowSpan(const Coverage& cv, int row, int
     *x1 = hi2 > lo2 ? hi2 : lo2;
 }
 
-__device__ __forceinline__ uint32_t coverageCount(const Coverage& cv) {
+__device__ __forceinline__ uint32_t coverageCount(const Coverage& cv, int S) {
     int r0, r1;
-    coverageRows(cv, &r0, &r1);
+    coverageRows(cv, S, &r0, &r1);
     if (!cv.useEllipse) {
         const int w = cv.bins.x1 - cv.bins.x0;
         return uint32_t((w > 0 ? w : 0) * (r1 - r0));
@@ -226,7 +234,7 @@ __device__ __forceinline__ uint32_t coverageCount(const Coverage& cv) {
     uint32_t n = 0;
     for (int r = r0; r < r1; ++r) {
         int x0, x1;
-        coverageRowSpan(cv, r, &x0, &x1);
+        coverageRowSpan(cv, r, S, &x0, &x1);
         n += uint32_t(x1 - x0);
     }
     return n;

@@ -33,11 +33,14 @@ struct BlockStats {
 static_assert(sizeof(BlockStats) == 32, "BlockStats is 32 bytes");
 
 // Kernel-side camera (SplatCamera, splat_math.h:43-50). view[c*4+r], column-major.
+// tilesX/Y: INRIA's 16-px tiles (getRect); binSize, binsX/Y: the grid the lists are built on
+// (TileSettings::tileSize, 16 or 8 px; splat_tile_renderer.cpp:537-541).
 struct DevCamera {
     float view[16];
     float pos[3];
     float tanX, tanY, focalX, focalY;
     uint32_t width, height, tilesX, tilesY;
+    uint32_t binSize, binsX, binsY;
 };
 
 // Device view of a scene. Records wider than 16 B are stored as planes of 16-B quads:
@@ -117,8 +120,12 @@ void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t boun
 // true: of their visible (non-zero count) gaussians (binning mode's compaction).
 void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
                       bool scanVisible, uint32_t* blockOffsets, gsr_control* control, hipStream_t stream);
+// The lists' grid: getRect's 16-px tiles and the bins of binSize (16 or 8) px the entries name.
+struct BinGrid {
+    uint32_t tilesX, tilesY, binSize, binsX, binsY;
+};
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
-                     const gsr_control* control, uint32_t count, uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
+                     const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
                      uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream);
 // Sorts by key bits [firstBit, keyBits) (stable). Returns which buffer (0 = keysA/valuesA,
 // 1 = B) holds the sorted result. *epoch is the caller's status-epoch counter (one per status
@@ -153,7 +160,7 @@ void launchGatherCounts(const uint32_t* tileCounts, const uint32_t* order, const
 void launchScanVisibleBlocks(uint32_t* blockSums, const gsr_control* control, hipStream_t stream);
 void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, const uint32_t* order,
                             const uint32_t* blockOffsets, const gsr_control* control, uint32_t maxVisible,
-                            uint32_t bounds, uint32_t tilesX, uint32_t tilesY, uint32_t* keys, uint32_t* values,
+                            uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values,
                             gsr_control* controlRW, hipStream_t stream);
 // sortWindow > 0: StopThePop with that window (rayDepth: 3 float4 per gaussian), reference alpha.
 // tileStats (culled loop only; null = none): 8 counters per tile, zeroed by the caller (the

@@ -184,12 +184,14 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                     }
                     o1 = make_float4(ra, rb, rc, ro);
                     reason = GSR_CULL_VISIBLE;
-                    if (bounds == GSR_BOUNDS_RECT) {
+                    // The bins the duplicate pass will write (splat_preprocess.comp:291-308): INRIA's
+                    // rectangle count on 16-px bins with rect bounds, else the coverage count.
+                    if (bounds == GSR_BOUNDS_RECT && cam.binSize == uint32_t(kTileSize)) {
                         touched = uint32_t((rect.x1 - rect.x0) * (rect.y1 - rect.y0));
                     } else {
-                        const Coverage cv = makeCoverage(bounds, rect, int(cam.tilesX), int(cam.tilesY), ctrX, ctrY,
-                                                         ra, rb, rc, ro);
-                        touched = coverageCount(cv);
+                        const Coverage cv = makeCoverage(bounds, rect, int(cam.binSize), int(cam.binsX),
+                                                         int(cam.binsY), ctrX, ctrY, ra, rb, rc, ro);
+                        touched = coverageCount(cv, int(cam.binSize));
                     }
                     if (rayDepth) {
                         // StopThePop (computeRayDepthParams, splat_math.cpp:195-205): P = Sigma^-1 in

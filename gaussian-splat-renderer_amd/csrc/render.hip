@@ -493,6 +493,104 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
     }
 }
 
+// 8x8 bins (tile_size 8, tile_render.comp with an 8x8 workgroup): each wave renders one bin from
+// the bin's own list, four bins per workgroup; the waves share nothing and never wait for each
+// other (no workgroup barrier). Per batch of 64 entries the wave stages each entry's tile-local
+// quadratic and colour as a 48-B LDS record, keeps the entries whose alpha >= 1/255 region can
+// reach the bin (quadrantMask on the bin: the box, then the exact ellipse-vs-rectangle test) and
+// walks them as the culled loop does, with the same early exit once all 64 pixels are done.
+template <int FORMAT, int RS>
+__global__ __launch_bounds__(256, 8) void kRenderBins8(const float4* __restrict__ splats,
+                                                    const uint32_t* __restrict__ packed,
+                                                    const uint32_t* __restrict__ values,
+                                                    const uint2* __restrict__ ranges, uint32_t width, uint32_t height,
+                                                    uint32_t binsX, uint32_t binCount, uint32_t gaussianCount,
+                                                    void* __restrict__ out) {
+    __shared__ SplatRecord sRec[4][65];                   // per wave: 64 batch slots + the null slot
+    __shared__ __align__(16) uint32_t sList[4][64 + 4];  // per wave: record offsets in list order
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t bin = blockIdx.x * 4u + wave;
+    if (bin >= binCount) return;  // uniform per wave
+    const uint32_t bx0 = (bin % binsX) * 8, by0 = (bin / binsX) * 8;
+    const uint32_t px = bx0 + (lane & 7u), py = by0 + (lane >> 3);
+    const PixelQ pq = pixelQ(lane & 7u, lane >> 3);
+    SplatRecord* rec = sRec[wave];
+    if (lane == 0) {
+        rec[64].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rec[64].s1 = make_float2(0.0f, -__builtin_inff());
+        rec[64].qmax = __builtin_inff();
+        rec[64].col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    const char* recBase = reinterpret_cast<const char*>(rec);
+    float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    float thr = (px < width && py < height) ? kLog2MinAlpha : __builtin_inff();
+    const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
+
+    const uint2 range = ranges[bin];
+    for (uint32_t batch = range.x; batch < range.y; batch += 64u) {
+        if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;  // the bin is done
+        // The previous batch's records and list are no longer read (same wave, in order).
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t e = batch + lane;
+        bool hit = false, indefinite = false;
+        if (e < range.y) {
+            uint32_t g = values[e];
+            g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
+            float cx, cy;
+            float4 co, col;
+            fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
+            rec[lane].col = col;
+            const float logO = __log2f(co.w);
+            tileQuadratic(cx, cy, co, float(bx0), float(by0), rec[lane].s0, rec[lane].s1);
+            indefinite = !conicPositiveDefinite(co.x, co.y, co.z);
+            rec[lane].qmax = indefinite ? logO : __builtin_inff();
+            const float r2 = 2.0f * (logO - kLog2MinAlpha) * (1.0f / kLog2e);
+            hit = (quadrantMask(cx, cy, co.x, co.y, co.z, r2, float(bx0), float(by0)) & 1u) != 0u;
+        }
+        const uint64_t b = __builtin_amdgcn_ballot_w64(hit);
+        if (hit) sList[wave][uint32_t(__popcll(b & laneMaskLt))] = lane * uint32_t(sizeof(SplatRecord));
+        const uint32_t count = uint32_t(__popcll(b));
+        const uint32_t count4 = (count + 3u) & ~3u;
+        if (lane < count4 - count) sList[wave][count + lane] = 64u * uint32_t(sizeof(SplatRecord));
+        const bool anyIndefinite = __builtin_amdgcn_ballot_w64(indefinite) != 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t j = 0; j < count4; j += 4) {
+            const uint4 four = *reinterpret_cast<const uint4*>(&sList[wave][j]);
+            const SplatRecord* r[4] = {reinterpret_cast<const SplatRecord*>(recBase + four.x),
+                                       reinterpret_cast<const SplatRecord*>(recBase + four.y),
+                                       reinterpret_cast<const SplatRecord*>(recBase + four.z),
+                                       reinterpret_cast<const SplatRecord*>(recBase + four.w)};
+            float q[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) q[u] = evalQ(r[u]->s0, r[u]->s1, pq);
+            if (anyIndefinite) {  // uniform: drop the power > 0 pairs of indefinite conics
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) q[u] = q[u] > r[u]->qmax ? -__builtin_inff() : q[u];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                if (__builtin_amdgcn_ballot_w64(q[u] >= thr) == 0) continue;
+                blendPredicated(q[u], r[u]->col, T, c0, c1, c2, thr);
+            }
+            if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;
+        }
+    }
+    if (px < width && py < height) {
+        const size_t p = size_t(py) * width + px;
+        if (FORMAT == GSR_TARGET_RGBA8) {
+            const uint32_t rgba = quantize8(c0) | (quantize8(c1) << 8) | (quantize8(c2) << 16) |
+                                  (quantize8(1.0f - T) << 24);
+            static_cast<uint32_t*>(out)[p] = rgba;
+        } else {
+            static_cast<float4*>(out)[p] = make_float4(c0, c1, c2, 1.0f - T);
+        }
+    }
+}
+
 // StopThePop (tile_render.comp:201-279, GSR_STOPTHEPOP; CPU twin reference_renderer.cpp:143-196):
 // the reference's alpha (expf, its operation order) and blend rule, applied in the order of a
 // per-pixel window of K slots sorted by the depth of each splat's maximum along the pixel's ray,
@@ -956,6 +1054,26 @@ void launchRender(const float4* splats, const uint32_t* packed, int renderSplat,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
                   uint32_t* tileStats, hipStream_t stream) {
     if (cam.tilesX * cam.tilesY == 0) return;
+    if (cam.binSize == 8) {  // 8x8 bins: the culled loop only (the renderer refuses the other forms)
+        const uint32_t bins = cam.binsX * cam.binsY;
+        const dim3 grid((bins + 3) / 4);
+        auto go = [&](auto kernel) {
+            hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, splats, packed, values, ranges, cam.width,
+                               cam.height, cam.binsX, bins, gaussianCount, out);
+        };
+        const bool f8 = format == GSR_TARGET_RGBA8;
+        if (renderSplat == GSR_RENDER_SPLAT_P32)
+            f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_P32>)
+               : go(kRenderBins8<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_P32>);
+        else if (renderSplat == GSR_RENDER_SPLAT_P24)
+            f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_P24>)
+               : go(kRenderBins8<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_P24>);
+        else
+            f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_FULL>)
+               : go(kRenderBins8<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_FULL>);
+        markKernel("kRenderBins8", stream);
+        return;
+    }
     if (format == GSR_TARGET_RGBA8)
         launchSplat<GSR_TARGET_RGBA8>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
                                       packed, values, ranges, cam, gaussianCount, out, tileStats, stream);

@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                                                           const uint32_t* __restrict__ order,
                                                           const uint32_t* __restrict__ blockOffsets,
                                                           const gsr_control* __restrict__ control, uint32_t count,
-                                                          uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
+                                                          uint32_t bounds, BinGrid grid,
                                                           Key* __restrict__ keys, uint32_t* __restrict__ values,
                                                           gsr_control* controlRW) {
     __shared__ uint32_t sScan[kDupThreads];
@@ -49,6 +49,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     __shared__ float sEl[BINNING ? 9 : 1][BINNING ? kDupThreads : 1];  // cx cy A B C D t exPad eyPad
 
     const uint32_t t = threadIdx.x;
+    const int S = int(grid.binSize);
     const uint32_t j = blockIdx.x * kDupThreads + t;
     const uint32_t live = BINNING ? control->visible : count;
     if (BINNING && blockIdx.x * kDupThreads >= live) return;  // uniform: past the visible splats
@@ -80,10 +81,11 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     if (c != 0u) {
         const float4* s = splats + 3 * size_t(i);
         const float4 o0 = s[0], o1 = s[1], o2 = s[2];
-        const Rect rect = tileRect(o0.x, o0.y, o2.w, tilesX, tilesY);
-        const Coverage cv = makeCoverage(bounds, rect, int(tilesX), int(tilesY), o0.x, o0.y, o1.x, o1.y, o1.z, o1.w);
+        const Rect rect = tileRect(o0.x, o0.y, o2.w, grid.tilesX, grid.tilesY);
+        const Coverage cv = makeCoverage(bounds, rect, S, int(grid.binsX), int(grid.binsY), o0.x, o0.y, o1.x, o1.y,
+                                         o1.z, o1.w);
         int r0, r1;
-        coverageRows(cv, &r0, &r1);
+        coverageRows(cv, S, &r0, &r1);
         sRow0[t] = r0;
         sRows[t] = r1 - r0;
         sDepth[t] = __float_as_uint(o0.z);
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
             uint32_t sum = 0;
             for (int row = r0; row < r1; ++row) {
                 int x0, x1;
-                coverageRowSpan(cv, row, &x0, &x1);
+                coverageRowSpan(cv, row, S, &x0, &x1);
                 if (row - r0 < kCachedRows) sSpan[row - r0][t] = uint32_t(x0) | (uint32_t(x1 - x0) << 16);
                 sum += uint32_t(x1 - x0);
             }
@@ -120,7 +122,8 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     __syncthreads();
 
     const uint32_t end = (base + total < entries) ? base + total : entries;
-    const uint32_t gridTiles = tilesX * tilesY;
+    const uint32_t binsX = grid.binsX;
+    const uint32_t gridTiles = binsX * grid.binsY;
     for (uint32_t k = base + t; k < end; k += kDupThreads) {
         const uint32_t rl = k - base;
         uint32_t lo = 0, hi = kDupThreads;
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
         if (rows < 0) {  // rectangle
             const uint32_t nr = uint32_t(-rows - 1);
             const uint32_t w = uint32_t(sBinX[1][lo]);
-            if (w > 0 && r < w * nr) tile = (uint32_t(r0) + r / w) * tilesX + uint32_t(sBinX[0][lo]) + r % w;
+            if (w > 0 && r < w * nr) tile = (uint32_t(r0) + r / w) * binsX + uint32_t(sBinX[0][lo]) + r % w;
         } else {
             uint32_t before = 0;
             const int cached = rows < kCachedRows ? rows : kCachedRows;
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                 const uint32_t sp = sSpan[row][lo];
                 const uint32_t w = sp >> 16;
                 if (r < before + w) {
-                    tile = uint32_t(r0 + row) * tilesX + (sp & 0xFFFFu) + (r - before);
+                    tile = uint32_t(r0 + row) * binsX + (sp & 0xFFFFu) + (r - before);
                     break;
                 }
                 before += w;
@@ -170,15 +173,16 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                 } else {
                     const float4* s = splats + 3 * size_t(sIndex[lo]);
                     const float4 o0 = s[0], o1 = s[1], o2 = s[2];
-                    const Rect rect = tileRect(o0.x, o0.y, o2.w, tilesX, tilesY);
-                    cv = makeCoverage(bounds, rect, int(tilesX), int(tilesY), o0.x, o0.y, o1.x, o1.y, o1.z, o1.w);
+                    const Rect rect = tileRect(o0.x, o0.y, o2.w, grid.tilesX, grid.tilesY);
+                    cv = makeCoverage(bounds, rect, S, int(grid.binsX), int(grid.binsY), o0.x, o0.y, o1.x, o1.y, o1.z,
+                                      o1.w);
                 }
                 for (int rr = r0 + kCachedRows; rr < r0 + rows; ++rr) {
                     int x0, x1;
-                    coverageRowSpan(cv, rr, &x0, &x1);
+                    coverageRowSpan(cv, rr, S, &x0, &x1);
                     const uint32_t w = uint32_t(x1 - x0);
                     if (r < before + w) {
-                        tile = uint32_t(rr) * tilesX + uint32_t(x0) + (r - before);
+                        tile = uint32_t(rr) * binsX + uint32_t(x0) + (r - before);
                         break;
                     }
                     before += w;
@@ -228,23 +232,23 @@ __global__ __launch_bounds__(kRangesThreads) void kRanges(const Key* __restrict_
 }  // namespace
 
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
-                     const gsr_control* control, uint32_t count, uint32_t bounds, uint32_t tilesX, uint32_t tilesY,
+                     const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
                      uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream) {
     const uint32_t blocks = (count + kDupThreads - 1) / kDupThreads;
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<false, uint64_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
-                       nullptr, blockOffsets, control, count, bounds, tilesX, tilesY, keys, values, controlRW);
+                       nullptr, blockOffsets, control, count, bounds, grid, keys, values, controlRW);
     markKernel("kDuplicate", stream);
 }
 
 void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, const uint32_t* order,
                             const uint32_t* blockOffsets, const gsr_control* control, uint32_t maxVisible,
-                            uint32_t bounds, uint32_t tilesX, uint32_t tilesY, uint32_t* keys, uint32_t* values,
+                            uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values,
                             gsr_control* controlRW, hipStream_t stream) {
     const uint32_t blocks = (maxVisible + kDupThreads - 1) / kDupThreads;
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<true, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, sortedCounts,
-                       order, blockOffsets, control, maxVisible, bounds, tilesX, tilesY, keys, values, controlRW);
+                       order, blockOffsets, control, maxVisible, bounds, grid, keys, values, controlRW);
     markKernel("kDuplicate", stream);
 }
 

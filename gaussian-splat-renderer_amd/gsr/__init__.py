@@ -109,7 +109,8 @@ class Settings(ctypes.Structure):
         ("frame_stats", ctypes.c_int32),
         ("differentiable", ctypes.c_int32),
         ("tile_stats", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 1),
+        ("tile_size", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -530,7 +531,8 @@ class Renderer:
                  exact_alpha: bool = False, sort_algorithm: str = "reduce-then-scan", render_loop: str = "culled",
                  sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full",
                  sort_window: int = 0, adaptive_list: bool = False, adaptive_initial_entries: int = 0,
-                 frame_stats: bool = False, differentiable: bool = False, tile_stats: bool = False):
+                 frame_stats: bool = False, differentiable: bool = False, tile_stats: bool = False,
+                 tile_size: int = 16):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -550,12 +552,14 @@ class Renderer:
         s.frame_stats = 1 if frame_stats else 0
         s.differentiable = 1 if differentiable else 0
         s.tile_stats = 1 if tile_stats else 0
+        s.tile_size = int(tile_size)
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
         scene._renderers.add(self)
         self.sort_mode = sort_mode
         self.render_splat = render_splat
+        self.tile_size = int(tile_size) or 16
         self._colour: Optional["ColourModel"] = None
         self._image: Optional[DeviceBuffer] = None
         self._cam: Optional[Camera] = None
@@ -656,11 +660,18 @@ class Renderer:
         _check(_read_control(self.handle, ctypes.byref(c)), "gsr_read_control")
         return c
 
+    def bins(self, cam: Optional[Camera] = None):
+        """The lists' grid for a camera: (bins_x, bins_y) of tile_size px."""
+        cam = cam or self._cam
+        S = self.tile_size
+        return (int(cam.width) + S - 1) // S, (int(cam.height) + S - 1) // S
+
     def read_back(self):
-        """readBack: sorted tile keys, depth keys, values (entries each), ranges (tiles, 2), control."""
+        """readBack: sorted tile keys, depth keys, values (entries each), ranges (bins, 2), control."""
         c = self.read_control()
         n = int(c.entries)
-        tiles = self._cam.tiles
+        bx, by = self.bins()
+        tiles = bx * by
         keys = np.empty(n, np.uint32)
         depth = np.empty(n, np.uint32)
         vals = np.empty(n, np.uint32)
@@ -699,8 +710,8 @@ class Renderer:
 
     def tile_stats(self) -> np.ndarray:
         """tile_stats=True: (tiles, 8) uint32 per-tile counters of the last frame (TILE_STATS order)."""
-        cam = self._cam
-        tiles = int(cam.tiles_x) * int(cam.tiles_y)
+        bx, by = self.bins()
+        tiles = bx * by
         out = np.zeros((tiles, len(TILE_STATS)), np.uint32)
         _check(_read_tile_stats(self.handle, _ptr(out), tiles), "gsr_read_tile_stats")
         return out

@@ -157,7 +157,13 @@ typedef struct gsr_settings {
     int32_t tile_stats;          /* 1: every frame also counts the per-tile render statistics of the reference's
                                     stats build (TileStat, splat_tile_renderer.h:130-141; gsr_read_tile_stats).
                                     Culled render loop only; the counting slows the render pass */
-    uint32_t reserved[1];
+    int32_t tile_size;           /* TileSettings::tileSize (splat_tile_renderer.h:107): the lists' bins, 16 or 8
+                                    px (0 = 16). INRIA's getRect stays on 16-px tiles; with 8 the coverage is
+                                    taken per 8x8 bin (tile ids, ranges, gsr_read_lists and the tile statistics
+                                    follow the bin grid ceil(w / 8) x ceil(h / 8)) and each bin is rendered by one
+                                    wave. 8 takes the culled loop only (no StopThePop, differentiable, exact
+                                    alpha or tile_stats) */
+    uint32_t reserved[2];
 } gsr_settings;
 
 /* RenderSplatMode (splat_tile_renderer.h:88-92; render_splat.glsl:1-21), same values. The

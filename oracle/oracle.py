@@ -76,6 +76,10 @@ _ray_depth = _sig("oracle_ray_depth", None, [_vp, _vp, _vp, _vp, _vp])
 _preprocess_f64 = _sig("oracle_preprocess_f64", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32,
                                                        _f32, _i32, _vp, _i32])
 _coverage_tiles = _sig("oracle_coverage_tiles", _u32, [_vp, _u32, _i32, _u32, _u32, _i32, _i32, _vp, _u32, _vp])
+_preprocess_bins = _sig("oracle_preprocess_bins", None, [_u32, _vp, _vp, _vp, _vp, _u32, _i32, _vp, _u32, _u32, _f32,
+                                                         _f32, _i32, _u32, _u32, _i32, _vp, _vp, _i32])
+_tile_lists_bins = _sig("oracle_tile_lists_bins", _u32, [_vp, _vp, _u32, _u32, _u32, _i32, _u32, _u32, _u32, _u32, _u32,
+                                                         _vp, _vp, _vp, _vp, _vp, _vp])
 _tile_lists = _sig("oracle_tile_lists", _u32, [_vp, _vp, _u32, _u32, _u32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp,
                                                _vp])
 _radix_sort_pairs = _sig("oracle_radix_sort_pairs", _u32, [_vp, _vp, _u32])
@@ -174,17 +178,19 @@ def splat_alpha_at(splat, px, py):
     return float(_alpha_at(_p(s), px, py))
 
 
-def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGHT, threads=None, render_splat="full"):
-    """preprocessGaussian over the scene: (N,) splats and the per-gaussian tile counts.
-    render_splat "p24": visible splats hold the conic and opacity as the p24 record gives them
-    back (the tiles are counted with those)."""
+def preprocess(scene, cam: OracleCamera, degree=None, gpu_cull=True, bounds=TIGHT, threads=None, render_splat="full",
+               bin_size=16):
+    """preprocessGaussian over the scene: (N,) splats and the per-gaussian tile counts (bins of
+    bin_size = 16 or 8 px). render_splat "p24": visible splats hold the conic and opacity as the p24
+    record gives them back (the tiles are counted with those)."""
     threads = default_threads() if threads is None else threads
     n = scene.count
     out = np.zeros(n, SPLAT_DTYPE)
     counts = np.zeros(n, np.uint32)
     deg = scene.sh_degree if degree is None else min(degree, scene.sh_degree)
-    _preprocess(n, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), scene.K, deg, *cam.args(),
-                1 if gpu_cull else 0, bounds, RENDER_SPLAT[render_splat], _p(out), _p(counts), threads)
+    _preprocess_bins(n, _p(scene.pos_alpha), _p(scene.scales), _p(scene.rotations), _p(scene.sh), scene.K, deg,
+                     *cam.args(), 1 if gpu_cull else 0, bounds, RENDER_SPLAT[render_splat], int(bin_size), _p(out),
+                     _p(counts), threads)
     return out, counts
 
 
@@ -226,8 +232,9 @@ def coverage_tiles(splat, bounds, tiles_x, tiles_y, S=16, grid=None):
     return buf[: min(cnt, cap)].copy(), int(cnt), bool(border.value)
 
 
-def tile_lists(splats, counts, tiles_x, tiles_y, bounds=TIGHT, capacity=None, mode=0):
-    """The GPU path's sorted list: tile keys, depth keys, values, ranges (tiles, 2), duplicates, mismatches."""
+def tile_lists(splats, counts, tiles_x, tiles_y, bounds=TIGHT, capacity=None, mode=0, bin_size=16, grid=None):
+    """The GPU path's sorted list: tile keys, depth keys, values, ranges (tiles, 2), duplicates, mismatches.
+    tiles_x/y: the 16-px tiles; bin_size 8: the list is of 8-px bins on grid = (bins_x, bins_y)."""
     splats = np.ascontiguousarray(splats, SPLAT_DTYPE)
     counts = np.ascontiguousarray(counts, np.uint32)
     n = splats.shape[0]
@@ -238,10 +245,11 @@ def tile_lists(splats, counts, tiles_x, tiles_y, bounds=TIGHT, capacity=None, mo
     hi = np.zeros(max(cap, 1), np.uint32)
     lo = np.zeros(max(cap, 1), np.uint32)
     vals = np.zeros(max(cap, 1), np.uint32)
-    ranges = np.zeros((tiles_x * tiles_y, 2), np.uint32)
+    gx, gy = grid if grid is not None else (tiles_x, tiles_y)
+    ranges = np.zeros((gx * gy, 2), np.uint32)
     cap_arg = cap if capacity is not None else 0xFFFFFFFF
-    e = _tile_lists(_p(splats), _p(counts), n, tiles_x, tiles_y, bounds, cap_arg, mode, _p(hi), _p(lo), _p(vals),
-                    _p(ranges), ctypes.byref(dup), ctypes.byref(mism))
+    e = _tile_lists_bins(_p(splats), _p(counts), n, tiles_x, tiles_y, int(bin_size), gx, gy, bounds, cap_arg, mode,
+                         _p(hi), _p(lo), _p(vals), _p(ranges), ctypes.byref(dup), ctypes.byref(mism))
     return hi[:e], lo[:e], vals[:e], ranges, int(dup.value), int(mism.value)
 
 

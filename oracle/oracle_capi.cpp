@@ -140,11 +140,15 @@ float oracle_splat_alpha_at(const void* splat, float px, float py) {
 
 // preprocessGaussian over a whole scene. out: 48-byte splats; counts: the bin tiles of each
 // drawn splat under `bounds` (0 if culled), as the GPU's tileCounts[].
-void oracle_preprocess(uint32_t n, const float* posAlpha, const float* scales, const float* rot, const float* sh,
-                       uint32_t K, int32_t degree, const float viewInria[16], uint32_t w, uint32_t h, float tanX,
-                       float tanY, int32_t gpuCull, uint32_t bounds, uint32_t renderSplat, void* outSplats,
-                       uint32_t* outCounts, int32_t threads) {
+// binSize S = 16 or 8: the counts are of bins of S pixels (TileSettings::tileSize; the grid
+// ceil(w / S) x ceil(h / S)), from getRect's 16-px rectangle (splat_preprocess.comp:291-308).
+void oracle_preprocess_bins(uint32_t n, const float* posAlpha, const float* scales, const float* rot, const float* sh,
+                            uint32_t K, int32_t degree, const float viewInria[16], uint32_t w, uint32_t h, float tanX,
+                            float tanY, int32_t gpuCull, uint32_t bounds, uint32_t renderSplat, int32_t binSize,
+                            void* outSplats, uint32_t* outCounts, int32_t threads) {
     const Camera<float> cam = makeCamera<float>(viewInria, w, h, tanX, tanY);
+    const int S = binSize;
+    const int gx = int((w + uint32_t(S) - 1) / uint32_t(S)), gy = int((h + uint32_t(S) - 1) / uint32_t(S));
     Splat* out = static_cast<Splat*>(outSplats);
     parallelFor(n, resolveThreads(threads), 4096, [&](uint32_t i) {
         const SplatT<float> s = preprocessGaussian<float>(posAlpha + size_t(i) * 4, scales + size_t(i) * 4,
@@ -156,13 +160,20 @@ void oracle_preprocess(uint32_t n, const float* posAlpha, const float* scales, c
         if (outCounts) {
             uint32_t c = 0;
             if (s.cullReason == kVisible) {
-                const Coverage cv = splatCoverage(bounds, out[i], cam.tilesX, cam.tilesY, kTileSize, int(cam.tilesX),
-                                                  int(cam.tilesY));
-                c = coverageCount(cv, kTileSize);
+                const Coverage cv = splatCoverage(bounds, out[i], cam.tilesX, cam.tilesY, S, gx, gy);
+                c = coverageCount(cv, S);
             }
             outCounts[i] = c;
         }
     });
+}
+
+void oracle_preprocess(uint32_t n, const float* posAlpha, const float* scales, const float* rot, const float* sh,
+                       uint32_t K, int32_t degree, const float viewInria[16], uint32_t w, uint32_t h, float tanX,
+                       float tanY, int32_t gpuCull, uint32_t bounds, uint32_t renderSplat, void* outSplats,
+                       uint32_t* outCounts, int32_t threads) {
+    oracle_preprocess_bins(n, posAlpha, scales, rot, sh, K, degree, viewInria, w, h, tanX, tanY, gpuCull, bounds,
+                           renderSplat, kTileSize, outSplats, outCounts, threads);
 }
 
 // FP64 twin: 12 doubles per gaussian: cx, cy, depth, cull, conic a b c, opacity, r g b, radius.
@@ -217,9 +228,24 @@ uint32_t oracle_coverage_tiles(const void* splat, uint32_t mode, int32_t S, uint
 // stable sort by tile). Entries past `capacity` are dropped (overflow). Outputs (may be
 // null): keysHi = tile, keysLo = depth bits, values = gaussian index (capacity each);
 // ranges = [start, end) per tile (2 * tiles). Returns entries; *duplicates = M.
+// The same with bins of S = 16 or 8 pixels on a grid of gridX x gridY bins (tilesX / tilesY: the
+// 16-px tiles of getRect); tile ids and ranges are bin ids.
+uint32_t oracle_tile_lists_bins(const void* splats, const uint32_t* counts, uint32_t n, uint32_t tilesX,
+                                uint32_t tilesY, int32_t S, uint32_t gridX, uint32_t gridY, uint32_t bounds,
+                                uint32_t capacity, uint32_t mode, uint32_t* keysHi, uint32_t* keysLo, uint32_t* values,
+                                uint32_t* ranges, uint64_t* duplicates, uint32_t* coverageMismatch);
+
 uint32_t oracle_tile_lists(const void* splats, const uint32_t* counts, uint32_t n, uint32_t tilesX, uint32_t tilesY,
                            uint32_t bounds, uint32_t capacity, uint32_t mode, uint32_t* keysHi, uint32_t* keysLo,
                            uint32_t* values, uint32_t* ranges, uint64_t* duplicates, uint32_t* coverageMismatch) {
+    return oracle_tile_lists_bins(splats, counts, n, tilesX, tilesY, kTileSize, tilesX, tilesY, bounds, capacity, mode,
+                                  keysHi, keysLo, values, ranges, duplicates, coverageMismatch);
+}
+
+uint32_t oracle_tile_lists_bins(const void* splats, const uint32_t* counts, uint32_t n, uint32_t tilesX,
+                                uint32_t tilesY, int32_t S, uint32_t gridX, uint32_t gridY, uint32_t bounds,
+                                uint32_t capacity, uint32_t mode, uint32_t* keysHi, uint32_t* keysLo, uint32_t* values,
+                                uint32_t* ranges, uint64_t* duplicates, uint32_t* coverageMismatch) {
     const Splat* sp = static_cast<const Splat*>(splats);
     std::vector<uint32_t> visible;
     for (uint32_t i = 0; i < n; ++i)
@@ -230,15 +256,15 @@ uint32_t oracle_tile_lists(const void* splats, const uint32_t* counts, uint32_t 
     std::vector<uint32_t> tiles, vals;
     uint64_t dup = 0;
     uint32_t mismatch = 0;
-    const int gx = int(tilesX), gy = int(tilesY);
+    const int gx = int(gridX), gy = int(gridY);
     for (uint32_t g : visible) {
-        const Coverage cv = splatCoverage(bounds, sp[g], tilesX, tilesY, kTileSize, gx, gy);
+        const Coverage cv = splatCoverage(bounds, sp[g], tilesX, tilesY, S, gx, gy);
         int r0, r1;
-        coverageRows(cv, kTileSize, &r0, &r1);
+        coverageRows(cv, S, &r0, &r1);
         uint32_t written = 0;
         for (int r = r0; r < r1; ++r) {
             int x0, x1;
-            coverageRowSpan(cv, r, kTileSize, &x0, &x1);
+            coverageRowSpan(cv, r, S, &x0, &x1);
             for (int x = x0; x < x1; ++x) {
                 ++dup;
                 ++written;
@@ -265,7 +291,7 @@ uint32_t oracle_tile_lists(const void* splats, const uint32_t* counts, uint32_t 
         if (values) values[k] = vals[o];
     }
     if (ranges) {
-        const size_t T = size_t(tilesX) * tilesY;
+        const size_t T = size_t(gridX) * gridY;
         std::fill(ranges, ranges + 2 * T, 0u);
         for (uint32_t k = 0; k < entries; ++k) {
             const uint32_t t = tiles[order[k]];

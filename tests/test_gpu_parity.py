@@ -803,3 +803,66 @@ def test_tile_depth_sort_long_ranges(gsr_ctx):
     full.close()
     r.close()
     scene.close()
+
+
+@pytest.mark.parametrize("bounds,sort_mode,render_splat,case", [
+    ("tight", "binning", "full", "100k-720p"), ("tight", "global", "full", "100k-720p"),
+    ("rect", "binning", "full", "100k-720p"), ("tight", "binning", "full", "needles"),
+    ("tight", "global", "p32", "needles"), ("rect", "binning", "full", "needles"),
+    ("cutoff-only", "binning", "full", "needles")])
+def test_tile_size_8(gsr_ctx, bounds, sort_mode, render_splat, case):
+    """tileSize 8 (splat_tile_renderer.cpp:139, 537-541; the cases of test_tight_bounds.cpp:413-423):
+    getRect stays on 16-px tiles and the lists are built on 8-px bins. Tile counts, lists and ranges
+    over the bin grid bit-exact against the oracle with 8-px bins; the image > 60 dB against the
+    oracle render, and with tight bounds bit-identical to the rect image of the same renderer form
+    (fewer entries, nothing visible dropped)."""
+    import gsr
+    import oracle
+    from gsr import camera, scenes
+
+    log(f"case tile size 8, {case}, {bounds} {sort_mode} {render_splat}")
+    if case == "needles":
+        sc, view, W, H = scenes.make_needle_scene(150_000, seed=77), camera.look_at([0.3, 0.5, 6.0], [0, 0, 0]), 640, 360
+    else:
+        sc, view, W, H = scenes.make_scene(100_000, seed=0, mean_sigma=2.0), camera.orbit_view(8.0, 0.0, 0.0), 1280, 720
+    cam = _camera(gsr, view, W, H)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    r = gsr.Renderer(gsr_ctx, scene, bounds=bounds, target="rgba32f", sort_mode=sort_mode, render_splat=render_splat,
+                     tile_size=8)
+    img = r.render_to_host(cam).copy()
+    ob = {"rect": oracle.RECT, "tight": oracle.TIGHT, "cutoff-only": oracle.CUTOFF}[bounds]
+    ocam = oracle.OracleCamera.from_gsr(cam)
+    os_, oc = oracle.preprocess(sc, ocam, bounds=ob, render_splat=render_splat, bin_size=8)
+    gs, gc = r.read_back_splats()
+    _check_splats(gs, gc, os_, oc)
+    bx, by = r.bins()
+    assert (bx, by) == ((W + 7) // 8, (H + 7) // 8)
+    keys, depth, vals, ranges, ctl = r.read_back()
+    hi, lo, ov, orng, dup, mism = oracle.tile_lists(os_, oc, cam.tiles_x, cam.tiles_y, bounds=ob,
+                                                    mode={"global": 0, "binning": 1}[sort_mode], bin_size=8,
+                                                    grid=(bx, by))
+    assert ctl.duplicates == dup and ctl.coverage_mismatch == 0 and mism == 0 and ctl.overflow == 0
+    assert np.array_equal(keys, hi) and np.array_equal(depth, lo) and np.array_equal(vals, ov)
+    assert np.array_equal(ranges, orng)
+    ref, _ = oracle.render(sc, ocam, bounds=oracle.TIGHT if bounds != "rect" else oracle.RECT,
+                           render_splat=render_splat)
+    p = oracle.psnr(img[..., :3], ref)
+    r16 = gsr.Renderer(gsr_ctx, scene, bounds=bounds, target="rgba32f", sort_mode=sort_mode,
+                       render_splat=render_splat)
+    r16.render_to_host(cam)
+    log(f"  {ctl.entries} entries on {bx}x{by} bins (16-px tiles: {r16.read_control().entries}), PSNR {p:.2f} dB")
+    assert p > 60.0
+    if bounds == "tight":
+        rr = gsr.Renderer(gsr_ctx, scene, bounds="rect", target="rgba32f", sort_mode=sort_mode,
+                          render_splat=render_splat, tile_size=8)
+        rimg = rr.render_to_host(cam)
+        assert rr.read_control().entries > ctl.entries
+        assert np.array_equal(img, rimg), "tight 8-px bins changed the image"
+        rr.close()
+    with pytest.raises(gsr.GsrError, match="tile_size"):
+        gsr.Renderer(gsr_ctx, scene, tile_size=4)
+    with pytest.raises(gsr.GsrError, match="tile_size 8"):
+        gsr.Renderer(gsr_ctx, scene, tile_size=8, sort_window=4)
+    r16.close()
+    r.close()
+    scene.close()

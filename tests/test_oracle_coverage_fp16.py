@@ -86,6 +86,50 @@ def test_tight_coverage_invariants(oracle):
     assert tight_tiles < rect_tiles
 
 
+def test_tight_coverage_invariants_8px_bins(oracle):
+    """8-px bins (tileSize 8, test_tight_bounds.cpp:78-176): the tight bins are a subset of the
+    16-px rectangle's bins, hold every pixel with alpha >= 1/255, lie inside the 16-px tight tiles
+    (checked for conics with |b| / sqrt(a c) < 0.999: for a nearly degenerate needle the two grids'
+    row spans are rounded from cancelling terms and may disagree by a bin at the far end, where no
+    pixel reaches the cutoff), and rect bounds enumerate the rectangle's four sub-bins per tile,
+    clipped to the bin grid."""
+    rng = np.random.default_rng(4321)
+    W, H = 404, 300  # 51 x 38 bins: the last bin column is clipped
+    tx, ty = (W + 15) // 16, (H + 15) // 16
+    gx, gy = (W + 7) // 8, (H + 7) // 8
+    tested = 0
+    for _ in range(1200):
+        s = _random_splat(rng, W, H)
+        if s is None:
+            continue
+        r = oracle.tile_rect(float(s["center"][0]), float(s["center"][1]), float(s["radius"]), tx, ty)
+        if r[2] <= r[0] or r[3] <= r[1]:
+            continue
+        tested += 1
+        bins, count, _ = oracle.coverage_tiles(s, oracle.TIGHT, tx, ty, S=8, grid=(gx, gy))
+        assert count == len(bins) and len(set(bins.tolist())) == count
+        kept = np.zeros((gy, gx), bool)
+        kept.reshape(-1)[bins] = True
+        rmask = np.zeros((gy, gx), bool)
+        rmask[2 * r[1]:min(2 * r[3], gy), 2 * r[0]:min(2 * r[2], gx)] = True
+        assert not np.any(kept & ~rmask)
+        tiles16, _, _ = oracle.coverage_tiles(s, oracle.TIGHT, tx, ty)
+        k16 = np.zeros((ty, tx), bool)
+        k16.reshape(-1)[tiles16] = True
+        a, b, c = (float(v) for v in s["conic_opacity"][:3])
+        if abs(b) / np.sqrt(a * c) < 0.999:
+            by, bx = np.nonzero(kept)
+            assert np.all(k16[by // 2, bx // 2]), "an 8-px bin outside the 16-px tight tiles"
+        xs = np.arange(r[0] * 16, min(r[2] * 16, W))
+        ys = np.arange(r[1] * 16, min(r[3] * 16, H))
+        if xs.size and ys.size:
+            vis = _alpha_grid(s, xs, ys)
+            assert not np.any(vis & ~kept[(ys // 8)[:, None], (xs // 8)[None, :]]), "a visible pixel was dropped"
+        rb, rc, _ = oracle.coverage_tiles(s, oracle.RECT, tx, ty, S=8, grid=(gx, gy))
+        assert rc == int(rmask.sum()) == len(rb)
+    assert tested > 500
+
+
 def test_tight_image_bit_identical_to_rect(oracle):
     from gsr import camera, scenes
 

@@ -81,3 +81,35 @@ def test_tile_lists_both_modes_vs_brute_force(oracle, capacity):
         kept_max = max(x[1] for x in by_depth[:cap])
         dropped_min = min(x[1] for x in by_depth[cap:])
         assert kept_max <= dropped_min
+
+
+@pytest.mark.parametrize("bounds", ["rect", "tight", "cutoff"])
+def test_tile_lists_8px_bins_vs_brute_force(oracle, bounds):
+    """tileSize 8: the preprocess counts are the 8-px coverage counts, and both sort modes' lists
+    over the bin grid equal the brute force (bins of each splat row by row, sorted as above)."""
+    from gsr import camera, scenes
+
+    b = {"rect": oracle.RECT, "tight": oracle.TIGHT, "cutoff": oracle.CUTOFF}[bounds]
+    W, H = 324, 180
+    sc = scenes.make_scene(2500, seed=23, mean_sigma=1.5)
+    cam = oracle.OracleCamera.from_viewer(camera.orbit_view(6.0, 10.0, 35.0), camera.perspective(60.0, W / H),
+                                          camera.IDENTITY, W, H)
+    splats, counts = oracle.preprocess(sc, cam, bounds=b, bin_size=8)
+    tx, ty, gx, gy = (W + 15) // 16, (H + 15) // 16, (W + 7) // 8, (H + 7) // 8
+    depth_bits = splats["depth"].view(np.uint32)
+    entries = []
+    for g in range(len(counts)):
+        if splats["cull_reason"][g] != 0:
+            assert counts[g] == 0
+            continue
+        bins, c, _ = oracle.coverage_tiles(splats[g], b, tx, ty, S=8, grid=(gx, gy))
+        assert c == counts[g]
+        entries.extend((int(t), int(depth_bits[g]), g) for t in bins)
+    expect = sorted(entries, key=lambda e: (e[0], e[1], e[2]))
+    for mode in (0, 1):
+        hi, lo, vals, ranges, dup, mism = oracle.tile_lists(splats, counts, tx, ty, bounds=b, mode=mode, bin_size=8,
+                                                            grid=(gx, gy))
+        assert dup == len(entries) and mism == 0
+        assert hi.tolist() == [e[0] for e in expect] and vals.tolist() == [e[2] for e in expect]
+        assert lo.tolist() == [e[1] for e in expect]
+        assert np.array_equal(ranges, _ranges(hi.tolist(), gx * gy))
