@@ -109,8 +109,9 @@ def multiview_bench(ctx, views=32, count=1_000_000, reps=5):
     scene = gsr.Scene(ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
     outs = [gsr.DeviceBuffer(ctx, int(c.width) * int(c.height) * 4) for c in cams]
 
-    def make():
-        return gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global")
+    def make(tile_size=16, render_batch=0):
+        return gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global", tile_size=tile_size,
+                            render_batch=render_batch)
 
     def timed(fn):
         ctx.synchronize()
@@ -130,18 +131,22 @@ def multiview_bench(ctx, views=32, count=1_000_000, reps=5):
     seq_ms = timed(sequential) / reps
     seq.close()
     best, by = gsr.tune_batch(ctx, make, cams, [o.ptr for o in outs], candidates=(1, 2, 4, 8), replays=reps,
-                              timer=timed)
+                              timer=timed, tile_sizes=(16, 8), batches=(0, 128, 512))
+    best_ms = by[(best["tile_size"], best["render_batch"], best["branches"])]
+    default16 = {k[2]: v for k, v in by.items() if k[0] == 16 and k[1] == 0}
     for o in outs:
         o.close()
     scene.close()
     print(f"# multiview: {views} views, {pixels / 1e6:.1f} Mpixel: sequential {seq_ms:.3f} ms, batched "
-          f"{by[best]:.3f} ms with {best} branches ({', '.join(f'{k}: {v:.3f}' for k, v in by.items())})",
-          file=sys.stderr, flush=True)
+          f"{best_ms:.3f} ms with {best} ({len(by)} configurations tuned; 16-px default batch: "
+          f"{', '.join(f'{k}: {v:.3f}' for k, v in default16.items())})", file=sys.stderr, flush=True)
     return {"views": views, "gaussians": count, "sizes": "uniform of 640x360, 1280x720, 1920x1080",
             "megapixels": round(pixels / 1e6, 2), "sequential_ms": round(seq_ms, 3),
-            "sequential_mpix_per_s": round(pixels / 1e6 / (seq_ms * 1e-3), 1), "batched_ms": round(by[best], 3),
-            "batched_mpix_per_s": round(pixels / 1e6 / (by[best] * 1e-3), 1), "branches": best,
-            "batched_ms_by_branches": {str(k): round(v, 3) for k, v in by.items()}, "graph": True}
+            "sequential_mpix_per_s": round(pixels / 1e6 / (seq_ms * 1e-3), 1), "batched_ms": round(best_ms, 3),
+            "batched_mpix_per_s": round(pixels / 1e6 / (best_ms * 1e-3), 1), "autotuned": best,
+            "batched_ms_by_config": {f"tile{k[0]}-batch{k[1] or 'default'}-branches{k[2]}": round(v, 3)
+                                     for k, v in sorted(by.items())},
+            "graph": True}
 
 
 def learned_bench(ctx, cams, frames, count=3_000_000, samples=1 << 24):

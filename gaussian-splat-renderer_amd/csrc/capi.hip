@@ -732,6 +732,19 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
         return fail(GSR_ERROR_INVALID_ARGUMENT,
                     "gsr_renderer_create: tile_size must be 8 or 16 (a divisor of INRIA's 16-px tiles), got %d",
                     s.tile_size);
+    const bool bins8 = s.tile_size == 8;
+    if (s.render_batch == 0) s.render_batch = bins8 ? 64u : 256u;
+    if (bins8 ? (s.render_batch != 32 && s.render_batch != 64 && s.render_batch != 128)
+              : (s.render_batch != 128 && s.render_batch != 256 && s.render_batch != 512))
+        return fail(GSR_ERROR_INVALID_ARGUMENT,
+                    "gsr_renderer_create: render_batch %u must be %s", s.render_batch,
+                    bins8 ? "32, 64 or 128 (splats per 8x8 bin and batch)" : "128, 256 or 512 (splats per tile and batch)");
+    if (s.render_batch != (bins8 ? 64u : 256u) &&
+        (s.render_loop != GSR_RENDER_LOOP_CULLED || s.render_pixels_per_thread != 1 || s.render_exact_alpha ||
+         s.sort_window != 0 || s.differentiable || s.tile_stats))
+        return fail(GSR_ERROR_UNSUPPORTED,
+                    "gsr_renderer_create: render_batch other than the default needs the culled loop (one pixel per "
+                    "thread, fast alpha, sort_window 0, not differentiable, no tile_stats)");
     if (s.tile_size == 8 &&
         (s.render_loop != GSR_RENDER_LOOP_CULLED || s.render_pixels_per_thread != 1 || s.render_exact_alpha ||
          s.sort_window != 0 || s.differentiable || s.tile_stats))
@@ -1070,7 +1083,7 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
                      r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
                      r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0,
                      r->settings.render_loop, int(r->settings.sort_window), r->rayDepth.as<float4>(), outImage,
-                     tileStats, stream);
+                     tileStats, int(r->settings.render_batch), stream);
     }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(7));

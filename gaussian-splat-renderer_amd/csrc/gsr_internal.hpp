@@ -169,7 +169,7 @@ constexpr int kTileStatCount = 8;
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
-                  uint32_t* tileStats, hipStream_t stream);
+                  uint32_t* tileStats, int batch, hipStream_t stream);
 
 // Learned colour (mlp.hip): a 16-value latent per gaussian and one shared MLP. Parameters:
 // mlpParamCount() floats; packs: mlpPackedTiles() x 64 uint2 (the bf16 A-operand tiles, rebuilt

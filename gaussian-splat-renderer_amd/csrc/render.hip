@@ -25,6 +25,8 @@
 //   - XCD-aware tile order: workgroup b runs on XCD b % 8; XCD x gets the tile rows x, x + 8,
 //     ... so horizontally neighbouring tiles (which share splats) share an L2 while every
 //     XCD gets an even share of the dense (image-centre) and sparse rows.
+#include <type_traits>
+
 #include "gsr_device.hpp"
 
 namespace gsr {
@@ -343,70 +345,81 @@ __device__ __forceinline__ uint32_t waveMax(uint32_t v) {
 // until it terminates, the terminating one included), 3 pairs blended, 4 pixels that
 // terminated (T < 1e-4), 5 lane slots this kernel spent in its blend loop (per wave and batch,
 // the list slots walked x 64), 6 the largest per-pixel evaluated count, 7 pixels in the image.
-template <int FORMAT, int RS, bool STATS = false>
-__global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict__ splats,
-                                                     const uint32_t* __restrict__ packed,
-                                                     const uint32_t* __restrict__ values,
-                                                     const uint2* __restrict__ ranges, uint32_t width,
-                                                     uint32_t height, uint32_t tilesX, uint32_t tileCount,
-                                                     uint32_t gaussianCount, void* __restrict__ out,
-                                                     uint32_t* __restrict__ tileStats = nullptr) {
-    // One 48-B record per batch slot: the tile-local quadratic of q (tileQuadratic) and the
-    // colour. Slot kBatch is a null splat (q = -inf for every pixel) that pads the wave lists.
-    // The lists hold the records' byte offsets, so a splat's reads need no address arithmetic.
-    __shared__ SplatRecord sRec[kBatch + 1];
-    __shared__ uint32_t sMask[kBatch];                     // bit w: the box meets wave w's quadrant
-    __shared__ __align__(16) uint32_t sList[4][kBatch + 4];  // per wave: record offsets in list order
+// The culled loop's batch of B splats in LDS: one 48-B record per slot (the tile-local quadratic of
+// q, tileQuadratic, qmax and the colour; slot B is a null splat, q = -inf for every pixel, that
+// pads the wave lists), the quadrant bits of each slot and each wave's list of record byte
+// offsets (so a splat's reads need no address arithmetic).
+template <int B>
+struct CulledBatch {
+    SplatRecord rec[B + 1];
+    uint32_t mask[B];                    // bit w: the splat can reach wave w's quadrant
+    __align__(16) uint32_t list[4][B + 4];
+};
 
-    const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
-    const uint32_t row = xcd + kXcds * centreOut(idx / tilesX, gridDim.x / kXcds / tilesX);
-    const uint32_t tile = row * tilesX + idx % tilesX;
-    if (tile >= tileCount) return;  // uniform
+// What one tile of a frame reads and writes.
+struct TileJob {
+    const float4* splats;
+    const uint32_t* packed;
+    const uint32_t* values;
+    const uint2* ranges;
+    uint32_t width, height, tilesX, gaussianCount;
+    void* out;
+    uint32_t* tileStats;
+};
+
+// One 16x16 tile by the culled loop (the whole workgroup of 256 threads: it waits at barriers).
+template <int FORMAT, int RS, bool STATS, int B>
+__device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t tile, CulledBatch<B>& sh) {
+    static_assert(B % 64 == 0 && B >= 64 && B <= 512, "batches of 64..512 splats");
+    constexpr uint32_t kFetch = (B + 255) / 256;  // slots each thread stages
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    const uint32_t tx0 = (tile % tilesX) * 16, ty0 = (tile / tilesX) * 16;
+    const uint32_t tx0 = (tile % job.tilesX) * 16, ty0 = (tile / job.tilesX) * 16;
     const uint32_t px = tx0 + (wave & 1u) * 8 + (lane & 7u);
     const uint32_t py = ty0 + (wave >> 1) * 8 + (lane >> 3);
     const PixelQ pq = pixelQ(px - tx0, py - ty0);
     if (t == 0) {
-        sRec[kBatch].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        sRec[kBatch].s1 = make_float2(0.0f, -__builtin_inff());
-        sRec[kBatch].qmax = __builtin_inff();
-        sRec[kBatch].col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        sh.rec[B].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        sh.rec[B].s1 = make_float2(0.0f, -__builtin_inff());
+        sh.rec[B].qmax = __builtin_inff();
+        sh.rec[B].col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    const char* recBase = reinterpret_cast<const char*>(sRec);
+    const char* recBase = reinterpret_cast<const char*>(sh.rec);
 
     float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-    float thr = (px < width && py < height) ? kLog2MinAlpha : __builtin_inff();
+    float thr = (px < job.width && py < job.height) ? kLog2MinAlpha : __builtin_inff();
     const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
 
-    const uint2 range = ranges[tile];
+    const uint2 range = job.ranges[tile];
     uint32_t statFetched = 0, statTermEntry = 0xFFFFFFFFu, statBlended = 0, statSlots = 0;
-    for (uint32_t batch = range.x; batch < range.y; batch += kBatch) {
+    for (uint32_t batch = range.x; batch < range.y; batch += B) {
         // Whole-tile vote; also the barrier that protects the previous batch's LDS.
         if (__syncthreads_count(thr < __builtin_inff() ? 0 : 1) == 256) break;
-        if (STATS) statFetched += range.y - batch < uint32_t(kBatch) ? range.y - batch : uint32_t(kBatch);
+        if (STATS) statFetched += range.y - batch < uint32_t(B) ? range.y - batch : uint32_t(B);
         bool indefinite = false;
-        {
-            const uint32_t e = batch + t;
+#pragma unroll
+        for (uint32_t f = 0; f < kFetch; ++f) {
+            const uint32_t slot = t + f * 256u;
+            if (B % 256 != 0 && slot >= uint32_t(B)) break;
+            const uint32_t e = batch + slot;
             uint32_t mask = 0u;
             if (e < range.y) {
-                uint32_t g = values[e];
-                g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
+                uint32_t g = job.values[e];
+                g = g < job.gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
                 float cx, cy;
                 float4 co, col;
-                fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
-                const float2 o0 = make_float2(cx, cy);
-                sRec[t].col = col;
+                fetchSplat<RS>(job.splats, job.packed, g, cx, cy, co, col);
+                sh.rec[slot].col = col;
                 const float logO = __log2f(co.w);
-                tileQuadratic(o0.x, o0.y, co, float(tx0), float(ty0), sRec[t].s0, sRec[t].s1);
-                indefinite = !conicPositiveDefinite(co.x, co.y, co.z);
-                sRec[t].qmax = indefinite ? logO : __builtin_inff();
+                tileQuadratic(cx, cy, co, float(tx0), float(ty0), sh.rec[slot].s0, sh.rec[slot].s1);
+                const bool ind = !conicPositiveDefinite(co.x, co.y, co.z);
+                indefinite = indefinite || ind;
+                sh.rec[slot].qmax = ind ? logO : __builtin_inff();
                 // The quadrants the splat can reach (quadrantMask: box, then the exact
                 // ellipse-vs-quadrant test, both with a margin sized to the rounding of q).
                 const float r2 = 2.0f * (logO - kLog2MinAlpha) * (1.0f / kLog2e);
-                mask = quadrantMask(o0.x, o0.y, co.x, co.y, co.z, r2, float(tx0), float(ty0));
+                mask = quadrantMask(cx, cy, co.x, co.y, co.z, r2, float(tx0), float(ty0));
             }
-            sMask[t] = mask;
+            sh.mask[slot] = mask;
         }
         // Also tells every wave whether the batch holds a splat with an indefinite conic (only
         // p24 records of thin splats): only such batches test power <= 0 per pair.
@@ -415,21 +428,21 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
         // (slot = j * 64 + lane), padded with the null slot to a multiple of 4.
         uint32_t count = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
+        for (uint32_t j = 0; j < uint32_t(B) / 64u; ++j) {
             const uint32_t slot = j * 64 + lane;
-            const bool hit = (sMask[slot] >> wave) & 1u;
+            const bool hit = (sh.mask[slot] >> wave) & 1u;
             const uint64_t b = __builtin_amdgcn_ballot_w64(hit);
-            if (hit) sList[wave][count + uint32_t(__popcll(b & laneMaskLt))] = slot * uint32_t(sizeof(SplatRecord));
+            if (hit) sh.list[wave][count + uint32_t(__popcll(b & laneMaskLt))] = slot * uint32_t(sizeof(SplatRecord));
             count += uint32_t(__popcll(b));
         }
         const uint32_t count4 = (count + 3u) & ~3u;
-        if (lane < count4 - count) sList[wave][count + lane] = kBatch * uint32_t(sizeof(SplatRecord));
+        if (lane < count4 - count) sh.list[wave][count + lane] = uint32_t(B) * uint32_t(sizeof(SplatRecord));
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) != 0) {
             for (uint32_t j = 0; j < count4; j += 4) {
-                const uint4 four = *reinterpret_cast<const uint4*>(&sList[wave][j]);
+                const uint4 four = *reinterpret_cast<const uint4*>(&sh.list[wave][j]);
                 // The four q first (their LDS reads in flight together), then the blends in order.
                 const SplatRecord* rec[4] = {reinterpret_cast<const SplatRecord*>(recBase + four.x),
                                              reinterpret_cast<const SplatRecord*>(recBase + four.y),
@@ -460,15 +473,15 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
             }
         }
     }
-    const bool inside = px < width && py < height;
+    const bool inside = px < job.width && py < job.height;
     if (inside) {
-        const size_t p = size_t(py) * width + px;
+        const size_t p = size_t(py) * job.width + px;
         if (FORMAT == GSR_TARGET_RGBA8) {
             const uint32_t rgba = quantize8(c0) | (quantize8(c1) << 8) | (quantize8(c2) << 16) |
                                   (quantize8(1.0f - T) << 24);
-            static_cast<uint32_t*>(out)[p] = rgba;
+            static_cast<uint32_t*>(job.out)[p] = rgba;
         } else {
-            static_cast<float4*>(out)[p] = make_float4(c0, c1, c2, 1.0f - T);
+            static_cast<float4*>(job.out)[p] = make_float4(c0, c1, c2, 1.0f - T);
         }
     }
     if (STATS) {
@@ -477,7 +490,7 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
         const uint32_t sumEval = waveSum(evaluated), sumBlend = waveSum(inside ? statBlended : 0u);
         const uint32_t sumTerm = waveSum(terminated ? 1u : 0u), sumInside = waveSum(inside ? 1u : 0u);
         const uint32_t maxEval = waveMax(evaluated);
-        uint32_t* st = tileStats + size_t(tile) * 8u;
+        uint32_t* st = job.tileStats + size_t(tile) * 8u;
         if (lane == 0) {
             atomicAdd(&st[2], sumEval);
             atomicAdd(&st[3], sumBlend);
@@ -493,21 +506,33 @@ __global__ __launch_bounds__(256, 8) void kRenderCulled(const float4* __restrict
     }
 }
 
+// The culled loop over a frame's tiles: workgroup b takes tile rows of XCD b % 8 centre-out.
+template <int FORMAT, int RS, bool STATS, int B>
+__global__ __launch_bounds__(256, B == 512 ? 4 : 8) void kRenderCulled(TileJob job, uint32_t tileCount) {
+    __shared__ CulledBatch<B> sh;
+    const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
+    const uint32_t row = xcd + kXcds * centreOut(idx / job.tilesX, gridDim.x / kXcds / job.tilesX);
+    const uint32_t tile = row * job.tilesX + idx % job.tilesX;
+    if (tile >= tileCount) return;  // uniform
+    renderTileCulled<FORMAT, RS, STATS, B>(job, tile, sh);
+}
+
 // 8x8 bins (tile_size 8, tile_render.comp with an 8x8 workgroup): each wave renders one bin from
 // the bin's own list, four bins per workgroup; the waves share nothing and never wait for each
 // other (no workgroup barrier). Per batch of 64 entries the wave stages each entry's tile-local
 // quadratic and colour as a 48-B LDS record, keeps the entries whose alpha >= 1/255 region can
 // reach the bin (quadrantMask on the bin: the box, then the exact ellipse-vs-rectangle test) and
 // walks them as the culled loop does, with the same early exit once all 64 pixels are done.
-template <int FORMAT, int RS>
-__global__ __launch_bounds__(256, 8) void kRenderBins8(const float4* __restrict__ splats,
+template <int FORMAT, int RS, int BW>
+__global__ __launch_bounds__(256, BW == 128 ? 6 : 8) void kRenderBins8(const float4* __restrict__ splats,
                                                     const uint32_t* __restrict__ packed,
                                                     const uint32_t* __restrict__ values,
                                                     const uint2* __restrict__ ranges, uint32_t width, uint32_t height,
                                                     uint32_t binsX, uint32_t binCount, uint32_t gaussianCount,
                                                     void* __restrict__ out) {
-    __shared__ SplatRecord sRec[4][65];                   // per wave: 64 batch slots + the null slot
-    __shared__ __align__(16) uint32_t sList[4][64 + 4];  // per wave: record offsets in list order
+    static_assert(BW == 32 || BW == 64 || BW == 128, "per-wave batches of 32, 64 or 128 splats");
+    __shared__ SplatRecord sRec[4][BW + 1];               // per wave: BW batch slots + the null slot
+    __shared__ __align__(16) uint32_t sList[4][BW + 4];  // per wave: record offsets in list order
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t bin = blockIdx.x * 4u + wave;
     if (bin >= binCount) return;  // uniform per wave
@@ -516,10 +541,10 @@ __global__ __launch_bounds__(256, 8) void kRenderBins8(const float4* __restrict_
     const PixelQ pq = pixelQ(lane & 7u, lane >> 3);
     SplatRecord* rec = sRec[wave];
     if (lane == 0) {
-        rec[64].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        rec[64].s1 = make_float2(0.0f, -__builtin_inff());
-        rec[64].qmax = __builtin_inff();
-        rec[64].col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rec[BW].s0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rec[BW].s1 = make_float2(0.0f, -__builtin_inff());
+        rec[BW].qmax = __builtin_inff();
+        rec[BW].col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     const char* recBase = reinterpret_cast<const char*>(rec);
     float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
@@ -527,33 +552,40 @@ __global__ __launch_bounds__(256, 8) void kRenderBins8(const float4* __restrict_
     const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
 
     const uint2 range = ranges[bin];
-    for (uint32_t batch = range.x; batch < range.y; batch += 64u) {
+    for (uint32_t batch = range.x; batch < range.y; batch += BW) {
         if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) == 0) break;  // the bin is done
         // The previous batch's records and list are no longer read (same wave, in order).
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const uint32_t e = batch + lane;
-        bool hit = false, indefinite = false;
-        if (e < range.y) {
-            uint32_t g = values[e];
-            g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
-            float cx, cy;
-            float4 co, col;
-            fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
-            rec[lane].col = col;
-            const float logO = __log2f(co.w);
-            tileQuadratic(cx, cy, co, float(bx0), float(by0), rec[lane].s0, rec[lane].s1);
-            indefinite = !conicPositiveDefinite(co.x, co.y, co.z);
-            rec[lane].qmax = indefinite ? logO : __builtin_inff();
-            const float r2 = 2.0f * (logO - kLog2MinAlpha) * (1.0f / kLog2e);
-            hit = (quadrantMask(cx, cy, co.x, co.y, co.z, r2, float(bx0), float(by0)) & 1u) != 0u;
+        bool indefinite = false;
+        uint32_t count = 0;
+#pragma unroll
+        for (uint32_t f = 0; f < uint32_t(BW + 63) / 64u; ++f) {
+            const uint32_t slot = f * 64u + lane;
+            const uint32_t e = batch + slot;
+            bool hit = false;
+            if (slot < uint32_t(BW) && e < range.y) {
+                uint32_t g = values[e];
+                g = g < gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
+                float cx, cy;
+                float4 co, col;
+                fetchSplat<RS>(splats, packed, g, cx, cy, co, col);
+                rec[slot].col = col;
+                const float logO = __log2f(co.w);
+                tileQuadratic(cx, cy, co, float(bx0), float(by0), rec[slot].s0, rec[slot].s1);
+                const bool ind = !conicPositiveDefinite(co.x, co.y, co.z);
+                indefinite = indefinite || ind;
+                rec[slot].qmax = ind ? logO : __builtin_inff();
+                const float r2 = 2.0f * (logO - kLog2MinAlpha) * (1.0f / kLog2e);
+                hit = (quadrantMask(cx, cy, co.x, co.y, co.z, r2, float(bx0), float(by0)) & 1u) != 0u;
+            }
+            const uint64_t b = __builtin_amdgcn_ballot_w64(hit);
+            if (hit) sList[wave][count + uint32_t(__popcll(b & laneMaskLt))] = slot * uint32_t(sizeof(SplatRecord));
+            count += uint32_t(__popcll(b));
         }
-        const uint64_t b = __builtin_amdgcn_ballot_w64(hit);
-        if (hit) sList[wave][uint32_t(__popcll(b & laneMaskLt))] = lane * uint32_t(sizeof(SplatRecord));
-        const uint32_t count = uint32_t(__popcll(b));
         const uint32_t count4 = (count + 3u) & ~3u;
-        if (lane < count4 - count) sList[wave][count + lane] = 64u * uint32_t(sizeof(SplatRecord));
+        if (lane < count4 - count) sList[wave][count + lane] = uint32_t(BW) * uint32_t(sizeof(SplatRecord));
         const bool anyIndefinite = __builtin_amdgcn_ballot_w64(indefinite) != 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -969,18 +1001,20 @@ void launchStp(int window, const float4* splats, const uint32_t* packed, const f
 
 template <int FORMAT, int RS>
 void launchCulled(const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
-                  const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, hipStream_t stream) {
+                  const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, int batch,
+                  hipStream_t stream) {
     const uint32_t tileCount = cam.tilesX * cam.tilesY;
     const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
-    const uint32_t perXcd = rowsPerXcd * cam.tilesX;
-    if (tileStats)
-        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, true>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats,
-                           packed, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out,
-                           tileStats);
+    const dim3 grid(rowsPerXcd * cam.tilesX * kXcds);
+    const TileJob job{splats, packed, values, ranges, cam.width, cam.height, cam.tilesX, gaussianCount, out, tileStats};
+    if (tileStats)  // the statistics build: the default batch only
+        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, true, kBatch>), grid, dim3(256), 0, stream, job, tileCount);
+    else if (batch == 128)
+        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, false, 128>), grid, dim3(256), 0, stream, job, tileCount);
+    else if (batch == 512)
+        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, false, 512>), grid, dim3(256), 0, stream, job, tileCount);
     else
-        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, false>), dim3(perXcd * kXcds), dim3(256), 0, stream, splats,
-                           packed, values, ranges, cam.width, cam.height, cam.tilesX, tileCount, gaussianCount, out,
-                           nullptr);
+        hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, false, kBatch>), grid, dim3(256), 0, stream, job, tileCount);
     markKernel("kRenderCulled", stream);
 }
 
@@ -999,7 +1033,7 @@ void launchOne(const float4* splats, const uint32_t* packed, const uint32_t* val
 template <int FORMAT, int RS>
 void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uint32_t* packed, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats,
-                  hipStream_t stream) {
+                  int batch, hipStream_t stream) {
     if (exact) {
         if (ppt == 4)
             launchOne<FORMAT, false, 4, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
@@ -1016,14 +1050,15 @@ void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uin
     } else if (loop == GSR_RENDER_LOOP_PREDICATED) {
         launchOne<FORMAT, true, 1, true, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else {
-        launchCulled<FORMAT, RS>(splats, packed, values, ranges, cam, gaussianCount, out, tileStats, stream);
+        launchCulled<FORMAT, RS>(splats, packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream);
     }
 }
 
 template <int FORMAT>
 void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow, const float4* rayDepth,
                  const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
-                 const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, hipStream_t stream) {
+                 const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, int batch,
+                 hipStream_t stream) {
     if (sortWindow > 0) {
         if (renderSplat == GSR_RENDER_SPLAT_P32)
             launchStp<FORMAT, GSR_RENDER_SPLAT_P32>(sortWindow, splats, packed, rayDepth, values, ranges, cam,
@@ -1038,13 +1073,13 @@ void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow,
     }
     if (renderSplat == GSR_RENDER_SPLAT_P32)
         launchFormat<FORMAT, GSR_RENDER_SPLAT_P32>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
-                                                   out, tileStats, stream);
+                                                   out, tileStats, batch, stream);
     else if (renderSplat == GSR_RENDER_SPLAT_P24)
         launchFormat<FORMAT, GSR_RENDER_SPLAT_P24>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
-                                                   out, tileStats, stream);
+                                                   out, tileStats, batch, stream);
     else
         launchFormat<FORMAT, GSR_RENDER_SPLAT_FULL>(ppt, exact, loop, splats, packed, values, ranges, cam,
-                                                    gaussianCount, out, tileStats, stream);
+                                                    gaussianCount, out, tileStats, batch, stream);
 }
 
 }  // namespace
@@ -1052,7 +1087,7 @@ void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow,
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
-                  uint32_t* tileStats, hipStream_t stream) {
+                  uint32_t* tileStats, int batch, hipStream_t stream) {
     if (cam.tilesX * cam.tilesY == 0) return;
     if (cam.binSize == 8) {  // 8x8 bins: the culled loop only (the renderer refuses the other forms)
         const uint32_t bins = cam.binsX * cam.binsY;
@@ -1061,25 +1096,26 @@ void launchRender(const float4* splats, const uint32_t* packed, int renderSplat,
             hipLaunchKernelGGL(kernel, grid, dim3(256), 0, stream, splats, packed, values, ranges, cam.width,
                                cam.height, cam.binsX, bins, gaussianCount, out);
         };
-        const bool f8 = format == GSR_TARGET_RGBA8;
-        if (renderSplat == GSR_RENDER_SPLAT_P32)
-            f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_P32>)
-               : go(kRenderBins8<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_P32>);
-        else if (renderSplat == GSR_RENDER_SPLAT_P24)
-            f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_P24>)
-               : go(kRenderBins8<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_P24>);
-        else
-            f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_FULL>)
-               : go(kRenderBins8<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_FULL>);
+        auto pick = [&](auto rs) {
+            constexpr int RS = decltype(rs)::value;
+            const bool f8 = format == GSR_TARGET_RGBA8;
+            if (batch == 32) f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, RS, 32>) : go(kRenderBins8<GSR_TARGET_RGBA32F, RS, 32>);
+            else if (batch == 128)
+                f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, RS, 128>) : go(kRenderBins8<GSR_TARGET_RGBA32F, RS, 128>);
+            else f8 ? go(kRenderBins8<GSR_TARGET_RGBA8, RS, 64>) : go(kRenderBins8<GSR_TARGET_RGBA32F, RS, 64>);
+        };
+        if (renderSplat == GSR_RENDER_SPLAT_P32) pick(std::integral_constant<int, GSR_RENDER_SPLAT_P32>{});
+        else if (renderSplat == GSR_RENDER_SPLAT_P24) pick(std::integral_constant<int, GSR_RENDER_SPLAT_P24>{});
+        else pick(std::integral_constant<int, GSR_RENDER_SPLAT_FULL>{});
         markKernel("kRenderBins8", stream);
         return;
     }
     if (format == GSR_TARGET_RGBA8)
         launchSplat<GSR_TARGET_RGBA8>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
-                                      packed, values, ranges, cam, gaussianCount, out, tileStats, stream);
+                                      packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream);
     else
         launchSplat<GSR_TARGET_RGBA32F>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
-                                        packed, values, ranges, cam, gaussianCount, out, tileStats, stream);
+                                        packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream);
 }
 
 }  // namespace gsr

@@ -110,7 +110,8 @@ class Settings(ctypes.Structure):
         ("differentiable", ctypes.c_int32),
         ("tile_stats", ctypes.c_int32),
         ("tile_size", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("render_batch", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 1),
     ]
 
 
@@ -532,7 +533,7 @@ class Renderer:
                  sort_mode: str = "binning", full_key_sort: bool = False, render_splat: str = "full",
                  sort_window: int = 0, adaptive_list: bool = False, adaptive_initial_entries: int = 0,
                  frame_stats: bool = False, differentiable: bool = False, tile_stats: bool = False,
-                 tile_size: int = 16):
+                 tile_size: int = 16, render_batch: int = 0):
         s = Settings()
         s.bounds = BOUNDS[bounds]
         s.budget_mb = budget_mb
@@ -553,6 +554,7 @@ class Renderer:
         s.differentiable = 1 if differentiable else 0
         s.tile_stats = 1 if tile_stats else 0
         s.tile_size = int(tile_size)
+        s.render_batch = int(render_batch)
         h = _vp()
         _check(_renderer_create(ctx.handle, scene.handle, ctypes.byref(s), ctypes.byref(h)), "gsr_renderer_create")
         self.handle, self.ctx, self.scene, self.target, self.bounds = h, ctx, scene, target, bounds
@@ -914,34 +916,41 @@ class ColourModel:
             pass
 
 
-def tune_batch(ctx: Context, make_renderer, cams, outs, candidates=(1, 2, 4, 8), replays: int = 5, timer=None):
-    """The batch's autotuner: renders the views with each branch count in `candidates` (a
-    captured graph, `replays` timed replays after one warm-up) and returns (best branch count,
-    {branches: ms per replay}). make_renderer() makes one renderer of the scene; timer(fn) times
-    fn on the context's stream and returns ms (hipEvents)."""
+def tune_batch(ctx: Context, make_renderer, cams, outs, candidates=(1, 2, 4, 8), replays: int = 5, timer=None,
+               tile_sizes=(16,), batches=(0,)):
+    """The batch's autotuner: for every (tile size, render batch, branch count) in tile_sizes x
+    batches x candidates, renders the views once (the renderers' buffers for every size), captures
+    the batch into a HIP graph and times `replays` replays after one warm-up; returns the fastest
+    configuration as {"tile_size", "render_batch", "branches"} and {(tile size, batch, branches): ms
+    per replay}. make_renderer(tile_size=, render_batch=) makes one renderer of the scene (batch 0 =
+    the default of the tile size); timer(fn) times fn on the context's stream and returns ms."""
     results = {}
-    pool = []
-    for b in candidates:
-        while len(pool) < b:
-            pool.append(make_renderer())
-        batch = Batch(ctx, b)
-        try:
-            batch.render(pool[:b], cams, outs)  # every size once per renderer: its buffers
-            ctx.synchronize()
-            batch.capture(pool[:b], cams, outs)
-            batch.replay()
-            ctx.synchronize()
+    for tile in tile_sizes:
+        for rb in batches:
+            pool = []
+            try:
+                for b in candidates:
+                    while len(pool) < b:
+                        pool.append(make_renderer(tile_size=tile, render_batch=rb))
+                    batch = Batch(ctx, b)
+                    try:
+                        batch.render(pool[:b], cams, outs)
+                        ctx.synchronize()
+                        batch.capture(pool[:b], cams, outs)
+                        batch.replay()
+                        ctx.synchronize()
 
-            def run():
-                for _ in range(replays):
-                    batch.replay()
-            results[b] = timer(run) / replays
-        finally:
-            batch.close()
+                        def run():
+                            for _ in range(replays):
+                                batch.replay()
+                        results[(tile, rb, b)] = timer(run) / replays
+                    finally:
+                        batch.close()
+            finally:
+                for r in pool:
+                    r.close()
     best = min(results, key=results.get)
-    for r in pool:
-        r.close()
-    return best, results
+    return {"tile_size": best[0], "render_batch": best[1], "branches": best[2]}, results
 
 
 NOALLOC_SORT_MAX_KEYS = 1 << 20  # the fixed scratch of gsr_sort_pairs_noalloc

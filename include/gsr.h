@@ -163,7 +163,11 @@ typedef struct gsr_settings {
                                     follow the bin grid ceil(w / 8) x ceil(h / 8)) and each bin is rendered by one
                                     wave. 8 takes the culled loop only (no StopThePop, differentiable, exact
                                     alpha or tile_stats) */
-    uint32_t reserved[2];
+    uint32_t render_batch;       /* TileSettings::renderBatch (splat_tile_renderer.h:110): splats staged in LDS per
+                                    round of the culled loop; 0 = one per thread (256 per 16-px tile, 64 per 8-px
+                                    bin). 16-px tiles: 128, 256 or 512; 8-px bins: 32, 64 or 128. Non-default
+                                    values need the culled loop (the autotuner's batch depth) */
+    uint32_t reserved[1];
 } gsr_settings;
 
 /* RenderSplatMode (splat_tile_renderer.h:88-92; render_splat.glsl:1-21), same values. The

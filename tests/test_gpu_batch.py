@@ -259,3 +259,53 @@ def test_batched_multiview_workload_32_views_1m(gsr_ctx):
     for o in outs:
         o.close()
     scene.close()
+
+
+@pytest.mark.gpu
+def test_autotuner_tile_size_and_batch_depth(gsr_ctx):
+    """tune_batch searches tile size x batch depth x branch count on a captured batch: every
+    configuration is timed, the best is one of them, and the batch captured in that configuration
+    renders every view exactly as a single renderer of that configuration."""
+    import time
+
+    import gsr
+    from gsr import scenes
+
+    log("case autotuner")
+    sc = scenes.make_scene(40_000, seed=2, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cams = _views(gsr, 6)
+    outs = [gsr.DeviceBuffer(gsr_ctx, int(c.width) * int(c.height) * 4) for c in cams]
+
+    def make(tile_size=16, render_batch=0):
+        return gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode="global", tile_size=tile_size,
+                            render_batch=render_batch)
+
+    def timer(fn):
+        gsr_ctx.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        gsr_ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    best, by = gsr.tune_batch(gsr_ctx, make, cams, [o.ptr for o in outs], candidates=(1, 2), replays=2, timer=timer,
+                              tile_sizes=(16, 8), batches=(0, 128))
+    assert len(by) == 8 and all(v > 0 for v in by.values())
+    assert (best["tile_size"], best["render_batch"], best["branches"]) in by
+    log(f"  best {best} of {len(by)}")
+    single = make(best["tile_size"], best["render_batch"])
+    want = [single.render_to_host(c).copy() for c in cams]
+    rs = [make(best["tile_size"], best["render_batch"]) for _ in range(best["branches"])]
+    batch = gsr.Batch(gsr_ctx, best["branches"])
+    batch.render(rs, cams, [o.ptr for o in outs])
+    batch.capture(rs, cams, [o.ptr for o in outs])
+    batch.replay()
+    gsr_ctx.synchronize()
+    for i, (o, c) in enumerate(zip(outs, cams)):
+        assert np.array_equal(_read(gsr, o, c), want[i])
+    batch.close()
+    for x in rs + [single]:
+        x.close()
+    for o in outs:
+        o.close()
+    scene.close()

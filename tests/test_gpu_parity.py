@@ -866,3 +866,29 @@ def test_tile_size_8(gsr_ctx, bounds, sort_mode, render_splat, case):
     r16.close()
     r.close()
     scene.close()
+
+
+@pytest.mark.parametrize("tile_size,batches", [(16, (128, 512)), (8, (32, 128))])
+def test_render_batch_depth_same_image(gsr_ctx, tile_size, batches):
+    """render_batch (TileSettings::renderBatch): the number of splats staged per round changes where
+    the rounds start and the whole-tile vote falls, not the per-pixel blend order, so every batch
+    depth renders the default's image bit for bit; other values are refused."""
+    import gsr
+    from gsr import camera, scenes
+
+    log(f"case render batch depths, tile size {tile_size}")
+    sc = scenes.make_scene(100_000, seed=0, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cam = _camera(gsr, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720)
+    base = gsr.Renderer(gsr_ctx, scene, target="rgba32f", sort_mode="global", tile_size=tile_size)
+    want = base.render_to_host(cam).copy()
+    for b in batches:
+        r = gsr.Renderer(gsr_ctx, scene, target="rgba32f", sort_mode="global", tile_size=tile_size, render_batch=b)
+        assert np.array_equal(r.render_to_host(cam), want), f"batch {b} changed the image"
+        r.close()
+    with pytest.raises(gsr.GsrError, match="render_batch"):
+        gsr.Renderer(gsr_ctx, scene, tile_size=tile_size, render_batch=96)
+    with pytest.raises(gsr.GsrError, match="render_batch"):
+        gsr.Renderer(gsr_ctx, scene, tile_size=16, render_batch=128, sort_window=4)
+    base.close()
+    scene.close()
