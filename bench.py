@@ -87,7 +87,9 @@ def multiview_bench(ctx, views=32, count=1_000_000, reps=5):
     """Batched multi-view: `views` cameras of mixed sizes ({640x360, 1280x720, 1920x1080}, uniform,
     seeded) on a shell of radius 8..12 around a 1M-gaussian scene of the same distributions,
     looking at the origin. Sequential = one renderer, one gsr_render per view on one stream;
-    batched = gsr_batch with the branch count the autotuner picks (1, 2, 4 or 8 streams), the
+    batched = gsr_batch in the configuration the autotuner picks (tile size 16 or 8, render batch
+    depth, 1, 2, 4 or 8 streams, stream or persistent mode: one renderer per view, every view's
+    frame up to the tile sort on the branches, then one render launch over all views' tiles), the
     whole batch captured into one HIP graph and replayed. Both: `reps` timed batches after a
     warm-up, host clock around them with the device synchronised."""
     import numpy as np
@@ -131,21 +133,29 @@ def multiview_bench(ctx, views=32, count=1_000_000, reps=5):
     seq_ms = timed(sequential) / reps
     seq.close()
     best, by = gsr.tune_batch(ctx, make, cams, [o.ptr for o in outs], candidates=(1, 2, 4, 8), replays=reps,
-                              timer=timed, tile_sizes=(16, 8), batches=(0, 128, 512))
-    best_ms = by[(best["tile_size"], best["render_batch"], best["branches"])]
-    default16 = {k[2]: v for k, v in by.items() if k[0] == 16 and k[1] == 0}
+                              timer=timed, tile_sizes=(16, 8), batches={16: (0, 128, 512), 8: (0, 32, 128)},
+                              persistent=(False, True))
+    best_ms = by[(best["tile_size"], best["render_batch"], best["branches"], best["persistent"])]
+    default16 = {k[2]: v for k, v in by.items() if k[0] == 16 and k[1] == 0 and not k[3]}
+    persistent = {k[2]: v for k, v in by.items() if k[3]}
+    pbest = min(persistent, key=persistent.get)
     for o in outs:
         o.close()
     scene.close()
     print(f"# multiview: {views} views, {pixels / 1e6:.1f} Mpixel: sequential {seq_ms:.3f} ms, batched "
           f"{best_ms:.3f} ms with {best} ({len(by)} configurations tuned; 16-px default batch: "
-          f"{', '.join(f'{k}: {v:.3f}' for k, v in default16.items())})", file=sys.stderr, flush=True)
+          f"{', '.join(f'{k}: {v:.3f}' for k, v in default16.items())}); persistent render "
+          f"{persistent[pbest]:.3f} ms with {pbest} branches "
+          f"({', '.join(f'{k}: {v:.3f}' for k, v in persistent.items())})", file=sys.stderr, flush=True)
     return {"views": views, "gaussians": count, "sizes": "uniform of 640x360, 1280x720, 1920x1080",
             "megapixels": round(pixels / 1e6, 2), "sequential_ms": round(seq_ms, 3),
             "sequential_mpix_per_s": round(pixels / 1e6 / (seq_ms * 1e-3), 1), "batched_ms": round(best_ms, 3),
             "batched_mpix_per_s": round(pixels / 1e6 / (best_ms * 1e-3), 1), "autotuned": best,
-            "batched_ms_by_config": {f"tile{k[0]}-batch{k[1] or 'default'}-branches{k[2]}": round(v, 3)
-                                     for k, v in sorted(by.items())},
+            "batched_ms_by_config": {f"tile{k[0]}-batch{k[1] or 'default'}-branches{k[2]}"
+                                     f"{'-persistent' if k[3] else ''}": round(v, 3) for k, v in sorted(by.items())},
+            "persistent_ms": round(persistent[pbest], 3), "persistent_branches": pbest,
+            "persistent_mpix_per_s": round(pixels / 1e6 / (persistent[pbest] * 1e-3), 1),
+            "persistent_ms_by_branches": {str(k): round(v, 3) for k, v in persistent.items()},
             "graph": True}
 
 

@@ -873,14 +873,17 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
     return s;
 }
 
-gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, void* streamArg) {
+// One frame on `stream`. deferred != null: everything but the render pass, whose tiles are
+// described in *deferred instead (a batch's persistent render draws them later).
+static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outImage, hipStream_t streamArg,
+                              TileJob* deferred) {
     if (!r || !cam || !outImage) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_render: null argument");
     if (cam->width == 0 || cam->height == 0 || cam->width > 65536 || cam->height > 65536)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_render: image size %ux%u", cam->width, cam->height);
     if (cam->tiles_x != (cam->width + 15) / 16 || cam->tiles_y != (cam->height + 15) / 16)
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_render: camera tiles do not match its size (use gsr_camera_make)");
     GSR_HIP(hipSetDevice(r->ctx->device));
-    hipStream_t stream = streamArg ? static_cast<hipStream_t>(streamArg) : r->ctx->stream;
+    hipStream_t stream = streamArg ? streamArg : r->ctx->stream;
     hipStreamCaptureStatus captureStatus = hipStreamCaptureStatusNone;
     GSR_HIP(hipStreamIsCapturing(stream, &captureStatus));
     const bool capturing = captureStatus != hipStreamCaptureStatusNone;
@@ -1070,7 +1073,10 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
-    if (r->settings.differentiable)
+    if (deferred)
+        *deferred = TileJob{r->splats.as<float4>(), r->packed.as<uint32_t>(), sortedValues, r->ranges.as<uint2>(),
+                            cam->width, cam->height, binsX, sc->count, outImage, nullptr};
+    else if (r->settings.differentiable)
         launchRenderTrain(r->splats.as<float4>(), sortedValues, r->ranges.as<uint2>(), dc,
                           static_cast<float4*>(outImage), r->pixelState.as<uint2>(), stream);
     else {
@@ -1113,6 +1119,10 @@ gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, vo
     r->lastStream = stream;
     r->eventsRecorded = prof;
     return GSR_OK;
+}
+
+gsr_status gsr_render(gsr_renderer* r, const gsr_camera* cam, void* outImage, void* streamArg) {
+    return renderFrame(r, cam, outImage, static_cast<hipStream_t>(streamArg), nullptr);
 }
 
 gsr_status gsr_read_control(gsr_renderer* r, gsr_control* out) {
@@ -1337,6 +1347,22 @@ struct gsr_batch {
         bool colourPass;
     };
     std::vector<Captured> captured;
+    // Persistent mode: one render launch over every view's tiles, after every view's sort. Its
+    // table (a TileJob per view, the views' first tiles, the work counter) goes to the device by
+    // a stream-ordered copy from pinned host memory: two tables alternate for eager batches (a
+    // table is refilled only once its previous copy has run), and a captured graph has its own,
+    // which stays as captured.
+    bool persistent = false;
+    uint32_t workgroups = 0;
+    struct Table {
+        DeviceBuffer dev;
+        void* host = nullptr;
+        size_t bytes = 0;
+        hipEvent_t copied = nullptr;
+        bool pending = false;
+    };
+    Table eager[2], capturedTable;
+    int nextEager = 0;
 };
 
 namespace {
@@ -1351,19 +1377,60 @@ void releaseGraph(gsr_batch* b) {
 
 bool colourPassOn(const gsr_renderer* r) { return r->colour && r->colour->mlpCount > 0; }
 
+// A renderer whose frames the persistent render can draw: the default culled loop on 16-px tiles.
+bool persistentReady(const gsr_renderer* r) {
+    const gsr_settings& s = r->settings;
+    return r->binSize == 16 && s.render_loop == GSR_RENDER_LOOP_CULLED && s.render_pixels_per_thread == 1 &&
+           !s.render_exact_alpha && s.sort_window == 0 && !s.differentiable && s.render_batch == 256 && !s.tile_stats;
+}
+
+hipError_t tableReserve(gsr_batch::Table& t, size_t bytes) {
+    if (t.bytes >= bytes) return hipSuccess;
+    if (t.pending) {
+        const hipError_t e = hipEventSynchronize(t.copied);
+        if (e != hipSuccess) return e;
+        t.pending = false;
+    }
+    if (t.host) (void)hipHostFree(t.host);
+    t.host = nullptr;
+    t.bytes = 0;
+    hipError_t e = hipHostMalloc(&t.host, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = t.dev.allocate(bytes);
+    if (e == hipSuccess && !t.copied) e = hipEventCreateWithFlags(&t.copied, hipEventDisableTiming);
+    if (e == hipSuccess) t.bytes = bytes;
+    return e;
+}
+
+void tableRelease(gsr_batch::Table& t) {
+    if (t.host) (void)hipHostFree(t.host);
+    if (t.copied) (void)hipEventDestroy(t.copied);
+    t.host = nullptr;
+    t.copied = nullptr;
+    t.dev.release();
+    t.bytes = 0;
+}
+
 gsr_status checkBatchArgs(gsr_batch* b, gsr_renderer* const* renderers, uint32_t viewCount, const gsr_camera* cams,
                           void* const* outImages, const char* what) {
     if (!b || !renderers || (viewCount && (!cams || !outImages)))
         return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: null argument", what);
-    const uint32_t used = std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
+    const uint32_t used = b->persistent ? viewCount : std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
     for (uint32_t i = 0; i < used; ++i) {
         if (!renderers[i]) return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: renderer %u is null", what, i);
+        if (b->persistent && !persistentReady(renderers[i]))
+            return fail(GSR_ERROR_UNSUPPORTED,
+                        "%s: renderer %u cannot join a persistent render (it needs the default culled loop on "
+                        "16-px tiles: no StopThePop, differentiable, exact alpha, tile_stats or other batch depth)",
+                        what, i);
+        if (b->persistent && (renderers[i]->settings.format != renderers[0]->settings.format ||
+                              renderers[i]->settings.render_splat != renderers[0]->settings.render_splat))
+            return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: a persistent render takes one image format and record", what);
         if (renderers[i]->ctx != b->ctx)
             return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: renderer %u belongs to another context", what, i);
         for (uint32_t j = 0; j < i; ++j)
             if (renderers[j] == renderers[i])
-                return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: renderers %u and %u are the same (one per branch)", what,
-                            j, i);
+                return fail(GSR_ERROR_INVALID_ARGUMENT, "%s: renderers %u and %u are the same (one per %s)", what, j, i,
+                            b->persistent ? "view in persistent mode" : "branch");
     }
     return GSR_OK;
 }
@@ -1373,18 +1440,65 @@ gsr_status batchRecord(gsr_batch* b, gsr_renderer* const* renderers, uint32_t vi
                        void* const* outImages, hipStream_t stream) {
     const uint32_t branches = std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
     if (branches == 0) return GSR_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    GSR_HIP(hipStreamIsCapturing(stream, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    gsr_batch::Table* table = nullptr;
+    if (b->persistent) {  // the table this call fills (an eager table waits for its previous copy)
+        table = capturing ? &b->capturedTable : &b->eager[b->nextEager];
+        if (!capturing) b->nextEager ^= 1;
+        const size_t bytes = size_t(viewCount) * sizeof(TileJob) + (size_t(viewCount) + 2) * 4;
+        if (capturing && table->bytes < bytes)
+            return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_capture: the persistent table was not reserved");
+        if (table->pending) {
+            GSR_HIP(hipEventSynchronize(table->copied));
+            table->pending = false;
+        }
+        GSR_HIP(tableReserve(*table, bytes));
+    }
+    std::vector<TileJob> jobs(b->persistent ? viewCount : 0);
     GSR_HIP(hipEventRecord(b->fork, stream));
     for (uint32_t i = 0; i < branches; ++i) GSR_HIP(hipStreamWaitEvent(b->streams[i], b->fork, 0));
     gsr_status st = GSR_OK;
     for (uint32_t v = 0; v < viewCount && st == GSR_OK; ++v)
-        st = gsr_render(renderers[v % branches], &cams[v], outImages[v], b->streams[v % branches]);
+        st = b->persistent ? renderFrame(renderers[v], &cams[v], outImages[v], b->streams[v % branches], &jobs[v])
+                           : gsr_render(renderers[v % branches], &cams[v], outImages[v], b->streams[v % branches]);
     const std::string err = st == GSR_OK ? std::string() : g_lastError;
     for (uint32_t i = 0; i < branches; ++i) {
         GSR_HIP(hipEventRecord(b->joins[i], b->streams[i]));
         GSR_HIP(hipStreamWaitEvent(stream, b->joins[i], 0));
     }
-    if (st != GSR_OK) g_lastError = err;
-    return st;
+    if (st != GSR_OK) {
+        g_lastError = err;
+        return st;
+    }
+    if (b->persistent) {
+        // Every view's lists are complete: the table, then one render over all views' tiles.
+        char* host = static_cast<char*>(table->host);
+        std::memcpy(host, jobs.data(), jobs.size() * sizeof(TileJob));
+        uint32_t* tileStart = reinterpret_cast<uint32_t*>(host + jobs.size() * sizeof(TileJob));
+        tileStart[0] = 0;
+        for (uint32_t v = 0; v < viewCount; ++v) tileStart[v + 1] = tileStart[v] + cams[v].tiles_x * cams[v].tiles_y;
+        const size_t bytes = jobs.size() * sizeof(TileJob) + (size_t(viewCount) + 2) * 4;
+        GSR_HIP(hipMemcpyAsync(table->dev.ptr, host, bytes, hipMemcpyHostToDevice, stream));
+        if (!capturing) {
+            GSR_HIP(hipEventRecord(table->copied, stream));
+            table->pending = true;
+        }
+        char* dev = table->dev.as<char>();
+        const uint32_t* dStart = reinterpret_cast<const uint32_t*>(dev + jobs.size() * sizeof(TileJob));
+        launchRenderViews(reinterpret_cast<const TileJob*>(dev), dStart, viewCount,
+                          std::min(b->workgroups, tileStart[viewCount]), const_cast<uint32_t*>(dStart) + viewCount + 1,
+                          renderers[0]->settings.format, renderers[0]->settings.render_splat, stream);
+        GSR_HIP(hipGetLastError());
+        // The views' renderers are in use until this render ends: later frames wait for it.
+        if (!capturing)
+            for (uint32_t v = 0; v < viewCount; ++v) {
+                GSR_HIP(hipEventRecord(renderers[v]->order, stream));
+                renderers[v]->lastStream = stream;
+            }
+    }
+    return GSR_OK;
 }
 
 }  // namespace
@@ -1410,6 +1524,9 @@ gsr_status gsr_batch_create(gsr_context* ctx, uint32_t branches, gsr_batch** out
         if (e == hipSuccess) b->joins.push_back(ev);
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->captureStream, hipStreamNonBlocking);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    b->workgroups = uint32_t(std::max(cus, 1)) * 8u;  // 8 workgroups of the culled render per CU
     if (e != hipSuccess) {
         gsr_batch_destroy(b);
         return hipFail(e, "gsr_batch_create");
@@ -1423,6 +1540,8 @@ void gsr_batch_destroy(gsr_batch* b) {
     (void)hipSetDevice(b->ctx->device);
     (void)hipDeviceSynchronize();
     releaseGraph(b);
+    for (gsr_batch::Table& t : b->eager) tableRelease(t);
+    tableRelease(b->capturedTable);
     for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
     for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
@@ -1431,6 +1550,17 @@ void gsr_batch_destroy(gsr_batch* b) {
 }
 
 uint32_t gsr_batch_branches(const gsr_batch* b) { return b ? uint32_t(b->streams.size()) : 0; }
+
+gsr_status gsr_batch_set_persistent(gsr_batch* b, int32_t persistent) {
+    if (!b) return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_batch_set_persistent: null batch");
+    if (b->persistent != (persistent != 0)) {
+        GSR_HIP(hipSetDevice(b->ctx->device));
+        GSR_HIP(hipDeviceSynchronize());
+        releaseGraph(b);  // a graph of the other mode
+        b->persistent = persistent != 0;
+    }
+    return GSR_OK;
+}
 
 gsr_status gsr_batch_render(gsr_batch* b, gsr_renderer* const* renderers, uint32_t viewCount, const gsr_camera* cams,
                             void* const* outImages, void* streamArg) {
@@ -1445,7 +1575,7 @@ gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint3
                              const gsr_camera* cams, void* const* outImages) {
     gsr_status st = checkBatchArgs(b, renderers, viewCount, cams, outImages, "gsr_batch_capture");
     if (st != GSR_OK) return st;
-    const uint32_t used = std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
+    const uint32_t used = b->persistent ? viewCount : std::min<uint32_t>(viewCount, uint32_t(b->streams.size()));
     for (uint32_t i = 0; i < used; ++i) {
         const gsr_renderer* r = renderers[i];
         if (r->statHost || r->settings.profile_stages || r->settings.tile_stats)
@@ -1472,7 +1602,10 @@ gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint3
                         "batch once (gsr_batch_render) before capturing it", v, cams[v].width, cams[v].height);
     }
     GSR_HIP(hipSetDevice(b->ctx->device));
+    GSR_HIP(hipDeviceSynchronize());  // replays of the previous graph may still run (and read its table)
     releaseGraph(b);
+    if (b->persistent)  // allocated here: a capture cannot allocate
+        GSR_HIP(tableReserve(b->capturedTable, size_t(viewCount) * sizeof(TileJob) + (size_t(viewCount) + 2) * 4));
     GSR_HIP(hipStreamBeginCapture(b->captureStream, hipStreamCaptureModeThreadLocal));
     st = batchRecord(b, renderers, viewCount, cams, outImages, b->captureStream);
     hipGraph_t graph = nullptr;

@@ -162,6 +162,21 @@ void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, 
                             const uint32_t* blockOffsets, const gsr_control* control, uint32_t maxVisible,
                             uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values,
                             gsr_control* controlRW, hipStream_t stream);
+// What the culled render of one frame's tiles reads and writes (16-px tiles, tilesX per row).
+struct TileJob {
+    const float4* splats;
+    const uint32_t* packed;
+    const uint32_t* values;
+    const uint2* ranges;
+    uint32_t width, height, tilesX, gaussianCount;
+    void* out;
+    uint32_t* tileStats;
+};
+// One persistent culled render over the tiles of `views` frames (jobs and tileStart on the
+// device: tileStart[v] = the tiles of views 0..v-1, tileStart[views] = all); `counter`: one device
+// word, zeroed by the launch itself.
+void launchRenderViews(const TileJob* jobs, const uint32_t* tileStart, uint32_t views, uint32_t workgroups,
+                       uint32_t* counter, gsr_target_format format, int renderSplat, hipStream_t stream);
 // sortWindow > 0: StopThePop with that window (rayDepth: 3 float4 per gaussian), reference alpha.
 // tileStats (culled loop only; null = none): 8 counters per tile, zeroed by the caller (the
 // reference's TileStat, splat_tile_renderer.h:130-141).

@@ -356,17 +356,6 @@ struct CulledBatch {
     __align__(16) uint32_t list[4][B + 4];
 };
 
-// What one tile of a frame reads and writes.
-struct TileJob {
-    const float4* splats;
-    const uint32_t* packed;
-    const uint32_t* values;
-    const uint2* ranges;
-    uint32_t width, height, tilesX, gaussianCount;
-    void* out;
-    uint32_t* tileStats;
-};
-
 // One 16x16 tile by the culled loop (the whole workgroup of 256 threads: it waits at barriers).
 template <int FORMAT, int RS, bool STATS, int B>
 __device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t tile, CulledBatch<B>& sh) {
@@ -505,6 +494,35 @@ __device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t ti
         }
     }
 }
+
+// The culled loop over the tiles of several frames (a batch of views) in one persistent launch:
+// each workgroup takes (view, tile) items from a device counter until all are done; item i
+// belongs to the view v with tileStart[v] <= i < tileStart[v + 1]. Every view's lists must be
+// complete (the launch follows every view's sort).
+template <int FORMAT, int RS>
+__global__ __launch_bounds__(256, 8) void kRenderViews(const TileJob* __restrict__ jobs,
+                                                    const uint32_t* __restrict__ tileStart, uint32_t views,
+                                                    uint32_t* __restrict__ counter) {
+    __shared__ CulledBatch<kBatch> sh;
+    __shared__ uint32_t sItem;
+    const uint32_t total = tileStart[views];
+    for (;;) {
+        if (threadIdx.x == 0) sItem = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t item = sItem;
+        if (item >= total) return;  // uniform: every item is taken
+        uint32_t lo = 0, hi = views;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tileStart[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        renderTileCulled<FORMAT, RS, false, kBatch>(jobs[lo], item - tileStart[lo], sh);
+        __syncthreads();  // every thread has read sItem and the batch before they are reused
+    }
+}
+
+__global__ void kZeroCounter(uint32_t* counter) { *counter = 0u; }
 
 // The culled loop over a frame's tiles: workgroup b takes tile rows of XCD b % 8 centre-out.
 template <int FORMAT, int RS, bool STATS, int B>
@@ -1083,6 +1101,25 @@ void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow,
 }
 
 }  // namespace
+
+void launchRenderViews(const TileJob* jobs, const uint32_t* tileStart, uint32_t views, uint32_t workgroups,
+                       uint32_t* counter, gsr_target_format format, int renderSplat, hipStream_t stream) {
+    if (views == 0 || workgroups == 0) return;
+    hipLaunchKernelGGL(kZeroCounter, dim3(1), dim3(1), 0, stream, counter);
+    markKernel("kZeroCounter", stream);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(workgroups), dim3(256), 0, stream, jobs, tileStart, views, counter);
+    };
+    const bool f8 = format == GSR_TARGET_RGBA8;
+    if (renderSplat == GSR_RENDER_SPLAT_P32)
+        f8 ? go(kRenderViews<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_P32>) : go(kRenderViews<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_P32>);
+    else if (renderSplat == GSR_RENDER_SPLAT_P24)
+        f8 ? go(kRenderViews<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_P24>) : go(kRenderViews<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_P24>);
+    else
+        f8 ? go(kRenderViews<GSR_TARGET_RGBA8, GSR_RENDER_SPLAT_FULL>)
+           : go(kRenderViews<GSR_TARGET_RGBA32F, GSR_RENDER_SPLAT_FULL>);
+    markKernel("kRenderViews", stream);
+}
 
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,

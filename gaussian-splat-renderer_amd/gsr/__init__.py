@@ -239,6 +239,7 @@ _batch_branches = _sig("gsr_batch_branches", _u32, [_vp])
 _batch_render = _sig("gsr_batch_render", _i32, [_vp, _vp, _u32, _vp, _vp, _vp])
 _batch_capture = _sig("gsr_batch_capture", _i32, [_vp, _vp, _u32, _vp, _vp])
 _batch_replay = _sig("gsr_batch_replay", _i32, [_vp, _vp])
+_batch_set_persistent = _sig("gsr_batch_set_persistent", _i32, [_vp, _i32])
 _cm_param_count = _sig("gsr_colour_model_param_count", _u32, [])
 _cm_create = _sig("gsr_colour_model_create", _i32, [_vp, _vp, _u32, ctypes.POINTER(_vp)])
 _cm_destroy = _sig("gsr_colour_model_destroy", None, [_vp])
@@ -270,6 +271,7 @@ EXPORTED = (
     "gsr_read_control gsr_read_lists gsr_read_splats gsr_read_packed_splats gsr_read_stage_times gsr_read_tile_stats gsr_read_kernel_times gsr_sort_pairs "
     "gsr_sort_pairs_noalloc gsr_exclusive_scan "
     "gsr_batch_create gsr_batch_destroy gsr_batch_branches gsr_batch_render gsr_batch_capture gsr_batch_replay "
+    "gsr_batch_set_persistent "
     "gsr_colour_model_param_count gsr_colour_model_create gsr_colour_model_destroy gsr_colour_model_train "
     "gsr_colour_model_loss gsr_colour_model_gradients gsr_colour_model_get gsr_colour_model_set "
     "gsr_colour_model_evaluate gsr_colour_model_fallback gsr_renderer_set_colour_model gsr_backward "
@@ -782,13 +784,24 @@ class Stream:
 class Batch:
     """Many views of one scene on `branches` streams at once (gsr_batch_*): view v is rendered by
     renderers[v % branches] on branch stream v % branches, forked from and joined back to the
-    caller's stream; capture() records the batch into a HIP graph, replay() launches it."""
+    caller's stream; capture() records the batch into a HIP graph, replay() launches it.
+    persistent=True: one renderer per view, the views' frames run up to the tile sort on the
+    branches and one render launch draws every view's tiles from a shared work counter."""
 
-    def __init__(self, ctx: Context, branches: int):
+    def __init__(self, ctx: Context, branches: int, persistent: bool = False):
         h = _vp()
         _check(_batch_create(ctx.handle, int(branches), ctypes.byref(h)), "gsr_batch_create")
         self.handle, self.ctx, self.branches = h, ctx, int(branches)
         self._captured = None  # keeps the captured arrays and objects alive
+        self.persistent = False
+        if persistent:
+            self.set_persistent(True)
+
+    def set_persistent(self, on: bool) -> None:
+        _check(_batch_set_persistent(self.handle, int(bool(on))), "gsr_batch_set_persistent")
+        if bool(on) != self.persistent:
+            self._captured = None
+        self.persistent = bool(on)
 
     @staticmethod
     def _arrays(renderers, cams, outs):
@@ -917,40 +930,47 @@ class ColourModel:
 
 
 def tune_batch(ctx: Context, make_renderer, cams, outs, candidates=(1, 2, 4, 8), replays: int = 5, timer=None,
-               tile_sizes=(16,), batches=(0,)):
-    """The batch's autotuner: for every (tile size, render batch, branch count) in tile_sizes x
-    batches x candidates, renders the views once (the renderers' buffers for every size), captures
-    the batch into a HIP graph and times `replays` replays after one warm-up; returns the fastest
-    configuration as {"tile_size", "render_batch", "branches"} and {(tile size, batch, branches): ms
-    per replay}. make_renderer(tile_size=, render_batch=) makes one renderer of the scene (batch 0 =
-    the default of the tile size); timer(fn) times fn on the context's stream and returns ms."""
+               tile_sizes=(16,), batches=(0,), persistent=(False,)):
+    """The batch's autotuner: for every (tile size, render batch, branch count, persistent mode) in
+    tile_sizes x batches x candidates x persistent, renders the views once (the renderers' buffers
+    for every size), captures the batch into a HIP graph and times `replays` replays after one
+    warm-up; returns the fastest configuration as {"tile_size", "render_batch", "branches",
+    "persistent"} and {(tile size, batch, branches, persistent): ms per replay}.
+    make_renderer(tile_size=, render_batch=) makes one renderer of the scene (batch 0 = the default
+    of the tile size); batches is one tuple for every tile size or a dict {tile size: tuple}. The
+    persistent mode takes one renderer per view and the default culled loop, so it is tried for
+    16-px tiles and the default batch only. timer(fn) times fn on the context's stream, in ms."""
     results = {}
     for tile in tile_sizes:
-        for rb in batches:
-            pool = []
-            try:
-                for b in candidates:
-                    while len(pool) < b:
-                        pool.append(make_renderer(tile_size=tile, render_batch=rb))
-                    batch = Batch(ctx, b)
-                    try:
-                        batch.render(pool[:b], cams, outs)
-                        ctx.synchronize()
-                        batch.capture(pool[:b], cams, outs)
-                        batch.replay()
-                        ctx.synchronize()
+        for rb in (batches[tile] if isinstance(batches, dict) else batches):
+            for pers in persistent:
+                if pers and (tile != 16 or rb not in (0, 256)):
+                    continue
+                pool = []
+                try:
+                    for b in candidates:
+                        while len(pool) < (len(cams) if pers else b):
+                            pool.append(make_renderer(tile_size=tile, render_batch=rb))
+                        use = pool if pers else pool[:b]
+                        batch = Batch(ctx, b, persistent=pers)
+                        try:
+                            batch.render(use, cams, outs)
+                            ctx.synchronize()
+                            batch.capture(use, cams, outs)
+                            batch.replay()
+                            ctx.synchronize()
 
-                        def run():
-                            for _ in range(replays):
-                                batch.replay()
-                        results[(tile, rb, b)] = timer(run) / replays
-                    finally:
-                        batch.close()
-            finally:
-                for r in pool:
-                    r.close()
+                            def run():
+                                for _ in range(replays):
+                                    batch.replay()
+                            results[(tile, rb, b, bool(pers))] = timer(run) / replays
+                        finally:
+                            batch.close()
+                finally:
+                    for r in pool:
+                        r.close()
     best = min(results, key=results.get)
-    return {"tile_size": best[0], "render_batch": best[1], "branches": best[2]}, results
+    return {"tile_size": best[0], "render_batch": best[1], "branches": best[2], "persistent": best[3]}, results
 
 
 NOALLOC_SORT_MAX_KEYS = 1 << 20  # the fixed scratch of gsr_sort_pairs_noalloc

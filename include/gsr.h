@@ -356,6 +356,15 @@ typedef struct gsr_batch gsr_batch;
 gsr_status gsr_batch_create(gsr_context* ctx, uint32_t branches, gsr_batch** out);
 void gsr_batch_destroy(gsr_batch* batch);
 uint32_t gsr_batch_branches(const gsr_batch* batch);
+/* Persistent mode (persistent != 0; off by default): every view gets its own renderer
+ * (renderers[0 .. view_count-1] distinct), runs its frame up to the tile depth sort on branch
+ * v % b, and after the join ONE render launch (about 8 workgroups per CU) draws all views' tiles,
+ * each workgroup taking the next (view, tile) from a device counter, with the same tile body as
+ * the default culled loop, so the images equal those of stream mode. Renderers must use that
+ * loop: 16-px tiles, one pixel per thread, no exact alpha, StopThePop, differentiable,
+ * tile_stats or non-default render_batch (GSR_ERROR_UNSUPPORTED), one image format and
+ * render_splat for all views. Switching the mode waits for the device and drops a captured graph. */
+gsr_status gsr_batch_set_persistent(gsr_batch* batch, int32_t persistent);
 gsr_status gsr_batch_render(gsr_batch* batch, gsr_renderer* const* renderers, uint32_t view_count,
                             const gsr_camera* cams, void* const* out_images, void* stream);
 gsr_status gsr_batch_capture(gsr_batch* batch, gsr_renderer* const* renderers, uint32_t view_count,

@@ -74,6 +74,108 @@ def test_batch_and_graph_equal_single_renders(gsr_ctx, branches, sort_mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("branches,sort_mode,target,splat",
+                         [(1, "global", "rgba8", "p32"), (4, "global", "rgba32f", "p24"),
+                          (3, "binning", "rgba8", "full")])
+def test_persistent_batch_equals_single_renders(gsr_ctx, branches, sort_mode, target, splat):
+    """Persistent mode: the views' frames up to the tile sort on the branches, then one render
+    over every view's tiles from a shared counter. Same tile body as the per-frame culled render:
+    every image bit-equal to the view's own gsr_render, eagerly (twice in a row, which reuses the
+    two alternating job tables) and in graph replays with eager work between them."""
+    import gsr
+    from gsr import scenes
+
+    log(f"case persistent branches={branches} sort_mode={sort_mode} target={target} splat={splat}")
+    sc = scenes.make_scene(40_000, seed=3, mean_sigma=2.0)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cams = _views(gsr, 9)
+    px = 16 if target == "rgba32f" else 4
+    outs = [gsr.DeviceBuffer(gsr_ctx, int(c.width) * int(c.height) * px) for c in cams]
+
+    def read(o, c):
+        if target == "rgba32f":
+            return o.read(np.empty((int(c.height), int(c.width), 4), np.float32))
+        return _read(gsr, o, c)
+
+    def make():
+        return gsr.Renderer(gsr_ctx, scene, bounds="tight", target=target, sort_mode=sort_mode, render_splat=splat)
+
+    single = make()
+    want = [single.render_to_host(c) for c in cams]
+    renderers = [make() for _ in cams]
+    batch = gsr.Batch(gsr_ctx, branches, persistent=True)
+    for rep in range(2):
+        for o in outs:
+            o.write(np.zeros(o.nbytes, np.uint8))
+        batch.render(renderers, cams, [o.ptr for o in outs])
+        gsr_ctx.synchronize()
+        for i, (o, c) in enumerate(zip(outs, cams)):
+            assert np.array_equal(read(o, c), want[i]), f"eager {rep}: persistent view {i} ({c.width}x{c.height})"
+
+    batch.capture(renderers, cams, [o.ptr for o in outs])
+    for rep in range(3):
+        for o in outs:
+            o.write(np.zeros(o.nbytes, np.uint8))
+        batch.replay()
+        gsr_ctx.synchronize()
+        for i, (o, c) in enumerate(zip(outs, cams)):
+            assert np.array_equal(read(o, c), want[i]), f"replay {rep}: persistent view {i} differs"
+        # eager frames of the captured renderers between the replays (on the context's stream)
+        img = renderers[rep].render_to_host(cams[rep])
+        assert np.array_equal(img, want[rep])
+    log(f"  {len(cams)} views: 2 eager persistent batches and 3 replays equal the single renders")
+
+    # back to stream mode on the same batch: the captured graph is dropped, images unchanged
+    batch.set_persistent(False)
+    batch.render(renderers[:branches], cams, [o.ptr for o in outs])
+    gsr_ctx.synchronize()
+    for i, (o, c) in enumerate(zip(outs, cams)):
+        assert np.array_equal(read(o, c), want[i]), f"stream mode after persistent: view {i}"
+    with pytest.raises(gsr.GsrError):
+        batch.replay()
+    batch.close()
+    for r in renderers + [single]:
+        r.close()
+    for o in outs:
+        o.close()
+    scene.close()
+
+
+@pytest.mark.gpu
+def test_persistent_batch_arguments(gsr_ctx):
+    import gsr
+    from gsr import scenes
+
+    log("case persistent batch arguments")
+    sc = scenes.make_scene(5_000, seed=4)
+    scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
+    cams = _views(gsr, 3)
+    outs = [gsr.DeviceBuffer(gsr_ctx, int(c.width) * int(c.height) * 16) for c in cams]
+    ptrs = [o.ptr for o in outs]
+    batch = gsr.Batch(gsr_ctx, 2, persistent=True)
+    plain = [gsr.Renderer(gsr_ctx, scene, target="rgba8") for _ in cams]
+    with pytest.raises(gsr.GsrError, match="same"):  # one renderer per view, not per branch
+        batch.render([plain[0], plain[1], plain[0]], cams, ptrs)
+    stp = gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_window=8)
+    with pytest.raises(gsr.GsrError, match="persistent"):
+        batch.render([plain[0], stp, plain[2]], cams, ptrs)
+    f32 = gsr.Renderer(gsr_ctx, scene, target="rgba32f")
+    with pytest.raises(gsr.GsrError, match="format"):
+        batch.render([plain[0], f32, plain[2]], cams, ptrs)
+    bins8 = gsr.Renderer(gsr_ctx, scene, target="rgba8", tile_size=8)
+    with pytest.raises(gsr.GsrError, match="persistent"):
+        batch.render([plain[0], plain[1], bins8], cams, ptrs)
+    batch.render(plain, cams, ptrs)  # valid after the refusals
+    gsr_ctx.synchronize()
+    for r in plain + [stp, f32, bins8]:
+        r.close()
+    batch.close()
+    for o in outs:
+        o.close()
+    scene.close()
+
+
+@pytest.mark.gpu
 def test_batch_arguments(gsr_ctx):
     import gsr
     from gsr import scenes
@@ -263,7 +365,7 @@ def test_batched_multiview_workload_32_views_1m(gsr_ctx):
 
 @pytest.mark.gpu
 def test_autotuner_tile_size_and_batch_depth(gsr_ctx):
-    """tune_batch searches tile size x batch depth x branch count on a captured batch: every
+    """tune_batch searches tile size x batch depth x branch count x persistent mode on a captured batch: every
     configuration is timed, the best is one of them, and the batch captured in that configuration
     renders every view exactly as a single renderer of that configuration."""
     import time
@@ -289,14 +391,17 @@ def test_autotuner_tile_size_and_batch_depth(gsr_ctx):
         return (time.perf_counter() - t0) * 1e3
 
     best, by = gsr.tune_batch(gsr_ctx, make, cams, [o.ptr for o in outs], candidates=(1, 2), replays=2, timer=timer,
-                              tile_sizes=(16, 8), batches=(0, 128))
-    assert len(by) == 8 and all(v > 0 for v in by.values())
-    assert (best["tile_size"], best["render_batch"], best["branches"]) in by
+                              tile_sizes=(16, 8), batches={16: (0, 512), 8: (0, 32)}, persistent=(False, True))
+    # 2 tile sizes x 2 batches x 2 branch counts in stream mode, + 16-px default batch persistent
+    assert len(by) == 10 and all(v > 0 for v in by.values())
+    assert sum(k[3] for k in by) == 2 and all(k[:2] == (16, 0) for k in by if k[3])
+    assert (best["tile_size"], best["render_batch"], best["branches"], best["persistent"]) in by
     log(f"  best {best} of {len(by)}")
     single = make(best["tile_size"], best["render_batch"])
     want = [single.render_to_host(c).copy() for c in cams]
-    rs = [make(best["tile_size"], best["render_batch"]) for _ in range(best["branches"])]
-    batch = gsr.Batch(gsr_ctx, best["branches"])
+    rs = [make(best["tile_size"], best["render_batch"])
+          for _ in range(len(cams) if best["persistent"] else best["branches"])]
+    batch = gsr.Batch(gsr_ctx, best["branches"], persistent=best["persistent"])
     batch.render(rs, cams, [o.ptr for o in outs])
     batch.capture(rs, cams, [o.ptr for o in outs])
     batch.replay()
