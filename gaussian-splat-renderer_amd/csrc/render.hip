@@ -120,24 +120,30 @@ __device__ __forceinline__ float evalQ(const float4& s0, const float2& s1, const
     return fmaf(s0.x, p.X2, fmaf(s0.y, p.XY, fmaf(s0.z, p.Y2, fmaf(s0.w, p.X, fmaf(s1.x, p.Y, s1.y)))));
 }
 
-// One splat into one pixel, branch-free: the lane's state is its transmittance T, colour and
-// cutoff thr (log2(1/255) while alive, +inf once done or outside the image). A pair below the
-// cutoff changes nothing (aEff = 0); the splat that would take T below 1e-4 is not blended and
-// ends the pixel (test-before-accept). The weight is T - T', which is exactly 0 in both cases.
+// One splat into one pixel: the lane's state is its transmittance T, colour and cutoff thr
+// (log2(1/255) while alive, +inf once done or outside the image). A pair below the cutoff changes
+// nothing (a = 0, so w = 0 and T' = T); otherwise the splat is blended with the reference's weight
+// w = alpha T and T' = T - w, unless T' < 1e-4: then it is not blended and ends the pixel
+// (test-before-accept). Termination is rare, so its selects run only when some lane of the wave
+// ends at this splat; a finished lane keeps T >= 1e-4 and never ends again.
 __device__ __forceinline__ void blendPredicated(float q, float4 col, float& T, float& c0, float& c1, float& c2,
                                                 float& thr) {
-    const float a = fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q));
-    const float aEff = q >= thr ? a : 0.0f;
-    const float testT = T * (1.0f - aEff);
+    const float a = q >= thr ? fminf(kMaxAlpha, __builtin_amdgcn_exp2f(q)) : 0.0f;
+    float w = a * T;
+    float testT = T - w;
     const bool term = testT < kMinTransmittance;
-    const float Tn = term ? T : testT;
-    const float w = T - Tn;
+    if (__builtin_amdgcn_ballot_w64(term) != 0) {  // rare: some lane of the wave ends here
+        __asm__ volatile("" ::: "memory");  // keeps this a branch: the compiler would make it selects
+        w = term ? 0.0f : w;
+        testT = term ? T : testT;
+        thr = term ? __builtin_inff() : thr;
+    }
     c0 = fmaf(col.x, w, c0);
     c1 = fmaf(col.y, w, c1);
     c2 = fmaf(col.z, w, c2);
-    T = Tn;
-    thr = term ? __builtin_inff() : thr;
+    T = testT;
 }
+
 
 __device__ __forceinline__ uint32_t quantize8(float v) {
     v = v > 0.0f ? v : 0.0f;  // also NaN -> 0
