@@ -198,7 +198,7 @@ def learned_bench(ctx, cams, frames, count=3_000_000, samples=1 << 24):
     step_samples = 1 << 15
     ctx.synchronize()
     t0 = time.perf_counter()
-    last = model.train(samples // step_samples, step_samples, lr_weights=2e-3, lr_latents=2e-2)
+    last = model.train(samples // step_samples, step_samples, lr_weights=4e-3, lr_latents=1e-2)
     train_ms = (time.perf_counter() - t0) * 1e3
     mse = model.loss(0x5EED, 1 << 20)
     r = gsr.Renderer(ctx, scene, bounds="tight", target="rgba8", sort_mode="global")

@@ -124,13 +124,15 @@ uint32_t bitsFor(uint32_t v) {  // bits needed to hold values 0..v
 // device array. Every GPU the library runs on holds its own copy until the process ends. In host
 // code the array's name is a host-side stand-in, not a device address: host code uses only the
 // address hipGetSymbolAddress gives for the current device (resolved once per device at
-// gsr_context_create). Nothing in it must start at zero: the histogram pass writes every
-// (digit, block) count and the scan every digit total before they are read.
+// gsr_context_create). Nothing in it must start at zero: the histogram pass writes the digit
+// counts of every block that holds keys and the scan every digit total before they are read.
 namespace {
 constexpr uint32_t kNoAllocSortMaxKeys = 1u << 20;
 constexpr size_t kNoAllocValuesOffset = size_t(kNoAllocSortMaxKeys) * 8;
 constexpr size_t kNoAllocWorkOffset = size_t(kNoAllocSortMaxKeys) * 12;
-constexpr size_t kNoAllocWorkWords = size_t(kMaxSortPasses) * kRadix + kMaxSortPasses + 1 + size_t(kRadix) * kMaxSortBlocks;
+constexpr size_t kNoAllocSortBlocks = (kNoAllocSortMaxKeys + kSortTile - 1) / kSortTile;  // the grid for that count
+constexpr size_t kNoAllocWorkWords =
+    size_t(kMaxSortPasses) * kRadix + kMaxSortPasses + 1 + size_t(kRadix) * kNoAllocSortBlocks;
 constexpr size_t kNoAllocScratchBytes = kNoAllocWorkOffset + kNoAllocWorkWords * 4;
 }  // namespace
 static __device__ __align__(256) uint64_t scratch[(kNoAllocScratchBytes + 7) / 8];
@@ -189,6 +191,7 @@ struct gsr_colour_model {
     DeviceBuffer params, gradParams, mParams, vParams, packs;
     DeviceBuffer latents, gradLatents, mLatents, vLatents, latentsBf16;
     DeviceBuffer useMlp, errors, sse;
+    DeviceBuffer partials;    // the training pass's per-workgroup weight-gradient rows (sized on use)
     uint32_t step = 0;        // Adam steps since creation or the last set
     uint32_t streamIndex = 0; // training sample streams used
     uint32_t mlpCount = 0;    // gaussians marked by the last fallback pass
@@ -1684,6 +1687,107 @@ hipError_t modelRefresh(gsr_colour_model* m, hipStream_t stream) {
     return e;
 }
 
+// The per-workgroup gradient rows of a training pass of `samples` (grown on use; the caller has
+// drained the device).
+hipError_t reservePartials(gsr_colour_model* m, uint32_t samples) {
+    const size_t bytes = size_t(mlpTrainBlocks(samples)) * size_t(mlpParamCount()) * 4;
+    return m->partials.bytes >= bytes ? hipSuccess : m->partials.allocate(bytes);
+}
+
+// Eigen-decomposition of the symmetric n x n matrix a (row-major; destroyed) by cyclic Jacobi
+// rotations: eigenvalues in w, eigenvectors in the columns of v.
+void symmetricEigen(std::vector<double>& a, int n, std::vector<double>& w, std::vector<double>& v) {
+    v.assign(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) v[size_t(i) * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, diag = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) (i == j ? diag : off) += a[size_t(i) * n + j] * a[size_t(i) * n + j];
+        if (off <= 1e-24 * (diag + 1e-300)) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a[size_t(p) * n + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double theta = (a[size_t(q) * n + q] - a[size_t(p) * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {  // columns p, q
+                    const double akp = a[size_t(k) * n + p], akq = a[size_t(k) * n + q];
+                    a[size_t(k) * n + p] = c * akp - s * akq;
+                    a[size_t(k) * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // rows p, q
+                    const double apk = a[size_t(p) * n + k], aqk = a[size_t(q) * n + k];
+                    a[size_t(p) * n + k] = c * apk - s * aqk;
+                    a[size_t(q) * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = v[size_t(k) * n + p], vkq = v[size_t(k) * n + q];
+                    v[size_t(k) * n + p] = c * vkp - s * vkq;
+                    v[size_t(k) * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize(size_t(n));
+    for (int i = 0; i < n; ++i) w[size_t(i)] = a[size_t(i) * n + i];
+}
+
+// The latents' start: every gaussian's SH values (D = 3 (degree + 1)^2 of them) projected on the
+// scene's leading principal components (up to 16; each scaled to standard deviation 0.5, the
+// others 0), so that a latent starts out carrying its gaussian's colour data and training only
+// has to shape the shared network around it.
+hipError_t initLatentsPca(gsr_colour_model* m, hipStream_t stream) {
+    const SceneRef ref = modelScene(m);
+    const int D = 3 * (ref.degree + 1) * (ref.degree + 1), pairs = D * (D + 1) / 2;
+    const uint32_t blocks = shMomentBlocks(m->count);
+    DeviceBuffer part, projMu;
+    hipError_t e = part.allocate(size_t(blocks) * size_t(D + pairs) * 4);
+    if (e != hipSuccess) return e;
+    launchShMoments(ref, part.as<float>(), stream);
+    std::vector<float> rows(size_t(blocks) * size_t(D + pairs));
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(rows.data(), part.ptr, rows.size() * 4, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    std::vector<double> mean(size_t(D), 0.0), cov(size_t(D) * D, 0.0);
+    for (uint32_t b = 0; b < blocks; ++b) {
+        const float* r = rows.data() + size_t(b) * size_t(D + pairs);
+        for (int i = 0; i < D; ++i) mean[size_t(i)] += r[i];
+        int p = D;
+        for (int i = 0; i < D; ++i)
+            for (int j = i; j < D; ++j) cov[size_t(i) * D + j] += r[p++];
+    }
+    const double n = double(m->count);
+    for (int i = 0; i < D; ++i) mean[size_t(i)] /= n;
+    for (int i = 0; i < D; ++i)
+        for (int j = i; j < D; ++j) {
+            const double c = cov[size_t(i) * D + j] / n - mean[size_t(i)] * mean[size_t(j)];
+            cov[size_t(i) * D + j] = cov[size_t(j) * D + i] = c;
+        }
+    std::vector<double> w, v;
+    symmetricEigen(cov, D, w, v);
+    std::vector<int> order(static_cast<size_t>(D));
+    for (int i = 0; i < D; ++i) order[size_t(i)] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[size_t(a)] > w[size_t(b)]; });
+    std::vector<float> pm(16 * 48 + 48, 0.0f);
+    const double top = D > 0 ? std::max(w[size_t(order[0])], 0.0) : 0.0;
+    for (int r = 0; r < std::min(16, D); ++r) {
+        const double lam = w[size_t(order[size_t(r)])];
+        // Components below 1e-6 of the largest are the moments' fp32 rounding: left at 0.
+        if (!(lam > 1e-6 * top) || !(lam > 0.0)) continue;
+        const double s = 0.5 / std::sqrt(lam);
+        for (int f = 0; f < D; ++f) pm[size_t(r) * 48 + f] = float(v[size_t(f) * D + order[size_t(r)]] * s);
+    }
+    for (int f = 0; f < D; ++f) pm[16 * 48 + size_t(f)] = float(mean[size_t(f)]);
+    e = projMu.allocate(pm.size() * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(projMu.ptr, pm.data(), pm.size() * 4, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+    launchInitLatentsPca(ref, projMu.as<float>(), m->latents.as<float>(), stream);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);  // the temporary buffers are freed on return
+    return e;
+}
+
 uint32_t trainSeed(const gsr_colour_model* m, uint32_t index) {
     uint32_t x = m->seed ^ (index * 0x9E3779B9u + 0x632BE5ABu);
     x ^= x >> 16;
@@ -1727,7 +1831,7 @@ gsr_status gsr_colour_model_create(gsr_context* ctx, const gsr_scene* scene, uin
     if (e == hipSuccess) e = m->sse.allocate(4);
     if (e != hipSuccess) return hipFail(e, "gsr_colour_model_create: allocation");
     // He-normal weights (fan-in 19 for layer 1: latent + direction; 64 for layers 2 and 3, the
-    // last scaled by 0.5), zero biases; latents uniform in [-0.5, 0.5).
+    // last scaled by 0.5), zero biases; latents from the SH's principal components (initLatentsPca).
     std::vector<float> p(P, 0.0f);
     uint64_t state = 0x853C49E6748FEA9Bull ^ (uint64_t(seed) << 1);
     auto uniform = [&]() {
@@ -1744,7 +1848,7 @@ gsr_status gsr_colour_model_create(gsr_context* ctx, const gsr_scene* scene, uin
     hipStream_t stream = ctx->stream;
     if ((e = hipMemcpy(m->params.ptr, p.data(), P * 4, hipMemcpyHostToDevice)) != hipSuccess)
         return hipFail(e, "gsr_colour_model_create: upload");
-    launchInitLatents(m->latents.as<float>(), uint32_t(L), seed, 0.5f, stream);
+    if ((e = initLatentsPca(m.get(), stream)) != hipSuccess) return hipFail(e, "gsr_colour_model_create: latents");
     if ((e = modelRefresh(m.get(), stream)) != hipSuccess) return hipFail(e, "gsr_colour_model_create: init");
     GSR_HIP(hipStreamSynchronize(stream));
     *out = m.release();
@@ -1767,19 +1871,27 @@ gsr_status gsr_colour_model_train(gsr_colour_model* m, uint32_t steps, uint32_t 
     if (!(lrWeights >= 0.0f) || !(lrLatents >= 0.0f))
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_train: negative or NaN learning rate");
     GSR_HIP(modelBegin(m));
+    GSR_HIP(reservePartials(m, samples));
     hipStream_t stream = m->ctx->stream;
     const SceneRef ref = modelScene(m);
     const uint32_t P = uint32_t(mlpParamCount()), L = m->count * 16;
+    // A step of at most `count` samples touches distinct gaussians: their latents take their Adam
+    // step inside the training pass (lazy Adam: untouched latents and moments keep their values).
+    // Larger steps (small scenes) add the latent gradients up and take a dense Adam step.
+    const bool lazy = samples <= m->count;
     for (uint32_t i = 0; i < steps; ++i) {
         ++m->step;
         GSR_HIP(hipMemsetAsync(m->sse.ptr, 0, 4, stream));
         launchMlpTrain(ref, m->params.as<float>(), m->packs.as<uint2>(), m->latents.as<float>(),
-                       m->gradParams.as<float>(), m->gradLatents.as<float>(), m->sse.as<float>(),
-                       trainSeed(m, m->streamIndex++), samples, true, stream);
+                       m->gradParams.as<float>(), m->partials.as<float>(), m->gradLatents.as<float>(),
+                       m->mLatents.as<float>(), m->vLatents.as<float>(), lrLatents, m->step, m->sse.as<float>(),
+                       sampleStream(trainSeed(m, m->streamIndex++), m->count), samples,
+                       lazy ? kTrainStep : kTrainGrad, stream);
         launchAdam(m->params.as<float>(), m->gradParams.as<float>(), m->mParams.as<float>(), m->vParams.as<float>(), P,
                    lrWeights, m->step, stream);
-        launchAdam(m->latents.as<float>(), m->gradLatents.as<float>(), m->mLatents.as<float>(),
-                   m->vLatents.as<float>(), L, lrLatents, m->step, stream);
+        if (!lazy)
+            launchAdam(m->latents.as<float>(), m->gradLatents.as<float>(), m->mLatents.as<float>(),
+                       m->vLatents.as<float>(), L, lrLatents, m->step, stream);
         launchMlpPack(m->params.as<float>(), m->packs.as<uint2>(), stream);
         GSR_HIP(hipGetLastError());
     }
@@ -1799,7 +1911,8 @@ gsr_status gsr_colour_model_loss(gsr_colour_model* m, uint32_t seed, uint32_t sa
     hipStream_t stream = m->ctx->stream;
     GSR_HIP(hipMemsetAsync(m->sse.ptr, 0, 4, stream));
     launchMlpTrain(modelScene(m), m->params.as<float>(), m->packs.as<uint2>(), m->latents.as<float>(), nullptr, nullptr,
-                   m->sse.as<float>(), seed, samples, false, stream);
+                   nullptr, nullptr, nullptr, 0.0f, 1u, m->sse.as<float>(), sampleStream(seed, m->count), samples,
+                   kTrainLoss, stream);
     GSR_HIP(hipGetLastError());
     float sse = 0.0f;
     GSR_HIP(hipMemcpyAsync(&sse, m->sse.ptr, 4, hipMemcpyDeviceToHost, stream));
@@ -1815,15 +1928,17 @@ gsr_status gsr_colour_model_gradients(gsr_colour_model* m, uint32_t seed, uint32
         return fail(GSR_ERROR_INVALID_ARGUMENT, "gsr_colour_model_gradients: samples %u (a positive multiple of 256)",
                     samples);
     GSR_HIP(modelBegin(m));
+    GSR_HIP(reservePartials(m, samples));
     hipStream_t stream = m->ctx->stream;
     const size_t P = size_t(mlpParamCount()), L = size_t(m->count) * 16;
-    // The gradient buffers are zero between training steps (Adam clears them); clear them anyway.
-    GSR_HIP(hipMemsetAsync(m->gradParams.ptr, 0, P * 4, stream));
+    // The latent gradients are zero between training steps (Adam clears them); clear them anyway.
+    // The weight gradient is written whole by the pass.
     GSR_HIP(hipMemsetAsync(m->gradLatents.ptr, 0, L * 4, stream));
     GSR_HIP(hipMemsetAsync(m->sse.ptr, 0, 4, stream));
+    const SampleStream st = sampleStream(seed, m->count);
     launchMlpTrain(modelScene(m), m->params.as<float>(), m->packs.as<uint2>(), m->latents.as<float>(),
-                   m->gradParams.as<float>(), m->gradLatents.as<float>(), m->sse.as<float>(), seed, samples, true,
-                   stream);
+                   m->gradParams.as<float>(), m->partials.as<float>(), m->gradLatents.as<float>(), nullptr, nullptr,
+                   0.0f, 1u, m->sse.as<float>(), st, samples, kTrainGrad, stream);
     GSR_HIP(hipGetLastError());
     if (gradParams) GSR_HIP(hipMemcpyAsync(gradParams, m->gradParams.ptr, P * 4, hipMemcpyDeviceToHost, stream));
     if (gradLatents) GSR_HIP(hipMemcpyAsync(gradLatents, m->gradLatents.ptr, L * 4, hipMemcpyDeviceToHost, stream));
@@ -1833,7 +1948,7 @@ gsr_status gsr_colour_model_gradients(gsr_colour_model* m, uint32_t seed, uint32
         DeviceBuffer g, d;
         GSR_HIP(g.allocate(size_t(samples) * 4));
         GSR_HIP(d.allocate(size_t(samples) * 12));
-        launchMlpSamples(seed, 0, samples, m->count, g.as<uint32_t>(), d.as<float>(), stream);
+        launchMlpSamples(st, 0, samples, g.as<uint32_t>(), d.as<float>(), stream);
         GSR_HIP(hipGetLastError());
         if (gaussians) GSR_HIP(hipMemcpyAsync(gaussians, g.ptr, size_t(samples) * 4, hipMemcpyDeviceToHost, stream));
         if (dirs) GSR_HIP(hipMemcpyAsync(dirs, d.ptr, size_t(samples) * 12, hipMemcpyDeviceToHost, stream));

@@ -80,7 +80,7 @@ constexpr uint32_t kMaxSortPasses = 8;                            // 64-bit keys
 // global digit histograms (passes x 256), per-pass tile tickets, look-back status words
 // (tiles x 256, never cleared: epoch-tagged) and a timeout counter (0 unless a look-back
 // wait gave up).
-constexpr uint32_t kMaxSortBlocks = 1024;                         // reduce-then-scan grid
+constexpr uint32_t kMaxSortBlocks = 4096;                         // reduce-then-scan grid: one tile per block up to 8.4M keys
 enum SortAlgorithm : int32_t { kSortAlgReduceThenScan = 0, kSortAlgOnesweep = 1 };
 struct SortWork {
     int32_t algorithm;
@@ -88,7 +88,7 @@ struct SortWork {
     uint32_t* tickets;     // one-sweep
     uint64_t* status;      // one-sweep
     uint32_t* timeouts;    // one-sweep
-    uint32_t* blockHist;   // reduce-then-scan: 256 x kMaxSortBlocks
+    uint32_t* blockHist;   // reduce-then-scan: 2^bits x blocks counts, blocks <= kMaxSortBlocks
 };
 size_t sortStatusBytes(uint32_t maxCount);
 
@@ -192,16 +192,34 @@ void launchRender(const float4* splats, const uint32_t* packed, int renderSplat,
 int mlpParamCount();
 int mlpPackedTiles();
 void launchMlpPack(const float* params, uint2* packs, hipStream_t stream);
-// Samples 0 .. samples - 1 (samples % 16 == 0) of stream `seed`: adds the gradient of the mean
-// squared error into gradParams / gradLatents (backward) and the squared-error sum into *sse.
-void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, const float* latents,
-                    float* gradParams, float* gradLatents, float* sse, uint32_t seed, uint32_t samples, bool backward,
-                    hipStream_t stream);
+// A training sample stream: sample s is gaussian (mulA s + addB) mod count (mulA coprime to count:
+// samples 0 .. count - 1 are distinct gaussians) and a direction hashed from (seed, s).
+struct SampleStream {
+    uint32_t seed, mulA, addB, count;
+};
+SampleStream sampleStream(uint32_t seed, uint32_t count);
+enum MlpTrainMode : int { kTrainLoss = 0, kTrainGrad = 1, kTrainStep = 2 };
+// The workgroups of a training pass over `samples` samples (the rows of its `partials`, at most 512).
+uint32_t mlpTrainBlocks(uint32_t samples);
+// Samples 0 .. samples - 1 (samples % 16 == 0) of stream st: the squared-error sum into *sse and,
+// unless kTrainLoss, the weight gradient of the mean squared error into gradParams (written, via
+// per-workgroup rows in partials: mlpTrainBlocks(samples) x mlpParamCount() floats). The latent
+// gradient: kTrainGrad adds it into gradLatents; kTrainStep (samples <= count: distinct gaussians)
+// applies Adam step `step` (lrLat, moments mLat / vLat) to the touched latents in place.
+void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, float* latents, float* gradParams,
+                    float* partials, float* gradLatents, float* mLat, float* vLat, float lrLat, uint32_t step,
+                    float* sse, const SampleStream& st, uint32_t samples, int mode, hipStream_t stream);
 // One Adam step (step >= 1) on n values; clears the gradients.
 void launchAdam(float* p, float* grad, float* m, float* v, uint32_t n, float lr, uint32_t step, hipStream_t stream);
 void launchToBf16(const float* in, uint16_t* out, uint32_t n, hipStream_t stream);
 void launchInitLatents(float* out, uint32_t n, uint32_t seed, float scale, hipStream_t stream);
-void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count, uint32_t* gaussians, float* dirs,
+// Principal-component latents: launchShMoments writes shMomentBlocks(count) rows of D SH sums and
+// D (D + 1) / 2 product sums (D = 3 (degree + 1)^2 <= 48); launchInitLatentsPca sets latent r of
+// every gaussian to projMu[r * 48 ..] . (x - mu), mu = projMu[16 * 48 ..] (48 floats each).
+uint32_t shMomentBlocks(uint32_t count);
+void launchShMoments(const SceneRef& scene, float* partials, hipStream_t stream);
+void launchInitLatentsPca(const SceneRef& scene, const float* projMu, float* latents, hipStream_t stream);
+void launchMlpSamples(const SampleStream& st, uint32_t first, uint32_t n, uint32_t* gaussians, float* dirs,
                       hipStream_t stream);
 void launchMlpFallback(const SceneRef& scene, const float* params, const uint2* packs, const uint16_t* latentsBf16,
                        float threshold, uint8_t* useMlp, float* err, hipStream_t stream);

@@ -43,6 +43,7 @@ constexpr int kTB1 = 0, kTB2 = 4, kTB3 = 12, kTBCount = 14;
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kTestDirs = 32;  // fallback test directions per gaussian
+constexpr int kParamCount = 64 * 32 + 64 * 64 + 64 + 3 * 64 + 3;
 
 // fp32 -> bf16, round to nearest even (gfx950's v_cvt_pk_bf16_f32, two values per instruction).
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -255,10 +256,12 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {  // a 32-bit integer has
 }
 __device__ __forceinline__ float unit01(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
 
-// Sample s of a stream: a gaussian and a direction uniform on the sphere.
-__device__ __forceinline__ void sampleOf(uint32_t seed, uint32_t s, uint32_t count, uint32_t* g, float dir[3]) {
-    const uint32_t h0 = mix32(s * 0x9E3779B9u ^ seed), h1 = mix32(h0 ^ 0x85EBCA6Bu), h2 = mix32(h1 ^ 0xC2B2AE35u);
-    *g = uint32_t((uint64_t(h0) * count) >> 32);
+// Sample s of a stream: a gaussian and a direction uniform on the sphere. The gaussians follow the
+// stream's permutation g = (a s + b) mod count (a coprime to count), so samples 0 .. count - 1 of a
+// stream are distinct gaussians: a training step of at most count samples touches each once.
+__device__ __forceinline__ void sampleOf(const SampleStream& st, uint32_t s, uint32_t* g, float dir[3]) {
+    const uint32_t h0 = mix32(s * 0x9E3779B9u ^ st.seed), h1 = mix32(h0 ^ 0x85EBCA6Bu), h2 = mix32(h1 ^ 0xC2B2AE35u);
+    *g = uint32_t((uint64_t(st.mulA) * (s % st.count) + st.addB) % st.count);
     const float z = 2.0f * unit01(h1) - 1.0f, phi = 6.28318530718f * unit01(h2);
     const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
     dir[0] = r * cosf(phi);
@@ -304,18 +307,29 @@ __device__ __forceinline__ void inputTiles(float4 latQuad, const float dir[3], u
     x1[3] = first ? 1.0f : 0.0f;
 }
 
-// Training pass over samples 0 .. total - 1 of stream `seed`, `perWave` consecutive ones (a
-// multiple of 16) per wave: accumulates the gradient of mean((y + 0.5 - sh)^2) over the total x 3
-// values into gradParams / gradLatents and the summed squared error into *sse. BACKWARD = false:
-// only the error.
-template <int FMT, bool BACKWARD>
+// Training pass over samples 0 .. total - 1 of a stream, `perWave` consecutive ones (a multiple of
+// 16) per wave, for the loss mean((y + 0.5 - sh)^2) over the total x 3 values; the summed squared
+// error into *sse. MODE kTrainLoss: only that. Otherwise the weight gradient: the workgroup's
+// waves add theirs in LDS and the workgroup writes its sum to partials[blockIdx.x] (P floats; no
+// global atomics; kReduceGrad adds the rows). The latent gradient: kTrainGrad adds it into
+// gradLatents (atomics: a gaussian may occur in several samples); kTrainStep (every gaussian of
+// the pass distinct: total <= count) applies lazy Adam to each sample's latent in place instead
+// (m, v and the latent of the touched gaussians only, bias corrections bc1, bc2 of the step).
+template <int FMT, int MODE>
 __global__ __launch_bounds__(kThreads) void kTrain(SceneRef scene, const float* __restrict__ params,
-                                                   const uint2* __restrict__ packs, const float* __restrict__ latents,
-                                                   float* __restrict__ gradParams, float* __restrict__ gradLatents,
-                                                   float* __restrict__ sse, uint32_t seed, uint32_t perWave,
-                                                   uint32_t total) {
+                                                   const uint2* __restrict__ packs, float* latents,
+                                                   float* __restrict__ partials, float* __restrict__ gradLatents,
+                                                   float* __restrict__ mLat, float* __restrict__ vLat, float lrLat,
+                                                   float bc1, float bc2, float* __restrict__ sse, SampleStream st,
+                                                   uint32_t perWave, uint32_t total) {
+    constexpr bool BACKWARD = MODE != kTrainLoss;
     __shared__ float sT[kWaves][16 * 17];
+    __shared__ float sGrad[BACKWARD ? kParamCount : 1];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if constexpr (BACKWARD) {
+        for (uint32_t i = threadIdx.x; i < uint32_t(kParamCount); i += kThreads) sGrad[i] = 0.0f;
+        __syncthreads();
+    }
     float* lds = sT[wave];
     const uint32_t rq = 4 * (lane / 16u), col = lane % 16u;
     const float scale = 2.0f / (3.0f * float(total));
@@ -337,9 +351,10 @@ __global__ __launch_bounds__(kThreads) void kTrain(SceneRef scene, const float* 
     for (uint32_t t0 = 0; t0 < perWave && base + t0 < total; t0 += 16) {  // whole tiles: total % 16 == 0
         uint32_t g;
         float dir[3];
-        sampleOf(seed, base + t0 + col, scene.count, &g, dir);
+        sampleOf(st, base + t0 + col, &g, dir);
         f32x4 x0, x1;
-        inputTiles(*reinterpret_cast<const float4*>(latents + size_t(g) * 16 + rq), dir, lane, x0, x1);
+        const float4 lat = *reinterpret_cast<const float4*>(latents + size_t(g) * 16 + rq);
+        inputTiles(lat, dir, lane, x0, x1);
         Forward f;
         forward(packs, params, x0, x1, lane, f);
         f32x4 dy = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -410,29 +425,46 @@ __global__ __launch_bounds__(kThreads) void kTrain(SceneRef scene, const float* 
         f32x4 dl = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int ki = 0; ki < 4; ++ki) dl = mfma(tileA(packs, kTA1T + ki, lane), bDH1[ki], dl);
+        if constexpr (MODE == kTrainGrad) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) atomicAdd(&gradLatents[size_t(g) * 16 + rq + j], dl[j]);
+            for (int j = 0; j < 4; ++j) atomicAdd(&gradLatents[size_t(g) * 16 + rq + j], dl[j]);
+        } else {  // lazy Adam on this sample's latent entries (the only sample of gaussian g in the pass)
+            constexpr float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+            const size_t li = size_t(g) * 16 + rq;
+            float4 m4 = *reinterpret_cast<const float4*>(mLat + li), v4 = *reinterpret_cast<const float4*>(vLat + li);
+            float mv[4] = {m4.x, m4.y, m4.z, m4.w}, vv[4] = {v4.x, v4.y, v4.z, v4.w};
+            float pv[4] = {lat.x, lat.y, lat.z, lat.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                mv[j] = b1 * mv[j] + (1.0f - b1) * dl[j];
+                vv[j] = b2 * vv[j] + (1.0f - b2) * dl[j] * dl[j];
+                pv[j] -= lrLat * (mv[j] / bc1) / (sqrtf(vv[j] / bc2) + eps);
+            }
+            *reinterpret_cast<float4*>(mLat + li) = make_float4(mv[0], mv[1], mv[2], mv[3]);
+            *reinterpret_cast<float4*>(vLat + li) = make_float4(vv[0], vv[1], vv[2], vv[3]);
+            *reinterpret_cast<float4*>(latents + li) = make_float4(pv[0], pv[1], pv[2], pv[3]);
+        }
     }
-    // The wave's sums: one atomic per parameter entry it holds.
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) err += __shfl_xor(err, off, 64);
     if (lane == 0) atomicAdd(sse, err);
-    if (!BACKWARD) return;
+    if constexpr (!BACKWARD) return;
+    // The workgroup's weight gradient: every wave adds its entries into LDS.
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int row = 16 * mi + int(rq) + j;
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni) atomicAdd(&gradParams[kW1 + row * 32 + 16 * ni + int(col)], dW1[mi][ni][j]);
+            for (int ni = 0; ni < 2; ++ni) atomicAdd(&sGrad[kW1 + row * 32 + 16 * ni + int(col)], dW1[mi][ni][j]);
 #pragma unroll
-            for (int ni = 0; ni < 4; ++ni) atomicAdd(&gradParams[kW2 + row * 64 + 16 * ni + int(col)], dW2[mi][ni][j]);
+            for (int ni = 0; ni < 4; ++ni) atomicAdd(&sGrad[kW2 + row * 64 + 16 * ni + int(col)], dW2[mi][ni][j]);
         }
 #pragma unroll
     for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            if (int(rq) + j < 3) atomicAdd(&gradParams[kW3 + (int(rq) + j) * 64 + 16 * ni + int(col)], dW3[ni][j]);
+            if (int(rq) + j < 3) atomicAdd(&sGrad[kW3 + (int(rq) + j) * 64 + 16 * ni + int(col)], dW3[ni][j]);
     // Bias sums over the 16 sample columns (lanes with the same l / 16).
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
@@ -441,15 +473,28 @@ __global__ __launch_bounds__(kThreads) void kTrain(SceneRef scene, const float* 
             float v = db2[mi][j];
 #pragma unroll
             for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (col == 0u) atomicAdd(&gradParams[kB2 + 16 * mi + int(rq) + j], v);
+            if (col == 0u) atomicAdd(&sGrad[kB2 + 16 * mi + int(rq) + j], v);
         }
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         float v = db3[j];
 #pragma unroll
         for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if (lane == 0u) atomicAdd(&gradParams[kB3 + j], v);
+        if (lane == 0u) atomicAdd(&sGrad[kB3 + j], v);
     }
+    __syncthreads();
+    float* row = partials + size_t(blockIdx.x) * kParamCount;
+    for (uint32_t i = threadIdx.x; i < uint32_t(kParamCount); i += kThreads) row[i] = sGrad[i];
+}
+
+// grad[i] = the sum of the workgroups' rows partials[b][i], b < rows.
+__global__ __launch_bounds__(256) void kReduceGrad(const float* __restrict__ partials, uint32_t rows,
+                                                   float* __restrict__ grad) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= uint32_t(kParamCount)) return;
+    float s = 0.0f;
+    for (uint32_t b = 0; b < rows; ++b) s += partials[size_t(b) * kParamCount + i];
+    grad[i] = s;
 }
 
 // Adam on n values; the gradients are cleared for the next step.
@@ -477,14 +522,101 @@ __global__ __launch_bounds__(256) void kInitLatents(float* __restrict__ out, uin
         out[i] = scale * (2.0f * unit01(mix32(i * 0x9E3779B9u ^ mix32(seed))) - 1.0f);
 }
 
+// Latent initialisation from the scene's SH (principal components). kShMoments: per workgroup,
+// the sums of the D = 3 (degree + 1)^2 SH values x (feature f = 3 k + c, the stored order) and of
+// the products x_i x_j (i <= j, row-major upper triangle) over its gaussians, staged 64 gaussians
+// at a time in LDS; row b of `partials` = D sums, then D (D + 1) / 2 product sums.
+constexpr int kMomentRows = 64;
+constexpr int kMaxShValues = 48;
+template <int FMT>
+__global__ __launch_bounds__(256) void kShMoments(SceneRef scene, float* __restrict__ partials) {
+    __shared__ float sX[kMomentRows][kMaxShValues + 1];
+    const int used = (scene.degree + 1) * (scene.degree + 1), D = used * 3, pairs = D * (D + 1) / 2;
+    const uint32_t t = threadIdx.x;
+    constexpr int kPer = (kMaxShValues * (kMaxShValues + 1) / 2 + 255) / 256;  // 5 pairs per thread
+    int pi[kPer], pj[kPer];
+    float acc[kPer], sum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        int p = int(t) + q * 256, i = 0;
+        const int pp = p;
+        while (p >= D - i && i < D) {  // row i of the upper triangle holds D - i pairs
+            p -= D - i;
+            ++i;
+        }
+        pi[q] = pp < pairs ? i : 0;
+        pj[q] = pp < pairs ? i + p : 0;
+        acc[q] = 0.0f;
+    }
+    for (uint32_t g0 = blockIdx.x * kMomentRows; g0 < scene.count; g0 += gridDim.x * kMomentRows) {
+        if (t < uint32_t(kMomentRows)) {
+            const uint32_t g = g0 + t;
+            float co[3][16];
+            if (g < scene.count) {
+                loadSh<FMT>(scene, g, used, co);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) co[0][k] = co[1][k] = co[2][k] = 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    if (3 * k + c < kMaxShValues) sX[t][3 * k + c] = co[c][k];
+        }
+        __syncthreads();
+        for (int r = 0; r < kMomentRows; ++r) {
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) acc[q] = fmaf(sX[r][pi[q]], sX[r][pj[q]], acc[q]);
+            if (int(t) < D) sum += sX[r][t];
+        }
+        __syncthreads();
+    }
+    float* row = partials + size_t(blockIdx.x) * size_t(D + pairs);
+    if (int(t) < D) row[t] = sum;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q)
+        if (int(t) + q * 256 < pairs) row[D + int(t) + q * 256] = acc[q];
+}
+
+// latents[g][r] = proj[r] . (x_g - mu) for the 16 rows of proj (D values each, stored 48 apart).
+template <int FMT>
+__global__ __launch_bounds__(256) void kInitLatentsPca(SceneRef scene, const float* __restrict__ projMu,
+                                                       float* __restrict__ latents) {
+    __shared__ float sP[16 * kMaxShValues + kMaxShValues];
+    for (uint32_t i = threadIdx.x; i < 16 * kMaxShValues + kMaxShValues; i += 256) sP[i] = projMu[i];
+    __syncthreads();
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= scene.count) return;
+    const int used = (scene.degree + 1) * (scene.degree + 1);
+    float co[3][16];
+    loadSh<FMT>(scene, g, used, co);
+    float x[kMaxShValues];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[3 * k + c] = co[c][k] - sP[16 * kMaxShValues + 3 * k + c];
+    float out[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        float a = 0.0f;
+#pragma unroll
+        for (int f = 0; f < kMaxShValues; ++f) a = fmaf(sP[r * kMaxShValues + f], x[f], a);
+        out[r] = a;
+    }
+    float4* dst = reinterpret_cast<float4*>(latents + size_t(g) * 16);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+}
+
 // The (gaussian, direction) pairs of samples first .. first + n - 1 of stream `seed`.
-__global__ __launch_bounds__(256) void kSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count,
+__global__ __launch_bounds__(256) void kSamples(SampleStream st, uint32_t first, uint32_t n,
                                                 uint32_t* __restrict__ gaussians, float* __restrict__ dirs) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     uint32_t g;
     float dir[3];
-    sampleOf(seed, first + i, count, &g, dir);
+    sampleOf(st, first + i, &g, dir);
     gaussians[i] = g;
     dirs[3 * size_t(i)] = dir[0];
     dirs[3 * size_t(i) + 1] = dir[1];
@@ -650,7 +782,37 @@ uint32_t gridFor(uint32_t n, uint32_t perBlock, uint32_t cap) {
 
 }  // namespace
 
-int mlpParamCount() { return kB3 + 3; }
+static_assert(kParamCount == kB3 + 3, "the parameter block");
+int mlpParamCount() { return kParamCount; }
+
+SampleStream sampleStream(uint32_t seed, uint32_t count) {
+    // a: a hash of the seed reduced to [1, count), moved up to the next value coprime to count
+    // (a permutation of 0 .. count - 1); b: another hash.
+    auto h = [](uint64_t x) {
+        x ^= x >> 33;
+        x *= 0xFF51AFD7ED558CCDull;
+        x ^= x >> 33;
+        x *= 0xC4CEB9FE1A85EC53ull;
+        x ^= x >> 33;
+        return x;
+    };
+    SampleStream st{seed, 1u, 0u, count ? count : 1u};
+    if (count > 1) {
+        uint32_t a = uint32_t(h(uint64_t(seed) * 2 + 1) % (count - 1)) + 1u;
+        auto gcd = [](uint32_t x, uint32_t y) {
+            while (y) {
+                const uint32_t t = x % y;
+                x = y;
+                y = t;
+            }
+            return x;
+        };
+        while (gcd(a, count) != 1u) a = a + 1u < count ? a + 1u : 1u;
+        st.mulA = a;
+        st.addB = uint32_t(h(uint64_t(seed) * 2 + 2) % count);
+    }
+    return st;
+}
 int mlpPackedTiles() { return kTACount + 2 * kTBCount; }  // units of 64 x uint2
 
 void launchMlpPack(const float* params, uint2* packs, hipStream_t stream) {
@@ -658,34 +820,41 @@ void launchMlpPack(const float* params, uint2* packs, hipStream_t stream) {
     markKernel("kPack", stream);
 }
 
-void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, const float* latents,
-                    float* gradParams, float* gradLatents, float* sse, uint32_t seed, uint32_t samples, bool backward,
-                    hipStream_t stream) {
-    // Each wave takes perWave consecutive samples (a multiple of 16, at least 64); at most 1024
-    // waves, so that the per-wave weight-gradient atomics (6.4 K per wave) stay few. samples is a
-    // multiple of 16.
+uint32_t mlpTrainBlocks(uint32_t samples) {
+    const uint32_t tiles = samples / 16;
+    const uint32_t tilesPerWave = std::max(2u, (tiles + 2047) / 2048);
+    const uint32_t waves = (tiles + tilesPerWave - 1) / tilesPerWave;
+    return (waves + kWaves - 1) / kWaves;
+}
+
+void launchMlpTrain(const SceneRef& scene, const float* params, const uint2* packs, float* latents, float* gradParams,
+                    float* partials, float* gradLatents, float* mLat, float* vLat, float lrLat, uint32_t step,
+                    float* sse, const SampleStream& st, uint32_t samples, int mode, hipStream_t stream) {
+    // Each wave takes perWave consecutive samples (a multiple of 16, at least 32); at most 2048
+    // waves (512 workgroups, the rows of `partials`). samples is a multiple of 16.
     const uint32_t tiles = samples / 16;
     if (tiles == 0 || scene.count == 0) return;
-    const uint32_t tilesPerWave = std::max(4u, (tiles + 1023) / 1024);
-    const uint32_t waves = (tiles + tilesPerWave - 1) / tilesPerWave;
-    const uint32_t blocks = (waves + kWaves - 1) / kWaves;
+    const uint32_t tilesPerWave = std::max(2u, (tiles + 2047) / 2048);
+    const uint32_t blocks = mlpTrainBlocks(samples);
     const uint32_t perWave = 16 * tilesPerWave;
-    if (scene.format == GSR_SCENE_F32_SOA) {
-        if (backward)
-            hipLaunchKernelGGL((kTrain<GSR_SCENE_F32_SOA, true>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
-                               packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
-        else
-            hipLaunchKernelGGL((kTrain<GSR_SCENE_F32_SOA, false>), dim3(blocks), dim3(kThreads), 0, stream, scene,
-                               params, packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
-            markKernel("kTrain", stream);
-    } else {
-        if (backward)
-            hipLaunchKernelGGL((kTrain<GSR_SCENE_F16, true>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
-                               packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
-        else
-            hipLaunchKernelGGL((kTrain<GSR_SCENE_F16, false>), dim3(blocks), dim3(kThreads), 0, stream, scene, params,
-                               packs, latents, gradParams, gradLatents, sse, seed, perWave, samples);
-            markKernel("kTrain", stream);
+    const float bc1 = 1.0f - powf(0.9f, float(step)), bc2 = 1.0f - powf(0.999f, float(step));
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kThreads), 0, stream, scene, params, packs, latents, partials,
+                           gradLatents, mLat, vLat, lrLat, bc1, bc2, sse, st, perWave, samples);
+        markKernel("kTrain", stream);
+    };
+    auto pick = [&](auto fmt) {
+        constexpr int FMT = decltype(fmt)::value;
+        if (mode == kTrainLoss) go(kTrain<FMT, kTrainLoss>);
+        else if (mode == kTrainGrad) go(kTrain<FMT, kTrainGrad>);
+        else go(kTrain<FMT, kTrainStep>);
+    };
+    if (scene.format == GSR_SCENE_F32_SOA) pick(std::integral_constant<int, GSR_SCENE_F32_SOA>{});
+    else pick(std::integral_constant<int, GSR_SCENE_F16>{});
+    if (mode != kTrainLoss) {
+        hipLaunchKernelGGL(kReduceGrad, dim3((kParamCount + 255) / 256), dim3(256), 0, stream, partials, blocks,
+                           gradParams);
+        markKernel("kReduceGrad", stream);
     }
 }
 
@@ -704,6 +873,29 @@ void launchToBf16(const float* in, uint16_t* out, uint32_t n, hipStream_t stream
     markKernel("kToBf16", stream);
 }
 
+uint32_t shMomentBlocks(uint32_t count) { return std::min<uint32_t>(512u, (count + kMomentRows - 1) / kMomentRows); }
+
+void launchShMoments(const SceneRef& scene, float* partials, hipStream_t stream) {
+    const uint32_t blocks = shMomentBlocks(scene.count);
+    if (blocks == 0) return;
+    if (scene.format == GSR_SCENE_F32_SOA)
+        hipLaunchKernelGGL(kShMoments<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(256), 0, stream, scene, partials);
+    else
+        hipLaunchKernelGGL(kShMoments<GSR_SCENE_F16>, dim3(blocks), dim3(256), 0, stream, scene, partials);
+    markKernel("kShMoments", stream);
+}
+
+void launchInitLatentsPca(const SceneRef& scene, const float* projMu, float* latents, hipStream_t stream) {
+    const uint32_t blocks = (scene.count + 255) / 256;
+    if (blocks == 0) return;
+    if (scene.format == GSR_SCENE_F32_SOA)
+        hipLaunchKernelGGL(kInitLatentsPca<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(256), 0, stream, scene, projMu,
+                           latents);
+    else
+        hipLaunchKernelGGL(kInitLatentsPca<GSR_SCENE_F16>, dim3(blocks), dim3(256), 0, stream, scene, projMu, latents);
+    markKernel("kInitLatentsPca", stream);
+}
+
 void launchInitLatents(float* out, uint32_t n, uint32_t seed, float scale, hipStream_t stream) {
     const uint32_t blocks = gridFor(n, 256, 4096);
     if (blocks == 0) return;
@@ -711,11 +903,11 @@ void launchInitLatents(float* out, uint32_t n, uint32_t seed, float scale, hipSt
     markKernel("kInitLatents", stream);
 }
 
-void launchMlpSamples(uint32_t seed, uint32_t first, uint32_t n, uint32_t count, uint32_t* gaussians, float* dirs,
+void launchMlpSamples(const SampleStream& st, uint32_t first, uint32_t n, uint32_t* gaussians, float* dirs,
                       hipStream_t stream) {
     const uint32_t blocks = (n + 255) / 256;
     if (blocks == 0) return;
-    hipLaunchKernelGGL(kSamples, dim3(blocks), dim3(256), 0, stream, seed, first, n, count, gaussians, dirs);
+    hipLaunchKernelGGL(kSamples, dim3(blocks), dim3(256), 0, stream, st, first, n, gaussians, dirs);
     markKernel("kSamples", stream);
 }
 

@@ -8,7 +8,7 @@
 // Reduce-then-scan (the default, GSR_SORT_REDUCE_THEN_SCAN): per digit of at most 8 bits (a
 // key's bits split evenly over the passes) three kernels:
 //   - kSortHistogram: each block counts the digits of its contiguous chunk of keys in LDS and
-//     writes one count per (digit, block);
+//     writes one count per (digit, block) for the 2^bits digits of the pass;
 //   - kSortScan: one workgroup per digit scans that digit's block counts;
 //   - kSortScatter: each wave64 ranks its keys with one 64-lane ballot per digit bit (the
 //     lanes holding the same digit) and a running per-wave digit counter in LDS, so ranks
@@ -37,6 +37,19 @@ __device__ __forceinline__ uint32_t sortCount(const uint32_t* countDevice, uint3
     if (!countDevice) return maxCount;
     const uint32_t c = *countDevice;
     return c < maxCount ? c : maxCount;
+}
+
+// The reduce-then-scan grid of G blocks: block b sorts keys [b chunk, b chunk + chunk), one tile
+// of kSortTile keys while the count allows (chunk > kSortTile only past G tiles); `used` blocks
+// hold keys. Every kernel of a pass derives the same values from the count on the device, so the
+// grid can be sized for the capacity while the work follows the count.
+struct SortChunks {
+    uint32_t chunk, used;
+};
+__device__ __forceinline__ SortChunks sortChunks(uint32_t count, uint32_t G) {
+    uint32_t chunk = (count + G - 1) / G;
+    chunk = chunk < uint32_t(kSortTile) ? uint32_t(kSortTile) : chunk;
+    return SortChunks{chunk, (count + chunk - 1) / chunk};
 }
 
 template <typename Key>
@@ -236,10 +249,10 @@ __global__ __launch_bounds__(kSortThreads) void kSortHistogram(const Key* __rest
     for (uint32_t d = t; d < kSortWaves * kRadix; d += kSortThreads) (&sHist[0][0])[d] = 0u;
     __syncthreads();
     const uint32_t count = sortCount(countDevice, maxCount);
-    const uint32_t G = gridDim.x;
-    const uint32_t chunk = (count + G - 1) / G;
-    const uint32_t begin = blockIdx.x * chunk;
-    const uint32_t end = begin + chunk < count ? begin + chunk : count;
+    const SortChunks sc = sortChunks(count, gridDim.x);
+    if (blockIdx.x >= sc.used) return;  // uniform: no keys (the scan reads only the used rows)
+    const uint32_t begin = blockIdx.x * sc.chunk;
+    const uint32_t end = begin + sc.chunk < count ? begin + sc.chunk : count;
     // four keys in flight per thread, then their counts
     for (uint32_t idx = begin + t; idx < end; idx += 4 * kSortThreads) {
         Key k[4];
@@ -253,36 +266,45 @@ __global__ __launch_bounds__(kSortThreads) void kSortHistogram(const Key* __rest
             if (idx + uint32_t(u) * kSortThreads < end) atomicAdd(&sHist[wave][digitOf(k[u], shift, bits)], 1u);
     }
     __syncthreads();
-    for (uint32_t d = t; d < kRadix; d += kSortThreads) {
+    const uint32_t R = 1u << bits;
+    for (uint32_t d = t; d < R; d += kSortThreads) {
         uint32_t s = 0;
 #pragma unroll
         for (int w = 0; w < kSortWaves; ++w) s += sHist[w][d];
-        hist[d * G + blockIdx.x] = s;
+        hist[d * gridDim.x + blockIdx.x] = s;
     }
 }
 
-// One workgroup per digit: exclusive scan of hist[d * G .. d * G + G), digit total.
-__global__ __launch_bounds__(kSortThreads) void kSortScan(uint32_t* __restrict__ hist, uint32_t G,
+// One workgroup per digit d < 2^bits: exclusive scan of the used blocks' counts of digit d
+// (hist[d * G + b], b < used), in place; the digit's total into digitTotals[d].
+__global__ __launch_bounds__(kSortThreads) void kSortScan(uint32_t* __restrict__ hist,
+                                                          const uint32_t* __restrict__ countDevice,
+                                                          uint32_t maxCount, uint32_t G, uint32_t bits,
                                                           uint32_t* __restrict__ digitTotals) {
     __shared__ uint32_t sWave[kSortWaves];
-    uint32_t* row = hist + size_t(blockIdx.x) * G;
-    constexpr int kPer = int(kMaxSortBlocks) / kSortThreads;  // 4 block counts per thread
+    const uint32_t d = blockIdx.x;
+    const uint32_t used = sortChunks(sortCount(countDevice, maxCount), G).used;
+    uint32_t* row = hist + size_t(d) * G;
+    constexpr int kPer = int(kMaxSortBlocks) / kSortThreads;  // up to 16 block counts per thread
     const uint32_t first = threadIdx.x * kPer;
     uint32_t v[kPer];
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        v[j] = first + j < G ? row[first + j] : 0u;
-        s += v[j];
+    for (int j = 0; j < kPer; ++j) {  // all loads first (independent), then the sum
+        const uint32_t b = first + uint32_t(j);
+        v[j] = b < used ? row[b] : 0u;
     }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) s += v[j];
     uint32_t total;
     uint32_t run = blockExclusiveScan256(s, sWave, &total);
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-        if (first + j < G) row[first + j] = run;
+        const uint32_t b = first + uint32_t(j);
+        if (b < used) row[b] = run;
         run += v[j];
     }
-    if (threadIdx.x == 0) digitTotals[blockIdx.x] = total;
+    if (threadIdx.x == 0) digitTotals[d] = total;
 }
 
 template <typename Key>
@@ -304,16 +326,16 @@ __global__ __launch_bounds__(kSortThreads) void kSortScatter(const Key* __restri
 
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
     const uint32_t count = sortCount(countDevice, maxCount);
-    const uint32_t G = gridDim.x;
-    const uint32_t chunk = (count + G - 1) / G;
-    const uint32_t begin = blockIdx.x * chunk;
-    const uint32_t end = begin + chunk < count ? begin + chunk : count;
-    if (begin >= end) return;  // uniform
+    const SortChunks sc = sortChunks(count, gridDim.x);
+    if (blockIdx.x >= sc.used) return;  // uniform: no keys
+    const uint32_t begin = blockIdx.x * sc.chunk;
+    const uint32_t end = begin + sc.chunk < count ? begin + sc.chunk : count;
 
     {  // global base of (digit, block) = exclusive prefix of the digit totals + the in-digit scan
-        const uint32_t d = t;  // kSortThreads == kRadix
-        const uint32_t start = blockExclusiveScan256(digitTotals[d], sWave, nullptr);
-        sGlobalBase[d] = start + hist[d * G + blockIdx.x];
+        const uint32_t d = t;  // kSortThreads == kRadix; digits >= 2^bits do not occur
+        const uint32_t R = 1u << bits;
+        const uint32_t start = blockExclusiveScan256(d < R ? digitTotals[d] : 0u, sWave, nullptr);
+        sGlobalBase[d] = start + (d < R ? hist[d * gridDim.x + blockIdx.x] : 0u);
     }
     const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
 
@@ -445,7 +467,8 @@ int launchRadixSort32(const SortWork& w, uint32_t* keysA, uint32_t* valuesA, uin
         hipLaunchKernelGGL(kSortHistogram<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
                            maxCount, shift, bits, w.blockHist);
         markKernel("kSortHistogram", stream);
-        hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks, w.globalHist);
+        hipLaunchKernelGGL(kSortScan, dim3(1u << bits), dim3(kSortThreads), 0, stream, w.blockHist, countDevice,
+                           maxCount, blocks, bits, w.globalHist);
         markKernel("kSortScan", stream);
         hipLaunchKernelGGL(kSortScatter<uint32_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout, vout,
                            countDevice, maxCount, shift, bits, w.blockHist, w.globalHist);
@@ -481,8 +504,8 @@ int launchRadixSort(const SortWork& w, uint64_t* keysA, uint32_t* valuesA, uint6
             hipLaunchKernelGGL(kSortHistogram<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, countDevice,
                                maxCount, shift, bits, w.blockHist);
             markKernel("kSortHistogram", stream);
-            hipLaunchKernelGGL(kSortScan, dim3(kRadix), dim3(kSortThreads), 0, stream, w.blockHist, blocks,
-                               w.globalHist);
+            hipLaunchKernelGGL(kSortScan, dim3(1u << bits), dim3(kSortThreads), 0, stream, w.blockHist, countDevice,
+                               maxCount, blocks, bits, w.globalHist);
             markKernel("kSortScan", stream);
             hipLaunchKernelGGL(kSortScatter<uint64_t>, dim3(blocks), dim3(kSortThreads), 0, stream, kin, vin, kout,
                                vout, countDevice, maxCount, shift, bits, w.blockHist, w.globalHist);

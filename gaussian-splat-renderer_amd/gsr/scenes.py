@@ -101,6 +101,25 @@ def make_scene(count: int, seed: int = 0, mean_sigma: float = 2.0, sh_degree: in
                           np.ascontiguousarray(sh.reshape(n, K * 3)), sh_degree, seed)
 
 
+def make_lowrank_scene(count: int, seed: int = 0, rank: int = 2, mean_sigma: float = 3.0, sh_degree: int = 3,
+                       rest_sigma: float = 0.1) -> SyntheticScene:
+    """make_scene's distributions except the view-dependent SH bands, which have rank `rank`:
+    sh[i, k, c] = sum_r a[i, r] P[r, k, c] for k >= 1, with `rank` patterns P[r] ((K - 1) x 3,
+    entries ~ N(0, rest_sigma^2)) shared by every gaussian and amplitudes a[i, r] ~ N(0, 1). Each
+    gaussian's colour is then its own base colour plus a mix of `rank` view-dependent functions
+    common to the scene (a material shared across the scene), the case a small per-gaussian latent
+    and one shared network can represent; the DC terms stay independent per gaussian."""
+    sc = make_scene(count, seed, mean_sigma, sh_degree)
+    n, K = sc.count, sc.K
+    sh = sc.sh.reshape(n, K, 3).copy()
+    if K > 1:
+        P = normal(seed, 11, rank * (K - 1) * 3).reshape(rank, K - 1, 3) * rest_sigma
+        a = normal(seed, 12, n * rank).reshape(n, rank)
+        sh[:, 1:, :] = np.einsum("nr,rkc->nkc", a, P).astype(np.float32)
+    return SyntheticScene(sc.pos_alpha, sc.scales, sc.rotations, np.ascontiguousarray(sh.reshape(n, K * 3)),
+                          sh_degree, seed)
+
+
 def make_test_scene(count: int, seed: int = 99, sh_degree: int = 1) -> SyntheticScene:
     """The renderer test scene of test_tile_raster.cpp:246-275, restated with this module's
     generator: a 16^3 box around a camera at z = +6, some gaussians behind it, inside the

@@ -321,7 +321,7 @@ gsr_status gsr_sort_pairs(gsr_context* ctx, uint64_t* keys, uint32_t* values, ui
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). Same arguments, result and stability as gsr_sort_pairs with the
  * reduce-then-scan algorithm, but the ping-pong copy and the histograms live in a fixed
- * library-owned device array (about 13 MiB, held by every GPU the library runs on until the
+ * library-owned device array (about 12.5 MiB, held by every GPU the library runs on until the
  * process ends), so max_count is at most 1,048,576 keys (GSR_ERROR_INVALID_ARGUMENT above that)
  * and the call makes no hipMalloc, hipFree or other allocating call. Every call on one GPU
  * shares that GPU's array: calls that overlap on one GPU (other streams, other threads) race;
@@ -389,7 +389,12 @@ uint32_t gsr_colour_model_param_count(void);
 gsr_status gsr_colour_model_create(gsr_context* ctx, const gsr_scene* scene, uint32_t seed, gsr_colour_model** out);
 void gsr_colour_model_destroy(gsr_colour_model* model);
 /* `steps` Adam steps of `samples_per_step` samples each (a multiple of 256); *last_loss (may be
- * null) = the mean squared error of the last step, per channel, before its update. */
+ * null) = the mean squared error of the last step, per channel, before its update. A sample stream
+ * draws its gaussians as a permutation (sample s: (a s + b) mod count, a coprime to count), so a
+ * step of at most `count` samples touches distinct gaussians; such a step updates the weights with
+ * Adam and only the touched latents (lazy Adam: the other latents and their moments keep their
+ * values). A larger step (a scene smaller than the step) adds the latent gradients of repeated
+ * gaussians and takes a dense Adam step on every latent. */
 gsr_status gsr_colour_model_train(gsr_colour_model* model, uint32_t steps, uint32_t samples_per_step,
                                   float lr_weights, float lr_latents, float* last_loss);
 /* The mean squared error over `samples` samples (a multiple of 256) of stream `seed`. */
