@@ -41,31 +41,56 @@ __device__ __forceinline__ uint32_t blockInclusiveScan1024(uint32_t v, uint32_t*
     return v + prefix;
 }
 
+constexpr uint32_t kSetupPer = 8;  // block records per thread and round
+
+// Rounds of 8 x 1024 block records: every record of the round is loaded at once (coalesced:
+// record j * 1024 + t by thread t), its scanned value staged in LDS; then thread t scans the 8
+// consecutive values 8 t .. 8 t + 7 and the workgroup scan adds the threads' sums. One load
+// latency and one workgroup scan per round (a single round up to 2M gaussians).
 __global__ __launch_bounds__(kSetupThreads) void kFrameSetup(const BlockStats* __restrict__ stats, uint32_t numBlocks,
                                                              uint32_t gaussianCount, uint32_t capacity,
                                                              uint32_t scanVisible, uint32_t* __restrict__ blockOffsets,
                                                              gsr_control* __restrict__ control) {
     __shared__ uint32_t sWave[kSetupThreads / 64];
     __shared__ uint32_t sCull[7];  // 5 cull reasons, non-zero counts, M
+    __shared__ uint32_t sV[kSetupPer * kSetupThreads];
     if (threadIdx.x < 7) sCull[threadIdx.x] = 0u;
     __syncthreads();
     uint32_t carry = 0, mTotal = 0;
     uint32_t cull[6] = {0, 0, 0, 0, 0, 0};  // [0..4] reasons, [5] non-zero counts
-    for (uint32_t base = 0; base < numBlocks; base += kSetupThreads) {
-        const uint32_t b = base + threadIdx.x;
-        uint32_t v = 0;
-        if (b < numBlocks) {
-            const BlockStats s = stats[b];
-            v = scanVisible ? s.nonZero : s.tileSum;
-            mTotal += s.tileSum;
-            cull[5] += s.nonZero;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t base = 0; base < numBlocks; base += kSetupPer * kSetupThreads) {
+        BlockStats rec[kSetupPer];
 #pragma unroll
-            for (int r = 0; r < 5; ++r) cull[r] += s.cull[r];
+        for (uint32_t j = 0; j < kSetupPer; ++j) {
+            const uint32_t b = base + j * kSetupThreads + t;
+            rec[j] = b < numBlocks ? stats[b] : BlockStats{};
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kSetupPer; ++j) {
+            sV[j * kSetupThreads + t] = scanVisible ? rec[j].nonZero : rec[j].tileSum;
+            mTotal += rec[j].tileSum;
+            cull[5] += rec[j].nonZero;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) cull[r] += rec[j].cull[r];
+        }
+        __syncthreads();
+        uint32_t v[kSetupPer], sum = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kSetupPer; ++k) {
+            v[k] = sV[t * kSetupPer + k];
+            sum += v[k];
         }
         uint32_t total;
-        const uint32_t inc = blockInclusiveScan1024(v, sWave, &total);
-        if (b < numBlocks) blockOffsets[b] = carry + inc - v;
+        uint32_t run = carry + blockInclusiveScan1024(sum, sWave, &total) - sum;
+#pragma unroll
+        for (uint32_t k = 0; k < kSetupPer; ++k) {
+            const uint32_t b = base + t * kSetupPer + k;
+            if (b < numBlocks) blockOffsets[b] = run;
+            run += v[k];
+        }
         carry += total;
+        __syncthreads();  // sV is rewritten by the next round
     }
     {  // M, reduced across the workgroup
         uint32_t m = mTotal;

@@ -21,14 +21,17 @@
 //     so no per-element direction bits are needed. The stage loops stay rolled (only the
 //     in-register stages unroll): the code stays small enough for the instruction cache when
 //     waves of every size class run side by side;
-//   - a longer range (rare: a tile covered by more than 2048 splats) is sorted afterwards by
-//     the whole workgroup: chunks of 2048 by an LDS bitonic network, then merge rounds of
-//     doubling run width through the scratch buffers (merge path: every thread finds its
-//     output slice by a binary search, then merges it sequentially). Every thread reads the
-//     workgroup's four range lengths first, so the choice is uniform: a workgroup without a
-//     long range never waits at a barrier. That path has no loop around the register sort
-//     (such a loop makes the compiler keep the 32 words' addresses live across iterations,
-//     doubling the kernel's registers).
+//   - a longer range (a tile covered by more than 1024 splats: the dense central tiles) is
+//     sorted afterwards by the whole workgroup: up to 8192 entries in registers, 4 waves of 8,
+//     16 or 32 words per lane with the cross-wave stages through LDS; longer ones by LDS chunks
+//     of 2048 (bitonic), then merge rounds of doubling run width through the scratch buffers
+//     (merge path: every thread finds its output slice by a binary search, then merges it
+//     sequentially). Workgroup b takes positions b, b + G, b + 2G, b + 3G of the tile order, so
+//     the dense tiles, first in the order, are spread one per workgroup; every thread reads the
+//     workgroup's four range lengths first, so the choice is uniform and a workgroup without a
+//     long range never waits at a barrier. The register forms are separate (noinline)
+//     functions: a loop around an inlined copy makes the compiler keep the words' addresses
+//     live across iterations, doubling the kernel's registers.
 // The loads are striped (word r of lane l is input entry r * 64 + l: coalesced; the input order
 // of a sort is arbitrary), the stores follow the network positions. Tiles are taken in the
 // order the caller gives (the frame passes the tiles centre-out: dense tiles, the long sorts,
@@ -234,27 +237,29 @@ __device__ __forceinline__ void waveSortAny(uint64_t* keys, uint32_t* values, ui
 // ---- long ranges: the whole workgroup ----
 
 constexpr uint32_t kGroupChunk = 4u * kWaveChunk;  // ranges up to 8192: the register form below
+constexpr uint32_t kWaveAlone = kWaveChunk;  // ranges a single wave sorts (longer: the whole workgroup)
 
-// Storage slot in sW of chunk position q = lane * 32 + r (slot r * 64 + lane: a wave's accesses
+// Storage slot in sW of chunk position q = lane * E + r (slot r * 64 + lane: a wave's accesses
 // to one register index hit 64 consecutive words).
 __device__ __forceinline__ uint32_t chunkSlot(uint32_t lane, uint32_t r) { return r * 64u + lane; }
 
-// Cross-wave compare-exchange of two 2048-word chunks held in registers by waves a (the lower
-// positions) and b: position q of a pairs with position q of b, or with 2047 - q when `flip`.
+// Cross-wave compare-exchange of two chunks of 64 E words held in registers by waves a (the lower
+// positions) and b: position q of a pairs with position q of b, or with 64 E - 1 - q when `flip`.
 // a keeps the minimum, b the maximum, exchanged in place through sW: b writes its words, a
 // reads each partner, keeps the minimum and writes the maximum into the same slot, b reads it
 // back. Every thread of the workgroup calls it (barriers).
-__device__ __forceinline__ void crossExchange(uint64_t (&w)[32], uint32_t wave, uint32_t lane, uint32_t a, uint32_t b,
+template <int E>
+__device__ __forceinline__ void crossExchange(uint64_t (&w)[E], uint32_t wave, uint32_t lane, uint32_t a, uint32_t b,
                                               bool flip, uint64_t* sW) {
     if (wave == b) {
 #pragma unroll
-        for (uint32_t r = 0; r < 32u; ++r) sW[chunkSlot(lane, r)] = w[r];
+        for (uint32_t r = 0; r < uint32_t(E); ++r) sW[chunkSlot(lane, r)] = w[r];
     }
     __syncthreads();
     if (wave == a) {
 #pragma unroll
-        for (uint32_t r = 0; r < 32u; ++r) {
-            const uint32_t slot = flip ? chunkSlot(63u - lane, 31u - r) : chunkSlot(lane, r);
+        for (uint32_t r = 0; r < uint32_t(E); ++r) {
+            const uint32_t slot = flip ? chunkSlot(63u - lane, uint32_t(E) - 1u - r) : chunkSlot(lane, r);
             const uint64_t y = sW[slot];
             sW[slot] = __builtin_elementwise_max(w[r], y);
             w[r] = __builtin_elementwise_min(w[r], y);
@@ -263,58 +268,54 @@ __device__ __forceinline__ void crossExchange(uint64_t (&w)[32], uint32_t wave, 
     __syncthreads();
     if (wave == b) {
 #pragma unroll
-        for (uint32_t r = 0; r < 32u; ++r) w[r] = sW[chunkSlot(lane, r)];
+        for (uint32_t r = 0; r < uint32_t(E); ++r) w[r] = sW[chunkSlot(lane, r)];
     }
     __syncthreads();
 }
 
-// The stages of a merge level below the chunk size (j = 1024 .. 1), within each wave's chunk.
-__device__ __forceinline__ void mergeChunk(uint64_t (&w)[32], uint32_t lane) {
+// The stages of a merge level below the chunk size (j = 32 E .. 1), within each wave's chunk.
+template <int E>
+__device__ __forceinline__ void mergeChunk(uint64_t (&w)[E], uint32_t lane) {
 #pragma unroll 1
-    for (uint32_t j = kWaveChunk / 2u; j >= 32u; j >>= 1) laneHalf<32>(w, lane, j / 32u);
-    registerHalves<32, 16>(w);
+    for (uint32_t j = 32u * E; j >= uint32_t(E); j >>= 1) laneHalf<E>(w, lane, j / E);
+    registerHalves<E, E / 2>(w);
 }
 
-// A range of 2048 < n <= 8192 entries sorted by W = 2 or 4 waves of the workgroup: positions
-// v * 2048 + lane * 32 + r (wave v), each wave's chunk sorted in registers, then the bitonic
-// merge levels 4096 (and 8192): the stages whose partners are in another wave through sW, the
-// rest in registers.
-__device__ __attribute__((noinline)) void groupSortRegisters(uint64_t* keys, uint32_t* values, uint32_t tile, uint2 range, uint32_t wave,
-                                   uint32_t lane, uint64_t* sW) {
+// A range of n <= 256 E entries sorted by the 4 waves of the workgroup: positions
+// v * 64 E + lane * E + r (wave v), each wave's chunk of 64 E sorted in registers, then the
+// bitonic merge levels 128 E and 256 E: the stages whose partners are in another wave through
+// sW, the rest in registers (the 16-word form: DPP for lane distances below 16). Padding words
+// sort last.
+template <int E>
+__device__ __attribute__((noinline)) void groupSortRegisters(uint64_t* keys, uint32_t* values, uint32_t tile,
+                                                             uint2 range, uint32_t wave, uint32_t lane, uint64_t* sW) {
+    constexpr uint32_t C = 64u * E;  // a wave's chunk
     const uint32_t n = range.y - range.x;
-    const uint32_t W = n <= 2u * kWaveChunk ? 2u : 4u;
-    const bool active = wave < W;
-    uint64_t w[32];
-    if (active) {
+    uint64_t w[E];
 #pragma unroll
-        for (uint32_t r = 0; r < 32u; ++r) {
-            const uint32_t e = wave * kWaveChunk + r * 64u + lane;
-            w[r] = e < n ? orderWord(keys[range.x + e], values[range.x + e]) : kPad;
-        }
-        waveSort<32>(w, lane);
+    for (uint32_t r = 0; r < uint32_t(E); ++r) {
+        const uint32_t e = wave * C + r * 64u + lane;
+        w[r] = e < n ? orderWord(keys[range.x + e], values[range.x + e]) : kPad;
     }
-    // Level 4096: chunks (0, 1) and (2, 3), flip.
-    crossExchange(w, wave, lane, 0u, 1u, true, sW);
-    if (W == 4u) crossExchange(w, wave, lane, 2u, 3u, true, sW);
-    if (active) mergeChunk(w, lane);
-    if (W == 4u) {
-        // Level 8192: the flip pairs chunk v with chunk 3 - v (position q with 2047 - q), then the
-        // stage 2048 apart pairs chunk v with chunk v ^ 1 (same position).
-        crossExchange(w, wave, lane, 0u, 3u, true, sW);
-        crossExchange(w, wave, lane, 1u, 2u, true, sW);
-        crossExchange(w, wave, lane, 0u, 1u, false, sW);
-        crossExchange(w, wave, lane, 2u, 3u, false, sW);
-        if (active) mergeChunk(w, lane);
-    }
-    if (active) {
-        const uint64_t tileHi = uint64_t(tile) << 32;
+    waveSort<E>(w, lane);
+    // Level 2C: chunks (0, 1) and (2, 3), flip.
+    crossExchange<E>(w, wave, lane, 0u, 1u, true, sW);
+    crossExchange<E>(w, wave, lane, 2u, 3u, true, sW);
+    mergeChunk<E>(w, lane);
+    // Level 4C: the flip pairs chunk v with chunk 3 - v (position q with C - 1 - q), then the
+    // stage C apart pairs chunk v with chunk v ^ 1 (same position).
+    crossExchange<E>(w, wave, lane, 0u, 3u, true, sW);
+    crossExchange<E>(w, wave, lane, 1u, 2u, true, sW);
+    crossExchange<E>(w, wave, lane, 0u, 1u, false, sW);
+    crossExchange<E>(w, wave, lane, 2u, 3u, false, sW);
+    mergeChunk<E>(w, lane);
+    const uint64_t tileHi = uint64_t(tile) << 32;
 #pragma unroll
-        for (uint32_t r = 0; r < 32u; ++r) {
-            const uint32_t e = wave * kWaveChunk + lane * 32u + r;
-            if (e < n) {
-                keys[range.x + e] = tileHi | (w[r] >> 32);
-                values[range.x + e] = uint32_t(w[r]);
-            }
+    for (uint32_t r = 0; r < uint32_t(E); ++r) {
+        const uint32_t e = wave * C + lane * E + r;
+        if (e < n) {
+            keys[range.x + e] = tileHi | (w[r] >> 32);
+            values[range.x + e] = uint32_t(w[r]);
         }
     }
 }
@@ -433,15 +434,15 @@ __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __rest
                                                                 uint32_t tileCount) {
     __shared__ uint64_t sW[kLdsChunk];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t first = blockIdx.x * kTsWaves;
-    uint32_t longMask = 0;  // the workgroup's ranges longer than 2048 (same in every thread)
+    uint32_t groupMask = 0;  // the workgroup's ranges longer than 2048 (same in every thread)
     uint32_t tiles[kTsWaves];
 #pragma unroll
     for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v) {
-        tiles[v] = first + v < tileCount ? order[first + v] : tileCount;
+        const uint32_t at = blockIdx.x * kTsWaves + v;
+        tiles[v] = at < tileCount ? order[at] : tileCount;
         if (tiles[v] < tileCount) {
             const uint2 rg = ranges[tiles[v]];
-            if (rg.y - rg.x > kWaveChunk) longMask |= 1u << v;
+            if (rg.y - rg.x > kWaveAlone) groupMask |= 1u << v;
         }
     }
     // 1. One range of up to 2048 entries per wave (the tile, its range and size class are
@@ -450,17 +451,20 @@ __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __rest
     if (tile < tileCount) {
         const uint2 range = ranges[tile];
         const uint32_t n = range.y - range.x;
-        if (n > 1u && n <= kWaveChunk) waveSortAny(keys, values, uint64_t(tile) << 32, range.x, n, lane);
+        if (n > 1u && n <= kWaveAlone) waveSortAny(keys, values, uint64_t(tile) << 32, range.x, n, lane);
     }
-    if (longMask == 0u) return;  // uniform per workgroup
-    // 2. The longer ones, one at a time by the whole workgroup: up to 8192 entries in registers,
-    // longer ones by LDS chunks and merge rounds.
+    if (groupMask == 0u) return;  // uniform per workgroup
+    // 2. The longer ones, one at a time by the whole workgroup: up to 8192 entries in registers
+    // (16 or 32 words per lane), longer ones by LDS chunks and merge rounds.
     __syncthreads();
     for (uint32_t v = 0; v < uint32_t(kTsWaves); ++v) {
-        if (!(longMask & (1u << v))) continue;
+        if (!(groupMask & (1u << v))) continue;
         const uint2 rg = ranges[tiles[v]];
-        if (rg.y - rg.x <= kGroupChunk) groupSortRegisters(keys, values, tiles[v], rg, wave, lane, sW);
+        const uint32_t n = rg.y - rg.x;
+        if (n <= 4096u) groupSortRegisters<16>(keys, values, tiles[v], rg, wave, lane, sW);
+        else if (n <= kGroupChunk) groupSortRegisters<32>(keys, values, tiles[v], rg, wave, lane, sW);
         else groupSortLong(keys, values, keysTmp, valuesTmp, tiles[v], rg, sW);
+        __syncthreads();  // sW is reused by the next range
     }
 }
 
