@@ -283,6 +283,35 @@ def test_camera_from_viewer_matches_closed_form(oracle):
     assert (c.tiles_x, c.tiles_y) == (80, 45)
 
 
+def test_fp32_preprocess_error_vs_fp64_small_workload(oracle):
+    """The FP32 preprocess error against the FP64 twin on the small workload (100k gaussians, seed 0,
+    means ~ N(0, 2^2 I), 1280x720 camera at radius 8 looking at the origin), stated per visible
+    splat as max / median: centre (px, worst axis), conic (relative to the largest entry),
+    colour (absolute, worst channel), depth (relative), radius (px). The GPU preprocess gives the
+    FP32 oracle's bits (the parity tests), so these are the kernels' errors too. The bounds below
+    are the stated figures rounded up: 1.6e-4 px, 5.1e-5, 2.0e-7, radius 0."""
+    from gsr.scenes import make_scene
+    from gsr import camera
+
+    sc = make_scene(100_000, seed=0, mean_sigma=2.0)
+    cam = oracle.OracleCamera.from_viewer(camera.orbit_view(8.0, 0.0, 0.0), camera.perspective(60.0, 16 / 9),
+                                          camera.IDENTITY, 1280, 720)
+    s32, _ = oracle.preprocess(sc, cam)
+    s64 = oracle.preprocess_f64(sc, cam)
+    assert np.array_equal(s32["cull_reason"], s64[:, 3].astype(np.uint32))
+    vis = s32["cull_reason"] == 0
+    centre = np.abs(s32["center"][vis] - s64[vis, 0:2]).max(axis=1)
+    conic = (np.abs(s32["conic_opacity"][vis, :3] - s64[vis, 4:7]) /
+             np.abs(s64[vis, 4:7]).max(axis=1, keepdims=True)).max(axis=1)
+    colour = np.abs(s32["color"][vis] - s64[vis, 8:11]).max(axis=1)
+    depth = np.abs(s32["depth"][vis] - s64[vis, 2]) / s64[vis, 2]
+    radius = np.abs(s32["radius"][vis] - s64[vis, 11])
+    for name, a in (("centre px", centre), ("conic rel", conic), ("colour abs", colour), ("depth rel", depth),
+                    ("radius px", radius)):
+        print(f"{name}: max {a.max():.3e}, median {np.median(a):.3e} over {vis.sum()} visible splats")
+    assert centre.max() < 2e-4 and conic.max() < 6e-5 and colour.max() < 3e-7 and radius.max() == 0
+
+
 def test_fp64_twin_agrees_with_fp32(oracle):
     from gsr.scenes import make_scene
     from gsr import camera
