@@ -509,26 +509,28 @@ def main():
     # 32 B scale + rotation unless culled at the near plane or for opacity and K x 12 B of SH per
     # visible gaussian, and writes the 48-B splat + 4-B tile count per gaussian and 8 B per tile
     # range zeroed; kFrameSetup reads the 32-B block records; kDuplicate reads 4 B count per gaussian
-    # and the 48-B splat per visible one and writes 8 + 4 B per entry; a radix pass reads 8-B keys
-    # (histogram), then 12 B per entry and writes 12 B per entry (scatter); kRanges reads the keys
-    # and writes 8 B per tile; the tile depth sort reads and writes 12 B per entry; the render
-    # reads 4 B value + 48 B splat per fetched entry and 8 B per tile range and writes 4 B per pixel.
+    # and the 48-B splat per visible one and writes a 4-B tile key + 4-B value per entry (global
+    # mode also the 4-B depth bits per visible gaussian); a radix pass reads the 4-B keys
+    # (histogram), then 8 B per entry and writes 8 B per entry (scatter); kRanges reads the keys
+    # and writes 8 B per tile; the tile depth sort reads 4 B value + 4 B depth bits and writes 4 B
+    # per entry; the render reads 4 B value + 48 B splat per fetched entry and 8 B per tile range
+    # and writes 4 B per pixel.
     n = args.count
     tiles = cams[0].tiles
     near, low_opacity = mean_culls[1], mean_culls[4]
     K = sc.K
     M, V = mean_entries, mean_visible
     blocks = min((int(M) + 2047) // 2048, 1024)
-    key_b = 8 if args.sort_mode == "global" else 4
+    key_b = 4
     pre_bytes = n * 16 + (n - near - low_opacity) * 32 + V * K * 12 + n * 52 + tiles * 8
     algo = {"kPreprocess": pre_bytes,
             "kFrameSetup": (n + 255) // 256 * 32 + (n + 255) // 256 * 4 + 80,
-            "kDuplicate": n * 4 + V * 48 + M * (key_b + 4),
+            "kDuplicate": n * 4 + V * 48 + M * (key_b + 4) + (V * 4 if args.sort_mode == "global" else 0),
             "kSortHistogram": M * key_b + blocks * 256 * 4,
             "kSortScan": 2 * blocks * 256 * 4,
             "kSortScatter": 2 * M * (key_b + 4) + blocks * 256 * 4,
             "kRanges": M * key_b + tiles * 8,
-            "kTileDepthSort": 2 * M * 12,
+            "kTileDepthSort": M * 12,
             "kRenderCulled": pairs["fetched"] * 52 + tiles * 8 + W * H * 4}
     traffic_by_kernel = {}
     traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic_1.2M-1080p-orbit.md")

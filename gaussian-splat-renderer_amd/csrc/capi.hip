@@ -210,13 +210,19 @@ struct gsr_renderer {
     // the largest camera seen; per gaussian the screen-space gradient planes of gsr_backward.
     DeviceBuffer pixelState, grad2D;
     size_t pixelStatePixels = 0;
-    // The list (ping-pong; keys are 32-bit tile ids in binning mode, 64-bit (tile, depth) in
-    // global mode) and the sort's work areas.
-    DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus;
+    // The list (ping-pong; keys are 32-bit tile ids in binning mode and in global mode with the
+    // per-tile depth sort, 64-bit (tile, depth) for the full-key and one-sweep sorts) and the
+    // sort's work areas. Global mode with the per-tile depth sort: per gaussian the depth bits the
+    // duplicate pass writes for the tile sort.
+    DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus, depthBits;
     // Binning mode, per gaussian: compacted depth keys / indices (ping-pong), depth-ordered
     // tile counts and their per-256 sums.
     DeviceBuffer depthKeysA, depthKeysB, depthValuesA, depthValuesB, sortedCounts, visibleSums;
     bool binning = false;
+    // Global mode sorted by tile id (reduce-then-scan, 2 passes of the 32-bit keys), then each
+    // tile's range by depth (the tile depth sort); otherwise global mode sorts whole 64-bit keys.
+    bool tileSort = false;
+    bool tileKeys32() const { return binning || tileSort; }
     uint32_t sortEpoch = 0;
     // Adaptive list (splat_tile_renderer.cpp:444-486) and the per-frame statistics read back
     // without stalling: each frame's control block is copied to pinned host memory and an event
@@ -600,16 +606,18 @@ static uint32_t roundUpToBlock(uint64_t n) {
 }
 
 // The list buffers for `capacity` entries: keys and values ping-pong (+ the one-sweep status
-// words), zeroed so that the lists start as valid keys and gaussian indices. Adaptive lists are
-// stream-ordered allocations, so a resize never waits for the frames in flight.
+// words), zeroed so that the lists start as valid keys and gaussian indices. The value buffers
+// hold kListPad more entries (zero: gaussian 0), which the tile depth sort may read past the
+// list's end. Adaptive lists are stream-ordered allocations, so a resize never waits for the
+// frames in flight.
 static hipError_t allocateList(gsr_renderer* r, uint32_t capacity, hipStream_t stream) {
-    const size_t keyBytes = r->binning ? 4 : 8;
+    const size_t keyBytes = r->tileKeys32() ? 4 : 8;
     const bool async = r->settings.adaptive_list != 0;
     hipError_t e = hipSuccess;
     for (DeviceBuffer* b : {&r->keysA, &r->keysB, &r->valuesA, &r->valuesB, &r->sortStatus}) {
         const size_t bytes = b == &r->sortStatus ? sortStatusBytes(capacity)
                              : (b == &r->keysA || b == &r->keysB) ? size_t(capacity) * keyBytes
-                                                                  : size_t(capacity) * 4;
+                                                                  : (size_t(capacity) + kListPad) * 4;
         if (e == hipSuccess) e = async ? b->allocateAsync(bytes, stream) : b->allocate(bytes);
         if (e == hipSuccess) e = async ? hipMemsetAsync(b->ptr, 0, b->bytes, stream) : hipMemset(b->ptr, 0, b->bytes);
     }
@@ -769,6 +777,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     r->binSize = uint32_t(s.tile_size);
     r->degree = std::min(scene->shDegree, std::max(0, std::min(3, s.max_sh_degree)));
     r->binning = s.sort_mode == GSR_SORT_MODE_BINNING;
+    r->tileSort = !r->binning && !s.full_key_sort && s.sort_algorithm == GSR_SORT_REDUCE_THEN_SCAN;
     r->maxCapacity = capacityForBytes(double(s.budget_mb) * 1024.0 * 1024.0, r->binning);
     uint32_t initialCapacity = r->maxCapacity;
     if (s.adaptive_list) {
@@ -795,6 +804,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
             alloc(*b, n * 4);
         alloc(r->visibleSums, gBlocks * 4);
     }
+    if (r->tileSort) alloc(r->depthBits, n * 4);
     alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses + size_t(kRadix) * kMaxSortBlocks) * 4);
     if (e == hipSuccess) e = allocateList(r, initialCapacity, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -869,7 +879,7 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
                                   &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth,
-                                  &r->pixelState, &r->grad2D, &r->tileStats};
+                                  &r->pixelState, &r->grad2D, &r->tileStats, &r->depthBits};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     for (const auto& o : r->tileOrders) s += o->table.bytes;
@@ -920,7 +930,7 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         r->tileStatsTiles = tiles;
     }
     const uint32_t* tileOrder = nullptr;
-    if (!r->binning && !r->settings.full_key_sort) {
+    if (r->tileSort) {
         const gsr_renderer::TileOrder* o = r->findOrder(binsX, binsY);
         if (!o) {  // a new grid: its table, kept for later frames of the size
             std::vector<uint32_t> order(tiles);
@@ -1043,7 +1053,32 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
         launchRanges32(which ? r->keysB.as<uint32_t>() : r->keysA.as<uint32_t>(), control, r->capacity, tiles,
                        r->ranges.as<uint2>(), stream);
+    } else if (r->tileSort) {
+        // The tile ids by the radix sort (2 passes of 32-bit keys), then each tile's range by
+        // (depth, index) in the tile depth sort: the list of the full 64-bit sort.
+        GSR_HIP(mark(2));
+        GSR_HIP(mark(3));
+        launchDuplicateTiles(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(),
+                             control, sc->count, bounds, grid, r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(),
+                             r->depthBits.as<uint32_t>(), control, stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(4));
+        which = launchRadixSort32(work, r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint32_t>(),
+                                  r->valuesB.as<uint32_t>(), r->capacity, &control->entries, tileBits, stream);
+        GSR_HIP(hipGetLastError());
+        uint32_t* vals = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
+        launchRanges32(which ? r->keysB.as<uint32_t>() : r->keysA.as<uint32_t>(), control, r->capacity, tiles,
+                       r->ranges.as<uint2>(), stream);
+        GSR_HIP(hipGetLastError());
+        launchTileDepthSort(vals, r->depthBits.as<uint32_t>(),
+                            which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>(), r->ranges.as<uint2>(),
+                            tileOrder, tiles, stream);
+        GSR_HIP(hipGetLastError());
+        GSR_HIP(mark(5));
+        sortedValues = vals;
     } else {
+        // The whole 64-bit (tile, depth) keys by the radix sort (full_key_sort, or the one-sweep
+        // sort).
         GSR_HIP(mark(2));
         GSR_HIP(mark(3));
         launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(), control,
@@ -1051,28 +1086,14 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
                         r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
-        // Full key: all 32 + tileBits bits by the radix sort. Otherwise only the tile bits,
-        // then each tile's range by (depth, index) in the tile depth sort.
-        const bool fullKey = r->settings.full_key_sort != 0;
         which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),
-                                r->valuesB.as<uint32_t>(), r->capacity, &control->entries, fullKey ? 0u : 32u,
-                                32 + tileBits, &r->sortEpoch, stream);
+                                r->valuesB.as<uint32_t>(), r->capacity, &control->entries, 0u, 32 + tileBits,
+                                &r->sortEpoch, stream);
         GSR_HIP(hipGetLastError());
-        uint64_t* keys = which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>();
-        uint32_t* vals = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
-        if (!fullKey) {
-            launchRanges(keys, control, r->capacity, tiles, r->ranges.as<uint2>(), stream);
-            GSR_HIP(hipGetLastError());
-            launchTileDepthSort(keys, vals, which ? r->keysA.as<uint64_t>() : r->keysB.as<uint64_t>(),
-                                which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>(), r->ranges.as<uint2>(),
-                                tileOrder, tiles, stream);
-            GSR_HIP(hipGetLastError());
-            GSR_HIP(mark(5));
-        } else {
-            GSR_HIP(mark(5));
-            launchRanges(keys, control, r->capacity, tiles, r->ranges.as<uint2>(), stream);
-        }
-        sortedValues = vals;
+        GSR_HIP(mark(5));
+        launchRanges(which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>(), control, r->capacity, tiles,
+                     r->ranges.as<uint2>(), stream);
+        sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
     }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(6));
@@ -1147,14 +1168,14 @@ gsr_status gsr_read_lists(gsr_renderer* r, uint32_t* tileKeys, uint32_t* depthKe
     const void* keySrc = r->sortedIn ? r->keysB.ptr : r->keysA.ptr;
     const void* valueSrc = r->sortedIn ? r->valuesB.ptr : r->valuesA.ptr;
     std::vector<uint32_t> vals;
-    if (n > 0 && (values || (depthKeys && r->binning))) {
+    if (n > 0 && (values || (depthKeys && r->tileKeys32()))) {
         vals.resize(n);
         GSR_HIP(hipMemcpy(vals.data(), valueSrc, size_t(n) * 4, hipMemcpyDeviceToHost));
         if (values) std::memcpy(values, vals.data(), size_t(n) * 4);
     }
-    if (n > 0 && r->binning) {
+    if (n > 0 && r->tileKeys32()) {
         if (tileKeys) GSR_HIP(hipMemcpy(tileKeys, keySrc, size_t(n) * 4, hipMemcpyDeviceToHost));
-        if (depthKeys) {  // binning keys carry no depth: it is the listed splat's depth bits
+        if (depthKeys) {  // 32-bit tile keys carry no depth: it is the listed splat's depth bits
             std::vector<float4> sp(size_t(r->scene->count) * 3);
             GSR_HIP(hipMemcpy(sp.data(), r->splats.ptr, sp.size() * sizeof(float4), hipMemcpyDeviceToHost));
             for (uint32_t i = 0; i < n; ++i) {
@@ -1598,7 +1619,7 @@ gsr_status gsr_batch_capture(gsr_batch* b, gsr_renderer* const* renderers, uint3
         const gsr_renderer* r = renderers[v % used];
         const uint32_t tiles = r->binsX(cams[v]) * r->binsY(cams[v]);
         const bool orderReady =
-            r->binning || r->settings.full_key_sort || r->findOrder(r->binsX(cams[v]), r->binsY(cams[v])) != nullptr;
+            !r->tileSort || r->findOrder(r->binsX(cams[v]), r->binsY(cams[v])) != nullptr;
         if (tiles > r->rangesTiles || !orderReady || !r->haveFrame)
             return fail(GSR_ERROR_INVALID_ARGUMENT,
                         "gsr_batch_capture: view %u (%ux%u) is a size its renderer has not rendered yet: render the "

@@ -127,18 +127,26 @@ struct BinGrid {
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
                      const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
                      uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream);
+// The same list with 32-bit keys = tile id; each listed gaussian's depth bits go to depthBits[i].
+void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
+                          const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
+                          uint32_t* keys, uint32_t* values, uint32_t* depthBits, gsr_control* controlRW,
+                          hipStream_t stream);
 // Sorts by key bits [firstBit, keyBits) (stable). Returns which buffer (0 = keysA/valuesA,
 // 1 = B) holds the sorted result. *epoch is the caller's status-epoch counter (one per status
 // array), advanced once per pass.
 int launchRadixSort(const SortWork& work, uint64_t* keysA, uint32_t* valuesA, uint64_t* keysB, uint32_t* valuesB,
                     uint32_t maxCount, const uint32_t* countDevice, uint32_t firstBit, uint32_t keyBits,
                     uint32_t* epoch, hipStream_t stream);
-// Per-tile depth order (tilesort.hip): after a stable sort of 64-bit (tile << 32 | depth) keys
-// by their tile bits only, sorts each tile's range by (depth bits, gaussian index) in place;
-// the lists then equal those of the full 64-bit sort. keysTmp/valuesTmp: scratch of the list's
-// size (used for ranges longer than 2048 entries). order: a permutation of the tile ids, the
-// order in which the tiles are taken (scheduling only; the lists do not depend on it).
-void launchTileDepthSort(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp, const uint2* ranges,
+// Per-tile depth order (tilesort.hip): after a stable sort of the list by its tile ids, sorts
+// each tile's range of gaussian indices by (depth bits, gaussian index) in place, the depth bits
+// of gaussian g being depthBits[g]; the lists then equal those of the full 64-bit (tile, depth)
+// sort. values must hold kListPad readable entries past the list (it reads whole rows of 64).
+// valuesTmp: scratch of the list's size (ranges longer than 8192 entries). order: a permutation
+// of the tile ids, the order in which the tiles are taken (scheduling only; the lists do not
+// depend on it).
+constexpr uint32_t kListPad = 8192;
+void launchTileDepthSort(uint32_t* values, const uint32_t* depthBits, uint32_t* valuesTmp, const uint2* ranges,
                          const uint32_t* order, uint32_t tileCount, hipStream_t stream);
 int launchRadixSort32(const SortWork& work, uint32_t* keysA, uint32_t* valuesA, uint32_t* keysB, uint32_t* valuesB,
                       uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, hipStream_t stream);

@@ -8,8 +8,10 @@
 // ellipse and its rows) is set up once into LDS; then the block's entries are handed out
 // to all 256 threads (load-balanced over big splats) and each finds its gaussian by an
 // 8-step binary search in LDS and its tile by walking that coverage's rows. Keys are
-// (tile << 32 | depth bits), values the gaussian index; entries >= capacity are dropped
-// (index order), the tile id #tiles marks an entry without a tile (never produced).
+// (tile << 32 | depth bits), or the tile id alone with the depth bits written once per
+// gaussian (depthBits[index]: the list then moves 8 B per entry through the tile sort, and the
+// per-tile depth sort looks the depth up); values the gaussian index; entries >= capacity are
+// dropped (index order), the tile id #tiles marks an entry without a tile (never produced).
 #include "gsr_device.hpp"
 
 namespace gsr {
@@ -21,9 +23,9 @@ constexpr int kDupThreads = kPreprocessThreads;
 constexpr int kCachedRows = 8;  // row spans kept in LDS per gaussian; longer splats walk the rest
 
 // BINNING = false: block b = gaussians [256 b, 256 b + 256) in index order, 64-bit keys
-// (tile << 32 | depth bits). BINNING = true: block b = the depth-sorted visible splats
-// [256 b, ...) of `order` (count control->visible), tileCounts in that order, 32-bit keys =
-// tile id.
+// (tile << 32 | depth bits), or 32-bit keys = tile id with each listed gaussian's depth bits in
+// depthBits[index]. BINNING = true: block b = the depth-sorted visible splats [256 b, ...) of
+// `order` (count control->visible), tileCounts in that order, 32-bit keys = tile id.
 template <bool BINNING, typename Key>
 __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restrict__ splats,
                                                           const uint32_t* __restrict__ tileCounts,
@@ -32,7 +34,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                                                           const gsr_control* __restrict__ control, uint32_t count,
                                                           uint32_t bounds, BinGrid grid,
                                                           Key* __restrict__ keys, uint32_t* __restrict__ values,
-                                                          gsr_control* controlRW) {
+                                                          uint32_t* __restrict__ depthBits, gsr_control* controlRW) {
     __shared__ uint32_t sScan[kDupThreads];
     __shared__ uint32_t sWave[kDupThreads / 64];
     // Per gaussian: first row, row count, depth bits, and either the rectangle's x0/width
@@ -89,6 +91,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
         sRow0[t] = r0;
         sRows[t] = r1 - r0;
         sDepth[t] = __float_as_uint(o0.z);
+        if (!BINNING && sizeof(Key) == 4) depthBits[i] = __float_as_uint(o0.z);
         if (!cv.useEllipse) {
             sBinX[0][t] = cv.bins.x0;
             sBinX[1][t] = cv.bins.x1 - cv.bins.x0;
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                 }
             }
         }
-        if (BINNING) keys[k] = Key(tile);
+        if (BINNING || sizeof(Key) == 4) keys[k] = Key(tile);
         else keys[k] = Key((uint64_t(tile) << 32) | uint64_t(sDepth[lo]));
         values[k] = sIndex[lo];
     }
@@ -237,7 +240,18 @@ void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uin
     const uint32_t blocks = (count + kDupThreads - 1) / kDupThreads;
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<false, uint64_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
-                       nullptr, blockOffsets, control, count, bounds, grid, keys, values, controlRW);
+                       nullptr, blockOffsets, control, count, bounds, grid, keys, values, nullptr, controlRW);
+    markKernel("kDuplicate", stream);
+}
+
+void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
+                          const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
+                          uint32_t* keys, uint32_t* values, uint32_t* depthBits, gsr_control* controlRW,
+                          hipStream_t stream) {
+    const uint32_t blocks = (count + kDupThreads - 1) / kDupThreads;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL((kDuplicate<false, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
+                       nullptr, blockOffsets, control, count, bounds, grid, keys, values, depthBits, controlRW);
     markKernel("kDuplicate", stream);
 }
 
@@ -248,7 +262,7 @@ void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, 
     const uint32_t blocks = (maxVisible + kDupThreads - 1) / kDupThreads;
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<true, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, sortedCounts,
-                       order, blockOffsets, control, maxVisible, bounds, grid, keys, values, controlRW);
+                       order, blockOffsets, control, maxVisible, bounds, grid, keys, values, nullptr, controlRW);
     markKernel("kDuplicate", stream);
 }
 

@@ -1,10 +1,14 @@
 // Per-tile depth order for global mode (same list as the reference's 64-bit (tile | depth)
 // sort, gpu_radix_sort.cpp:132-195 with splat_tile_renderer.cpp:740-760).
 //
-// gfx950 design: the global radix sort only orders the list by its tile bits (2 passes for a
-// 13-bit tile id instead of 6 for the full 45-bit key). Within a tile the entries are then in
-// duplication (gaussian-index) order, and this kernel orders each tile's range by
-// (depth bits, gaussian index), which is the stable (tile, depth, index) order of the full sort.
+// gfx950 design: the global radix sort only orders the list by its 32-bit tile ids (2 passes
+// for a 13-bit tile id instead of 6 for the full 45-bit key; 8 B per entry moved instead of 12).
+// Within a tile the entries are then in duplication (gaussian-index) order, and this kernel
+// orders each tile's range by (depth bits, gaussian index), which is the stable
+// (tile, depth, index) order of the full sort. The depth bits are looked up per gaussian
+// (depthBits[g], written once per listed gaussian by the duplicate pass) when a range is loaded:
+// whole rows of 64 list values first (the value buffers are padded past the list's end, so these
+// loads need no per-lane bounds branch), then the lookups, then the padding selects.
 //
 // One wave per tile, no LDS and no workgroup barrier (the 4 waves of a workgroup work on 4
 // different tiles and never wait for each other):
@@ -47,8 +51,9 @@ constexpr int kTsWaves = kTsThreads / 64;
 constexpr uint32_t kWaveChunk = 2048;  // entries one wave sorts in registers (E = 32)
 constexpr uint64_t kPad = ~uint64_t(0);
 
-__device__ __forceinline__ uint64_t orderWord(uint64_t key, uint32_t value) {
-    return (key << 32) | value;  // (depth bits, gaussian index); the tile bits shift out
+// (depth bits, gaussian index) of list value g.
+__device__ __forceinline__ uint64_t orderWord(const uint32_t* __restrict__ depthBits, uint32_t g) {
+    return (uint64_t(depthBits[g]) << 32) | g;
 }
 
 __device__ __forceinline__ uint64_t shflXor64(uint64_t v, uint32_t mask) {
@@ -203,35 +208,46 @@ __device__ __forceinline__ void waveSort(uint64_t (&w)[E], uint32_t lane) {
     }
 }
 
-// The wave sorts entries [at, at + n) in place (n <= 64 E); tileHi is the tile's key bits.
+// Words r of a lane: list entries first + r * 64 + lane (rows of 64, read whole: the value buffers
+// are padded), their depth bits looked up, kPad past the n entries.
 template <int E>
-__device__ __forceinline__ void waveSortRange(uint64_t* keys, uint32_t* values, uint64_t tileHi, uint32_t at,
-                                              uint32_t n, uint32_t lane) {
-    uint64_t w[E];
+__device__ __forceinline__ void loadWords(uint64_t (&w)[E], const uint32_t* values,
+                                          const uint32_t* __restrict__ depthBits, uint32_t first, uint32_t n,
+                                          uint32_t lane) {
+    const uint32_t* row = values + first + lane;
+#pragma unroll
+    for (int r = 0; r < E; ++r) w[r] = row[r * 64];
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-        const uint32_t e = uint32_t(r) * 64u + lane;
-        w[r] = e < n ? orderWord(keys[at + e], values[at + e]) : kPad;
+        const bool valid = uint32_t(r) * 64u + lane < n;
+        const uint32_t g = valid ? uint32_t(w[r]) : 0u;
+        const uint64_t word = orderWord(depthBits, g);
+        w[r] = valid ? word : kPad;
     }
+}
+
+// The wave sorts entries [at, at + n) in place (n <= 64 E).
+template <int E>
+__device__ __forceinline__ void waveSortRange(uint32_t* values, const uint32_t* __restrict__ depthBits, uint32_t at,
+                                              uint32_t n, uint32_t lane) {
+    uint64_t w[E];
+    loadWords<E>(w, values, depthBits, at, n, lane);
     waveSort<E>(w, lane);
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         const uint32_t e = lane * E + uint32_t(r);
-        if (e < n) {
-            keys[at + e] = tileHi | (w[r] >> 32);
-            values[at + e] = uint32_t(w[r]);
-        }
+        if (e < n) values[at + e] = uint32_t(w[r]);
     }
 }
 
-__device__ __forceinline__ void waveSortAny(uint64_t* keys, uint32_t* values, uint64_t tileHi, uint32_t at,
+__device__ __forceinline__ void waveSortAny(uint32_t* values, const uint32_t* __restrict__ depthBits, uint32_t at,
                                             uint32_t n, uint32_t lane) {
-    if (n <= 64u) waveSortRange<1>(keys, values, tileHi, at, n, lane);  // uniform per wave
-    else if (n <= 128u) waveSortRange<2>(keys, values, tileHi, at, n, lane);
-    else if (n <= 256u) waveSortRange<4>(keys, values, tileHi, at, n, lane);
-    else if (n <= 512u) waveSortRange<8>(keys, values, tileHi, at, n, lane);
-    else if (n <= 1024u) waveSortRange<16>(keys, values, tileHi, at, n, lane);
-    else waveSortRange<32>(keys, values, tileHi, at, n, lane);
+    if (n <= 64u) waveSortRange<1>(values, depthBits, at, n, lane);  // uniform per wave
+    else if (n <= 128u) waveSortRange<2>(values, depthBits, at, n, lane);
+    else if (n <= 256u) waveSortRange<4>(values, depthBits, at, n, lane);
+    else if (n <= 512u) waveSortRange<8>(values, depthBits, at, n, lane);
+    else if (n <= 1024u) waveSortRange<16>(values, depthBits, at, n, lane);
+    else waveSortRange<32>(values, depthBits, at, n, lane);
 }
 
 // ---- long ranges: the whole workgroup ----
@@ -287,16 +303,13 @@ __device__ __forceinline__ void mergeChunk(uint64_t (&w)[E], uint32_t lane) {
 // sW, the rest in registers (the 16-word form: DPP for lane distances below 16). Padding words
 // sort last.
 template <int E>
-__device__ __attribute__((noinline)) void groupSortRegisters(uint64_t* keys, uint32_t* values, uint32_t tile,
+__device__ __attribute__((noinline)) void groupSortRegisters(uint32_t* values, const uint32_t* __restrict__ depthBits,
                                                              uint2 range, uint32_t wave, uint32_t lane, uint64_t* sW) {
     constexpr uint32_t C = 64u * E;  // a wave's chunk
     const uint32_t n = range.y - range.x;
+    const uint32_t chunkN = n > wave * C ? n - wave * C : 0u;  // entries of this wave's chunk
     uint64_t w[E];
-#pragma unroll
-    for (uint32_t r = 0; r < uint32_t(E); ++r) {
-        const uint32_t e = wave * C + r * 64u + lane;
-        w[r] = e < n ? orderWord(keys[range.x + e], values[range.x + e]) : kPad;
-    }
+    loadWords<E>(w, values, depthBits, range.x + wave * C, chunkN, lane);
     waveSort<E>(w, lane);
     // Level 2C: chunks (0, 1) and (2, 3), flip.
     crossExchange<E>(w, wave, lane, 0u, 1u, true, sW);
@@ -309,14 +322,10 @@ __device__ __attribute__((noinline)) void groupSortRegisters(uint64_t* keys, uin
     crossExchange<E>(w, wave, lane, 0u, 1u, false, sW);
     crossExchange<E>(w, wave, lane, 2u, 3u, false, sW);
     mergeChunk<E>(w, lane);
-    const uint64_t tileHi = uint64_t(tile) << 32;
 #pragma unroll
     for (uint32_t r = 0; r < uint32_t(E); ++r) {
         const uint32_t e = wave * C + lane * E + r;
-        if (e < n) {
-            keys[range.x + e] = tileHi | (w[r] >> 32);
-            values[range.x + e] = uint32_t(w[r]);
-        }
+        if (e < n) values[range.x + e] = uint32_t(w[r]);
     }
 }
 
@@ -343,21 +352,20 @@ __device__ __forceinline__ void bitonicSortLds(uint64_t* s, uint32_t n2) {
 }
 
 // Number of elements of run A (length la) among the first d outputs of merge(A, B).
-__device__ __forceinline__ uint32_t coRank(const uint64_t* ka, const uint32_t* va, uint32_t la, const uint64_t* kb,
+__device__ __forceinline__ uint32_t coRank(const uint32_t* __restrict__ depthBits, const uint32_t* va, uint32_t la,
                                            const uint32_t* vb, uint32_t lb, uint32_t d) {
     uint32_t lo = d > lb ? d - lb : 0u, hi = d < la ? d : la;
     while (lo < hi) {
         const uint32_t i = (lo + hi) >> 1;
-        if (orderWord(ka[i], va[i]) < orderWord(kb[d - i - 1], vb[d - i - 1])) lo = i + 1;
+        if (orderWord(depthBits, va[i]) < orderWord(depthBits, vb[d - i - 1])) lo = i + 1;
         else hi = i;
     }
     return lo;
 }
 
-__device__ void groupSortLong(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp,
-                              uint32_t tile, uint2 range, uint64_t* sW) {
+__device__ void groupSortLong(uint32_t* values, const uint32_t* __restrict__ depthBits, uint32_t* valuesTmp,
+                              uint2 range, uint64_t* sW) {
     const uint32_t n = range.y - range.x;
-    const uint64_t tileHi = uint64_t(tile) << 32;
 
     // 1. Chunks of up to 2048 entries, sorted in LDS, in place.
     for (uint32_t c0 = 0; c0 < n; c0 += kLdsChunk) {
@@ -366,21 +374,15 @@ __device__ void groupSortLong(uint64_t* keys, uint32_t* values, uint64_t* keysTm
         while (n2 < m) n2 <<= 1;
         const uint32_t base = range.x + c0;
         for (uint32_t i = threadIdx.x; i < n2; i += kTsThreads)
-            sW[i] = i < m ? orderWord(keys[base + i], values[base + i]) : kPad;
+            sW[i] = i < m ? orderWord(depthBits, values[base + i]) : kPad;
         __syncthreads();
         bitonicSortLds(sW, n2);
-        for (uint32_t i = threadIdx.x; i < m; i += kTsThreads) {
-            const uint64_t w = sW[i];
-            keys[base + i] = tileHi | (w >> 32);
-            values[base + i] = uint32_t(w);
-        }
+        for (uint32_t i = threadIdx.x; i < m; i += kTsThreads) values[base + i] = uint32_t(sW[i]);
         __syncthreads();
     }
 
     // 2. Merge rounds: runs of width w -> 2w, alternating between the list and the scratch.
-    uint64_t* ks = keys + range.x;
     uint32_t* vs = values + range.x;
-    uint64_t* kd = keysTmp + range.x;
     uint32_t* vd = valuesTmp + range.x;
     for (uint32_t w = kLdsChunk; w < n; w <<= 1) {
         for (uint32_t lo = 0; lo < n; lo += 2 * w) {
@@ -390,44 +392,36 @@ __device__ void groupSortLong(uint64_t* keys, uint32_t* values, uint64_t* keysTm
             const uint32_t per = (total + kTsThreads - 1) / kTsThreads;
             const uint32_t d0 = threadIdx.x * per < total ? threadIdx.x * per : total;
             const uint32_t d1 = d0 + per < total ? d0 + per : total;
-            uint32_t ia = coRank(ks + lo, vs + lo, la, ks + mid, vs + mid, lb, d0);
+            uint32_t ia = coRank(depthBits, vs + lo, la, vs + mid, lb, d0);
             uint32_t ib = d0 - ia;
             for (uint32_t d = d0; d < d1; ++d) {
                 bool takeA;
                 if (ia >= la) takeA = false;
                 else if (ib >= lb) takeA = true;
-                else takeA = orderWord(ks[lo + ia], vs[lo + ia]) < orderWord(ks[mid + ib], vs[mid + ib]);
+                else takeA = orderWord(depthBits, vs[lo + ia]) < orderWord(depthBits, vs[mid + ib]);
                 if (takeA) {
-                    kd[lo + d] = ks[lo + ia];
                     vd[lo + d] = vs[lo + ia];
                     ++ia;
                 } else {
-                    kd[lo + d] = ks[mid + ib];
                     vd[lo + d] = vs[mid + ib];
                     ++ib;
                 }
             }
         }
         __syncthreads();  // the round's output is the next round's input (same workgroup)
-        uint64_t* tk = ks;
-        ks = kd;
-        kd = tk;
         uint32_t* tv = vs;
         vs = vd;
         vd = tv;
     }
-    if (ks != keys + range.x) {  // the result is in the scratch: copy it back
-        for (uint32_t i = threadIdx.x; i < n; i += kTsThreads) {
-            keys[range.x + i] = ks[i];
-            values[range.x + i] = vs[i];
-        }
+    if (vs != values + range.x) {  // the result is in the scratch: copy it back
+        for (uint32_t i = threadIdx.x; i < n; i += kTsThreads) values[range.x + i] = vs[i];
     }
     __syncthreads();  // before the next long range reuses the LDS
 }
 
 // 5 workgroups per CU: 96 VGPRs, the most the 32-word sort fits in without spills.
-__global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
-                                                                uint64_t* __restrict__ keysTmp,
+__global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint32_t* __restrict__ values,
+                                                                const uint32_t* __restrict__ depthBits,
                                                                 uint32_t* __restrict__ valuesTmp,
                                                                 const uint2* __restrict__ ranges,
                                                                 const uint32_t* __restrict__ order,
@@ -451,7 +445,7 @@ __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __rest
     if (tile < tileCount) {
         const uint2 range = ranges[tile];
         const uint32_t n = range.y - range.x;
-        if (n > 1u && n <= kWaveAlone) waveSortAny(keys, values, uint64_t(tile) << 32, range.x, n, lane);
+        if (n > 1u && n <= kWaveAlone) waveSortAny(values, depthBits, range.x, n, lane);
     }
     if (groupMask == 0u) return;  // uniform per workgroup
     // 2. The longer ones, one at a time by the whole workgroup: up to 8192 entries in registers
@@ -461,21 +455,21 @@ __global__ __launch_bounds__(kTsThreads, 5) void kTileDepthSort(uint64_t* __rest
         if (!(groupMask & (1u << v))) continue;
         const uint2 rg = ranges[tiles[v]];
         const uint32_t n = rg.y - rg.x;
-        if (n <= 4096u) groupSortRegisters<16>(keys, values, tiles[v], rg, wave, lane, sW);
-        else if (n <= kGroupChunk) groupSortRegisters<32>(keys, values, tiles[v], rg, wave, lane, sW);
-        else groupSortLong(keys, values, keysTmp, valuesTmp, tiles[v], rg, sW);
+        if (n <= 4096u) groupSortRegisters<16>(values, depthBits, rg, wave, lane, sW);
+        else if (n <= kGroupChunk) groupSortRegisters<32>(values, depthBits, rg, wave, lane, sW);
+        else groupSortLong(values, depthBits, valuesTmp, rg, sW);
         __syncthreads();  // sW is reused by the next range
     }
 }
 
 }  // namespace
 
-void launchTileDepthSort(uint64_t* keys, uint32_t* values, uint64_t* keysTmp, uint32_t* valuesTmp, const uint2* ranges,
+void launchTileDepthSort(uint32_t* values, const uint32_t* depthBits, uint32_t* valuesTmp, const uint2* ranges,
                          const uint32_t* order, uint32_t tileCount, hipStream_t stream) {
     if (tileCount == 0) return;
     const uint32_t blocks = (tileCount + kTsWaves - 1) / kTsWaves;
-    hipLaunchKernelGGL(kTileDepthSort, dim3(blocks), dim3(kTsThreads), 0, stream, keys, values, keysTmp, valuesTmp,
-                       ranges, order, tileCount);
+    hipLaunchKernelGGL(kTileDepthSort, dim3(blocks), dim3(kTsThreads), 0, stream, values, depthBits, valuesTmp, ranges,
+                       order, tileCount);
     markKernel("kTileDepthSort", stream);
 }
 

@@ -134,9 +134,10 @@ typedef struct gsr_settings {
     int32_t sort_algorithm;      /* GSR_SORT_*: radix-sort pass structure (same result, bit for bit) */
     int32_t render_loop;         /* GSR_RENDER_LOOP_*: loop form of the fast one-pixel render */
     int32_t sort_mode;           /* GSR_SORT_MODE_*: how the tile list is built (same list without overflow) */
-    int32_t full_key_sort;       /* global mode: 0 (default) = the radix sort orders the tile bits only and a
+    int32_t full_key_sort;       /* global mode: 0 (default) = the radix sort orders 32-bit tile ids only and a
                                     per-tile sort orders each tile's range by (depth, index); 1 = the radix sort
-                                    orders all (tile, depth) key bits. Same list, bit for bit. */
+                                    orders all bits of 64-bit (tile, depth) keys (so does the one-sweep sort).
+                                    Same list, bit for bit. */
     int32_t render_splat;        /* GSR_RENDER_SPLAT_*: the record the render pass gathers per list entry */
     uint32_t sort_window;        /* StopThePop per-pixel re-sorting window K (TileSettings::sortWindow,
                                     splat_tile_renderer.h:108): 0 = INRIA order; 1..64 = blend in the order
@@ -187,8 +188,8 @@ typedef enum gsr_render_splat {
 typedef enum gsr_sort_mode {
     GSR_SORT_MODE_BINNING = 0, /* depth sort of the visible splats (32-bit keys), duplicate in depth order,
                                   stable sort of 32-bit tile ids; 16.5 B per list entry (default) */
-    GSR_SORT_MODE_GLOBAL = 1   /* duplicate in index order, one sort of 64-bit (tile, depth) keys;
-                                  24.5 B per list entry */
+    GSR_SORT_MODE_GLOBAL = 1   /* duplicate in index order, the list in (tile, depth, index) order as one sort
+                                  of 64-bit (tile, depth) keys gives it; 24.5 B per list entry */
 } gsr_sort_mode;
 
 /* Loop form of the fast one-pixel tile render (same blend, same image within 1/255). */
@@ -204,7 +205,7 @@ typedef enum gsr_render_loop {
 typedef enum gsr_sort_algorithm {
     GSR_SORT_REDUCE_THEN_SCAN = 0, /* per digit: block histograms, scan, ballot-ranked scatter (default) */
     GSR_SORT_ONESWEEP = 1          /* one histogram pass, then one scatter per digit with decoupled look-back;
-                                      64-bit keys only (global mode, gsr_sort_pairs) */
+                                      64-bit keys only (global mode: all key bits; gsr_sort_pairs) */
 } gsr_sort_algorithm;
 
 /* Stage timings of the last profiled frame, ms (the GpuProfiler scopes, profiler.h:1-87). */

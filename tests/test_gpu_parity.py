@@ -171,14 +171,19 @@ def test_rgba8_within_one_level(gsr_ctx, exact_alpha, ppt, loop):
     scene.close()
 
 
-@pytest.mark.parametrize("case", ["100k-720p", "test-scene-640x360", "needles-giants-1280x720"])
+@pytest.mark.parametrize("case", ["100k-720p", "test-scene-640x360", "needles-giants-1280x720",
+                                  "near-clamp-opacities-1280x720"])
 def test_culled_loop_equals_uncompacted_loop(gsr_ctx, case):
     """The quadrant-culled loop drops only pairs below the alpha cutoff (a box, then the exact
     ellipse-vs-quadrant test with a margin sized to the FP32 rounding of q): its RGBA32F image
     equals the uncompacted predicated loop's bit for bit (same q, same blend order). The
     needles-and-giants scene stresses that margin: one long axis (up to e^0.5) and two tiny ones
     (e^-7) at every rotation, so the projected conics are nearly singular and the quadratic's terms
-    cancel, 1 % large isotropic splats near the camera, opacities mostly near 1."""
+    cancel, 1 % large isotropic splats near the camera, opacities mostly near 1. The culled loop
+    also leaves out the 0.99 alpha clamp for wave lists whose opacities are all <= 0.98, where it
+    cannot apply; the uncompacted loop always clamps. The near-clamp scene puts every opacity in
+    [0.975, 0.9999] (a third exactly 0.98 or 0.99) at 1-2 px splats, so both paths run side by side
+    and the clamp binds at many centre pixels."""
     import gsr
     from gsr import camera, scenes
 
@@ -196,6 +201,17 @@ def test_culled_loop_equals_uncompacted_loop(gsr_ctx, case):
         sc.scales[:, :3] = sig
         sc.pos_alpha[:, 3] = (1.0 / (1.0 + np.exp(-rng.normal(3.0, 1.0, n)))).astype(np.float32)
         view, W, H = camera.orbit_view(6.0, 10.0, 30.0), 1280, 720
+    elif case == "near-clamp-opacities-1280x720":
+        sc = scenes.make_scene(60_000, seed=5, mean_sigma=2.0)
+        rng = np.random.default_rng(5)
+        n = sc.count
+        o = rng.uniform(0.975, 0.9999, n)
+        pick = rng.random(n)
+        o[pick < 1 / 6] = 0.98
+        o[(pick >= 1 / 6) & (pick < 1 / 3)] = 0.99
+        sc.pos_alpha[:, 3] = o.astype(np.float32)
+        sc.scales[:, :3] = np.exp(rng.normal(-5.0, 0.5, (n, 3))).astype(np.float32)
+        view, W, H = camera.orbit_view(8.0, 0.0, 0.0), 1280, 720
     else:
         sc, view, W, H = scenes.make_test_scene(250_000, seed=99), camera.look_at([0.3, 0.5, 6.0], [0, 0, 0]), 640, 360
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
