@@ -215,6 +215,9 @@ struct gsr_renderer {
     // sort's work areas. Global mode with the per-tile depth sort: per gaussian the depth bits the
     // duplicate pass writes for the tile sort.
     DeviceBuffer keysA, keysB, valuesA, valuesB, sortSmall, sortStatus, depthBits;
+    // Global mode: the preprocess blocks' sums per superblock (superBlockCount BlockStats, zero
+    // between frames: the ranges pass clears them).
+    DeviceBuffer superStats;
     // Binning mode, per gaussian: compacted depth keys / indices (ping-pong), depth-ordered
     // tile counts and their per-256 sums.
     DeviceBuffer depthKeysA, depthKeysB, depthValuesA, depthValuesB, sortedCounts, visibleSums;
@@ -805,10 +808,12 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
         alloc(r->visibleSums, gBlocks * 4);
     }
     if (r->tileSort) alloc(r->depthBits, n * 4);
+    if (!r->binning) alloc(r->superStats, size_t(superBlockCount(uint32_t(n))) * sizeof(BlockStats));
     alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses + size_t(kRadix) * kMaxSortBlocks) * 4);
     if (e == hipSuccess) e = allocateList(r, initialCapacity, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = hipMemset(r->control.ptr, 0, sizeof(gsr_control));
+    if (e == hipSuccess && r->superStats.ptr) e = hipMemset(r->superStats.ptr, 0, r->superStats.bytes);
     for (int i = 0; i < GSR_STAGE_COUNT && e == hipSuccess; ++i) e = hipEventCreate(&r->events[i]);
     if (s.profile_stages == 2 && e == hipSuccess) {
         r->marks.reset(new (std::nothrow) KernelMarks());
@@ -879,7 +884,7 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
                                   &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth,
-                                  &r->pixelState, &r->grad2D, &r->tileStats, &r->depthBits};
+                                  &r->pixelState, &r->grad2D, &r->tileStats, &r->depthBits, &r->superStats};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     for (const auto& o : r->tileOrders) s += o->table.bytes;
@@ -1004,8 +1009,10 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         GSR_HIP(hipEventRecord(r->marks->events[0], stream));
         t_kernelMarks = r->marks.get();
     }
+    BlockStats* superStats = r->binning ? nullptr : r->superStats.as<BlockStats>();
+    const uint32_t numSuper = superBlockCount(sc->count);
     launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
-                     r->blockStats.as<BlockStats>(), r->ranges.as<uint32_t>(), rangeWords,
+                     r->blockStats.as<BlockStats>(), superStats, control, r->ranges.as<uint32_t>(), rangeWords,
                      uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), r->rayDepth.as<float4>(),
                      r->colour ? r->colour->useMlp.as<uint8_t>() : nullptr, stream);
     GSR_HIP(hipGetLastError());
@@ -1018,9 +1025,15 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         GSR_HIP(hipGetLastError());
     }
     GSR_HIP(mark(1));
-    launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, r->binning,
-                     r->blockOffsets.as<uint32_t>(), control, stream);
-    GSR_HIP(hipGetLastError());
+    // Binning mode scans the block sums here; global mode's duplicate pass takes its block offsets
+    // from the preprocess's superblock sums and writes the control block itself.
+    if (r->binning) {
+        launchFrameSetup(r->blockStats.as<BlockStats>(), gBlocks, sc->count, r->capacity, true,
+                         r->blockOffsets.as<uint32_t>(), control, stream);
+        GSR_HIP(hipGetLastError());
+    }
+    const DupFrame dupFrame{r->blockStats.as<BlockStats>(), superStats, numSuper, r->capacity};
+    const uint32_t superWords = superStats ? numSuper * uint32_t(sizeof(BlockStats) / 4) : 0u;
     int which = 0;
     const uint32_t* sortedValues = nullptr;
     if (r->binning) {
@@ -1052,15 +1065,15 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         GSR_HIP(mark(5));
         sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
         launchRanges32(which ? r->keysB.as<uint32_t>() : r->keysA.as<uint32_t>(), control, r->capacity, tiles,
-                       r->ranges.as<uint2>(), stream);
+                       r->ranges.as<uint2>(), nullptr, 0, stream);
     } else if (r->tileSort) {
         // The tile ids by the radix sort (2 passes of 32-bit keys), then each tile's range by
         // (depth, index) in the tile depth sort: the list of the full 64-bit sort.
         GSR_HIP(mark(2));
         GSR_HIP(mark(3));
-        launchDuplicateTiles(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(),
-                             control, sc->count, bounds, grid, r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(),
-                             r->depthBits.as<uint32_t>(), control, stream);
+        launchDuplicateTiles(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, sc->count, bounds, grid,
+                             r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), r->depthBits.as<uint32_t>(), control,
+                             stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
         which = launchRadixSort32(work, r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint32_t>(),
@@ -1068,7 +1081,7 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         GSR_HIP(hipGetLastError());
         uint32_t* vals = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
         launchRanges32(which ? r->keysB.as<uint32_t>() : r->keysA.as<uint32_t>(), control, r->capacity, tiles,
-                       r->ranges.as<uint2>(), stream);
+                       r->ranges.as<uint2>(), reinterpret_cast<uint32_t*>(superStats), superWords, stream);
         GSR_HIP(hipGetLastError());
         launchTileDepthSort(vals, r->depthBits.as<uint32_t>(),
                             which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>(), r->ranges.as<uint2>(),
@@ -1081,9 +1094,8 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         // sort).
         GSR_HIP(mark(2));
         GSR_HIP(mark(3));
-        launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), r->blockOffsets.as<uint32_t>(), control,
-                        sc->count, bounds, grid, r->keysA.as<uint64_t>(),
-                        r->valuesA.as<uint32_t>(), control, stream);
+        launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, sc->count, bounds, grid,
+                        r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
         which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),
@@ -1092,7 +1104,7 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(5));
         launchRanges(which ? r->keysB.as<uint64_t>() : r->keysA.as<uint64_t>(), control, r->capacity, tiles,
-                     r->ranges.as<uint2>(), stream);
+                     r->ranges.as<uint2>(), reinterpret_cast<uint32_t*>(superStats), superWords, stream);
         sortedValues = which ? r->valuesB.as<uint32_t>() : r->valuesA.as<uint32_t>();
     }
     GSR_HIP(hipGetLastError());

@@ -112,10 +112,22 @@ inline void markKernel(const char* name, hipStream_t stream) {
 }
 
 // Launch wrappers (each in its .hip file).
+// Global mode: the preprocess also adds each block's BlockStats into the sums of its superblock of
+// kSuperBlocks blocks, superStats[block / kSuperBlocks] (atomics; one 32-B record per superblock,
+// so a workgroup's atomics meet one cache line), and zeroes the control block's accumulated
+// counters; the duplicate pass then finds its block's first entry from those sums and the earlier
+// blocks of its superblock (no frame-setup scan), and its workgroup 0 writes the control block;
+// the ranges pass zeroes the sums for the next frame. superStats = nullptr: binning mode
+// (kFrameSetup scans the block sums).
+constexpr uint32_t kSuperBlocks = 64;
+inline uint32_t superBlockCount(uint32_t count) {
+    const uint32_t blocks = (count + kPreprocessThreads - 1) / kPreprocessThreads;
+    return (blocks + kSuperBlocks - 1) / kSuperBlocks;
+}
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
-                      uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, const uint8_t* useMlp,
-                      hipStream_t stream);
+                      uint32_t* tileCounts, BlockStats* blockStats, BlockStats* superStats, gsr_control* control,
+                      uint32_t* rangesWords, uint32_t rangeWordCount, uint32_t renderSplat, uint32_t* packed,
+                      float4* rayDepth, const uint8_t* useMlp, hipStream_t stream);
 // scanVisible = false: blockOffsets = exclusive scan of the blocks' tile sums (global mode);
 // true: of their visible (non-zero count) gaussians (binning mode's compaction).
 void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
@@ -124,14 +136,20 @@ void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t
 struct BinGrid {
     uint32_t tilesX, tilesY, binSize, binsX, binsY;
 };
-void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
-                     const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
-                     uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream);
+// Global mode's duplicate: block b's first entry from the preprocess's superblock and block sums;
+// the list is clamped to `capacity`; the last workgroup writes the control block.
+struct DupFrame {
+    const BlockStats* blockStats;
+    const BlockStats* superStats;  // numSuper sums
+    uint32_t numSuper, capacity;
+};
+void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, uint32_t count,
+                     uint32_t bounds, const BinGrid& grid, uint64_t* keys, uint32_t* values, gsr_control* control,
+                     hipStream_t stream);
 // The same list with 32-bit keys = tile id; each listed gaussian's depth bits go to depthBits[i].
-void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
-                          const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
-                          uint32_t* keys, uint32_t* values, uint32_t* depthBits, gsr_control* controlRW,
-                          hipStream_t stream);
+void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, uint32_t count,
+                          uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values, uint32_t* depthBits,
+                          gsr_control* control, hipStream_t stream);
 // Sorts by key bits [firstBit, keyBits) (stable). Returns which buffer (0 = keysA/valuesA,
 // 1 = B) holds the sorted result. *epoch is the caller's status-epoch counter (one per status
 // array), advanced once per pass.
@@ -152,10 +170,11 @@ int launchRadixSort32(const SortWork& work, uint32_t* keysA, uint32_t* valuesA, 
                       uint32_t maxCount, const uint32_t* countDevice, uint32_t keyBits, hipStream_t stream);
 void launchCopyPairs(const uint64_t* ks, const uint32_t* vs, uint64_t* kd, uint32_t* vd, const uint32_t* countDevice,
                      uint32_t maxCount, hipStream_t stream);
+// zeroWords[0, zeroCount): words the ranges pass zeroes (global mode's superblock sums; none: 0).
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
-                  uint2* ranges, hipStream_t stream);
+                  uint2* ranges, uint32_t* zeroWords, uint32_t zeroCount, hipStream_t stream);
 void launchRanges32(const uint32_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
-                    uint2* ranges, hipStream_t stream);
+                    uint2* ranges, uint32_t* zeroWords, uint32_t zeroCount, hipStream_t stream);
 
 // Binning mode (the reference's default tile sort, splat_tile_renderer.h:8-26): compact the
 // visible splats (depth bits, index) in index order, sort them by depth (32-bit keys), scan

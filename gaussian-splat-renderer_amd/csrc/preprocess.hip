@@ -20,6 +20,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                                   float4* __restrict__ splats,
                                                                   uint32_t* __restrict__ tileCounts,
                                                                   BlockStats* __restrict__ blockStats,
+                                                                  BlockStats* __restrict__ superStats,
+                                                                  gsr_control* __restrict__ control,
                                                                   uint32_t* __restrict__ rangesWords,
                                                                   uint32_t rangeWordCount, uint32_t renderSplat,
                                                                   uint32_t* __restrict__ packed,
@@ -27,6 +29,10 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                                   const uint8_t* __restrict__ useMlp) {
     const uint32_t i = blockIdx.x * kPreprocessThreads + threadIdx.x;
     if (i < rangeWordCount) rangesWords[i] = 0u;
+    if (superStats && i == 0) {  // the counters the frame's passes accumulate into
+        control->coverage_mismatch = 0u;
+        control->sort_timeouts = 0u;
+    }
 
     uint32_t reason = kNoGaussian;
     uint32_t touched = 0;
@@ -279,6 +285,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
 #pragma unroll
         for (int w = 0; w < kPreprocessThreads / 64; ++w) acc += sStats[w][threadIdx.x];
         reinterpret_cast<uint32_t*>(&blockStats[blockIdx.x])[threadIdx.x] = acc;
+        if (superStats && acc != 0u)  // global mode: the superblock's sums (the duplicate's block offsets)
+            atomicAdd(reinterpret_cast<uint32_t*>(&superStats[blockIdx.x / kSuperBlocks]) + threadIdx.x, acc);
     }
     if (threadIdx.x == 7) blockStats[blockIdx.x].pad = 0u;
 }
@@ -286,20 +294,20 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
 }  // namespace
 
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
-                      uint32_t* tileCounts, BlockStats* blockStats, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, const uint8_t* useMlp,
-                      hipStream_t stream) {
+                      uint32_t* tileCounts, BlockStats* blockStats, BlockStats* superStats, gsr_control* control,
+                      uint32_t* rangesWords, uint32_t rangeWordCount, uint32_t renderSplat, uint32_t* packed,
+                      float4* rayDepth, const uint8_t* useMlp, hipStream_t stream) {
     uint32_t n = scene.count > rangeWordCount ? scene.count : rangeWordCount;
     const uint32_t blocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene,
-                           cam, bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat,
-                           packed, rayDepth, useMlp);
+                           cam, bounds, splats, tileCounts, blockStats, superStats, control, rangesWords,
+                           rangeWordCount, renderSplat, packed, rayDepth, useMlp);
     else
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
-                           bounds, splats, tileCounts, blockStats, rangesWords, rangeWordCount, renderSplat, packed,
-                           rayDepth, useMlp);
+                           bounds, splats, tileCounts, blockStats, superStats, control, rangesWords, rangeWordCount,
+                           renderSplat, packed, rayDepth, useMlp);
     markKernel("kPreprocess", stream);
 }
 

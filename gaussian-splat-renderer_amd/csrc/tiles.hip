@@ -26,17 +26,55 @@ constexpr int kCachedRows = 8;  // row spans kept in LDS per gaussian; longer sp
 // (tile << 32 | depth bits), or 32-bit keys = tile id with each listed gaussian's depth bits in
 // depthBits[index]. BINNING = true: block b = the depth-sorted visible splats [256 b, ...) of
 // `order` (count control->visible), tileCounts in that order, 32-bit keys = tile id.
+__device__ __forceinline__ uint32_t waveSum64(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Global mode, workgroup 0 of the duplicate pass: the control block from the superblock sums (N,
+// V, M, entries, cull counts, overflow; the counters the passes accumulate into were zeroed by the
+// preprocess). Every thread of the workgroup calls it (barrier).
+__device__ void writeControl(const DupFrame& frame, uint32_t count, gsr_control* control) {
+    __shared__ uint32_t sSum[7];
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    for (uint32_t f = wave; f < 7u; f += kDupThreads / 64) {  // field f of BlockStats, summed by a wave
+        uint32_t acc = 0;
+        for (uint32_t sb = lane; sb < frame.numSuper; sb += 64)
+            acc += reinterpret_cast<const uint32_t*>(&frame.superStats[sb])[f];
+        acc = waveSum64(acc);
+        if (lane == 0) sSum[f] = acc;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const uint32_t m = sSum[0];
+        const uint32_t entries = m < frame.capacity ? m : frame.capacity;
+        control->gaussian_count = count;
+        control->visible = sSum[1];
+        control->duplicates = m;
+        control->entries = entries;
+        for (int r = 0; r < 5; ++r) control->cull_counts[r] = sSum[2 + r];
+        control->overflow = m - entries;
+        control->ranges_groups[0] = (entries + 255u) / 256u;
+        control->ranges_groups[1] = 1u;
+        control->ranges_groups[2] = 1u;
+    }
+}
+
+// BINNING = false (global mode): the block offsets come from frame (see DupFrame) and workgroup 0
+// writes the control block; BINNING = true: from blockOffsets and control (kFrameSetup).
 template <bool BINNING, typename Key>
 __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restrict__ splats,
                                                           const uint32_t* __restrict__ tileCounts,
                                                           const uint32_t* __restrict__ order,
                                                           const uint32_t* __restrict__ blockOffsets,
-                                                          const gsr_control* __restrict__ control, uint32_t count,
-                                                          uint32_t bounds, BinGrid grid,
+                                                          const gsr_control* __restrict__ control, DupFrame frame,
+                                                          uint32_t count, uint32_t bounds, BinGrid grid,
                                                           Key* __restrict__ keys, uint32_t* __restrict__ values,
                                                           uint32_t* __restrict__ depthBits, gsr_control* controlRW) {
     __shared__ uint32_t sScan[kDupThreads];
     __shared__ uint32_t sWave[kDupThreads / 64];
+    __shared__ uint32_t sFrame[3];  // global mode: superblocks before this block, earlier blocks of it, M
     // Per gaussian: first row, row count, depth bits, and either the rectangle's x0/width
     // (one span for every row) or the first kCachedRows row spans (x0 | width << 16). For the
     // rows past them (long splats) the binning form also keeps the ellipse: its blocks hold
@@ -62,6 +100,30 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
 
     // Block exclusive scan of the counts (the downsweep of the reduce-then-scan).
     const uint32_t lane = t & 63u, wave = t >> 6;
+    if (!BINNING) {
+        // The block's first entry: the tile sums of the superblocks before it (and of all: M) by
+        // wave 0, of the earlier blocks of its own superblock by wave 1.
+        const uint32_t sb = blockIdx.x / kSuperBlocks;
+        if (wave == 0) {
+            uint32_t before = 0, all = 0;
+            for (uint32_t q = lane; q < frame.numSuper; q += 64) {
+                const uint32_t m = frame.superStats[q].tileSum;
+                all += m;
+                before += q < sb ? m : 0u;
+            }
+            before = waveSum64(before);
+            all = waveSum64(all);
+            if (lane == 0) {
+                sFrame[0] = before;
+                sFrame[2] = all;
+            }
+        } else if (wave == 1) {
+            const uint32_t b = sb * kSuperBlocks + lane;
+            uint32_t m = b < blockIdx.x ? frame.blockStats[b].tileSum : 0u;
+            m = waveSum64(m);
+            if (lane == 0) sFrame[1] = m;
+        }
+    }
     uint32_t v = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -76,11 +138,13 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     const uint32_t excl = prefix + v - c;
     sScan[t] = excl;
 
-    const uint32_t base = blockOffsets[blockIdx.x];
-    const uint32_t entries = control->entries;
-    if (base >= entries || total == 0) return;  // uniform: the whole block is past the capacity or empty
+    const uint32_t base = BINNING ? blockOffsets[blockIdx.x] : sFrame[0] + sFrame[1];
+    const uint32_t entries = BINNING ? control->entries : (sFrame[2] < frame.capacity ? sFrame[2] : frame.capacity);
+    // Uniform: the whole block is past the capacity or empty.
+    const bool work = !(base >= entries || total == 0);
+    if (BINNING && !work) return;
 
-    if (c != 0u) {
+    if (work && c != 0u) {
         const float4* s = splats + 3 * size_t(i);
         const float4 o0 = s[0], o1 = s[1], o2 = s[2];
         const Rect rect = tileRect(o0.x, o0.y, o2.w, grid.tilesX, grid.tilesY);
@@ -196,6 +260,7 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
         else keys[k] = Key((uint64_t(tile) << 32) | uint64_t(sDepth[lo]));
         values[k] = sIndex[lo];
     }
+    if (!BINNING && blockIdx.x == 0) writeControl(frame, count, controlRW);
 }
 
 constexpr int kRangesThreads = 256;
@@ -205,10 +270,15 @@ __device__ __forceinline__ uint32_t tileOfKey(Key k) {
     return sizeof(Key) == 8 ? uint32_t(uint64_t(k) >> 32) : uint32_t(k);
 }
 
+// Also zeroes zeroWords[0, zeroCount) (workgroup 0): global mode's superblock sums, which the
+// frame's duplicate pass has read, for the next frame's preprocess.
 template <typename Key>
 __global__ __launch_bounds__(kRangesThreads) void kRanges(const Key* __restrict__ keys,
                                                           const gsr_control* __restrict__ control, uint32_t tileCount,
-                                                          uint2* __restrict__ ranges) {
+                                                          uint2* __restrict__ ranges, uint32_t* __restrict__ zeroWords,
+                                                          uint32_t zeroCount) {
+    if (blockIdx.x == 0)
+        for (uint32_t w = threadIdx.x; w < zeroCount; w += kRangesThreads) zeroWords[w] = 0u;
     const uint32_t n = control->entries;
     for (uint32_t k = blockIdx.x * kRangesThreads + threadIdx.x; k < n; k += gridDim.x * kRangesThreads) {
         const uint32_t tile = tileOfKey(keys[k]);
@@ -234,24 +304,22 @@ __global__ __launch_bounds__(kRangesThreads) void kRanges(const Key* __restrict_
 
 }  // namespace
 
-void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
-                     const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
-                     uint64_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream) {
-    const uint32_t blocks = (count + kDupThreads - 1) / kDupThreads;
-    if (blocks == 0) return;
+// Global mode: at least one workgroup, so that the control block is written for an empty scene.
+void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, uint32_t count,
+                     uint32_t bounds, const BinGrid& grid, uint64_t* keys, uint32_t* values, gsr_control* control,
+                     hipStream_t stream) {
+    const uint32_t blocks = count > 0 ? (count + kDupThreads - 1) / kDupThreads : 1u;
     hipLaunchKernelGGL((kDuplicate<false, uint64_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
-                       nullptr, blockOffsets, control, count, bounds, grid, keys, values, nullptr, controlRW);
+                       nullptr, nullptr, nullptr, frame, count, bounds, grid, keys, values, nullptr, control);
     markKernel("kDuplicate", stream);
 }
 
-void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const uint32_t* blockOffsets,
-                          const gsr_control* control, uint32_t count, uint32_t bounds, const BinGrid& grid,
-                          uint32_t* keys, uint32_t* values, uint32_t* depthBits, gsr_control* controlRW,
-                          hipStream_t stream) {
-    const uint32_t blocks = (count + kDupThreads - 1) / kDupThreads;
-    if (blocks == 0) return;
+void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, uint32_t count,
+                          uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values, uint32_t* depthBits,
+                          gsr_control* control, hipStream_t stream) {
+    const uint32_t blocks = count > 0 ? (count + kDupThreads - 1) / kDupThreads : 1u;
     hipLaunchKernelGGL((kDuplicate<false, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
-                       nullptr, blockOffsets, control, count, bounds, grid, keys, values, depthBits, controlRW);
+                       nullptr, nullptr, nullptr, frame, count, bounds, grid, keys, values, depthBits, control);
     markKernel("kDuplicate", stream);
 }
 
@@ -262,28 +330,30 @@ void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, 
     const uint32_t blocks = (maxVisible + kDupThreads - 1) / kDupThreads;
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<true, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, sortedCounts,
-                       order, blockOffsets, control, maxVisible, bounds, grid, keys, values, nullptr, controlRW);
+                       order, blockOffsets, control, DupFrame{}, maxVisible, bounds, grid, keys, values, nullptr,
+                       controlRW);
     markKernel("kDuplicate", stream);
 }
 
 template <typename Key>
 void launchRangesT(const Key* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount, uint2* ranges,
-                   hipStream_t stream) {
+                   uint32_t* zeroWords, uint32_t zeroCount, hipStream_t stream) {
     uint32_t blocks = (capacity + kRangesThreads - 1) / kRangesThreads;
     if (blocks > 2048) blocks = 2048;
-    if (blocks == 0) return;
-    hipLaunchKernelGGL(kRanges<Key>, dim3(blocks), dim3(kRangesThreads), 0, stream, keys, control, tileCount, ranges);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(kRanges<Key>, dim3(blocks), dim3(kRangesThreads), 0, stream, keys, control, tileCount, ranges,
+                       zeroWords, zeroCount);
     markKernel("kRanges", stream);
 }
 
 void launchRanges(const uint64_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
-                  uint2* ranges, hipStream_t stream) {
-    launchRangesT(keys, control, capacity, tileCount, ranges, stream);
+                  uint2* ranges, uint32_t* zeroWords, uint32_t zeroCount, hipStream_t stream) {
+    launchRangesT(keys, control, capacity, tileCount, ranges, zeroWords, zeroCount, stream);
 }
 
 void launchRanges32(const uint32_t* keys, const gsr_control* control, uint32_t capacity, uint32_t tileCount,
-                    uint2* ranges, hipStream_t stream) {
-    launchRangesT(keys, control, capacity, tileCount, ranges, stream);
+                    uint2* ranges, uint32_t* zeroWords, uint32_t zeroCount, hipStream_t stream) {
+    launchRangesT(keys, control, capacity, tileCount, ranges, zeroWords, zeroCount, stream);
 }
 
 }  // namespace gsr
