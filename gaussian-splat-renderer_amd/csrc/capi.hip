@@ -1036,6 +1036,13 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
     const uint32_t superWords = superStats ? numSuper * uint32_t(sizeof(BlockStats) / 4) : 0u;
     int which = 0;
     const uint32_t* sortedValues = nullptr;
+    // The per-tile depth sort runs inside the culled 16-px render (its workgroup sorts its tile
+    // first) when that render draws the frame now; otherwise as its own pass.
+    const bool fuseSort = r->tileSort && !deferred && !r->settings.differentiable && r->settings.sort_window == 0 &&
+                          r->settings.render_loop == GSR_RENDER_LOOP_CULLED &&
+                          r->settings.render_pixels_per_thread == 1 && !r->settings.render_exact_alpha &&
+                          r->binSize == uint32_t(kTileSize);
+    FusedSort fused{};
     if (r->binning) {
         // Binning (splat_tile_renderer.cpp:680-760): compact -> depth sort -> gather + scan
         // of the depth-ordered counts -> duplicate (tile keys) -> stable tile sort.
@@ -1083,10 +1090,14 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         launchRanges32(which ? r->keysB.as<uint32_t>() : r->keysA.as<uint32_t>(), control, r->capacity, tiles,
                        r->ranges.as<uint2>(), reinterpret_cast<uint32_t*>(superStats), superWords, stream);
         GSR_HIP(hipGetLastError());
-        launchTileDepthSort(vals, r->depthBits.as<uint32_t>(),
-                            which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>(), r->ranges.as<uint2>(),
-                            tileOrder, tiles, stream);
-        GSR_HIP(hipGetLastError());
+        uint32_t* scratch = which ? r->valuesA.as<uint32_t>() : r->valuesB.as<uint32_t>();
+        if (fuseSort) {
+            fused = FusedSort{vals, r->depthBits.as<uint32_t>(), scratch};
+        } else {
+            launchTileDepthSort(vals, r->depthBits.as<uint32_t>(), scratch, r->ranges.as<uint2>(), tileOrder, tiles,
+                                stream);
+            GSR_HIP(hipGetLastError());
+        }
         GSR_HIP(mark(5));
         sortedValues = vals;
     } else {
@@ -1125,7 +1136,7 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
                      r->ranges.as<uint2>(), dc, sc->count, r->settings.format,
                      r->settings.render_pixels_per_thread, r->settings.render_exact_alpha != 0,
                      r->settings.render_loop, int(r->settings.sort_window), r->rayDepth.as<float4>(), outImage,
-                     tileStats, int(r->settings.render_batch), stream);
+                     tileStats, int(r->settings.render_batch), stream, fuseSort ? &fused : nullptr);
     }
     GSR_HIP(hipGetLastError());
     GSR_HIP(mark(7));

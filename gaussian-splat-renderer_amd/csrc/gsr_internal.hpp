@@ -190,6 +190,10 @@ void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, 
                             uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values,
                             gsr_control* controlRW, hipStream_t stream);
 // What the culled render of one frame's tiles reads and writes (16-px tiles, tilesX per row).
+// sortValues != nullptr: the list is grouped by tile but each tile's range is not yet in depth
+// order; the render's workgroup sorts its tile's range in place first (sortValues = values; the
+// depth bits of gaussian g in depthBits[g]; sortScratch: a list-sized scratch for ranges over 4096
+// entries), as the tile depth sort would (tilesort.hpp).
 struct TileJob {
     const float4* splats;
     const uint32_t* packed;
@@ -198,6 +202,15 @@ struct TileJob {
     uint32_t width, height, tilesX, gaussianCount;
     void* out;
     uint32_t* tileStats;
+    uint32_t* sortValues = nullptr;
+    const uint32_t* depthBits = nullptr;
+    uint32_t* sortScratch = nullptr;
+};
+// The per-tile depth sort fused into the culled render (TileJob::sortValues).
+struct FusedSort {
+    uint32_t* values;
+    const uint32_t* depthBits;
+    uint32_t* scratch;
 };
 // One persistent culled render over the tiles of `views` frames (jobs and tileStart on the
 // device: tileStart[v] = the tiles of views 0..v-1, tileStart[views] = all); `counter`: one device
@@ -208,10 +221,12 @@ void launchRenderViews(const TileJob* jobs, const uint32_t* tileStart, uint32_t 
 // tileStats (culled loop only; null = none): 8 counters per tile, zeroed by the caller (the
 // reference's TileStat, splat_tile_renderer.h:130-141).
 constexpr int kTileStatCount = 8;
+// fused != nullptr: the culled 16-px render (loop CULLED, one pixel per thread, fast alpha, no
+// window) sorts each tile's range by depth itself (TileJob::sortValues).
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
-                  uint32_t* tileStats, int batch, hipStream_t stream);
+                  uint32_t* tileStats, int batch, hipStream_t stream, const FusedSort* fused = nullptr);
 
 // Learned colour (mlp.hip): a 16-value latent per gaussian and one shared MLP. Parameters:
 // mlpParamCount() floats; packs: mlpPackedTiles() x 64 uint2 (the bf16 A-operand tiles, rebuilt

@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "gsr_device.hpp"
+#include "tilesort.hpp"
 
 namespace gsr {
 
@@ -361,13 +362,53 @@ struct CulledBatch {
     uint32_t mask[B];                    // bit w: the splat can reach wave w's quadrant
     __align__(16) uint32_t list[4][B + 4];
 };
+// The culled render's LDS: the batch, or, before the tile's first batch, the fused depth sort's
+// words (kLdsChunk of them for the long form).
+template <int B>
+union CulledShared {
+    CulledBatch<B> batch;
+    uint64_t sortWords[kLdsChunk];
+};
+
+// The fused per-tile depth sort: the tile's range of the list by (depth bits, gaussian index), in
+// place, by the whole workgroup (a range of at most 256 by wave 0 alone), then a barrier that
+// hands the LDS to the render. Every thread of the workgroup calls it.
+__device__ __forceinline__ void sortTileFused(const TileJob& job, uint2 range, uint32_t wave, uint32_t lane,
+                                              uint64_t* sW) {
+    const uint32_t n = range.y - range.x;
+    if (n <= 1u) return;  // uniform
+    uint32_t* values = job.sortValues;
+    const uint32_t* depth = job.depthBits;
+    if (n <= 256u) {
+        if (wave == 0) {
+            if (n <= 64u) waveSortRange<1>(values, depth, range.x, n, lane);
+            else if (n <= 128u) waveSortRange<2>(values, depth, range.x, n, lane);
+            else waveSortRange<4>(values, depth, range.x, n, lane);
+        }
+    } else if (n <= 512u) {
+        groupSortRegisters<2>(values, depth, range, wave, lane, sW);
+    } else if (n <= 1024u) {
+        groupSortRegisters<4>(values, depth, range, wave, lane, sW);
+    } else if (n <= 2048u) {
+        groupSortRegisters<8>(values, depth, range, wave, lane, sW);
+    } else if (n <= 4096u) {
+        groupSortRegisters<16>(values, depth, range, wave, lane, sW);
+    } else {
+        groupSortLong(values, depth, job.sortScratch, range, sW);
+    }
+    __syncthreads();
+}
 
 // One 16x16 tile by the culled loop (the whole workgroup of 256 threads: it waits at barriers).
-template <int FORMAT, int RS, bool STATS, int B>
-__device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t tile, CulledBatch<B>& sh) {
+// FUSED: the code of the fused depth sort is compiled in (it runs when job.sortValues is set).
+template <int FORMAT, int RS, bool STATS, int B, bool FUSED>
+__device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t tile, CulledShared<B>& shm) {
     static_assert(B % 64 == 0 && B >= 64 && B <= 512, "batches of 64..512 splats");
     constexpr uint32_t kFetch = (B + 255) / 256;  // slots each thread stages
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint2 range = job.ranges[tile];
+    if (FUSED && job.sortValues) sortTileFused(job, range, wave, lane, shm.sortWords);  // uniform
+    CulledBatch<B>& sh = shm.batch;
     const uint32_t tx0 = (tile % job.tilesX) * 16, ty0 = (tile / job.tilesX) * 16;
     const uint32_t px = tx0 + (wave & 1u) * 8 + (lane & 7u);
     const uint32_t py = ty0 + (wave >> 1) * 8 + (lane >> 3);
@@ -384,7 +425,6 @@ __device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t ti
     float thr = (px < job.width && py < job.height) ? kLog2MinAlpha : __builtin_inff();
     const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
 
-    const uint2 range = job.ranges[tile];
     uint32_t statFetched = 0, statTermEntry = 0xFFFFFFFFu, statBlended = 0, statSlots = 0;
     for (uint32_t batch = range.x; batch < range.y; batch += B) {
         // Whole-tile vote; also the barrier that protects the previous batch's LDS.
@@ -509,7 +549,7 @@ template <int FORMAT, int RS>
 __global__ __launch_bounds__(256, 8) void kRenderViews(const TileJob* __restrict__ jobs,
                                                     const uint32_t* __restrict__ tileStart, uint32_t views,
                                                     uint32_t* __restrict__ counter) {
-    __shared__ CulledBatch<kBatch> sh;
+    __shared__ CulledShared<kBatch> sh;
     __shared__ uint32_t sItem;
     const uint32_t total = tileStart[views];
     for (;;) {
@@ -523,7 +563,7 @@ __global__ __launch_bounds__(256, 8) void kRenderViews(const TileJob* __restrict
             if (tileStart[mid] <= item) lo = mid;
             else hi = mid;
         }
-        renderTileCulled<FORMAT, RS, false, kBatch>(jobs[lo], item - tileStart[lo], sh);
+        renderTileCulled<FORMAT, RS, false, kBatch, false>(jobs[lo], item - tileStart[lo], sh);
         __syncthreads();  // every thread has read sItem and the batch before they are reused
     }
 }
@@ -533,12 +573,12 @@ __global__ void kZeroCounter(uint32_t* counter) { *counter = 0u; }
 // The culled loop over a frame's tiles: workgroup b takes tile rows of XCD b % 8 centre-out.
 template <int FORMAT, int RS, bool STATS, int B>
 __global__ __launch_bounds__(256, B == 512 ? 4 : 8) void kRenderCulled(TileJob job, uint32_t tileCount) {
-    __shared__ CulledBatch<B> sh;
+    __shared__ CulledShared<B> sh;
     const uint32_t xcd = blockIdx.x % kXcds, idx = blockIdx.x / kXcds;
     const uint32_t row = xcd + kXcds * centreOut(idx / job.tilesX, gridDim.x / kXcds / job.tilesX);
     const uint32_t tile = row * job.tilesX + idx % job.tilesX;
     if (tile >= tileCount) return;  // uniform
-    renderTileCulled<FORMAT, RS, STATS, B>(job, tile, sh);
+    renderTileCulled<FORMAT, RS, STATS, B, true>(job, tile, sh);
 }
 
 // 8x8 bins (tile_size 8, tile_render.comp with an 8x8 workgroup): each wave renders one bin from
@@ -1026,11 +1066,16 @@ void launchStp(int window, const float4* splats, const uint32_t* packed, const f
 template <int FORMAT, int RS>
 void launchCulled(const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
                   const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, int batch,
-                  hipStream_t stream) {
+                  hipStream_t stream, const FusedSort* fused) {
     const uint32_t tileCount = cam.tilesX * cam.tilesY;
     const uint32_t rowsPerXcd = (cam.tilesY + kXcds - 1) / kXcds;
     const dim3 grid(rowsPerXcd * cam.tilesX * kXcds);
-    const TileJob job{splats, packed, values, ranges, cam.width, cam.height, cam.tilesX, gaussianCount, out, tileStats};
+    TileJob job{splats, packed, values, ranges, cam.width, cam.height, cam.tilesX, gaussianCount, out, tileStats};
+    if (fused) {
+        job.sortValues = fused->values;
+        job.depthBits = fused->depthBits;
+        job.sortScratch = fused->scratch;
+    }
     if (tileStats)  // the statistics build: the default batch only
         hipLaunchKernelGGL((kRenderCulled<FORMAT, RS, true, kBatch>), grid, dim3(256), 0, stream, job, tileCount);
     else if (batch == 128)
@@ -1057,7 +1102,7 @@ void launchOne(const float4* splats, const uint32_t* packed, const uint32_t* val
 template <int FORMAT, int RS>
 void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uint32_t* packed, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats,
-                  int batch, hipStream_t stream) {
+                  int batch, hipStream_t stream, const FusedSort* fused) {
     if (exact) {
         if (ppt == 4)
             launchOne<FORMAT, false, 4, false, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
@@ -1074,7 +1119,8 @@ void launchFormat(int ppt, bool exact, int loop, const float4* splats, const uin
     } else if (loop == GSR_RENDER_LOOP_PREDICATED) {
         launchOne<FORMAT, true, 1, true, RS>(splats, packed, values, ranges, cam, gaussianCount, out, stream);
     } else {
-        launchCulled<FORMAT, RS>(splats, packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream);
+        launchCulled<FORMAT, RS>(splats, packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream,
+                                 fused);
     }
 }
 
@@ -1082,7 +1128,7 @@ template <int FORMAT>
 void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow, const float4* rayDepth,
                  const float4* splats, const uint32_t* packed, const uint32_t* values, const uint2* ranges,
                  const DevCamera& cam, uint32_t gaussianCount, void* out, uint32_t* tileStats, int batch,
-                 hipStream_t stream) {
+                 hipStream_t stream, const FusedSort* fused) {
     if (sortWindow > 0) {
         if (renderSplat == GSR_RENDER_SPLAT_P32)
             launchStp<FORMAT, GSR_RENDER_SPLAT_P32>(sortWindow, splats, packed, rayDepth, values, ranges, cam,
@@ -1097,13 +1143,13 @@ void launchSplat(int renderSplat, int ppt, bool exact, int loop, int sortWindow,
     }
     if (renderSplat == GSR_RENDER_SPLAT_P32)
         launchFormat<FORMAT, GSR_RENDER_SPLAT_P32>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
-                                                   out, tileStats, batch, stream);
+                                                   out, tileStats, batch, stream, fused);
     else if (renderSplat == GSR_RENDER_SPLAT_P24)
         launchFormat<FORMAT, GSR_RENDER_SPLAT_P24>(ppt, exact, loop, splats, packed, values, ranges, cam, gaussianCount,
-                                                   out, tileStats, batch, stream);
+                                                   out, tileStats, batch, stream, fused);
     else
         launchFormat<FORMAT, GSR_RENDER_SPLAT_FULL>(ppt, exact, loop, splats, packed, values, ranges, cam,
-                                                    gaussianCount, out, tileStats, batch, stream);
+                                                    gaussianCount, out, tileStats, batch, stream, fused);
 }
 
 }  // namespace
@@ -1130,7 +1176,7 @@ void launchRenderViews(const TileJob* jobs, const uint32_t* tileStart, uint32_t 
 void launchRender(const float4* splats, const uint32_t* packed, int renderSplat, const uint32_t* values,
                   const uint2* ranges, const DevCamera& cam, uint32_t gaussianCount, gsr_target_format format,
                   int pixelsPerThread, bool exactAlpha, int loop, int sortWindow, const float4* rayDepth, void* out,
-                  uint32_t* tileStats, int batch, hipStream_t stream) {
+                  uint32_t* tileStats, int batch, hipStream_t stream, const FusedSort* fused) {
     if (cam.tilesX * cam.tilesY == 0) return;
     if (cam.binSize == 8) {  // 8x8 bins: the culled loop only (the renderer refuses the other forms)
         const uint32_t bins = cam.binsX * cam.binsY;
@@ -1155,10 +1201,11 @@ void launchRender(const float4* splats, const uint32_t* packed, int renderSplat,
     }
     if (format == GSR_TARGET_RGBA8)
         launchSplat<GSR_TARGET_RGBA8>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
-                                      packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream);
+                                      packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream, fused);
     else
         launchSplat<GSR_TARGET_RGBA32F>(renderSplat, pixelsPerThread, exactAlpha, loop, sortWindow, rayDepth, splats,
-                                        packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream);
+                                        packed, values, ranges, cam, gaussianCount, out, tileStats, batch, stream,
+                                        fused);
 }
 
 }  // namespace gsr

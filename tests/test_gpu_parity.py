@@ -767,7 +767,8 @@ def test_tile_stats_match_oracle_counts(gsr_ctx, sort_mode):
 def test_kernel_times(gsr_ctx):
     """profile_stages=2: one event after every kernel of the frame; the kernels appear in launch
     order and their times add up to the frame's stage total. Global mode has no frame-setup scan
-    (the duplicate pass takes its block offsets from the preprocess's superblock sums)."""
+    (the duplicate pass takes its block offsets from the preprocess's superblock sums), and with
+    the culled render no tile depth sort pass (the render's workgroups sort their own tiles)."""
     import gsr
     from gsr import camera, scenes
 
@@ -776,14 +777,14 @@ def test_kernel_times(gsr_ctx):
     scene = gsr.Scene(gsr_ctx, sc.pos_alpha, sc.scales, sc.rotations, sc.sh, sc.sh_degree)
     cam = _camera(gsr, camera.orbit_view(8.0, 0.0, 0.0), 1280, 720)
     for mode, expect in (("global", ["kPreprocess", "kDuplicate", "kSortHistogram", "kSortScan",
-                                     "kSortScatter", "kRanges", "kTileDepthSort", "kRenderCulled"]),
+                                     "kSortScatter", "kRanges", "kRenderCulled"]),
                          ("binning", ["kPreprocess", "kFrameSetup", "kCompactVisible", "kGatherCounts",
                                       "kDuplicate", "kRanges", "kRenderCulled"])):
         r = gsr.Renderer(gsr_ctx, scene, target="rgba8", sort_mode=mode, profile_stages=2)
         r.render_to_host(cam)
         kt = r.kernel_times()
         names = [n for n, _ in kt]
-        assert mode == "binning" or "kFrameSetup" not in names, names
+        assert mode == "binning" or ("kFrameSetup" not in names and "kTileDepthSort" not in names), names
         pos = [names.index(k) for k in expect]
         assert pos == sorted(pos), names
         assert all(ms >= 0.0 for _, ms in kt)
@@ -798,7 +799,8 @@ def test_tile_depth_sort_long_ranges(gsr_ctx):
     cloud (means ~ N(0, 1.5^2 I)) at 640x360 from radius 7 has ranges up to 2048 entries (one wave,
     registers), 2049..4096 and 4097..8192 (2 or 4 waves, registers with cross-wave stages through
     LDS) and over 8192 (LDS chunks and merge rounds): lists and ranges bit-exact against the oracle
-    and equal to the full 64-bit key sort."""
+    and equal to the full 64-bit key sort, both for the sort inside the culled render (the default)
+    and for the tile depth sort pass (the other render loops)."""
     import gsr
     import oracle
     from gsr import camera, scenes
@@ -819,6 +821,11 @@ def test_tile_depth_sort_long_ranges(gsr_ctx):
     for a, b in zip(full.read_back()[:4], r.read_back()[:4]):
         assert np.array_equal(a, b)
     full.close()
+    sep = gsr.Renderer(gsr_ctx, scene, bounds="tight", target="rgba32f", sort_mode="global", render_loop="predicated")
+    sep.render_to_host(cam)
+    for a, b in zip(sep.read_back()[:4], r.read_back()[:4]):
+        assert np.array_equal(a, b)
+    sep.close()
     r.close()
     scene.close()
 
