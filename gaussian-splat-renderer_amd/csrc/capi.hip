@@ -205,6 +205,9 @@ struct gsr_renderer {
     int32_t degree = 0;
     // Per gaussian (+ the packed render splats in the p32 / p24 modes).
     DeviceBuffer splats, tileCounts, blockOffsets, blockStats, control, packed;
+    // Per gaussian with tiles: its coverage as the duplicate pass walks it (4-B info + 2 planes of
+    // 16 B: the first 8 row spans), written by the preprocess.
+    DeviceBuffer spanInfo, spanPlanes;
     DeviceBuffer rayDepth;  // StopThePop: 3 float4 per gaussian
     // Differentiable renderers: per pixel (T bits, last contributor) of the last frame, sized for
     // the largest camera seen; per gaussian the screen-space gradient planes of gsr_backward.
@@ -800,6 +803,8 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
     if (s.sort_window > 0) alloc(r->rayDepth, n * 48);
     if (s.differentiable) alloc(r->grad2D, n * 4 * size_t(grad2DComponents()));
     alloc(r->tileCounts, n * 4);
+    alloc(r->spanInfo, n * 4);
+    alloc(r->spanPlanes, n * 32);
     alloc(r->blockOffsets, gBlocks * 4);
     alloc(r->control, sizeof(gsr_control));
     if (r->binning) {
@@ -884,7 +889,7 @@ uint64_t gsr_renderer_total_bytes(const gsr_renderer* r) {
                                   &r->keysA,  &r->keysB,      &r->valuesA,      &r->valuesB,    &r->sortSmall,
                                   &r->sortStatus, &r->ranges, &r->depthKeysA, &r->depthKeysB, &r->depthValuesA,
                                   &r->depthValuesB, &r->sortedCounts, &r->visibleSums, &r->packed, &r->rayDepth,
-                                  &r->pixelState, &r->grad2D, &r->tileStats, &r->depthBits, &r->superStats};
+                                  &r->pixelState, &r->grad2D, &r->tileStats, &r->depthBits, &r->superStats, &r->spanInfo, &r->spanPlanes};
     uint64_t s = 0;
     for (const DeviceBuffer* b : bufs) s += b->bytes;
     for (const auto& o : r->tileOrders) s += o->table.bytes;
@@ -1012,7 +1017,8 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
     BlockStats* superStats = r->binning ? nullptr : r->superStats.as<BlockStats>();
     const uint32_t numSuper = superBlockCount(sc->count);
     launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
-                     r->blockStats.as<BlockStats>(), superStats, control, r->ranges.as<uint32_t>(), rangeWords,
+                     r->blockStats.as<BlockStats>(), superStats, control, r->spanInfo.as<uint32_t>(),
+                     r->spanPlanes.as<uint4>(), r->ranges.as<uint32_t>(), rangeWords,
                      uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), r->rayDepth.as<float4>(),
                      r->colour ? r->colour->useMlp.as<uint8_t>() : nullptr, stream);
     GSR_HIP(hipGetLastError());
@@ -1062,7 +1068,8 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         launchScanVisibleBlocks(r->visibleSums.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         launchDuplicateBinning(r->splats.as<float4>(), r->sortedCounts.as<uint32_t>(), order,
-                               r->visibleSums.as<uint32_t>(), control, sc->count, bounds, grid,
+                               r->visibleSums.as<uint32_t>(), control, r->spanInfo.as<uint32_t>(),
+                               r->spanPlanes.as<uint4>(), sc->count, bounds, grid,
                                r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
@@ -1078,9 +1085,9 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         // (depth, index) in the tile depth sort: the list of the full 64-bit sort.
         GSR_HIP(mark(2));
         GSR_HIP(mark(3));
-        launchDuplicateTiles(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, sc->count, bounds, grid,
-                             r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), r->depthBits.as<uint32_t>(), control,
-                             stream);
+        launchDuplicateTiles(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, r->spanInfo.as<uint32_t>(),
+                             r->spanPlanes.as<uint4>(), sc->count, bounds, grid, r->keysA.as<uint32_t>(),
+                             r->valuesA.as<uint32_t>(), r->depthBits.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
         which = launchRadixSort32(work, r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint32_t>(),
@@ -1105,8 +1112,9 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         // sort).
         GSR_HIP(mark(2));
         GSR_HIP(mark(3));
-        launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, sc->count, bounds, grid,
-                        r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), control, stream);
+        launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, r->spanInfo.as<uint32_t>(),
+                        r->spanPlanes.as<uint4>(), sc->count, bounds, grid, r->keysA.as<uint64_t>(),
+                        r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
         which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),

@@ -224,6 +224,49 @@ __device__ __forceinline__ void coverageRowSpan(const Coverage& cv, int row, int
     *x1 = hi2 > lo2 ? hi2 : lo2;
 }
 
+// The splat's coverage as the duplicate pass walks it: the first bin row, the row count (negative,
+// -(rows) - 1, for a rectangle: one span for every row) and the first kSpanRows row spans
+// (x0 | width << 16; a rectangle: span 0). Written by the preprocess with the tile count, so the
+// duplicate pass need not set the coverage up again.
+constexpr int kSpanRows = 8;
+struct SpanRecord {
+    uint32_t info;  // first row (low 16 bits) | int16 row code (high 16 bits)
+    uint32_t span[kSpanRows];
+};
+__device__ __forceinline__ uint32_t coverageCountSpans(const Coverage& cv, int S, SpanRecord& rec) {
+    int r0, r1;
+    coverageRows(cv, S, &r0, &r1);
+#pragma unroll
+    for (int k = 0; k < kSpanRows; ++k) rec.span[k] = 0u;
+    if (!cv.useEllipse) {
+        const int w = cv.bins.x1 - cv.bins.x0;
+        rec.info = uint32_t(r0 & 0xFFFF) | (uint32_t(uint16_t(int16_t(-(r1 - r0) - 1))) << 16);
+        rec.span[0] = uint32_t(cv.bins.x0) | (uint32_t(w > 0 ? w : 0) << 16);
+        return uint32_t((w > 0 ? w : 0) * (r1 - r0));
+    }
+    rec.info = uint32_t(r0 & 0xFFFF) | (uint32_t(uint16_t(int16_t(r1 - r0))) << 16);
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < kSpanRows / 2; ++k) {  // the first rows straight into their slots
+        if (r0 + k < r1) {
+            int x0, x1;
+            coverageRowSpan(cv, r0 + k, S, &x0, &x1);
+            rec.span[k] = uint32_t(x0) | (uint32_t(x1 - x0) << 16);
+            n += uint32_t(x1 - x0);
+        }
+    }
+    for (int r = r0 + kSpanRows / 2; r < r1; ++r) {  // taller splats
+        int x0, x1;
+        coverageRowSpan(cv, r, S, &x0, &x1);
+        const uint32_t sp = uint32_t(x0) | (uint32_t(x1 - x0) << 16);
+#pragma unroll
+        for (int k = kSpanRows / 2; k < kSpanRows; ++k)
+            if (r - r0 == k) rec.span[k] = sp;
+        n += uint32_t(x1 - x0);
+    }
+    return n;
+}
+
 __device__ __forceinline__ uint32_t coverageCount(const Coverage& cv, int S) {
     int r0, r1;
     coverageRows(cv, S, &r0, &r1);

@@ -22,6 +22,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                                   BlockStats* __restrict__ blockStats,
                                                                   BlockStats* __restrict__ superStats,
                                                                   gsr_control* __restrict__ control,
+                                                                  uint32_t* __restrict__ spanInfo,
+                                                                  uint4* __restrict__ spanPlanes,
                                                                   uint32_t* __restrict__ rangesWords,
                                                                   uint32_t rangeWordCount, uint32_t renderSplat,
                                                                   uint32_t* __restrict__ packed,
@@ -192,12 +194,25 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                     reason = GSR_CULL_VISIBLE;
                     // The bins the duplicate pass will write (splat_preprocess.comp:291-308): INRIA's
                     // rectangle count on 16-px bins with rect bounds, else the coverage count.
+                    SpanRecord spans;
                     if (bounds == GSR_BOUNDS_RECT && cam.binSize == uint32_t(kTileSize)) {
                         touched = uint32_t((rect.x1 - rect.x0) * (rect.y1 - rect.y0));
+                        spans.info = uint32_t(rect.y0) |
+                                     (uint32_t(uint16_t(int16_t(-(rect.y1 - rect.y0) - 1))) << 16);
+#pragma unroll
+                        for (int k = 0; k < kSpanRows; ++k) spans.span[k] = 0u;
+                        spans.span[0] = uint32_t(rect.x0) | (uint32_t(rect.x1 - rect.x0) << 16);
                     } else {
                         const Coverage cv = makeCoverage(bounds, rect, int(cam.binSize), int(cam.binsX),
                                                          int(cam.binsY), ctrX, ctrY, ra, rb, rc, ro);
-                        touched = coverageCount(cv, int(cam.binSize));
+                        touched = coverageCountSpans(cv, int(cam.binSize), spans);
+                    }
+                    if (touched != 0u) {  // the duplicate pass reads them for its listed gaussians
+                        spanInfo[i] = spans.info;
+                        spanPlanes[i] = make_uint4(spans.span[0], spans.span[1], spans.span[2], spans.span[3]);
+                        if (int(int16_t(uint16_t(spans.info >> 16))) > kSpanRows / 2)  // the second plane if used
+                            spanPlanes[size_t(scene.count) + i] =
+                                make_uint4(spans.span[4], spans.span[5], spans.span[6], spans.span[7]);
                     }
                     if (rayDepth) {
                         // StopThePop (computeRayDepthParams, splat_math.cpp:195-205): P = Sigma^-1 in
@@ -295,18 +310,21 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
 
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, BlockStats* superStats, gsr_control* control,
-                      uint32_t* rangesWords, uint32_t rangeWordCount, uint32_t renderSplat, uint32_t* packed,
-                      float4* rayDepth, const uint8_t* useMlp, hipStream_t stream) {
+                      uint32_t* spanInfo, uint4* spanPlanes, uint32_t* rangesWords, uint32_t rangeWordCount,
+                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, const uint8_t* useMlp,
+                      hipStream_t stream) {
     uint32_t n = scene.count > rangeWordCount ? scene.count : rangeWordCount;
     const uint32_t blocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene,
-                           cam, bounds, splats, tileCounts, blockStats, superStats, control, rangesWords,
+                           cam, bounds, splats, tileCounts, blockStats, superStats, control, spanInfo, spanPlanes,
+                           rangesWords,
                            rangeWordCount, renderSplat, packed, rayDepth, useMlp);
     else
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
-                           bounds, splats, tileCounts, blockStats, superStats, control, rangesWords, rangeWordCount,
+                           bounds, splats, tileCounts, blockStats, superStats, control, spanInfo, spanPlanes,
+                           rangesWords, rangeWordCount,
                            renderSplat, packed, rayDepth, useMlp);
     markKernel("kPreprocess", stream);
 }

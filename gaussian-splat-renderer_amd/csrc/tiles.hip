@@ -20,7 +20,7 @@ namespace {
 
 constexpr int kDupThreads = kPreprocessThreads;
 
-constexpr int kCachedRows = 8;  // row spans kept in LDS per gaussian; longer splats walk the rest
+constexpr int kCachedRows = kSpanRows;  // row spans kept in LDS per gaussian; longer splats walk the rest
 
 // BINNING = false: block b = gaussians [256 b, 256 b + 256) in index order, 64-bit keys
 // (tile << 32 | depth bits), or 32-bit keys = tile id with each listed gaussian's depth bits in
@@ -69,6 +69,8 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                                                           const uint32_t* __restrict__ order,
                                                           const uint32_t* __restrict__ blockOffsets,
                                                           const gsr_control* __restrict__ control, DupFrame frame,
+                                                          const uint32_t* __restrict__ spanInfo,
+                                                          const uint4* __restrict__ spanPlanes,
                                                           uint32_t count, uint32_t bounds, BinGrid grid,
                                                           Key* __restrict__ keys, uint32_t* __restrict__ values,
                                                           uint32_t* __restrict__ depthBits, gsr_control* controlRW) {
@@ -144,7 +146,40 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     const bool work = !(base >= entries || total == 0);
     if (BINNING && !work) return;
 
+    // The coverage as the preprocess recorded it (first row, row code, the first kCachedRows
+    // spans); only a binning-mode splat of more rows sets its ellipse up again (for the rows past
+    // the cached ones, which its entries walk from LDS).
+    bool recorded = false;
     if (work && c != 0u) {
+        const uint32_t info = spanInfo[i];
+        const int code = int(int16_t(uint16_t(info >> 16)));
+        recorded = !BINNING || code <= kCachedRows;
+        if (recorded) {
+            const uint32_t depth = __float_as_uint(splats[3 * size_t(i)].z);
+            sRow0[t] = int(info & 0xFFFFu);
+            sRows[t] = code;
+            sDepth[t] = depth;
+            if (!BINNING && sizeof(Key) == 4) depthBits[i] = depth;
+            const uint4 p0 = spanPlanes[i];
+            if (code < 0) {  // a rectangle: x0 and width of every row
+                sBinX[0][t] = int(p0.x & 0xFFFFu);
+                sBinX[1][t] = int(p0.x >> 16);
+            } else {
+                sSpan[0][t] = p0.x;
+                sSpan[1][t] = p0.y;
+                sSpan[2][t] = p0.z;
+                sSpan[3][t] = p0.w;
+                if (code > 4) {
+                    const uint4 p1 = spanPlanes[size_t(count) + i];
+                    sSpan[4][t] = p1.x;
+                    sSpan[5][t] = p1.y;
+                    sSpan[6][t] = p1.z;
+                    sSpan[7][t] = p1.w;
+                }
+            }
+        }
+    }
+    if (work && c != 0u && !recorded) {
         const float4* s = splats + 3 * size_t(i);
         const float4 o0 = s[0], o1 = s[1], o2 = s[2];
         const Rect rect = tileRect(o0.x, o0.y, o2.w, grid.tilesX, grid.tilesY);
@@ -305,33 +340,36 @@ __global__ __launch_bounds__(kRangesThreads) void kRanges(const Key* __restrict_
 }  // namespace
 
 // Global mode: at least one workgroup, so that the control block is written for an empty scene.
-void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, uint32_t count,
-                     uint32_t bounds, const BinGrid& grid, uint64_t* keys, uint32_t* values, gsr_control* control,
-                     hipStream_t stream) {
+void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, const uint32_t* spanInfo,
+                     const uint4* spanPlanes, uint32_t count, uint32_t bounds, const BinGrid& grid, uint64_t* keys,
+                     uint32_t* values, gsr_control* control, hipStream_t stream) {
     const uint32_t blocks = count > 0 ? (count + kDupThreads - 1) / kDupThreads : 1u;
     hipLaunchKernelGGL((kDuplicate<false, uint64_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
-                       nullptr, nullptr, nullptr, frame, count, bounds, grid, keys, values, nullptr, control);
+                       nullptr, nullptr, nullptr, frame, spanInfo, spanPlanes, count, bounds, grid, keys, values, nullptr,
+                       control);
     markKernel("kDuplicate", stream);
 }
 
-void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, uint32_t count,
-                          uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values, uint32_t* depthBits,
+void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame,
+                          const uint32_t* spanInfo, const uint4* spanPlanes, uint32_t count, uint32_t bounds,
+                          const BinGrid& grid, uint32_t* keys, uint32_t* values, uint32_t* depthBits,
                           gsr_control* control, hipStream_t stream) {
     const uint32_t blocks = count > 0 ? (count + kDupThreads - 1) / kDupThreads : 1u;
     hipLaunchKernelGGL((kDuplicate<false, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
-                       nullptr, nullptr, nullptr, frame, count, bounds, grid, keys, values, depthBits, control);
+                       nullptr, nullptr, nullptr, frame, spanInfo, spanPlanes, count, bounds, grid, keys, values,
+                       depthBits, control);
     markKernel("kDuplicate", stream);
 }
 
 void launchDuplicateBinning(const float4* splats, const uint32_t* sortedCounts, const uint32_t* order,
-                            const uint32_t* blockOffsets, const gsr_control* control, uint32_t maxVisible,
-                            uint32_t bounds, const BinGrid& grid, uint32_t* keys, uint32_t* values,
-                            gsr_control* controlRW, hipStream_t stream) {
+                            const uint32_t* blockOffsets, const gsr_control* control, const uint32_t* spanInfo,
+                            const uint4* spanPlanes, uint32_t maxVisible, uint32_t bounds, const BinGrid& grid,
+                            uint32_t* keys, uint32_t* values, gsr_control* controlRW, hipStream_t stream) {
     const uint32_t blocks = (maxVisible + kDupThreads - 1) / kDupThreads;
     if (blocks == 0) return;
     hipLaunchKernelGGL((kDuplicate<true, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, sortedCounts,
-                       order, blockOffsets, control, DupFrame{}, maxVisible, bounds, grid, keys, values, nullptr,
-                       controlRW);
+                       order, blockOffsets, control, DupFrame{}, spanInfo, spanPlanes, maxVisible, bounds, grid, keys,
+                       values, nullptr, controlRW);
     markKernel("kDuplicate", stream);
 }
 
