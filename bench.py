@@ -514,7 +514,8 @@ def main():
     # (histogram), then 8 B per entry and writes 8 B per entry (scatter); kRanges reads the keys
     # and writes 8 B per tile; the tile depth sort reads 4 B value + 4 B depth bits and writes 4 B
     # per entry; the render reads 4 B value + 48 B splat per fetched entry and 8 B per tile range
-    # and writes 4 B per pixel.
+    # and writes 4 B per pixel (in global mode it also sorts its tile first: the tile depth sort's
+    # 12 B per entry).
     n = args.count
     tiles = cams[0].tiles
     near, low_opacity = mean_culls[1], mean_culls[4]
@@ -531,7 +532,8 @@ def main():
             "kSortScatter": 2 * M * (key_b + 4) + blocks * 256 * 4,
             "kRanges": M * key_b + tiles * 8,
             "kTileDepthSort": M * 12,
-            "kRenderCulled": pairs["fetched"] * 52 + tiles * 8 + W * H * 4}
+            "kRenderCulled": pairs["fetched"] * 52 + tiles * 8 + W * H * 4 +
+                             (M * 12 if args.sort_mode == "global" else 0)}
     traffic_by_kernel = {}
     traffic_file = os.path.join(ROOT, "profiles", "pmc_traffic_1.2M-1080p-orbit.md")
     if os.path.exists(traffic_file) and not args.ply and args.count == 1_200_000 and args.sort_mode == "global":
@@ -571,15 +573,16 @@ def main():
                 "pairs_per_launch": {k: int(v) for k, v in pairs.items() if k not in ("max_depth",)},
                 "note": "achieved = (16 ops x evaluated + 10 ops x blended pairs of the reference's tile loop, "
                         "counted by the tile-stats build on the same frames) / launch time (hipEvents on the render "
-                        "stream); peak = MI355X FP32 157.3 TFLOP/s; traffic = rocprofv3 FETCH_SIZE + WRITE_SIZE per "
-                        "launch scaled by a plain copy (profiles/); lane_slots = (pixel, splat) evaluations this "
-                        "kernel's loop issued"}
+                        "stream; in global mode the launch also sorts each tile's range by depth, whose work is not "
+                        "counted in ops); peak = MI355X FP32 157.3 TFLOP/s; traffic = rocprofv3 FETCH_SIZE + "
+                        "WRITE_SIZE per launch scaled by a plain copy (profiles/); lane_slots = (pixel, splat) "
+                        "evaluations this kernel's loop issued"}
     if args.sort_mode == "binning":
         passes = {"depth_radix": 4, "tile_radix": (int(tiles).bit_length() + 7) // 8}
         sort_ms = stages["sort"] + stages["depth_sort"]
     else:
-        passes = {"tile_radix": (int(tiles).bit_length() + 7) // 8, "per_tile_depth_sort": 1}
-        sort_ms = stages["sort"]
+        passes = {"tile_radix": (int(tiles).bit_length() + 7) // 8, "per_tile_depth_sort_in_render": 1}
+        sort_ms = stages["sort"]  # the tile-id radix passes and ranges; the depth order within a tile is the render's
     full_key = None
     if args.sort_mode == "global":
         fpasses = (32 + int(tiles).bit_length() + 7) // 8
@@ -711,6 +714,10 @@ def main():
         "mean_visible": int(mean_visible),
         "sort_passes": passes,
         "sorted_keys_per_s": round(mean_entries / (sort_ms * 1e-3), 1),
+        "sorted_keys_note": ("list entries / sort stage time: the two 32-bit tile-id radix passes and the ranges; "
+                             "each tile's depth order is sorted inside the render (full_key_sort gives the keys/s of "
+                             "the whole 64-bit sort)") if args.sort_mode == "global" else
+                            "list entries / (depth sort + tile sort) stage time",
         "full_key_sort": full_key,
         "kernels": kernel_table,
         "sort_modes": modes,
