@@ -812,7 +812,7 @@ gsr_status gsr_renderer_create(gsr_context* ctx, const gsr_scene* scene, const g
             alloc(*b, n * 4);
         alloc(r->visibleSums, gBlocks * 4);
     }
-    if (r->tileSort) alloc(r->depthBits, n * 4);
+    if (!r->binning) alloc(r->depthBits, n * 4);
     if (!r->binning) alloc(r->superStats, size_t(superBlockCount(uint32_t(n))) * sizeof(BlockStats));
     alloc(r->sortSmall, (kMaxSortPasses * kRadix + kMaxSortPasses + size_t(kRadix) * kMaxSortBlocks) * 4);
     if (e == hipSuccess) e = allocateList(r, initialCapacity, ctx->stream);
@@ -1018,7 +1018,8 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
     const uint32_t numSuper = superBlockCount(sc->count);
     launchPreprocess(ref, dc, bounds, r->splats.as<float4>(), r->tileCounts.as<uint32_t>(),
                      r->blockStats.as<BlockStats>(), superStats, control, r->spanInfo.as<uint32_t>(),
-                     r->spanPlanes.as<uint4>(), r->ranges.as<uint32_t>(), rangeWords,
+                     r->spanPlanes.as<uint4>(), r->binning ? nullptr : r->depthBits.as<uint32_t>(),
+                     r->ranges.as<uint32_t>(), rangeWords,
                      uint32_t(r->settings.render_splat), r->packed.as<uint32_t>(), r->rayDepth.as<float4>(),
                      r->colour ? r->colour->useMlp.as<uint8_t>() : nullptr, stream);
     GSR_HIP(hipGetLastError());
@@ -1087,7 +1088,7 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         GSR_HIP(mark(3));
         launchDuplicateTiles(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, r->spanInfo.as<uint32_t>(),
                              r->spanPlanes.as<uint4>(), sc->count, bounds, grid, r->keysA.as<uint32_t>(),
-                             r->valuesA.as<uint32_t>(), r->depthBits.as<uint32_t>(), control, stream);
+                             r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
         which = launchRadixSort32(work, r->keysA.as<uint32_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint32_t>(),
@@ -1113,8 +1114,8 @@ static gsr_status renderFrame(gsr_renderer* r, const gsr_camera* cam, void* outI
         GSR_HIP(mark(2));
         GSR_HIP(mark(3));
         launchDuplicate(r->splats.as<float4>(), r->tileCounts.as<uint32_t>(), dupFrame, r->spanInfo.as<uint32_t>(),
-                        r->spanPlanes.as<uint4>(), sc->count, bounds, grid, r->keysA.as<uint64_t>(),
-                        r->valuesA.as<uint32_t>(), control, stream);
+                        r->spanPlanes.as<uint4>(), r->depthBits.as<uint32_t>(), sc->count, bounds, grid,
+                        r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), control, stream);
         GSR_HIP(hipGetLastError());
         GSR_HIP(mark(4));
         which = launchRadixSort(work, r->keysA.as<uint64_t>(), r->valuesA.as<uint32_t>(), r->keysB.as<uint64_t>(),

@@ -125,12 +125,13 @@ inline uint32_t superBlockCount(uint32_t count) {
     return (blocks + kSuperBlocks - 1) / kSuperBlocks;
 }
 // spanInfo / spanPlanes (2 planes of uint4, count each): per gaussian with tiles, its coverage as
-// the duplicate pass walks it (SpanRecord, gsr_device.hpp), written with the tile count.
+// the duplicate pass walks it (SpanRecord, gsr_device.hpp), written with the tile count; so is
+// depthBits[i] (the splat's depth as bits, the list's depth key; nullptr: not written).
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, BlockStats* superStats, gsr_control* control,
-                      uint32_t* spanInfo, uint4* spanPlanes, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, const uint8_t* useMlp,
-                      hipStream_t stream);
+                      uint32_t* spanInfo, uint4* spanPlanes, uint32_t* depthBits, uint32_t* rangesWords,
+                      uint32_t rangeWordCount, uint32_t renderSplat, uint32_t* packed, float4* rayDepth,
+                      const uint8_t* useMlp, hipStream_t stream);
 // scanVisible = false: blockOffsets = exclusive scan of the blocks' tile sums (global mode);
 // true: of their visible (non-zero count) gaussians (binning mode's compaction).
 void launchFrameSetup(const BlockStats* blockStats, uint32_t numBlocks, uint32_t gaussianCount, uint32_t capacity,
@@ -147,13 +148,13 @@ struct DupFrame {
     uint32_t numSuper, capacity;
 };
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, const uint32_t* spanInfo,
-                     const uint4* spanPlanes, uint32_t count, uint32_t bounds, const BinGrid& grid, uint64_t* keys,
-                     uint32_t* values, gsr_control* control, hipStream_t stream);
-// The same list with 32-bit keys = tile id; each listed gaussian's depth bits go to depthBits[i].
+                     const uint4* spanPlanes, const uint32_t* depthBits, uint32_t count, uint32_t bounds,
+                     const BinGrid& grid, uint64_t* keys, uint32_t* values, gsr_control* control, hipStream_t stream);
+// The same list with 32-bit keys = tile id (the depth keys stay in the preprocess's depthBits).
 void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame,
                           const uint32_t* spanInfo, const uint4* spanPlanes, uint32_t count, uint32_t bounds,
-                          const BinGrid& grid, uint32_t* keys, uint32_t* values, uint32_t* depthBits,
-                          gsr_control* control, hipStream_t stream);
+                          const BinGrid& grid, uint32_t* keys, uint32_t* values, gsr_control* control,
+                          hipStream_t stream);
 // Sorts by key bits [firstBit, keyBits) (stable). Returns which buffer (0 = keysA/valuesA,
 // 1 = B) holds the sorted result. *epoch is the caller's status-epoch counter (one per status
 // array), advanced once per pass.

@@ -24,6 +24,7 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                                                                   gsr_control* __restrict__ control,
                                                                   uint32_t* __restrict__ spanInfo,
                                                                   uint4* __restrict__ spanPlanes,
+                                                                  uint32_t* __restrict__ depthBits,
                                                                   uint32_t* __restrict__ rangesWords,
                                                                   uint32_t rangeWordCount, uint32_t renderSplat,
                                                                   uint32_t* __restrict__ packed,
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                         touched = coverageCountSpans(cv, int(cam.binSize), spans);
                     }
                     if (touched != 0u) {  // the duplicate pass reads them for its listed gaussians
+                        if (depthBits) depthBits[i] = __float_as_uint(o0.z);  // the list's depth keys
                         spanInfo[i] = spans.info;
                         spanPlanes[i] = make_uint4(spans.span[0], spans.span[1], spans.span[2], spans.span[3]);
                         if (int(int16_t(uint16_t(spans.info >> 16))) > kSpanRows / 2)  // the second plane if used
@@ -310,22 +312,20 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
 
 void launchPreprocess(const SceneRef& scene, const DevCamera& cam, uint32_t bounds, float4* splats,
                       uint32_t* tileCounts, BlockStats* blockStats, BlockStats* superStats, gsr_control* control,
-                      uint32_t* spanInfo, uint4* spanPlanes, uint32_t* rangesWords, uint32_t rangeWordCount,
-                      uint32_t renderSplat, uint32_t* packed, float4* rayDepth, const uint8_t* useMlp,
-                      hipStream_t stream) {
+                      uint32_t* spanInfo, uint4* spanPlanes, uint32_t* depthBits, uint32_t* rangesWords,
+                      uint32_t rangeWordCount, uint32_t renderSplat, uint32_t* packed, float4* rayDepth,
+                      const uint8_t* useMlp, hipStream_t stream) {
     uint32_t n = scene.count > rangeWordCount ? scene.count : rangeWordCount;
     const uint32_t blocks = (n + kPreprocessThreads - 1) / kPreprocessThreads;
     if (blocks == 0) return;
     if (scene.format == GSR_SCENE_F32_SOA)
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F32_SOA>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene,
                            cam, bounds, splats, tileCounts, blockStats, superStats, control, spanInfo, spanPlanes,
-                           rangesWords,
-                           rangeWordCount, renderSplat, packed, rayDepth, useMlp);
+                           depthBits, rangesWords, rangeWordCount, renderSplat, packed, rayDepth, useMlp);
     else
         hipLaunchKernelGGL(kPreprocess<GSR_SCENE_F16>, dim3(blocks), dim3(kPreprocessThreads), 0, stream, scene, cam,
                            bounds, splats, tileCounts, blockStats, superStats, control, spanInfo, spanPlanes,
-                           rangesWords, rangeWordCount,
-                           renderSplat, packed, rayDepth, useMlp);
+                           depthBits, rangesWords, rangeWordCount, renderSplat, packed, rayDepth, useMlp);
     markKernel("kPreprocess", stream);
 }
 

@@ -8,9 +8,9 @@
 // ellipse and its rows) is set up once into LDS; then the block's entries are handed out
 // to all 256 threads (load-balanced over big splats) and each finds its gaussian by an
 // 8-step binary search in LDS and its tile by walking that coverage's rows. Keys are
-// (tile << 32 | depth bits), or the tile id alone with the depth bits written once per
-// gaussian (depthBits[index]: the list then moves 8 B per entry through the tile sort, and the
-// per-tile depth sort looks the depth up); values the gaussian index; entries >= capacity are
+// (tile << 32 | depth bits), the depth bits read from the preprocess's depthBits[index], or the
+// tile id alone (the list then moves 8 B per entry through the tile sort, and the per-tile depth
+// sort looks the depth up in depthBits); values the gaussian index; entries >= capacity are
 // dropped (index order), the tile id #tiles marks an entry without a tile (never produced).
 #include "gsr_device.hpp"
 
@@ -73,11 +73,12 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
                                                           const uint4* __restrict__ spanPlanes,
                                                           uint32_t count, uint32_t bounds, BinGrid grid,
                                                           Key* __restrict__ keys, uint32_t* __restrict__ values,
-                                                          uint32_t* __restrict__ depthBits, gsr_control* controlRW) {
+                                                          const uint32_t* __restrict__ depthBits,
+                                                          gsr_control* controlRW) {
     __shared__ uint32_t sScan[kDupThreads];
     __shared__ uint32_t sWave[kDupThreads / 64];
     __shared__ uint32_t sFrame[3];  // global mode: superblocks before this block, earlier blocks of it, M
-    // Per gaussian: first row, row count, depth bits, and either the rectangle's x0/width
+    // Per gaussian: first row, row count, depth bits (64-bit keys), and either the rectangle's x0/width
     // (one span for every row) or the first kCachedRows row spans (x0 | width << 16). For the
     // rows past them (long splats) the binning form also keeps the ellipse: its blocks hold
     // depth-ordered splats, so the near, long ones come together and rebuilding their coverage
@@ -99,6 +100,16 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     const uint32_t c = j < live ? tileCounts[BINNING ? j : i] : 0u;  // binning: counts in depth order
     __shared__ uint32_t sIndex[kDupThreads];
     sIndex[t] = i;  // read after the barrier below
+    // Global mode: every gaussian's coverage record (and, for 64-bit keys, the preprocess's depth
+    // bits) is loaded with its count, before the block offsets are known (a gaussian without tiles
+    // holds a stale record, which is never used).
+    uint32_t rInfo = 0u, rDepth = 0u;
+    uint4 rPlane0 = make_uint4(0u, 0u, 0u, 0u);
+    if (!BINNING && j < live) {
+        rInfo = spanInfo[i];
+        rPlane0 = spanPlanes[i];
+        if (sizeof(Key) == 8) rDepth = depthBits[i];
+    }
 
     // Block exclusive scan of the counts (the downsweep of the reduce-then-scan).
     const uint32_t lane = t & 63u, wave = t >> 6;
@@ -151,16 +162,14 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
     // the cached ones, which its entries walk from LDS).
     bool recorded = false;
     if (work && c != 0u) {
-        const uint32_t info = spanInfo[i];
+        const uint32_t info = BINNING ? spanInfo[i] : rInfo;
         const int code = int(int16_t(uint16_t(info >> 16)));
         recorded = !BINNING || code <= kCachedRows;
         if (recorded) {
-            const uint32_t depth = __float_as_uint(splats[3 * size_t(i)].z);
             sRow0[t] = int(info & 0xFFFFu);
             sRows[t] = code;
-            sDepth[t] = depth;
-            if (!BINNING && sizeof(Key) == 4) depthBits[i] = depth;
-            const uint4 p0 = spanPlanes[i];
+            if (!BINNING && sizeof(Key) == 8) sDepth[t] = rDepth;
+            const uint4 p0 = BINNING ? spanPlanes[i] : rPlane0;
             if (code < 0) {  // a rectangle: x0 and width of every row
                 sBinX[0][t] = int(p0.x & 0xFFFFu);
                 sBinX[1][t] = int(p0.x >> 16);
@@ -189,8 +198,6 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
         coverageRows(cv, S, &r0, &r1);
         sRow0[t] = r0;
         sRows[t] = r1 - r0;
-        sDepth[t] = __float_as_uint(o0.z);
-        if (!BINNING && sizeof(Key) == 4) depthBits[i] = __float_as_uint(o0.z);
         if (!cv.useEllipse) {
             sBinX[0][t] = cv.bins.x0;
             sBinX[1][t] = cv.bins.x1 - cv.bins.x0;
@@ -341,23 +348,23 @@ __global__ __launch_bounds__(kRangesThreads) void kRanges(const Key* __restrict_
 
 // Global mode: at least one workgroup, so that the control block is written for an empty scene.
 void launchDuplicate(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame, const uint32_t* spanInfo,
-                     const uint4* spanPlanes, uint32_t count, uint32_t bounds, const BinGrid& grid, uint64_t* keys,
-                     uint32_t* values, gsr_control* control, hipStream_t stream) {
+                     const uint4* spanPlanes, const uint32_t* depthBits, uint32_t count, uint32_t bounds,
+                     const BinGrid& grid, uint64_t* keys, uint32_t* values, gsr_control* control, hipStream_t stream) {
     const uint32_t blocks = count > 0 ? (count + kDupThreads - 1) / kDupThreads : 1u;
     hipLaunchKernelGGL((kDuplicate<false, uint64_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
-                       nullptr, nullptr, nullptr, frame, spanInfo, spanPlanes, count, bounds, grid, keys, values, nullptr,
-                       control);
+                       nullptr, nullptr, nullptr, frame, spanInfo, spanPlanes, count, bounds, grid, keys, values,
+                       depthBits, control);
     markKernel("kDuplicate", stream);
 }
 
 void launchDuplicateTiles(const float4* splats, const uint32_t* tileCounts, const DupFrame& frame,
                           const uint32_t* spanInfo, const uint4* spanPlanes, uint32_t count, uint32_t bounds,
-                          const BinGrid& grid, uint32_t* keys, uint32_t* values, uint32_t* depthBits,
-                          gsr_control* control, hipStream_t stream) {
+                          const BinGrid& grid, uint32_t* keys, uint32_t* values, gsr_control* control,
+                          hipStream_t stream) {
     const uint32_t blocks = count > 0 ? (count + kDupThreads - 1) / kDupThreads : 1u;
     hipLaunchKernelGGL((kDuplicate<false, uint32_t>), dim3(blocks), dim3(kDupThreads), 0, stream, splats, tileCounts,
                        nullptr, nullptr, nullptr, frame, spanInfo, spanPlanes, count, bounds, grid, keys, values,
-                       depthBits, control);
+                       nullptr, control);
     markKernel("kDuplicate", stream);
 }
 
