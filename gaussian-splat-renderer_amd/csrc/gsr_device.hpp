@@ -321,6 +321,114 @@ __device__ __forceinline__ float evalShChannel(int degree, const float* c, float
     return result < 0.0f ? 0.0f : result;
 }
 
+// evalSH's basis values for the evaluated bands, each formed as evalShRaw forms it (term k of a
+// channel is b[k] * c[k]; terms 1 and 3 are subtracted, the rest added, in k order).
+__device__ __forceinline__ void shBasis(int degree, float x, float y, float z, float b[16]) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) b[k] = 0.0f;
+    b[0] = kC0;
+    if (degree > 0) {
+        b[1] = kC1 * y;
+        b[2] = kC1 * z;
+        b[3] = kC1 * x;
+        if (degree > 1) {
+            const float xx = x * x, yy = y * y, zz = z * z;
+            const float xy = x * y, yz = y * z, xz = x * z;
+            b[4] = kC2_0 * xy;
+            b[5] = kC2_1 * yz;
+            b[6] = kC2_2 * (2.0f * zz - xx - yy);
+            b[7] = kC2_3 * xz;
+            b[8] = kC2_4 * (xx - yy);
+            if (degree > 2) {
+                b[9] = kC3_0 * y * (3.0f * xx - yy);
+                b[10] = kC3_1 * xy * z;
+                b[11] = kC3_2 * y * (4.0f * zz - xx - yy);
+                b[12] = kC3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+                b[13] = kC3_4 * x * (4.0f * zz - xx - yy);
+                b[14] = kC3_5 * z * (xx - yy);
+                b[15] = kC3_6 * x * (xx - 3.0f * yy);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float shTerm(int k, float acc, float b, float c) {
+    return k == 0 ? b * c : (k == 1 || k == 3) ? acc - b * c : acc + b * c;
+}
+
+// evalShChannel for the three channels, reading the coefficient quads in record order (word j =
+// channel j % 3 of coefficient j / 3) and adding each word to its channel's sum as it arrives:
+// every channel gets its terms in k order, so the sums are evalShRaw's, bit for bit, and only one
+// quad of coefficients is live at a time.
+// The f32 scene's coefficient quads of gaussian i that the evaluated bands use (q[k] = plane k;
+// the unused ones zero).
+__device__ __forceinline__ void loadShQuads(const SceneRef& s, uint32_t i, int degree, float4 (&q)[12]) {
+    const int used = (degree + 1) * (degree + 1);
+    const int quads = (used * 3 + 3) / 4;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) q[k] = k < quads ? s.shPlanes[k * size_t(s.count) + i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// evalShStreamed of an f32 scene over quads loaded by loadShQuads.
+__device__ __forceinline__ void evalShQuads(int degree, const float4 (&q)[12], float x, float y, float z,
+                                            float out[3]) {
+    const int used = (degree + 1) * (degree + 1);
+    float b[16];
+    shBasis(degree, x, y, z, b);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const float vv[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int j = k * 4 + e;
+            if (j < used * 3) acc[j % 3] = shTerm(j / 3, acc[j % 3], b[j / 3], vv[e]);
+        }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const float r = acc[ch] + 0.5f;
+        out[ch] = r < 0.0f ? 0.0f : r;
+    }
+}
+
+template <int FMT>
+__device__ __forceinline__ void evalShStreamed(const SceneRef& s, uint32_t i, int degree, float x, float y, float z,
+                                               float out[3]) {
+    if (FMT == GSR_SCENE_F32_SOA) {
+        float4 q[12];
+        loadShQuads(s, i, degree, q);
+        evalShQuads(degree, q, x, y, z, out);
+        return;
+    }
+    const int used = (degree + 1) * (degree + 1);
+    float b[16];
+    shBasis(degree, x, y, z, b);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    const size_t plane = s.count;
+    {
+        const int quads = (used * 3 + 7) / 8;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            if (q < quads) {
+                const uint4 v = s.color[q * plane + i];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int j = q * 8 + e;
+                    const uint32_t h = (e & 1) ? (w[e >> 1] >> 16) : (w[e >> 1] & 0xFFFFu);
+                    if (j < used * 3) acc[j % 3] = shTerm(j / 3, acc[j % 3], b[j / 3], halfBitsToFloat(h));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const float r = acc[ch] + 0.5f;
+        out[ch] = r < 0.0f ? 0.0f : r;
+    }
+}
+
 // The coefficients of the evaluated bands, channel-major: co[ch][k].
 template <int FMT>
 __device__ __forceinline__ void loadSh(const SceneRef& s, uint32_t i, int used, float co[3][16]) {

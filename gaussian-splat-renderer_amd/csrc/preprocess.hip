@@ -42,7 +42,14 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
     if (i < scene.count) {
         float px, py, pz, alpha;
         uint4 g0, g1;
+        // f32 scene: the scales, rotation and (without a colour model) SH quads are loaded with the
+        // position, one memory round trip for the gaussian whatever it is culled by.
+        float4 scF, rotF;
+        float4 shq[12];
         if (FMT == GSR_SCENE_F32_SOA) {
+            scF = scene.scales[i];
+            rotF = scene.rotations[i];
+            if (!useMlp) loadShQuads(scene, i, scene.degree, shq);
             const float4 pa = scene.posAlpha[i];
             px = pa.x;
             py = pa.y;
@@ -72,8 +79,8 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
         } else {
             float sx, sy, sz, qw, qx, qy, qz;
             if (FMT == GSR_SCENE_F32_SOA) {
-                const float4 sc = scene.scales[i];
-                const float4 q = scene.rotations[i];
+                const float4 sc = scF;
+                const float4 q = rotF;
                 sx = sc.x;
                 sy = sc.y;
                 sz = sc.z;
@@ -170,16 +177,16 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                         o2.x = o2.y = o2.z = 0.0f;  // the learned colour pass writes it
                     } else {
                         // SH colour, dir = normalize(mean - camPos) in data space.
-                        const int degree = scene.degree;
-                        const int used = (degree + 1) * (degree + 1);
-                        float co[3][16];
-                        loadSh<FMT>(scene, i, used, co);
                         const float dx = px - cam.pos[0], dy = py - cam.pos[1], dz = pz - cam.pos[2];
                         const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
-                        const float x = dx * inv, y = dy * inv, z = dz * inv;
-                        o2.x = evalShChannel(degree, co[0], x, y, z);
-                        o2.y = evalShChannel(degree, co[1], x, y, z);
-                        o2.z = evalShChannel(degree, co[2], x, y, z);
+                        float rgb[3];
+                        if (FMT == GSR_SCENE_F32_SOA && !useMlp)
+                            evalShQuads(scene.degree, shq, dx * inv, dy * inv, dz * inv, rgb);
+                        else
+                            evalShStreamed<FMT>(scene, i, scene.degree, dx * inv, dy * inv, dz * inv, rgb);
+                        o2.x = rgb[0];
+                        o2.y = rgb[1];
+                        o2.z = rgb[2];
                     }
                     o2.w = radius;
                     // p24 keeps the conic and opacity in fp16: the tiles are counted, duplicated

@@ -1,6 +1,6 @@
 // K6 radix sort (reference GpuRadixSort, render/gpu_radix_sort.{h,cpp}; shaders
 // radix_histogram.comp, radix_scan.comp, radix_scatter_reorder.comp): stable LSD sort of
-// (64-bit key, u32 value) pairs by 8-bit digits.
+// (64-bit or 32-bit key, u32 value) pairs by digits of at most 8 bits.
 //
 // gfx950 design. The key count lives on the device (the frame's entries = min(M, capacity)):
 // every kernel reads it there; grids are sized for the capacity and idle blocks exit.
