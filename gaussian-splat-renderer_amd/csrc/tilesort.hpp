@@ -227,34 +227,51 @@ constexpr uint32_t kWaveAlone = kWaveChunk;  // ranges a single wave sorts (long
 // to one register index hit 64 consecutive words).
 __device__ __forceinline__ uint32_t chunkSlot(uint32_t lane, uint32_t r) { return r * 64u + lane; }
 
-// Cross-wave compare-exchange of two chunks of 64 E words held in registers by waves a (the lower
-// positions) and b: position q of a pairs with position q of b, or with 64 E - 1 - q when `flip`.
-// a keeps the minimum, b the maximum, exchanged in place through sW: b writes its words, a
-// reads each partner, keeps the minimum and writes the maximum into the same slot, b reads it
-// back. Every thread of the workgroup calls it (barriers).
+constexpr uint32_t kLdsChunk = 2048;  // entries sorted in LDS at once (16 KB)
+
+// Cross-wave compare-exchange of two chunks of 64 E words held in registers by a lower and an
+// upper wave: position q of the lower pairs with position q of the upper, or with 64 E - 1 - q
+// when `flip`. The lower keeps the minimum, the upper the maximum, exchanged in place through s:
+// the upper writes its words, the lower reads each partner, keeps the minimum and writes the
+// maximum into the same slot, the upper reads it back. `active` (the same for both waves of the
+// pair) false: the pair does nothing. Every thread of the workgroup calls it (barriers).
 template <int E>
-__device__ __forceinline__ void crossExchange(uint64_t (&w)[E], uint32_t wave, uint32_t lane, uint32_t a, uint32_t b,
-                                              bool flip, uint64_t* sW) {
-    if (wave == b) {
+__device__ __forceinline__ void exchangeStep(uint64_t (&w)[E], uint32_t lane, bool upper, bool active, bool flip,
+                                             uint64_t* s) {
+    if (active && upper) {
 #pragma unroll
-        for (uint32_t r = 0; r < uint32_t(E); ++r) sW[chunkSlot(lane, r)] = w[r];
+        for (uint32_t r = 0; r < uint32_t(E); ++r) s[chunkSlot(lane, r)] = w[r];
     }
     __syncthreads();
-    if (wave == a) {
+    if (active && !upper) {
 #pragma unroll
         for (uint32_t r = 0; r < uint32_t(E); ++r) {
             const uint32_t slot = flip ? chunkSlot(63u - lane, uint32_t(E) - 1u - r) : chunkSlot(lane, r);
-            const uint64_t y = sW[slot];
-            sW[slot] = __builtin_elementwise_max(w[r], y);
+            const uint64_t y = s[slot];
+            s[slot] = __builtin_elementwise_max(w[r], y);
             w[r] = __builtin_elementwise_min(w[r], y);
         }
     }
     __syncthreads();
-    if (wave == b) {
+    if (active && upper) {
 #pragma unroll
-        for (uint32_t r = 0; r < uint32_t(E); ++r) w[r] = sW[chunkSlot(lane, r)];
+        for (uint32_t r = 0; r < uint32_t(E); ++r) w[r] = s[chunkSlot(lane, r)];
     }
     __syncthreads();
+}
+
+// Two disjoint pairs of waves exchanged at once (pair 0 in sW[0, 64 E), pair 1 after it), or one
+// after the other where two chunks do not fit in the kLdsChunk words of sW.
+template <int E>
+__device__ __forceinline__ void exchangePairs(uint64_t (&w)[E], uint32_t lane, uint32_t pair, bool upper,
+                                              bool active, bool flip, uint64_t* sW) {
+    constexpr uint32_t C = 64u * E;
+    if constexpr (2u * C <= kLdsChunk) {
+        exchangeStep<E>(w, lane, upper, active, flip, sW + pair * C);
+    } else {
+        exchangeStep<E>(w, lane, upper, active && pair == 0u, flip, sW);
+        exchangeStep<E>(w, lane, upper, active && pair == 1u, flip, sW);
+    }
 }
 
 // The stages of a merge level below the chunk size (j = 32 E .. 1), within each wave's chunk.
@@ -268,28 +285,35 @@ __device__ __forceinline__ void mergeChunk(uint64_t (&w)[E], uint32_t lane) {
 // A range of n <= 256 E entries sorted by the 4 waves of the workgroup: positions
 // v * 64 E + lane * E + r (wave v), each wave's chunk of 64 E sorted in registers, then the
 // bitonic merge levels 128 E and 256 E: the stages whose partners are in another wave through
-// sW, the rest in registers (the 16-word form: DPP for lane distances below 16). Padding words
-// sort last.
+// sW (the two pairs of a stage at once), the rest in registers (the 16-word form: DPP for lane
+// distances below 16). Padding words sort last. The waves holding entries are a prefix; a wave
+// past the range (all padding) is only ever the upper wave of a pair, where max(x, padding) leaves
+// it as it is, so it skips its sorts and its pairs skip their exchanges.
 template <int E>
 __device__ __attribute__((noinline)) void groupSortRegisters(uint32_t* values, const uint32_t* __restrict__ depthBits,
                                                              uint2 range, uint32_t wave, uint32_t lane, uint64_t* sW) {
     constexpr uint32_t C = 64u * E;  // a wave's chunk
     const uint32_t n = range.y - range.x;
-    const uint32_t chunkN = n > wave * C ? n - wave * C : 0u;  // entries of this wave's chunk
+    const bool mine = n > wave * C;  // this wave's chunk holds entries
     uint64_t w[E];
-    loadWords<E>(w, values, depthBits, range.x + wave * C, chunkN, lane);
-    waveSort<E>(w, lane);
+    if (mine) {
+        loadWords<E>(w, values, depthBits, range.x + wave * C, n - wave * C, lane);
+        waveSort<E>(w, lane);
+    } else {
+#pragma unroll
+        for (int r = 0; r < E; ++r) w[r] = kPad;
+    }
     // Level 2C: chunks (0, 1) and (2, 3), flip.
-    crossExchange<E>(w, wave, lane, 0u, 1u, true, sW);
-    crossExchange<E>(w, wave, lane, 2u, 3u, true, sW);
-    mergeChunk<E>(w, lane);
+    const uint32_t pairA = wave >> 1;
+    const bool upperA = (wave & 1u) != 0u, activeA = n > (2u * pairA + 1u) * C;
+    exchangePairs<E>(w, lane, pairA, upperA, activeA, true, sW);
+    if (mine) mergeChunk<E>(w, lane);
     // Level 4C: the flip pairs chunk v with chunk 3 - v (position q with C - 1 - q), then the
     // stage C apart pairs chunk v with chunk v ^ 1 (same position).
-    crossExchange<E>(w, wave, lane, 0u, 3u, true, sW);
-    crossExchange<E>(w, wave, lane, 1u, 2u, true, sW);
-    crossExchange<E>(w, wave, lane, 0u, 1u, false, sW);
-    crossExchange<E>(w, wave, lane, 2u, 3u, false, sW);
-    mergeChunk<E>(w, lane);
+    const uint32_t pairB = (wave == 1u || wave == 2u) ? 1u : 0u;
+    exchangePairs<E>(w, lane, pairB, wave >= 2u, n > (pairB == 0u ? 3u : 2u) * C, true, sW);
+    exchangePairs<E>(w, lane, pairA, upperA, activeA, false, sW);
+    if (mine) mergeChunk<E>(w, lane);
 #pragma unroll
     for (uint32_t r = 0; r < uint32_t(E); ++r) {
         const uint32_t e = wave * C + lane * E + r;
@@ -298,7 +322,7 @@ __device__ __attribute__((noinline)) void groupSortRegisters(uint32_t* values, c
 }
 
 
-constexpr uint32_t kLdsChunk = 2048;  // entries sorted in LDS at once (16 KB)
+
 
 // Ascending bitonic sort of s[0, n2) (n2 a power of two) by the workgroup.
 __device__ __forceinline__ void bitonicSortLds(uint64_t* s, uint32_t n2) {
