@@ -507,10 +507,11 @@ def main():
     # Per-kernel algorithmic bytes (one launch; the frame's mean counts). N gaussians, V visible,
     # M list entries, T tiles, G sort blocks. kPreprocess reads 16 B position + opacity per gaussian,
     # 32 B scale + rotation unless culled at the near plane or for opacity and K x 12 B of SH per
-    # visible gaussian, and writes the 48-B splat + 4-B tile count per gaussian and 8 B per tile
-    # range zeroed; kFrameSetup reads the 32-B block records; kDuplicate reads 4 B count per gaussian
-    # and the 48-B splat per visible one and writes a 4-B tile key + 4-B value per entry (global
-    # mode also the 4-B depth bits per visible gaussian); a radix pass reads the 4-B keys
+    # visible gaussian, and writes the 48-B splat + 4-B tile count per gaussian, 8 B per tile range
+    # zeroed and per visible gaussian its 20-B coverage record (global mode also its 4-B depth
+    # bits); kFrameSetup reads the 32-B block records; kDuplicate reads 4 B count per gaussian and
+    # the coverage record per visible one and writes a 4-B tile key + 4-B value per entry; a radix
+    # pass reads the 4-B keys
     # (histogram), then 8 B per entry and writes 8 B per entry (scatter); kRanges reads the keys
     # and writes 8 B per tile; the tile depth sort reads 4 B value + 4 B depth bits and writes 4 B
     # per entry; the render reads 4 B value + 48 B splat per fetched entry and 8 B per tile range
@@ -523,10 +524,11 @@ def main():
     M, V = mean_entries, mean_visible
     blocks = min((int(M) + 2047) // 2048, 1024)
     key_b = 4
-    pre_bytes = n * 16 + (n - near - low_opacity) * 32 + V * K * 12 + n * 52 + tiles * 8
+    pre_bytes = (n * 16 + (n - near - low_opacity) * 32 + V * K * 12 + n * 52 + tiles * 8 +
+                 V * (24 if args.sort_mode == "global" else 20))
     algo = {"kPreprocess": pre_bytes,
             "kFrameSetup": (n + 255) // 256 * 32 + (n + 255) // 256 * 4 + 80,
-            "kDuplicate": n * 4 + V * 48 + M * (key_b + 4) + (V * 4 if args.sort_mode == "global" else 0),
+            "kDuplicate": n * 4 + V * 20 + M * (key_b + 4),
             "kSortHistogram": M * key_b + blocks * 256 * 4,
             "kSortScan": 2 * blocks * 256 * 4,
             "kSortScatter": 2 * M * (key_b + 4) + blocks * 256 * 4,
