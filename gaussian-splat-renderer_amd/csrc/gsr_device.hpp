@@ -502,8 +502,11 @@ __device__ __forceinline__ uint32_t quadrantMask(float cx, float cy, float a, fl
     const float M = a * Dx * Dx + 2.0f * fabsf(b) * Dx * Dy + c * Dy * Dy;
     const float lim = r2 + 2e-3f * fabsf(r2) + 1e-4f + 4e-6f * M;
     if (lim < 0.0f) return 0u;
-    const float ex = sqrtf(lim * c / det) * 1.001f + 1.0f;
-    const float ey = sqrtf(lim * a / det) * 1.001f + 1.0f;
+    // The hardware reciprocal and square root (errors near 1e-7, relative) are well inside the
+    // 0.1 % widening; 1 / det overflowing gives +inf, caught below.
+    const float rdet = __builtin_amdgcn_rcpf(det);
+    const float ex = __builtin_amdgcn_sqrtf(lim * c * rdet) * 1.001f + 1.0f;
+    const float ey = __builtin_amdgcn_sqrtf(lim * a * rdet) * 1.001f + 1.0f;
     if (!(ex < 1e30f && ey < 1e30f && lim < 1e30f)) return 0xFu;  // also false for NaN
     const float kb = b * __builtin_amdgcn_rcpf(c);
     const float kc = b * __builtin_amdgcn_rcpf(a);
