@@ -42,14 +42,12 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
     if (i < scene.count) {
         float px, py, pz, alpha;
         uint4 g0, g1;
-        // f32 scene: the scales, rotation and (without a colour model) SH quads are loaded with the
-        // position, one memory round trip for the gaussian whatever it is culled by.
+        // f32 scene: the scales and rotation are loaded with the position (one memory round trip
+        // for the geometry); the SH quads only once the splat is on screen.
         float4 scF, rotF;
-        float4 shq[12];
         if (FMT == GSR_SCENE_F32_SOA) {
             scF = scene.scales[i];
             rotF = scene.rotations[i];
-            if (!useMlp) loadShQuads(scene, i, scene.degree, shq);
             const float4 pa = scene.posAlpha[i];
             px = pa.x;
             py = pa.y;
@@ -180,10 +178,7 @@ __global__ __launch_bounds__(kPreprocessThreads) void kPreprocess(SceneRef scene
                         const float dx = px - cam.pos[0], dy = py - cam.pos[1], dz = pz - cam.pos[2];
                         const float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
                         float rgb[3];
-                        if (FMT == GSR_SCENE_F32_SOA && !useMlp)
-                            evalShQuads(scene.degree, shq, dx * inv, dy * inv, dz * inv, rgb);
-                        else
-                            evalShStreamed<FMT>(scene, i, scene.degree, dx * inv, dy * inv, dz * inv, rgb);
+                        evalShStreamed<FMT>(scene, i, scene.degree, dx * inv, dy * inv, dz * inv, rgb);
                         o2.x = rgb[0];
                         o2.y = rgb[1];
                         o2.z = rgb[2];
