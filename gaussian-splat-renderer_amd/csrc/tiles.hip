@@ -118,11 +118,22 @@ __global__ __launch_bounds__(kDupThreads) void kDuplicate(const float4* __restri
         // wave 0, of the earlier blocks of its own superblock by wave 1.
         const uint32_t sb = blockIdx.x / kSuperBlocks;
         if (wave == 0) {
-            uint32_t before = 0, all = 0;
-            for (uint32_t q = lane; q < frame.numSuper; q += 64) {
-                const uint32_t m = frame.superStats[q].tileSum;
-                all += m;
-                before += q < sb ? m : 0u;
+            // Up to 256 superblocks in one round of loads (4 a lane), any further ones in a loop.
+            uint32_t before = 0, all = 0, m[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+                const uint32_t q = lane + 64u * u;
+                m[u] = q < frame.numSuper ? frame.superStats[q].tileSum : 0u;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4u; ++u) {
+                all += m[u];
+                before += lane + 64u * u < sb ? m[u] : 0u;
+            }
+            for (uint32_t q = lane + 256u; q < frame.numSuper; q += 64) {
+                const uint32_t x = frame.superStats[q].tileSum;
+                all += x;
+                before += q < sb ? x : 0u;
             }
             before = waveSum64(before);
             all = waveSum64(all);
