@@ -573,12 +573,16 @@ def main():
                 "hbm_frac": round(render_bytes / (render_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "launch_us": round(render_ms * 1e3, 2), "ops_per_launch": int(ops),
                 "pairs_per_launch": {k: int(v) for k, v in pairs.items() if k not in ("max_depth",)},
+                "unpacked_fp32_peak": FP32_PEAK_TFLOPS / 2,
+                "frac_of_unpacked_peak": round(achieved / (FP32_PEAK_TFLOPS / 2), 4),
                 "note": "achieved = (16 ops x evaluated + 10 ops x blended pairs of the reference's tile loop, "
                         "counted by the tile-stats build on the same frames) / launch time (hipEvents on the render "
                         "stream; in global mode the launch also sorts each tile's range by depth, whose work is not "
                         "counted in ops); peak = MI355X FP32 157.3 TFLOP/s; traffic = rocprofv3 FETCH_SIZE + "
                         "WRITE_SIZE per launch scaled by a plain copy (profiles/); lane_slots = (pixel, splat) "
-                        "evaluations this kernel's loop issued"}
+                        "evaluations this kernel's loop issued; the library is built without packed FP32 "
+                        "(v_pk_*_f32), so one wave64 FP32 instruction takes a SIMD 4 cycles: "
+                        "unpacked_fp32_peak = half the published peak is the rate its VALU code can reach"}
     if args.sort_mode == "binning":
         passes = {"depth_radix": 4, "tile_radix": (int(tiles).bit_length() + 7) // 8}
         sort_ms = stages["sort"] + stages["depth_sort"]
