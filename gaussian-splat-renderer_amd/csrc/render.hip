@@ -426,6 +426,14 @@ __device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t ti
     const uint64_t laneMaskLt = (uint64_t(1) << lane) - 1u;
 
     uint32_t statFetched = 0, statTermEntry = 0xFFFFFFFFu, statBlended = 0, statSlots = 0;
+    // Each thread's list values of the next batch, loaded while the current batch is walked (so a
+    // batch's splat loads do not wait for its value loads).
+    uint32_t nextValue[kFetch];
+#pragma unroll
+    for (uint32_t f = 0; f < kFetch; ++f) {
+        const uint32_t e = range.x + t + f * 256u;
+        nextValue[f] = e < range.y && (B % 256 == 0 || t + f * 256u < uint32_t(B)) ? job.values[e] : 0u;
+    }
     for (uint32_t batch = range.x; batch < range.y; batch += B) {
         // Whole-tile vote; also the barrier that protects the previous batch's LDS.
         if (__syncthreads_count(thr < __builtin_inff() ? 0 : 1) == 256) break;
@@ -438,7 +446,7 @@ __device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t ti
             const uint32_t e = batch + slot;
             uint32_t mask = 0u;
             if (e < range.y) {
-                uint32_t g = job.values[e];
+                uint32_t g = nextValue[f];
                 g = g < job.gaussianCount ? g : 0u;  // list values are gaussian indices; never read past the scene
                 float cx, cy;
                 float4 co, col;
@@ -475,6 +483,11 @@ __device__ __forceinline__ void renderTileCulled(const TileJob& job, uint32_t ti
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (uint32_t f = 0; f < kFetch; ++f) {
+            const uint32_t e = batch + uint32_t(B) + t + f * 256u;
+            nextValue[f] = e < range.y && (B % 256 == 0 || t + f * 256u < uint32_t(B)) ? job.values[e] : 0u;
+        }
         if (__builtin_amdgcn_ballot_w64(thr < __builtin_inff()) != 0) {
             for (uint32_t j = 0; j < count4; j += 4) {
                 const uint4 four = *reinterpret_cast<const uint4*>(&sh.list[wave][j]);
