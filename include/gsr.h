@@ -211,13 +211,15 @@ typedef enum gsr_sort_algorithm {
 /* Stage timings of the last profiled frame, ms (the GpuProfiler scopes, profiler.h:1-87). */
 typedef enum gsr_stage {
     GSR_STAGE_PREPROCESS = 0, /* preprocess + cull + tile counts + block sums */
-    GSR_STAGE_SCAN = 1,       /* block-sum scan + control (tile_setup); binning: + compaction of the visible */
+    GSR_STAGE_SCAN = 1,       /* binning: block-sum scan + control (tile_setup) + compaction of the visible;
+                                 global: nothing (the duplicate takes its offsets from the preprocess's sums) */
     GSR_STAGE_DEPTH_SORT = 2, /* binning: depth sort of the visible splats (global: 0) */
     GSR_STAGE_DUPLICATE = 3,  /* in-block scan + duplication; binning: + count gather and block-sum scan */
     GSR_STAGE_SORT = 4,       /* all radix passes of the tile-list sort; global mode, full_key_sort 0: + ranges
-                                 + the per-tile depth sort */
+                                 (+ the per-tile depth sort, unless the culled render loop does it) */
     GSR_STAGE_RANGES = 5,     /* tile ranges (global mode, full_key_sort 0: nothing, counted in SORT) */
-    GSR_STAGE_RENDER = 6,     /* tile render incl. RGBA8 resolve */
+    GSR_STAGE_RENDER = 6,     /* tile render incl. RGBA8 resolve (global mode's culled loop: + each tile's
+                                 depth sort) */
     GSR_STAGE_TOTAL = 7,
     GSR_STAGE_COUNT = 8
 } gsr_stage;
